@@ -1,0 +1,265 @@
+"""graphwalker_amd -- MI355X-native walk-update engine (Python host binding).
+
+Thin ctypes layer over ``libgwhip.so`` (include/gwhip.h).  Every call goes to
+the HIP library; there is no CPU fallback: if the library is missing or no GPU
+is present the constructor raises.
+
+The names follow the GraphWalker engine they replace: a walk *program*
+(RandomWalkwithJump / Stop / Prob / RestartwithJoint / Restart,
+src/walks/randomwalk*.hpp), *blocks* of vertices (graphwalker.hpp:146-166),
+*walk records* (walk.hpp:69-91) and the engine *run* (graphwalker.hpp:272-311).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgwhip.so")
+
+# walk programs (include/gwhip.h)
+JUMP, STOP, PROB, RESTART_JOINT, RESTART = 0, 1, 2, 3, 4
+KIND_NAMES = {"jump": JUMP, "stop": STOP, "prob": PROB, "restart_joint": RESTART_JOINT, "restart": RESTART}
+# recorders
+REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY = 0, 1, 2
+# schedules
+SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
+SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}
+
+WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
+
+
+class GwError(RuntimeError):
+    pass
+
+
+class _Cfg(C.Structure):
+    _fields_ = [("device", C.c_int), ("reserved", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class Program(C.Structure):
+    """gw_program: one walk program and how to run it."""
+
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("L", C.c_uint32),
+        ("stop_thr", C.c_uint64),
+        ("nvertices", C.c_uint32),
+        ("stream", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("recorder", C.c_uint32),
+        ("schedule", C.c_uint32),
+        ("block_threads", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64),
+        ("visits", C.c_uint64),
+        ("rounds", C.c_uint64),
+        ("launches", C.c_uint64),
+        ("step_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("bucket_ms", C.c_double),
+        ("schedule", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+def stop_threshold(p: float) -> int:
+    """A walk follows an edge iff its 32-bit coin r0 >= p * 2^32; p = 0.15 is
+    the reference's ``rand_r/RAND_MAX > 0.15`` (randomwalkwithstop.hpp:35)."""
+    if not 0.0 <= p <= 1.0:
+        raise ValueError("probability out of range")
+    return int(round(p * 2.0 ** 32))
+
+
+def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
+            schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0) -> Program:
+    if isinstance(kind, str):
+        kind = KIND_NAMES[kind]
+    if isinstance(schedule, str):
+        schedule = SCHED_NAMES[schedule]
+    return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
+                   waves_per_simd)
+
+
+def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:
+    """Cumulative R-MAT quadrant probabilities as 16-bit thresholds."""
+    return (int(round(a * 65536)), int(round((a + b) * 65536)), int(round((a + b + c) * 65536)))
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GwError(f"{LIB_PATH} is not built; run `python -m graphwalker_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P, u32, u64, i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    sig = {
+        "gw_create": ([P, P], i),
+        "gw_destroy": ([P], None),
+        "gw_last_error": ([P], C.c_char_p),
+        "gw_device_info": ([P, C.c_char_p, C.c_size_t, P, P], i),
+        "gw_graph_set": ([P, P, P, u32, u64], i),
+        "gw_graph_rmat": ([P, u32, u64, u64, u32, u32, u32], i),
+        "gw_graph_blocks": ([P, P, u32], i),
+        "gw_graph_info": ([P, P, P, P], i),
+        "gw_graph_read": ([P, P, P], i),
+        "gw_walks_load": ([P, P, u64], i),
+        "gw_walks_init_per_vertex": ([P, u32], i),
+        "gw_walks_init_sources": ([P, P, P, u32, u32], i),
+        "gw_walks_read": ([P, P, u64], i),
+        "gw_walks_current": ([P, P, u64], i),
+        "gw_run": ([P, P, P], i),
+        "gw_bucket": ([P, P], i),
+        "gw_counts_read": ([P, P, u32], i),
+        "gw_summary_read": ([P, P, P, u64], i),
+        "gw_gather_probe": ([P, u64, u64, P], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Engine:
+    """One engine context on one GPU (gw_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._L = lib()
+        self._ctx = C.c_void_p()
+        cfg = _Cfg(device, 0, hbm_budget_bytes)
+        rc = self._L.gw_create(C.byref(self._ctx), C.byref(cfg))
+        if rc != 0:
+            raise GwError(f"gw_create failed ({rc}): no usable HIP device {device}")
+        self.n = 0
+        self.m = 0
+        self.nwalks = 0
+
+    def close(self) -> None:
+        if self._ctx:
+            self._L.gw_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self._L.gw_last_error(self._ctx)
+            raise GwError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---------------------------------------------------------------- device
+    def device_info(self) -> dict:
+        name = C.create_string_buffer(256)
+        cu = C.c_uint32()
+        mem = C.c_uint64()
+        self._check(self._L.gw_device_info(self._ctx, name, 256, C.byref(cu), C.byref(mem)), "gw_device_info")
+        return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": mem.value}
+
+    # ----------------------------------------------------------------- graph
+    def graph_set(self, off: np.ndarray, col: np.ndarray) -> None:
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        n = off.shape[0] - 1
+        self._check(self._L.gw_graph_set(self._ctx, _ptr(off), _ptr(col), n, col.shape[0]), "gw_graph_set")
+        self.n, self.m = n, col.shape[0]
+
+    def graph_rmat(self, scale: int, m: int, seed: int, a: float = 0.57, b: float = 0.19, c: float = 0.19) -> None:
+        ta, tab, tabc = rmat_thresholds(a, b, c)
+        self._check(self._L.gw_graph_rmat(self._ctx, scale, m, seed, ta, tab, tabc), "gw_graph_rmat")
+        self.n, self.m = 1 << scale, m
+
+    def graph_blocks(self, block_first) -> None:
+        bf = np.ascontiguousarray(block_first, dtype=np.uint32)
+        self._check(self._L.gw_graph_blocks(self._ctx, _ptr(bf), bf.shape[0] - 1), "gw_graph_blocks")
+
+    def graph_read(self) -> tuple[np.ndarray, np.ndarray]:
+        off = np.empty(self.n + 1, dtype=np.uint64)
+        col = np.empty(self.m, dtype=np.uint32)
+        self._check(self._L.gw_graph_read(self._ctx, _ptr(off), _ptr(col)), "gw_graph_read")
+        return off, col
+
+    # ----------------------------------------------------------------- walks
+    def walks_load(self, walks: np.ndarray) -> None:
+        walks = np.ascontiguousarray(walks, dtype=WALK_DTYPE)
+        self._check(self._L.gw_walks_load(self._ctx, _ptr(walks), walks.shape[0]), "gw_walks_load")
+        self.nwalks = walks.shape[0]
+
+    def walks_init_per_vertex(self, per_vertex: int = 1) -> None:
+        self._check(self._L.gw_walks_init_per_vertex(self._ctx, per_vertex), "gw_walks_init_per_vertex")
+        self.nwalks = self.n * per_vertex
+
+    def walks_init_sources(self, sources, walks_per_source: int, source_field=None) -> None:
+        s = np.ascontiguousarray(sources, dtype=np.uint32)
+        f = None if source_field is None else np.ascontiguousarray(source_field, dtype=np.uint32)
+        self._check(self._L.gw_walks_init_sources(self._ctx, _ptr(s), _ptr(f), s.shape[0], walks_per_source),
+                    "gw_walks_init_sources")
+        self.nwalks = s.shape[0] * walks_per_source
+
+    def walks_final(self) -> np.ndarray:
+        """Final walk records, indexed by walk id."""
+        out = np.empty(self.nwalks, dtype=WALK_DTYPE)
+        self._check(self._L.gw_walks_read(self._ctx, _ptr(out), self.nwalks), "gw_walks_read")
+        return out
+
+    # ------------------------------------------------------------------- run
+    def run(self, prog: Program) -> dict:
+        st = Stats()
+        self._check(self._L.gw_run(self._ctx, C.byref(prog), C.byref(st)), "gw_run")
+        return st.as_dict()
+
+    def bucket(self, nblocks: int) -> np.ndarray:
+        counts = np.zeros(nblocks, dtype=np.uint64)
+        self._check(self._L.gw_bucket(self._ctx, _ptr(counts)), "gw_bucket")
+        self.nwalks = int(counts.sum())
+        return counts
+
+    def walks_current(self) -> np.ndarray:
+        """The loaded walk records in their current slot order."""
+        out = np.empty(self.nwalks, dtype=WALK_DTYPE)
+        self._check(self._L.gw_walks_current(self._ctx, _ptr(out), self.nwalks), "gw_walks_current")
+        return out
+
+    def counts(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.uint32)
+        self._check(self._L.gw_counts_read(self._ctx, _ptr(out), self.n), "gw_counts_read")
+        return out
+
+    def summary(self) -> tuple[np.ndarray, np.ndarray]:
+        h = np.empty(self.nwalks, dtype=np.uint64)
+        v = np.empty(self.nwalks, dtype=np.uint32)
+        self._check(self._L.gw_summary_read(self._ctx, _ptr(h), _ptr(v), self.nwalks), "gw_summary_read")
+        return h, v
+
+    def gather_probe(self, nbytes: int, gathers: int) -> float:
+        ms = C.c_double()
+        self._check(self._L.gw_gather_probe(self._ctx, nbytes, gathers, C.byref(ms)), "gw_gather_probe")
+        return ms.value

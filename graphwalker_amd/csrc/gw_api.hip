@@ -1,0 +1,416 @@
+// gw_api.hip -- the C ABI (include/gwhip.h): context, graph and walk residency,
+// run dispatch, result read-back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gw_device.h"
+#include "gw_internal.h"
+
+int gw_fail(gw_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int gw_hip_check(gw_ctx* ctx, hipError_t e, const char* what) {
+    return gw_fail(ctx, e == hipErrorOutOfMemory ? GW_ENOMEM : GW_EHIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int gw_alloc(gw_ctx* ctx, void** p, uint64_t bytes, const char* what) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (ctx->hbm_budget && ctx->hbm_used + bytes > ctx->hbm_budget)
+        return gw_fail(ctx, GW_ENOMEM, std::string(what) + ": HBM budget exceeded (" +
+                                           std::to_string(ctx->hbm_used + bytes) + " > " +
+                                           std::to_string(ctx->hbm_budget) + " bytes)");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, std::string(what) + ": hipMalloc(" + std::to_string(bytes) +
+                                           ") failed: " + hipGetErrorString(e));
+    }
+    ctx->hbm_used += bytes;
+    return GW_OK;
+}
+
+void gw_free(gw_ctx* ctx, void* p, uint64_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    if (bytes == 0) bytes = 16;
+    ctx->hbm_used = ctx->hbm_used >= bytes ? ctx->hbm_used - bytes : 0;
+}
+
+namespace {
+
+void free_graph(gw_ctx* ctx) {
+    gw_free(ctx, ctx->d_off, (ctx->n + 1ull) * 8);
+    gw_free(ctx, ctx->d_col, ctx->m * 4);
+    gw_free(ctx, ctx->d_counts, ctx->n * 4ull);
+    gw_free(ctx, ctx->d_block_first, (ctx->nblocks + 1ull) * 4);
+    ctx->d_off = nullptr;
+    ctx->d_col = nullptr;
+    ctx->d_counts = nullptr;
+    ctx->d_block_first = nullptr;
+    ctx->n = 0;
+    ctx->m = 0;
+    ctx->nblocks = 0;
+    ctx->block_first.clear();
+}
+
+void free_walks(gw_ctx* ctx) {
+    gw_free(ctx, ctx->d_walks, ctx->walk_cap * sizeof(gw_walk));
+    gw_free(ctx, ctx->d_walks_alt, ctx->walk_cap * sizeof(gw_walk));
+    gw_free(ctx, ctx->d_final, ctx->walk_cap * sizeof(gw_walk));
+    gw_free(ctx, ctx->d_hash, ctx->walk_cap * 8);
+    gw_free(ctx, ctx->d_visits, ctx->walk_cap * 4);
+    ctx->d_walks = ctx->d_walks_alt = ctx->d_final = nullptr;
+    ctx->d_hash = nullptr;
+    ctx->d_visits = nullptr;
+    ctx->walk_cap = 0;
+    ctx->nwalks = 0;
+}
+
+int reserve_walks(gw_ctx* ctx, uint64_t n) {
+    if (n <= ctx->walk_cap) {
+        ctx->nwalks = n;
+        return GW_OK;
+    }
+    free_walks(ctx);
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_walks, n * sizeof(gw_walk), "walks"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_walks_alt, n * sizeof(gw_walk), "walks (alt)")) ||
+        (rc = gw_alloc(ctx, (void**)&ctx->d_final, n * sizeof(gw_walk), "final walks"))) {
+        gw_free(ctx, ctx->d_walks, n * sizeof(gw_walk));
+        gw_free(ctx, ctx->d_walks_alt, n * sizeof(gw_walk));
+        ctx->d_walks = ctx->d_walks_alt = nullptr;
+        return rc;
+    }
+    ctx->walk_cap = n;
+    ctx->nwalks = n;
+    return GW_OK;
+}
+
+int single_block(gw_ctx* ctx) {
+    ctx->nblocks = 1;
+    ctx->block_first = {0u, ctx->n};
+    int rc = gw_alloc(ctx, (void**)&ctx->d_block_first, 2 * 4, "block table");
+    if (rc) return rc;
+    GW_HIP(ctx, hipMemcpy(ctx->d_block_first, ctx->block_first.data(), 8, hipMemcpyHostToDevice));
+    return GW_OK;
+}
+
+__global__ void init_per_vertex_kernel(gw_walk* __restrict__ w, uint64_t nw, uint32_t per_vertex) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nw) return;
+    const uint32_t v = static_cast<uint32_t>(i / per_vertex);
+    w[i] = gw_walk{static_cast<uint32_t>(i), v, v, 0u};
+}
+
+__global__ void init_sources_kernel(gw_walk* __restrict__ w, uint64_t nw, const uint32_t* __restrict__ src,
+                                    const uint32_t* __restrict__ field, uint32_t wps) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nw) return;
+    const uint32_t k = static_cast<uint32_t>(i / wps);
+    const uint32_t v = src[k];
+    w[i] = gw_walk{static_cast<uint32_t>(i), field ? field[k] : v, v, 0u};
+}
+
+// Independent random 4-byte gathers: index i's address comes from Philox(i),
+// not from loaded data, so the loads of a lane are all in flight together.
+__global__ void gather_probe_kernel(const uint32_t* __restrict__ buf, uint32_t nwords, uint64_t gathers,
+                                    uint32_t* __restrict__ out) {
+    const uint64_t tid = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    uint32_t acc = 0;
+    for (uint64_t i = tid; i < gathers; i += stride) {
+        const gw::u32x4 r = gw::philox4x32_10(static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32),
+                                              0x47415448u, 0, 1, 2);
+        acc ^= buf[gw::mulhi32(r.x, nwords)];
+    }
+    out[tid] = acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_create(gw_ctx** out, const gw_cfg* cfg) {
+    if (!out) return GW_EINVAL;
+    *out = nullptr;
+    gw_ctx* ctx = new gw_ctx();
+    ctx->device = cfg ? cfg->device : 0;
+    ctx->hbm_budget = cfg ? cfg->hbm_budget_bytes : 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= ctx->device || ctx->device < 0) {
+        delete ctx;
+        return GW_EHIP;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return GW_EHIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    for (auto& e : ctx->ev) (void)hipEventCreate(&e);
+    if (gw_alloc(ctx, (void**)&ctx->d_counters, 16 * sizeof(unsigned long long), "counters")) {
+        gw_destroy(ctx);
+        return GW_ENOMEM;
+    }
+    *out = ctx;
+    return GW_OK;
+}
+
+void gw_destroy(gw_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_graph(ctx);
+    free_walks(ctx);
+    gw_free(ctx, ctx->d_counters, 16 * sizeof(unsigned long long));
+    gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
+    gw_free(ctx, ctx->d_bucket_off, 0);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* gw_last_error(gw_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gw_device_info(gw_ctx* ctx, char* name, size_t name_len, uint32_t* cu_count, uint64_t* hbm_bytes) {
+    if (!ctx) return GW_EINVAL;
+    hipDeviceProp_t prop;
+    GW_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    return GW_OK;
+}
+
+int gw_graph_set(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uint32_t n, uint64_t m) {
+    if (!ctx || !off || (m && !col) || n == 0 || n == 0xFFFFFFFFu) return gw_fail(ctx, GW_EINVAL, "gw_graph_set: bad arguments");
+    if (off[0] != 0 || off[n] != m) return gw_fail(ctx, GW_EINVAL, "gw_graph_set: off[0] must be 0 and off[n] == m");
+    for (uint32_t v = 0; v < n; ++v)
+        if (off[v + 1] < off[v]) return gw_fail(ctx, GW_EINVAL, "gw_graph_set: offsets not monotone");
+    for (uint64_t e = 0; e < m; ++e)
+        if (col[e] >= n) return gw_fail(ctx, GW_EINVAL, "gw_graph_set: column out of range");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    free_graph(ctx);
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_off, (n + 1ull) * 8, "offsets"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_col, m * 4, "columns"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, n * 4ull, "vertex counts"))) return rc;
+    ctx->n = n;
+    ctx->m = m;
+    GW_HIP(ctx, hipMemcpy(ctx->d_off, off, (n + 1ull) * 8, hipMemcpyHostToDevice));
+    if (m) GW_HIP(ctx, hipMemcpy(ctx->d_col, col, m * 4, hipMemcpyHostToDevice));
+    return single_block(ctx);
+}
+
+int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
+                  uint32_t tabc) {
+    if (!ctx || scale == 0 || scale > 31 || !(ta <= tab && tab <= tabc && tabc <= 65536))
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_rmat: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    free_graph(ctx);
+    int rc = gw_rmat_build(ctx, scale, m, seed, ta, tab, tabc);
+    if (rc) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, ctx->n * 4ull, "vertex counts"))) return rc;
+    return single_block(ctx);
+}
+
+int gw_graph_blocks(gw_ctx* ctx, const uint32_t* block_first, uint32_t nblocks) {
+    if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_blocks: no graph");
+    if (!block_first || nblocks == 0) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: empty block table");
+    if (block_first[0] != 0 || block_first[nblocks] != ctx->n)
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: table must start at 0 and end at n");
+    for (uint32_t b = 0; b < nblocks; ++b)
+        if (block_first[b + 1] <= block_first[b]) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: empty or unordered block");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_free(ctx, ctx->d_block_first, (ctx->nblocks + 1ull) * 4);
+    ctx->d_block_first = nullptr;
+    int rc = gw_alloc(ctx, (void**)&ctx->d_block_first, (nblocks + 1ull) * 4, "block table");
+    if (rc) return rc;
+    ctx->nblocks = nblocks;
+    ctx->block_first.assign(block_first, block_first + nblocks + 1);
+    GW_HIP(ctx, hipMemcpy(ctx->d_block_first, block_first, (nblocks + 1ull) * 4, hipMemcpyHostToDevice));
+    return GW_OK;
+}
+
+int gw_graph_info(gw_ctx* ctx, uint32_t* n, uint64_t* m, uint32_t* nblocks) {
+    if (!ctx) return GW_EINVAL;
+    if (n) *n = ctx->n;
+    if (m) *m = ctx->m;
+    if (nblocks) *nblocks = ctx->nblocks;
+    return GW_OK;
+}
+
+int gw_graph_read(gw_ctx* ctx, uint64_t* off, uint32_t* col) {
+    if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_read: no graph");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (off) GW_HIP(ctx, hipMemcpy(off, ctx->d_off, (ctx->n + 1ull) * 8, hipMemcpyDeviceToHost));
+    if (col && ctx->m) GW_HIP(ctx, hipMemcpy(col, ctx->d_col, ctx->m * 4, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
+    if (!ctx || (n && !walks)) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: bad arguments");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_load: no graph");
+    if (n >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: walk ids are 32-bit");
+    uint32_t src_max = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (walks[i].id >= n || walks[i].vertex >= ctx->n)
+            return gw_fail(ctx, GW_EINVAL, "gw_walks_load: walk " + std::to_string(i) + " has id >= n or vertex out of range");
+        src_max = std::max(src_max, walks[i].source);
+    }
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = reserve_walks(ctx, n);
+    if (rc) return rc;
+    if (n) GW_HIP(ctx, hipMemcpy(ctx->d_walks, walks, n * sizeof(gw_walk), hipMemcpyHostToDevice));
+    ctx->walk_src_max = src_max;
+    return GW_OK;
+}
+
+int gw_walks_init_per_vertex(gw_ctx* ctx, uint32_t per_vertex) {
+    if (!ctx || per_vertex == 0) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_per_vertex: bad arguments");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_per_vertex: no graph");
+    const uint64_t nw = static_cast<uint64_t>(ctx->n) * per_vertex;
+    if (nw >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_per_vertex: walk ids are 32-bit");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = reserve_walks(ctx, nw);
+    if (rc) return rc;
+    const uint32_t tb = 256;
+    init_per_vertex_kernel<<<dim3(static_cast<uint32_t>((nw + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(ctx->d_walks, nw, per_vertex);
+    GW_HIP(ctx, hipGetLastError());
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->walk_src_max = ctx->n - 1;
+    return GW_OK;
+}
+
+int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* source_field, uint32_t nsources,
+                          uint32_t walks_per_source) {
+    if (!ctx || !sources || nsources == 0 || walks_per_source == 0)
+        return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: bad arguments");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_sources: no graph");
+    uint32_t src_max = 0;
+    for (uint32_t k = 0; k < nsources; ++k) {
+        if (sources[k] >= ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: source out of range");
+        src_max = std::max(src_max, source_field ? source_field[k] : sources[k]);
+    }
+    const uint64_t nw = static_cast<uint64_t>(nsources) * walks_per_source;
+    if (nw >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: walk ids are 32-bit");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = reserve_walks(ctx, nw);
+    if (rc) return rc;
+    uint32_t *d_src = nullptr, *d_field = nullptr;
+    if ((rc = gw_alloc(ctx, (void**)&d_src, nsources * 4ull, "sources"))) return rc;
+    GW_HIP(ctx, hipMemcpy(d_src, sources, nsources * 4ull, hipMemcpyHostToDevice));
+    if (source_field) {
+        if ((rc = gw_alloc(ctx, (void**)&d_field, nsources * 4ull, "source field"))) { gw_free(ctx, d_src, nsources * 4ull); return rc; }
+        GW_HIP(ctx, hipMemcpy(d_field, source_field, nsources * 4ull, hipMemcpyHostToDevice));
+    }
+    const uint32_t tb = 256;
+    init_sources_kernel<<<dim3(static_cast<uint32_t>((nw + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(ctx->d_walks, nw, d_src, d_field, walks_per_source);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    gw_free(ctx, d_src, nsources * 4ull);
+    gw_free(ctx, d_field, nsources * 4ull);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "init_sources_kernel");
+    ctx->walk_src_max = src_max;
+    return GW_OK;
+}
+
+int gw_walks_read(gw_ctx* ctx, gw_walk* out, uint64_t n) {
+    if (!ctx || (n && !out)) return gw_fail(ctx, GW_EINVAL, "gw_walks_read: bad arguments");
+    if (n > ctx->nwalks) return gw_fail(ctx, GW_EINVAL, "gw_walks_read: more records requested than loaded");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) GW_HIP(ctx, hipMemcpy(out, ctx->d_final, n * sizeof(gw_walk), hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_walks_current(gw_ctx* ctx, gw_walk* out, uint64_t n) {
+    if (!ctx || (n && !out)) return gw_fail(ctx, GW_EINVAL, "gw_walks_current: bad arguments");
+    if (n > ctx->nwalks) return gw_fail(ctx, GW_EINVAL, "gw_walks_current: more records requested than loaded");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) GW_HIP(ctx, hipMemcpy(out, ctx->d_walks, n * sizeof(gw_walk), hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
+    if (!ctx || !prog) return gw_fail(ctx, GW_EINVAL, "gw_run: bad arguments");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_run: no graph");
+    if (prog->kind > GW_RESTART) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown program kind");
+    if (prog->L == 0 && prog->kind != GW_PROB) return gw_fail(ctx, GW_EINVAL, "gw_run: L must be >= 1");
+    if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");
+    if (prog->kind == GW_JUMP && prog->nvertices == 0) return gw_fail(ctx, GW_EINVAL, "gw_run: JUMP needs nvertices");
+    if (prog->kind == GW_JUMP && prog->nvertices > ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_run: nvertices above graph size");
+    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY)) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
+    // Restart programs jump to the source field, which must then name a vertex.
+    if (ctx->nwalks && (prog->kind == GW_RESTART || prog->kind == GW_RESTART_JOINT) && ctx->walk_src_max >= ctx->n)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: restart program with a source field that is not a vertex");
+    // An uncapped program must end: require a stop probability of at least 2^-12.
+    if (prog->kind == GW_PROB && prog->stop_thr < (1ull << 20))
+        return gw_fail(ctx, GW_EINVAL, "gw_run: PROB needs stop_thr >= 2^20 (mean walk length <= 4096)");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    return gw_step_run(ctx, prog, stats);
+}
+
+int gw_counts_read(gw_ctx* ctx, uint32_t* counts, uint32_t n) {
+    if (!ctx || !counts) return gw_fail(ctx, GW_EINVAL, "gw_counts_read: bad arguments");
+    if (!ctx->d_counts || n != ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_counts_read: n must equal the graph's vertex count");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GW_HIP(ctx, hipMemcpy(counts, ctx->d_counts, n * 4ull, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_summary_read(gw_ctx* ctx, uint64_t* hash, uint32_t* visits, uint64_t n) {
+    if (!ctx) return GW_EINVAL;
+    if (!ctx->d_hash || n > ctx->nwalks) return gw_fail(ctx, GW_ESTATE, "gw_summary_read: no summary recorded for that many walks");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hash) GW_HIP(ctx, hipMemcpy(hash, ctx->d_hash, n * 8, hipMemcpyDeviceToHost));
+    if (visits) GW_HIP(ctx, hipMemcpy(visits, ctx->d_visits, n * 4, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_gather_probe(gw_ctx* ctx, uint64_t bytes, uint64_t gathers, double* ms) {
+    if (!ctx || bytes < 4096 || gathers == 0 || bytes / 4 > 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_gather_probe: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t *buf = nullptr, *out = nullptr;
+    const uint32_t tb = 256, grid = std::max(1u, ctx->cu_count) * 64;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&buf, bytes, "gather buffer"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&out, static_cast<uint64_t>(grid) * tb * 4, "gather out"))) { gw_free(ctx, buf, bytes); return rc; }
+    hipError_t e = hipMemsetAsync(buf, 1, bytes, ctx->stream);
+    const uint32_t nwords = static_cast<uint32_t>(bytes / 4);
+    if (e == hipSuccess) {
+        gather_probe_kernel<<<dim3(grid), dim3(tb), 0, ctx->stream>>>(buf, nwords, gathers / 8, out);  // warm
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess) {
+        gather_probe_kernel<<<dim3(grid), dim3(tb), 0, ctx->stream>>>(buf, nwords, gathers, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    gw_free(ctx, buf, bytes);
+    gw_free(ctx, out, static_cast<uint64_t>(grid) * tb * 4);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "gather probe");
+    if (ms) *ms = t;
+    return GW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// gw_device.h -- device-side building blocks of the walk engine (gfx950).
+//
+//  * Philox4x32-10 (Salmon et al., SC'11): counter-based, so a walk's random
+//    stream is a pure function of (seed, walk id, hop) and trajectories do not
+//    depend on the launch shape or the order walks are scheduled in.
+//  * The per-step semantics of the reference's updateByWalk variants
+//    (src/walks/randomwalkwith*.hpp), written as one templated advance() whose
+//    else-branch is selected at compile time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gwhip.h"
+
+namespace gw {
+
+constexpr uint32_t kTagWalk = 0x57414C4Bu;  // 'WALK'
+constexpr uint32_t kTagRmat = 0x524D4154u;  // 'RMAT'
+constexpr uint64_t kHashInit = 0xCBF29CE484222325ull;
+constexpr uint64_t kHashPrime = 0x100000001B3ull;
+
+struct u32x4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// Program parameters as the kernels see them (kind is a template argument).
+struct Prog {
+    uint32_t L;
+    uint32_t nvertices;
+    uint64_t stop_thr;
+    uint32_t k0, k1;   // Philox key = seed
+    uint32_t stream;
+};
+
+struct Graph {
+    const uint64_t* __restrict__ off;
+    const uint32_t* __restrict__ col;
+    uint32_t n;
+};
+
+enum : int { kMoved = 0, kFinished = 1 };
+
+// Recorder hooks, compile-time selected.
+template <bool kVertexCount, bool kSummary>
+struct Recorder {
+    uint32_t* __restrict__ counts;
+    uint32_t visits;
+    uint32_t steps;
+    uint64_t hash;
+    __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
+        ++visits;
+        if (kVertexCount) atomicAdd(&counts[v], 1u);
+        if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
+    }
+};
+
+// Advance one walk while its vertex lies in [lo, hi).  Returns kFinished when
+// the walk ends, kMoved when it left [lo, hi) alive.  Else-branch per kind:
+//   GW_JUMP          teleport to mulhi32(r3, N)        randomwalkwithjump.hpp:49-51
+//   GW_STOP/GW_PROB  die                                randomwalkwithstop.hpp:39-41
+//   GW_RESTART_JOINT restart at source; end when hop % L == L-1
+//                                                       randomwalkwithrestartwithjoint.hpp:47-53
+//   GW_RESTART       restart at source, cap L           randomwalkwithrestart.hpp:163-176
+// The loop guard (vertex in block, hop < L) is the while-condition of the
+// reference loops (randomwalkwithjump.hpp:42).
+template <int KIND, class Rec>
+__device__ __forceinline__ int advance(const Prog& p, const Graph& g, uint32_t id, uint32_t source,
+                                       uint32_t& v_io, uint32_t& hop_io, uint32_t lo, uint32_t hi,
+                                       Rec& rec) {
+    constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
+    uint32_t v = v_io, hop = hop_io;
+    while (v - lo < hi - lo) {  // lo <= v < hi
+        if (capped && hop >= p.L) { v_io = v; hop_io = hop; return kFinished; }
+        rec.visit(v, hop);
+        const u32x4 r = philox4x32_10(id, hop, kTagWalk, p.stream, p.k0, p.k1);
+        const uint64_t b = g.off[v];
+        const uint64_t deg = g.off[v + 1] - b;
+        if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
+            const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
+            v = g.col[b + mulhi64(x, deg)];
+        } else {
+            if (KIND == GW_JUMP) {
+                v = mulhi32(r.w, p.nvertices);
+            } else if (KIND == GW_STOP || KIND == GW_PROB) {
+                v_io = v; hop_io = hop;
+                return kFinished;
+            } else {
+                v = source;
+            }
+        }
+        ++hop;
+        ++rec.steps;
+        if (KIND == GW_RESTART_JOINT && hop % p.L == p.L - 1) { v_io = v; hop_io = hop; return kFinished; }
+    }
+    v_io = v;
+    hop_io = hop;
+    if (capped && hop >= p.L) return kFinished;
+    return kMoved;
+}
+
+// Wavefront-wide sum of a 32-bit value (64 lanes), result valid in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+}  // namespace gw

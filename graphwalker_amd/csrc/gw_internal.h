@@ -1,0 +1,74 @@
+// gw_internal.h -- engine context and the launchers shared by the .hip units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gwhip.h"
+
+struct gw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cu_count = 0;
+    uint64_t hbm_budget = 0;
+    uint64_t hbm_used = 0;
+    std::string err;
+
+    // graph (resident in HBM)
+    uint32_t n = 0;
+    uint64_t m = 0;
+    uint64_t* d_off = nullptr;
+    uint32_t* d_col = nullptr;
+    uint32_t nblocks = 0;
+    std::vector<uint32_t> block_first;  // host copy, nblocks + 1
+    uint32_t* d_block_first = nullptr;
+
+    // walks
+    uint64_t nwalks = 0;
+    uint64_t walk_cap = 0;
+    uint32_t walk_src_max = 0;  // largest source field among loaded walks
+    gw_walk* d_walks = nullptr;
+    gw_walk* d_walks_alt = nullptr;
+    gw_walk* d_final = nullptr;      // final record of each walk, indexed by walk id
+
+    // recorder outputs
+    uint32_t* d_counts = nullptr;   // n
+    uint64_t* d_hash = nullptr;     // walk_cap, indexed by walk id
+    uint32_t* d_visits = nullptr;   // walk_cap, indexed by walk id
+
+    // device counters: [0] steps [1] visits [2] work cursor [3..] scratch
+    unsigned long long* d_counters = nullptr;
+    // bucketing scratch
+    uint32_t* d_tile_hist = nullptr;
+    uint64_t tile_hist_cap = 0;
+    unsigned long long* d_bucket_off = nullptr;  // nblocks + 2
+
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+// Record an error message on the context and return `code`.
+int gw_fail(gw_ctx* ctx, int code, const std::string& msg);
+int gw_hip_check(gw_ctx* ctx, hipError_t e, const char* what);
+int gw_alloc(gw_ctx* ctx, void** p, uint64_t bytes, const char* what);
+void gw_free(gw_ctx* ctx, void* p, uint64_t bytes);
+
+#define GW_HIP(ctx, call)                                        \
+    do {                                                         \
+        hipError_t _e = (call);                                  \
+        if (_e != hipSuccess) return gw_hip_check(ctx, _e, #call); \
+    } while (0)
+
+// gw_rmat.hip
+int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
+                  uint32_t tabc);
+
+// gw_step.hip
+int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);
+
+// gw_bucket.hip: stable counting sort of `in` (n records) by block into `out`;
+// bucket_off[b] = first index of bucket b (nblocks + 1 entries written, host).
+int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out,
+                   std::vector<uint64_t>& bucket_off);

@@ -1,0 +1,99 @@
+// gw_rmat.hip -- R-MAT graph generation and CSR build on the device.
+//
+// Produces the CSR layout the reference engine reads (uint64 beg_pos,
+// uint32 csr; conversions.hpp:181-298), generated directly in HBM:
+//   1. edge e -> (src, dst) from Philox(seed, e): one 16-bit uniform per level,
+//      quadrant a/b/c/d by cumulative 16-bit thresholds (self-loops and
+//      duplicates kept, no permutation);
+//   2. keys (src << scale | dst) radix-sorted (rocPRIM through hipCUB: a library
+//      sort, off the walk path);
+//   3. col[e] = low `scale` bits of key e; off[v] = first e with src(e) >= v, written
+//      by the edge that starts each run (no atomics, deterministic).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "gw_device.h"
+#include "gw_internal.h"
+
+namespace {
+
+__global__ void rmat_keys_kernel(uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t k0, uint32_t k1,
+                                 uint32_t ta, uint32_t tab, uint32_t tabc) {
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (e >= m) return;
+    uint32_t s = 0, d = 0;
+    gw::u32x4 r{0, 0, 0, 0};
+    for (uint32_t lvl = 0; lvl < scale; ++lvl) {
+        if ((lvl & 7u) == 0)
+            r = gw::philox4x32_10(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32), gw::kTagRmat, lvl >> 3, k0, k1);
+        const uint32_t word = ((lvl & 7u) >> 1) == 0 ? r.x : ((lvl & 7u) >> 1) == 1 ? r.y : ((lvl & 7u) >> 1) == 2 ? r.z : r.w;
+        const uint32_t u = (lvl & 1u) ? (word >> 16) : (word & 0xFFFFu);
+        const uint32_t sb = u >= tab ? 1u : 0u;
+        const uint32_t db = ((u >= ta && u < tab) || u >= tabc) ? 1u : 0u;
+        s = (s << 1) | sb;
+        d = (d << 1) | db;
+    }
+    keys[e] = (static_cast<uint64_t>(s) << scale) | d;
+}
+
+__global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t scale,
+                                       uint64_t* __restrict__ off, uint32_t* __restrict__ col) {
+    const uint32_t n = 1u << scale;
+    const uint64_t dmask = (1ull << scale) - 1;
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (e > m) return;
+    // The run of source s starts at e: off[v] = e for every v in (prev_src, s].
+    const uint64_t s = e < m ? (keys[e] >> scale) : static_cast<uint64_t>(n);
+    const int64_t prev = e > 0 ? static_cast<int64_t>(keys[e - 1] >> scale) : -1;
+    if (e < m) col[e] = static_cast<uint32_t>(keys[e] & dmask);
+    for (int64_t v = prev + 1; v <= static_cast<int64_t>(s); ++v) off[v] = e;
+}
+
+}  // namespace
+
+int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab, uint32_t tabc) {
+    const uint32_t n = 1u << scale;
+    if (scale == 32) return gw_fail(ctx, GW_EINVAL, "gw_graph_rmat: scale 32 needs 64-bit vertex ids");
+    int rc;
+    uint64_t *k_in = nullptr, *k_out = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    if ((rc = gw_alloc(ctx, (void**)&k_in, m * 8, "rmat keys"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&k_out, m * 8, "rmat keys (sorted)"))) { gw_free(ctx, k_in, m * 8); return rc; }
+    auto cleanup = [&]() {
+        gw_free(ctx, k_in, m * 8);
+        gw_free(ctx, k_out, m * 8);
+        gw_free(ctx, temp, temp_bytes);
+    };
+    const uint32_t tb = 256;
+    hipError_t e = hipSuccess;
+    if (m) {
+        rmat_keys_kernel<<<dim3(static_cast<uint32_t>((m + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+            k_in, m, scale, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32), ta, tab, tabc);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, k_in, k_out, m, 0, 2 * scale, ctx->stream);
+        if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "sort temp"))) { cleanup(); return rc; }
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, k_in, k_out, m, 0, 2 * scale, ctx->stream);
+    }
+    if (e != hipSuccess) { cleanup(); return gw_hip_check(ctx, e, "rmat generate/sort"); }
+    // keys sorted in k_out; k_in is free to be reused for nothing: release it
+    // before the CSR arrays to keep the peak at 2 key arrays.
+    gw_free(ctx, k_in, m * 8);
+    k_in = nullptr;
+    gw_free(ctx, temp, temp_bytes);
+    temp = nullptr;
+    temp_bytes = 0;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_off, (n + 1ull) * 8, "offsets"))) { cleanup(); return rc; }
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_col, m * 4, "columns"))) { cleanup(); return rc; }
+    ctx->n = n;
+    ctx->m = m;
+    csr_from_sorted_kernel<<<dim3(static_cast<uint32_t>((m + 1 + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+        k_out, m, scale, ctx->d_off, ctx->d_col);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "csr build");
+    return GW_OK;
+}

@@ -1,0 +1,161 @@
+/*
+ * gwhip.h -- C ABI of the MI355X walk engine (libgwhip.so).
+ *
+ * This is the only layer that calls HIP.  The C++ front end (the
+ * graphwalker_engine / WalkManager / RandomWalk headers under
+ * graphwalker_amd/cxx/) and the Python host module (graphwalker_amd/__init__.py)
+ * both sit on top of it.  Plain C types only: pointers, sizes, PODs.
+ * No exceptions cross this boundary: every entry point returns an int status
+ * (GW_OK = 0, negative = error class) and gw_last_error() describes the last
+ * failure on that context.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * GraphWalker tree):
+ *   gw_create / gw_destroy     graphwalker_engine ctor/dtor (src/engine/graphwalker.hpp:74-144)
+ *   gw_graph_set               loadSubGraph / findSubGraph: beg_pos + csr made
+ *                              resident (graphwalker.hpp:168-234); block table
+ *                              load_block_range (graphwalker.hpp:146-166)
+ *   gw_graph_rmat              (no counterpart: the reference reads a
+ *                              preprocessed CSR, conversions.hpp:181-298; this
+ *                              builds the same CSR layout on the device)
+ *   gw_walks_load              the initial buckets written by startWalksbyApp
+ *                              through WalkManager::moveWalk (walk.hpp:93-101)
+ *   gw_walks_init_*            device-side walk placement for the app patterns
+ *                              of rawrandomwalks.cpp:27-50 / rwdomination.cpp:264-284
+ *                              (one walk per vertex) and msppr.cpp:151-179
+ *                              (walkspersource walks per source)
+ *   gw_run                     graphwalker_engine::run (graphwalker.hpp:272-311)
+ *                              with exec_updates (graphwalker.hpp:257-270)
+ *                              calling updateByWalk per walk (randomwalk.hpp:39)
+ *   gw_bucket                  the re-bucketing of walks by block done by
+ *                              moveWalk + getCurrentWalks (walk.hpp:93-137)
+ *   gw_counts_read             the per-vertex recorder of updateInfo
+ *                              (rwdomination.cpp:286-289 vertex_value[dstId]++)
+ *   gw_walks_read / gw_summary_read   final walk records and per-walk visit
+ *                              summaries (no reference counterpart; test access)
+ */
+#ifndef GWHIP_H
+#define GWHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gw_ctx gw_ctx;
+
+/* status codes */
+#define GW_OK 0
+#define GW_EINVAL (-1)   /* bad argument */
+#define GW_ENOMEM (-2)   /* device allocation failed or budget exceeded */
+#define GW_EHIP (-3)     /* HIP runtime error */
+#define GW_ESTATE (-4)   /* call out of order (no graph, no walks, ...) */
+
+/* walk programs: the updateByWalk variants */
+#define GW_JUMP 0          /* RandomWalkwithJump            randomwalkwithjump.hpp:30-62 */
+#define GW_STOP 1          /* RandomWalkwithStop            randomwalkwithstop.hpp:20-52 */
+#define GW_PROB 2          /* RandomWalkwithProb            randomwalkwithprob.hpp:20-57 */
+#define GW_RESTART_JOINT 3 /* RandomWalkwithRestartwithJoint randomwalkwithrestartwithjoint.hpp:29-61 */
+#define GW_RESTART 4       /* RandomWalkwithRestart          randomwalkwithrestart.hpp:33-63 */
+
+/* recorders (bit flags): device replacements for the per-step updateInfo */
+#define GW_REC_NONE 0u
+#define GW_REC_VERTEX_COUNT 1u /* u32 visits per vertex (rwdomination.cpp:288) */
+#define GW_REC_WALK_SUMMARY 2u /* per-walk visit count + trajectory hash */
+
+/* schedules */
+#define GW_SCHED_AUTO 0u
+#define GW_SCHED_PLAIN 1u      /* one walk per thread, whole graph resident */
+#define GW_SCHED_PERSISTENT 2u /* persistent wavefronts that refill finished lanes */
+#define GW_SCHED_BLOCKED 3u    /* device block loop: bucket by block, step block-local */
+
+typedef struct gw_cfg {
+    int device;                 /* HIP device ordinal */
+    uint32_t reserved;
+    uint64_t hbm_budget_bytes;  /* 0 = no cap */
+} gw_cfg;
+
+/* One walk record (16 bytes).  The reference packs source:24|offset:26|hop:14
+ * into 64 bits (walk.hpp:69-84); the device record keeps the same fields
+ * unpacked, plus the stable walk id that keys the random stream. */
+typedef struct gw_walk {
+    uint32_t id;     /* walk id: Philox counter word 0 */
+    uint32_t source; /* source field (walk.hpp:74-76); restart target for RESTART / RESTART_JOINT */
+    uint32_t vertex; /* current vertex, global id */
+    uint32_t hop;    /* hop field (walk.hpp:82-84) */
+} gw_walk;
+
+typedef struct gw_program {
+    uint32_t kind;      /* GW_JUMP .. GW_RESTART */
+    uint32_t L;         /* hop cap, or the period of RESTART_JOINT */
+    uint64_t stop_thr;  /* follow an edge iff deg > 0 and r0 >= stop_thr (p * 2^32) */
+    uint32_t nvertices; /* teleport range of GW_JUMP */
+    uint32_t stream;    /* Philox counter word 3 */
+    uint64_t seed;      /* Philox key */
+    uint32_t recorder;  /* GW_REC_* flags */
+    uint32_t schedule;  /* GW_SCHED_* */
+    uint32_t block_threads;   /* 0 = default workgroup size */
+    uint32_t waves_per_simd;  /* persistent schedule occupancy target, 0 = default */
+} gw_program;
+
+typedef struct gw_stats {
+    uint64_t steps;        /* hop increments (device counter) */
+    uint64_t visits;       /* updateInfo-equivalent visits (device counter) */
+    uint64_t rounds;       /* block rounds (BLOCKED) or 1 */
+    uint64_t launches;     /* step-kernel launches */
+    double step_ms;        /* sum of step-kernel durations, HIP events on the engine stream */
+    double total_ms;       /* whole gw_run on the engine stream (includes bucketing) */
+    double bucket_ms;      /* bucketing kernels (BLOCKED) */
+    uint32_t schedule;     /* schedule actually used */
+    uint32_t reserved;
+} gw_stats;
+
+int gw_create(gw_ctx **out, const gw_cfg *cfg);
+void gw_destroy(gw_ctx *ctx);
+const char *gw_last_error(gw_ctx *ctx);
+int gw_device_info(gw_ctx *ctx, char *name, size_t name_len, uint32_t *cu_count, uint64_t *hbm_bytes);
+
+/* Graph: uint64 offsets off[n+1] (absolute), uint32 columns col[m]; host
+ * pointers, copied into HBM.  block_first[nblocks+1] partitions the vertices;
+ * pass NULL / 0 for a single block. */
+int gw_graph_set(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uint32_t n, uint64_t m);
+int gw_graph_rmat(gw_ctx *ctx, uint32_t scale, uint64_t m, uint64_t seed,
+                  uint32_t ta, uint32_t tab, uint32_t tabc);
+int gw_graph_blocks(gw_ctx *ctx, const uint32_t *block_first, uint32_t nblocks);
+int gw_graph_info(gw_ctx *ctx, uint32_t *n, uint64_t *m, uint32_t *nblocks);
+int gw_graph_read(gw_ctx *ctx, uint64_t *off, uint32_t *col);
+
+/* Walks: records with ids < n (ids key the random streams). */
+int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
+/* count walks per vertex: walk i = vertex i / per_vertex, source = vertex, id = i */
+int gw_walks_init_per_vertex(gw_ctx *ctx, uint32_t per_vertex);
+/* walks_per_source walks per source: id = k * walks_per_source + j, source =
+ * source_field[k] (or sources[k] when source_field is NULL), vertex = sources[k] */
+int gw_walks_init_sources(gw_ctx *ctx, const uint32_t *sources, const uint32_t *source_field,
+                          uint32_t nsources, uint32_t walks_per_source);
+/* Final record of every walk after the last gw_run, indexed by walk id. */
+int gw_walks_read(gw_ctx *ctx, gw_walk *out, uint64_t n);
+/* The loaded records in their current slot order (after gw_bucket: by block). */
+int gw_walks_current(gw_ctx *ctx, gw_walk *out, uint64_t n);
+
+/* Run every loaded walk to completion. */
+int gw_run(gw_ctx *ctx, const gw_program *prog, gw_stats *stats);
+
+/* Stable counting sort of the loaded walks by block of their vertex (out of
+ * range vertices are dropped); bucket_counts[nblocks] receives the sizes. */
+int gw_bucket(gw_ctx *ctx, uint64_t *bucket_counts);
+
+int gw_counts_read(gw_ctx *ctx, uint32_t *counts, uint32_t n);
+int gw_summary_read(gw_ctx *ctx, uint64_t *hash, uint32_t *visits, uint64_t n);
+
+/* Random-gather ceiling: `gathers` independent random 4-byte loads from a
+ * buffer of `bytes` bytes; returns the kernel time in ms. */
+int gw_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GWHIP_H */

@@ -1,0 +1,104 @@
+"""FP64 finite-horizon expectations for the walk programs -- TEST INFRASTRUCTURE.
+
+For a walk that starts at hop 0, the expected number of visits (updateInfo
+calls, randomwalk.hpp:32) to each vertex is sum_{k<H} x_k with x_0 the start
+distribution and x_{k+1} = T(x_k), where T is the same chain the step
+programs run:
+
+* follow an edge with probability (1-p) when the vertex has out-edges; the
+  neighbour is uniform over the vertex's CSR slots, so duplicate edges count
+  with multiplicity (randomwalkwithstop.hpp:35-38);
+* otherwise (coin, or a dead end):
+  JUMP teleports uniformly (randomwalkwithjump.hpp:49-51), STOP/PROB die
+  (randomwalkwithstop.hpp:39-41), RESTART/RESTART_JOINT return to the source
+  (randomwalkwithrestartwithjoint.hpp:47-49).
+
+H is L for the capped programs and L-1 per period for RESTART_JOINT (the walk
+records hop offsets 0..L-2 and stops when hop % L == L-1,
+randomwalkwithrestartwithjoint.hpp:52-54).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import JUMP, PROB, RESTART, RESTART_JOINT, STOP
+
+
+def transition(off: np.ndarray, col: np.ndarray) -> tuple[sp.csr_matrix, np.ndarray]:
+    """Row-stochastic transition over the CSR (rows with no edges are zero)
+    and the dangling indicator."""
+    off = np.asarray(off, dtype=np.int64)
+    n = off.shape[0] - 1
+    deg = np.diff(off)
+    rows = np.repeat(np.arange(n), deg)
+    w = np.repeat(np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0), deg)
+    P = sp.csr_matrix((w, (rows, np.asarray(col, dtype=np.int64))), shape=(n, n))
+    P.sum_duplicates()
+    return P, deg == 0
+
+
+def expected_visits(kind: int, off, col, source: int, walks: float, L: int, p: float) -> np.ndarray:
+    """Expected per-vertex visit counts of ``walks`` walks started at ``source``
+    with hop 0."""
+    P, dangling = transition(off, col)
+    PT = P.T.tocsr()
+    n = P.shape[0]
+    x = np.zeros(n)
+    x[source] = float(walks)
+    H = L if kind != RESTART_JOINT else max(L - 1, 1)
+    if kind == PROB:
+        H = 10_000  # uncapped: iterate until the surviving mass is negligible
+    acc = np.zeros(n)
+    for _ in range(H):
+        acc += x
+        moving = x * (~dangling)
+        nxt = (1.0 - p) * (PT @ moving)
+        lost = x.sum() - nxt.sum()  # mass that took the else-branch
+        if kind == JUMP:
+            nxt += lost / n
+        elif kind in (RESTART, RESTART_JOINT):
+            nxt[source] += lost
+        elif kind in (STOP, PROB):
+            pass
+        x = nxt
+        if kind == PROB and x.sum() < 1e-12 * walks:
+            break
+    return acc
+
+
+def expected_visits_start(kind: int, off, col, start: np.ndarray, L: int, p: float) -> np.ndarray:
+    """Same as expected_visits for walks whose source equals their start
+    vertex, given a start histogram (e.g. one walk per vertex)."""
+    P, dangling = transition(off, col)
+    PT = P.T.tocsr()
+    n = P.shape[0]
+    if kind in (RESTART, RESTART_JOINT):
+        raise ValueError("restart programs depend on each walk's source; sum per-source results")
+    x = np.asarray(start, dtype=np.float64).copy()
+    acc = np.zeros(n)
+    for _ in range(L):
+        acc += x
+        moving = x * (~dangling)
+        nxt = (1.0 - p) * (PT @ moving)
+        if kind == JUMP:
+            nxt += (x.sum() - nxt.sum()) / n
+        x = nxt
+    return acc
+
+
+def mc_l1_bound(expected: np.ndarray, walks: int, visits_per_walk: float, H: int) -> float:
+    """Upper bound on E||p_hat - p||_1 for the normalised visit vector.
+
+    Each walk's visit count at v lies in [0, H], so Var <= E[c^2] <= H * E[c];
+    with mu_v the expected visits per walk, E|mean - mu_v| <= sqrt(H mu_v / W)
+    and the normalised L1 error is at most sum_v sqrt(H mu_v / W) / visits.
+    """
+    mu = np.asarray(expected, dtype=np.float64) / float(walks)
+    return float(np.sqrt(H * mu / walks).sum() / visits_per_walk)
+
+
+def l1_normalised(counts: np.ndarray, expected: np.ndarray) -> float:
+    c = np.asarray(counts, dtype=np.float64)
+    e = np.asarray(expected, dtype=np.float64)
+    return float(np.abs(c / c.sum() - e / e.sum()).sum())

@@ -1,0 +1,160 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact for everything (graphs, trajectories, counts, step totals): the
+walk programs are integer-only and keyed by (seed, walk id, hop).
+"""
+import numpy as np
+import pytest
+
+import graphwalker_amd as gw
+import oracle as O
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [gw.JUMP, gw.STOP, gw.PROB, gw.RESTART_JOINT, gw.RESTART]
+SCHEDS = [gw.SCHED_PLAIN, gw.SCHED_PERSISTENT]
+
+
+def oprog(p: gw.Program) -> O.Program:
+    return O.Program(p.kind, p.L, p.stop_thr, p.nvertices, p.stream, p.seed)
+
+
+@pytest.fixture(scope="module")
+def small_graph(engine):
+    off, col = O.rmat_graph(12, 1 << 16, seed=7)
+    return off, col
+
+
+@pytest.mark.parametrize("scale,m,seed", [(4, 40, 3), (10, 9000, 5), (16, 1 << 20, 1)])
+def test_rmat_device_graph_equals_oracle(engine, scale, m, seed):
+    engine.graph_rmat(scale, m, seed)
+    off, col = engine.graph_read()
+    roff, rcol = O.rmat_graph(scale, m, seed)
+    assert np.array_equal(off, roff)
+    assert np.array_equal(col, rcol)
+
+
+def _walks(n, nv, kind, seed=0):
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, nv, n).astype(np.uint32)
+    hops = (np.arange(n) % 3) * 11 if kind == gw.RESTART_JOINT else np.zeros(n, np.uint32)
+    w = O.make_walks(v, sources_=v, hops=hops)
+    return w
+
+
+@pytest.mark.parametrize("sched", SCHEDS)
+@pytest.mark.parametrize("kind", KINDS)
+def test_step_programs_bit_exact(engine, small_graph, kind, sched):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    walks = _walks(20_000, n, kind, seed=kind)
+    engine.walks_load(walks)
+    prog = gw.program(kind, 11, 0.15, n, seed=0xDEADBEEF12345, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
+                      schedule=sched)
+    st = engine.run(prog)
+    ref = O.replay(oprog(prog), off, col, walks, per_walk=True)
+    assert st["steps"] == ref["steps"] and st["visits"] == ref["visits"]
+    assert np.array_equal(engine.counts(), ref["counts"])
+    h, v = engine.summary()
+    assert np.array_equal(v, ref["walk_visits"])
+    assert np.array_equal(h, ref["hash"])
+    fin = engine.walks_final()
+    assert np.array_equal(fin, ref["final"])
+
+
+def test_config1_restart_ppr(engine):
+    """R-MAT 2^16 / 2^20, seed 1; 100k restart walks from vertex 0, p = 0.15,
+    10 hops: 1,000,000 steps exactly, counts bit-exact, L1 to FP64 within the
+    Monte-Carlo bound."""
+    engine.graph_rmat(16, 1 << 20, 1)
+    off, col = O.rmat_graph(16, 1 << 20, 1)
+    W, L = 100_000, 10
+    engine.walks_init_sources([0], W)
+    prog = gw.program(gw.RESTART, L, 0.15, 1 << 16, seed=1, recorder=gw.REC_VERTEX_COUNT)
+    for sched in SCHEDS:
+        prog.schedule = sched
+        st = engine.run(prog)
+        assert st["steps"] == W * L and st["visits"] == W * L
+        c = engine.counts()
+        ref = O.replay(oprog(prog), off, col, O.make_walks(np.zeros(W, np.uint32)))
+        assert np.array_equal(c, ref["counts"])
+    exp = fp64.expected_visits(O.RESTART, off, col, 0, W, L, 0.15)
+    l1 = fp64.l1_normalised(c, exp)
+    assert l1 < fp64.mc_l1_bound(exp, W, L, L)
+    print(f"[config1] L1 = {l1:.4f}, bound {fp64.mc_l1_bound(exp, W, L, L):.4f}")
+
+
+@pytest.mark.parametrize("tb,wps", [(64, 1), (128, 3), (256, 8), (192, 5)])
+def test_launch_shape_independence(engine, small_graph, tb, wps):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    walks = _walks(7_777, n, gw.STOP, seed=9)
+    engine.walks_load(walks)
+    prog = gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
+                      schedule=gw.SCHED_PERSISTENT, block_threads=tb, waves_per_simd=wps)
+    engine.run(prog)
+    c1, (h1, _) = engine.counts(), engine.summary()
+    prog.schedule = gw.SCHED_PLAIN
+    engine.run(prog)
+    assert np.array_equal(engine.counts(), c1)
+    assert np.array_equal(engine.summary()[0], h1)
+
+
+def test_edge_cases(engine):
+    # dangling-only graph, empty walk set, walks already at the cap
+    off = np.zeros(9, dtype=np.uint64)
+    col = np.zeros(0, dtype=np.uint32)
+    engine.graph_set(off, col)
+    engine.walks_load(np.zeros(0, dtype=gw.WALK_DTYPE))
+    st = engine.run(gw.program(gw.STOP, 5, 0.15, 8, seed=1))
+    assert st["steps"] == 0 and st["visits"] == 0
+    w = O.make_walks(np.arange(8), hops=[0, 5, 9, 0, 0, 0, 0, 0])
+    engine.walks_load(w)
+    for kind in (gw.JUMP, gw.STOP, gw.RESTART):
+        prog = gw.program(kind, 5, 0.15, 8, seed=2, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY)
+        st = engine.run(prog)
+        ref = O.replay(oprog(prog), off, col, w, per_walk=True)
+        assert (st["steps"], st["visits"]) == (ref["steps"], ref["visits"])
+        assert np.array_equal(engine.counts(), ref["counts"])
+        assert np.array_equal(engine.walks_final(), ref["final"])
+    # STOP on a dangling-only graph: each live walk visits once and dies
+    assert ref is not None
+
+
+def test_invalid_inputs_raise(engine, small_graph):
+    off, col = small_graph
+    engine.graph_set(off, col)
+    bad = O.make_walks([1 << 20])
+    with pytest.raises(gw.GwError):
+        engine.walks_load(bad)
+    with pytest.raises(gw.GwError):
+        engine.run(gw.program(gw.PROB, 1, 0.0, 10, seed=1))
+    with pytest.raises(gw.GwError):
+        engine.graph_blocks([0, 5, 5, off.shape[0] - 1])
+    engine.walks_load(O.make_walks([1, 2], sources_=[1, 1 << 30]))
+    with pytest.raises(gw.GwError):
+        engine.run(gw.program(gw.RESTART, 4, 0.15, 10, seed=1))
+
+
+@pytest.mark.parametrize("nb", [1, 5, 64, 700])
+def test_bucketing_is_a_stable_counting_sort(engine, small_graph, nb):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    rng = np.random.default_rng(nb)
+    cuts = np.unique(rng.integers(1, n, nb - 1)) if nb > 1 else np.array([], dtype=np.int64)
+    bf = np.concatenate([[0], cuts, [n]]).astype(np.uint32)
+    engine.graph_blocks(bf)
+    for nw in (0, 1, 4095, 4097, 100_003):
+        w = O.make_walks(rng.integers(0, n, nw).astype(np.uint32), ids=np.arange(nw))
+        if nw:
+            engine.walks_load(w)
+        else:
+            engine.walks_load(np.zeros(0, dtype=gw.WALK_DTYPE))
+        cnt = engine.bucket(bf.shape[0] - 1)
+        ref, rcnt = O.bucket_stable(w, bf)
+        assert np.array_equal(cnt, rcnt[:-1])
+        assert np.array_equal(engine.walks_current(), ref)

@@ -1,0 +1,355 @@
+"""Pins the CPU oracle before anything is compared with it.
+
+Philox: Random123 known-answer vectors and a second implementation (rocRAND's
+header engine, vectors in tests/golden/philox_vectors.json).
+R-MAT / CSR: a pure-Python restatement on small cases, quadrant frequencies,
+numpy lexsort equality.
+Step programs: closed forms (cycles, sinks, p = 0 / p = 1), walk-length laws,
+neighbour uniformity, FP64 finite-horizon expectations within the Monte-Carlo
+bound, whole-graph replay == block-engine replay, stable bucketing ==
+numpy stable sort.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle as O
+from oracle import fp64
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# ---------------------------------------------------------------- Philox
+
+
+def test_philox_random123_kat():
+    # Random123 kat_vectors, philox4x32 with 10 rounds.
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, out in kat:
+        assert O.philox(ctr, key)[0].tolist() == out
+
+
+def test_philox_matches_rocrand_vectors():
+    d = json.load(open(os.path.join(HERE, "golden", "philox_vectors.json")))
+    assert len(d["vectors"]) == 64
+    for v in d["vectors"]:
+        assert O.philox(v["ctr"], v["key"])[0].tolist() == v["out"]
+
+
+def _py_philox(c, k):
+    c = list(c)
+    k = list(k)
+    M = 0xFFFFFFFF
+    for r in range(10):
+        if r:
+            k = [(k[0] + 0x9E3779B9) & M, (k[1] + 0xBB67AE85) & M]
+        p0 = 0xD2511F53 * c[0]
+        p1 = 0xCD9E8D57 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & M, p1 & M, ((p0 >> 32) ^ c[3] ^ k[1]) & M, p0 & M]
+    return c
+
+
+def test_philox_uniformity():
+    ctr = np.zeros((20000, 4), dtype=np.uint32)
+    ctr[:, 0] = np.arange(20000)
+    out = O.philox(ctr, [7, 0]).reshape(-1)
+    hist = np.bincount(out >> 28, minlength=16)
+    assert stats.chisquare(hist).pvalue > 1e-4
+
+
+# ------------------------------------------------------------- R-MAT/CSR
+
+
+def _py_rmat_edge(e, scale, seed, ta, tab, tabc):
+    s = d = 0
+    r = None
+    for lvl in range(scale):
+        if lvl % 8 == 0:
+            r = _py_philox([e & 0xFFFFFFFF, e >> 32, 0x524D4154, lvl // 8], [seed & 0xFFFFFFFF, seed >> 32])
+        w = r[(lvl % 8) // 2]
+        u = (w >> 16) if lvl % 2 else (w & 0xFFFF)
+        if u < ta:
+            q = (0, 0)
+        elif u < tab:
+            q = (0, 1)
+        elif u < tabc:
+            q = (1, 0)
+        else:
+            q = (1, 1)
+        s = (s << 1) | q[0]
+        d = (d << 1) | q[1]
+    return s, d
+
+
+def test_rmat_matches_python_restatement():
+    ta, tab, tabc = O.rmat_thresholds(0.57, 0.19, 0.19)
+    assert (ta, tab, tabc) == (37356, 49807, 62259)
+    for scale in (5, 9, 17):
+        src, dst = O.rmat_edges(scale, 300, seed=11, e_begin=(1 << 33) if scale == 17 else 0)
+        for k in range(300):
+            e = k + ((1 << 33) if scale == 17 else 0)
+            assert (int(src[k]), int(dst[k])) == _py_rmat_edge(e, scale, 11, ta, tab, tabc)
+
+
+def test_rmat_quadrant_frequencies():
+    # Top-level bit pair of each edge follows (a, b, c, d).
+    src, dst = O.rmat_edges(16, 200_000, seed=1)
+    q = (src >> 15) * 2 + (dst >> 15)
+    hist = np.bincount(q, minlength=4)
+    exp = np.array([0.57, 0.19, 0.19, 0.05]) * src.shape[0]
+    assert stats.chisquare(hist, exp).pvalue > 1e-4
+    # Level bits are independent draws: low-level pair has the same law.
+    q0 = (src & 1) * 2 + (dst & 1)
+    assert stats.chisquare(np.bincount(q0, minlength=4), exp).pvalue > 1e-4
+
+
+def test_csr_build_matches_lexsort():
+    src, dst = O.rmat_edges(12, 50_000, seed=5)
+    off, col = O.csr_build(1 << 12, src, dst)
+    order = np.lexsort((dst, src))
+    assert np.array_equal(col, dst[order])
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.bincount(src, minlength=1 << 12))]).astype(np.uint64))
+
+
+def test_csr_rejects_bad_vertices():
+    with pytest.raises(ValueError):
+        O.csr_build(4, np.array([0, 5], np.uint32), np.array([1, 1], np.uint32))
+
+
+# -------------------------------------------------------- step programs
+
+
+def cycle(n):
+    off = np.arange(n + 1, dtype=np.uint64)
+    col = ((np.arange(n) + 1) % n).astype(np.uint32)
+    return off, col
+
+
+def test_cycle_p0_is_deterministic():
+    # Out-degree 1 and p = 0: v_k = (s + k) mod n whatever the random stream.
+    off, col = cycle(7)
+    for kind in (O.JUMP, O.STOP, O.RESTART):
+        prog = O.program(kind, 10, 0.0, 7, seed=99)
+        tv, th, fin = O.trace(prog, off, col, O.make_walks([3])[0])
+        assert tv.tolist() == [(3 + k) % 7 for k in range(10)]
+        assert th.tolist() == list(range(10))
+        assert int(fin["vertex"]) == (3 + 10) % 7 and int(fin["hop"]) == 10
+
+
+def test_joint_period_and_restart():
+    off, col = cycle(5)
+    # p = 1: every step restarts at the source; period L = 4 records 3 visits.
+    prog = O.program(O.RESTART_JOINT, 4, 1.0, 5, seed=1)
+    w = O.make_walks([2], sources_=[2], hops=[8])[0]
+    tv, th, fin = O.trace(prog, off, col, w)
+    assert tv.tolist() == [2, 2, 2] and th.tolist() == [8, 9, 10]
+    assert int(fin["hop"]) == 11
+    # p = 0 on the cycle: L - 1 moves then stop.
+    prog = O.program(O.RESTART_JOINT, 11, 0.0, 5, seed=1)
+    tv, th, fin = O.trace(prog, off, col, O.make_walks([0], hops=[0])[0])
+    assert tv.tolist() == [k % 5 for k in range(10)] and int(fin["hop"]) == 10
+
+
+def sink_graph():
+    # 0 -> 1, 1 -> 2, 2 has no out-edges (dangling)
+    off = np.array([0, 1, 2, 2], dtype=np.uint64)
+    col = np.array([1, 2], dtype=np.uint32)
+    return off, col
+
+
+def test_dead_end_rules():
+    off, col = sink_graph()
+    w = O.make_walks([0], sources_=[0])[0]
+    # STOP dies at the sink: visits 0, 1, 2 then ends without moving.
+    tv, _, fin = O.trace(O.program(O.STOP, 10, 0.0, 3, seed=4), off, col, w)
+    assert tv.tolist() == [0, 1, 2] and int(fin["vertex"]) == 2 and int(fin["hop"]) == 2
+    # PROB: same, no cap.
+    tv, _, _ = O.trace(O.program(O.PROB, 1, 0.0, 3, seed=4), off, col, w)
+    assert tv.tolist() == [0, 1, 2]
+    # RESTART returns to the source at the sink: 0 1 2 0 1 2 ...
+    tv, _, _ = O.trace(O.program(O.RESTART, 9, 0.0, 3, seed=4), off, col, w)
+    assert tv.tolist() == [0, 1, 2] * 3
+    # JUMP teleports from the sink: exactly L visits, every 3rd step random.
+    prog = O.program(O.JUMP, 30, 0.0, 3, seed=4)
+    tv, _, fin = O.trace(prog, off, col, w)
+    assert len(tv) == 30 and int(fin["hop"]) == 30
+    for k in range(29):
+        if tv[k] != 2:
+            assert tv[k + 1] == tv[k] + 1
+
+
+def test_p1_never_follows_edges():
+    off, col = cycle(6)
+    tv, _, _ = O.trace(O.program(O.RESTART, 8, 1.0, 6, seed=2), off, col, O.make_walks([4])[0])
+    assert tv.tolist() == [4] * 8
+    tv, _, fin = O.trace(O.program(O.STOP, 8, 1.0, 6, seed=2), off, col, O.make_walks([4])[0])
+    assert tv.tolist() == [4] and int(fin["hop"]) == 0
+
+
+def _py_trace(kind, L, thr, N, seed, off, col, w):
+    """Second, independent implementation of the step loop (pure Python)."""
+    v, hop, src, wid = int(w["vertex"]), int(w["hop"]), int(w["source"]), int(w["id"])
+    out = []
+    while True:
+        if kind in (O.JUMP, O.STOP, O.RESTART) and hop >= L:
+            break
+        out.append((v, hop))
+        r = _py_philox([wid, hop, 0x57414C4B, 0], [seed & 0xFFFFFFFF, seed >> 32])
+        b, deg = int(off[v]), int(off[v + 1] - off[v])
+        if deg > 0 and r[0] >= thr:
+            x = r[1] | (r[2] << 32)
+            v = int(col[b + ((x * deg) >> 64)])
+        elif kind == O.JUMP:
+            v = (r[3] * N) >> 32
+        elif kind in (O.STOP, O.PROB):
+            break
+        else:
+            v = src
+        hop += 1
+        if kind == O.RESTART_JOINT and hop % L == L - 1:
+            break
+    return out
+
+
+@pytest.mark.parametrize("kind", [O.JUMP, O.STOP, O.PROB, O.RESTART_JOINT, O.RESTART])
+def test_trajectories_match_python_restatement(kind):
+    off, col = O.rmat_graph(8, 2048, seed=21)
+    N = 256
+    prog = O.program(kind, 12, 0.15, N, seed=0x1234_5678_9ABC)
+    rng = np.random.default_rng(kind)
+    for i in range(40):
+        v = int(rng.integers(0, N))
+        w = O.make_walks([v], sources_=[v], hops=[0], ids=[int(rng.integers(0, 1 << 32))])[0]
+        tv, th, _ = O.trace(prog, off, col, w)
+        ref = _py_trace(kind, 12, prog.stop_thr, N, prog.seed, off, col, w)
+        assert list(zip(tv.tolist(), th.tolist())) == ref
+
+
+def test_jump_takes_exactly_L_steps():
+    off, col = O.rmat_graph(10, 8000, seed=3)
+    walks = O.make_walks(np.arange(1024))
+    r = O.replay(O.program(O.JUMP, 10, 0.15, 1024, seed=8), off, col, walks, per_walk=True)
+    assert r["steps"] == 10 * 1024 and r["visits"] == 10 * 1024
+    assert np.all(r["final"]["hop"] == 10)
+    assert int(r["counts"].sum()) == 10 * 1024
+
+
+def test_stop_length_is_geometric():
+    # On a graph without dead ends (cycle), STOP visits ~ min(Geom(0.15), L).
+    off, col = cycle(101)
+    L = 20
+    walks = O.make_walks(np.zeros(200_000, dtype=np.uint32))
+    r = O.replay(O.program(O.STOP, L, 0.15, 101, seed=5), off, col, walks, per_walk=True)
+    k = r["walk_visits"].astype(np.int64)
+    pk = np.array([0.85 ** (j - 1) * 0.15 for j in range(1, L)] + [0.85 ** (L - 1)])
+    hist = np.bincount(k, minlength=L + 1)[1:]
+    assert stats.chisquare(hist, pk * walks.shape[0]).pvalue > 1e-4
+    # moves = final hop; a walk that died before the cap moved k - 1 times
+    hop = r["final"]["hop"].astype(np.int64)
+    assert r["steps"] == int(hop.sum()) and r["visits"] == int(k.sum())
+    assert np.all(hop[k < L] == k[k < L] - 1)
+    assert set(np.unique(hop[k == L]).tolist()) <= {L - 1, L}
+
+
+def test_neighbour_choice_is_uniform_over_slots():
+    # star: vertex 0 -> 1..40 (vertex 5 listed twice); others -> 0
+    nb = list(range(1, 41)) + [5]
+    off = np.array([0, len(nb)] + [len(nb) + i for i in range(1, 41)], dtype=np.uint64)
+    col = np.array(nb + [0] * 40, dtype=np.uint32)
+    walks = O.make_walks(np.zeros(100_000, dtype=np.uint32))
+    r = O.replay(O.program(O.JUMP, 2, 0.0, 41, seed=17), off, col, walks)
+    c = r["counts"][1:].astype(np.float64)
+    exp = np.full(40, 1.0)
+    exp[4] = 2.0
+    exp = exp / exp.sum() * c.sum()
+    assert stats.chisquare(c, exp).pvalue > 1e-4
+
+
+def test_walk_ids_key_the_streams():
+    off, col = O.rmat_graph(10, 8000, seed=3)
+    prog = O.program(O.STOP, 10, 0.15, 1024, seed=8)
+    a = O.make_walks(np.arange(64), ids=np.arange(64))
+    b = a[::-1].copy()
+    ra = O.replay(prog, off, col, a, per_walk=True)
+    rb = O.replay(prog, off, col, b, per_walk=True)
+    assert np.array_equal(ra["hash"], rb["hash"][::-1])
+    assert np.array_equal(ra["counts"], rb["counts"])
+
+
+# ------------------------------------------------ FP64 finite horizon
+
+
+def test_restart_counts_match_fp64_within_mc_bound():
+    off, col = O.rmat_graph(12, 1 << 16, seed=1)
+    W, L, p = 200_000, 10, 0.15
+    walks = O.make_walks(np.zeros(W, dtype=np.uint32))
+    r = O.replay(O.program(O.RESTART, L, p, 1 << 12, seed=1), off, col, walks)
+    exp = fp64.expected_visits(O.RESTART, off, col, 0, W, L, p)
+    assert abs(exp.sum() - W * L) < 1e-6 * W * L
+    l1 = fp64.l1_normalised(r["counts"], exp)
+    bound = fp64.mc_l1_bound(exp, W, L, L)
+    assert l1 < bound, (l1, bound)
+    # Expected visits at the source in closed form for the first two hops:
+    # 1 (hop 0) + P(back at 0 at hop 1) where hop-1 mass at 0 = p + (1-p) P[0,0].
+    P, dang = fp64.transition(off, col)
+    assert not dang[0]
+
+
+def test_stop_and_jump_fp64():
+    off, col = O.rmat_graph(11, 1 << 14, seed=2)
+    n = 1 << 11
+    W, L = 150_000, 8
+    walks = O.make_walks(np.full(W, 3, dtype=np.uint32))
+    for kind in (O.STOP, O.JUMP):
+        r = O.replay(O.program(kind, L, 0.15, n, seed=9), off, col, walks)
+        exp = fp64.expected_visits(kind, off, col, 3, W, L, 0.15)
+        assert abs(r["visits"] - exp.sum()) < 5 * np.sqrt(exp.sum() * L)
+        assert fp64.l1_normalised(r["counts"], exp) < fp64.mc_l1_bound(exp, W, exp.sum() / W, L)
+
+
+def test_fp64_single_vertex_closed_form():
+    # one vertex with a self-loop: STOP visits sum_{k<L} (1-p)^k per walk.
+    off = np.array([0, 1], dtype=np.uint64)
+    col = np.array([0], dtype=np.uint32)
+    e = fp64.expected_visits(O.STOP, off, col, 0, 1.0, 6, 0.25)
+    assert abs(e[0] - sum(0.75 ** k for k in range(6))) < 1e-12
+    e = fp64.expected_visits(O.RESTART, off, col, 0, 1.0, 6, 0.25)
+    assert abs(e[0] - 6.0) < 1e-12
+
+
+# ------------------------------------------------ block engine, buckets
+
+
+def test_bucket_stable_matches_numpy():
+    rng = np.random.default_rng(0)
+    w = O.make_walks(rng.integers(0, 1000, 5000))
+    bf = np.array([0, 100, 350, 351, 900, 990], dtype=np.uint32)  # 990..999 out of range
+    out, cnt = O.bucket_stable(w, bf)
+    blk = np.searchsorted(bf, w["vertex"], side="right") - 1
+    keep = w["vertex"] < bf[-1]
+    order = np.argsort(blk[keep], kind="stable")
+    assert np.array_equal(out, w[keep][order])
+    assert cnt[-1] == (~keep).sum()
+
+
+@pytest.mark.parametrize("kind", [O.JUMP, O.STOP, O.RESTART_JOINT, O.RESTART])
+def test_block_engine_equals_whole_graph_replay(kind):
+    off, col = O.rmat_graph(12, 1 << 15, seed=7)
+    n = 1 << 12
+    bf = np.array([0, 300, 1024, 1500, 2048, 4096], dtype=np.uint32)
+    rng = np.random.default_rng(1)
+    v = rng.integers(0, n, 3000).astype(np.uint32)
+    walks = O.make_walks(v, hops=(np.arange(3000) % 5) * 11 if kind == O.RESTART_JOINT else None)
+    prog = O.program(kind, 11, 0.15, n, seed=77)
+    whole = O.replay(prog, off, col, walks)
+    counts, steps, visits, rounds = O.block_engine(prog, off, col, bf, walks)
+    assert np.array_equal(counts, whole["counts"])
+    assert (steps, visits) == (whole["steps"], whole["visits"])
+    assert rounds > 5
