@@ -121,6 +121,10 @@ def main() -> int:
     gathers = 1 << 30
     gms = eng.gather_probe(4 << 30, gathers)
     rand_gbps = gathers * SECTOR / (gms * 1e-3) / 1e9
+    # streaming copy of 1 GiB: sequential ceiling, and the known-size kernel
+    # that calibrates FETCH_SIZE / WRITE_SIZE in profiles
+    copy_bytes = 1 << 30
+    cms = eng.copy_probe(copy_bytes)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.schedule}.json")
@@ -148,6 +152,7 @@ def main() -> int:
                      "bytes_per_step": bstep},
         "random_access_ceiling": {"gathers_per_s": gathers / (gms * 1e-3), "GBps_at_64B": rand_gbps,
                                   "frac_of_ceiling": achieved / rand_gbps},
+        "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
         "plain_kernel": {"walk_steps_per_s": steps_p / wall_p, "kernel_ms": float(np.mean(kms_p)),
                          "speedup": value / (steps_p / wall_p)},
     }

@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         "gw_counts_read": ([P, P, u32], i),
         "gw_summary_read": ([P, P, P, u64], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
+        "gw_copy_probe": ([P, u64, P], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -258,6 +259,11 @@ class Engine:
         v = np.empty(self.nwalks, dtype=np.uint32)
         self._check(self._L.gw_summary_read(self._ctx, _ptr(h), _ptr(v), self.nwalks), "gw_summary_read")
         return h, v
+
+    def copy_probe(self, nbytes: int) -> float:
+        ms = C.c_double()
+        self._check(self._L.gw_copy_probe(self._ctx, nbytes, C.byref(ms)), "gw_copy_probe")
+        return ms.value
 
     def gather_probe(self, nbytes: int, gathers: int) -> float:
         ms = C.c_double()

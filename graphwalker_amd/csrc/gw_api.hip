@@ -134,9 +134,41 @@ __global__ void gather_probe_kernel(const uint32_t* __restrict__ buf, uint32_t n
     out[tid] = acc;
 }
 
+// Streaming copy of `n16` 16-byte words: the known-size kernel that hardware
+// byte counters are calibrated against.
+__global__ void copy_probe_kernel(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t n16) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n16; i += stride) b[i] = a[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int gw_copy_probe(gw_ctx* ctx, uint64_t bytes, double* ms) {
+    if (!ctx || bytes < 4096 || bytes % 16) return gw_fail(ctx, GW_EINVAL, "gw_copy_probe: bytes must be >= 4096 and a multiple of 16");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    uint4 *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&a, bytes, "copy src"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&b, bytes, "copy dst"))) { gw_free(ctx, a, bytes); return rc; }
+    const uint32_t tb = 256, grid = std::max(1u, ctx->cu_count) * 16;
+    hipError_t e = hipMemsetAsync(a, 3, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess) {
+        copy_probe_kernel<<<dim3(grid), dim3(tb), 0, ctx->stream>>>(a, b, bytes / 16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    gw_free(ctx, a, bytes);
+    gw_free(ctx, b, bytes);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "copy probe");
+    if (ms) *ms = t;
+    return GW_OK;
+}
 
 int gw_create(gw_ctx** out, const gw_cfg* cfg) {
     if (!out) return GW_EINVAL;

@@ -153,6 +153,9 @@ int gw_summary_read(gw_ctx *ctx, uint64_t *hash, uint32_t *visits, uint64_t n);
 /* Random-gather ceiling: `gathers` independent random 4-byte loads from a
  * buffer of `bytes` bytes; returns the kernel time in ms. */
 int gw_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);
+/* Streaming copy of `bytes` bytes between two fresh buffers (the known-size
+ * kernel hardware byte counters are calibrated with); kernel time in ms. */
+int gw_copy_probe(gw_ctx *ctx, uint64_t bytes, double *ms);
 
 #ifdef __cplusplus
 }
