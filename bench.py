@@ -38,10 +38,13 @@ WORKLOADS = {
 }
 
 
-def bytes_per_step(recorder: int) -> int:
-    # one random sector for the offset pair, one for the column, one more for
-    # a per-visit count atomic
-    return SECTOR * (2 + (1 if recorder & gw.REC_VERTEX_COUNT else 0))
+def bytes_per_step(recorder: int, layout: int) -> int:
+    """Algorithmic random-access bytes per walk-step, at 64-B sector
+    granularity: the CSR layout reads one sector for the offset pair and one
+    for the column; the successor-header layout reads one 16-B edge record
+    (one sector).  A per-visit count adds one sector (its atomic)."""
+    reads = 2 if layout == gw.LAYOUT_CSR else 1
+    return SECTOR * (reads + (1 if recorder & gw.REC_VERTEX_COUNT else 0))
 
 
 def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) -> dict:
@@ -74,7 +77,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rawwalks-rmat24", choices=sorted(WORKLOADS))
-    ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent"])
+    ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent", "blocked"])
+    ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -92,30 +96,42 @@ def main() -> int:
     eng.walks_init_per_vertex(1)
     print(f"[bench] graph + walks ready in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
 
-    def make(sched):
-        return gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched)
+    bf = np.arange(0, n + 1, wl["block_vertices"], dtype=np.uint32)
+    eng.graph_blocks(bf)
 
-    def timed(sched, K, W):
-        prog = make(sched)
+    def make(sched, layout):
+        return gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
+                          layout=layout)
+
+    def timed(sched, layout, K, W):
+        prog = make(sched, layout)
         for _ in range(W):
             eng.run(prog)
         t = time.perf_counter()
-        kms, steps = [], 0
+        kms, steps, st = [], 0, None
         for _ in range(K):
             st = eng.run(prog)
             kms.append(st["step_ms"])
             steps += st["steps"]
         wall = time.perf_counter() - t
-        return prog, wall, steps, kms
+        return prog, wall, steps, kms, st
 
-    # the plain one-walk-per-thread kernel, as the in-run baseline
-    _, wall_p, steps_p, kms_p = timed("plain", max(3, args.steps // 2), 1)
-    prog, wall, steps, kms = timed(args.schedule, args.steps, args.warmup)
+    # the other schedules/layouts of the same workload, for comparison
+    variants = {}
+    for sched, layout in (("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "ext")):
+        if (sched, layout) == (args.schedule, args.layout):
+            continue
+        _, w_, s_, k_, st_ = timed(sched, layout, max(2, args.steps // 3), 1)
+        variants[f"{sched}-{layout}"] = {"walk_steps_per_s": s_ / w_, "kernel_ms": float(np.mean(k_)),
+                                         "rounds": st_["rounds"], "bucket_ms": st_["bucket_ms"]}
+        print(f"[bench] {sched}-{layout}: {s_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+    prog, wall, steps, kms, st = timed(args.schedule, args.layout, args.steps, args.warmup)
     steps_per_pass = steps // args.steps
     value = steps / wall
     kavg = float(np.mean(kms))
-    bstep = bytes_per_step(wl["recorder"])
+    bstep = bytes_per_step(wl["recorder"], st["layout"])
     achieved = steps_per_pass * bstep / (kavg * 1e-3) / 1e9
+    print(f"[bench] {args.schedule}-{args.layout}: {value / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
 
     # random-gather ceiling: independent 4-byte gathers over 4 GiB
     gathers = 1 << 30
@@ -127,7 +143,7 @@ def main() -> int:
     cms = eng.copy_probe(copy_bytes)
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.schedule}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.schedule}_{args.layout}.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
@@ -145,17 +161,18 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic (R-MAT generated on the device)",
         "config": {"workload": args.workload, "desc": wl["desc"], "walks": n, "walk_steps_per_pass": steps_per_pass,
-                   "schedule": args.schedule, "device": info["name"]},
+                   "schedule": args.schedule, "layout": args.layout, "device": info["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
-                     "kernel": f"step_{args.schedule}_kernel", "kernel_ms": kavg,
+                     "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": kavg,
                      "bytes_per_step": bstep},
         "random_access_ceiling": {"gathers_per_s": gathers / (gms * 1e-3), "GBps_at_64B": rand_gbps,
                                   "frac_of_ceiling": achieved / rand_gbps},
         "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
-        "plain_kernel": {"walk_steps_per_s": steps_p / wall_p, "kernel_ms": float(np.mean(kms_p)),
-                         "speedup": value / (steps_p / wall_p)},
+        "variants": variants,
     }
+    if "plain-csr" in variants:
+        out["speedup_vs_plain"] = value / variants["plain-csr"]["walk_steps_per_s"]
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(eng, wl, prog, args.cpu_seconds)

@@ -27,6 +27,9 @@ REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY = 0, 1, 2
 # schedules
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
 SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}
+# graph layouts the step kernels read
+LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_EXT = 0, 1, 2
+LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
 
@@ -53,6 +56,8 @@ class Program(C.Structure):
         ("schedule", C.c_uint32),
         ("block_threads", C.c_uint32),
         ("waves_per_simd", C.c_uint32),
+        ("layout", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -66,11 +71,11 @@ class Stats(C.Structure):
         ("total_ms", C.c_double),
         ("bucket_ms", C.c_double),
         ("schedule", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("layout", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def stop_threshold(p: float) -> int:
@@ -82,13 +87,16 @@ def stop_threshold(p: float) -> int:
 
 
 def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
-            schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0) -> Program:
+            schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0,
+            layout=LAYOUT_AUTO) -> Program:
     if isinstance(kind, str):
         kind = KIND_NAMES[kind]
     if isinstance(schedule, str):
         schedule = SCHED_NAMES[schedule]
+    if isinstance(layout, str):
+        layout = LAYOUT_NAMES[layout]
     return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
-                   waves_per_simd)
+                   waves_per_simd, layout, 0)
 
 
 def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:

@@ -51,6 +51,9 @@ void free_graph(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_col, ctx->m * 4);
     gw_free(ctx, ctx->d_counts, ctx->n * 4ull);
     gw_free(ctx, ctx->d_block_first, (ctx->nblocks + 1ull) * 4);
+    gw_free(ctx, ctx->d_ext, std::max<uint64_t>(1, ctx->m) * 16);
+    ctx->d_ext = nullptr;
+    ctx->ext_failed = false;
     ctx->d_off = nullptr;
     ctx->d_col = nullptr;
     ctx->d_counts = nullptr;
@@ -65,6 +68,8 @@ void free_walks(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_walks, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_walks_alt, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_final, ctx->walk_cap * sizeof(gw_walk));
+    gw_free(ctx, ctx->d_work, ctx->walk_cap * sizeof(gw_walk));
+    ctx->d_work = nullptr;
     gw_free(ctx, ctx->d_hash, ctx->walk_cap * 8);
     gw_free(ctx, ctx->d_visits, ctx->walk_cap * 4);
     ctx->d_walks = ctx->d_walks_alt = ctx->d_final = nullptr;
@@ -260,6 +265,7 @@ int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 int gw_graph_blocks(gw_ctx* ctx, const uint32_t* block_first, uint32_t nblocks) {
     if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_blocks: no graph");
     if (!block_first || nblocks == 0) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: empty block table");
+    if (nblocks > 2048) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: at most 2048 blocks");
     if (block_first[0] != 0 || block_first[nblocks] != ctx->n)
         return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: table must start at 0 and end at n");
     for (uint32_t b = 0; b < nblocks; ++b)

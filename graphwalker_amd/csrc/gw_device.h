@@ -51,9 +51,19 @@ struct Prog {
     uint32_t stream;
 };
 
+// Successor-header edge (16 bytes): the column entry plus the destination's
+// degree and row start, so one random read per step yields everything the
+// next step needs (the CSR layout needs two dependent reads: offsets, column).
+struct __attribute__((aligned(16))) Edge {
+    uint32_t dst;
+    uint32_t deg;
+    uint64_t row;
+};
+
 struct Graph {
     const uint64_t* __restrict__ off;
     const uint32_t* __restrict__ col;
+    const Edge* __restrict__ ext;  // null unless the successor-header layout is built
     uint32_t n;
 };
 

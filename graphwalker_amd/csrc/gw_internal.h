@@ -25,6 +25,8 @@ struct gw_ctx {
     uint32_t nblocks = 0;
     std::vector<uint32_t> block_first;  // host copy, nblocks + 1
     uint32_t* d_block_first = nullptr;
+    void* d_ext = nullptr;              // successor-header edges (m x 16 B), built on first use
+    bool ext_failed = false;
 
     // walks
     uint64_t nwalks = 0;
@@ -33,6 +35,7 @@ struct gw_ctx {
     gw_walk* d_walks = nullptr;
     gw_walk* d_walks_alt = nullptr;
     gw_walk* d_final = nullptr;      // final record of each walk, indexed by walk id
+    gw_walk* d_work = nullptr;       // blocked schedule: second bucket buffer
 
     // recorder outputs
     uint32_t* d_counts = nullptr;   // n
@@ -46,7 +49,7 @@ struct gw_ctx {
     uint64_t tile_hist_cap = 0;
     unsigned long long* d_bucket_off = nullptr;  // nblocks + 2
 
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 // Record an error message on the context and return `code`.

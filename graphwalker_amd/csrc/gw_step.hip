@@ -1,16 +1,22 @@
 // gw_step.hip -- step kernels: the device form of exec_updates + updateByWalk
-// (src/engine/graphwalker.hpp:257-270, src/walks/randomwalkwith*.hpp).
+// (src/engine/graphwalker.hpp:257-311, src/walks/randomwalkwith*.hpp).
 //
-// PLAIN       one walk per thread, each thread runs its walk to completion over
-//             the whole resident graph.  The straightforward baseline: lanes of a
-//             wavefront whose walks end early idle until the longest walk ends.
+// PLAIN       one walk per thread, run to completion over the whole resident
+//             CSR.  The straightforward baseline: lanes whose walks end early
+//             idle until the longest walk of their wavefront ends, and every
+//             step makes two dependent random reads (offsets, then column).
 // PERSISTENT  a grid sized to the machine; each wavefront keeps 64 walks in
 //             flight and refills a lane as soon as its walk ends, from a
 //             wave-private chunk of walk indices (one global atomic per chunk).
-//             Uneven, geometric walk lengths then cost no idle lanes.
+//             With the successor-header layout a step is one random read.
+// BLOCKED     the reference's block loop on the device: walks are bucketed by
+//             block (stable counting sort, gw_bucket.hip), each block's bucket
+//             steps while its walks stay inside the block, walks that leave are
+//             re-bucketed for the next round, finished walks drop out.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "gw_device.h"
 #include "gw_internal.h"
@@ -18,32 +24,79 @@
 namespace gw {
 namespace {
 
-constexpr uint32_t kChunk = 512;  // walk indices a wavefront claims per atomic
+constexpr uint32_t kChunk = 512;           // walk indices a wavefront claims per atomic
+constexpr uint32_t kDead = 0xFFFFFFFFu;    // vertex of a finished walk in the blocked schedule
 
-// One step of a walk on the whole graph.  Returns true when the walk has
-// ended (the final vertex/hop are left in v/hop, as the reference leaves them
-// in the record it drops).
-template <int KIND, class Rec>
-__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, uint32_t id, uint32_t source,
-                                          uint32_t& v, uint32_t& hop, Rec& rec) {
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Walk state carried in registers.  With the successor-header layout the
+// current vertex's row start and degree ride along, fetched with the edge
+// that led to the vertex.
+struct State {
+    uint32_t id, source, v, hop;
+    uint64_t row;
+    uint32_t deg;
+};
+
+template <bool EXT>
+__device__ __forceinline__ void load_row(const Graph& g, State& s) {
+    if (EXT) {
+        const uint64_t b = g.off[s.v];
+        s.row = b;
+        s.deg = static_cast<uint32_t>(g.off[s.v + 1] - b);
+    }
+}
+
+// One step of a walk.  Returns true when the walk has ended; the final
+// vertex/hop stay in the state, as the reference leaves them in the record it
+// drops.  Else-branches per kind follow advance() in gw_device.h.
+template <int KIND, bool EXT, class Rec>
+__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
     constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
-    if (capped && hop >= p.L) return true;
-    rec.visit(v, hop);
-    const u32x4 r = philox4x32_10(id, hop, kTagWalk, p.stream, p.k0, p.k1);
-    const uint64_t b = g.off[v];
-    const uint64_t deg = g.off[v + 1] - b;
+    if (capped && s.hop >= p.L) return true;
+    rec.visit(s.v, s.hop);
+    const u32x4 r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
+    uint64_t b, deg;
+    if (EXT) {
+        b = s.row;
+        deg = s.deg;
+    } else {
+        b = g.off[s.v];
+        deg = g.off[s.v + 1] - b;
+    }
     if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
         const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
-        v = g.col[b + mulhi64(x, deg)];
+        const uint64_t e = b + mulhi64(x, deg);
+        if (EXT) {
+            const Edge ed = g.ext[e];
+            s.v = ed.dst;
+            s.deg = ed.deg;
+            s.row = ed.row;
+        } else {
+            s.v = g.col[e];
+        }
     } else {
-        if (KIND == GW_JUMP) v = mulhi32(r.w, p.nvertices);
-        else if (KIND == GW_STOP || KIND == GW_PROB) return true;
-        else v = source;
+        if (KIND == GW_JUMP) {
+            s.v = mulhi32(r.w, p.nvertices);
+            load_row<EXT>(g, s);
+        } else if (KIND == GW_STOP || KIND == GW_PROB) {
+            return true;
+        } else {
+            s.v = s.source;
+            load_row<EXT>(g, s);
+        }
     }
-    ++hop;
+    ++s.hop;
     ++rec.steps;
-    if (KIND == GW_RESTART_JOINT) return hop % p.L == p.L - 1;
-    if (capped) return hop >= p.L;
+    if (KIND == GW_RESTART_JOINT) return s.hop % p.L == p.L - 1;
+    if (capped) return s.hop >= p.L;
     return false;
 }
 
@@ -72,16 +125,10 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
     }
 }
 
-__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
-
-template <int KIND, bool VC, bool SUM>
+template <int KIND, bool VC, bool SUM, bool EXT>
 __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, const gw_walk* __restrict__ walks,
-                                                             gw_walk* __restrict__ final_walks, uint64_t nw, uint32_t* __restrict__ counts,
+                                                             gw_walk* __restrict__ final_walks, uint64_t nw,
+                                                             uint32_t* __restrict__ counts,
                                                              uint64_t* __restrict__ hash,
                                                              uint32_t* __restrict__ visits,
                                                              unsigned long long* __restrict__ counters) {
@@ -89,8 +136,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
     bool active = false;
-    uint64_t slot = 0;
-    uint32_t id = 0, source = 0, v = 0, hop = 0;
+    State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
     Recorder<VC, SUM> rec{counts, 0u, 0u, kHashInit};
 
@@ -114,12 +160,12 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             const uint32_t take = avail < want ? static_cast<uint32_t>(avail) : want;
             const uint32_t r = rank_in(idle);
             if (!active && r < take) {
-                slot = chunk_next + r;
-                const gw_walk w = walks[slot];
-                id = w.id;
-                source = w.source;
-                v = w.vertex;
-                hop = w.hop;
+                const gw_walk w = walks[chunk_next + r];
+                s.id = w.id;
+                s.source = w.source;
+                s.v = w.vertex;
+                s.hop = w.hop;
+                load_row<EXT>(g, s);
                 active = true;
                 rec.visits = 0;
                 rec.steps = 0;
@@ -130,11 +176,11 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            if (step_once<KIND>(p, g, id, source, v, hop, rec)) {
-                final_walks[id] = gw_walk{id, source, v, hop};
+            if (step_once<KIND, EXT>(p, g, s, rec)) {
+                final_walks[s.id] = gw_walk{s.id, s.source, s.v, s.hop};
                 if (SUM) {
-                    hash[id] = rec.hash;
-                    visits[id] = rec.visits;
+                    hash[s.id] = rec.hash;
+                    visits[s.id] = rec.visits;
                 }
                 tot_steps += rec.steps;
                 tot_visits += rec.visits;
@@ -150,6 +196,135 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     }
 }
 
+constexpr uint32_t kMaxBlocks = 2048;
+
+__device__ __forceinline__ void block_bounds(const uint32_t* bf, uint32_t nb, uint32_t v, uint32_t& lo,
+                                             uint32_t& hi) {
+    uint32_t a = 0, b = nb;  // bf[a] <= v < bf[b]
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (bf[mid] <= v) a = mid; else b = mid;
+    }
+    lo = bf[a];
+    hi = bf[a + 1];
+}
+
+// One block round over all buckets, which lie back to back in block order in
+// [walks, walks + nw).  Persistent wavefronts take walks in bucket order (so
+// the blocks being stepped at any moment are one or two consecutive ones),
+// and a walk steps while its vertex stays inside its block.  A walk that ends
+// writes its final record and leaves a dead slot (vertex = kDead, dropped by
+// the next bucketing); a walk that leaves the block is written back in place
+// with its new vertex/hop for the next round.
+template <int KIND, bool VC, bool SUM, bool EXT>
+__global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const uint32_t* __restrict__ block_first,
+                                                        uint32_t nb, gw_walk* __restrict__ walks, uint64_t nw,
+                                                        gw_walk* __restrict__ final_walks,
+                                                        uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
+                                                        uint32_t* __restrict__ visits,
+                                                        unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t s_bf[kMaxBlocks + 1];
+    for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_bf[i] = block_first[i];
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
+    bool exhausted = false;                  // wave-uniform
+    bool active = false;
+    uint64_t slot = 0;
+    uint32_t lo = 0, hi = 0, visits0 = 0;
+    State s{0, 0, 0, 0, 0, 0};
+    uint32_t tot_steps = 0, tot_visits = 0;
+    Recorder<VC, SUM> rec{counts, 0u, 0u, kHashInit};
+
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        while (idle && !exhausted) {
+            if (chunk_next == chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(kChunk));
+                base = __shfl(base, 0, 64);
+                if (base >= nw) {
+                    exhausted = true;
+                    break;
+                }
+                chunk_next = base;
+                chunk_end = std::min<uint64_t>(base + kChunk, nw);
+            }
+            const uint32_t want = __popcll(idle);
+            const uint64_t avail = chunk_end - chunk_next;
+            const uint32_t take = avail < want ? static_cast<uint32_t>(avail) : want;
+            const uint32_t r = rank_in(idle);
+            if (!active && r < take) {
+                slot = chunk_next + r;
+                const gw_walk w = walks[slot];
+                s.id = w.id;
+                s.source = w.source;
+                s.v = w.vertex;
+                s.hop = w.hop;
+                block_bounds(s_bf, nb, s.v, lo, hi);
+                load_row<EXT>(g, s);
+                if (SUM) {
+                    rec.hash = hash[s.id];
+                    rec.visits = visits[s.id];
+                } else {
+                    rec.visits = 0;
+                }
+                visits0 = rec.visits;
+                rec.steps = 0;
+                active = true;
+            }
+            chunk_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            bool done = false, left = false;
+            if (s.v - lo < hi - lo) {
+                done = step_once<KIND, EXT>(p, g, s, rec);
+            } else {
+                left = true;
+            }
+            if (done || left) {
+                if (done) final_walks[s.id] = gw_walk{s.id, s.source, s.v, s.hop};
+                walks[slot] = gw_walk{s.id, s.source, done ? kDead : s.v, s.hop};
+                if (SUM) {
+                    hash[s.id] = rec.hash;
+                    visits[s.id] = rec.visits;
+                }
+                tot_steps += rec.steps;
+                tot_visits += rec.visits - visits0;
+                active = false;
+            }
+        }
+    }
+    const uint32_t st = wave_sum(tot_steps);
+    const uint32_t vi = wave_sum(tot_visits);
+    if (lane == 0) {
+        atomicAdd(&counters[0], static_cast<unsigned long long>(st));
+        atomicAdd(&counters[1], static_cast<unsigned long long>(vi));
+    }
+}
+
+__global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t nw, uint64_t* __restrict__ hash,
+                                    uint32_t* __restrict__ visits) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nw) return;
+    const uint32_t id = walks[i].id;
+    hash[id] = kHashInit;
+    visits[id] = 0;
+}
+
+__global__ void build_ext_kernel(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col, uint64_t m,
+                                 Edge* __restrict__ ext, unsigned long long* __restrict__ overflow) {
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t d = col[e];
+    const uint64_t b = off[d];
+    const uint64_t deg = off[d + 1] - b;
+    if (deg > 0xFFFFFFFFull) atomicAdd(overflow, 1ull);
+    ext[e] = Edge{d, static_cast<uint32_t>(deg), b};
+}
+
 struct Launch {
     Prog p;
     Graph g;
@@ -157,20 +332,34 @@ struct Launch {
     uint32_t sched;
     uint32_t tb;
     uint32_t grid_persistent;
+    bool ext;
+    // blocked schedule: the buckets to step (back to back, block order)
+    gw_walk* bucket;
+    uint64_t bucket_n;
 };
 
-template <int KIND, bool VC, bool SUM>
-hipError_t launch3(const Launch& L) {
+template <int KIND, bool VC, bool SUM, bool EXT>
+hipError_t launch4(const Launch& L) {
     gw_ctx* c = L.ctx;
     if (L.sched == GW_SCHED_PLAIN) {
         const uint32_t grid = static_cast<uint32_t>((c->nwalks + L.tb - 1) / L.tb);
         step_plain_kernel<KIND, VC, SUM><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters);
-    } else {
-        step_persistent_kernel<KIND, VC, SUM><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+    } else if (L.sched == GW_SCHED_PERSISTENT) {
+        step_persistent_kernel<KIND, VC, SUM, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters);
+    } else {
+        step_block_kernel<KIND, VC, SUM, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+            L.p, L.g, c->d_block_first, c->nblocks, L.bucket, L.bucket_n, c->d_final, c->d_counts, c->d_hash,
+            c->d_visits, c->d_counters);
     }
     return hipGetLastError();
+}
+
+template <int KIND, bool VC, bool SUM>
+hipError_t launch3(const Launch& L) {
+    if (L.ext && L.sched != GW_SCHED_PLAIN) return launch4<KIND, VC, SUM, true>(L);
+    return launch4<KIND, VC, SUM, false>(L);
 }
 
 template <int KIND>
@@ -192,27 +381,115 @@ hipError_t launch(const Launch& L, uint32_t kind, uint32_t rec) {
     }
 }
 
+// Builds the successor-header edges once per graph.  Returns false (and
+// leaves the CSR as the only layout) when it does not fit the HBM budget.
+bool ensure_ext(gw_ctx* ctx) {
+    if (ctx->d_ext) return true;
+    if (ctx->ext_failed) return false;
+    if (gw_alloc(ctx, (void**)&ctx->d_ext, std::max<uint64_t>(1, ctx->m) * sizeof(Edge), "successor-header edges")) {
+        ctx->ext_failed = true;
+        return false;
+    }
+    hipError_t e = hipMemsetAsync(ctx->d_counters + 9, 0, 8, ctx->stream);
+    if (e == hipSuccess && ctx->m) {
+        build_ext_kernel<<<dim3(static_cast<uint32_t>((ctx->m + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            ctx->d_off, ctx->d_col, ctx->m, reinterpret_cast<Edge*>(ctx->d_ext), ctx->d_counters + 9);
+        e = hipGetLastError();
+    }
+    unsigned long long overflow = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&overflow, ctx->d_counters + 9, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess || overflow) {
+        gw_free(ctx, ctx->d_ext, std::max<uint64_t>(1, ctx->m) * sizeof(Edge));
+        ctx->d_ext = nullptr;
+        ctx->ext_failed = true;
+        return false;
+    }
+    return true;
+}
+
+int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, gw_stats& st) {
+    // Working copy of the walks: the loaded records stay untouched so a run
+    // can be repeated.
+    gw_walk* a = ctx->d_walks_alt;
+    gw_walk* b = ctx->d_work;
+    GW_HIP(ctx, hipMemcpyAsync(a, ctx->d_walks, ctx->nwalks * sizeof(gw_walk), hipMemcpyDeviceToDevice, ctx->stream));
+    if (prog->recorder & GW_REC_WALK_SUMMARY) {
+        summary_init_kernel<<<dim3(static_cast<uint32_t>((ctx->nwalks + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            ctx->d_walks, ctx->nwalks, ctx->d_hash, ctx->d_visits);
+        GW_HIP(ctx, hipGetLastError());
+    }
+    const uint32_t nb = ctx->nblocks;
+    uint64_t live = ctx->nwalks;
+    std::vector<uint64_t> off;
+    float ms = 0.f;
+    double step_ms = 0, bucket_ms = 0;
+    while (live > 0) {
+        GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        int rc = gw_bucket_sort(ctx, a, live, b, off);
+        if (rc) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[3]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        bucket_ms += ms;
+        live = off[nb];
+        std::swap(a, b);  // buckets now in `a`
+        if (live == 0) break;
+        // One launch steps every bucket; wavefronts take walks in block order.
+        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 2, 0, sizeof(unsigned long long), ctx->stream));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        L.bucket = a;
+        L.bucket_n = live;
+        hipError_t e = launch(L, prog->kind, prog->recorder);
+        if (e != hipSuccess) return gw_hip_check(ctx, e, "block step launch");
+        ++st.launches;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        step_ms += ms;
+        ++st.rounds;
+    }
+    st.step_ms = step_ms;
+    st.bucket_ms = bucket_ms;
+    return GW_OK;
+}
+
 }  // namespace
 }  // namespace gw
 
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     using namespace gw;
     gw_stats st{};
-    uint32_t sched = prog->schedule == GW_SCHED_AUTO ? GW_SCHED_PERSISTENT : prog->schedule;
-    if (sched != GW_SCHED_PLAIN && sched != GW_SCHED_PERSISTENT)
-        return gw_fail(ctx, GW_EINVAL, "gw_run: schedule not available");
+    const uint32_t sched = prog->schedule == GW_SCHED_AUTO ? GW_SCHED_PERSISTENT : prog->schedule;
+    if (sched != GW_SCHED_PLAIN && sched != GW_SCHED_PERSISTENT && sched != GW_SCHED_BLOCKED)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: unknown schedule");
+    if (prog->layout > GW_LAYOUT_EXT) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown layout");
+    if (sched == GW_SCHED_PLAIN && prog->layout == GW_LAYOUT_EXT)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: the plain schedule reads the CSR layout");
     st.schedule = sched;
+    bool ext = false;
+    if (sched != GW_SCHED_PLAIN && prog->layout != GW_LAYOUT_CSR) {
+        ext = ensure_ext(ctx);
+        if (!ext && prog->layout == GW_LAYOUT_EXT)
+            return gw_fail(ctx, GW_ENOMEM, "gw_run: successor-header layout does not fit (HBM budget or degree >= 2^32)");
+    }
+    st.layout = ext ? GW_LAYOUT_EXT : GW_LAYOUT_CSR;
     if ((prog->recorder & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
         int rc;
         if ((rc = gw_alloc(ctx, (void**)&ctx->d_hash, ctx->walk_cap * 8, "walk hash"))) return rc;
         if ((rc = gw_alloc(ctx, (void**)&ctx->d_visits, ctx->walk_cap * 4, "walk visits"))) return rc;
     }
-    Launch L;
+    if (sched == GW_SCHED_BLOCKED && !ctx->d_work && ctx->walk_cap) {
+        int rc = gw_alloc(ctx, (void**)&ctx->d_work, ctx->walk_cap * sizeof(gw_walk), "blocked work buffer");
+        if (rc) return rc;
+    }
+    Launch L{};
     L.ctx = ctx;
     L.sched = sched;
+    L.ext = ext;
     L.p = Prog{prog->L, prog->nvertices, prog->stop_thr, static_cast<uint32_t>(prog->seed),
                static_cast<uint32_t>(prog->seed >> 32), prog->stream};
-    L.g = Graph{ctx->d_off, ctx->d_col, ctx->n};
+    L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -221,23 +498,33 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
 
     GW_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
     if (prog->recorder & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
-    GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if (ctx->nwalks) {
-        hipError_t e = launch(L, prog->kind, prog->recorder);
-        if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
-        st.launches = 1;
-    }
-    GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
     float ms = 0.f;
-    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (sched == GW_SCHED_BLOCKED) {
+        GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        int rc = ctx->nwalks ? run_blocked(ctx, L, prog, st) : GW_OK;
+        if (rc) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+        st.total_ms = ms;
+    } else {
+        GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        if (ctx->nwalks) {
+            hipError_t e = launch(L, prog->kind, prog->recorder);
+            if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
+            st.launches = 1;
+        }
+        GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        st.rounds = 1;
+        st.step_ms = ms;
+        st.total_ms = ms;
+    }
     unsigned long long cnt[4];
     GW_HIP(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     st.steps = cnt[0];
     st.visits = cnt[1];
-    st.rounds = 1;
-    st.step_ms = ms;
-    st.total_ms = ms;
     if (stats) *stats = st;
     return GW_OK;
 }

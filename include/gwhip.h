@@ -98,7 +98,14 @@ typedef struct gw_program {
     uint32_t schedule;  /* GW_SCHED_* */
     uint32_t block_threads;   /* 0 = default workgroup size */
     uint32_t waves_per_simd;  /* persistent schedule occupancy target, 0 = default */
+    uint32_t layout;          /* GW_LAYOUT_*: graph layout the step kernel reads */
+    uint32_t reserved;
 } gw_program;
+
+/* graph layouts the step kernels read (the CSR is always resident) */
+#define GW_LAYOUT_AUTO 0u
+#define GW_LAYOUT_CSR 1u  /* offsets then column: two dependent random reads per step */
+#define GW_LAYOUT_EXT 2u  /* successor-header edges {dst, deg(dst), row(dst)}: one random read per step */
 
 typedef struct gw_stats {
     uint64_t steps;        /* hop increments (device counter) */
@@ -109,7 +116,7 @@ typedef struct gw_stats {
     double total_ms;       /* whole gw_run on the engine stream (includes bucketing) */
     double bucket_ms;      /* bucketing kernels (BLOCKED) */
     uint32_t schedule;     /* schedule actually used */
-    uint32_t reserved;
+    uint32_t layout;       /* layout actually used */
 } gw_stats;
 
 int gw_create(gw_ctx **out, const gw_cfg *cfg);

@@ -13,7 +13,9 @@ from oracle import fp64
 pytestmark = pytest.mark.gpu
 
 KINDS = [gw.JUMP, gw.STOP, gw.PROB, gw.RESTART_JOINT, gw.RESTART]
-SCHEDS = [gw.SCHED_PLAIN, gw.SCHED_PERSISTENT]
+SCHEDS = [gw.SCHED_PLAIN, gw.SCHED_PERSISTENT, gw.SCHED_BLOCKED]
+# (schedule, layout) pairs the engine runs
+VARIANTS = [("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "csr"), ("blocked", "ext")]
 
 
 def oprog(p: gw.Program) -> O.Program:
@@ -43,17 +45,22 @@ def _walks(n, nv, kind, seed=0):
     return w
 
 
-@pytest.mark.parametrize("sched", SCHEDS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=["-".join(v) for v in VARIANTS])
 @pytest.mark.parametrize("kind", KINDS)
-def test_step_programs_bit_exact(engine, small_graph, kind, sched):
+def test_step_programs_bit_exact(engine, small_graph, kind, variant):
     off, col = small_graph
     n = off.shape[0] - 1
     engine.graph_set(off, col)
+    if variant[0] == "blocked":
+        engine.graph_blocks([0, 100, 700, 1024, 2500, 4095, n])
     walks = _walks(20_000, n, kind, seed=kind)
     engine.walks_load(walks)
     prog = gw.program(kind, 11, 0.15, n, seed=0xDEADBEEF12345, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
-                      schedule=sched)
+                      schedule=variant[0], layout=variant[1])
     st = engine.run(prog)
+    assert st["layout"] == gw.LAYOUT_NAMES[variant[1]]
+    if variant[0] == "blocked":
+        assert st["rounds"] > 1
     ref = O.replay(oprog(prog), off, col, walks, per_walk=True)
     assert st["steps"] == ref["steps"] and st["visits"] == ref["visits"]
     assert np.array_equal(engine.counts(), ref["counts"])
@@ -69,12 +76,14 @@ def test_config1_restart_ppr(engine):
     10 hops: 1,000,000 steps exactly, counts bit-exact, L1 to FP64 within the
     Monte-Carlo bound."""
     engine.graph_rmat(16, 1 << 20, 1)
+    engine.graph_blocks(np.arange(0, (1 << 16) + 1, 4096))
     off, col = O.rmat_graph(16, 1 << 20, 1)
     W, L = 100_000, 10
     engine.walks_init_sources([0], W)
     prog = gw.program(gw.RESTART, L, 0.15, 1 << 16, seed=1, recorder=gw.REC_VERTEX_COUNT)
-    for sched in SCHEDS:
-        prog.schedule = sched
+    for sched, layout in VARIANTS:
+        prog.schedule = gw.SCHED_NAMES[sched]
+        prog.layout = gw.LAYOUT_NAMES[layout]
         st = engine.run(prog)
         assert st["steps"] == W * L and st["visits"] == W * L
         c = engine.counts()
