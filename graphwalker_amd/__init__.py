@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgwhip.so")
 JUMP, STOP, PROB, RESTART_JOINT, RESTART = 0, 1, 2, 3, 4
 KIND_NAMES = {"jump": JUMP, "stop": STOP, "prob": PROB, "restart_joint": RESTART_JOINT, "restart": RESTART}
 # recorders
-REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY = 0, 1, 2
+REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY, REC_EVENTS = 0, 1, 2, 4
 # schedules
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
 SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}
@@ -57,7 +57,7 @@ class Program(C.Structure):
         ("block_threads", C.c_uint32),
         ("waves_per_simd", C.c_uint32),
         ("layout", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("event_cap", C.c_uint32),
     ]
 
 
@@ -88,7 +88,7 @@ def stop_threshold(p: float) -> int:
 
 def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
             schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0,
-            layout=LAYOUT_AUTO) -> Program:
+            layout=LAYOUT_AUTO, event_cap: int = 0) -> Program:
     if isinstance(kind, str):
         kind = KIND_NAMES[kind]
     if isinstance(schedule, str):
@@ -96,7 +96,7 @@ def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, re
     if isinstance(layout, str):
         layout = LAYOUT_NAMES[layout]
     return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
-                   waves_per_simd, layout, 0)
+                   waves_per_simd, layout, event_cap)
 
 
 def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:
@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
         "gw_bucket": ([P, P], i),
         "gw_counts_read": ([P, P, u32], i),
         "gw_summary_read": ([P, P, P, u64], i),
+        "gw_events_read": ([P, P, u64, u32], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
     }
@@ -272,6 +273,12 @@ class Engine:
         ms = C.c_double()
         self._check(self._L.gw_copy_probe(self._ctx, nbytes, C.byref(ms)), "gw_copy_probe")
         return ms.value
+
+    def events(self, cap: int) -> np.ndarray:
+        """Event log of the last run: (nwalks, cap) u64, (hop << 32) | vertex."""
+        out = np.empty((self.nwalks, cap), dtype=np.uint64)
+        self._check(self._L.gw_events_read(self._ctx, _ptr(out), self.nwalks, cap), "gw_events_read")
+        return out
 
     def gather_probe(self, nbytes: int, gathers: int) -> float:
         ms = C.c_double()

@@ -18,6 +18,10 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libgwhip.so")
+APPS = os.path.join(HERE, "apps")
+BIN = os.path.join(HERE, "bin")
+CXX_INC = os.path.join(HERE, "cxx")
+C_INC = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("GW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -46,6 +50,23 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_apps() -> list[str]:
+    """Host apps on the GraphWalker API (apps/*.cpp) -> bin/, linked to lib/libgwhip.so."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for f in sorted(os.listdir(APPS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, f[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-fopenmp", "-Wall", "-I", CXX_INC, "-I", C_INC, os.path.join(APPS, f),
+               "-L", LIB_DIR, "-lgwhip", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"app build failed: {' '.join(cmd)}\n{r.stderr}")
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
@@ -58,8 +79,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    apps = build_apps()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {len(apps)} apps")
     return LIB
 
 

@@ -72,6 +72,9 @@ void free_walks(gw_ctx* ctx) {
     ctx->d_work = nullptr;
     gw_free(ctx, ctx->d_hash, ctx->walk_cap * 8);
     gw_free(ctx, ctx->d_visits, ctx->walk_cap * 4);
+    gw_free(ctx, ctx->d_events, ctx->walk_cap * ctx->events_cap * 8);
+    ctx->d_events = nullptr;
+    ctx->events_cap = 0;
     ctx->d_walks = ctx->d_walks_alt = ctx->d_final = nullptr;
     ctx->d_hash = nullptr;
     ctx->d_visits = nullptr;
@@ -391,7 +394,10 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");
     if (prog->kind == GW_JUMP && prog->nvertices == 0) return gw_fail(ctx, GW_EINVAL, "gw_run: JUMP needs nvertices");
     if (prog->kind == GW_JUMP && prog->nvertices > ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_run: nvertices above graph size");
-    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY)) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
+    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS))
+        return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
+    if ((prog->recorder & GW_REC_EVENTS) && prog->event_cap == 0)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_EVENTS needs event_cap > 0");
     // Restart programs jump to the source field, which must then name a vertex.
     if (ctx->nwalks && (prog->kind == GW_RESTART || prog->kind == GW_RESTART_JOINT) && ctx->walk_src_max >= ctx->n)
         return gw_fail(ctx, GW_EINVAL, "gw_run: restart program with a source field that is not a vertex");
@@ -418,6 +424,16 @@ int gw_summary_read(gw_ctx* ctx, uint64_t* hash, uint32_t* visits, uint64_t n) {
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (hash) GW_HIP(ctx, hipMemcpy(hash, ctx->d_hash, n * 8, hipMemcpyDeviceToHost));
     if (visits) GW_HIP(ctx, hipMemcpy(visits, ctx->d_visits, n * 4, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+int gw_events_read(gw_ctx* ctx, uint64_t* out, uint64_t n_walks, uint32_t cap) {
+    if (!ctx || !out) return gw_fail(ctx, GW_EINVAL, "gw_events_read: bad arguments");
+    if (!ctx->d_events || cap != ctx->events_cap || n_walks > ctx->walk_cap)
+        return gw_fail(ctx, GW_ESTATE, "gw_events_read: no event log of that shape");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n_walks) GW_HIP(ctx, hipMemcpy(out, ctx->d_events, n_walks * cap * 8ull, hipMemcpyDeviceToHost));
     return GW_OK;
 }
 

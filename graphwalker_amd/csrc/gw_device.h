@@ -69,14 +69,26 @@ struct Graph {
 
 enum : int { kMoved = 0, kFinished = 1 };
 
-// Recorder hooks, compile-time selected.
-template <bool kVertexCount, bool kSummary>
+// Per-walk event log: visit k of walk `id` goes to slot id * cap + k as
+// (hop << 32 | vertex); visits beyond `cap` are counted but not stored.
+struct EventLog {
+    uint64_t* __restrict__ ev;
+    uint32_t cap;
+};
+
+// Recorder hooks, compile-time selected: the device replacements for the
+// per-step updateInfo callback (randomwalk.hpp:32).
+template <bool kVertexCount, bool kSummary, bool kEvents = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
     uint32_t steps;
     uint64_t hash;
+    EventLog log;
+    uint32_t id;
     __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
+        if (kEvents && visits < log.cap)
+            log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
         ++visits;
         if (kVertexCount) atomicAdd(&counts[v], 1u);
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;

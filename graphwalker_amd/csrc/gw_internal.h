@@ -41,6 +41,8 @@ struct gw_ctx {
     uint32_t* d_counts = nullptr;   // n
     uint64_t* d_hash = nullptr;     // walk_cap, indexed by walk id
     uint32_t* d_visits = nullptr;   // walk_cap, indexed by walk id
+    uint64_t* d_events = nullptr;   // walk_cap x events_cap event slots
+    uint32_t events_cap = 0;
 
     // device counters: [0] steps [1] visits [2] work cursor [3..] scratch
     unsigned long long* d_counters = nullptr;

@@ -100,16 +100,18 @@ __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& 
     return false;
 }
 
-template <int KIND, bool VC, bool SUM>
+template <int KIND, int REC>
 __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const gw_walk* __restrict__ walks,
                                                         gw_walk* __restrict__ final_walks, uint64_t nw,
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
-                                                        unsigned long long* __restrict__ counters) {
+                                                        unsigned long long* __restrict__ counters, EventLog log) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    Recorder<VC, SUM> rec{counts, 0u, 0u, kHashInit};
+    constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
     if (i < nw) {
         gw_walk w = walks[i];
+        rec.id = w.id;
         advance<KIND>(p, g, w.id, w.source, w.vertex, w.hop, 0u, g.n, rec);
         final_walks[w.id] = w;
         if (SUM) {
@@ -125,20 +127,21 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
     }
 }
 
-template <int KIND, bool VC, bool SUM, bool EXT>
+template <int KIND, int REC, bool EXT>
 __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, const gw_walk* __restrict__ walks,
                                                              gw_walk* __restrict__ final_walks, uint64_t nw,
                                                              uint32_t* __restrict__ counts,
                                                              uint64_t* __restrict__ hash,
                                                              uint32_t* __restrict__ visits,
-                                                             unsigned long long* __restrict__ counters) {
+                                                             unsigned long long* __restrict__ counters, EventLog log) {
     const uint32_t lane = lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
     bool active = false;
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
-    Recorder<VC, SUM> rec{counts, 0u, 0u, kHashInit};
+    constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
 
     for (;;) {
         // Refill idle lanes from the wave's chunk; claim a new chunk when dry.
@@ -162,6 +165,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             if (!active && r < take) {
                 const gw_walk w = walks[chunk_next + r];
                 s.id = w.id;
+                rec.id = w.id;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -216,13 +220,13 @@ __device__ __forceinline__ void block_bounds(const uint32_t* bf, uint32_t nb, ui
 // writes its final record and leaves a dead slot (vertex = kDead, dropped by
 // the next bucketing); a walk that leaves the block is written back in place
 // with its new vertex/hop for the next round.
-template <int KIND, bool VC, bool SUM, bool EXT>
+template <int KIND, int REC, bool EXT>
 __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const uint32_t* __restrict__ block_first,
                                                         uint32_t nb, gw_walk* __restrict__ walks, uint64_t nw,
                                                         gw_walk* __restrict__ final_walks,
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
-                                                        unsigned long long* __restrict__ counters) {
+                                                        unsigned long long* __restrict__ counters, EventLog log) {
     __shared__ uint32_t s_bf[kMaxBlocks + 1];
     for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_bf[i] = block_first[i];
     __syncthreads();
@@ -234,7 +238,8 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     uint32_t lo = 0, hi = 0, visits0 = 0;
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
-    Recorder<VC, SUM> rec{counts, 0u, 0u, kHashInit};
+    constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -258,6 +263,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 slot = chunk_next + r;
                 const gw_walk w = walks[slot];
                 s.id = w.id;
+                rec.id = w.id;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -333,42 +339,49 @@ struct Launch {
     uint32_t tb;
     uint32_t grid_persistent;
     bool ext;
+    EventLog log;
     // blocked schedule: the buckets to step (back to back, block order)
     gw_walk* bucket;
     uint64_t bucket_n;
 };
 
-template <int KIND, bool VC, bool SUM, bool EXT>
+template <int KIND, int REC, bool EXT>
 hipError_t launch4(const Launch& L) {
     gw_ctx* c = L.ctx;
     if (L.sched == GW_SCHED_PLAIN) {
         const uint32_t grid = static_cast<uint32_t>((c->nwalks + L.tb - 1) / L.tb);
-        step_plain_kernel<KIND, VC, SUM><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters);
+        step_plain_kernel<KIND, REC><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log);
     } else if (L.sched == GW_SCHED_PERSISTENT) {
-        step_persistent_kernel<KIND, VC, SUM, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters);
+        step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log);
     } else {
-        step_block_kernel<KIND, VC, SUM, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+        step_block_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_block_first, c->nblocks, L.bucket, L.bucket_n, c->d_final, c->d_counts, c->d_hash,
-            c->d_visits, c->d_counters);
+            c->d_visits, c->d_counters, L.log);
     }
     return hipGetLastError();
 }
 
-template <int KIND, bool VC, bool SUM>
+template <int KIND, int REC>
 hipError_t launch3(const Launch& L) {
-    if (L.ext && L.sched != GW_SCHED_PLAIN) return launch4<KIND, VC, SUM, true>(L);
-    return launch4<KIND, VC, SUM, false>(L);
+    if (L.ext && L.sched != GW_SCHED_PLAIN) return launch4<KIND, REC, true>(L);
+    return launch4<KIND, REC, false>(L);
 }
 
+// Recorder combinations that are instantiated; the event log always comes
+// with the per-walk summary (its visit counts index the log).
 template <int KIND>
 hipError_t launch2(const Launch& L, uint32_t rec) {
-    const bool vc = rec & GW_REC_VERTEX_COUNT, sum = rec & GW_REC_WALK_SUMMARY;
-    if (vc && sum) return launch3<KIND, true, true>(L);
-    if (vc) return launch3<KIND, true, false>(L);
-    if (sum) return launch3<KIND, false, true>(L);
-    return launch3<KIND, false, false>(L);
+    constexpr int VC = GW_REC_VERTEX_COUNT, SU = GW_REC_WALK_SUMMARY, EV = GW_REC_EVENTS;
+    switch (rec) {
+    case 0: return launch3<KIND, 0>(L);
+    case VC: return launch3<KIND, VC>(L);
+    case SU: return launch3<KIND, SU>(L);
+    case VC | SU: return launch3<KIND, VC | SU>(L);
+    case SU | EV: return launch3<KIND, SU | EV>(L);
+    default: return launch3<KIND, VC | SU | EV>(L);
+    }
 }
 
 hipError_t launch(const Launch& L, uint32_t kind, uint32_t rec) {
@@ -408,13 +421,13 @@ bool ensure_ext(gw_ctx* ctx) {
     return true;
 }
 
-int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, gw_stats& st) {
+int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw_stats& st) {
     // Working copy of the walks: the loaded records stay untouched so a run
     // can be repeated.
     gw_walk* a = ctx->d_walks_alt;
     gw_walk* b = ctx->d_work;
     GW_HIP(ctx, hipMemcpyAsync(a, ctx->d_walks, ctx->nwalks * sizeof(gw_walk), hipMemcpyDeviceToDevice, ctx->stream));
-    if (prog->recorder & GW_REC_WALK_SUMMARY) {
+    if (rec & GW_REC_WALK_SUMMARY) {
         summary_init_kernel<<<dim3(static_cast<uint32_t>((ctx->nwalks + 255) / 256)), dim3(256), 0, ctx->stream>>>(
             ctx->d_walks, ctx->nwalks, ctx->d_hash, ctx->d_visits);
         GW_HIP(ctx, hipGetLastError());
@@ -440,7 +453,7 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, gw_stats& st) {
         GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
         L.bucket = a;
         L.bucket_n = live;
-        hipError_t e = launch(L, prog->kind, prog->recorder);
+        hipError_t e = launch(L, prog->kind, rec);
         if (e != hipSuccess) return gw_hip_check(ctx, e, "block step launch");
         ++st.launches;
         GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -474,7 +487,17 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
             return gw_fail(ctx, GW_ENOMEM, "gw_run: successor-header layout does not fit (HBM budget or degree >= 2^32)");
     }
     st.layout = ext ? GW_LAYOUT_EXT : GW_LAYOUT_CSR;
-    if ((prog->recorder & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
+    uint32_t rec = prog->recorder;
+    if (rec & GW_REC_EVENTS) rec |= GW_REC_WALK_SUMMARY;
+    if ((rec & GW_REC_EVENTS) && ctx->events_cap != prog->event_cap) {
+        gw_free(ctx, ctx->d_events, ctx->walk_cap * ctx->events_cap * 8);
+        ctx->d_events = nullptr;
+        ctx->events_cap = 0;
+        int rc = gw_alloc(ctx, (void**)&ctx->d_events, std::max<uint64_t>(1, ctx->walk_cap) * prog->event_cap * 8, "event log");
+        if (rc) return rc;
+        ctx->events_cap = prog->event_cap;
+    }
+    if ((rec & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
         int rc;
         if ((rc = gw_alloc(ctx, (void**)&ctx->d_hash, ctx->walk_cap * 8, "walk hash"))) return rc;
         if ((rc = gw_alloc(ctx, (void**)&ctx->d_visits, ctx->walk_cap * 4, "walk visits"))) return rc;
@@ -490,6 +513,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     L.p = Prog{prog->L, prog->nvertices, prog->stop_thr, static_cast<uint32_t>(prog->seed),
                static_cast<uint32_t>(prog->seed >> 32), prog->stream};
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+    L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -497,11 +521,11 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     L.grid_persistent = static_cast<uint32_t>(std::max<uint64_t>(1, waves * 64 / L.tb));
 
     GW_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
-    if (prog->recorder & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
+    if (rec & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
     float ms = 0.f;
     if (sched == GW_SCHED_BLOCKED) {
         GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        int rc = ctx->nwalks ? run_blocked(ctx, L, prog, st) : GW_OK;
+        int rc = ctx->nwalks ? run_blocked(ctx, L, prog, rec, st) : GW_OK;
         if (rc) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
         GW_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
@@ -510,7 +534,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     } else {
         GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
         if (ctx->nwalks) {
-            hipError_t e = launch(L, prog->kind, prog->recorder);
+            hipError_t e = launch(L, prog->kind, rec);
             if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
             st.launches = 1;
         }

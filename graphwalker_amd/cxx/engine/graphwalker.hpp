@@ -1,0 +1,234 @@
+// engine/graphwalker.hpp -- graphwalker_engine: the block loop the apps drive
+// (src/engine/graphwalker.hpp:26-311), running on one MI355X through the C ABI
+// of libgwhip (include/gwhip.h).
+//
+//   graphwalker_engine engine(file, blocksize_kb, nblocks, nmblocks, m);
+//   engine.run(program, prob);
+//
+// The constructor loads the preprocessed CSR and block table and makes them
+// resident in HBM (the whole graph: the device block cache is the HBM).
+// run() calls the program's startWalks, gathers the per-(thread, block)
+// buckets into device walk records, runs the program's step semantics on the
+// GPU with a per-walk event log, and replays every visit to the program's
+// updateInfo on the host in walk order -- the same calls the reference makes
+// per step, so app recorders see the same (source, vertex, hop) visits.
+//
+// Options (get_option_*): execthreads, schedule (persistent|blocked|plain),
+// seed (Philox key), device.
+#ifndef DEF_GRAPHCHWALKER_ENGINE
+#define DEF_GRAPHCHWALKER_ENGINE
+
+#include <assert.h>
+#include <fcntl.h>
+#include <omp.h>
+#include <string.h>
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "api/cmdopts.hpp"
+#include "api/datatype.hpp"
+#include "api/filename.hpp"
+#include "api/io.hpp"
+#include "gwhip.h"
+#include "logger/logger.hpp"
+#include "metrics/metrics.hpp"
+#include "walks/randomwalk.hpp"
+
+class graphwalker_engine {
+public:
+    std::string base_filename;
+    unsigned long long blocksize_kb;
+    bid_t nblocks;
+    vid_t nvertices;
+    tid_t exec_threads;
+    vid_t* blocks;
+    timeval start;
+    bid_t nmblocks;
+    bid_t exec_block;
+    metrics& m;
+    WalkManager* walk_manager;
+
+    graphwalker_engine(std::string _base_filename, unsigned long long _blocksize_kb, bid_t _nblocks, bid_t _nmblocks,
+                       metrics& _m)
+        : base_filename(_base_filename), blocksize_kb(_blocksize_kb), nblocks(_nblocks), nvertices(0), blocks(nullptr),
+          nmblocks(_nmblocks), exec_block(0), m(_m), walk_manager(nullptr), ctx_(nullptr) {
+        exec_threads = static_cast<tid_t>(get_option_int("execthreads", omp_get_max_threads()));
+        omp_set_num_threads(exec_threads);
+        load_block_range();
+        load_graph();
+        walk_manager = new WalkManager(m, nblocks, exec_threads, base_filename);
+        m.set("file", base_filename);
+        m.set("engine", std::string("gwhip"));
+        m.set("nblocks", static_cast<size_t>(nblocks));
+        logstream(LOG_INFO) << "Engine configuration: exec_threads = " << static_cast<int>(exec_threads)
+                            << ", blocksize_kb = " << blocksize_kb << ", nblocks = " << nblocks
+                            << ", nvertices = " << nvertices << std::endl;
+    }
+
+    virtual ~graphwalker_engine() {
+        delete walk_manager;
+        free(blocks);
+        if (ctx_) gw_destroy(ctx_);
+    }
+
+    graphwalker_engine(const graphwalker_engine&) = delete;
+    graphwalker_engine& operator=(const graphwalker_engine&) = delete;
+
+    virtual size_t num_vertices() { return blocks[nblocks]; }
+
+    gw_ctx* context() { return ctx_; }
+
+    // `prob` chose between the min-step and max-walks block heuristics
+    // (walk.hpp:245-252); with the graph resident in HBM every block is
+    // stepped each round, so it only affects the order of work.
+    void run(RandomWalk& userprogram, float prob) {
+        (void)prob;
+        m.start_time("0_startWalks");
+        userprogram.startWalks(*walk_manager, nblocks, blocks, base_filename);
+        m.stop_time("0_startWalks");
+        gettimeofday(&start, nullptr);
+        m.start_time("00_runtime");
+
+        m.start_time("3_getCurrentWalks");
+        std::vector<gw_walk> walks = gather_walks();
+        m.stop_time("3_getCurrentWalks");
+
+        const gw_device_walk dp = userprogram.device_program();
+        gw_program prog;
+        memset(&prog, 0, sizeof prog);
+        prog.kind = dp.kind;
+        prog.L = dp.L;
+        prog.stop_thr = static_cast<uint64_t>(llround(static_cast<double>(dp.p) * 4294967296.0));
+        prog.nvertices = dp.kind == GW_JUMP ? (dp.N ? dp.N : nvertices) : 0;
+        prog.seed = get_option_long("seed", 1);
+        prog.recorder = GW_REC_EVENTS | GW_REC_WALK_SUMMARY;
+        const std::string sched = get_option_string("schedule", "persistent");
+        prog.schedule = sched == "plain" ? GW_SCHED_PLAIN : sched == "blocked" ? GW_SCHED_BLOCKED : GW_SCHED_PERSISTENT;
+        if (prog.schedule == GW_SCHED_BLOCKED && !blocks_on_device_) prog.schedule = GW_SCHED_PERSISTENT;
+        prog.event_cap = dp.kind == GW_PROB ? 64u : std::max<uint32_t>(1u, dp.L);
+
+        m.start_time("2_findSubGraph");
+        check(gw_walks_load(ctx_, walks.data(), walks.size()), "gw_walks_load");
+        m.stop_time("2_findSubGraph");
+
+        std::vector<uint64_t> events;
+        std::vector<uint32_t> visits(walks.size());
+        gw_stats st;
+        for (;;) {
+            m.start_time("5_exec_updates");
+            check(gw_run(ctx_, &prog, &st), "gw_run");
+            m.stop_time("5_exec_updates");
+            check(gw_summary_read(ctx_, nullptr, visits.data(), visits.size()), "gw_summary_read");
+            const uint32_t vmax = visits.empty() ? 0 : *std::max_element(visits.begin(), visits.end());
+            if (vmax <= prog.event_cap) break;
+            prog.event_cap = vmax;  // deterministic: the rerun takes the same walks
+        }
+        events.resize(walks.size() * static_cast<size_t>(prog.event_cap));
+        if (!walks.empty()) check(gw_events_read(ctx_, events.data(), walks.size(), prog.event_cap), "gw_events_read");
+        m.set("walk_steps", static_cast<size_t>(st.steps));
+        m.set("device_ms", st.total_ms);
+
+        // Replay every visit to updateInfo, walk by walk, hop by hop.
+        m.start_time("z_replay_updateInfo");
+        for (size_t i = 0; i < walks.size(); ++i) {
+            const gw_walk& w = walks[i];
+            const uint64_t* ev = events.data() + static_cast<size_t>(w.id) * prog.event_cap;
+            for (uint32_t k = 0; k < visits[w.id]; ++k)
+                userprogram.updateInfo(w.source, static_cast<vid_t>(ev[k] & 0xffffffffu), 0,
+                                       static_cast<hid_t>(ev[k] >> 32));
+        }
+        m.stop_time("z_replay_updateInfo");
+
+        walk_manager->walksum = 0;
+        for (bid_t b = 0; b < nblocks; ++b) {
+            walk_manager->walknum[b] = 0;
+            walk_manager->minstep[b] = 0xffff;
+        }
+        m.stop_time("00_runtime");
+    }
+
+private:
+    gw_ctx* ctx_;
+    bool blocks_on_device_ = false;
+
+    void check(int rc, const char* what) {
+        if (rc != GW_OK) logstream(LOG_FATAL) << what << " failed (" << rc << "): " << gw_last_error(ctx_) << std::endl;
+    }
+
+    void load_block_range() {
+        const std::string f = blockrangename(base_filename, blocksize_kb);
+        std::ifstream in(f.c_str());
+        if (!in.good()) logstream(LOG_FATAL) << "Could not load block range file: " << f << std::endl;
+        blocks = static_cast<vid_t*>(malloc((nblocks + 1) * sizeof(vid_t)));
+        for (bid_t i = 0; i <= nblocks; ++i) {
+            if (!(in >> blocks[i])) logstream(LOG_FATAL) << "block range file too short: " << f << std::endl;
+        }
+    }
+
+    void load_graph() {
+        const std::string base = fidname(base_filename, 0);
+        std::vector<eid_t> off;
+        std::vector<vid_t> col;
+        read_file(base + ".beg_pos", off);
+        read_file(base + ".csr", col);
+        if (off.size() < 2) logstream(LOG_FATAL) << "empty beg_pos file " << base << ".beg_pos" << std::endl;
+        nvertices = static_cast<vid_t>(off.size() - 1);
+        gw_cfg cfg;
+        memset(&cfg, 0, sizeof cfg);
+        cfg.device = get_option_int("device", 0);
+        if (gw_create(&ctx_, &cfg) != GW_OK) logstream(LOG_FATAL) << "no usable HIP device" << std::endl;
+        m.start_time("g_loadSubGraph");
+        check(gw_graph_set(ctx_, off.data(), col.data(), nvertices, col.size()), "gw_graph_set");
+        m.stop_time("g_loadSubGraph");
+        // The block table of the reference cut rule may hold empty blocks;
+        // the device block loop takes non-empty ones only.
+        std::vector<uint32_t> bf;
+        for (bid_t b = 0; b <= nblocks; ++b)
+            if (bf.empty() || blocks[b] > bf.back()) bf.push_back(blocks[b]);
+        if (bf.back() == nvertices && bf.size() >= 2 && bf.size() - 1 <= 2048)
+            blocks_on_device_ = gw_graph_blocks(ctx_, bf.data(), static_cast<uint32_t>(bf.size() - 1)) == GW_OK;
+    }
+
+    template <class T>
+    static void read_file(const std::string& f, std::vector<T>& out) {
+        int fd = open(f.c_str(), O_RDONLY);
+        if (fd < 0) logstream(LOG_FATAL) << "Could not load :" << f << ", error: " << strerror(errno) << std::endl;
+        const off_t bytes = lseek(fd, 0, SEEK_END);
+        out.resize(static_cast<size_t>(bytes) / sizeof(T));
+        if (!out.empty()) preada(fd, out.data(), out.size() * sizeof(T), 0);
+        close(fd);
+    }
+
+    // Buckets -> device records.  Walk ids follow the order of the packed
+    // records sorted by value (then block), so they do not depend on which
+    // thread placed which walk.
+    std::vector<gw_walk> gather_walks() {
+        struct rec { WalkDataType w; bid_t p; };
+        std::vector<rec> all;
+        for (bid_t p = 0; p < nblocks; ++p)
+            for (tid_t t = 0; t < exec_threads; ++t) {
+                WalkBuffer& b = walk_manager->pwalks[t][p];
+                for (wid_t k = 0; k < b.size_w; ++k) all.push_back({b[k], p});
+                b.size_w = 0;
+            }
+        std::stable_sort(all.begin(), all.end(), [](const rec& a, const rec& b) {
+            return a.w != b.w ? a.w < b.w : a.p < b.p;
+        });
+        std::vector<gw_walk> out(all.size());
+        for (size_t i = 0; i < all.size(); ++i) {
+            const WalkDataType w = all[i].w;
+            out[i].id = static_cast<uint32_t>(i);
+            out[i].source = walk_manager->getSourceId(w);
+            out[i].vertex = blocks[all[i].p] + walk_manager->getCurrentId(w);
+            out[i].hop = walk_manager->getHop(w);
+        }
+        return out;
+    }
+};
+
+#endif

@@ -31,6 +31,8 @@
  *                              moveWalk + getCurrentWalks (walk.hpp:93-137)
  *   gw_counts_read             the per-vertex recorder of updateInfo
  *                              (rwdomination.cpp:286-289 vertex_value[dstId]++)
+ *   gw_events_read             the per-step updateInfo callback (randomwalk.hpp:32):
+ *                              each walk's visits, replayed on the host in walk order
  *   gw_walks_read / gw_summary_read   final walk records and per-walk visit
  *                              summaries (no reference counterpart; test access)
  */
@@ -64,6 +66,8 @@ typedef struct gw_ctx gw_ctx;
 #define GW_REC_NONE 0u
 #define GW_REC_VERTEX_COUNT 1u /* u32 visits per vertex (rwdomination.cpp:288) */
 #define GW_REC_WALK_SUMMARY 2u /* per-walk visit count + trajectory hash */
+#define GW_REC_EVENTS 4u       /* per-walk event log of (vertex, hop) visits, replayed to updateInfo
+                                  on the host (randomwalk.hpp:32); implies GW_REC_WALK_SUMMARY */
 
 /* schedules */
 #define GW_SCHED_AUTO 0u
@@ -99,7 +103,7 @@ typedef struct gw_program {
     uint32_t block_threads;   /* 0 = default workgroup size */
     uint32_t waves_per_simd;  /* persistent schedule occupancy target, 0 = default */
     uint32_t layout;          /* GW_LAYOUT_*: graph layout the step kernel reads */
-    uint32_t reserved;
+    uint32_t event_cap;       /* GW_REC_EVENTS: visits stored per walk (more are counted, not stored) */
 } gw_program;
 
 /* graph layouts the step kernels read (the CSR is always resident) */
@@ -156,6 +160,10 @@ int gw_bucket(gw_ctx *ctx, uint64_t *bucket_counts);
 
 int gw_counts_read(gw_ctx *ctx, uint32_t *counts, uint32_t n);
 int gw_summary_read(gw_ctx *ctx, uint64_t *hash, uint32_t *visits, uint64_t n);
+/* Event log of the last run with GW_REC_EVENTS: walk id w's visit k is
+ * out[w * cap + k] = (hop << 32) | vertex for k < min(visits[w], cap).
+ * `n_walks` and `cap` must equal the loaded walk count and the run's event_cap. */
+int gw_events_read(gw_ctx *ctx, uint64_t *out, uint64_t n_walks, uint32_t cap);
 
 /* Random-gather ceiling: `gathers` independent random 4-byte loads from a
  * buffer of `bytes` bytes; returns the kernel time in ms. */

@@ -1,0 +1,95 @@
+"""GPU runs through the C++ front end: an app on the GraphWalker API and
+reference apps compiled unchanged against it (oracle/_ref/compat/, built by
+oracle/build_ref.sh) -- results checked against the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gwfiles import write_graph
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "graphwalker_amd", "bin")
+COMPAT = os.path.join(ROOT, "oracle", "_ref", "compat")
+KINDS = {"jump": O.JUMP, "stop": O.STOP, "prob": O.PROB, "joint": O.RESTART_JOINT, "restart": O.RESTART}
+
+
+@pytest.fixture(scope="module")
+def graph(tmp_path_factory):
+    d = tmp_path_factory.mktemp("g")
+    (d / "conf").mkdir()
+    (d / "conf" / "graphwalker.cnf").write_text("# empty\n")
+    off, col = O.rmat_graph(12, 1 << 16, seed=7)
+    base = str(d / "rmat12")
+    write_graph(base, off, col, 16)
+    write_graph(base, off, col, 1024)
+    return d, base, off, col
+
+
+def run(cmd, cwd, timeout=120):
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd, timeout=timeout)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-3000:])
+    return r.stdout
+
+
+@pytest.mark.parametrize("prog", sorted(KINDS))
+@pytest.mark.parametrize("schedule", ["persistent", "blocked"])
+def test_walkapp_counts_bit_exact(graph, prog, schedule):
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    R, L, src0 = 5000, 11, 7
+    out = str(d / f"counts_{prog}_{schedule}.u32")
+    stdout = run([os.path.join(BIN, "walkapp"), "file", base, "program", prog, "R", str(R), "L", str(L),
+                  "source", str(src0), "blocksize_kb", "16", "seed", "5", "schedule", schedule, "out", out], d)
+    counts = np.fromfile(out, dtype=np.uint32)
+    # walk ids follow the packed records' sorted order: by start vertex
+    starts = np.sort((src0 + np.arange(R)) % n).astype(np.uint32)
+    walks = O.make_walks(starts, sources_=starts, ids=np.arange(R))
+    ref = O.replay(O.program(KINDS[prog], L, 0.15, n, seed=5), off, col, walks)
+    assert np.array_equal(counts, ref["counts"])
+    assert f"walk_visits {ref['visits']}" in stdout
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "simrank"), os.X_OK), reason="compat apps not built")
+def test_reference_simrank_unchanged(graph):
+    """apps/simrank.cpp of the reference, compiled unchanged against the
+    front end, prints the SimRank the oracle's trajectories give."""
+    d, base, off, col = graph
+    a, b, R, L = 3, 9, 60, 11
+    stdout = run([os.path.join(COMPAT, "simrank"), "file", base, "a", str(a), "b", str(b), "R", str(R), "L", str(L),
+                  "blocksize_kb", "1024"], d)
+    m = re.search(r"SimRank for (\d+) and (\d+) = (\S+)", stdout)
+    assert m, stdout[-2000:]
+    # records encode(a, cur, i*L) then encode(b, ...): sorted by source, then hop
+    prog = O.program(O.RESTART_JOINT, L, 0.15, off.shape[0] - 1, seed=1)
+    paths = {}
+    for side, s in enumerate((a, b)):
+        arr = np.full(R * L, 0xFFFFFFFF, dtype=np.uint64)
+        for i in range(R):
+            w = O.make_walks([s], sources_=[s], hops=[i * L], ids=[side * R + i])[0]
+            tv, th, _ = O.trace(prog, off, col, w)
+            arr[th] = tv
+        paths[s] = arr
+    sim = np.float32(0)
+    A, B = paths[a], paths[b]
+    for i in range(R):
+        for j in range(R):
+            for l in range(L):
+                if A[i * L + l] == B[j * L + l] and A[i * L + l] != 0xFFFFFFFF:
+                    sim = np.float32(np.float64(sim) + (1.0 / (R * R)) * 0.8 ** l)
+                    break
+    assert m.group(3) == f"{float(sim):g}", (m.group(3), float(sim))
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "rawrandomwalks"), os.X_OK), reason="compat apps not built")
+def test_reference_rawrandomwalks_unchanged(graph):
+    d, base, off, col = graph
+    stdout = run([os.path.join(COMPAT, "rawrandomwalks"), "file", base, "N", str(off.shape[0] - 1), "R", "20000",
+                  "L", "10", "blocksize_kb", "16"], d)
+    m = re.search(r"\[walk_steps\] (\d+)", stdout)
+    assert m and int(m.group(1)) == 20000 * 10, stdout[-2000:]

@@ -167,3 +167,29 @@ def test_bucketing_is_a_stable_counting_sort(engine, small_graph, nb):
         ref, rcnt = O.bucket_stable(w, bf)
         assert np.array_equal(cnt, rcnt[:-1])
         assert np.array_equal(engine.walks_current(), ref)
+
+
+@pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("blocked", "ext")])
+@pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART_JOINT, gw.PROB])
+def test_event_log_is_the_trajectory(engine, small_graph, kind, variant):
+    """The event log replays updateInfo exactly: walk w's k-th visit."""
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    if variant[0] == "blocked":
+        engine.graph_blocks([0, 512, 1024, 3000, n])
+    walks = _walks(3000, n, kind, seed=11)
+    engine.walks_load(walks)
+    cap = 12
+    prog = gw.program(kind, 11, 0.15, n, seed=77, recorder=gw.REC_EVENTS, schedule=variant[0], layout=variant[1],
+                      event_cap=cap)
+    engine.run(prog)
+    ev = engine.events(cap)
+    _, vis = engine.summary()
+    op = oprog(prog)
+    for i in range(0, 3000, 7):
+        tv, th, _ = O.trace(op, off, col, walks[i])
+        k = min(len(tv), cap)
+        assert vis[walks[i]["id"]] == len(tv)
+        row = ev[walks[i]["id"], :k]
+        assert np.array_equal(row & 0xFFFFFFFF, tv[:k]) and np.array_equal(row >> 32, th[:k])
