@@ -16,6 +16,7 @@ public:
             std::cout << "[" << kv.first << "] ";
             if (e.valtype == STRING) std::cout << e.stringval;
             else if (e.valtype == TIME) std::cout << "time=" << e.sum << "s count=" << e.count;
+            else if (e.valtype == INTEGER) std::cout << static_cast<unsigned long long>(e.value);
             else std::cout << e.value;
             std::cout << std::endl;
         }
