@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgwhip.so")
 JUMP, STOP, PROB, RESTART_JOINT, RESTART = 0, 1, 2, 3, 4
 KIND_NAMES = {"jump": JUMP, "stop": STOP, "prob": PROB, "restart_joint": RESTART_JOINT, "restart": RESTART}
 # recorders
-REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY, REC_EVENTS = 0, 1, 2, 4
+REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY, REC_EVENTS, REC_SOURCE_COUNT = 0, 1, 2, 4, 8
 # schedules
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
 SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}
@@ -58,6 +58,8 @@ class Program(C.Structure):
         ("waves_per_simd", C.c_uint32),
         ("layout", C.c_uint32),
         ("event_cap", C.c_uint32),
+        ("table_log2", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -88,7 +90,7 @@ def stop_threshold(p: float) -> int:
 
 def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
             schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0,
-            layout=LAYOUT_AUTO, event_cap: int = 0) -> Program:
+            layout=LAYOUT_AUTO, event_cap: int = 0, table_log2: int = 0) -> Program:
     if isinstance(kind, str):
         kind = KIND_NAMES[kind]
     if isinstance(schedule, str):
@@ -96,7 +98,7 @@ def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, re
     if isinstance(layout, str):
         layout = LAYOUT_NAMES[layout]
     return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
-                   waves_per_simd, layout, event_cap)
+                   waves_per_simd, layout, event_cap, table_log2, 0)
 
 
 def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:
@@ -135,6 +137,7 @@ def lib() -> C.CDLL:
         "gw_counts_read": ([P, P, u32], i),
         "gw_summary_read": ([P, P, P, u64], i),
         "gw_events_read": ([P, P, u64, u32], i),
+        "gw_source_counts_read": ([P, P, P, P, u64, P], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
     }
@@ -273,6 +276,17 @@ class Engine:
         ms = C.c_double()
         self._check(self._L.gw_copy_probe(self._ctx, nbytes, C.byref(ms)), "gw_copy_probe")
         return ms.value
+
+    def source_counts(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(source, vertex, count) of the last GW_REC_SOURCE_COUNT run, sorted by source then vertex."""
+        n = C.c_uint64()
+        self._check(self._L.gw_source_counts_read(self._ctx, None, None, None, 0, C.byref(n)), "gw_source_counts_read")
+        s = np.empty(n.value, dtype=np.uint32)
+        v = np.empty(n.value, dtype=np.uint32)
+        c = np.empty(n.value, dtype=np.uint32)
+        self._check(self._L.gw_source_counts_read(self._ctx, _ptr(s), _ptr(v), _ptr(c), n.value, C.byref(n)),
+                    "gw_source_counts_read")
+        return s, v, c
 
     def events(self, cap: int) -> np.ndarray:
         """Event log of the last run: (nwalks, cap) u64, (hop << 32) | vertex."""

@@ -212,6 +212,7 @@ void gw_destroy(gw_ctx* ctx) {
     free_walks(ctx);
     gw_free(ctx, ctx->d_counters, 16 * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
+    gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
     gw_free(ctx, ctx->d_bucket_off, 0);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -394,8 +395,10 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");
     if (prog->kind == GW_JUMP && prog->nvertices == 0) return gw_fail(ctx, GW_EINVAL, "gw_run: JUMP needs nvertices");
     if (prog->kind == GW_JUMP && prog->nvertices > ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_run: nvertices above graph size");
-    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS))
+    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS | GW_REC_SOURCE_COUNT))
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
+    if ((prog->recorder & GW_REC_SOURCE_COUNT) && (prog->recorder & (GW_REC_VERTEX_COUNT | GW_REC_EVENTS)))
+        return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_SOURCE_COUNT combines with GW_REC_WALK_SUMMARY only");
     if ((prog->recorder & GW_REC_EVENTS) && prog->event_cap == 0)
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_EVENTS needs event_cap > 0");
     // Restart programs jump to the source field, which must then name a vertex.

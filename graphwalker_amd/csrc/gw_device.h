@@ -76,9 +76,38 @@ struct EventLog {
     uint32_t cap;
 };
 
+// Device-wide open-addressing hash of (source field, vertex) -> visit count.
+// A slot holds (key << cbits) | count with key = (source << vbits) | vertex;
+// 0 marks an empty slot (every stored entry has count >= 1).  Linear probing
+// from a multiplicative hash; a full table raises `overflow`.
+struct SourceTable {
+    unsigned long long* __restrict__ tab;
+    unsigned long long* __restrict__ overflow;
+    uint64_t mask;
+    uint32_t shift;  // 64 - log2(slots)
+    uint32_t vbits;
+    uint32_t cbits;
+};
+
+__device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source, uint32_t v) {
+    const uint64_t key = (static_cast<uint64_t>(source) << t.vbits) | v;
+    const unsigned long long entry = (static_cast<unsigned long long>(key) << t.cbits) | 1ull;
+    uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> t.shift;
+    for (uint64_t probe = 0; probe <= t.mask; ++probe) {
+        const unsigned long long old = atomicCAS(&t.tab[h], 0ull, entry);
+        if (old == 0ull) return;
+        if ((old >> t.cbits) == key) {
+            atomicAdd(&t.tab[h], 1ull);
+            return;
+        }
+        h = (h + 1) & t.mask;
+    }
+    atomicAdd(t.overflow, 1ull);
+}
+
 // Recorder hooks, compile-time selected: the device replacements for the
 // per-step updateInfo callback (randomwalk.hpp:32).
-template <bool kVertexCount, bool kSummary, bool kEvents = false>
+template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
@@ -86,12 +115,17 @@ struct Recorder {
     uint64_t hash;
     EventLog log;
     uint32_t id;
+    SourceTable table;
+    uint32_t source;
+    __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t)
+        : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0) {}
     __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
         if (kEvents && visits < log.cap)
             log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
         ++visits;
         if (kVertexCount) atomicAdd(&counts[v], 1u);
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
+        if (kSourceCount) table_add(table, source, v);
     }
 };
 

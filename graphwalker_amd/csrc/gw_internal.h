@@ -43,6 +43,9 @@ struct gw_ctx {
     uint32_t* d_visits = nullptr;   // walk_cap, indexed by walk id
     uint64_t* d_events = nullptr;   // walk_cap x events_cap event slots
     uint32_t events_cap = 0;
+    unsigned long long* d_table = nullptr;  // (source, vertex) -> count hash slots
+    uint64_t table_slots = 0;
+    uint32_t table_log2 = 0, table_vbits = 0, table_cbits = 0;
 
     // device counters: [0] steps [1] visits [2] work cursor [3..] scratch
     unsigned long long* d_counters = nullptr;

@@ -105,13 +105,16 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
                                                         gw_walk* __restrict__ final_walks, uint64_t nw,
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
-                                                        unsigned long long* __restrict__ counters, EventLog log) {
+                                                        unsigned long long* __restrict__ counters, EventLog log,
+                                                        SourceTable table) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
+        counts, log, table);
     if (i < nw) {
         gw_walk w = walks[i];
         rec.id = w.id;
+        rec.source = w.source;
         advance<KIND>(p, g, w.id, w.source, w.vertex, w.hop, 0u, g.n, rec);
         final_walks[w.id] = w;
         if (SUM) {
@@ -133,7 +136,8 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                                                              uint32_t* __restrict__ counts,
                                                              uint64_t* __restrict__ hash,
                                                              uint32_t* __restrict__ visits,
-                                                             unsigned long long* __restrict__ counters, EventLog log) {
+                                                             unsigned long long* __restrict__ counters, EventLog log,
+                                                        SourceTable table) {
     const uint32_t lane = lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
@@ -141,7 +145,8 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
+        counts, log, table);
 
     for (;;) {
         // Refill idle lanes from the wave's chunk; claim a new chunk when dry.
@@ -166,6 +171,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 const gw_walk w = walks[chunk_next + r];
                 s.id = w.id;
                 rec.id = w.id;
+                rec.source = w.source;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -226,7 +232,8 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                                                         gw_walk* __restrict__ final_walks,
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
-                                                        unsigned long long* __restrict__ counters, EventLog log) {
+                                                        unsigned long long* __restrict__ counters, EventLog log,
+                                                        SourceTable table) {
     __shared__ uint32_t s_bf[kMaxBlocks + 1];
     for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_bf[i] = block_first[i];
     __syncthreads();
@@ -239,7 +246,8 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0> rec{counts, 0u, 0u, kHashInit, log, 0u};
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
+        counts, log, table);
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -264,6 +272,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 const gw_walk w = walks[slot];
                 s.id = w.id;
                 rec.id = w.id;
+                rec.source = w.source;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -340,6 +349,7 @@ struct Launch {
     uint32_t grid_persistent;
     bool ext;
     EventLog log;
+    SourceTable table;
     // blocked schedule: the buckets to step (back to back, block order)
     gw_walk* bucket;
     uint64_t bucket_n;
@@ -351,14 +361,14 @@ hipError_t launch4(const Launch& L) {
     if (L.sched == GW_SCHED_PLAIN) {
         const uint32_t grid = static_cast<uint32_t>((c->nwalks + L.tb - 1) / L.tb);
         step_plain_kernel<KIND, REC><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log);
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table);
     } else if (L.sched == GW_SCHED_PERSISTENT) {
         step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log);
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table);
     } else {
         step_block_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_block_first, c->nblocks, L.bucket, L.bucket_n, c->d_final, c->d_counts, c->d_hash,
-            c->d_visits, c->d_counters, L.log);
+            c->d_visits, c->d_counters, L.log, L.table);
     }
     return hipGetLastError();
 }
@@ -373,14 +383,17 @@ hipError_t launch3(const Launch& L) {
 // with the per-walk summary (its visit counts index the log).
 template <int KIND>
 hipError_t launch2(const Launch& L, uint32_t rec) {
-    constexpr int VC = GW_REC_VERTEX_COUNT, SU = GW_REC_WALK_SUMMARY, EV = GW_REC_EVENTS;
+    constexpr int VC = GW_REC_VERTEX_COUNT, SU = GW_REC_WALK_SUMMARY, EV = GW_REC_EVENTS, SC = GW_REC_SOURCE_COUNT;
     switch (rec) {
     case 0: return launch3<KIND, 0>(L);
     case VC: return launch3<KIND, VC>(L);
     case SU: return launch3<KIND, SU>(L);
     case VC | SU: return launch3<KIND, VC | SU>(L);
     case SU | EV: return launch3<KIND, SU | EV>(L);
-    default: return launch3<KIND, VC | SU | EV>(L);
+    case VC | SU | EV: return launch3<KIND, VC | SU | EV>(L);
+    case SC: return launch3<KIND, SC>(L);
+    case SC | SU: return launch3<KIND, SC | SU>(L);
+    default: return hipErrorInvalidValue;
     }
 }
 
@@ -467,6 +480,37 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
     return GW_OK;
 }
 
+// Sizes, clears and records the layout of the (source, vertex) count table.
+int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
+    auto bits = [](uint64_t x) {
+        uint32_t b = 0;
+        while (b < 64 && (x >> b) != 0) ++b;
+        return b;
+    };
+    const uint32_t vbits = std::max(1u, bits(ctx->n ? ctx->n - 1ull : 0ull));
+    const uint32_t sbits = bits(ctx->walk_src_max);
+    if (vbits + sbits > 44)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: source field and vertex ids need more than 44 key bits");
+    uint32_t lg = prog->table_log2;
+    if (lg == 0) lg = std::max(12u, std::min(31u, bits(std::max<uint64_t>(1, ctx->nwalks) * 4 - 1)));
+    if (lg < 4 || lg > 34) return gw_fail(ctx, GW_EINVAL, "gw_run: table_log2 out of range [4, 34]");
+    const uint64_t slots = 1ull << lg;
+    if (slots != ctx->table_slots) {
+        gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
+        ctx->d_table = nullptr;
+        ctx->table_slots = 0;
+        int rc = gw_alloc(ctx, (void**)&ctx->d_table, slots * 8, "source-count table");
+        if (rc) return rc;
+        ctx->table_slots = slots;
+    }
+    ctx->table_log2 = lg;
+    ctx->table_vbits = vbits;
+    ctx->table_cbits = 64 - vbits - sbits;
+    GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, slots * 8, ctx->stream));
+    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 10, 0, 8, ctx->stream));
+    return GW_OK;
+}
+
 }  // namespace
 }  // namespace gw
 
@@ -497,6 +541,10 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         if (rc) return rc;
         ctx->events_cap = prog->event_cap;
     }
+    if (rec & GW_REC_SOURCE_COUNT) {
+        int rc = prepare_source_table(ctx, prog);
+        if (rc) return rc;
+    }
     if ((rec & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
         int rc;
         if ((rc = gw_alloc(ctx, (void**)&ctx->d_hash, ctx->walk_cap * 8, "walk hash"))) return rc;
@@ -514,6 +562,8 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
                static_cast<uint32_t>(prog->seed >> 32), prog->stream};
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
+    L.table = SourceTable{ctx->d_table, ctx->d_counters + 10, ctx->table_slots - 1, 64u - ctx->table_log2,
+                          ctx->table_vbits, ctx->table_cbits};
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -545,10 +595,13 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         st.step_ms = ms;
         st.total_ms = ms;
     }
-    unsigned long long cnt[4];
+    unsigned long long cnt[11];
     GW_HIP(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     st.steps = cnt[0];
     st.visits = cnt[1];
+    if ((rec & GW_REC_SOURCE_COUNT) && cnt[10])
+        return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[10]) +
+                                           " visits not recorded); raise table_log2");
     if (stats) *stats = st;
     return GW_OK;
 }

@@ -31,6 +31,8 @@
  *                              moveWalk + getCurrentWalks (walk.hpp:93-137)
  *   gw_counts_read             the per-vertex recorder of updateInfo
  *                              (rwdomination.cpp:286-289 vertex_value[dstId]++)
+ *   gw_source_counts_read      msppr's per-source DiscreteDistribution recorder
+ *                              (msppr.cpp:71-76, discretedistribution.hpp:44-57), exact
  *   gw_events_read             the per-step updateInfo callback (randomwalk.hpp:32):
  *                              each walk's visits, replayed on the host in walk order
  *   gw_walks_read / gw_summary_read   final walk records and per-walk visit
@@ -68,6 +70,8 @@ typedef struct gw_ctx gw_ctx;
 #define GW_REC_WALK_SUMMARY 2u /* per-walk visit count + trajectory hash */
 #define GW_REC_EVENTS 4u       /* per-walk event log of (vertex, hop) visits, replayed to updateInfo
                                   on the host (randomwalk.hpp:32); implies GW_REC_WALK_SUMMARY */
+#define GW_REC_SOURCE_COUNT 8u /* sparse u32 visit counts per (source field, vertex): the per-source
+                                  visit distributions of msppr (msppr.cpp:71-76), exact */
 
 /* schedules */
 #define GW_SCHED_AUTO 0u
@@ -104,6 +108,8 @@ typedef struct gw_program {
     uint32_t waves_per_simd;  /* persistent schedule occupancy target, 0 = default */
     uint32_t layout;          /* GW_LAYOUT_*: graph layout the step kernel reads */
     uint32_t event_cap;       /* GW_REC_EVENTS: visits stored per walk (more are counted, not stored) */
+    uint32_t table_log2;      /* GW_REC_SOURCE_COUNT: log2 of hash-table slots, 0 = sized from the walks */
+    uint32_t reserved;
 } gw_program;
 
 /* graph layouts the step kernels read (the CSR is always resident) */
@@ -160,6 +166,11 @@ int gw_bucket(gw_ctx *ctx, uint64_t *bucket_counts);
 
 int gw_counts_read(gw_ctx *ctx, uint32_t *counts, uint32_t n);
 int gw_summary_read(gw_ctx *ctx, uint64_t *hash, uint32_t *visits, uint64_t n);
+/* Per-(source, vertex) counts of the last run with GW_REC_SOURCE_COUNT, sorted
+ * by source then vertex.  With null arrays only *n_out (the number of pairs)
+ * is returned; otherwise `cap` must be at least that number. */
+int gw_source_counts_read(gw_ctx *ctx, uint32_t *source, uint32_t *vertex, uint32_t *count, uint64_t cap,
+                          uint64_t *n_out);
 /* Event log of the last run with GW_REC_EVENTS: walk id w's visit k is
  * out[w * cap + k] = (hop << 32) | vertex for k < min(visits[w], cap).
  * `n_walks` and `cap` must equal the loaded walk count and the run's event_cap. */

@@ -193,3 +193,49 @@ def test_event_log_is_the_trajectory(engine, small_graph, kind, variant):
         assert vis[walks[i]["id"]] == len(tv)
         row = ev[walks[i]["id"], :k]
         assert np.array_equal(row & 0xFFFFFFFF, tv[:k]) and np.array_equal(row >> 32, th[:k])
+
+
+def _expected_source_counts(prog, off, col, sources, field, wps):
+    out_s, out_v, out_c = [], [], []
+    for k, (s, f) in enumerate(zip(sources, field)):
+        w = O.make_walks(np.full(wps, s, np.uint32), sources_=np.full(wps, f, np.uint32),
+                         ids=k * wps + np.arange(wps))
+        c = O.replay(oprog(prog), off, col, w)["counts"]
+        nz = np.nonzero(c)[0]
+        out_s.append(np.full(nz.shape[0], f, np.uint32))
+        out_v.append(nz.astype(np.uint32))
+        out_c.append(c[nz])
+    s, v, c = np.concatenate(out_s), np.concatenate(out_v), np.concatenate(out_c)
+    order = np.lexsort((v, s))
+    return s[order], v[order], c[order]
+
+
+@pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("persistent", "csr"), ("blocked", "ext")])
+@pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART])
+def test_source_counts_bit_exact(engine, small_graph, kind, variant):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    if variant[0] == "blocked":
+        engine.graph_blocks([0, 64, 1000, 2048, n])
+    sources = O.sources(3, n, 50)
+    # STOP: the source field is the source index (msppr.cpp:59); RESTART: the vertex
+    field = np.arange(50, dtype=np.uint32) if kind == gw.STOP else sources
+    wps = 300
+    engine.walks_init_sources(sources, wps, source_field=field)
+    prog = gw.program(kind, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT | gw.REC_WALK_SUMMARY,
+                      schedule=variant[0], layout=variant[1])
+    st = engine.run(prog)
+    s, v, c = engine.source_counts()
+    es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
+    assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
+    assert int(c.sum()) == st["visits"]
+
+
+def test_source_table_overflow_is_an_error(engine, small_graph):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.walks_init_sources(O.sources(3, n, 20), 200, source_field=np.arange(20, dtype=np.uint32))
+    with pytest.raises(gw.GwError, match="table full"):
+        engine.run(gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT, table_log2=6))
