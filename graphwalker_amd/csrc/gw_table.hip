@@ -1,6 +1,6 @@
 // gw_table.hip -- read-out of the (source, vertex) -> count hash table that
 // GW_REC_SOURCE_COUNT fills during a run: non-empty slots are compacted,
-// radix-sorted by key (rocPRIM through hipCUB, off the walk path) and
+// radix-sorted (rocPRIM through hipCUB, off the walk path) and
 // unpacked into (source, vertex, count) triples sorted by source, vertex.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -80,7 +80,10 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
     }
     if (n == 0) { release(); return GW_OK; }
     if ((rc = gw_alloc(ctx, (void**)&b, slots * 8, "table sort"))) { release(); return rc; }
-    const int begin_bit = static_cast<int>(ctx->table_cbits);
+    // Sort the whole 64-bit entry: keys are unique per slot, so this orders by
+    // key.  (rocPRIM's onesweep sort of 64-bit keys with begin_bit > 0 returns
+    // a non-permutation on gfx950 with ROCm 7.2: measured, so it is avoided.)
+    const int begin_bit = 0;
     e = hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, a, b, n, begin_bit, 64, ctx->stream);
     if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "table sort temp"))) { release(); return rc; }
     if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, a, b, n, begin_bit, 64, ctx->stream);
