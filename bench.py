@@ -1,13 +1,19 @@
 """Walk-steps/s of the walk-update hot path on one MI355X.
 
-Default workload: raw random walks on R-MAT scale 24 (16,777,216 vertices,
-edge factor 16 = 268,435,456 edges, a/b/c/d = .57/.19/.19/.05, seed 2), one
-walk per vertex, fixed length 10, no restart (dangling vertices teleport):
-167,772,160 walk-steps per pass.  A bench "step" is one pass of the engine over
-all walks (gw_run), graph and walk records resident in HBM before timing.
+Default workload (`rawwalks-rmat24`): raw random walks on R-MAT scale 24
+(16,777,216 vertices, edge factor 16 = 268,435,456 edges, a/b/c/d =
+.57/.19/.19/.05, seed 2), one walk per vertex, fixed length 10, no restart
+(dangling vertices teleport): 167,772,160 walk-steps per pass.
 
-Prints one JSON line (rank 0).  The CPU baseline is the oracle's OpenMP
-replay of the same walk programs on a bounded sample of the same walks.
+Secondary workload (`ppr4096-rmat26`, reported under "workloads"): batched
+multi-source PPR on R-MAT scale 26 (67,108,864 vertices, 1,073,741,824 edges,
+seed 3), 4096 uniform sources x 2000 walks, stop probability 0.15, cap 20,
+exact sparse per-(source, vertex) u32 counts.
+
+A bench "step" is one pass of the engine over all walks (gw_run), graph and
+walk records resident in HBM before timing.  Prints one JSON line.  The CPU
+baseline is the oracle's OpenMP replay of the same walk program on a bounded
+sample of the same walks, on the host's cores.
 """
 from __future__ import annotations
 
@@ -28,23 +34,65 @@ PEAK_HBM_GBPS = 8000.0
 SECTOR = 64  # bytes per random access at HBM granularity
 
 WORKLOADS = {
-    # name: (scale, edge_factor, seed, kind, L, p, recorder, block_vertices)
     "rawwalks-rmat24": dict(scale=24, ef=16, seed=2, kind=gw.JUMP, L=10, p=0.0, recorder=gw.REC_NONE,
-                            block_vertices=1 << 20,
+                            block_vertices=1 << 20, sources=0, wps=0,
                             desc="R-MAT scale 24 ef 16 seed 2, 1 walk/vertex, L=10, jump p=0"),
     "rawwalks-rmat20": dict(scale=20, ef=16, seed=2, kind=gw.JUMP, L=10, p=0.0, recorder=gw.REC_NONE,
-                            block_vertices=1 << 16,
+                            block_vertices=1 << 16, sources=0, wps=0,
                             desc="R-MAT scale 20 ef 16 seed 2, 1 walk/vertex, L=10, jump p=0 (reduced)"),
+    "ppr4096-rmat26": dict(scale=26, ef=16, seed=3, kind=gw.STOP, L=20, p=0.15, recorder=gw.REC_SOURCE_COUNT,
+                           block_vertices=1 << 22, sources=4096, wps=2000,
+                           desc="R-MAT scale 26 ef 16 seed 3, 4096 sources x 2000 walks, stop 0.15, cap 20, "
+                                "sparse per-source counts"),
+    "ppr256-rmat22": dict(scale=22, ef=16, seed=3, kind=gw.STOP, L=20, p=0.15, recorder=gw.REC_SOURCE_COUNT,
+                          block_vertices=1 << 20, sources=256, wps=2000,
+                          desc="R-MAT scale 22 ef 16 seed 3, 256 sources x 2000 walks (reduced)"),
 }
 
 
-def bytes_per_step(recorder: int, layout: int) -> int:
-    """Algorithmic random-access bytes per walk-step, at 64-B sector
-    granularity: the CSR layout reads one sector for the offset pair and one
-    for the column; the successor-header layout reads one 16-B edge record
-    (one sector).  A per-visit count adds one sector (its atomic)."""
-    reads = 2 if layout == gw.LAYOUT_CSR else 1
-    return SECTOR * (reads + (1 if recorder & gw.REC_VERTEX_COUNT else 0))
+def random_sectors_per_step(layout: int) -> int:
+    """Random reads per walk-step at 64-B sector granularity: the CSR layout
+    reads one sector for the offset pair and one for the column; the
+    successor-header layout reads one 16-B edge record."""
+    return 2 if layout == gw.LAYOUT_CSR else 1
+
+
+def algorithmic_bytes(steps: int, visits: int, recorder: int, layout: int) -> int:
+    """Random-access bytes one pass needs: the step reads, plus one sector
+    read-modify-write per visit for a count recorder (vertex atomics or the
+    source-count hash)."""
+    rec = 1 if recorder & (gw.REC_VERTEX_COUNT | gw.REC_SOURCE_COUNT) else 0
+    return SECTOR * (steps * random_sectors_per_step(layout) + visits * rec)
+
+
+def setup(eng: gw.Engine, wl: dict) -> int:
+    n = 1 << wl["scale"]
+    t0 = time.perf_counter()
+    eng.graph_rmat(wl["scale"], n * wl["ef"], wl["seed"])
+    if wl["sources"]:
+        src = gw.sources(wl["seed"], n, wl["sources"])
+        eng.walks_init_sources(src, wl["wps"], source_field=np.arange(wl["sources"], dtype=np.uint32))
+    else:
+        eng.walks_init_per_vertex(1)
+    eng.graph_blocks(np.arange(0, n + 1, wl["block_vertices"], dtype=np.uint32))
+    print(f"[bench] graph + walks ready in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    return n
+
+
+def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int):
+    prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
+                      layout=layout)
+    for _ in range(W):
+        eng.run(prog)
+    t = time.perf_counter()
+    kms, steps, visits, st = [], 0, 0, None
+    for _ in range(K):
+        st = eng.run(prog)
+        kms.append(st["step_ms"])
+        steps += st["steps"]
+        visits += st["visits"]
+    wall = time.perf_counter() - t
+    return prog, wall, steps, visits, kms, st
 
 
 def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) -> dict:
@@ -54,21 +102,54 @@ def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) ->
     op = O.Program(prog.kind, prog.L, prog.stop_thr, prog.nvertices, prog.stream, prog.seed)
     n = off.shape[0] - 1
     threads = O.num_threads()
+    if wl["sources"]:
+        src = gw.sources(wl["seed"], n, wl["sources"])
+        starts = np.repeat(src, wl["wps"])
+        total = starts.shape[0]
+    else:
+        starts = None
+        total = n
     nw = 1 << 18
-    elapsed = 0.0
-    steps = 0
     while True:
-        walks = O.make_walks(np.arange(nw, dtype=np.uint32) % n)
+        v = starts[:nw] if starts is not None else np.arange(nw, dtype=np.uint32) % n
+        walks = O.make_walks(v)
         t0 = time.perf_counter()
-        r = O.replay(op, off, col, walks, counts=bool(prog.recorder & gw.REC_VERTEX_COUNT))
+        r = O.replay(op, off, col, walks, counts=True)
         elapsed = time.perf_counter() - t0
-        steps = r["steps"]
-        if elapsed >= target_s / 4 or nw >= n:
+        if elapsed >= target_s / 4 or nw >= total:
             break
-        nw = min(n, int(nw * max(2.0, target_s / max(elapsed, 1e-3))))
-    return {"value": steps / elapsed, "unit": "walk-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle OpenMP replay of walks 0..{nw - 1} (one per vertex) of the same program on the "
-                      f"same device-generated graph, {steps} steps in {elapsed:.2f} s"}
+        nw = min(total, int(nw * max(2.0, target_s / max(elapsed, 1e-3))))
+    return {"value": r["steps"] / elapsed, "unit": "walk-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle OpenMP replay (per-vertex counts) of the first {nw} walks of the same program on the "
+                      f"same device-generated graph: {r['steps']} steps in {elapsed:.2f} s"}
+
+
+def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool) -> dict:
+    wl = WORKLOADS[name]
+    n = setup(eng, wl)
+    variants = {}
+    if compare:
+        for s_, l_ in (("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "ext")):
+            if (s_, l_) == (sched, layout):
+                continue
+            _, w_, st_, vi_, k_, last = timed(eng, wl, n, s_, l_, max(2, K // 3), 1)
+            variants[f"{s_}-{l_}"] = {"walk_steps_per_s": st_ / w_, "kernel_ms": float(np.mean(k_)),
+                                      "rounds": last["rounds"], "bucket_ms": last["bucket_ms"]}
+            print(f"[bench] {name} {s_}-{l_}: {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+    prog, wall, steps, visits, kms, st = timed(eng, wl, n, sched, layout, K, W)
+    per_pass_steps, per_pass_visits = steps // K, visits // K
+    kavg = float(np.mean(kms))
+    abytes = algorithmic_bytes(per_pass_steps, per_pass_visits, wl["recorder"], st["layout"])
+    achieved = abytes / (kavg * 1e-3) / 1e9
+    print(f"[bench] {name} {sched}-{layout}: {steps / wall / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+    res = {
+        "value": steps / wall, "wall_s": wall, "steps": per_pass_steps, "visits": per_pass_visits, "prog": prog,
+        "kernel_ms": kavg, "algorithmic_bytes": abytes, "achieved": achieved, "layout": st["layout"], "n": n,
+        "variants": variants, "wl": wl,
+    }
+    if "plain-csr" in variants:
+        res["speedup_vs_plain"] = res["value"] / variants["plain-csr"]["walk_steps_per_s"]
+    return res
 
 
 def main() -> int:
@@ -77,6 +158,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rawwalks-rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="ppr4096-rmat26", help="extra workload reported under 'workloads' ('' = none)")
     ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent", "blocked"])
     ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -85,53 +167,16 @@ def main() -> int:
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
-    wl = WORKLOADS[args.workload]
     eng = gw.Engine(0)
     info = eng.device_info()
     print(f"[bench] {info}", file=sys.stderr, flush=True)
-    n = 1 << wl["scale"]
-    m = n * wl["ef"]
-    t0 = time.perf_counter()
-    eng.graph_rmat(wl["scale"], m, wl["seed"])
-    eng.walks_init_per_vertex(1)
-    print(f"[bench] graph + walks ready in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
 
-    bf = np.arange(0, n + 1, wl["block_vertices"], dtype=np.uint32)
-    eng.graph_blocks(bf)
-
-    def make(sched, layout):
-        return gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
-                          layout=layout)
-
-    def timed(sched, layout, K, W):
-        prog = make(sched, layout)
-        for _ in range(W):
-            eng.run(prog)
-        t = time.perf_counter()
-        kms, steps, st = [], 0, None
-        for _ in range(K):
-            st = eng.run(prog)
-            kms.append(st["step_ms"])
-            steps += st["steps"]
-        wall = time.perf_counter() - t
-        return prog, wall, steps, kms, st
-
-    # the other schedules/layouts of the same workload, for comparison
-    variants = {}
-    for sched, layout in (("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "ext")):
-        if (sched, layout) == (args.schedule, args.layout):
-            continue
-        _, w_, s_, k_, st_ = timed(sched, layout, max(2, args.steps // 3), 1)
-        variants[f"{sched}-{layout}"] = {"walk_steps_per_s": s_ / w_, "kernel_ms": float(np.mean(k_)),
-                                         "rounds": st_["rounds"], "bucket_ms": st_["bucket_ms"]}
-        print(f"[bench] {sched}-{layout}: {s_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
-    prog, wall, steps, kms, st = timed(args.schedule, args.layout, args.steps, args.warmup)
-    steps_per_pass = steps // args.steps
-    value = steps / wall
-    kavg = float(np.mean(kms))
-    bstep = bytes_per_step(wl["recorder"], st["layout"])
-    achieved = steps_per_pass * bstep / (kavg * 1e-3) / 1e9
-    print(f"[bench] {args.schedule}-{args.layout}: {value / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+    m = measure(eng, args.workload, args.steps, args.warmup, args.schedule, args.layout, compare=True)
+    wl = m["wl"]
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(eng, wl, m["prog"], args.cpu_seconds)
 
     # random-gather ceiling: independent 4-byte gathers over 4 GiB
     gathers = 1 << 30
@@ -147,35 +192,48 @@ def main() -> int:
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
+    workloads = {}
+    if args.secondary and args.secondary != args.workload:
+        s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False)
+        _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1)
+        workloads[args.secondary] = {
+            "desc": s["wl"]["desc"], "walk_steps_per_s": s["value"], "visits_per_s": s["visits"] * (s["value"] / s["steps"]),
+            "walk_steps_per_pass": s["steps"], "visits_per_pass": s["visits"], "kernel_ms": s["kernel_ms"],
+            "roofline_achieved_GBps": s["achieved"], "roofline_frac": s["achieved"] / PEAK_HBM_GBPS,
+            "plain_csr_walk_steps_per_s": st_ / w_, "speedup_vs_plain": s["value"] / (st_ / w_),
+            "schedule": f"{args.schedule}-{args.layout}",
+        }
+
     out = {
         "metric": "walk-steps/s",
-        "value": value,
+        "value": m["value"],
         "unit": "walk-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": m["wall_s"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (R-MAT generated on the device)",
-        "config": {"workload": args.workload, "desc": wl["desc"], "walks": n, "walk_steps_per_pass": steps_per_pass,
+        "config": {"workload": args.workload, "desc": wl["desc"], "walk_steps_per_pass": m["steps"],
                    "schedule": args.schedule, "layout": args.layout, "device": info["name"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
-                     "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": kavg,
-                     "bytes_per_step": bstep},
+        "roofline": {"bound": "hbm", "achieved": m["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": m["achieved"] / PEAK_HBM_GBPS, "traffic": traffic,
+                     "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": m["kernel_ms"],
+                     "algorithmic_bytes_per_launch": m["algorithmic_bytes"],
+                     "bytes_per_step": m["algorithmic_bytes"] / max(1, m["steps"])},
         "random_access_ceiling": {"gathers_per_s": gathers / (gms * 1e-3), "GBps_at_64B": rand_gbps,
-                                  "frac_of_ceiling": achieved / rand_gbps},
+                                  "frac_of_ceiling": m["achieved"] / rand_gbps},
         "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
-        "variants": variants,
+        "variants": m["variants"],
+        "workloads": workloads,
     }
-    if "plain-csr" in variants:
-        out["speedup_vs_plain"] = value / variants["plain-csr"]["walk_steps_per_s"]
-    if not args.no_cpu:
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(eng, wl, prog, args.cpu_seconds)
+    if "speedup_vs_plain" in m:
+        out["speedup_vs_plain"] = m["speedup_vs_plain"]
+    if cpu:
+        out["cpu_baseline"] = cpu
     eng.close()
     print(json.dumps(out), flush=True)
     return 0

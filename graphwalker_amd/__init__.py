@@ -109,6 +109,14 @@ def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:
 _lib = None
 
 
+def sources(seed: int, n: int, count: int) -> np.ndarray:
+    """Uniform source vertices reproducible from (seed, index) (gw_sources)."""
+    out = np.empty(count, dtype=np.uint32)
+    if lib().gw_sources(seed, n, count, _ptr(out)) != 0:
+        raise GwError("gw_sources: bad arguments")
+    return out
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is not None:
@@ -140,6 +148,7 @@ def lib() -> C.CDLL:
         "gw_source_counts_read": ([P, P, P, P, u64, P], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
+        "gw_sources": ([u64, u32, u32, P], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -430,6 +430,16 @@ int gw_summary_read(gw_ctx* ctx, uint64_t* hash, uint32_t* visits, uint64_t n) {
     return GW_OK;
 }
 
+int gw_sources(uint64_t seed, uint32_t n, uint32_t count, uint32_t* out) {
+    if (!out || n == 0) return GW_EINVAL;
+    for (uint32_t k = 0; k < count; ++k) {
+        const gw::u32x4 r = gw::philox4x32_10(k, 0, gw::kTagSrcs, 0, static_cast<uint32_t>(seed),
+                                              static_cast<uint32_t>(seed >> 32));
+        out[k] = gw::umulhi32(r.x, n);
+    }
+    return GW_OK;
+}
+
 int gw_events_read(gw_ctx* ctx, uint64_t* out, uint64_t n_walks, uint32_t cap) {
     if (!ctx || !out) return gw_fail(ctx, GW_EINVAL, "gw_events_read: bad arguments");
     if (!ctx->d_events || cap != ctx->events_cap || n_walks > ctx->walk_cap)

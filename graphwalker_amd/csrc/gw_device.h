@@ -17,19 +17,24 @@ namespace gw {
 
 constexpr uint32_t kTagWalk = 0x57414C4Bu;  // 'WALK'
 constexpr uint32_t kTagRmat = 0x524D4154u;  // 'RMAT'
+constexpr uint32_t kTagSrcs = 0x53524353u;  // 'SRCS'
 constexpr uint64_t kHashInit = 0xCBF29CE484222325ull;
 constexpr uint64_t kHashPrime = 0x100000001B3ull;
 
 struct u32x4 { uint32_t x, y, z, w; };
 
-__device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                                uint32_t k0, uint32_t k1) {
+__host__ __device__ __forceinline__ uint32_t umulhi32(uint32_t a, uint32_t b) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(a) * b) >> 32);
+}
+
+__host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                         uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t hi0 = umulhi32(0xD2511F53u, c0);
         const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t hi1 = umulhi32(0xCD9E8D57u, c2);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;

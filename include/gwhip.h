@@ -157,6 +157,11 @@ int gw_walks_read(gw_ctx *ctx, gw_walk *out, uint64_t n);
 /* The loaded records in their current slot order (after gw_bucket: by block). */
 int gw_walks_current(gw_ctx *ctx, gw_walk *out, uint64_t n);
 
+/* Multi-source workload sources: out[k] = mulhi32(r0, n) with
+ * r = Philox4x32-10(counter {k, 0, 'SRCS', 0}, key = seed) -- uniform vertices,
+ * reproducible from (seed, k).  No context needed. */
+int gw_sources(uint64_t seed, uint32_t n, uint32_t count, uint32_t *out);
+
 /* Run every loaded walk to completion. */
 int gw_run(gw_ctx *ctx, const gw_program *prog, gw_stats *stats);
 

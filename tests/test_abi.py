@@ -91,3 +91,10 @@ def test_null_context_is_rejected():
     assert L.gw_run(None, None, None) != 0
     assert L.gw_graph_set(None, None, None, 0, 0) != 0
     assert L.gw_last_error(None) == b"null context"
+
+
+def test_sources_match_oracle():
+    import oracle as O
+
+    for seed, n, k in ((3, 1 << 26, 4096), (1, 1000, 77), (2**40 + 5, 7, 10)):
+        assert np.array_equal(gw.sources(seed, n, k), O.sources(seed, n, k))

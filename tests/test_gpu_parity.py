@@ -28,7 +28,7 @@ def small_graph(engine):
     return off, col
 
 
-@pytest.mark.parametrize("scale,m,seed", [(4, 40, 3), (10, 9000, 5), (16, 1 << 20, 1)])
+@pytest.mark.parametrize("scale,m,seed", [(4, 40, 3), (10, 9000, 5), (16, 1 << 20, 1), (20, 1 << 24, 2)])
 def test_rmat_device_graph_equals_oracle(engine, scale, m, seed):
     engine.graph_rmat(scale, m, seed)
     off, col = engine.graph_read()
@@ -93,6 +93,22 @@ def test_config1_restart_ppr(engine):
     l1 = fp64.l1_normalised(c, exp)
     assert l1 < fp64.mc_l1_bound(exp, W, L, L)
     print(f"[config1] L1 = {l1:.4f}, bound {fp64.mc_l1_bound(exp, W, L, L):.4f}")
+
+
+def test_config1_l1_below_001_with_enough_walks(engine):
+    """The Monte-Carlo estimate converges to the FP64 finite-horizon vector:
+    on the config-1 graph, 10 M restart walks bring the normalised L1 below
+    0.01 (100 k walks give ~0.09, inside their own MC bound)."""
+    engine.graph_rmat(16, 1 << 20, 1)
+    off, col = engine.graph_read()
+    W, L = 10_000_000, 10
+    engine.walks_init_sources([0], W)
+    st = engine.run(gw.program(gw.RESTART, L, 0.15, 1 << 16, seed=11, recorder=gw.REC_VERTEX_COUNT))
+    assert st["steps"] == W * L
+    exp = fp64.expected_visits(O.RESTART, off, col, 0, W, L, 0.15)
+    l1 = fp64.l1_normalised(engine.counts(), exp)
+    print(f"[config1-10M] L1 = {l1:.5f}")
+    assert l1 < 0.01
 
 
 @pytest.mark.parametrize("tb,wps", [(64, 1), (128, 3), (256, 8), (192, 5)])
