@@ -79,9 +79,9 @@ def setup(eng: gw.Engine, wl: dict) -> int:
     return n
 
 
-def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int):
+def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int, source_method=gw.SRC_SORT):
     prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
-                      layout=layout)
+                      layout=layout, source_method=source_method)
     for _ in range(W):
         eng.run(prog)
     t = time.perf_counter()
@@ -196,11 +196,13 @@ def main() -> int:
     if args.secondary and args.secondary != args.workload:
         s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False)
         _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1)
+        _, wh_, sth_, _, _, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_HASH)
         workloads[args.secondary] = {
             "desc": s["wl"]["desc"], "walk_steps_per_s": s["value"], "visits_per_s": s["visits"] * (s["value"] / s["steps"]),
             "walk_steps_per_pass": s["steps"], "visits_per_pass": s["visits"], "kernel_ms": s["kernel_ms"],
             "roofline_achieved_GBps": s["achieved"], "roofline_frac": s["achieved"] / PEAK_HBM_GBPS,
             "plain_csr_walk_steps_per_s": st_ / w_, "speedup_vs_plain": s["value"] / (st_ / w_),
+            "plain_csr_hash_walk_steps_per_s": sth_ / wh_, "speedup_vs_plain_hash": s["value"] / (sth_ / wh_),
             "schedule": f"{args.schedule}-{args.layout}",
         }
 

@@ -30,6 +30,8 @@ SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PER
 # graph layouts the step kernels read
 LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_EXT = 0, 1, 2
 LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
+# per-source count aggregation
+SRC_SORT, SRC_HASH = 0, 1
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
 
@@ -59,7 +61,7 @@ class Program(C.Structure):
         ("layout", C.c_uint32),
         ("event_cap", C.c_uint32),
         ("table_log2", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("source_method", C.c_uint32),
     ]
 
 
@@ -90,7 +92,7 @@ def stop_threshold(p: float) -> int:
 
 def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
             schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0,
-            layout=LAYOUT_AUTO, event_cap: int = 0, table_log2: int = 0) -> Program:
+            layout=LAYOUT_AUTO, event_cap: int = 0, table_log2: int = 0, source_method: int = SRC_SORT) -> Program:
     if isinstance(kind, str):
         kind = KIND_NAMES[kind]
     if isinstance(schedule, str):
@@ -98,7 +100,7 @@ def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, re
     if isinstance(layout, str):
         layout = LAYOUT_NAMES[layout]
     return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
-                   waves_per_simd, layout, event_cap, table_log2, 0)
+                   waves_per_simd, layout, event_cap, table_log2, source_method)
 
 
 def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:

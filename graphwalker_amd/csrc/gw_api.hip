@@ -213,6 +213,11 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_counters, 16 * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
     gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
+    gw_free(ctx, ctx->d_keys, ctx->keys_cap * 8);
+    gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
+    gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);
+    gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
+    gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
     gw_free(ctx, ctx->d_bucket_off, 0);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -399,6 +404,7 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
     if ((prog->recorder & GW_REC_SOURCE_COUNT) && (prog->recorder & (GW_REC_VERTEX_COUNT | GW_REC_EVENTS)))
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_SOURCE_COUNT combines with GW_REC_WALK_SUMMARY only");
+    if (prog->source_method > GW_SRC_HASH) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown source_method");
     if ((prog->recorder & GW_REC_EVENTS) && prog->event_cap == 0)
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_EVENTS needs event_cap > 0");
     // Restart programs jump to the source field, which must then name a vertex.

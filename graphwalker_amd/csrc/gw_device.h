@@ -110,9 +110,57 @@ __device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source,
     atomicAdd(t.overflow, 1ull);
 }
 
+// Append-only log of (source, vertex) keys for sort-reduce aggregation.  Each
+// wavefront owns a slab of `slab` consecutive slots (claimed with one global
+// atomic) and fills it with the keys of its visiting lanes, ranked by ballot;
+// keys that do not fit spill into the next slab.  At kernel exit the open
+// slab's tail is padded with `sentinel`, which sorts after every key.
+struct KeyLog {
+    unsigned long long* __restrict__ keys;
+    unsigned long long* __restrict__ cursor;
+    unsigned long long* __restrict__ overflow;
+    uint64_t cap;
+    uint64_t sentinel;
+    uint32_t vbits;
+    uint32_t slab;
+};
+
+// Called by the lanes that visit; `slab` = this wavefront's {next, end} in LDS.
+__device__ __forceinline__ void keylog_add(const KeyLog& k, uint64_t* slab, uint32_t source, uint32_t v) {
+    const uint64_t key = (static_cast<uint64_t>(source) << k.vbits) | v;
+    const uint64_t mask = __ballot(1);
+    const uint32_t cnt = __popcll(mask);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    const uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    const uint64_t next = slab[0], end = slab[1];
+    const uint32_t room = end - next < cnt ? static_cast<uint32_t>(end - next) : cnt;
+    if (room == cnt) {
+        k.keys[next + rank] = key;
+        if (rank == 0) slab[0] = next + cnt;
+        return;
+    }
+    unsigned long long b = 0;
+    if (rank == 0) b = atomicAdd(k.cursor, static_cast<unsigned long long>(k.slab));
+    b = __shfl(b, leader, 64);
+    const bool ok = b + k.slab <= k.cap;
+    if (rank < room) k.keys[next + rank] = key;
+    else if (ok) k.keys[b + (rank - room)] = key;
+    if (rank == 0) {
+        if (!ok) atomicAdd(k.overflow, 1ull);
+        slab[0] = ok ? b + (cnt - room) : 0;
+        slab[1] = ok ? b + k.slab : 0;
+    }
+}
+
+// At kernel exit (all lanes of the wavefront present): pad the open slab.
+__device__ __forceinline__ void keylog_close(const KeyLog& k, uint64_t* slab, uint32_t lane) {
+    for (uint64_t i = slab[0] + lane; i < slab[1]; i += 64) k.keys[i] = k.sentinel;
+}
+
 // Recorder hooks, compile-time selected: the device replacements for the
 // per-step updateInfo callback (randomwalk.hpp:32).
-template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false>
+template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false, bool kKeyLog = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
@@ -122,8 +170,11 @@ struct Recorder {
     uint32_t id;
     SourceTable table;
     uint32_t source;
-    __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t)
-        : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0) {}
+    KeyLog klog;
+    uint64_t* slab;
+    __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t, const KeyLog& kl,
+                                        uint64_t* sl)
+        : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl) {}
     __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
         if (kEvents && visits < log.cap)
             log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
@@ -131,6 +182,7 @@ struct Recorder {
         if (kVertexCount) atomicAdd(&counts[v], 1u);
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
         if (kSourceCount) table_add(table, source, v);
+        if (kKeyLog) keylog_add(klog, slab, source, v);
     }
 };
 

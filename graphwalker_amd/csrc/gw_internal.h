@@ -46,6 +46,16 @@ struct gw_ctx {
     unsigned long long* d_table = nullptr;  // (source, vertex) -> count hash slots
     uint64_t table_slots = 0;
     uint32_t table_log2 = 0, table_vbits = 0, table_cbits = 0;
+    uint32_t src_method = 1;                 // GW_SRC_* of the last source-count run
+    unsigned long long* d_keys = nullptr;    // sort-reduce key log
+    uint64_t keys_cap = 0;
+    uint64_t keys_sentinel = 0;
+    unsigned long long* d_keys_alt = nullptr;  // sort output
+    void* d_sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    unsigned long long* d_runs = nullptr;    // sorted unique keys of the last sort-reduce
+    uint32_t* d_run_counts = nullptr;
+    uint64_t runs_cap = 0, n_runs = 0;
 
     // device counters: [0] steps [1] visits [2] work cursor [3..] scratch
     unsigned long long* d_counters = nullptr;
@@ -75,6 +85,9 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);
+
+// gw_table.hip: sort + run-length encode the key log (n keys); ms = device time
+int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms);
 
 // gw_bucket.hip: stable counting sort of `in` (n records) by block into `out`;
 // bucket_off[b] = first index of bucket b (nblocks + 1 entries written, host).

@@ -25,6 +25,7 @@ namespace gw {
 namespace {
 
 constexpr uint32_t kChunk = 512;           // walk indices a wavefront claims per atomic
+constexpr int kRecKeyLog = 1 << 16;        // internal recorder bit: source counts by sort-reduce
 constexpr uint32_t kDead = 0xFFFFFFFFu;    // vertex of a finished walk in the blocked schedule
 
 __device__ __forceinline__ uint32_t lane_id() {
@@ -106,11 +107,17 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
                                                         unsigned long long* __restrict__ counters, EventLog log,
-                                                        SourceTable table) {
+                                                        SourceTable table, KeyLog klog) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
-        counts, log, table);
+    constexpr bool KL = (REC & kRecKeyLog) != 0;
+    __shared__ uint64_t s_slab[4][2];
+    if (KL && (threadIdx.x & 63u) == 0) {
+        s_slab[threadIdx.x >> 6][0] = 0;
+        s_slab[threadIdx.x >> 6][1] = 0;
+    }
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
     if (i < nw) {
         gw_walk w = walks[i];
         rec.id = w.id;
@@ -122,6 +129,7 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
             visits[w.id] = rec.visits;
         }
     }
+    if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], threadIdx.x & 63u);
     const uint32_t st = wave_sum(rec.steps);
     const uint32_t vi = wave_sum(rec.visits);
     if ((threadIdx.x & 63u) == 0) {
@@ -137,7 +145,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                                                              uint64_t* __restrict__ hash,
                                                              uint32_t* __restrict__ visits,
                                                              unsigned long long* __restrict__ counters, EventLog log,
-                                                        SourceTable table) {
+                                                        SourceTable table, KeyLog klog) {
     const uint32_t lane = lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
@@ -145,8 +153,14 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
-        counts, log, table);
+    constexpr bool KL = (REC & kRecKeyLog) != 0;
+    __shared__ uint64_t s_slab[4][2];
+    if (KL && (threadIdx.x & 63u) == 0) {
+        s_slab[threadIdx.x >> 6][0] = 0;
+        s_slab[threadIdx.x >> 6][1] = 0;
+    }
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
 
     for (;;) {
         // Refill idle lanes from the wave's chunk; claim a new chunk when dry.
@@ -198,6 +212,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             }
         }
     }
+    if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
     const uint32_t st = wave_sum(tot_steps);
     const uint32_t vi = wave_sum(tot_visits);
     if (lane == 0) {
@@ -233,7 +248,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
                                                         unsigned long long* __restrict__ counters, EventLog log,
-                                                        SourceTable table) {
+                                                        SourceTable table, KeyLog klog) {
     __shared__ uint32_t s_bf[kMaxBlocks + 1];
     for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_bf[i] = block_first[i];
     __syncthreads();
@@ -246,8 +261,14 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     State s{0, 0, 0, 0, 0, 0};
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0> rec(
-        counts, log, table);
+    constexpr bool KL = (REC & kRecKeyLog) != 0;
+    __shared__ uint64_t s_slab[4][2];
+    if (KL && (threadIdx.x & 63u) == 0) {
+        s_slab[threadIdx.x >> 6][0] = 0;
+        s_slab[threadIdx.x >> 6][1] = 0;
+    }
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -312,6 +333,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
             }
         }
     }
+    if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
     const uint32_t st = wave_sum(tot_steps);
     const uint32_t vi = wave_sum(tot_visits);
     if (lane == 0) {
@@ -350,6 +372,7 @@ struct Launch {
     bool ext;
     EventLog log;
     SourceTable table;
+    KeyLog klog;
     // blocked schedule: the buckets to step (back to back, block order)
     gw_walk* bucket;
     uint64_t bucket_n;
@@ -361,14 +384,14 @@ hipError_t launch4(const Launch& L) {
     if (L.sched == GW_SCHED_PLAIN) {
         const uint32_t grid = static_cast<uint32_t>((c->nwalks + L.tb - 1) / L.tb);
         step_plain_kernel<KIND, REC><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table);
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else if (L.sched == GW_SCHED_PERSISTENT) {
         step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table);
+            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else {
         step_block_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_block_first, c->nblocks, L.bucket, L.bucket_n, c->d_final, c->d_counts, c->d_hash,
-            c->d_visits, c->d_counters, L.log, L.table);
+            c->d_visits, c->d_counters, L.log, L.table, L.klog);
     }
     return hipGetLastError();
 }
@@ -393,6 +416,8 @@ hipError_t launch2(const Launch& L, uint32_t rec) {
     case VC | SU | EV: return launch3<KIND, VC | SU | EV>(L);
     case SC: return launch3<KIND, SC>(L);
     case SC | SU: return launch3<KIND, SC | SU>(L);
+    case kRecKeyLog: return launch3<KIND, kRecKeyLog>(L);
+    case kRecKeyLog | SU: return launch3<KIND, kRecKeyLog | SU>(L);
     default: return hipErrorInvalidValue;
     }
 }
@@ -480,6 +505,40 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
     return GW_OK;
 }
 
+// Key log for sort-reduce source counts: capacity for every visit the
+// program can make (capped kinds: L per walk; uncapped: 64 per walk, more is
+// an overflow error) plus slab padding.
+int prepare_keylog(gw_ctx* ctx, const gw_program* prog, uint32_t sched, uint32_t grid_waves, uint32_t* slab) {
+    uint64_t per_walk = prog->L;
+    if (prog->kind == GW_PROB) per_walk = 64;
+    // Slab per wavefront: large for the persistent grid (few, long-lived
+    // wavefronts), small for the plain grid (one short-lived wavefront per 64
+    // walks).  Each launch leaves at most one partly used slab per wavefront.
+    uint64_t launches = 1, waves = grid_waves;
+    if (sched == GW_SCHED_PLAIN) {
+        *slab = 64;
+        waves = (ctx->nwalks + 63) / 64;
+    } else if (sched == GW_SCHED_BLOCKED) {
+        *slab = 256;
+        launches = per_walk + 1;  // every live walk steps at least once per round
+    } else {
+        *slab = 1024;
+    }
+    const uint64_t cap = std::max<uint64_t>(1, ctx->nwalks) * per_walk + launches * waves * *slab + *slab;
+    if (cap > ctx->keys_cap) {
+        gw_free(ctx, ctx->d_keys, ctx->keys_cap * 8);
+        ctx->d_keys = nullptr;
+        ctx->keys_cap = 0;
+        int rc = gw_alloc(ctx, (void**)&ctx->d_keys, cap * 8, "source key log");
+        if (rc) return rc;
+        ctx->keys_cap = cap;
+    }
+    const uint32_t key_bits = 64 - ctx->table_cbits;
+    ctx->keys_sentinel = 1ull << key_bits;
+    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 12, 0, 16, ctx->stream));
+    return GW_OK;
+}
+
 // Sizes, clears and records the layout of the (source, vertex) count table.
 int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     auto bits = [](uint64_t x) {
@@ -494,6 +553,10 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     uint32_t lg = prog->table_log2;
     if (lg == 0) lg = std::max(12u, std::min(31u, bits(std::max<uint64_t>(1, ctx->nwalks) * 4 - 1)));
     if (lg < 4 || lg > 34) return gw_fail(ctx, GW_EINVAL, "gw_run: table_log2 out of range [4, 34]");
+    ctx->table_log2 = lg;
+    ctx->table_vbits = vbits;
+    ctx->table_cbits = 64 - vbits - sbits;
+    if (prog->source_method == GW_SRC_SORT) return GW_OK;  // no hash table
     const uint64_t slots = 1ull << lg;
     if (slots != ctx->table_slots) {
         gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
@@ -541,9 +604,19 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         if (rc) return rc;
         ctx->events_cap = prog->event_cap;
     }
+    ctx->src_method = GW_SRC_HASH;
+    uint32_t keylog_slab = 1024;
     if (rec & GW_REC_SOURCE_COUNT) {
         int rc = prepare_source_table(ctx, prog);
         if (rc) return rc;
+        if (prog->source_method == GW_SRC_SORT) {
+            const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
+            const uint32_t tb = prog->block_threads ? prog->block_threads : 256u;
+            const uint32_t grid_waves = std::max(1u, ctx->cu_count) * 4u * wps / (tb / 64) * (tb / 64);
+            if ((rc = prepare_keylog(ctx, prog, sched, grid_waves, &keylog_slab))) return rc;
+            rec = (rec & ~GW_REC_SOURCE_COUNT) | kRecKeyLog;
+            ctx->src_method = GW_SRC_SORT;
+        }
     }
     if ((rec & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
         int rc;
@@ -564,6 +637,8 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
     L.table = SourceTable{ctx->d_table, ctx->d_counters + 10, ctx->table_slots - 1, 64u - ctx->table_log2,
                           ctx->table_vbits, ctx->table_cbits};
+    L.klog = KeyLog{ctx->d_keys, ctx->d_counters + 12, ctx->d_counters + 13, ctx->keys_cap, ctx->keys_sentinel,
+                    ctx->table_vbits, keylog_slab};
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -595,13 +670,21 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         st.step_ms = ms;
         st.total_ms = ms;
     }
-    unsigned long long cnt[11];
+    unsigned long long cnt[14];
     GW_HIP(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     st.steps = cnt[0];
     st.visits = cnt[1];
     if ((rec & GW_REC_SOURCE_COUNT) && cnt[10])
         return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[10]) +
                                            " visits not recorded); raise table_log2");
+    if (rec & kRecKeyLog) {
+        if (cnt[13]) return gw_fail(ctx, GW_ENOMEM, "gw_run: source key log full (walks longer than the log allows)");
+        double rms = 0;
+        int rc = gw_keylog_reduce(ctx, cnt[12], &rms);
+        if (rc) return rc;
+        st.total_ms += rms;
+        st.bucket_ms += rms;
+    }
     if (stats) *stats = st;
     return GW_OK;
 }

@@ -108,9 +108,13 @@ typedef struct gw_program {
     uint32_t waves_per_simd;  /* persistent schedule occupancy target, 0 = default */
     uint32_t layout;          /* GW_LAYOUT_*: graph layout the step kernel reads */
     uint32_t event_cap;       /* GW_REC_EVENTS: visits stored per walk (more are counted, not stored) */
-    uint32_t table_log2;      /* GW_REC_SOURCE_COUNT: log2 of hash-table slots, 0 = sized from the walks */
-    uint32_t reserved;
+    uint32_t table_log2;      /* GW_REC_SOURCE_COUNT with GW_SRC_HASH: log2 of hash slots, 0 = auto */
+    uint32_t source_method;   /* GW_REC_SOURCE_COUNT: GW_SRC_SORT or GW_SRC_HASH */
 } gw_program;
+
+/* how GW_REC_SOURCE_COUNT aggregates (source, vertex) visits */
+#define GW_SRC_SORT 0u  /* append keys to wave-private slabs, then radix sort + run-length encode */
+#define GW_SRC_HASH 1u  /* device-wide open-addressing hash, one or two atomics per visit */
 
 /* graph layouts the step kernels read (the CSR is always resident) */
 #define GW_LAYOUT_AUTO 0u

@@ -226,9 +226,10 @@ def _expected_source_counts(prog, off, col, sources, field, wps):
     return s[order], v[order], c[order]
 
 
+@pytest.mark.parametrize("method", [gw.SRC_SORT, gw.SRC_HASH])
 @pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("persistent", "csr"), ("blocked", "ext")])
 @pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART])
-def test_source_counts_bit_exact(engine, small_graph, kind, variant):
+def test_source_counts_bit_exact(engine, small_graph, kind, variant, method):
     off, col = small_graph
     n = off.shape[0] - 1
     engine.graph_set(off, col)
@@ -240,7 +241,7 @@ def test_source_counts_bit_exact(engine, small_graph, kind, variant):
     wps = 300
     engine.walks_init_sources(sources, wps, source_field=field)
     prog = gw.program(kind, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT | gw.REC_WALK_SUMMARY,
-                      schedule=variant[0], layout=variant[1])
+                      schedule=variant[0], layout=variant[1], source_method=method)
     st = engine.run(prog)
     s, v, c = engine.source_counts()
     es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
@@ -254,4 +255,5 @@ def test_source_table_overflow_is_an_error(engine, small_graph):
     engine.graph_set(off, col)
     engine.walks_init_sources(O.sources(3, n, 20), 200, source_field=np.arange(20, dtype=np.uint32))
     with pytest.raises(gw.GwError, match="table full"):
-        engine.run(gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT, table_log2=6))
+        engine.run(gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT, table_log2=6,
+                              source_method=gw.SRC_HASH))
