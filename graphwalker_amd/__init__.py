@@ -65,6 +65,26 @@ class Program(C.Structure):
     ]
 
 
+class Query(C.Structure):
+    _fields_ = [("qid", C.c_uint64), ("source", C.c_uint32), ("walks", C.c_uint32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64),
+        ("visits", C.c_uint64),
+        ("live_walks", C.c_uint64),
+        ("admitted", C.c_uint32),
+        ("completed", C.c_uint32),
+        ("active_queries", C.c_uint32),
+        ("queued", C.c_uint32),
+        ("step_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("steps", C.c_uint64),
@@ -151,6 +171,12 @@ def lib() -> C.CDLL:
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
         "gw_sources": ([u64, u32, u32, P], i),
+        "gw_serve_open": ([P, P, u64, u64, u32, u32], i),
+        "gw_serve_submit": ([P, P, u32], i),
+        "gw_serve_step": ([P, P], i),
+        "gw_serve_poll": ([P, P, P], i),
+        "gw_serve_result": ([P, u64, P, P, u64], i),
+        "gw_serve_close": ([P], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -298,6 +324,41 @@ class Engine:
         self._check(self._L.gw_source_counts_read(self._ctx, _ptr(s), _ptr(v), _ptr(c), n.value, C.byref(n)),
                     "gw_source_counts_read")
         return s, v, c
+
+    # --------------------------------------------------------------- serving
+    def serve_open(self, prog: Program, pool_walks: int, arena_bytes: int, max_queries: int = 1024,
+                   hops_per_step: int = 4) -> None:
+        self._check(self._L.gw_serve_open(self._ctx, C.byref(prog), pool_walks, arena_bytes, max_queries,
+                                          hops_per_step), "gw_serve_open")
+
+    def serve_submit(self, queries) -> None:
+        """queries: iterable of (qid, source, walks)."""
+        q = list(queries)
+        arr = (Query * max(1, len(q)))(*[Query(int(a), int(b), int(c)) for a, b, c in q])
+        self._check(self._L.gw_serve_submit(self._ctx, arr, len(q)), "gw_serve_submit")
+
+    def serve_step(self) -> dict:
+        st = ServeStats()
+        self._check(self._L.gw_serve_step(self._ctx, C.byref(st)), "gw_serve_step")
+        return st.as_dict()
+
+    def serve_poll(self):
+        """Next completed query as (qid, npairs), or None."""
+        qid, n = C.c_uint64(), C.c_uint64()
+        rc = self._L.gw_serve_poll(self._ctx, C.byref(qid), C.byref(n))
+        if rc == 1:
+            return None
+        self._check(rc, "gw_serve_poll")
+        return qid.value, n.value
+
+    def serve_result(self, qid: int, npairs: int) -> tuple[np.ndarray, np.ndarray]:
+        v = np.empty(npairs, dtype=np.uint32)
+        c = np.empty(npairs, dtype=np.uint32)
+        self._check(self._L.gw_serve_result(self._ctx, qid, _ptr(v), _ptr(c), npairs), "gw_serve_result")
+        return v, c
+
+    def serve_close(self) -> None:
+        self._check(self._L.gw_serve_close(self._ctx), "gw_serve_close")
 
     def events(self, cap: int) -> np.ndarray:
         """Event log of the last run: (nwalks, cap) u64, (hop << 32) | vertex."""

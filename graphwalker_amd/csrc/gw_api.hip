@@ -206,6 +206,7 @@ int gw_create(gw_ctx** out, const gw_cfg* cfg) {
 
 void gw_destroy(gw_ctx* ctx) {
     if (!ctx) return;
+    gw_serve_close(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_graph(ctx);

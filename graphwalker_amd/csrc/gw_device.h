@@ -230,6 +230,32 @@ __device__ __forceinline__ int advance(const Prog& p, const Graph& g, uint32_t i
     return kMoved;
 }
 
+// One step of a walk on the CSR layout (the body of advance() without the
+// block test).  Returns true when the walk has ended.
+template <int KIND, class Rec>
+__device__ __forceinline__ bool step_once_plain(const Prog& p, const Graph& g, uint32_t id, uint32_t source,
+                                                uint32_t& v, uint32_t& hop, Rec& rec) {
+    constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
+    if (capped && hop >= p.L) return true;
+    rec.visit(v, hop);
+    const u32x4 r = philox4x32_10(id, hop, kTagWalk, p.stream, p.k0, p.k1);
+    const uint64_t b = g.off[v];
+    const uint64_t deg = g.off[v + 1] - b;
+    if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
+        const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
+        v = g.col[b + mulhi64(x, deg)];
+    } else {
+        if (KIND == GW_JUMP) v = mulhi32(r.w, p.nvertices);
+        else if (KIND == GW_STOP || KIND == GW_PROB) return true;
+        else v = source;
+    }
+    ++hop;
+    ++rec.steps;
+    if (KIND == GW_RESTART_JOINT) return hop % p.L == p.L - 1;
+    if (capped) return hop >= p.L;
+    return false;
+}
+
 // Wavefront-wide sum of a 32-bit value (64 lanes), result valid in every lane.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll

@@ -1,0 +1,456 @@
+// gw_serve.hip -- continuous-batching query serving on the walk engine.
+//
+// Queries (source vertex, walk count) arrive at any time.  Admitted queries
+// share one device pool of walk records; every serve step advances each live
+// walk by at most `hops_per_step` hops in one persistent launch, so queries
+// admitted later join walks already in flight, and a query completes when the
+// last of its walks ends.  Per-query visit counts accumulate in a private
+// open-addressing hash region of a device arena; admission is by free walk
+// slots and arena space (the HBM budget), first come first served.  The
+// random stream of a walk is keyed by (seed, query id; walk index, hop), so a
+// query's result does not depend on what else is in flight.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "gw_device.h"
+#include "gw_internal.h"
+
+namespace gw {
+namespace {
+
+struct SlotInfo {
+    uint64_t base;     // region offset in the arena (slots)
+    uint32_t mask;     // region slots - 1
+    uint32_t source;   // source vertex: start and restart target
+    uint32_t qid_lo;   // Philox counter word 3
+    uint32_t qid_hi;   // mixed into Philox key word 1
+};
+
+struct ServeState {
+    gw_program prog{};
+    uint32_t hops = 0;
+    uint64_t pool_cap = 0, arena_slots = 0, live = 0;
+    uint32_t max_queries = 0, cbits = 0;
+    gw_walk* pool[2] = {nullptr, nullptr};
+    int cur = 0;
+    unsigned long long* arena = nullptr;
+    SlotInfo* d_slots = nullptr;
+    uint32_t* d_remaining = nullptr;
+    unsigned long long* d_cnt = nullptr;  // [0] out cursor [1] steps [2] visits [3] overflow [4] compact n
+    unsigned long long* d_pairs = nullptr;  // compaction buffer (largest region)
+    uint64_t pairs_cap = 0;
+    struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; };
+    std::vector<Active> active;
+    std::vector<uint32_t> free_slots;
+    std::vector<std::pair<uint64_t, uint64_t>> free_ranges;  // arena [start, size), sorted by start
+    std::deque<gw_query> queue;
+    std::map<uint64_t, std::vector<unsigned long long>> done;  // qid -> packed (vertex << cbits | count)
+    std::deque<uint64_t> done_order;
+};
+
+std::map<gw_ctx*, ServeState*>& states() {
+    static std::map<gw_ctx*, ServeState*> m;
+    return m;
+}
+
+uint32_t bits_for(uint64_t x) {
+    uint32_t b = 0;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b;
+}
+
+bool arena_alloc(ServeState& S, uint64_t size, uint64_t& base) {
+    for (size_t i = 0; i < S.free_ranges.size(); ++i) {
+        auto& r = S.free_ranges[i];
+        if (r.second >= size) {
+            base = r.first;
+            r.first += size;
+            r.second -= size;
+            if (r.second == 0) S.free_ranges.erase(S.free_ranges.begin() + i);
+            return true;
+        }
+    }
+    return false;
+}
+
+void arena_free(ServeState& S, uint64_t base, uint64_t size) {
+    auto it = std::lower_bound(S.free_ranges.begin(), S.free_ranges.end(), std::make_pair(base, uint64_t(0)));
+    it = S.free_ranges.insert(it, {base, size});
+    // coalesce with neighbours
+    size_t i = it - S.free_ranges.begin();
+    if (i + 1 < S.free_ranges.size() && S.free_ranges[i].first + S.free_ranges[i].second == S.free_ranges[i + 1].first) {
+        S.free_ranges[i].second += S.free_ranges[i + 1].second;
+        S.free_ranges.erase(S.free_ranges.begin() + i + 1);
+    }
+    if (i > 0 && S.free_ranges[i - 1].first + S.free_ranges[i - 1].second == S.free_ranges[i].first) {
+        S.free_ranges[i - 1].second += S.free_ranges[i].second;
+        S.free_ranges.erase(S.free_ranges.begin() + i);
+    }
+}
+
+struct ServeRec {
+    unsigned long long* arena;
+    uint64_t base;
+    uint32_t mask;
+    uint32_t cbits;
+    unsigned long long* overflow;
+    uint32_t visits = 0, steps = 0;
+    __device__ __forceinline__ void visit(uint32_t v, uint32_t) {
+        ++visits;
+        const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | 1ull;
+        uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            unsigned long long* slot = arena + base + h;
+            const unsigned long long old = atomicCAS(slot, 0ull, entry);
+            if (old == 0ull) return;
+            if ((old >> cbits) == v) {
+                atomicAdd(slot, 1ull);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        atomicAdd(overflow, 1ull);
+    }
+};
+
+__device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// One serve step: every walk of pool_in advances at most `hops` hops; walks
+// still alive are appended to pool_out; ended walks decrement their query's
+// remaining count.  Persistent wavefronts take chunks of the pool.
+template <int KIND>
+__global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const gw_walk* __restrict__ pool_in,
+                                                        uint64_t n, gw_walk* __restrict__ pool_out,
+                                                        const SlotInfo* __restrict__ slots,
+                                                        uint32_t* __restrict__ remaining,
+                                                        unsigned long long* __restrict__ arena, uint32_t cbits,
+                                                        uint32_t hops, unsigned long long* __restrict__ cnt,
+                                                        unsigned long long* __restrict__ cursor) {
+    constexpr uint32_t kChunk = 256;
+    const uint32_t lane = __lane_id();
+    uint64_t chunk_next = 0, chunk_end = 0;
+    bool exhausted = false, active = false;
+    uint32_t slot = 0, budget = 0, tot_steps = 0, tot_visits = 0;
+    Prog q = p;
+    ServeRec rec{arena, 0, 0, cbits, cnt + 3};
+    struct { uint32_t id, source, v, hop; } s{0, 0, 0, 0};
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        while (idle && !exhausted) {
+            if (chunk_next == chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(cursor, static_cast<unsigned long long>(kChunk));
+                base = __shfl(base, 0, 64);
+                if (base >= n) { exhausted = true; break; }
+                chunk_next = base;
+                chunk_end = base + kChunk < n ? base + kChunk : n;
+            }
+            const uint32_t want = __popcll(idle);
+            const uint64_t avail = chunk_end - chunk_next;
+            const uint32_t take = avail < want ? static_cast<uint32_t>(avail) : want;
+            const uint32_t r = rank_of(idle);
+            if (!active && r < take) {
+                const gw_walk w = pool_in[chunk_next + r];
+                slot = w.source;
+                const SlotInfo si = slots[slot];
+                s.id = w.id;
+                s.source = si.source;
+                s.v = w.vertex;
+                s.hop = w.hop;
+                q.stream = si.qid_lo;
+                q.k1 = p.k1 ^ si.qid_hi;
+                rec.base = si.base;
+                rec.mask = si.mask;
+                rec.visits = 0;
+                rec.steps = 0;
+                budget = hops;
+                active = true;
+            }
+            chunk_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+        bool leave = false, done = false;
+        if (active) {
+            uint32_t hop_before = s.hop;
+            done = step_once_plain<KIND>(q, g, s.id, s.source, s.v, s.hop, rec);
+            budget -= (s.hop != hop_before) ? 1u : 0u;
+            if (done) {
+                atomicSub(&remaining[slot], 1u);
+            } else if (budget == 0) {
+                leave = true;
+            }
+            if (done || leave) {
+                tot_steps += rec.steps;
+                tot_visits += rec.visits;
+                active = false;
+            }
+        }
+        // survivors go to the next pool: one atomic per wavefront
+        const uint64_t lm = __ballot(leave);
+        if (lm) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(lm)) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[0], static_cast<unsigned long long>(__popcll(lm)));
+            base = __shfl(base, leader, 64);
+            if (leave) pool_out[base + rank_of(lm)] = gw_walk{s.id, slot, s.v, s.hop};
+        }
+    }
+    const uint32_t st = wave_sum(tot_steps), vi = wave_sum(tot_visits);
+    if (lane == 0) {
+        atomicAdd(&cnt[1], static_cast<unsigned long long>(st));
+        atomicAdd(&cnt[2], static_cast<unsigned long long>(vi));
+    }
+}
+
+__global__ void serve_init_kernel(gw_walk* __restrict__ pool, uint64_t at, uint32_t walks, uint32_t slot,
+                                  uint32_t source) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < walks) pool[at + i] = gw_walk{static_cast<uint32_t>(i), slot, source, 0u};
+}
+
+__global__ void serve_compact_kernel(const unsigned long long* __restrict__ region, uint64_t size,
+                                     unsigned long long* __restrict__ out, unsigned long long* __restrict__ n) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const bool full = i < size && region[i] != 0ull;
+    const uint64_t m = __ballot(full);
+    if (m == 0) return;
+    const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (__lane_id() == leader) base = atomicAdd(n, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader, 64);
+    if (full) out[base + rank_of(m)] = region[i];
+}
+
+template <int KIND>
+hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, uint32_t grid) {
+    const Graph g{ctx->d_off, ctx->d_col, nullptr, ctx->n};
+    serve_step_kernel<KIND><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+        p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.arena, S.cbits, S.hops, S.d_cnt,
+        S.d_cnt + 5);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace gw
+
+using namespace gw;
+
+extern "C" {
+
+int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint64_t arena_bytes,
+                  uint32_t max_queries, uint32_t hops_per_step) {
+    if (!ctx || !prog || pool_walks == 0 || max_queries == 0 || hops_per_step == 0 || arena_bytes < 4096)
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad arguments");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: no graph");
+    if (prog->kind != GW_RESTART && prog->kind != GW_STOP && prog->kind != GW_JUMP)
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_open: serving runs RESTART, STOP or JUMP programs");
+    if (prog->L == 0 || prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad program");
+    if (states().count(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: already open");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    ServeState* S = new ServeState();
+    S->prog = *prog;
+    S->prog.nvertices = prog->kind == GW_JUMP ? (prog->nvertices ? prog->nvertices : ctx->n) : prog->nvertices;
+    S->hops = hops_per_step;
+    S->pool_cap = pool_walks;
+    S->arena_slots = arena_bytes / 8;
+    S->max_queries = max_queries;
+    S->cbits = 64 - std::max(1u, bits_for(ctx->n ? ctx->n - 1ull : 0ull));
+    int rc = 0;
+    if ((rc = gw_alloc(ctx, (void**)&S->pool[0], pool_walks * sizeof(gw_walk), "serve pool")) ||
+        (rc = gw_alloc(ctx, (void**)&S->pool[1], pool_walks * sizeof(gw_walk), "serve pool (alt)")) ||
+        (rc = gw_alloc(ctx, (void**)&S->arena, S->arena_slots * 8, "serve count arena")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_slots, max_queries * sizeof(SlotInfo), "serve slots")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_remaining, max_queries * 4ull, "serve remaining")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters"))) {
+        gw_free(ctx, S->pool[0], pool_walks * sizeof(gw_walk));
+        gw_free(ctx, S->pool[1], pool_walks * sizeof(gw_walk));
+        gw_free(ctx, S->arena, S->arena_slots * 8);
+        gw_free(ctx, S->d_slots, max_queries * sizeof(SlotInfo));
+        gw_free(ctx, S->d_remaining, max_queries * 4ull);
+        delete S;
+        return rc;
+    }
+    GW_HIP(ctx, hipMemsetAsync(S->arena, 0, S->arena_slots * 8, ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = max_queries; i-- > 0;) S->free_slots.push_back(i);
+    S->free_ranges.push_back({0, S->arena_slots});
+    states()[ctx] = S;
+    return GW_OK;
+}
+
+int gw_serve_submit(gw_ctx* ctx, const gw_query* q, uint32_t n) {
+    if (!ctx || (n && !q)) return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: bad arguments");
+    auto it = states().find(ctx);
+    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_submit: not open");
+    ServeState& S = *it->second;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (q[i].source >= ctx->n || q[i].walks == 0 || q[i].walks > S.pool_cap)
+            return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) +
+                                               " has a source out of range or a walk count outside [1, pool]");
+    }
+    for (uint32_t i = 0; i < n; ++i) S.queue.push_back(q[i]);
+    return GW_OK;
+}
+
+int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
+    if (!ctx) return GW_EINVAL;
+    auto it = states().find(ctx);
+    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_step: not open");
+    ServeState& S = *it->second;
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_serve_stats st{};
+    // 1. admit queued queries in arrival order while walk slots, a query slot
+    //    and arena space for their count region are free
+    const uint32_t L = S.prog.L;
+    std::vector<SlotInfo> new_slots;
+    std::vector<uint32_t> new_ids;
+    while (!S.queue.empty()) {
+        const gw_query& q = S.queue.front();
+        if (S.live + q.walks > S.pool_cap || S.free_slots.empty()) break;
+        const uint64_t distinct = std::min<uint64_t>(static_cast<uint64_t>(q.walks) * L, ctx->n);
+        uint64_t size = 64;
+        while (size < 2 * distinct) size <<= 1;
+        uint64_t base = 0;
+        if (size > S.arena_slots || !arena_alloc(S, size, base)) break;
+        const uint32_t slot = S.free_slots.back();
+        S.free_slots.pop_back();
+        SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
+                    static_cast<uint32_t>(q.qid >> 32)};
+        GW_HIP(ctx, hipMemcpyAsync(S.d_slots + slot, &si, sizeof si, hipMemcpyHostToDevice, ctx->stream));
+        const uint32_t w = q.walks;
+        GW_HIP(ctx, hipMemcpyAsync(S.d_remaining + slot, &w, 4, hipMemcpyHostToDevice, ctx->stream));
+        serve_init_kernel<<<dim3((w + 255) / 256), dim3(256), 0, ctx->stream>>>(S.pool[S.cur], S.live, w, slot, q.source);
+        GW_HIP(ctx, hipGetLastError());
+        S.live += w;
+        S.active.push_back({q.qid, slot, w, base, size});
+        S.queue.pop_front();
+        ++st.admitted;
+        // the host copies above read &si / &w before returning for pageable
+        // memory, so the locals may go out of scope
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    // 2. one step over every live walk
+    GW_HIP(ctx, hipMemsetAsync(S.d_cnt, 0, 8 * 8, ctx->stream));
+    GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (S.live) {
+        const Prog p{L, S.prog.nvertices, S.prog.stop_thr, static_cast<uint32_t>(S.prog.seed),
+                     static_cast<uint32_t>(S.prog.seed >> 32), 0u};
+        const uint32_t grid = std::max(1u, ctx->cu_count) * 4u * 8u * 64u / 256u;
+        hipError_t e = S.prog.kind == GW_RESTART ? launch_serve<GW_RESTART>(ctx, S, p, S.live, grid)
+                     : S.prog.kind == GW_STOP    ? launch_serve<GW_STOP>(ctx, S, p, S.live, grid)
+                                                 : launch_serve<GW_JUMP>(ctx, S, p, S.live, grid);
+        if (e != hipSuccess) return gw_hip_check(ctx, e, "serve step launch");
+    }
+    GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    unsigned long long cnt[4];
+    GW_HIP(ctx, hipMemcpyAsync(cnt, S.d_cnt, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint32_t> rem(S.max_queries);
+    GW_HIP(ctx, hipMemcpyAsync(rem.data(), S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (cnt[3]) return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed");
+    st.steps = cnt[1];
+    st.visits = cnt[2];
+    st.step_ms = ms;
+    S.live = S.live ? cnt[0] : 0;
+    S.cur ^= 1;
+    // 3. completed queries: compact their count regions, release slot + region
+    std::vector<ServeState::Active> still;
+    for (const auto& a : S.active) {
+        if (rem[a.slot] != 0) {
+            still.push_back(a);
+            continue;
+        }
+        if (a.size > S.pairs_cap) {
+            gw_free(ctx, S.d_pairs, S.pairs_cap * 8);
+            S.d_pairs = nullptr;
+            S.pairs_cap = 0;
+            int rc = gw_alloc(ctx, (void**)&S.d_pairs, a.size * 8, "serve compaction");
+            if (rc) return rc;
+            S.pairs_cap = a.size;
+        }
+        GW_HIP(ctx, hipMemsetAsync(S.d_cnt + 4, 0, 8, ctx->stream));
+        serve_compact_kernel<<<dim3(static_cast<uint32_t>((a.size + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            S.arena + a.base, a.size, S.d_pairs, S.d_cnt + 4);
+        GW_HIP(ctx, hipGetLastError());
+        unsigned long long np = 0;
+        GW_HIP(ctx, hipMemcpyAsync(&np, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<unsigned long long> pairs(np);
+        if (np) GW_HIP(ctx, hipMemcpy(pairs.data(), S.d_pairs, np * 8, hipMemcpyDeviceToHost));
+        GW_HIP(ctx, hipMemsetAsync(S.arena + a.base, 0, a.size * 8, ctx->stream));
+        arena_free(S, a.base, a.size);
+        S.free_slots.push_back(a.slot);
+        S.done[a.qid] = std::move(pairs);
+        S.done_order.push_back(a.qid);
+        ++st.completed;
+    }
+    S.active.swap(still);
+    st.live_walks = S.live;
+    st.active_queries = S.active.size();
+    st.queued = S.queue.size();
+    if (out) *out = st;
+    return GW_OK;
+}
+
+int gw_serve_poll(gw_ctx* ctx, uint64_t* qid, uint64_t* npairs) {
+    if (!ctx || !qid || !npairs) return gw_fail(ctx, GW_EINVAL, "gw_serve_poll: bad arguments");
+    auto it = states().find(ctx);
+    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: not open");
+    ServeState& S = *it->second;
+    if (S.done_order.empty()) return 1;  // nothing completed
+    *qid = S.done_order.front();
+    S.done_order.pop_front();
+    *npairs = S.done[*qid].size();
+    return GW_OK;
+}
+
+int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count, uint64_t cap) {
+    if (!ctx) return GW_EINVAL;
+    auto it = states().find(ctx);
+    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_result: not open");
+    ServeState& S = *it->second;
+    auto d = S.done.find(qid);
+    if (d == S.done.end()) return gw_fail(ctx, GW_EINVAL, "gw_serve_result: no completed query with that id");
+    std::vector<unsigned long long>& pairs = d->second;
+    if (cap < pairs.size() || (!pairs.empty() && (!vertex || !count)))
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_result: output arrays too small");
+    std::sort(pairs.begin(), pairs.end());  // by vertex (the key is in the high bits)
+    const unsigned long long cmask = (1ull << S.cbits) - 1;
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        vertex[i] = static_cast<uint32_t>(pairs[i] >> S.cbits);
+        count[i] = static_cast<uint32_t>(pairs[i] & cmask);
+    }
+    S.done.erase(d);
+    return GW_OK;
+}
+
+int gw_serve_close(gw_ctx* ctx) {
+    if (!ctx) return GW_EINVAL;
+    auto it = states().find(ctx);
+    if (it == states().end()) return GW_OK;
+    ServeState* S = it->second;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    gw_free(ctx, S->pool[0], S->pool_cap * sizeof(gw_walk));
+    gw_free(ctx, S->pool[1], S->pool_cap * sizeof(gw_walk));
+    gw_free(ctx, S->arena, S->arena_slots * 8);
+    gw_free(ctx, S->d_slots, S->max_queries * sizeof(SlotInfo));
+    gw_free(ctx, S->d_remaining, S->max_queries * 4ull);
+    gw_free(ctx, S->d_cnt, 64);
+    gw_free(ctx, S->d_pairs, S->pairs_cap * 8);
+    delete S;
+    states().erase(it);
+    return GW_OK;
+}
+
+}  // extern "C"

@@ -185,6 +185,42 @@ int gw_source_counts_read(gw_ctx *ctx, uint32_t *source, uint32_t *vertex, uint3
  * `n_walks` and `cap` must equal the loaded walk count and the run's event_cap. */
 int gw_events_read(gw_ctx *ctx, uint64_t *out, uint64_t n_walks, uint32_t cap);
 
+/* ---- serving: continuous batching of PPR-style queries -------------------
+ * Admitted queries share one device pool of walk records; each gw_serve_step
+ * advances every live walk by at most `hops_per_step` hops, so queries that
+ * arrive while others are in flight join at the next step.  A query's visit
+ * counts accumulate in its own hash region of a count arena in HBM; queries
+ * are admitted first come first served while walk slots, a query slot and
+ * arena space (2 x min(walks * L, n) slots rounded up to a power of two) are
+ * free.  The random stream of walk i of query q is Philox(counter {i, hop,
+ * 'WALK', q.qid low word}, key {seed low, seed high ^ q.qid high word}), so a
+ * query's result does not depend on the other queries in flight.  There is
+ * no counterpart in the reference (it has no serving loop). */
+typedef struct gw_query {
+    uint64_t qid;     /* caller's id: keys the query's random streams */
+    uint32_t source;  /* source vertex */
+    uint32_t walks;   /* walk count */
+} gw_query;
+
+typedef struct gw_serve_stats {
+    uint64_t steps, visits;   /* walk-steps / visits made by this serve step */
+    uint64_t live_walks;      /* walks still in flight after the step */
+    uint32_t admitted, completed, active_queries, queued;
+    double step_ms;           /* step kernel time, HIP events on the engine stream */
+} gw_serve_stats;
+
+int gw_serve_open(gw_ctx *ctx, const gw_program *prog, uint64_t pool_walks, uint64_t arena_bytes,
+                  uint32_t max_queries, uint32_t hops_per_step);
+int gw_serve_submit(gw_ctx *ctx, const gw_query *queries, uint32_t n);
+int gw_serve_step(gw_ctx *ctx, gw_serve_stats *stats);
+/* Pops the next completed query: returns 0 and its id and pair count, or 1
+ * when none is waiting. */
+int gw_serve_poll(gw_ctx *ctx, uint64_t *qid, uint64_t *npairs);
+/* Copies a completed query's (vertex, count) pairs, sorted by vertex, and
+ * releases them. */
+int gw_serve_result(gw_ctx *ctx, uint64_t qid, uint32_t *vertex, uint32_t *count, uint64_t cap);
+int gw_serve_close(gw_ctx *ctx);
+
 /* Random-gather ceiling: `gathers` independent random 4-byte loads from a
  * buffer of `bytes` bytes; returns the kernel time in ms. */
 int gw_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);

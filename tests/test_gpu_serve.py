@@ -1,0 +1,92 @@
+"""Continuous-batching query serving (gw_serve_*): every query's result is
+bit-exact against the oracle, and identical alone and in the mix."""
+import numpy as np
+import pytest
+
+import graphwalker_amd as gw
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED
+
+
+def expected(off, col, kind, L, qid, source, walks):
+    prog = O.program(kind, L, 0.15, off.shape[0] - 1, seed=SEED ^ ((qid >> 32) << 32), stream=qid & 0xFFFFFFFF)
+    c = O.replay(prog, off, col, O.make_walks(np.full(walks, source, np.uint32)))["counts"]
+    nz = np.nonzero(c)[0]
+    return nz.astype(np.uint32), c[nz]
+
+
+def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64):
+    """arrivals: list of (step, qid, source, walks); submits each at its step."""
+    engine.serve_open(gw.program(kind, L, 0.15, 0, seed=SEED), pool, arena, max_queries, hops)
+    results, step, pending = {}, 0, sorted(arrivals)
+    try:
+        while pending or len(results) < len(arrivals):
+            now = [a for a in pending if a[0] <= step]
+            pending = [a for a in pending if a[0] > step]
+            if now:
+                engine.serve_submit([(q, s, w) for _, q, s, w in now])
+            st = engine.serve_step()
+            assert st["live_walks"] <= pool
+            while (r := engine.serve_poll()) is not None:
+                results[r[0]] = engine.serve_result(*r)
+            step += 1
+            assert step < 10_000
+    finally:
+        engine.serve_close()
+    return results
+
+
+@pytest.fixture(scope="module")
+def g12(engine):
+    off, col = O.rmat_graph(12, 1 << 16, seed=7)
+    engine.graph_set(off, col)
+    return off, col
+
+
+@pytest.mark.parametrize("kind,hops", [(gw.RESTART, 3), (gw.STOP, 2), (gw.RESTART, 10)])
+def test_served_results_bit_exact(engine, g12, kind, hops):
+    off, col = g12
+    rng = np.random.default_rng(kind * 10 + hops)
+    arrivals = [(int(rng.integers(0, 8)), (int(rng.integers(0, 4)) << 32) | i, int(rng.integers(0, 4096)),
+                 int(rng.integers(50, 3000))) for i in range(40)]
+    res = serve_all(engine, arrivals, kind, 10, hops, pool=200_000, arena=64 << 20)
+    assert len(res) == len(arrivals)
+    for _, qid, src, w in arrivals:
+        v, c = res[qid]
+        ev, ec = expected(off, col, kind, 10, qid, src, w)
+        assert np.array_equal(v, ev) and np.array_equal(c, ec), qid
+
+
+def test_result_alone_equals_result_in_mix(engine, g12):
+    rng = np.random.default_rng(5)
+    arrivals = [(i // 5, 1000 + i, int(rng.integers(0, 4096)), int(rng.integers(100, 5000))) for i in range(25)]
+    mixed = serve_all(engine, arrivals, gw.RESTART, 10, 3, pool=60_000, arena=64 << 20)
+    for a in arrivals[::6]:
+        alone = serve_all(engine, [(0,) + a[1:]], gw.RESTART, 10, 3, pool=60_000, arena=64 << 20)
+        assert np.array_equal(alone[a[1]][0], mixed[a[1]][0])
+        assert np.array_equal(alone[a[1]][1], mixed[a[1]][1])
+
+
+def test_admission_is_bounded_by_the_pool(engine, g12):
+    # 30 queries of 3000 walks with a 7000-walk pool: at most two in flight
+    arrivals = [(0, i, i * 37 % 4096, 3000) for i in range(30)]
+    res = serve_all(engine, arrivals, gw.RESTART, 10, 4, pool=7000, arena=64 << 20)
+    assert sorted(res) == list(range(30))
+    off, col = g12
+    v, c = res[17]
+    ev, ec = expected(off, col, gw.RESTART, 10, 17, 17 * 37 % 4096, 3000)
+    assert np.array_equal(v, ev) and np.array_equal(c, ec)
+
+
+def test_serve_rejects_bad_queries(engine, g12):
+    engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), 1000, 1 << 20, 8, 4)
+    try:
+        with pytest.raises(gw.GwError):
+            engine.serve_submit([(1, 1 << 20, 10)])
+        with pytest.raises(gw.GwError):
+            engine.serve_submit([(1, 5, 5000)])  # more walks than the pool
+    finally:
+        engine.serve_close()
