@@ -152,6 +152,56 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     return res
 
 
+def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5, hops: int = 5,
+              walks_lo: int = 1000, walks_hi: int = 100_000) -> dict:
+    """Poisson arrivals of restart-PPR queries (p = 0.15, 10 hops) at `qps`,
+    walk counts log-uniform in [walks_lo, walks_hi], uniform sources, served
+    by continuous batching.  Latency = completion (result on the host) minus
+    arrival, on the host clock."""
+    rng = np.random.default_rng(seed)
+    nq = max(1, int(qps * seconds))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, nq))
+    walks = np.exp(rng.uniform(np.log(walks_lo), np.log(walks_hi), nq)).astype(np.int64)
+    sources = gw.sources(seed, n, nq)
+    eng.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=seed), pool_walks=1 << 24, arena_bytes=16 << 30,
+                   max_queries=4096, hops_per_step=hops)
+    lat = np.zeros(nq)
+    steps = visits = nsteps = 0
+    step_ms = 0.0
+    done = nxt = 0
+    t0 = time.perf_counter()
+    try:
+        while done < nq:
+            now = time.perf_counter() - t0
+            k = nxt
+            while k < nq and arrivals[k] <= now:
+                k += 1
+            if k > nxt:
+                eng.serve_submit([(i, int(sources[i]), int(walks[i])) for i in range(nxt, k)])
+                nxt = k
+            if done + 0 < nxt:
+                st = eng.serve_step()
+                steps += st["steps"]
+                visits += st["visits"]
+                step_ms += st["step_ms"]
+                nsteps += 1
+                t_done = time.perf_counter() - t0
+                while (r := eng.serve_poll()) is not None:
+                    eng.serve_result(*r)
+                    lat[r[0]] = t_done - arrivals[r[0]]
+                    done += 1
+            elif nxt < nq:
+                time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
+    finally:
+        eng.serve_close()
+    wall = time.perf_counter() - t0
+    return {"qps_offered": qps, "queries": nq, "seconds": wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+            "walk_steps_per_s": steps / wall, "visits": visits, "serve_steps": nsteps,
+            "gpu_busy_frac": step_ms * 1e-3 / wall, "hops_per_step": hops,
+            "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +213,8 @@ def main() -> int:
     ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -206,6 +258,16 @@ def main() -> int:
             "schedule": f"{args.schedule}-{args.layout}",
         }
 
+    serving = {}
+    if args.serve_qps:
+        # continuous batching on the graph left resident by the last workload
+        n_last = 1 << (WORKLOADS[args.secondary]["scale"] if workloads else wl["scale"])
+        for q in args.serve_qps:
+            r = serve_sim(eng, n_last, q, args.serve_seconds)
+            serving[f"qps_{int(q)}"] = r
+            print(f"[bench] serving {q:.0f} qps: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
+                  f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+
     out = {
         "metric": "walk-steps/s",
         "value": m["value"],
@@ -231,6 +293,7 @@ def main() -> int:
         "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
         "variants": m["variants"],
         "workloads": workloads,
+        "serving": serving,
     }
     if "speedup_vs_plain" in m:
         out["speedup_vs_plain"] = m["speedup_vs_plain"]

@@ -30,6 +30,9 @@ struct SlotInfo {
     uint32_t qid_hi;   // mixed into Philox key word 1
 };
 
+struct Admit;
+struct Done;
+
 struct ServeState {
     gw_program prog{};
     uint32_t hops = 0;
@@ -41,8 +44,11 @@ struct ServeState {
     SlotInfo* d_slots = nullptr;
     uint32_t* d_remaining = nullptr;
     unsigned long long* d_cnt = nullptr;  // [0] out cursor [1] steps [2] visits [3] overflow [4] compact n
-    unsigned long long* d_pairs = nullptr;  // compaction buffer (largest region)
+    unsigned long long* d_pairs = nullptr;  // compaction output
     uint64_t pairs_cap = 0;
+    Admit* d_admit = nullptr;               // max_queries admission records
+    Done* d_done = nullptr;                 // max_queries completion records
+    unsigned long long* d_region_n = nullptr;  // 2 x max_queries: counts, cursors
     struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; };
     std::vector<Active> active;
     std::vector<uint32_t> free_slots;
@@ -210,23 +216,67 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
     }
 }
 
-__global__ void serve_init_kernel(gw_walk* __restrict__ pool, uint64_t at, uint32_t walks, uint32_t slot,
-                                  uint32_t source) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i < walks) pool[at + i] = gw_walk{static_cast<uint32_t>(i), slot, source, 0u};
+// A query admitted in this step: its slot record, walk count and where its
+// walks start in the pool.
+struct Admit {
+    SlotInfo info;
+    uint64_t at;
+    uint32_t slot, walks;
+};
+
+// One workgroup per admitted query: writes its slot record and remaining
+// count, then its walk records (id = walk index, source field = slot).
+__global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __restrict__ pool,
+                                   SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining) {
+    const Admit a = adm[blockIdx.x];
+    if (threadIdx.x == 0) {
+        slots[a.slot] = a.info;
+        remaining[a.slot] = a.walks;
+    }
+    for (uint32_t i = threadIdx.x; i < a.walks; i += blockDim.x)
+        pool[a.at + i] = gw_walk{i, a.slot, a.info.source, 0u};
 }
 
-__global__ void serve_compact_kernel(const unsigned long long* __restrict__ region, uint64_t size,
-                                     unsigned long long* __restrict__ out, unsigned long long* __restrict__ n) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    const bool full = i < size && region[i] != 0ull;
-    const uint64_t m = __ballot(full);
-    if (m == 0) return;
-    const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-    unsigned long long base = 0;
-    if (__lane_id() == leader) base = atomicAdd(n, static_cast<unsigned long long>(__popcll(m)));
-    base = __shfl(base, leader, 64);
-    if (full) out[base + rank_of(m)] = region[i];
+// A completed query's count region and where its pairs go.
+struct Done {
+    uint64_t base, size;   // region
+    uint64_t out;          // first output index (after the scan)
+};
+
+// Pass 1: non-empty slots per region (grid.y = region).
+__global__ void serve_count_kernel(const unsigned long long* __restrict__ arena, const Done* __restrict__ done,
+                                   unsigned long long* __restrict__ per_region) {
+    const Done d = done[blockIdx.y];
+    uint32_t c = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < d.size;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        c += arena[d.base + i] != 0ull;
+    c = wave_sum(c);
+    if (__lane_id() == 0 && c) atomicAdd(&per_region[blockIdx.y], static_cast<unsigned long long>(c));
+}
+
+// Pass 2: compact each region's entries to its output range, then clear the
+// region for reuse.
+__global__ void serve_compact_kernel(unsigned long long* __restrict__ arena, const Done* __restrict__ done,
+                                     unsigned long long* __restrict__ cursor, unsigned long long* __restrict__ out) {
+    const Done d = done[blockIdx.y];
+    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < d.size;
+         i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        const unsigned long long x = i < d.size ? arena[d.base + i] : 0ull;
+        const bool full = x != 0ull;
+        const uint64_t m = __ballot(full);
+        if (m) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            unsigned long long b = 0;
+            if (__lane_id() == leader) b = atomicAdd(&cursor[blockIdx.y], static_cast<unsigned long long>(__popcll(m)));
+            b = __shfl(b, leader, 64);
+            if (full) {
+                out[d.out + b + rank_of(m)] = x;
+                arena[d.base + i] = 0ull;
+            }
+        }
+    }
 }
 
 template <int KIND>
@@ -269,7 +319,10 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         (rc = gw_alloc(ctx, (void**)&S->arena, S->arena_slots * 8, "serve count arena")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_slots, max_queries * sizeof(SlotInfo), "serve slots")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_remaining, max_queries * 4ull, "serve remaining")) ||
-        (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters"))) {
+        (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 16ull, "serve region counts"))) {
         gw_free(ctx, S->pool[0], pool_walks * sizeof(gw_walk));
         gw_free(ctx, S->pool[1], pool_walks * sizeof(gw_walk));
         gw_free(ctx, S->arena, S->arena_slots * 8);
@@ -310,8 +363,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     // 1. admit queued queries in arrival order while walk slots, a query slot
     //    and arena space for their count region are free
     const uint32_t L = S.prog.L;
-    std::vector<SlotInfo> new_slots;
-    std::vector<uint32_t> new_ids;
+    std::vector<Admit> adm;
     while (!S.queue.empty()) {
         const gw_query& q = S.queue.front();
         if (S.live + q.walks > S.pool_cap || S.free_slots.empty()) break;
@@ -322,20 +374,20 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         if (size > S.arena_slots || !arena_alloc(S, size, base)) break;
         const uint32_t slot = S.free_slots.back();
         S.free_slots.pop_back();
-        SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
-                    static_cast<uint32_t>(q.qid >> 32)};
-        GW_HIP(ctx, hipMemcpyAsync(S.d_slots + slot, &si, sizeof si, hipMemcpyHostToDevice, ctx->stream));
-        const uint32_t w = q.walks;
-        GW_HIP(ctx, hipMemcpyAsync(S.d_remaining + slot, &w, 4, hipMemcpyHostToDevice, ctx->stream));
-        serve_init_kernel<<<dim3((w + 255) / 256), dim3(256), 0, ctx->stream>>>(S.pool[S.cur], S.live, w, slot, q.source);
-        GW_HIP(ctx, hipGetLastError());
-        S.live += w;
-        S.active.push_back({q.qid, slot, w, base, size});
+        const SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
+                          static_cast<uint32_t>(q.qid >> 32)};
+        adm.push_back(Admit{si, S.live, slot, q.walks});
+        S.live += q.walks;
+        S.active.push_back({q.qid, slot, q.walks, base, size});
         S.queue.pop_front();
-        ++st.admitted;
-        // the host copies above read &si / &w before returning for pageable
-        // memory, so the locals may go out of scope
-        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (!adm.empty()) {
+        GW_HIP(ctx, hipMemcpyAsync(S.d_admit, adm.data(), adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        serve_admit_kernel<<<dim3(static_cast<uint32_t>(adm.size())), dim3(256), 0, ctx->stream>>>(
+            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining);
+        GW_HIP(ctx, hipGetLastError());
+        st.admitted = static_cast<uint32_t>(adm.size());
     }
     // 2. one step over every live walk
     GW_HIP(ctx, hipMemsetAsync(S.d_cnt, 0, 8 * 8, ctx->stream));
@@ -363,36 +415,55 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     st.step_ms = ms;
     S.live = S.live ? cnt[0] : 0;
     S.cur ^= 1;
-    // 3. completed queries: compact their count regions, release slot + region
-    std::vector<ServeState::Active> still;
-    for (const auto& a : S.active) {
-        if (rem[a.slot] != 0) {
-            still.push_back(a);
-            continue;
+    // 3. completed queries: count, scan and compact their regions in one pass
+    //    each (regions are cleared for reuse as they are compacted)
+    std::vector<ServeState::Active> still, fin;
+    for (const auto& a : S.active) (rem[a.slot] != 0 ? still : fin).push_back(a);
+    if (!fin.empty()) {
+        std::vector<Done> dn(fin.size());
+        uint64_t maxsize = 0;
+        for (size_t i = 0; i < fin.size(); ++i) {
+            dn[i] = Done{fin[i].base, fin[i].size, 0};
+            maxsize = std::max(maxsize, fin[i].size);
         }
-        if (a.size > S.pairs_cap) {
+        const uint32_t nd = static_cast<uint32_t>(fin.size());
+        const uint32_t gx = static_cast<uint32_t>(std::min<uint64_t>(64, (maxsize + 255) / 256));
+        GW_HIP(ctx, hipMemcpyAsync(S.d_done, dn.data(), nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
+        GW_HIP(ctx, hipMemsetAsync(S.d_region_n, 0, nd * 16ull, ctx->stream));
+        serve_count_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_done, S.d_region_n);
+        GW_HIP(ctx, hipGetLastError());
+        std::vector<unsigned long long> per(nd);
+        GW_HIP(ctx, hipMemcpyAsync(per.data(), S.d_region_n, nd * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < nd; ++i) {
+            dn[i].out = total;
+            total += per[i];
+        }
+        if (total > S.pairs_cap) {
             gw_free(ctx, S.d_pairs, S.pairs_cap * 8);
             S.d_pairs = nullptr;
             S.pairs_cap = 0;
-            int rc = gw_alloc(ctx, (void**)&S.d_pairs, a.size * 8, "serve compaction");
+            int rc = gw_alloc(ctx, (void**)&S.d_pairs, std::max<uint64_t>(total, 1 << 20) * 8, "serve compaction");
             if (rc) return rc;
-            S.pairs_cap = a.size;
+            S.pairs_cap = std::max<uint64_t>(total, 1 << 20);
         }
-        GW_HIP(ctx, hipMemsetAsync(S.d_cnt + 4, 0, 8, ctx->stream));
-        serve_compact_kernel<<<dim3(static_cast<uint32_t>((a.size + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            S.arena + a.base, a.size, S.d_pairs, S.d_cnt + 4);
+        GW_HIP(ctx, hipMemcpyAsync(S.d_done, dn.data(), nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
+        serve_compact_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_done, S.d_region_n + nd,
+                                                                          S.d_pairs);
         GW_HIP(ctx, hipGetLastError());
-        unsigned long long np = 0;
-        GW_HIP(ctx, hipMemcpyAsync(&np, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<unsigned long long> all(total);
+        if (total)
+            GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
         GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        std::vector<unsigned long long> pairs(np);
-        if (np) GW_HIP(ctx, hipMemcpy(pairs.data(), S.d_pairs, np * 8, hipMemcpyDeviceToHost));
-        GW_HIP(ctx, hipMemsetAsync(S.arena + a.base, 0, a.size * 8, ctx->stream));
-        arena_free(S, a.base, a.size);
-        S.free_slots.push_back(a.slot);
-        S.done[a.qid] = std::move(pairs);
-        S.done_order.push_back(a.qid);
-        ++st.completed;
+        for (uint32_t i = 0; i < nd; ++i) {
+            const auto& a = fin[i];
+            S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
+            S.done_order.push_back(a.qid);
+            arena_free(S, a.base, a.size);
+            S.free_slots.push_back(a.slot);
+        }
+        st.completed = nd;
     }
     S.active.swap(still);
     st.live_walks = S.live;
@@ -448,6 +519,9 @@ int gw_serve_close(gw_ctx* ctx) {
     gw_free(ctx, S->d_remaining, S->max_queries * 4ull);
     gw_free(ctx, S->d_cnt, 64);
     gw_free(ctx, S->d_pairs, S->pairs_cap * 8);
+    gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
+    gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
+    gw_free(ctx, S->d_region_n, S->max_queries * 16ull);
     delete S;
     states().erase(it);
     return GW_OK;
