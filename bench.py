@@ -187,7 +187,7 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
                 nsteps += 1
                 t_done = time.perf_counter() - t0
                 while (r := eng.serve_poll()) is not None:
-                    eng.serve_result(*r)
+                    eng.serve_result(*r, sort=False)
                     lat[r[0]] = t_done - arrivals[r[0]]
                     done += 1
             elif nxt < nq:

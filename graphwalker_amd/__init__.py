@@ -351,10 +351,14 @@ class Engine:
         self._check(rc, "gw_serve_poll")
         return qid.value, n.value
 
-    def serve_result(self, qid: int, npairs: int) -> tuple[np.ndarray, np.ndarray]:
+    def serve_result(self, qid: int, npairs: int, sort: bool = True) -> tuple[np.ndarray, np.ndarray]:
+        """A completed query's (vertex, count) pairs (sorted by vertex unless sort=False)."""
         v = np.empty(npairs, dtype=np.uint32)
         c = np.empty(npairs, dtype=np.uint32)
         self._check(self._L.gw_serve_result(self._ctx, qid, _ptr(v), _ptr(c), npairs), "gw_serve_result")
+        if sort:
+            o = np.argsort(v, kind="stable")
+            v, c = v[o], c[o]
         return v, c
 
     def serve_close(self) -> None:

@@ -230,29 +230,67 @@ __device__ __forceinline__ int advance(const Prog& p, const Graph& g, uint32_t i
     return kMoved;
 }
 
-// One step of a walk on the CSR layout (the body of advance() without the
-// block test).  Returns true when the walk has ended.
-template <int KIND, class Rec>
-__device__ __forceinline__ bool step_once_plain(const Prog& p, const Graph& g, uint32_t id, uint32_t source,
-                                                uint32_t& v, uint32_t& hop, Rec& rec) {
+// Walk state carried in registers.  With the successor-header layout the
+// current vertex's row start and degree ride along, fetched with the edge
+// that led to the vertex.
+struct State {
+    uint32_t id, source, v, hop;
+    uint64_t row;
+    uint32_t deg;
+};
+
+template <bool EXT>
+__device__ __forceinline__ void load_row(const Graph& g, State& s) {
+    if (EXT) {
+        const uint64_t b = g.off[s.v];
+        s.row = b;
+        s.deg = static_cast<uint32_t>(g.off[s.v + 1] - b);
+    }
+}
+
+// One step of a walk.  Returns true when the walk has ended; the final
+// vertex/hop stay in the state, as the reference leaves them in the record it
+// drops.  Else-branches per kind follow advance() in gw_device.h.
+template <int KIND, bool EXT, class Rec>
+__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
     constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
-    if (capped && hop >= p.L) return true;
-    rec.visit(v, hop);
-    const u32x4 r = philox4x32_10(id, hop, kTagWalk, p.stream, p.k0, p.k1);
-    const uint64_t b = g.off[v];
-    const uint64_t deg = g.off[v + 1] - b;
+    if (capped && s.hop >= p.L) return true;
+    rec.visit(s.v, s.hop);
+    const u32x4 r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
+    uint64_t b, deg;
+    if (EXT) {
+        b = s.row;
+        deg = s.deg;
+    } else {
+        b = g.off[s.v];
+        deg = g.off[s.v + 1] - b;
+    }
     if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
         const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
-        v = g.col[b + mulhi64(x, deg)];
+        const uint64_t e = b + mulhi64(x, deg);
+        if (EXT) {
+            const Edge ed = g.ext[e];
+            s.v = ed.dst;
+            s.deg = ed.deg;
+            s.row = ed.row;
+        } else {
+            s.v = g.col[e];
+        }
     } else {
-        if (KIND == GW_JUMP) v = mulhi32(r.w, p.nvertices);
-        else if (KIND == GW_STOP || KIND == GW_PROB) return true;
-        else v = source;
+        if (KIND == GW_JUMP) {
+            s.v = mulhi32(r.w, p.nvertices);
+            load_row<EXT>(g, s);
+        } else if (KIND == GW_STOP || KIND == GW_PROB) {
+            return true;
+        } else {
+            s.v = s.source;
+            load_row<EXT>(g, s);
+        }
     }
-    ++hop;
+    ++s.hop;
     ++rec.steps;
-    if (KIND == GW_RESTART_JOINT) return hop % p.L == p.L - 1;
-    if (capped) return hop >= p.L;
+    if (KIND == GW_RESTART_JOINT) return s.hop % p.L == p.L - 1;
+    if (capped) return s.hop >= p.L;
     return false;
 }
 

@@ -85,6 +85,8 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);
+// Builds the successor-header edges if they fit; false when the CSR is the only layout.
+bool gw_ensure_ext(gw_ctx* ctx);
 
 // gw_table.hip: sort + run-length encode the key log (n keys); ms = device time
 int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms);

@@ -43,13 +43,15 @@ struct ServeState {
     unsigned long long* arena = nullptr;
     SlotInfo* d_slots = nullptr;
     uint32_t* d_remaining = nullptr;
+    uint32_t* d_srccount = nullptr;       // visits of each query to its own source
+    bool ext = false;
     unsigned long long* d_cnt = nullptr;  // [0] out cursor [1] steps [2] visits [3] overflow [4] compact n
     unsigned long long* d_pairs = nullptr;  // compaction output
     uint64_t pairs_cap = 0;
     Admit* d_admit = nullptr;               // max_queries admission records
     Done* d_done = nullptr;                 // max_queries completion records
     unsigned long long* d_region_n = nullptr;  // 2 x max_queries: counts, cursors
-    struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; };
+    struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; uint32_t source; };
     std::vector<Active> active;
     std::vector<uint32_t> free_slots;
     std::vector<std::pair<uint64_t, uint64_t>> free_ranges;  // arena [start, size), sorted by start
@@ -98,23 +100,48 @@ void arena_free(ServeState& S, uint64_t base, uint64_t size) {
     }
 }
 
+__device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Per-visit recorder of a served walk.  Visits to the query's own source (a
+// quarter or more of a restart walk's visits) go to a per-query counter, one
+// atomic per (wavefront, query); other vertices go to the query's hash region:
+// a plain load finds an existing entry (one atomic add), an empty slot is
+// claimed with compare-and-swap.
 struct ServeRec {
     unsigned long long* arena;
+    uint32_t* srccount;
     uint64_t base;
     uint32_t mask;
     uint32_t cbits;
     unsigned long long* overflow;
+    uint32_t slot, source;
     uint32_t visits = 0, steps = 0;
     __device__ __forceinline__ void visit(uint32_t v, uint32_t) {
         ++visits;
+        const bool at_src = v == source;
+        uint64_t todo = __ballot(at_src);
+        while (todo) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+            const uint32_t ls = __shfl(slot, leader, 64);
+            const uint64_t grp = __ballot(at_src && slot == ls);
+            if (__lane_id() == leader) atomicAdd(&srccount[ls], static_cast<uint32_t>(__popcll(grp)));
+            todo &= ~grp;
+        }
+        if (at_src) return;
         const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | 1ull;
         uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
-            unsigned long long* slot = arena + base + h;
-            const unsigned long long old = atomicCAS(slot, 0ull, entry);
-            if (old == 0ull) return;
-            if ((old >> cbits) == v) {
-                atomicAdd(slot, 1ull);
+            unsigned long long* cell = arena + base + h;
+            unsigned long long cur = __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == 0ull) {
+                cur = atomicCAS(cell, 0ull, entry);
+                if (cur == 0ull) return;
+            }
+            if ((cur >> cbits) == v) {
+                atomicAdd(cell, 1ull);
                 return;
             }
             h = (h + 1) & mask;
@@ -123,19 +150,15 @@ struct ServeRec {
     }
 };
 
-__device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
-
 // One serve step: every walk of pool_in advances at most `hops` hops; walks
 // still alive are appended to pool_out; ended walks decrement their query's
 // remaining count.  Persistent wavefronts take chunks of the pool.
-template <int KIND>
+template <int KIND, bool EXT>
 __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const gw_walk* __restrict__ pool_in,
                                                         uint64_t n, gw_walk* __restrict__ pool_out,
                                                         const SlotInfo* __restrict__ slots,
                                                         uint32_t* __restrict__ remaining,
+                                                        uint32_t* __restrict__ srccount,
                                                         unsigned long long* __restrict__ arena, uint32_t cbits,
                                                         uint32_t hops, unsigned long long* __restrict__ cnt,
                                                         unsigned long long* __restrict__ cursor) {
@@ -145,8 +168,8 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
     bool exhausted = false, active = false;
     uint32_t slot = 0, budget = 0, tot_steps = 0, tot_visits = 0;
     Prog q = p;
-    ServeRec rec{arena, 0, 0, cbits, cnt + 3};
-    struct { uint32_t id, source, v, hop; } s{0, 0, 0, 0};
+    ServeRec rec{arena, srccount, 0, 0, cbits, cnt + 3, 0, 0};
+    State s{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint64_t idle = __ballot(!active);
         while (idle && !exhausted) {
@@ -170,10 +193,13 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
                 s.source = si.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
+                load_row<EXT>(g, s);
                 q.stream = si.qid_lo;
                 q.k1 = p.k1 ^ si.qid_hi;
                 rec.base = si.base;
                 rec.mask = si.mask;
+                rec.slot = slot;
+                rec.source = si.source;
                 rec.visits = 0;
                 rec.steps = 0;
                 budget = hops;
@@ -185,8 +211,8 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
         if (__ballot(active) == 0) break;
         bool leave = false, done = false;
         if (active) {
-            uint32_t hop_before = s.hop;
-            done = step_once_plain<KIND>(q, g, s.id, s.source, s.v, s.hop, rec);
+            const uint32_t hop_before = s.hop;
+            done = step_once<KIND, EXT>(q, g, s, rec);
             budget -= (s.hop != hop_before) ? 1u : 0u;
             if (done) {
                 atomicSub(&remaining[slot], 1u);
@@ -227,11 +253,13 @@ struct Admit {
 // One workgroup per admitted query: writes its slot record and remaining
 // count, then its walk records (id = walk index, source field = slot).
 __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __restrict__ pool,
-                                   SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining) {
+                                   SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining,
+                                   uint32_t* __restrict__ srccount) {
     const Admit a = adm[blockIdx.x];
     if (threadIdx.x == 0) {
         slots[a.slot] = a.info;
         remaining[a.slot] = a.walks;
+        srccount[a.slot] = 0;
     }
     for (uint32_t i = threadIdx.x; i < a.walks; i += blockDim.x)
         pool[a.at + i] = gw_walk{i, a.slot, a.info.source, 0u};
@@ -281,10 +309,15 @@ __global__ void serve_compact_kernel(unsigned long long* __restrict__ arena, con
 
 template <int KIND>
 hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, uint32_t grid) {
-    const Graph g{ctx->d_off, ctx->d_col, nullptr, ctx->n};
-    serve_step_kernel<KIND><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-        p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.arena, S.cbits, S.hops, S.d_cnt,
-        S.d_cnt + 5);
+    const Graph g{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+    if (S.ext)
+        serve_step_kernel<KIND, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.cbits,
+            S.hops, S.d_cnt, S.d_cnt + 5);
+    else
+        serve_step_kernel<KIND, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.cbits,
+            S.hops, S.d_cnt, S.d_cnt + 5);
     return hipGetLastError();
 }
 
@@ -292,6 +325,8 @@ hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, u
 }  // namespace gw
 
 using namespace gw;
+
+static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S);
 
 extern "C" {
 
@@ -319,6 +354,7 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         (rc = gw_alloc(ctx, (void**)&S->arena, S->arena_slots * 8, "serve count arena")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_slots, max_queries * sizeof(SlotInfo), "serve slots")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_remaining, max_queries * 4ull, "serve remaining")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_srccount, max_queries * 4ull, "serve source counts")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
@@ -330,6 +366,11 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         gw_free(ctx, S->d_remaining, max_queries * 4ull);
         delete S;
         return rc;
+    }
+    S->ext = prog->layout != GW_LAYOUT_CSR && gw_ensure_ext(ctx);
+    if (prog->layout == GW_LAYOUT_EXT && !S->ext) {
+        gw_serve_close_state(ctx, S);
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: successor-header layout does not fit");
     }
     GW_HIP(ctx, hipMemsetAsync(S->arena, 0, S->arena_slots * 8, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -378,14 +419,14 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                           static_cast<uint32_t>(q.qid >> 32)};
         adm.push_back(Admit{si, S.live, slot, q.walks});
         S.live += q.walks;
-        S.active.push_back({q.qid, slot, q.walks, base, size});
+        S.active.push_back({q.qid, slot, q.walks, base, size, q.source});
         S.queue.pop_front();
     }
     if (!adm.empty()) {
         GW_HIP(ctx, hipMemcpyAsync(S.d_admit, adm.data(), adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
                                    ctx->stream));
         serve_admit_kernel<<<dim3(static_cast<uint32_t>(adm.size())), dim3(256), 0, ctx->stream>>>(
-            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining);
+            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount);
         GW_HIP(ctx, hipGetLastError());
         st.admitted = static_cast<uint32_t>(adm.size());
     }
@@ -456,9 +497,13 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         if (total)
             GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
         GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<uint32_t> srcc(S.max_queries);
+        GW_HIP(ctx, hipMemcpy(srcc.data(), S.d_srccount, S.max_queries * 4ull, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < nd; ++i) {
             const auto& a = fin[i];
             S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
+            if (srcc[a.slot])
+                S.done[a.qid].push_back((static_cast<unsigned long long>(a.source) << S.cbits) | srcc[a.slot]);
             S.done_order.push_back(a.qid);
             arena_free(S, a.base, a.size);
             S.free_slots.push_back(a.slot);
@@ -495,7 +540,6 @@ int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count
     std::vector<unsigned long long>& pairs = d->second;
     if (cap < pairs.size() || (!pairs.empty() && (!vertex || !count)))
         return gw_fail(ctx, GW_EINVAL, "gw_serve_result: output arrays too small");
-    std::sort(pairs.begin(), pairs.end());  // by vertex (the key is in the high bits)
     const unsigned long long cmask = (1ull << S.cbits) - 1;
     for (size_t i = 0; i < pairs.size(); ++i) {
         vertex[i] = static_cast<uint32_t>(pairs[i] >> S.cbits);
@@ -510,6 +554,14 @@ int gw_serve_close(gw_ctx* ctx) {
     auto it = states().find(ctx);
     if (it == states().end()) return GW_OK;
     ServeState* S = it->second;
+    states().erase(it);
+    gw_serve_close_state(ctx, S);
+    return GW_OK;
+}
+
+}  // extern "C"
+
+static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     gw_free(ctx, S->pool[0], S->pool_cap * sizeof(gw_walk));
@@ -517,14 +569,11 @@ int gw_serve_close(gw_ctx* ctx) {
     gw_free(ctx, S->arena, S->arena_slots * 8);
     gw_free(ctx, S->d_slots, S->max_queries * sizeof(SlotInfo));
     gw_free(ctx, S->d_remaining, S->max_queries * 4ull);
+    gw_free(ctx, S->d_srccount, S->max_queries * 4ull);
     gw_free(ctx, S->d_cnt, 64);
     gw_free(ctx, S->d_pairs, S->pairs_cap * 8);
     gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
     gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
     gw_free(ctx, S->d_region_n, S->max_queries * 16ull);
     delete S;
-    states().erase(it);
-    return GW_OK;
 }
-
-}  // extern "C"

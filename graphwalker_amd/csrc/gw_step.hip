@@ -37,70 +37,6 @@ __device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Walk state carried in registers.  With the successor-header layout the
-// current vertex's row start and degree ride along, fetched with the edge
-// that led to the vertex.
-struct State {
-    uint32_t id, source, v, hop;
-    uint64_t row;
-    uint32_t deg;
-};
-
-template <bool EXT>
-__device__ __forceinline__ void load_row(const Graph& g, State& s) {
-    if (EXT) {
-        const uint64_t b = g.off[s.v];
-        s.row = b;
-        s.deg = static_cast<uint32_t>(g.off[s.v + 1] - b);
-    }
-}
-
-// One step of a walk.  Returns true when the walk has ended; the final
-// vertex/hop stay in the state, as the reference leaves them in the record it
-// drops.  Else-branches per kind follow advance() in gw_device.h.
-template <int KIND, bool EXT, class Rec>
-__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
-    constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
-    if (capped && s.hop >= p.L) return true;
-    rec.visit(s.v, s.hop);
-    const u32x4 r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
-    uint64_t b, deg;
-    if (EXT) {
-        b = s.row;
-        deg = s.deg;
-    } else {
-        b = g.off[s.v];
-        deg = g.off[s.v + 1] - b;
-    }
-    if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
-        const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
-        const uint64_t e = b + mulhi64(x, deg);
-        if (EXT) {
-            const Edge ed = g.ext[e];
-            s.v = ed.dst;
-            s.deg = ed.deg;
-            s.row = ed.row;
-        } else {
-            s.v = g.col[e];
-        }
-    } else {
-        if (KIND == GW_JUMP) {
-            s.v = mulhi32(r.w, p.nvertices);
-            load_row<EXT>(g, s);
-        } else if (KIND == GW_STOP || KIND == GW_PROB) {
-            return true;
-        } else {
-            s.v = s.source;
-            load_row<EXT>(g, s);
-        }
-    }
-    ++s.hop;
-    ++rec.steps;
-    if (KIND == GW_RESTART_JOINT) return s.hop % p.L == p.L - 1;
-    if (capped) return s.hop >= p.L;
-    return false;
-}
-
 template <int KIND, int REC>
 __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const gw_walk* __restrict__ walks,
                                                         gw_walk* __restrict__ final_walks, uint64_t nw,
@@ -576,6 +512,8 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
 
 }  // namespace
 }  // namespace gw
+
+bool gw_ensure_ext(gw_ctx* ctx) { return gw::ensure_ext(ctx); }
 
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     using namespace gw;

@@ -216,8 +216,8 @@ int gw_serve_step(gw_ctx *ctx, gw_serve_stats *stats);
 /* Pops the next completed query: returns 0 and its id and pair count, or 1
  * when none is waiting. */
 int gw_serve_poll(gw_ctx *ctx, uint64_t *qid, uint64_t *npairs);
-/* Copies a completed query's (vertex, count) pairs, sorted by vertex, and
- * releases them. */
+/* Copies a completed query's (vertex, count) pairs, in no particular order,
+ * and releases them. */
 int gw_serve_result(gw_ctx *ctx, uint64_t qid, uint32_t *vertex, uint32_t *count, uint64_t cap);
 int gw_serve_close(gw_ctx *ctx);
 
