@@ -145,7 +145,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     res = {
         "value": steps / wall, "wall_s": wall, "steps": per_pass_steps, "visits": per_pass_visits, "prog": prog,
         "kernel_ms": kavg, "algorithmic_bytes": abytes, "achieved": achieved, "layout": st["layout"], "n": n,
-        "variants": variants, "wl": wl,
+        "variants": variants, "wl": wl, "last": st,
     }
     if "plain-csr" in variants:
         res["speedup_vs_plain"] = res["value"] / variants["plain-csr"]["walk_steps_per_s"]
@@ -247,8 +247,11 @@ def main() -> int:
     workloads = {}
     if args.secondary and args.secondary != args.workload:
         s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False)
-        _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1)
+        _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_GLOBAL)
         _, wh_, sth_, _, _, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_HASH)
+        _, wg_, stg_, _, _, lg_ = timed(eng, s["wl"], s["n"], args.schedule, args.layout, 2, 1,
+                                        source_method=gw.SRC_GLOBAL)
+        last = s["last"]
         workloads[args.secondary] = {
             "desc": s["wl"]["desc"], "walk_steps_per_s": s["value"], "visits_per_s": s["visits"] * (s["value"] / s["steps"]),
             "walk_steps_per_pass": s["steps"], "visits_per_pass": s["visits"], "kernel_ms": s["kernel_ms"],
@@ -256,6 +259,10 @@ def main() -> int:
             "plain_csr_walk_steps_per_s": st_ / w_, "speedup_vs_plain": s["value"] / (st_ / w_),
             "plain_csr_hash_walk_steps_per_s": sth_ / wh_, "speedup_vs_plain_hash": s["value"] / (sth_ / wh_),
             "schedule": f"{args.schedule}-{args.layout}",
+            "source_method": {gw.SRC_PARTITIONED: "partitioned", gw.SRC_GLOBAL: "global-sort",
+                              gw.SRC_HASH: "hash"}.get(last["source_method"], "?"),
+            "reduce_ms": last["reduce_ms"], "flushed_sources": last["flushed_sources"],
+            "global_sort_walk_steps_per_s": stg_ / wg_, "global_sort_reduce_ms": lg_["reduce_ms"],
         }
 
     serving = {}

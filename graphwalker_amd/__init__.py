@@ -31,7 +31,7 @@ SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PER
 LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_EXT = 0, 1, 2
 LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
 # per-source count aggregation
-SRC_SORT, SRC_HASH = 0, 1
+SRC_SORT, SRC_HASH, SRC_GLOBAL, SRC_PARTITIONED = 0, 1, 2, 3
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
 
@@ -96,6 +96,9 @@ class Stats(C.Structure):
         ("bucket_ms", C.c_double),
         ("schedule", C.c_uint32),
         ("layout", C.c_uint32),
+        ("reduce_ms", C.c_double),
+        ("source_method", C.c_uint32),
+        ("flushed_sources", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

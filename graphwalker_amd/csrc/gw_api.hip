@@ -196,7 +196,7 @@ int gw_create(gw_ctx** out, const gw_cfg* cfg) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
-    if (gw_alloc(ctx, (void**)&ctx->d_counters, 16 * sizeof(unsigned long long), "counters")) {
+    if (gw_alloc(ctx, (void**)&ctx->d_counters, kCounters * sizeof(unsigned long long), "counters")) {
         gw_destroy(ctx);
         return GW_ENOMEM;
     }
@@ -211,12 +211,17 @@ void gw_destroy(gw_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_graph(ctx);
     free_walks(ctx);
-    gw_free(ctx, ctx->d_counters, 16 * sizeof(unsigned long long));
+    gw_free(ctx, ctx->d_counters, kCounters * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
     gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
     gw_free(ctx, ctx->d_keys, ctx->keys_cap * 8);
     gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);
+    gw_free(ctx, ctx->d_part_keys, ctx->part_keys_cap * 4);
+    gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
+    gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
+    gw_free(ctx, ctx->d_split_keys, ctx->split_cap * 8);
+    gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
     gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
     gw_free(ctx, ctx->d_bucket_off, 0);
@@ -323,6 +328,12 @@ int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
     if (rc) return rc;
     if (n) GW_HIP(ctx, hipMemcpy(ctx->d_walks, walks, n * sizeof(gw_walk), hipMemcpyHostToDevice));
     ctx->walk_src_max = src_max;
+    ctx->field_walks.clear();
+    ctx->part_per_walk = 0;
+    if (n && src_max < kPartFieldLimit) {
+        ctx->field_walks.assign(src_max + 1ull, 0);
+        for (uint64_t i = 0; i < n; ++i) ctx->field_walks[walks[i].source]++;
+    }
     return GW_OK;
 }
 
@@ -339,6 +350,9 @@ int gw_walks_init_per_vertex(gw_ctx* ctx, uint32_t per_vertex) {
     GW_HIP(ctx, hipGetLastError());
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->walk_src_max = ctx->n - 1;
+    ctx->field_walks.clear();
+    ctx->part_per_walk = 0;
+    if (ctx->n <= kPartFieldLimit) ctx->field_walks.assign(ctx->n, per_vertex);
     return GW_OK;
 }
 
@@ -372,6 +386,12 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     gw_free(ctx, d_field, nsources * 4ull);
     if (e != hipSuccess) return gw_hip_check(ctx, e, "init_sources_kernel");
     ctx->walk_src_max = src_max;
+    ctx->field_walks.clear();
+    ctx->part_per_walk = 0;
+    if (src_max < kPartFieldLimit) {
+        ctx->field_walks.assign(src_max + 1ull, 0);
+        for (uint32_t k = 0; k < nsources; ++k) ctx->field_walks[source_field ? source_field[k] : sources[k]] += walks_per_source;
+    }
     return GW_OK;
 }
 
@@ -405,7 +425,7 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
     if ((prog->recorder & GW_REC_SOURCE_COUNT) && (prog->recorder & (GW_REC_VERTEX_COUNT | GW_REC_EVENTS)))
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_SOURCE_COUNT combines with GW_REC_WALK_SUMMARY only");
-    if (prog->source_method > GW_SRC_HASH) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown source_method");
+    if (prog->source_method > GW_SRC_PARTITIONED) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown source_method");
     if ((prog->recorder & GW_REC_EVENTS) && prog->event_cap == 0)
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_EVENTS needs event_cap > 0");
     // Restart programs jump to the source field, which must then name a vertex.

@@ -246,7 +246,7 @@ int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std
     // overwrites the histogram, by scanning and reading bucket starts.
     scan_chunks_kernel<<<dim3(static_cast<uint32_t>(nchunks)), dim3(kScanThreads), 0, ctx->stream>>>(hist, entries, sums);
     GW_HIP(ctx, hipGetLastError());
-    scan_sums_kernel<<<dim3(1), dim3(kScanThreads), 0, ctx->stream>>>(sums, static_cast<uint32_t>(nchunks), ctx->d_counters + 8);
+    scan_sums_kernel<<<dim3(1), dim3(kScanThreads), 0, ctx->stream>>>(sums, static_cast<uint32_t>(nchunks), ctx->d_counters + kCntBucketTotal);
     GW_HIP(ctx, hipGetLastError());
     scan_add_kernel<<<dim3(static_cast<uint32_t>((entries + 255) / 256)), dim3(256), 0, ctx->stream>>>(hist, entries, sums);
     GW_HIP(ctx, hipGetLastError());

@@ -9,6 +9,25 @@
 
 #include "../../include/gwhip.h"
 
+// Source fields below this bound get a partitioned key log (one region per field).
+constexpr uint32_t kPartFieldLimit = 1u << 20;
+
+// Device counters (unsigned long long each).
+constexpr uint32_t kCounters = 32;
+constexpr uint32_t kCntSteps = 0, kCntVisits = 1, kCntWorkCursor = 2;
+constexpr uint32_t kCntBucketTotal = 8;       // bucketing scan total
+constexpr uint32_t kCntExtOverflow = 9;       // successor-header build: degree >= 2^32
+constexpr uint32_t kCntTableOverflow = 10;    // hash source counts: table full
+constexpr uint32_t kCntCompact = 11;          // hash read-out: occupied slots
+constexpr uint32_t kCntLogCursor = 12;        // key log slots claimed
+constexpr uint32_t kCntLogOverflow = 13;      // key log full
+constexpr uint32_t kCntRuns = 14;             // global sort-reduce: runs
+constexpr uint32_t kCntPartOverflow = 15;     // partitioned: keys that missed their region
+constexpr uint32_t kCntOutPairs = 16;         // partitioned: exact pairs emitted
+constexpr uint32_t kCntSplitPairs = 17;       // partitioned: partial pairs of flushed sources
+constexpr uint32_t kCntFlushed = 18;          // partitioned: flushed sources
+constexpr uint32_t kCntSplitRuns = 19;        // partitioned: sums of the split pairs
+
 struct gw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,18 +72,34 @@ struct gw_ctx {
     unsigned long long* d_keys_alt = nullptr;  // sort output
     void* d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    unsigned long long* d_runs = nullptr;    // sorted unique keys of the last sort-reduce
+    unsigned long long* d_runs = nullptr;    // (source, vertex) keys of the last source-count run
     uint32_t* d_run_counts = nullptr;
     uint64_t runs_cap = 0, n_runs = 0;
+    bool runs_sorted = true;                 // false: d_runs is in no particular order
+    // Walks per source-field value of the loaded walks (empty when a field is
+    // kPartFieldLimit or more): sizes the per-source regions of the
+    // partitioned key log.
+    std::vector<uint64_t> field_walks;
+    uint32_t part_per_walk = 0;              // visits per walk the uploaded regions are sized for (0 = stale)
+    uint32_t* d_part_keys = nullptr;         // vertices, grouped by source field
+    uint64_t part_keys_cap = 0;
+    uint64_t* d_part_off = nullptr;          // region start per field (fields + 1)
+    unsigned long long* d_part_cursor = nullptr;  // keys written per field
+    uint32_t part_fields = 0, part_fields_cap = 0;
+    unsigned long long* d_split_keys = nullptr;  // partial pairs of sources that outgrew the LDS table
+    uint32_t* d_split_counts = nullptr;          // 2 x split_cap (input, sorted)
+    uint64_t split_cap = 0;
+    uint64_t part_total = 0;                 // region slots in all (sum of the caps)
+    uint64_t part_flushed = 0;               // sources whose pairs went through the split sort (last run)
 
-    // device counters: [0] steps [1] visits [2] work cursor [3..] scratch
+    // device counters (kCounters of them, see the kCnt* indices)
     unsigned long long* d_counters = nullptr;
     // bucketing scratch
     uint32_t* d_tile_hist = nullptr;
     uint64_t tile_hist_cap = 0;
     unsigned long long* d_bucket_off = nullptr;  // nblocks + 2
 
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 // Record an error message on the context and return `code`.
@@ -90,6 +125,18 @@ bool gw_ensure_ext(gw_ctx* ctx);
 
 // gw_table.hip: sort + run-length encode the key log (n keys); ms = device time
 int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms);
+// gw_table.hip, partitioned source counts.  prepare: size the per-source
+// regions for `per_walk` visits per walk and the pair buffers; *ok = false
+// when the fields or the memory do not allow it.  reset: zero the device
+// counters (and the region cursors when `part`) before a run.  enqueue:
+// partition the key log and aggregate every source, with no host sync.
+// finish: given the counters read back after the run, grow-and-repeat or
+// sort-reduce split pairs as needed; *done = false when a region overflowed
+// (the caller falls back to gw_keylog_reduce).  ms = extra device time.
+int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok);
+int gw_partlog_reset(gw_ctx* ctx, bool part);
+int gw_partlog_enqueue(gw_ctx* ctx);
+int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, double* ms);
 
 // gw_bucket.hip: stable counting sort of `in` (n records) by block into `out`;
 // bucket_off[b] = first index of bucket b (nblocks + 1 entries written, host).

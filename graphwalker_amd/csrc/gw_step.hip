@@ -377,14 +377,14 @@ bool ensure_ext(gw_ctx* ctx) {
         ctx->ext_failed = true;
         return false;
     }
-    hipError_t e = hipMemsetAsync(ctx->d_counters + 9, 0, 8, ctx->stream);
+    hipError_t e = hipMemsetAsync(ctx->d_counters + kCntExtOverflow, 0, 8, ctx->stream);
     if (e == hipSuccess && ctx->m) {
         build_ext_kernel<<<dim3(static_cast<uint32_t>((ctx->m + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_off, ctx->d_col, ctx->m, reinterpret_cast<Edge*>(ctx->d_ext), ctx->d_counters + 9);
+            ctx->d_off, ctx->d_col, ctx->m, reinterpret_cast<Edge*>(ctx->d_ext), ctx->d_counters + kCntExtOverflow);
         e = hipGetLastError();
     }
     unsigned long long overflow = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&overflow, ctx->d_counters + 9, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&overflow, ctx->d_counters + kCntExtOverflow, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess || overflow) {
         gw_free(ctx, ctx->d_ext, std::max<uint64_t>(1, ctx->m) * sizeof(Edge));
@@ -423,7 +423,7 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
         std::swap(a, b);  // buckets now in `a`
         if (live == 0) break;
         // One launch steps every bucket; wavefronts take walks in block order.
-        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 2, 0, sizeof(unsigned long long), ctx->stream));
+        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntWorkCursor, 0, sizeof(unsigned long long), ctx->stream));
         GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
         L.bucket = a;
         L.bucket_n = live;
@@ -444,9 +444,10 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
 // Key log for sort-reduce source counts: capacity for every visit the
 // program can make (capped kinds: L per walk; uncapped: 64 per walk, more is
 // an overflow error) plus slab padding.
+uint32_t keylog_per_walk(const gw_program* prog) { return prog->kind == GW_PROB ? 64u : prog->L; }
+
 int prepare_keylog(gw_ctx* ctx, const gw_program* prog, uint32_t sched, uint32_t grid_waves, uint32_t* slab) {
-    uint64_t per_walk = prog->L;
-    if (prog->kind == GW_PROB) per_walk = 64;
+    const uint64_t per_walk = keylog_per_walk(prog);
     // Slab per wavefront: large for the persistent grid (few, long-lived
     // wavefronts), small for the plain grid (one short-lived wavefront per 64
     // walks).  Each launch leaves at most one partly used slab per wavefront.
@@ -471,7 +472,6 @@ int prepare_keylog(gw_ctx* ctx, const gw_program* prog, uint32_t sched, uint32_t
     }
     const uint32_t key_bits = 64 - ctx->table_cbits;
     ctx->keys_sentinel = 1ull << key_bits;
-    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 12, 0, 16, ctx->stream));
     return GW_OK;
 }
 
@@ -506,7 +506,6 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     ctx->table_vbits = vbits;
     ctx->table_cbits = 64 - vbits - sbits;
     GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, slots * 8, ctx->stream));
-    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 10, 0, 8, ctx->stream));
     return GW_OK;
 }
 
@@ -547,13 +546,15 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (rec & GW_REC_SOURCE_COUNT) {
         int rc = prepare_source_table(ctx, prog);
         if (rc) return rc;
-        if (prog->source_method == GW_SRC_SORT) {
+        if (prog->source_method != GW_SRC_HASH) {
+            if (prog->source_method == GW_SRC_PARTITIONED && ctx->field_walks.empty())
+                return gw_fail(ctx, GW_EINVAL, "gw_run: GW_SRC_PARTITIONED needs source fields below 2^20");
             const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
             const uint32_t tb = prog->block_threads ? prog->block_threads : 256u;
             const uint32_t grid_waves = std::max(1u, ctx->cu_count) * 4u * wps / (tb / 64) * (tb / 64);
             if ((rc = prepare_keylog(ctx, prog, sched, grid_waves, &keylog_slab))) return rc;
             rec = (rec & ~GW_REC_SOURCE_COUNT) | kRecKeyLog;
-            ctx->src_method = GW_SRC_SORT;
+            ctx->src_method = GW_SRC_GLOBAL;
         }
     }
     if ((rec & GW_REC_WALK_SUMMARY) && !ctx->d_hash) {
@@ -573,9 +574,9 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
                static_cast<uint32_t>(prog->seed >> 32), prog->stream};
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
-    L.table = SourceTable{ctx->d_table, ctx->d_counters + 10, ctx->table_slots - 1, 64u - ctx->table_log2,
+    L.table = SourceTable{ctx->d_table, ctx->d_counters + kCntTableOverflow, ctx->table_slots - 1, 64u - ctx->table_log2,
                           ctx->table_vbits, ctx->table_cbits};
-    L.klog = KeyLog{ctx->d_keys, ctx->d_counters + 12, ctx->d_counters + 13, ctx->keys_cap, ctx->keys_sentinel,
+    L.klog = KeyLog{ctx->d_keys, ctx->d_counters + kCntLogCursor, ctx->d_counters + kCntLogOverflow, ctx->keys_cap, ctx->keys_sentinel,
                     ctx->table_vbits, keylog_slab};
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
@@ -583,7 +584,20 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     const uint64_t waves = static_cast<uint64_t>(std::max(1u, ctx->cu_count)) * 4u * wps;
     L.grid_persistent = static_cast<uint32_t>(std::max<uint64_t>(1, waves * 64 / L.tb));
 
-    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    // Partitioned source counts: regions and pair buffers are sized before
+    // the run so partition + aggregation follow the step kernel on the
+    // stream with no host round trip in between.
+    bool part = false;
+    if ((rec & kRecKeyLog) && prog->source_method != GW_SRC_GLOBAL && !ctx->field_walks.empty()) {
+        int rc = gw_partlog_prepare(ctx, keylog_per_walk(prog), &part);
+        if (rc) return rc;
+        if (!part && prog->source_method == GW_SRC_PARTITIONED)
+            return gw_fail(ctx, GW_ENOMEM, "gw_run: the partitioned key log does not fit the HBM budget");
+    }
+    {
+        int rc = gw_partlog_reset(ctx, part);
+        if (rc) return rc;
+    }
     if (rec & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
     float ms = 0.f;
     if (sched == GW_SCHED_BLOCKED) {
@@ -591,38 +605,60 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         int rc = ctx->nwalks ? run_blocked(ctx, L, prog, rec, st) : GW_OK;
         if (rc) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        GW_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
-        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-        st.total_ms = ms;
     } else {
-        GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
         if (ctx->nwalks) {
             hipError_t e = launch(L, prog->kind, rec);
             if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
             st.launches = 1;
         }
-        GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-        GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
-        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
         st.rounds = 1;
-        st.step_ms = ms;
-        st.total_ms = ms;
     }
-    unsigned long long cnt[14];
-    GW_HIP(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    st.steps = cnt[0];
-    st.visits = cnt[1];
-    if ((rec & GW_REC_SOURCE_COUNT) && cnt[10])
-        return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[10]) +
+    if (part) {
+        int rc = gw_partlog_enqueue(ctx);
+        if (rc) return rc;
+    }
+    GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    unsigned long long cnt[kCounters];
+    GW_HIP(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    st.total_ms = ms;
+    if (sched != GW_SCHED_BLOCKED) st.step_ms = ms;
+    st.steps = cnt[kCntSteps];
+    st.visits = cnt[kCntVisits];
+    if ((rec & GW_REC_SOURCE_COUNT) && cnt[kCntTableOverflow])
+        return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[kCntTableOverflow]) +
                                            " visits not recorded); raise table_log2");
     if (rec & kRecKeyLog) {
-        if (cnt[13]) return gw_fail(ctx, GW_ENOMEM, "gw_run: source key log full (walks longer than the log allows)");
+        if (cnt[kCntLogOverflow])
+            return gw_fail(ctx, GW_ENOMEM, "gw_run: source key log full (walks longer than the log allows)");
         double rms = 0;
-        int rc = gw_keylog_reduce(ctx, cnt[12], &rms);
-        if (rc) return rc;
+        bool done = false;
+        int rc;
+        if (part) {
+            GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]));
+            rms = ms;
+            double extra = 0;
+            if ((rc = gw_partlog_finish(ctx, cnt, &done, &extra))) return rc;
+            rms += extra;
+            if (!done && prog->source_method == GW_SRC_PARTITIONED)
+                return gw_fail(ctx, GW_ENOMEM, "gw_run: a source's visits outgrew its key-log region (GW_SRC_PARTITIONED)");
+        }
+        if (done) {
+            ctx->src_method = GW_SRC_PARTITIONED;
+            st.flushed_sources = static_cast<uint32_t>(ctx->part_flushed);
+        } else {
+            double gms = 0;
+            if ((rc = gw_keylog_reduce(ctx, cnt[kCntLogCursor], &gms))) return rc;
+            ctx->src_method = GW_SRC_GLOBAL;
+            rms += gms;
+        }
         st.total_ms += rms;
-        st.bucket_ms += rms;
+        st.reduce_ms = rms;
     }
+    st.source_method = (rec & (GW_REC_SOURCE_COUNT | kRecKeyLog)) ? ctx->src_method : 0u;
     if (stats) *stats = st;
     return GW_OK;
 }

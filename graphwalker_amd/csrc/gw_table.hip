@@ -1,7 +1,14 @@
-// gw_table.hip -- read-out of the (source, vertex) -> count hash table that
-// GW_REC_SOURCE_COUNT fills during a run: non-empty slots are compacted,
-// radix-sorted (rocPRIM through hipCUB, off the walk path) and
-// unpacked into (source, vertex, count) triples sorted by source, vertex.
+// gw_table.hip -- aggregation and read-out of GW_REC_SOURCE_COUNT results.
+//
+//  * Global sort-reduce: the step kernel's (source, vertex) key log is
+//    radix-sorted (rocPRIM through hipCUB) and run-length encoded.
+//  * Partitioned: the key log is scattered into one region per source field
+//    (vertex ids only), then one workgroup per source counts its region in an
+//    LDS hash table.  A source whose distinct vertices outgrow the table
+//    flushes partial pairs to a split list that is sort-reduced exactly.
+//  * Hash: non-empty slots of the device hash table are compacted and sorted
+//    at read-out.
+// Read-out unpacks (source, vertex, count) triples sorted by source, vertex.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -54,42 +61,286 @@ __global__ void unpack_runs_kernel(const unsigned long long* __restrict__ keys, 
     c[i] = counts[i];
 }
 
+
+constexpr uint32_t kAggLog2 = 14;
+constexpr uint32_t kAggSlots = 1u << kAggLog2;               // LDS hash slots per workgroup (128 KiB)
+constexpr uint32_t kAggThreads = 1024;
+constexpr uint32_t kAggWaves = kAggThreads / 64;
+constexpr uint32_t kAggPerThread = 4;
+constexpr uint32_t kAggTile = kAggThreads * kAggPerThread;   // keys loaded per iteration
+constexpr uint32_t kAggMaxFill = kAggSlots * 3 / 4;          // the table is never fuller than this
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;                   // never a vertex id (n <= 2^32 - 1)
+constexpr uint32_t kPartPerLane = 4;                         // log slots per lane per partition sweep
+
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Zero the engine's device counters and the per-source region cursors before
+// a run, in one launch.
+__global__ void reset_counters_kernel(unsigned long long* __restrict__ counters, uint32_t ncounters,
+                                      unsigned long long* __restrict__ region_n, uint32_t nf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncounters) counters[i] = 0ull;
+    if (i < nf) region_n[i] = 0ull;
+}
+
+// Scatter the key log into per-source regions: region f receives the vertex
+// ids of source f's visits in [region_off[f], region_off[f + 1]).  Each
+// wavefront sweeps 256 consecutive log slots at a time (4 per lane); the
+// slots are grouped by source with a ballot loop, one atomic per group.  The
+// slots of a wavefront-sized stretch of the log come from one step-kernel
+// wavefront and name one to three sources, so a sweep costs one to three
+// atomics.  The log length is read from the device (*n_ptr, capped at `cap`),
+// so no host round trip separates the step kernel from this one.  A key that
+// does not fit its region, or names a field >= nf, counts in `overflow` and
+// the host discards the partition.
+__global__ __launch_bounds__(256) void partition_kernel(const unsigned long long* __restrict__ log,
+                                                       const unsigned long long* __restrict__ n_ptr, uint64_t cap,
+                                                       uint64_t sentinel, uint32_t vbits, uint32_t nf,
+                                                       const uint64_t* __restrict__ region_off,
+                                                       unsigned long long* __restrict__ region_n,
+                                                       uint32_t* __restrict__ out,
+                                                       unsigned long long* __restrict__ overflow) {
+    const uint64_t n = std::min<uint64_t>(*n_ptr, cap);
+    const uint32_t lane = __lane_id();
+    const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+    const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+    constexpr uint32_t kSweep = 64 * kPartPerLane;
+    for (uint64_t base = wave * kSweep; base < n; base += nwaves * kSweep) {
+        uint32_t f[kPartPerLane], v[kPartPerLane];
+        bool pend[kPartPerLane];
+#pragma unroll
+        for (uint32_t j = 0; j < kPartPerLane; ++j) {
+            const uint64_t i = base + j * 64 + lane;
+            const uint64_t key = i < n ? log[i] : sentinel;
+            pend[j] = key != sentinel;
+            f[j] = static_cast<uint32_t>(key >> vbits);
+            v[j] = static_cast<uint32_t>(key & ((1ull << vbits) - 1));
+            if (pend[j] && f[j] >= nf) {
+                atomicAdd(overflow, 1ull);
+                pend[j] = false;
+            }
+        }
+        for (;;) {
+            // leader: the first pending slot of the sweep (wave-uniform search)
+            bool any = false;
+            uint32_t fl = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kPartPerLane; ++j) {
+                const uint64_t b = __ballot(pend[j]);
+                if (!any && b) {
+                    fl = __shfl(f[j], static_cast<int>(__ffsll(static_cast<unsigned long long>(b)) - 1), 64);
+                    any = true;
+                }
+            }
+            if (!any) break;
+            uint64_t mt[kPartPerLane];
+            uint32_t tot = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kPartPerLane; ++j) {
+                mt[j] = __ballot(pend[j] && f[j] == fl);
+                tot += __popcll(mt[j]);
+            }
+            unsigned long long at = 0;
+            if (lane == 0) at = atomicAdd(&region_n[fl], static_cast<unsigned long long>(tot));
+            at = __shfl(at, 0, 64);
+            const uint64_t lo = region_off[fl], hi = region_off[fl + 1];
+#pragma unroll
+            for (uint32_t j = 0; j < kPartPerLane; ++j) {
+                if (pend[j] && f[j] == fl) {
+                    const uint64_t pos = lo + at + rank_in(mt[j]);
+                    if (pos < hi) out[pos] = v[j];
+                    else atomicAdd(overflow, 1ull);
+                    pend[j] = false;
+                }
+                at += __popcll(mt[j]);
+            }
+        }
+    }
+}
+
+// Emit every occupied LDS slot as ((source << vbits) | vertex, count) to
+// out[base..] and clear the table, with one global atomic per workgroup: each
+// wavefront counts the occupied slots of its contiguous share of the table,
+// the counts are scanned in LDS, then every wavefront writes its share in
+// slot order.  The atomic reserves the pairs even when they do not fit
+// `cap`; then nothing is written and the host, which reads the total, grows
+// the buffer and runs the aggregation again.  All threads of the workgroup.
+__device__ void agg_emit(uint32_t* s_key, uint32_t* s_cnt, uint32_t* s_wave, unsigned long long* s_base,
+                         uint64_t fbase, unsigned long long* __restrict__ out_keys, uint32_t* __restrict__ out_counts,
+                         unsigned long long* __restrict__ out_n, uint64_t cap) {
+    constexpr uint32_t kShare = kAggSlots / kAggWaves;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lo = wave * kShare;
+    uint32_t c = 0;
+    for (uint32_t i = lo + lane; i < lo + kShare; i += 64) c += __popcll(__ballot(s_key[i] != kEmptyKey));
+    if (lane == 0) s_wave[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kAggWaves; ++w) {
+            const uint32_t x = s_wave[w];
+            s_wave[w] = total;
+            total += x;
+        }
+        unsigned long long b = total ? atomicAdd(out_n, static_cast<unsigned long long>(total)) : 0ull;
+        s_wave[kAggWaves] = (b + total <= cap) ? 1u : 0u;
+        *s_base = b;
+    }
+    __syncthreads();
+    const bool fits = s_wave[kAggWaves] != 0;
+    unsigned long long at = *s_base + s_wave[wave];
+    for (uint32_t i = lo + lane; i < lo + kShare; i += 64) {
+        const uint32_t k = s_key[i];
+        const bool full = k != kEmptyKey;
+        const uint64_t mask = __ballot(full);
+        if (full && fits) {
+            const uint64_t pos = at + rank_in(mask);
+            out_keys[pos] = fbase | k;
+            out_counts[pos] = s_cnt[i];
+        }
+        at += __popcll(mask);
+        s_key[i] = kEmptyKey;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();  // s_wave / s_base are reused by the next emit
+}
+
+// One workgroup per source field: count the region's vertex ids in an LDS
+// open-addressing table (linear probing, multiplicative hash).  Keys are
+// loaded kAggTile at a time, the next tile in flight while the current one is
+// inserted.  Before a tile would lift the fill above kAggMaxFill (3/4 of the
+// slots) the table is flushed to the split list, so every probe sequence
+// ends.  Sources that never flushed emit their exact pairs to `out`; the
+// others emit partial pairs to `split`, summed later by sort-reduce.
+__global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
+    const uint32_t* __restrict__ keys, const uint64_t* __restrict__ region_off,
+    const unsigned long long* __restrict__ region_n, uint32_t vbits, unsigned long long* __restrict__ out_keys,
+    uint32_t* __restrict__ out_counts, unsigned long long* __restrict__ out_n, uint64_t out_cap,
+    unsigned long long* __restrict__ split_keys, uint32_t* __restrict__ split_counts,
+    unsigned long long* __restrict__ split_n, uint64_t split_cap, unsigned long long* __restrict__ flushed_sources) {
+    __shared__ uint32_t s_key[kAggSlots];
+    __shared__ uint32_t s_cnt[kAggSlots];
+    __shared__ uint32_t s_fill;
+    __shared__ uint32_t s_wave[kAggWaves + 1];
+    __shared__ unsigned long long s_base;
+    const uint32_t f = blockIdx.x;
+    const uint64_t lo = region_off[f];
+    // clamp to the region: an overflowed partition is discarded by the host,
+    // but must still be read in bounds
+    const uint64_t n = std::min<uint64_t>(region_n[f], region_off[f + 1] - lo);
+    if (n == 0) return;
+    const uint32_t* __restrict__ k = keys + lo;
+    const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
+    uint32_t cur[kAggPerThread], nxt[kAggPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kAggPerThread; ++j) {
+        const uint64_t idx = j * kAggThreads + threadIdx.x;
+        cur[j] = idx < n ? k[idx] : kEmptyKey;
+    }
+    // clear the table while the first tile is in flight
+    for (uint32_t i = threadIdx.x; i < kAggSlots; i += kAggThreads) {
+        s_key[i] = kEmptyKey;
+        s_cnt[i] = 0;
+    }
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+    bool flushed = false;
+    for (uint64_t base = 0; base < n; base += kAggTile) {
+        const uint64_t next = base + kAggTile;  // start of the next tile
+        if (next < n) {
+#pragma unroll
+            for (uint32_t j = 0; j < kAggPerThread; ++j) {
+                const uint64_t idx = next + j * kAggThreads + threadIdx.x;
+                nxt[j] = idx < n ? k[idx] : kEmptyKey;
+            }
+        }
+        uint32_t added = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kAggPerThread; ++j) {
+            const uint32_t v = cur[j];
+            if (v == kEmptyKey) continue;
+            uint32_t h = (v * 0x9E3779B1u) >> (32 - kAggLog2);
+            for (;;) {
+                const uint32_t old = atomicCAS(&s_key[h], kEmptyKey, v);
+                if (old == kEmptyKey || old == v) {
+                    atomicAdd(&s_cnt[h], 1u);
+                    added += old == kEmptyKey;
+                    break;
+                }
+                h = (h + 1) & (kAggSlots - 1);
+            }
+        }
+        if (added) atomicAdd(&s_fill, added);
+        __syncthreads();
+        const uint32_t fill = s_fill;
+        __syncthreads();  // every thread has read the fill before it is reset
+        if (next >= n) break;
+        const uint64_t next_keys = std::min<uint64_t>(kAggTile, n - next);
+        if (fill + next_keys > kAggMaxFill) {  // workgroup-uniform
+            agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, split_keys, split_counts, split_n, split_cap);
+            if (threadIdx.x == 0) s_fill = 0;
+            flushed = true;
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = nxt[j];
+    }
+    if (flushed) {
+        agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, split_keys, split_counts, split_n, split_cap);
+        if (threadIdx.x == 0) atomicAdd(flushed_sources, 1ull);
+    } else {
+        agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, out_keys, out_counts, out_n, out_cap);
+    }
+}
+
 }  // namespace
 
 // Sort-reduce of the key log: radix sort of the n logged slots on the key
 // bits plus one (the sentinel bit), then run-length encoding; the runs are
 // the (source, vertex) pairs in order with their visit counts.  The trailing
 // sentinel run (slab padding), if any, is dropped.
+static int ensure_runs(gw_ctx* ctx, uint64_t n) {
+    if (n <= ctx->runs_cap) return GW_OK;
+    gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
+    gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
+    gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
+    ctx->d_keys_alt = ctx->d_runs = nullptr;
+    ctx->d_run_counts = nullptr;
+    ctx->runs_cap = 0;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_keys_alt, n * 8, "key sort")) ||
+        (rc = gw_alloc(ctx, (void**)&ctx->d_runs, n * 8, "key runs")) ||
+        (rc = gw_alloc(ctx, (void**)&ctx->d_run_counts, n * 4, "key run counts")))
+        return rc;
+    ctx->runs_cap = n;
+    return GW_OK;
+}
+
+static int ensure_sort_temp(gw_ctx* ctx, size_t need) {
+    if (need <= ctx->sort_temp_bytes) return GW_OK;
+    gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);
+    ctx->d_sort_temp = nullptr;
+    ctx->sort_temp_bytes = 0;
+    int rc = gw_alloc(ctx, &ctx->d_sort_temp, need, "key sort temp");
+    if (rc) return rc;
+    ctx->sort_temp_bytes = need;
+    return GW_OK;
+}
+
 int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms) {
     const int end_bit = static_cast<int>(64 - ctx->table_cbits) + 1;
-    if (n > ctx->runs_cap) {
-        gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
-        gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
-        gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
-        ctx->d_keys_alt = ctx->d_runs = nullptr;
-        ctx->d_run_counts = nullptr;
-        ctx->runs_cap = 0;
-        int rc;
-        if ((rc = gw_alloc(ctx, (void**)&ctx->d_keys_alt, n * 8, "key sort")) ||
-            (rc = gw_alloc(ctx, (void**)&ctx->d_runs, n * 8, "key runs")) ||
-            (rc = gw_alloc(ctx, (void**)&ctx->d_run_counts, n * 4, "key run counts")))
-            return rc;
-        ctx->runs_cap = n;
-    }
+    int rc0 = ensure_runs(ctx, n);
+    if (rc0) return rc0;
     size_t need_sort = 0, need_rle = 0;
     GW_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, n, 0, end_bit,
                                                   ctx->stream));
     GW_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(nullptr, need_rle, ctx->d_keys_alt, ctx->d_runs,
-                                                      ctx->d_run_counts, ctx->d_counters + 14, n, ctx->stream));
-    const size_t need = std::max(need_sort, need_rle);
-    if (need > ctx->sort_temp_bytes) {
-        gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);
-        ctx->d_sort_temp = nullptr;
-        ctx->sort_temp_bytes = 0;
-        int rc = gw_alloc(ctx, &ctx->d_sort_temp, need, "key sort temp");
-        if (rc) return rc;
-        ctx->sort_temp_bytes = need;
-    }
+                                                      ctx->d_run_counts, ctx->d_counters + kCntRuns, n, ctx->stream));
+    if ((rc0 = ensure_sort_temp(ctx, std::max(need_sort, need_rle)))) return rc0;
     size_t tb = ctx->sort_temp_bytes;
     GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (n) {
@@ -97,13 +348,13 @@ int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms) {
                                                       ctx->stream));
         tb = ctx->sort_temp_bytes;
         GW_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(ctx->d_sort_temp, tb, ctx->d_keys_alt, ctx->d_runs,
-                                                          ctx->d_run_counts, ctx->d_counters + 14, n, ctx->stream));
+                                                          ctx->d_run_counts, ctx->d_counters + kCntRuns, n, ctx->stream));
     } else {
-        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + 14, 0, 8, ctx->stream));
+        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntRuns, 0, 8, ctx->stream));
     }
     GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     unsigned long long runs = 0;
-    GW_HIP(ctx, hipMemcpyAsync(&runs, ctx->d_counters + 14, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipMemcpyAsync(&runs, ctx->d_counters + kCntRuns, 8, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float t = 0.f;
     GW_HIP(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
@@ -113,7 +364,165 @@ int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms) {
         if (last == ctx->keys_sentinel) --runs;
     }
     ctx->n_runs = runs;
+    ctx->runs_sorted = true;
     if (ms) *ms = t;
+    return GW_OK;
+}
+
+static int ensure_pairs(gw_ctx* ctx, uint64_t out_need, uint64_t split_need) {
+    int rc;
+    if (out_need > ctx->runs_cap && (rc = ensure_runs(ctx, out_need))) return rc;
+    if (split_need > ctx->split_cap) {
+        gw_free(ctx, ctx->d_split_keys, ctx->split_cap * 8);
+        gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
+        ctx->d_split_keys = nullptr;
+        ctx->d_split_counts = nullptr;
+        ctx->split_cap = 0;
+        if ((rc = gw_alloc(ctx, (void**)&ctx->d_split_keys, split_need * 8, "split pairs")) ||
+            (rc = gw_alloc(ctx, (void**)&ctx->d_split_counts, split_need * 8, "split counts")))
+            return rc;
+        ctx->split_cap = split_need;
+    }
+    return GW_OK;
+}
+
+int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
+    *ok = false;
+    const uint32_t nf = static_cast<uint32_t>(ctx->field_walks.size());
+    if (nf == 0) return GW_OK;
+    int rc;
+    // Regions sized for per_walk visits of every walk of a field.
+    if (ctx->part_per_walk != per_walk || ctx->part_fields != nf) {
+        std::vector<uint64_t> off(nf + 1ull, 0);
+        for (uint32_t f = 0; f < nf; ++f) off[f + 1] = off[f] + ctx->field_walks[f] * per_walk;
+        if (nf > ctx->part_fields_cap) {
+            gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
+            gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
+            ctx->d_part_off = nullptr;
+            ctx->d_part_cursor = nullptr;
+            ctx->part_fields_cap = 0;
+            if ((rc = gw_alloc(ctx, (void**)&ctx->d_part_off, (nf + 1ull) * 8, "source regions")) ||
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_cursor, nf * 8ull, "source region cursors")))
+                return rc;
+            ctx->part_fields_cap = nf;
+        }
+        const uint64_t need = std::max<uint64_t>(1, off[nf]);
+        if (need > ctx->part_keys_cap) {
+            gw_free(ctx, ctx->d_part_keys, ctx->part_keys_cap * 4);
+            ctx->d_part_keys = nullptr;
+            ctx->part_keys_cap = 0;
+            if (gw_alloc(ctx, (void**)&ctx->d_part_keys, need * 4, "partitioned key log")) {
+                ctx->err.clear();
+                return GW_OK;  // does not fit: the global sort-reduce runs instead
+            }
+            ctx->part_keys_cap = need;
+        }
+        GW_HIP(ctx, hipMemcpy(ctx->d_part_off, off.data(), (nf + 1ull) * 8, hipMemcpyHostToDevice));
+        ctx->part_per_walk = per_walk;
+        ctx->part_fields = nf;
+        ctx->part_total = off[nf];
+    }
+    // Pair buffers: a first guess of four pairs per walk (at most one per
+    // visit); a run that needs more grows them and aggregates again.
+    const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(1, ctx->part_total), std::max<uint64_t>(ctx->nwalks * 4, 1 << 16));
+    if ((rc = ensure_pairs(ctx, std::max(guess, ctx->runs_cap), std::max(guess / 4, ctx->split_cap)))) return rc;
+    *ok = true;
+    return GW_OK;
+}
+
+int gw_partlog_reset(gw_ctx* ctx, bool part) {
+    const uint32_t nf = part ? ctx->part_fields : 0u;
+    const uint32_t n = std::max<uint32_t>(kCounters, nf);
+    reset_counters_kernel<<<dim3((n + 255) / 256), dim3(256), 0, ctx->stream>>>(ctx->d_counters, kCounters,
+                                                                                ctx->d_part_cursor, nf);
+    GW_HIP(ctx, hipGetLastError());
+    return GW_OK;
+}
+
+static int launch_aggregate(gw_ctx* ctx) {
+    unsigned long long* c = ctx->d_counters;
+    part_aggregate_kernel<<<dim3(ctx->part_fields), dim3(kAggThreads), 0, ctx->stream>>>(
+        ctx->d_part_keys, ctx->d_part_off, ctx->d_part_cursor, ctx->table_vbits, ctx->d_runs, ctx->d_run_counts,
+        c + kCntOutPairs, ctx->runs_cap, ctx->d_split_keys, ctx->d_split_counts, c + kCntSplitPairs, ctx->split_cap,
+        c + kCntFlushed);
+    GW_HIP(ctx, hipGetLastError());
+    return GW_OK;
+}
+
+int gw_partlog_enqueue(gw_ctx* ctx) {
+    unsigned long long* c = ctx->d_counters;
+    const uint32_t grid = std::max(1u, ctx->cu_count) * 8u;
+    partition_kernel<<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+        ctx->d_keys, c + kCntLogCursor, ctx->keys_cap, ctx->keys_sentinel, ctx->table_vbits, ctx->part_fields,
+        ctx->d_part_off, ctx->d_part_cursor, ctx->d_part_keys, c + kCntPartOverflow);
+    GW_HIP(ctx, hipGetLastError());
+    return launch_aggregate(ctx);
+}
+
+int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, double* ms) {
+    *done = false;
+    *ms = 0;
+    if (cnt[kCntPartOverflow]) return GW_OK;  // a region overflowed: discard
+    uint64_t runs = cnt[kCntOutPairs];
+    uint64_t split = cnt[kCntSplitPairs];
+    uint64_t flushed = cnt[kCntFlushed];
+    int rc;
+    float t = 0.f;
+    // the split sums are appended after the exact pairs, so the pair buffer
+    // must hold runs + split
+    if (runs + split > ctx->runs_cap || split > ctx->split_cap) {
+        // The pair buffers were too small: grow them to what the run needs
+        // (the counters hold the exact totals) and aggregate again; the
+        // partitioned regions are still intact.
+        if ((rc = ensure_pairs(ctx, std::max(runs + split, ctx->runs_cap), std::max(split, ctx->split_cap)))) return rc;
+        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntOutPairs, 0, 3 * 8, ctx->stream));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        if ((rc = launch_aggregate(ctx))) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        unsigned long long c3[3];
+        GW_HIP(ctx, hipMemcpyAsync(c3, ctx->d_counters + kCntOutPairs, sizeof(c3), hipMemcpyDeviceToHost, ctx->stream));
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GW_HIP(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+        *ms += t;
+        runs = c3[0];
+        split = c3[1];
+        flushed = c3[2];
+    }
+    if (split) {
+        // Flushed sources: sum their partial pairs exactly by sort-reduce and
+        // append the sums.  The key log is consumed, so it takes the sorted keys.
+        if (split > ctx->keys_cap) return gw_fail(ctx, GW_ESTATE, "gw_run: split pairs exceed the key log");
+        const int end_bit = static_cast<int>(64 - ctx->table_cbits);
+        uint32_t* cnt_in = ctx->d_split_counts;
+        uint32_t* cnt_sorted = ctx->d_split_counts + ctx->split_cap;
+        size_t need_sort = 0, need_red = 0;
+        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, ctx->d_split_keys, ctx->d_keys, cnt_in,
+                                                       cnt_sorted, split, 0, end_bit, ctx->stream));
+        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(nullptr, need_red, ctx->d_keys, ctx->d_runs + runs, cnt_sorted,
+                                                      ctx->d_run_counts + runs, ctx->d_counters + kCntSplitRuns,
+                                                      hipcub::Sum(), split, ctx->stream));
+        if ((rc = ensure_sort_temp(ctx, std::max(need_sort, need_red)))) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        size_t tb = ctx->sort_temp_bytes;
+        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_split_keys, ctx->d_keys, cnt_in,
+                                                       cnt_sorted, split, 0, end_bit, ctx->stream));
+        tb = ctx->sort_temp_bytes;
+        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_runs + runs,
+                                                      cnt_sorted, ctx->d_run_counts + runs,
+                                                      ctx->d_counters + kCntSplitRuns, hipcub::Sum(), split,
+                                                      ctx->stream));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        unsigned long long extra = 0;
+        GW_HIP(ctx, hipMemcpyAsync(&extra, ctx->d_counters + kCntSplitRuns, 8, hipMemcpyDeviceToHost, ctx->stream));
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GW_HIP(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+        *ms += t;
+        runs += extra;
+    }
+    ctx->n_runs = runs;
+    ctx->runs_sorted = false;
+    ctx->part_flushed = flushed;
+    *done = true;
     return GW_OK;
 }
 
@@ -121,13 +530,42 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
                                      uint64_t* n_out) {
     if (!ctx || !n_out) return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: bad arguments");
     GW_HIP(ctx, hipSetDevice(ctx->device));
-    if (ctx->src_method == GW_SRC_SORT) {
+    if (ctx->src_method != GW_SRC_HASH) {
         const uint64_t n = ctx->n_runs;
         *n_out = n;
         if (!source && !vertex && !count) return GW_OK;
         if (!source || !vertex || !count || cap < n)
             return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: output arrays missing or smaller than the pair count");
         if (n == 0) return GW_OK;
+        if (!ctx->runs_sorted) {
+            // Partitioned aggregation leaves the pairs unordered: sort them by
+            // key (unique keys) once, through temporaries.
+            const int end_bit = static_cast<int>(64 - ctx->table_cbits);
+            unsigned long long* k2 = nullptr;
+            uint32_t* c2 = nullptr;
+            int rs;
+            if ((rs = gw_alloc(ctx, (void**)&k2, n * 8, "pair sort keys"))) return rs;
+            if ((rs = gw_alloc(ctx, (void**)&c2, n * 4, "pair sort counts"))) { gw_free(ctx, k2, n * 8); return rs; }
+            size_t need = 0;
+            hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, ctx->d_runs, k2, ctx->d_run_counts, c2, n,
+                                                              0, end_bit, ctx->stream);
+            if (e == hipSuccess && (rs = ensure_sort_temp(ctx, need))) {
+                gw_free(ctx, k2, n * 8);
+                gw_free(ctx, c2, n * 4);
+                return rs;
+            }
+            size_t tb = ctx->sort_temp_bytes;
+            if (e == hipSuccess)
+                e = hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_runs, k2, ctx->d_run_counts, c2, n,
+                                                       0, end_bit, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_runs, k2, n * 8, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_run_counts, c2, n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            gw_free(ctx, k2, n * 8);
+            gw_free(ctx, c2, n * 4);
+            if (e != hipSuccess) return gw_hip_check(ctx, e, "pair sort");
+            ctx->runs_sorted = true;
+        }
         uint32_t* d_out = nullptr;
         int rc = gw_alloc(ctx, (void**)&d_out, n * 12, "runs unpack");
         if (rc) return rc;
@@ -156,14 +594,14 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
         gw_free(ctx, temp, temp_bytes);
         gw_free(ctx, d_out, slots * 12);
     };
-    hipError_t e = hipMemsetAsync(ctx->d_counters + 11, 0, 8, ctx->stream);
+    hipError_t e = hipMemsetAsync(ctx->d_counters + kCntCompact, 0, 8, ctx->stream);
     if (e == hipSuccess) {
         compact_kernel<<<dim3(static_cast<uint32_t>((slots + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_table, slots, a, ctx->d_counters + 11);
+            ctx->d_table, slots, a, ctx->d_counters + kCntCompact);
         e = hipGetLastError();
     }
     unsigned long long n = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&n, ctx->d_counters + 11, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, ctx->d_counters + kCntCompact, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { release(); return gw_hip_check(ctx, e, "table compaction"); }
     *n_out = n;

@@ -109,12 +109,17 @@ typedef struct gw_program {
     uint32_t layout;          /* GW_LAYOUT_*: graph layout the step kernel reads */
     uint32_t event_cap;       /* GW_REC_EVENTS: visits stored per walk (more are counted, not stored) */
     uint32_t table_log2;      /* GW_REC_SOURCE_COUNT with GW_SRC_HASH: log2 of hash slots, 0 = auto */
-    uint32_t source_method;   /* GW_REC_SOURCE_COUNT: GW_SRC_SORT or GW_SRC_HASH */
+    uint32_t source_method;   /* GW_REC_SOURCE_COUNT: GW_SRC_* */
 } gw_program;
 
-/* how GW_REC_SOURCE_COUNT aggregates (source, vertex) visits */
-#define GW_SRC_SORT 0u  /* append keys to wave-private slabs, then radix sort + run-length encode */
-#define GW_SRC_HASH 1u  /* device-wide open-addressing hash, one or two atomics per visit */
+/* How GW_REC_SOURCE_COUNT aggregates (source, vertex) visits.  The sort-based
+ * methods append keys to wave-private slabs of a key log during the run. */
+#define GW_SRC_SORT 0u         /* GW_SRC_PARTITIONED when the source fields allow it, else GW_SRC_GLOBAL */
+#define GW_SRC_HASH 1u         /* device-wide open-addressing hash, one or two atomics per visit */
+#define GW_SRC_GLOBAL 2u       /* radix sort of the whole key log + run-length encode */
+#define GW_SRC_PARTITIONED 3u  /* scatter the log into one region per source field (fields < 2^20),
+                                  count each region in an LDS hash table; exact sort-reduce for
+                                  sources whose distinct vertices outgrow the table */
 
 /* graph layouts the step kernels read (the CSR is always resident) */
 #define GW_LAYOUT_AUTO 0u
@@ -131,6 +136,9 @@ typedef struct gw_stats {
     double bucket_ms;      /* bucketing kernels (BLOCKED) */
     uint32_t schedule;     /* schedule actually used */
     uint32_t layout;       /* layout actually used */
+    double reduce_ms;      /* source-count aggregation after the step kernels (sort-based methods) */
+    uint32_t source_method;    /* GW_SRC_* actually used (GW_REC_SOURCE_COUNT) */
+    uint32_t flushed_sources;  /* GW_SRC_PARTITIONED: sources that went through the exact split sort */
 } gw_stats;
 
 int gw_create(gw_ctx **out, const gw_cfg *cfg);

@@ -226,7 +226,10 @@ def _expected_source_counts(prog, off, col, sources, field, wps):
     return s[order], v[order], c[order]
 
 
-@pytest.mark.parametrize("method", [gw.SRC_SORT, gw.SRC_HASH])
+SRC_METHODS = [gw.SRC_SORT, gw.SRC_HASH, gw.SRC_GLOBAL, gw.SRC_PARTITIONED]
+
+
+@pytest.mark.parametrize("method", SRC_METHODS, ids=["sort", "hash", "global", "partitioned"])
 @pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("persistent", "csr"), ("blocked", "ext")])
 @pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART])
 def test_source_counts_bit_exact(engine, small_graph, kind, variant, method):
@@ -247,6 +250,61 @@ def test_source_counts_bit_exact(engine, small_graph, kind, variant, method):
     es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
     assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
     assert int(c.sum()) == st["visits"]
+    used = {gw.SRC_SORT: gw.SRC_PARTITIONED}.get(method, method)
+    assert st["source_method"] == used
+
+
+def test_source_counts_partitioned_flush_is_exact(engine):
+    # JUMP teleports uniformly, so each source's 1500 x 40 visits cover far more
+    # distinct vertices than one LDS table holds: the sources flush partial
+    # pairs and are summed by the split sort-reduce.
+    off, col = O.rmat_graph(16, 1 << 19, seed=4)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    sources = O.sources(9, n, 3)
+    field = np.array([0, 7, 2], np.uint32)
+    wps = 1500
+    engine.walks_init_sources(sources, wps, source_field=field)
+    prog = gw.program(gw.JUMP, 40, 0.15, n, seed=5, recorder=gw.REC_SOURCE_COUNT, source_method=gw.SRC_PARTITIONED)
+    st = engine.run(prog)
+    assert st["source_method"] == gw.SRC_PARTITIONED and st["flushed_sources"] == 3
+    s, v, c = engine.source_counts()
+    es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
+    assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
+    # a second run on the same context reuses the regions and gives the same pairs
+    st2 = engine.run(prog)
+    s2, v2, c2 = engine.source_counts()
+    assert st2["steps"] == st["steps"]
+    assert np.array_equal(s2, es) and np.array_equal(v2, ev) and np.array_equal(c2, ec)
+
+
+def test_source_counts_region_overflow_falls_back(engine):
+    # PROB walks are uncapped; regions are sized for 64 visits per walk.  Source
+    # field 0 starts on a sink-free cycle (about 128 visits per walk at p = 1/128),
+    # field 1 on a sink (one visit per walk): field 0's region overflows while the
+    # whole log does not.  GW_SRC_SORT falls back to the global sort-reduce;
+    # GW_SRC_PARTITIONED reports the overflow.
+    n = 200
+    deg = np.r_[np.ones(100, np.uint64), np.zeros(100, np.uint64)]
+    off = np.r_[np.zeros(1, np.uint64), np.cumsum(deg)].astype(np.uint64)
+    col = ((np.arange(100) + 1) % 100).astype(np.uint32)
+    engine.graph_set(off, col)
+    w = O.make_walks(np.r_[np.zeros(100, np.uint32), np.full(10_000, 150, np.uint32)],
+                     sources_=np.r_[np.zeros(100, np.uint32), np.ones(10_000, np.uint32)])
+    engine.walks_load(w)
+    prog = gw.program(gw.PROB, 0, 1.0 / 128, n, seed=21, recorder=gw.REC_SOURCE_COUNT, source_method=gw.SRC_SORT)
+    st = engine.run(prog)
+    assert st["source_method"] == gw.SRC_GLOBAL
+    assert st["visits"] > 100 * 64
+    got = engine.source_counts()
+    ref = O.replay(oprog(prog), off, col, w[:100])["counts"]
+    nz = np.nonzero(ref)[0]
+    assert np.array_equal(got[0][:nz.shape[0]], np.zeros(nz.shape[0], np.uint32))
+    assert np.array_equal(got[1][:nz.shape[0]], nz) and np.array_equal(got[2][:nz.shape[0]], ref[nz])
+    assert got[0].shape[0] == nz.shape[0] + 1 and got[1][-1] == 150 and got[2][-1] == 10_000
+    with pytest.raises(gw.GwError, match="outgrew"):
+        engine.run(gw.program(gw.PROB, 0, 1.0 / 128, n, seed=21, recorder=gw.REC_SOURCE_COUNT,
+                              source_method=gw.SRC_PARTITIONED))
 
 
 def test_source_table_overflow_is_an_error(engine, small_graph):
