@@ -218,9 +218,10 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);
     gw_free(ctx, ctx->d_part_keys, ctx->part_keys_cap * 4);
+    gw_free(ctx, ctx->d_part_counts, ctx->part_keys_cap * 4);
     gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
     gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
-    gw_free(ctx, ctx->d_split_keys, ctx->split_cap * 8);
+    gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
     gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
     gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);

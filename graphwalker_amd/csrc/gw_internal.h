@@ -27,6 +27,7 @@ constexpr uint32_t kCntOutPairs = 16;         // partitioned: exact pairs emitte
 constexpr uint32_t kCntSplitPairs = 17;       // partitioned: partial pairs of flushed sources
 constexpr uint32_t kCntFlushed = 18;          // partitioned: flushed sources
 constexpr uint32_t kCntSplitRuns = 19;        // partitioned: sums of the split pairs
+constexpr uint32_t kCntAggQueue = 20;         // partitioned: next source for the aggregation workgroups
 
 struct gw_ctx {
     int device = 0;
@@ -75,22 +76,25 @@ struct gw_ctx {
     unsigned long long* d_runs = nullptr;    // (source, vertex) keys of the last source-count run
     uint32_t* d_run_counts = nullptr;
     uint64_t runs_cap = 0, n_runs = 0;
-    bool runs_sorted = true;                 // false: d_runs is in no particular order
-    // Walks per source-field value of the loaded walks (empty when a field is
-    // kPartFieldLimit or more): sizes the per-source regions of the
-    // partitioned key log.
+    // Partitioned source counts.  Walks per source-field value of the loaded
+    // walks (empty when a field is kPartFieldLimit or more) size one region
+    // of the partitioned key log per field.
     std::vector<uint64_t> field_walks;
     uint32_t part_per_walk = 0;              // visits per walk the uploaded regions are sized for (0 = stale)
-    uint32_t* d_part_keys = nullptr;         // vertices, grouped by source field
+    uint32_t part_fields = 0, part_fields_cap = 0;
+    uint64_t part_total = 0;                 // region slots in all (sum of the caps)
+    uint32_t* d_part_keys = nullptr;         // regions of vertex ids; after aggregation an exact source's
+                                             // region starts with its distinct vertices
+    uint32_t* d_part_counts = nullptr;       // visit count beside each distinct vertex
     uint64_t part_keys_cap = 0;
     uint64_t* d_part_off = nullptr;          // region start per field (fields + 1)
     unsigned long long* d_part_cursor = nullptr;  // keys written per field
-    uint32_t part_fields = 0, part_fields_cap = 0;
-    unsigned long long* d_split_keys = nullptr;  // partial pairs of sources that outgrew the LDS table
-    uint32_t* d_split_counts = nullptr;          // 2 x split_cap (input, sorted)
+    unsigned long long* d_part_pairs = nullptr;   // distinct vertices per field (0 for a flushed source)
+    uint32_t* d_split_counts = nullptr;      // 2 x split_cap: counts of the split pairs (their keys go to the key log)
     uint64_t split_cap = 0;
-    uint64_t part_total = 0;                 // region slots in all (sum of the caps)
-    uint64_t part_flushed = 0;               // sources whose pairs went through the split sort (last run)
+    uint64_t part_pairs = 0;                 // last run: pairs held in the regions
+    uint64_t split_runs = 0;                 // last run: summed split pairs, in d_runs / d_run_counts
+    uint64_t part_flushed = 0;               // last run: sources that went through the split sort
 
     // device counters (kCounters of them, see the kCnt* indices)
     unsigned long long* d_counters = nullptr;

@@ -63,7 +63,7 @@ __global__ void unpack_runs_kernel(const unsigned long long* __restrict__ keys, 
 
 
 constexpr uint32_t kAggLog2 = 14;
-constexpr uint32_t kAggSlots = 1u << kAggLog2;               // LDS hash slots per workgroup (128 KiB)
+constexpr uint32_t kAggSlots = 1u << kAggLog2;               // LDS hash slots per workgroup (key + count: 128 KiB)
 constexpr uint32_t kAggThreads = 1024;
 constexpr uint32_t kAggWaves = kAggThreads / 64;
 constexpr uint32_t kAggPerThread = 4;
@@ -161,22 +161,26 @@ __global__ __launch_bounds__(256) void partition_kernel(const unsigned long long
     }
 }
 
-// Emit every occupied LDS slot as ((source << vbits) | vertex, count) to
-// out[base..] and clear the table, with one global atomic per workgroup: each
-// wavefront counts the occupied slots of its contiguous share of the table,
-// the counts are scanned in LDS, then every wavefront writes its share in
-// slot order.  The atomic reserves the pairs even when they do not fit
-// `cap`; then nothing is written and the host, which reads the total, grows
-// the buffer and runs the aggregation again.  All threads of the workgroup.
-__device__ void agg_emit(uint32_t* s_key, uint32_t* s_cnt, uint32_t* s_wave, unsigned long long* s_base,
-                         uint64_t fbase, unsigned long long* __restrict__ out_keys, uint32_t* __restrict__ out_counts,
-                         unsigned long long* __restrict__ out_n, uint64_t cap) {
-    constexpr uint32_t kShare = kAggSlots / kAggWaves;
+// Table size for a region of n keys: the smallest power of two with n at
+// most 3/4 of it (so a region that fits never flushes), within
+// [2^10, kAggSlots].
+__device__ __forceinline__ uint32_t agg_log2(uint64_t n) {
+    const uint64_t want = n + n / 3 + 1;
+    uint32_t lg = 10;
+    while (lg < kAggLog2 && (1ull << lg) < want) ++lg;
+    return lg;
+}
+
+// Occupied slots of each wavefront's contiguous share of the first `tslots`
+// slots, exclusive-scanned into s_wave[0..kAggWaves); s_wave[kAggWaves] = total.
+// All threads of the workgroup; ends synchronised.
+__device__ __forceinline__ void agg_count(const uint32_t* s_key, uint32_t tslots, uint32_t* s_wave) {
+    const uint32_t share = tslots / kAggWaves;
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lo = wave * kShare;
     uint32_t c = 0;
-    for (uint32_t i = lo + lane; i < lo + kShare; i += 64) c += __popcll(__ballot(s_key[i] != kEmptyKey));
+    for (uint32_t i = wave * share + lane; i < (wave + 1) * share; i += 64)
+        c += __popcll(__ballot(s_key[i] != kEmptyKey));
     if (lane == 0) s_wave[wave] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -186,114 +190,245 @@ __device__ void agg_emit(uint32_t* s_key, uint32_t* s_cnt, uint32_t* s_wave, uns
             s_wave[w] = total;
             total += x;
         }
-        unsigned long long b = total ? atomicAdd(out_n, static_cast<unsigned long long>(total)) : 0ull;
-        s_wave[kAggWaves] = (b + total <= cap) ? 1u : 0u;
-        *s_base = b;
+        s_wave[kAggWaves] = total;
     }
     __syncthreads();
-    const bool fits = s_wave[kAggWaves] != 0;
-    unsigned long long at = *s_base + s_wave[wave];
-    for (uint32_t i = lo + lane; i < lo + kShare; i += 64) {
+}
+
+// Write the occupied slots of the table (first `tslots`) in slot order from
+// position `at0` on, then clear them.  kSplit: as 64-bit keys
+// (fbase | vertex) + counts into the split list (only if `fits`); else as
+// vertex + count into the source's own region.  Ends synchronised.
+template <bool kSplit>
+__device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, const uint32_t* s_wave,
+                                          uint64_t at0, bool fits, uint64_t fbase, uint32_t* __restrict__ seg_v,
+                                          uint32_t* __restrict__ seg_c, unsigned long long* __restrict__ split_k,
+                                          uint32_t* __restrict__ split_c) {
+    const uint32_t share = tslots / kAggWaves;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    uint64_t at = at0 + s_wave[wave];
+    for (uint32_t i = wave * share + lane; i < (wave + 1) * share; i += 64) {
         const uint32_t k = s_key[i];
         const bool full = k != kEmptyKey;
         const uint64_t mask = __ballot(full);
         if (full && fits) {
             const uint64_t pos = at + rank_in(mask);
-            out_keys[pos] = fbase | k;
-            out_counts[pos] = s_cnt[i];
+            const uint32_t c = s_cnt[i];
+            if (kSplit) {
+                split_k[pos] = fbase | k;
+                split_c[pos] = c;
+            } else {
+                seg_v[pos] = k;
+                seg_c[pos] = c;
+            }
         }
         at += __popcll(mask);
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
-    __syncthreads();  // s_wave / s_base are reused by the next emit
+    __syncthreads();
 }
 
-// One workgroup per source field: count the region's vertex ids in an LDS
-// open-addressing table (linear probing, multiplicative hash).  Keys are
-// loaded kAggTile at a time, the next tile in flight while the current one is
-// inserted.  Before a tile would lift the fill above kAggMaxFill (3/4 of the
-// slots) the table is flushed to the split list, so every probe sequence
-// ends.  Sources that never flushed emit their exact pairs to `out`; the
-// others emit partial pairs to `split`, summed later by sort-reduce.
+// Flush the table to the split list: one returning atomic reserves the
+// pairs (counted even when they do not fit `cap`; the host checks).
+__device__ __forceinline__ void agg_flush(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t* s_wave,
+                                          unsigned long long* s_base, uint64_t fbase,
+                                          unsigned long long* __restrict__ split_k, uint32_t* __restrict__ split_c,
+                                          unsigned long long* __restrict__ split_n, uint64_t cap) {
+    agg_count(s_key, tslots, s_wave);
+    if (threadIdx.x == 0) {
+        const uint32_t total = s_wave[kAggWaves];
+        *s_base = total ? atomicAdd(split_n, static_cast<unsigned long long>(total)) : 0ull;
+    }
+    __syncthreads();
+    const uint64_t b = *s_base;
+    agg_write<true>(s_key, s_cnt, tslots, s_wave, b, b + s_wave[kAggWaves] <= cap, fbase, nullptr, nullptr, split_k,
+                    split_c);
+}
+
+__device__ __forceinline__ void region_meta(const uint64_t* __restrict__ region_off,
+                                            const unsigned long long* __restrict__ region_n, uint32_t nf, uint32_t f,
+                                            uint64_t& lo, uint64_t& n) {
+    lo = 0;
+    n = 0;
+    if (f < nf) {
+        lo = region_off[f];
+        // clamp to the region: an overflowed partition is discarded by the
+        // host, but must still be read in bounds
+        n = std::min<uint64_t>(region_n[f], region_off[f + 1] - lo);
+    }
+}
+
+__device__ __forceinline__ void load_tile(const uint32_t* __restrict__ keys, uint64_t lo, uint64_t n, uint64_t first,
+                                          uint32_t (&t)[kAggPerThread]) {
+#pragma unroll
+    for (uint32_t j = 0; j < kAggPerThread; ++j) {
+        const uint64_t idx = first + j * kAggThreads + threadIdx.x;
+        t[j] = idx < n ? keys[lo + idx] : kEmptyKey;
+    }
+}
+
+// Persistent workgroups, one per CU, each counting one source at a time in
+// an LDS open-addressing table (linear probing, multiplicative hash) sized to
+// the source's key count.  Keys are
+// loaded kAggTile at a time, the next tile -- or the next source's first tile
+// -- in flight while the current one is inserted.  Runs of one vertex across
+// a wavefront are combined before the LDS atomics.  Before a tile would lift
+// the fill above 3/4 of the table it is flushed to the split list, so every
+// probe sequence ends.  A source that never flushed writes its distinct
+// vertices and counts over the start of its own region (its keys are all in
+// registers or consumed by then) and its pair count to pairs[f], with no
+// atomic on the critical path; a flushed source emits partial pairs to the
+// split list, summed later by sort-reduce, and pairs[f] = 0.
 __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
-    const uint32_t* __restrict__ keys, const uint64_t* __restrict__ region_off,
-    const unsigned long long* __restrict__ region_n, uint32_t vbits, unsigned long long* __restrict__ out_keys,
-    uint32_t* __restrict__ out_counts, unsigned long long* __restrict__ out_n, uint64_t out_cap,
-    unsigned long long* __restrict__ split_keys, uint32_t* __restrict__ split_counts,
-    unsigned long long* __restrict__ split_n, uint64_t split_cap, unsigned long long* __restrict__ flushed_sources) {
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ counts, unsigned long long* __restrict__ pairs,
+    const uint64_t* __restrict__ region_off, const unsigned long long* __restrict__ region_n, uint32_t nf,
+    uint32_t vbits, unsigned long long* __restrict__ total_pairs, unsigned long long* __restrict__ split_k,
+    uint32_t* __restrict__ split_c, unsigned long long* __restrict__ split_n, uint64_t split_cap,
+    unsigned long long* __restrict__ flushed_sources, unsigned long long* __restrict__ queue) {
     __shared__ uint32_t s_key[kAggSlots];
     __shared__ uint32_t s_cnt[kAggSlots];
     __shared__ uint32_t s_fill;
     __shared__ uint32_t s_wave[kAggWaves + 1];
     __shared__ unsigned long long s_base;
-    const uint32_t f = blockIdx.x;
-    const uint64_t lo = region_off[f];
-    // clamp to the region: an overflowed partition is discarded by the host,
-    // but must still be read in bounds
-    const uint64_t n = std::min<uint64_t>(region_n[f], region_off[f + 1] - lo);
-    if (n == 0) return;
-    const uint32_t* __restrict__ k = keys + lo;
-    const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
-    uint32_t cur[kAggPerThread], nxt[kAggPerThread];
-#pragma unroll
-    for (uint32_t j = 0; j < kAggPerThread; ++j) {
-        const uint64_t idx = j * kAggThreads + threadIdx.x;
-        cur[j] = idx < n ? k[idx] : kEmptyKey;
-    }
-    // clear the table while the first tile is in flight
+    __shared__ uint32_t s_next;
+    const uint32_t G = gridDim.x;
+    const uint32_t lane = __lane_id();
+    // Sources 0..G-1 go one per workgroup; the others are taken from a queue
+    // (one atomic per source, issued a source ahead), so workgroups that draw
+    // light sources take more of them.
+    uint32_t f = blockIdx.x;
+    if (threadIdx.x == 0) s_next = G + static_cast<uint32_t>(atomicAdd(queue, 1ull));
     for (uint32_t i = threadIdx.x; i < kAggSlots; i += kAggThreads) {
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
     if (threadIdx.x == 0) s_fill = 0;
     __syncthreads();
-    bool flushed = false;
-    for (uint64_t base = 0; base < n; base += kAggTile) {
-        const uint64_t next = base + kAggTile;  // start of the next tile
-        if (next < n) {
+    uint32_t fn = s_next;
+    uint64_t lo, n, nlo, nn;
+    region_meta(region_off, region_n, nf, f, lo, n);
+    region_meta(region_off, region_n, nf, fn, nlo, nn);
+    uint32_t cur[kAggPerThread], pre[kAggPerThread];
+    load_tile(keys, lo, n, 0, cur);
+    while (f < nf) {
+        unsigned long long ticket = 0;
+        if (threadIdx.x == 0) ticket = atomicAdd(queue, 1ull);  // consumed at the end of this source
+        if (n == 0) {
+            if (threadIdx.x == 0) pairs[f] = 0;
+            load_tile(keys, nlo, nn, 0, pre);
+        } else {
+            const uint32_t lg = agg_log2(n);
+            const uint32_t tslots = 1u << lg;
+            const uint32_t max_fill = tslots / 4 * 3;
+            const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
+            bool flushed = false;
+            for (uint64_t base = 0;; base += kAggTile) {
+                const uint64_t next = base + kAggTile;
+                if (next < n) load_tile(keys, lo, n, next, pre);
+                else load_tile(keys, nlo, nn, 0, pre);
+                // Runs of one vertex across the wavefront (a source's own vertex
+                // opens every one of its walks) are combined before touching
+                // LDS: the lowest lane holding the first valid lane's vertex
+                // adds the whole group.  Then linear probing, one key of the
+                // lane at a time.  (Measured alternatives that were slower:
+                // probing the lane's keys together, 4-slot buckets read with
+                // one 128-bit load, and 64-bit key|count slots read before
+                // the atomic.)
+                uint32_t added = 0;
 #pragma unroll
-            for (uint32_t j = 0; j < kAggPerThread; ++j) {
-                const uint64_t idx = next + j * kAggThreads + threadIdx.x;
-                nxt[j] = idx < n ? k[idx] : kEmptyKey;
-            }
-        }
-        uint32_t added = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kAggPerThread; ++j) {
-            const uint32_t v = cur[j];
-            if (v == kEmptyKey) continue;
-            uint32_t h = (v * 0x9E3779B1u) >> (32 - kAggLog2);
-            for (;;) {
-                const uint32_t old = atomicCAS(&s_key[h], kEmptyKey, v);
-                if (old == kEmptyKey || old == v) {
-                    atomicAdd(&s_cnt[h], 1u);
-                    added += old == kEmptyKey;
-                    break;
+                for (uint32_t j = 0; j < kAggPerThread; ++j) {
+                    uint32_t v = cur[j];
+                    uint32_t inc = 1;
+                    const uint64_t valid = __ballot(v != kEmptyKey);
+                    if (valid) {
+                        const uint32_t v0 =
+                            __shfl(v, static_cast<int>(__ffsll(static_cast<unsigned long long>(valid)) - 1), 64);
+                        const uint64_t same = __ballot(v == v0);
+                        if (v == v0) {
+                            if (rank_in(same) == 0) inc = __popcll(same);
+                            else v = kEmptyKey;
+                        }
+                    }
+                    if (v == kEmptyKey) continue;
+                    uint32_t h = (v * 0x9E3779B1u) >> (32 - lg);
+                    for (;;) {
+                        const uint32_t old = atomicCAS(&s_key[h], kEmptyKey, v);
+                        if (old == kEmptyKey || old == v) {
+                            atomicAdd(&s_cnt[h], inc);
+                            added += old == kEmptyKey;
+                            break;
+                        }
+                        h = (h + 1) & (tslots - 1);
+                    }
                 }
-                h = (h + 1) & (kAggSlots - 1);
-            }
-        }
-        if (added) atomicAdd(&s_fill, added);
-        __syncthreads();
-        const uint32_t fill = s_fill;
-        __syncthreads();  // every thread has read the fill before it is reset
-        if (next >= n) break;
-        const uint64_t next_keys = std::min<uint64_t>(kAggTile, n - next);
-        if (fill + next_keys > kAggMaxFill) {  // workgroup-uniform
-            agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, split_keys, split_counts, split_n, split_cap);
-            if (threadIdx.x == 0) s_fill = 0;
-            flushed = true;
-            __syncthreads();
-        }
+                added = gw::wave_sum(added);
+                if (lane == 0 && added) atomicAdd(&s_fill, added);
+                __syncthreads();
+                const uint32_t fill = s_fill;
+                __syncthreads();  // every thread has read the fill before it is reset
+                if (next >= n) break;
+                if (fill + std::min<uint64_t>(kAggTile, n - next) > max_fill) {  // workgroup-uniform
+                    agg_flush(s_key, s_cnt, tslots, s_wave, &s_base, fbase, split_k, split_c, split_n, split_cap);
+                    if (threadIdx.x == 0) s_fill = 0;
+                    flushed = true;
+                    __syncthreads();
+                }
 #pragma unroll
-        for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = nxt[j];
+                for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
+            }
+            if (flushed) {
+                agg_flush(s_key, s_cnt, tslots, s_wave, &s_base, fbase, split_k, split_c, split_n, split_cap);
+                if (threadIdx.x == 0) {
+                    pairs[f] = 0;
+                    atomicAdd(flushed_sources, 1ull);
+                }
+            } else {
+                agg_count(s_key, tslots, s_wave);
+                const uint32_t total = s_wave[kAggWaves];
+                agg_write<false>(s_key, s_cnt, tslots, s_wave, lo, true, 0, keys, counts, nullptr, nullptr);
+                if (threadIdx.x == 0) {
+                    pairs[f] = total;
+                    atomicAdd(total_pairs, static_cast<unsigned long long>(total));
+                }
+            }
+            if (threadIdx.x == 0) s_fill = 0;
+        }
+        __syncthreads();  // every thread has read the previous s_next
+        if (threadIdx.x == 0) s_next = G + static_cast<uint32_t>(ticket);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
+        f = fn;
+        fn = s_next;
+        lo = nlo;
+        n = nn;
+        region_meta(region_off, region_n, nf, fn, nlo, nn);
     }
-    if (flushed) {
-        agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, split_keys, split_counts, split_n, split_cap);
-        if (threadIdx.x == 0) atomicAdd(flushed_sources, 1ull);
-    } else {
-        agg_emit(s_key, s_cnt, s_wave, &s_base, fbase, out_keys, out_counts, out_n, out_cap);
+}
+
+// Read-out of the partitioned result: the pairs at the start of each source's
+// region are packed into (fbase | vertex, count) at a cursor, one workgroup
+// per source (order is restored by the sort that follows).
+__global__ __launch_bounds__(256) void gather_regions_kernel(const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ counts,
+                                                            const unsigned long long* __restrict__ pairs,
+                                                            const uint64_t* __restrict__ region_off, uint32_t vbits,
+                                                            unsigned long long* __restrict__ out_k,
+                                                            uint32_t* __restrict__ out_c,
+                                                            unsigned long long* __restrict__ cursor) {
+    __shared__ unsigned long long s_at;
+    const uint32_t f = blockIdx.x;
+    const uint64_t np = pairs[f];
+    if (np == 0) return;
+    if (threadIdx.x == 0) s_at = atomicAdd(cursor, static_cast<unsigned long long>(np));
+    __syncthreads();
+    const uint64_t lo = region_off[f], at = s_at;
+    const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
+    for (uint64_t i = threadIdx.x; i < np; i += blockDim.x) {
+        out_k[at + i] = fbase | keys[lo + i];
+        out_c[at + i] = counts[lo + i];
     }
 }
 
@@ -364,25 +499,7 @@ int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms) {
         if (last == ctx->keys_sentinel) --runs;
     }
     ctx->n_runs = runs;
-    ctx->runs_sorted = true;
     if (ms) *ms = t;
-    return GW_OK;
-}
-
-static int ensure_pairs(gw_ctx* ctx, uint64_t out_need, uint64_t split_need) {
-    int rc;
-    if (out_need > ctx->runs_cap && (rc = ensure_runs(ctx, out_need))) return rc;
-    if (split_need > ctx->split_cap) {
-        gw_free(ctx, ctx->d_split_keys, ctx->split_cap * 8);
-        gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
-        ctx->d_split_keys = nullptr;
-        ctx->d_split_counts = nullptr;
-        ctx->split_cap = 0;
-        if ((rc = gw_alloc(ctx, (void**)&ctx->d_split_keys, split_need * 8, "split pairs")) ||
-            (rc = gw_alloc(ctx, (void**)&ctx->d_split_counts, split_need * 8, "split counts")))
-            return rc;
-        ctx->split_cap = split_need;
-    }
     return GW_OK;
 }
 
@@ -391,27 +508,37 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
     const uint32_t nf = static_cast<uint32_t>(ctx->field_walks.size());
     if (nf == 0) return GW_OK;
     int rc;
-    // Regions sized for per_walk visits of every walk of a field.
+    // Regions sized for per_walk visits of every walk of a field.  Counts are
+    // 32-bit, so a region stays below 2^32 keys.
     if (ctx->part_per_walk != per_walk || ctx->part_fields != nf) {
         std::vector<uint64_t> off(nf + 1ull, 0);
-        for (uint32_t f = 0; f < nf; ++f) off[f + 1] = off[f] + ctx->field_walks[f] * per_walk;
+        for (uint32_t f = 0; f < nf; ++f) {
+            if (ctx->field_walks[f] * per_walk >= (1ull << 32)) return GW_OK;
+            off[f + 1] = off[f] + ctx->field_walks[f] * per_walk;
+        }
         if (nf > ctx->part_fields_cap) {
             gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
             gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
+            gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
             ctx->d_part_off = nullptr;
-            ctx->d_part_cursor = nullptr;
+            ctx->d_part_cursor = ctx->d_part_pairs = nullptr;
             ctx->part_fields_cap = 0;
             if ((rc = gw_alloc(ctx, (void**)&ctx->d_part_off, (nf + 1ull) * 8, "source regions")) ||
-                (rc = gw_alloc(ctx, (void**)&ctx->d_part_cursor, nf * 8ull, "source region cursors")))
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_cursor, nf * 8ull, "source region cursors")) ||
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_pairs, nf * 8ull, "source pair counts")))
                 return rc;
             ctx->part_fields_cap = nf;
         }
         const uint64_t need = std::max<uint64_t>(1, off[nf]);
         if (need > ctx->part_keys_cap) {
             gw_free(ctx, ctx->d_part_keys, ctx->part_keys_cap * 4);
-            ctx->d_part_keys = nullptr;
+            gw_free(ctx, ctx->d_part_counts, ctx->part_keys_cap * 4);
+            ctx->d_part_keys = ctx->d_part_counts = nullptr;
             ctx->part_keys_cap = 0;
-            if (gw_alloc(ctx, (void**)&ctx->d_part_keys, need * 4, "partitioned key log")) {
+            if (gw_alloc(ctx, (void**)&ctx->d_part_keys, need * 4, "partitioned key log") ||
+                gw_alloc(ctx, (void**)&ctx->d_part_counts, need * 4, "partitioned counts")) {
+                gw_free(ctx, ctx->d_part_keys, need * 4);
+                ctx->d_part_keys = nullptr;
                 ctx->err.clear();
                 return GW_OK;  // does not fit: the global sort-reduce runs instead
             }
@@ -422,10 +549,15 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
         ctx->part_fields = nf;
         ctx->part_total = off[nf];
     }
-    // Pair buffers: a first guess of four pairs per walk (at most one per
-    // visit); a run that needs more grows them and aggregates again.
-    const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(1, ctx->part_total), std::max<uint64_t>(ctx->nwalks * 4, 1 << 16));
-    if ((rc = ensure_pairs(ctx, std::max(guess, ctx->runs_cap), std::max(guess / 4, ctx->split_cap)))) return rc;
+    // Split pairs (flushed sources) go to the key log, which the partition has
+    // consumed by then, with their counts here: at most one pair per logged key.
+    if (ctx->keys_cap > ctx->split_cap) {
+        gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
+        ctx->d_split_counts = nullptr;
+        ctx->split_cap = 0;
+        if ((rc = gw_alloc(ctx, (void**)&ctx->d_split_counts, ctx->keys_cap * 8, "split counts"))) return rc;
+        ctx->split_cap = ctx->keys_cap;
+    }
     *ok = true;
     return GW_OK;
 }
@@ -439,90 +571,102 @@ int gw_partlog_reset(gw_ctx* ctx, bool part) {
     return GW_OK;
 }
 
-static int launch_aggregate(gw_ctx* ctx) {
-    unsigned long long* c = ctx->d_counters;
-    part_aggregate_kernel<<<dim3(ctx->part_fields), dim3(kAggThreads), 0, ctx->stream>>>(
-        ctx->d_part_keys, ctx->d_part_off, ctx->d_part_cursor, ctx->table_vbits, ctx->d_runs, ctx->d_run_counts,
-        c + kCntOutPairs, ctx->runs_cap, ctx->d_split_keys, ctx->d_split_counts, c + kCntSplitPairs, ctx->split_cap,
-        c + kCntFlushed);
-    GW_HIP(ctx, hipGetLastError());
-    return GW_OK;
-}
-
 int gw_partlog_enqueue(gw_ctx* ctx) {
     unsigned long long* c = ctx->d_counters;
-    const uint32_t grid = std::max(1u, ctx->cu_count) * 8u;
-    partition_kernel<<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+    const uint32_t cus = std::max(1u, ctx->cu_count);
+    partition_kernel<<<dim3(cus * 8u), dim3(256), 0, ctx->stream>>>(
         ctx->d_keys, c + kCntLogCursor, ctx->keys_cap, ctx->keys_sentinel, ctx->table_vbits, ctx->part_fields,
         ctx->d_part_off, ctx->d_part_cursor, ctx->d_part_keys, c + kCntPartOverflow);
     GW_HIP(ctx, hipGetLastError());
-    return launch_aggregate(ctx);
+    const uint32_t grid = std::min(ctx->part_fields, cus);
+    part_aggregate_kernel<<<dim3(grid), dim3(kAggThreads), 0, ctx->stream>>>(
+        ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->d_part_cursor,
+        ctx->part_fields, ctx->table_vbits, c + kCntOutPairs, ctx->d_keys, ctx->d_split_counts, c + kCntSplitPairs,
+        ctx->split_cap, c + kCntFlushed, c + kCntAggQueue);
+    GW_HIP(ctx, hipGetLastError());
+    return GW_OK;
 }
 
 int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, double* ms) {
     *done = false;
     *ms = 0;
     if (cnt[kCntPartOverflow]) return GW_OK;  // a region overflowed: discard
-    uint64_t runs = cnt[kCntOutPairs];
-    uint64_t split = cnt[kCntSplitPairs];
-    uint64_t flushed = cnt[kCntFlushed];
-    int rc;
-    float t = 0.f;
-    // the split sums are appended after the exact pairs, so the pair buffer
-    // must hold runs + split
-    if (runs + split > ctx->runs_cap || split > ctx->split_cap) {
-        // The pair buffers were too small: grow them to what the run needs
-        // (the counters hold the exact totals) and aggregate again; the
-        // partitioned regions are still intact.
-        if ((rc = ensure_pairs(ctx, std::max(runs + split, ctx->runs_cap), std::max(split, ctx->split_cap)))) return rc;
-        GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntOutPairs, 0, 3 * 8, ctx->stream));
-        GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        if ((rc = launch_aggregate(ctx))) return rc;
-        GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        unsigned long long c3[3];
-        GW_HIP(ctx, hipMemcpyAsync(c3, ctx->d_counters + kCntOutPairs, sizeof(c3), hipMemcpyDeviceToHost, ctx->stream));
-        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        GW_HIP(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
-        *ms += t;
-        runs = c3[0];
-        split = c3[1];
-        flushed = c3[2];
-    }
+    const uint64_t split = cnt[kCntSplitPairs];
+    if (split > ctx->split_cap) return gw_fail(ctx, GW_ESTATE, "gw_run: split pairs exceed the key log");
+    uint64_t sums = 0;
     if (split) {
-        // Flushed sources: sum their partial pairs exactly by sort-reduce and
-        // append the sums.  The key log is consumed, so it takes the sorted keys.
-        if (split > ctx->keys_cap) return gw_fail(ctx, GW_ESTATE, "gw_run: split pairs exceed the key log");
+        // Flushed sources: sum their partial pairs exactly by sort-reduce.
+        int rc;
+        if ((rc = ensure_runs(ctx, split))) return rc;
         const int end_bit = static_cast<int>(64 - ctx->table_cbits);
         uint32_t* cnt_in = ctx->d_split_counts;
         uint32_t* cnt_sorted = ctx->d_split_counts + ctx->split_cap;
         size_t need_sort = 0, need_red = 0;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, ctx->d_split_keys, ctx->d_keys, cnt_in,
+        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, cnt_in,
                                                        cnt_sorted, split, 0, end_bit, ctx->stream));
-        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(nullptr, need_red, ctx->d_keys, ctx->d_runs + runs, cnt_sorted,
-                                                      ctx->d_run_counts + runs, ctx->d_counters + kCntSplitRuns,
+        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(nullptr, need_red, ctx->d_keys_alt, ctx->d_runs, cnt_sorted,
+                                                      ctx->d_run_counts, ctx->d_counters + kCntSplitRuns,
                                                       hipcub::Sum(), split, ctx->stream));
         if ((rc = ensure_sort_temp(ctx, std::max(need_sort, need_red)))) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
         size_t tb = ctx->sort_temp_bytes;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_split_keys, ctx->d_keys, cnt_in,
+        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_keys_alt, cnt_in,
                                                        cnt_sorted, split, 0, end_bit, ctx->stream));
         tb = ctx->sort_temp_bytes;
-        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_runs + runs,
-                                                      cnt_sorted, ctx->d_run_counts + runs,
-                                                      ctx->d_counters + kCntSplitRuns, hipcub::Sum(), split,
-                                                      ctx->stream));
+        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(ctx->d_sort_temp, tb, ctx->d_keys_alt, ctx->d_runs, cnt_sorted,
+                                                      ctx->d_run_counts, ctx->d_counters + kCntSplitRuns,
+                                                      hipcub::Sum(), split, ctx->stream));
         GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        unsigned long long extra = 0;
-        GW_HIP(ctx, hipMemcpyAsync(&extra, ctx->d_counters + kCntSplitRuns, 8, hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long r = 0;
+        GW_HIP(ctx, hipMemcpyAsync(&r, ctx->d_counters + kCntSplitRuns, 8, hipMemcpyDeviceToHost, ctx->stream));
         GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        float t = 0.f;
         GW_HIP(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
-        *ms += t;
-        runs += extra;
+        *ms = t;
+        sums = r;
     }
-    ctx->n_runs = runs;
-    ctx->runs_sorted = false;
-    ctx->part_flushed = flushed;
+    ctx->part_pairs = cnt[kCntOutPairs];
+    ctx->split_runs = sums;
+    ctx->n_runs = ctx->part_pairs + sums;
+    ctx->part_flushed = cnt[kCntFlushed];
     *done = true;
+    return GW_OK;
+}
+
+// Packs the partitioned result -- pairs at the start of each source's region
+// plus the summed split pairs -- into (key, count) arrays sorted by key.
+static int partitioned_sorted(gw_ctx* ctx, unsigned long long** keys_out, uint32_t** counts_out, void** mem,
+                              uint64_t* mem_bytes) {
+    const uint64_t n = ctx->n_runs;
+    const uint64_t bytes = n * 24;  // two key arrays, two count arrays
+    int rc = gw_alloc(ctx, mem, bytes, "pair read-out");
+    if (rc) return rc;
+    *mem_bytes = bytes;
+    unsigned long long* k1 = static_cast<unsigned long long*>(*mem);
+    unsigned long long* k2 = k1 + n;
+    uint32_t* c1 = reinterpret_cast<uint32_t*>(k2 + n);
+    uint32_t* c2 = c1 + n;
+    GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntCompact, 0, 8, ctx->stream));
+    if (ctx->part_pairs) {
+        gather_regions_kernel<<<dim3(ctx->part_fields), dim3(256), 0, ctx->stream>>>(
+            ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->table_vbits, k1, c1,
+            ctx->d_counters + kCntCompact);
+        GW_HIP(ctx, hipGetLastError());
+    }
+    if (ctx->split_runs) {
+        GW_HIP(ctx, hipMemcpyAsync(k1 + ctx->part_pairs, ctx->d_runs, ctx->split_runs * 8, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+        GW_HIP(ctx, hipMemcpyAsync(c1 + ctx->part_pairs, ctx->d_run_counts, ctx->split_runs * 4,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    const int end_bit = static_cast<int>(64 - ctx->table_cbits);
+    size_t need = 0;
+    GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
+    if ((rc = ensure_sort_temp(ctx, need))) return rc;
+    size_t tb = ctx->sort_temp_bytes;
+    GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
+    *keys_out = k2;
+    *counts_out = c2;
     return GW_OK;
 }
 
@@ -537,46 +681,36 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
         if (!source || !vertex || !count || cap < n)
             return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: output arrays missing or smaller than the pair count");
         if (n == 0) return GW_OK;
-        if (!ctx->runs_sorted) {
-            // Partitioned aggregation leaves the pairs unordered: sort them by
-            // key (unique keys) once, through temporaries.
-            const int end_bit = static_cast<int>(64 - ctx->table_cbits);
-            unsigned long long* k2 = nullptr;
-            uint32_t* c2 = nullptr;
-            int rs;
-            if ((rs = gw_alloc(ctx, (void**)&k2, n * 8, "pair sort keys"))) return rs;
-            if ((rs = gw_alloc(ctx, (void**)&c2, n * 4, "pair sort counts"))) { gw_free(ctx, k2, n * 8); return rs; }
-            size_t need = 0;
-            hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, ctx->d_runs, k2, ctx->d_run_counts, c2, n,
-                                                              0, end_bit, ctx->stream);
-            if (e == hipSuccess && (rs = ensure_sort_temp(ctx, need))) {
-                gw_free(ctx, k2, n * 8);
-                gw_free(ctx, c2, n * 4);
-                return rs;
+        const unsigned long long* keys = ctx->d_runs;
+        const uint32_t* counts = ctx->d_run_counts;
+        void* mem = nullptr;
+        uint64_t mem_bytes = 0;
+        if (ctx->src_method == GW_SRC_PARTITIONED) {
+            unsigned long long* k = nullptr;
+            uint32_t* c = nullptr;
+            int rp = partitioned_sorted(ctx, &k, &c, &mem, &mem_bytes);
+            if (rp) {
+                gw_free(ctx, mem, mem_bytes);
+                return rp;
             }
-            size_t tb = ctx->sort_temp_bytes;
-            if (e == hipSuccess)
-                e = hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_runs, k2, ctx->d_run_counts, c2, n,
-                                                       0, end_bit, ctx->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_runs, k2, n * 8, hipMemcpyDeviceToDevice, ctx->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_run_counts, c2, n * 4, hipMemcpyDeviceToDevice, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            gw_free(ctx, k2, n * 8);
-            gw_free(ctx, c2, n * 4);
-            if (e != hipSuccess) return gw_hip_check(ctx, e, "pair sort");
-            ctx->runs_sorted = true;
+            keys = k;
+            counts = c;
         }
         uint32_t* d_out = nullptr;
         int rc = gw_alloc(ctx, (void**)&d_out, n * 12, "runs unpack");
-        if (rc) return rc;
+        if (rc) {
+            gw_free(ctx, mem, mem_bytes);
+            return rc;
+        }
         unpack_runs_kernel<<<dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_runs, ctx->d_run_counts, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
+            keys, counts, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(source, d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(vertex, d_out + n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(count, d_out + 2 * n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         gw_free(ctx, d_out, n * 12);
+        gw_free(ctx, mem, mem_bytes);
         if (e != hipSuccess) return gw_hip_check(ctx, e, "runs read-out");
         return GW_OK;
     }
