@@ -99,6 +99,10 @@ class Stats(C.Structure):
         ("reduce_ms", C.c_double),
         ("source_method", C.c_uint32),
         ("flushed_sources", C.c_uint32),
+        ("h2d_bytes", C.c_uint64),
+        ("blocks_loaded", C.c_uint32),
+        ("cache_slots", C.c_uint32),
+        ("copy_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -156,6 +160,9 @@ def lib() -> C.CDLL:
         "gw_last_error": ([P], C.c_char_p),
         "gw_device_info": ([P, C.c_char_p, C.c_size_t, P, P], i),
         "gw_graph_set": ([P, P, P, u32, u64], i),
+        "gw_graph_set_host": ([P, P, P, u32, u64, u64], i),
+        "gw_host_alloc": ([P, u64, P], i),
+        "gw_host_free": ([P, P], i),
         "gw_graph_rmat": ([P, u32, u64, u64, u32, u32, u32], i),
         "gw_graph_blocks": ([P, P, u32], i),
         "gw_graph_info": ([P, P, P, P], i),
@@ -206,11 +213,13 @@ class Engine:
         self.n = 0
         self.m = 0
         self.nwalks = 0
+        self._host_graph = None
 
     def close(self) -> None:
         if self._ctx:
-            self._L.gw_destroy(self._ctx)
+            self._L.gw_destroy(self._ctx)  # frees gw_host_alloc memory too
             self._ctx = C.c_void_p()
+            self._host_graph = None
 
     def __del__(self):
         try:
@@ -244,6 +253,27 @@ class Engine:
         n = off.shape[0] - 1
         self._check(self._L.gw_graph_set(self._ctx, _ptr(off), _ptr(col), n, col.shape[0]), "gw_graph_set")
         self.n, self.m = n, col.shape[0]
+
+    def host_empty(self, shape, dtype) -> np.ndarray:
+        """A numpy array in pinned host memory (gw_host_alloc), freed with the
+        engine.  A host-resident graph built in such arrays is used in place."""
+        dtype = np.dtype(dtype)
+        count = int(np.prod(shape))
+        ptr = C.c_void_p()
+        self._check(self._L.gw_host_alloc(self._ctx, max(1, count * dtype.itemsize), C.byref(ptr)), "gw_host_alloc")
+        buf = (C.c_char * max(1, count * dtype.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+
+    def graph_set_host(self, off: np.ndarray, col: np.ndarray, cache_bytes: int) -> None:
+        """Out-of-core graph: the CSR stays in host memory and runs step it
+        through a device block cache of at most cache_bytes."""
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        n = off.shape[0] - 1
+        self._check(self._L.gw_graph_set_host(self._ctx, _ptr(off), _ptr(col), n, col.shape[0], cache_bytes),
+                    "gw_graph_set_host")
+        self.n, self.m = n, col.shape[0]
+        self._host_graph = (off, col)  # the engine may use them in place
 
     def graph_rmat(self, scale: int, m: int, seed: int, a: float = 0.57, b: float = 0.19, c: float = 0.19) -> None:
         ta, tab, tabc = rmat_thresholds(a, b, c)

@@ -44,9 +44,8 @@ void gw_free(gw_ctx* ctx, void* p, uint64_t bytes) {
     ctx->hbm_used = ctx->hbm_used >= bytes ? ctx->hbm_used - bytes : 0;
 }
 
-namespace {
-
-void free_graph(gw_ctx* ctx) {
+void gw_graph_free(gw_ctx* ctx) {
+    gw_host_graph_free(ctx);
     gw_free(ctx, ctx->d_off, (ctx->n + 1ull) * 8);
     gw_free(ctx, ctx->d_col, ctx->m * 4);
     gw_free(ctx, ctx->d_counts, ctx->n * 4ull);
@@ -63,6 +62,8 @@ void free_graph(gw_ctx* ctx) {
     ctx->nblocks = 0;
     ctx->block_first.clear();
 }
+
+namespace {
 
 void free_walks(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_walks, ctx->walk_cap * sizeof(gw_walk));
@@ -209,7 +210,7 @@ void gw_destroy(gw_ctx* ctx) {
     gw_serve_close(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    free_graph(ctx);
+    gw_graph_free(ctx);
     free_walks(ctx);
     gw_free(ctx, ctx->d_counters, kCounters * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
@@ -228,6 +229,10 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_bucket_off, 0);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->copy_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& a : ctx->pinned) (void)hipHostFree(a.first);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -254,7 +259,7 @@ int gw_graph_set(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uint32_t
     for (uint64_t e = 0; e < m; ++e)
         if (col[e] >= n) return gw_fail(ctx, GW_EINVAL, "gw_graph_set: column out of range");
     GW_HIP(ctx, hipSetDevice(ctx->device));
-    free_graph(ctx);
+    gw_graph_free(ctx);
     int rc;
     if ((rc = gw_alloc(ctx, (void**)&ctx->d_off, (n + 1ull) * 8, "offsets"))) return rc;
     if ((rc = gw_alloc(ctx, (void**)&ctx->d_col, m * 4, "columns"))) return rc;
@@ -271,7 +276,7 @@ int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     if (!ctx || scale == 0 || scale > 31 || !(ta <= tab && tab <= tabc && tabc <= 65536))
         return gw_fail(ctx, GW_EINVAL, "gw_graph_rmat: bad arguments");
     GW_HIP(ctx, hipSetDevice(ctx->device));
-    free_graph(ctx);
+    gw_graph_free(ctx);
     int rc = gw_rmat_build(ctx, scale, m, seed, ta, tab, tabc);
     if (rc) return rc;
     if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, ctx->n * 4ull, "vertex counts"))) return rc;
@@ -279,7 +284,7 @@ int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 }
 
 int gw_graph_blocks(gw_ctx* ctx, const uint32_t* block_first, uint32_t nblocks) {
-    if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_blocks: no graph");
+    if (!ctx || !gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_graph_blocks: no graph");
     if (!block_first || nblocks == 0) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: empty block table");
     if (nblocks > 2048) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: at most 2048 blocks");
     if (block_first[0] != 0 || block_first[nblocks] != ctx->n)
@@ -287,6 +292,7 @@ int gw_graph_blocks(gw_ctx* ctx, const uint32_t* block_first, uint32_t nblocks) 
     for (uint32_t b = 0; b < nblocks; ++b)
         if (block_first[b + 1] <= block_first[b]) return gw_fail(ctx, GW_EINVAL, "gw_graph_blocks: empty or unordered block");
     GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_stream_reset(ctx);  // the block cache is sized for the old table
     gw_free(ctx, ctx->d_block_first, (ctx->nblocks + 1ull) * 4);
     ctx->d_block_first = nullptr;
     int rc = gw_alloc(ctx, (void**)&ctx->d_block_first, (nblocks + 1ull) * 4, "block table");
@@ -306,7 +312,12 @@ int gw_graph_info(gw_ctx* ctx, uint32_t* n, uint64_t* m, uint32_t* nblocks) {
 }
 
 int gw_graph_read(gw_ctx* ctx, uint64_t* off, uint32_t* col) {
-    if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_read: no graph");
+    if (!ctx || !gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_graph_read: no graph");
+    if (ctx->host_graph) {
+        if (off) std::memcpy(off, ctx->h_off, (ctx->n + 1ull) * 8);
+        if (col && ctx->m) std::memcpy(col, ctx->h_col, ctx->m * 4);
+        return GW_OK;
+    }
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (off) GW_HIP(ctx, hipMemcpy(off, ctx->d_off, (ctx->n + 1ull) * 8, hipMemcpyDeviceToHost));
@@ -316,7 +327,7 @@ int gw_graph_read(gw_ctx* ctx, uint64_t* off, uint32_t* col) {
 
 int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
     if (!ctx || (n && !walks)) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: bad arguments");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_load: no graph");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_load: no graph");
     if (n >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: walk ids are 32-bit");
     uint32_t src_max = 0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -340,7 +351,7 @@ int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
 
 int gw_walks_init_per_vertex(gw_ctx* ctx, uint32_t per_vertex) {
     if (!ctx || per_vertex == 0) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_per_vertex: bad arguments");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_per_vertex: no graph");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_per_vertex: no graph");
     const uint64_t nw = static_cast<uint64_t>(ctx->n) * per_vertex;
     if (nw >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_per_vertex: walk ids are 32-bit");
     GW_HIP(ctx, hipSetDevice(ctx->device));
@@ -361,7 +372,7 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
                           uint32_t walks_per_source) {
     if (!ctx || !sources || nsources == 0 || walks_per_source == 0)
         return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: bad arguments");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_sources: no graph");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_sources: no graph");
     uint32_t src_max = 0;
     for (uint32_t k = 0; k < nsources; ++k) {
         if (sources[k] >= ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: source out of range");
@@ -416,7 +427,7 @@ int gw_walks_current(gw_ctx* ctx, gw_walk* out, uint64_t n) {
 
 int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (!ctx || !prog) return gw_fail(ctx, GW_EINVAL, "gw_run: bad arguments");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_run: no graph");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_run: no graph");
     if (prog->kind > GW_RESTART) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown program kind");
     if (prog->L == 0 && prog->kind != GW_PROB) return gw_fail(ctx, GW_EINVAL, "gw_run: L must be >= 1");
     if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");

@@ -48,6 +48,24 @@ struct gw_ctx {
     void* d_ext = nullptr;              // successor-header edges (m x 16 B), built on first use
     bool ext_failed = false;
 
+    // host-resident (out-of-core) graph: the CSR stays in pinned host memory
+    // and blocks are copied into a device block cache as rounds need them
+    bool host_graph = false;
+    const uint64_t* h_off = nullptr;
+    const uint32_t* h_col = nullptr;
+    void* h_owned_off = nullptr;        // pinned copies the context made of caller arrays
+    void* h_owned_col = nullptr;
+    uint64_t cache_bytes = 0;
+    std::vector<std::pair<void*, uint64_t>> pinned;  // gw_host_alloc allocations
+    std::vector<int32_t> slot_block;    // block held by each cache slot, -1 = empty
+    std::vector<int32_t> block_slot;    // cache slot of each block, -1 = not resident
+    std::vector<void*> slot_mem;        // per slot: offsets then columns
+    uint64_t slot_bytes = 0, slot_off_bytes = 0;
+    void* h_views = nullptr;            // pinned: per block {rebased offsets, rebased columns}
+    void* d_views = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ev[3] = {nullptr, nullptr, nullptr};
+
     // walks
     uint64_t nwalks = 0;
     uint64_t walk_cap = 0;
@@ -106,6 +124,10 @@ struct gw_ctx {
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
+inline bool gw_has_graph(const gw_ctx* ctx) { return ctx->d_off != nullptr || ctx->host_graph; }
+// Frees the graph (device-resident or host-resident) and its block table.
+void gw_graph_free(gw_ctx* ctx);
+
 // Record an error message on the context and return `code`.
 int gw_fail(gw_ctx* ctx, int code, const std::string& msg);
 int gw_hip_check(gw_ctx* ctx, hipError_t e, const char* what);
@@ -141,6 +163,14 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok);
 int gw_partlog_reset(gw_ctx* ctx, bool part);
 int gw_partlog_enqueue(gw_ctx* ctx);
 int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, double* ms);
+
+// gw_stream.hip: device block cache of a host-resident graph.  prepare
+// sizes the slots for the current block table (once); reset drops them.
+int gw_stream_prepare(gw_ctx* ctx);
+void gw_stream_reset(gw_ctx* ctx);
+void gw_host_graph_free(gw_ctx* ctx);
+// true when the pointer range lies inside one gw_host_alloc allocation
+bool gw_is_pinned(const gw_ctx* ctx, const void* p, uint64_t bytes);
 
 // gw_bucket.hip: stable counting sort of `in` (n records) by block into `out`;
 // bucket_off[b] = first index of bucket b (nblocks + 1 entries written, host).

@@ -334,7 +334,7 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
                   uint32_t max_queries, uint32_t hops_per_step) {
     if (!ctx || !prog || pool_walks == 0 || max_queries == 0 || hops_per_step == 0 || arena_bytes < 4096)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad arguments");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: no graph");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, ctx->host_graph ? "gw_serve_open: serving needs a device-resident graph" : "gw_serve_open: no graph");
     if (prog->kind != GW_RESTART && prog->kind != GW_STOP && prog->kind != GW_JUMP)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: serving runs RESTART, STOP or JUMP programs");
     if (prog->L == 0 || prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad program");

@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
 
 constexpr uint32_t kMaxBlocks = 2048;
 
-__device__ __forceinline__ void block_bounds(const uint32_t* bf, uint32_t nb, uint32_t v, uint32_t& lo,
-                                             uint32_t& hi) {
+__device__ __forceinline__ uint32_t block_bounds(const uint32_t* bf, uint32_t nb, uint32_t v, uint32_t& lo,
+                                                 uint32_t& hi) {
     uint32_t a = 0, b = nb;  // bf[a] <= v < bf[b]
     while (b - a > 1) {
         const uint32_t mid = (a + b) >> 1;
@@ -168,18 +168,31 @@ __device__ __forceinline__ void block_bounds(const uint32_t* bf, uint32_t nb, ui
     }
     lo = bf[a];
     hi = bf[a + 1];
+    return a;
 }
 
-// One block round over all buckets, which lie back to back in block order in
-// [walks, walks + nw).  Persistent wavefronts take walks in bucket order (so
+// A resident block of a host-resident graph, as the step code reads it:
+// offsets and columns rebased so that off[v] and col[pos] take the graph's
+// absolute vertex ids and edge positions (valid for v in the block and pos in
+// its rows only).
+struct BlockView {
+    const uint64_t* off;
+    const uint32_t* col;
+};
+
+// One block round over the buckets in [walks, walks + nw), back to back in
+// block order.  STREAM: the graph is host-resident and each walk reads its
+// block through views[block] (a device cache slot); the range holds only
+// buckets of resident blocks.  Persistent wavefronts take walks in bucket order (so
 // the blocks being stepped at any moment are one or two consecutive ones),
 // and a walk steps while its vertex stays inside its block.  A walk that ends
 // writes its final record and leaves a dead slot (vertex = kDead, dropped by
 // the next bucketing); a walk that leaves the block is written back in place
 // with its new vertex/hop for the next round.
-template <int KIND, int REC, bool EXT>
+template <int KIND, int REC, bool EXT, bool STREAM>
 __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const uint32_t* __restrict__ block_first,
-                                                        uint32_t nb, gw_walk* __restrict__ walks, uint64_t nw,
+                                                        uint32_t nb, const BlockView* __restrict__ views,
+                                                        gw_walk* __restrict__ walks, uint64_t nw,
                                                         gw_walk* __restrict__ final_walks,
                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ hash,
                                                         uint32_t* __restrict__ visits,
@@ -195,6 +208,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     uint64_t slot = 0;
     uint32_t lo = 0, hi = 0, visits0 = 0;
     State s{0, 0, 0, 0, 0, 0};
+    Graph gl = g;  // the graph as this lane's walk sees it (its block's view when streaming)
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
@@ -233,8 +247,13 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
-                block_bounds(s_bf, nb, s.v, lo, hi);
-                load_row<EXT>(g, s);
+                const uint32_t bi = block_bounds(s_bf, nb, s.v, lo, hi);
+                if (STREAM) {
+                    const BlockView bv = views[bi];
+                    gl.off = bv.off;
+                    gl.col = bv.col;
+                }
+                load_row<EXT>(gl, s);
                 if (SUM) {
                     rec.hash = hash[s.id];
                     rec.visits = visits[s.id];
@@ -252,7 +271,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
         if (active) {
             bool done = false, left = false;
             if (s.v - lo < hi - lo) {
-                done = step_once<KIND, EXT>(p, g, s, rec);
+                done = step_once<KIND, EXT>(p, gl, s, rec);
             } else {
                 left = true;
             }
@@ -312,6 +331,7 @@ struct Launch {
     // blocked schedule: the buckets to step (back to back, block order)
     gw_walk* bucket;
     uint64_t bucket_n;
+    const BlockView* views;  // non-null: host-resident graph, step through the block cache
 };
 
 template <int KIND, int REC, bool EXT>
@@ -324,10 +344,14 @@ hipError_t launch4(const Launch& L) {
     } else if (L.sched == GW_SCHED_PERSISTENT) {
         step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
+    } else if (L.views) {
+        step_block_kernel<KIND, REC, false, true><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+            L.p, L.g, c->d_block_first, c->nblocks, L.views, L.bucket, L.bucket_n, c->d_final, c->d_counts,
+            c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else {
-        step_block_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_block_first, c->nblocks, L.bucket, L.bucket_n, c->d_final, c->d_counts, c->d_hash,
-            c->d_visits, c->d_counters, L.log, L.table, L.klog);
+        step_block_kernel<KIND, REC, EXT, false><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+            L.p, L.g, c->d_block_first, c->nblocks, nullptr, L.bucket, L.bucket_n, c->d_final, c->d_counts,
+            c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     }
     return hipGetLastError();
 }
@@ -441,6 +465,138 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
     return GW_OK;
 }
 
+// Host-resident graph: the blocked schedule over a device block cache.
+// Each round buckets the live walks by block, picks the blocks to step --
+// up to one per cache slot, by walk count, a resident block counting double
+// (the device analogue of chooseBlock's max-walks choice, walk.hpp:204-252,
+// with swapOut's fewest-walks eviction, graphwalker.hpp:236-251) -- copies
+// the missing ones into slots of unchosen blocks on the copy stream, steps
+// the buckets of the already-resident blocks while the copies run, then
+// (after an event wait) the buckets of the newly loaded ones.  Walks leave a
+// block exactly as in the in-memory blocked schedule, so trajectories do not
+// depend on the cache size.
+int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw_stats& st) {
+    int rc = gw_stream_prepare(ctx);
+    if (rc) return rc;
+    gw_walk* a = ctx->d_walks_alt;
+    gw_walk* b = ctx->d_work;
+    GW_HIP(ctx, hipMemcpyAsync(a, ctx->d_walks, ctx->nwalks * sizeof(gw_walk), hipMemcpyDeviceToDevice, ctx->stream));
+    if (rec & GW_REC_WALK_SUMMARY) {
+        summary_init_kernel<<<dim3(static_cast<uint32_t>((ctx->nwalks + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            ctx->d_walks, ctx->nwalks, ctx->d_hash, ctx->d_visits);
+        GW_HIP(ctx, hipGetLastError());
+    }
+    const uint32_t nb = ctx->nblocks;
+    const uint32_t nslots = static_cast<uint32_t>(ctx->slot_mem.size());
+    BlockView* hv = static_cast<BlockView*>(ctx->h_views);
+    L.views = static_cast<const BlockView*>(ctx->d_views);
+    uint64_t live = ctx->nwalks;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> order, chosen, loaded;
+    std::vector<char> in_round(nb, 0);
+    float ms = 0.f;
+    double step_ms = 0, bucket_ms = 0, copy_ms = 0;
+    // Steps the buckets of `blocks` (sorted), one launch per run of adjacent blocks.
+    auto step_blocks = [&](const std::vector<uint32_t>& blocks) -> int {
+        for (size_t i = 0; i < blocks.size();) {
+            size_t j = i + 1;
+            while (j < blocks.size() && blocks[j] == blocks[j - 1] + 1) ++j;
+            const uint64_t first = off[blocks[i]], last = off[blocks[j - 1] + 1];
+            i = j;
+            if (last == first) continue;
+            GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntWorkCursor, 0, sizeof(unsigned long long), ctx->stream));
+            L.bucket = a + first;
+            L.bucket_n = last - first;
+            hipError_t e = launch(L, prog->kind, rec);
+            if (e != hipSuccess) return gw_hip_check(ctx, e, "streamed step launch");
+            ++st.launches;
+        }
+        return GW_OK;
+    };
+    while (live > 0) {
+        GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        if ((rc = gw_bucket_sort(ctx, a, live, b, off))) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[3]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        bucket_ms += ms;
+        live = off[nb];
+        std::swap(a, b);  // buckets now in `a`
+        if (live == 0) break;
+        // choose the round's blocks
+        order.clear();
+        for (uint32_t k = 0; k < nb; ++k)
+            if (off[k + 1] > off[k]) order.push_back(k);
+        auto score = [&](uint32_t k) { return (off[k + 1] - off[k]) * (ctx->block_slot[k] >= 0 ? 2u : 1u); };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return score(x) > score(y); });
+        if (order.size() > nslots) order.resize(nslots);
+        chosen.clear();
+        loaded.clear();
+        for (uint32_t k : order) in_round[k] = 1;
+        // slots whose block is not in this round, emptiest first
+        std::vector<uint32_t> spare;
+        for (uint32_t sl = 0; sl < nslots; ++sl)
+            if (ctx->slot_block[sl] < 0 || !in_round[ctx->slot_block[sl]]) spare.push_back(sl);
+        size_t next_spare = 0;
+        bool copies = false;
+        for (uint32_t k : order) {
+            if (ctx->block_slot[k] >= 0) {
+                chosen.push_back(k);
+                continue;
+            }
+            const uint32_t sl = spare[next_spare++];  // one slot per chosen block, so one is spare
+            if (ctx->slot_block[sl] >= 0) ctx->block_slot[ctx->slot_block[sl]] = -1;
+            ctx->slot_block[sl] = static_cast<int32_t>(k);
+            ctx->block_slot[k] = static_cast<int32_t>(sl);
+            const uint32_t lo = ctx->block_first[k], hi = ctx->block_first[k + 1];
+            const uint64_t e0 = ctx->h_off[lo], e1 = ctx->h_off[hi];
+            char* mem = static_cast<char*>(ctx->slot_mem[sl]);
+            if (!copies) GW_HIP(ctx, hipEventRecord(ctx->copy_ev[1], ctx->copy_stream));
+            copies = true;
+            GW_HIP(ctx, hipMemcpyAsync(mem, ctx->h_off + lo, (hi - lo + 1ull) * 8, hipMemcpyHostToDevice,
+                                       ctx->copy_stream));
+            if (e1 > e0)
+                GW_HIP(ctx, hipMemcpyAsync(mem + ctx->slot_off_bytes, ctx->h_col + e0, (e1 - e0) * 4,
+                                           hipMemcpyHostToDevice, ctx->copy_stream));
+            st.h2d_bytes += (hi - lo + 1ull) * 8 + (e1 - e0) * 4;
+            ++st.blocks_loaded;
+            // rebased so the kernels index with absolute vertex ids / edge positions
+            hv[k].off = reinterpret_cast<const uint64_t*>(reinterpret_cast<uintptr_t>(mem) - uintptr_t{lo} * 8);
+            hv[k].col = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(mem + ctx->slot_off_bytes) -
+                                                          static_cast<uintptr_t>(e0) * 4);
+            loaded.push_back(k);
+        }
+        for (uint32_t k : order) in_round[k] = 0;
+        if (copies) {
+            GW_HIP(ctx, hipEventRecord(ctx->copy_ev[2], ctx->copy_stream));
+            GW_HIP(ctx, hipEventRecord(ctx->copy_ev[0], ctx->copy_stream));
+        }
+        GW_HIP(ctx, hipMemcpyAsync(ctx->d_views, ctx->h_views, nb * sizeof(BlockView), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        std::sort(chosen.begin(), chosen.end());
+        std::sort(loaded.begin(), loaded.end());
+        GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        if ((rc = step_blocks(chosen))) return rc;  // overlaps the copies
+        if (copies) GW_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->copy_ev[0], 0));
+        if ((rc = step_blocks(loaded))) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        step_ms += ms;
+        if (copies) {
+            GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->copy_ev[1], ctx->copy_ev[2]));
+            copy_ms += ms;
+        }
+        ++st.rounds;
+    }
+    st.step_ms = step_ms;
+    st.bucket_ms = bucket_ms;
+    st.copy_ms = copy_ms;
+    st.cache_slots = nslots;
+    L.views = nullptr;
+    return GW_OK;
+}
+
 // Key log for sort-reduce source counts: capacity for every visit the
 // program can make (capped kinds: L per walk; uncapped: 64 per walk, more is
 // an overflow error) plus slab padding.
@@ -492,7 +648,7 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     ctx->table_log2 = lg;
     ctx->table_vbits = vbits;
     ctx->table_cbits = 64 - vbits - sbits;
-    if (prog->source_method == GW_SRC_SORT) return GW_OK;  // no hash table
+    if (prog->source_method != GW_SRC_HASH && !ctx->host_graph) return GW_OK;  // key log, no hash table
     const uint64_t slots = 1ull << lg;
     if (slots != ctx->table_slots) {
         gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
@@ -517,15 +673,26 @@ bool gw_ensure_ext(gw_ctx* ctx) { return gw::ensure_ext(ctx); }
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     using namespace gw;
     gw_stats st{};
-    const uint32_t sched = prog->schedule == GW_SCHED_AUTO ? GW_SCHED_PERSISTENT : prog->schedule;
+    const uint32_t sched = prog->schedule != GW_SCHED_AUTO ? prog->schedule
+                           : ctx->host_graph            ? GW_SCHED_BLOCKED
+                                                        : GW_SCHED_PERSISTENT;
     if (sched != GW_SCHED_PLAIN && sched != GW_SCHED_PERSISTENT && sched != GW_SCHED_BLOCKED)
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown schedule");
     if (prog->layout > GW_LAYOUT_EXT) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown layout");
+    if (ctx->host_graph) {
+        if (sched != GW_SCHED_BLOCKED)
+            return gw_fail(ctx, GW_EINVAL, "gw_run: a host-resident graph runs the blocked schedule");
+        if (prog->layout == GW_LAYOUT_EXT)
+            return gw_fail(ctx, GW_EINVAL, "gw_run: a host-resident graph is read in the CSR layout");
+        if ((prog->recorder & GW_REC_SOURCE_COUNT) &&
+            (prog->source_method == GW_SRC_GLOBAL || prog->source_method == GW_SRC_PARTITIONED))
+            return gw_fail(ctx, GW_EINVAL, "gw_run: a host-resident graph counts sources with GW_SRC_HASH");
+    }
     if (sched == GW_SCHED_PLAIN && prog->layout == GW_LAYOUT_EXT)
         return gw_fail(ctx, GW_EINVAL, "gw_run: the plain schedule reads the CSR layout");
     st.schedule = sched;
     bool ext = false;
-    if (sched != GW_SCHED_PLAIN && prog->layout != GW_LAYOUT_CSR) {
+    if (sched != GW_SCHED_PLAIN && prog->layout != GW_LAYOUT_CSR && !ctx->host_graph) {
         ext = ensure_ext(ctx);
         if (!ext && prog->layout == GW_LAYOUT_EXT)
             return gw_fail(ctx, GW_ENOMEM, "gw_run: successor-header layout does not fit (HBM budget or degree >= 2^32)");
@@ -546,7 +713,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (rec & GW_REC_SOURCE_COUNT) {
         int rc = prepare_source_table(ctx, prog);
         if (rc) return rc;
-        if (prog->source_method != GW_SRC_HASH) {
+        if (prog->source_method != GW_SRC_HASH && !ctx->host_graph) {
             if (prog->source_method == GW_SRC_PARTITIONED && ctx->field_walks.empty())
                 return gw_fail(ctx, GW_EINVAL, "gw_run: GW_SRC_PARTITIONED needs source fields below 2^20");
             const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -602,7 +769,9 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     float ms = 0.f;
     if (sched == GW_SCHED_BLOCKED) {
         GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        int rc = ctx->nwalks ? run_blocked(ctx, L, prog, rec, st) : GW_OK;
+        int rc = !ctx->nwalks     ? GW_OK
+                 : ctx->host_graph ? run_streamed(ctx, L, prog, rec, st)
+                                   : run_blocked(ctx, L, prog, rec, st);
         if (rc) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
     } else {

@@ -1,0 +1,209 @@
+// gw_stream.hip -- out-of-core graphs: the CSR stays in pinned host memory
+// and a device block cache holds the blocks a round steps.
+//
+// The reference engine keeps `nmblocks` blocks in memory, reads a missing
+// block's beg_pos / csr slices from disk and evicts the resident block with
+// the fewest walks (findSubGraph / loadSubGraph / swapOut,
+// src/engine/graphwalker.hpp:168-251).  Here the cache is a set of equal
+// device slots sized for the largest block; gw_step.hip's streamed rounds
+// choose the blocks, copy them over PCIe on a side stream (hipMemcpyAsync
+// from pinned memory, default stream priority) and step their buckets.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gw_internal.h"
+
+namespace {
+
+constexpr uint64_t kSlotAlign = 256;
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+uint64_t block_bytes(const gw_ctx* ctx, uint32_t b, uint64_t* off_bytes) {
+    const uint32_t lo = ctx->block_first[b], hi = ctx->block_first[b + 1];
+    const uint64_t ob = align_up((hi - lo + 1ull) * 8, kSlotAlign);
+    if (off_bytes) *off_bytes = ob;
+    return ob + (ctx->h_off[hi] - ctx->h_off[lo]) * 4;
+}
+
+// Greedy cut of the vertices into blocks of at most `target` bytes (offsets +
+// columns), each at least one vertex: the rule of compute_block
+// (conversions.hpp:300-364) with bytes in place of the file block size.
+std::vector<uint32_t> cut_blocks(const gw_ctx* ctx, uint64_t target) {
+    std::vector<uint32_t> bf{0};
+    uint32_t lo = 0;
+    while (lo < ctx->n) {
+        uint32_t hi = lo + 1;
+        // grow while the block [lo, hi + 1) still fits
+        while (hi < ctx->n && (hi + 1 - lo + 1ull) * 8 + (ctx->h_off[hi + 1] - ctx->h_off[lo]) * 4 <= target) ++hi;
+        bf.push_back(hi);
+        lo = hi;
+    }
+    return bf;
+}
+
+}  // namespace
+
+bool gw_is_pinned(const gw_ctx* ctx, const void* p, uint64_t bytes) {
+    const char* c = static_cast<const char*>(p);
+    for (const auto& a : ctx->pinned) {
+        const char* b = static_cast<const char*>(a.first);
+        if (c >= b && c + bytes <= b + a.second) return true;
+    }
+    return false;
+}
+
+void gw_stream_reset(gw_ctx* ctx) {
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    for (void* m : ctx->slot_mem) gw_free(ctx, m, ctx->slot_bytes);
+    ctx->slot_mem.clear();
+    ctx->slot_block.clear();
+    ctx->block_slot.clear();
+    ctx->slot_bytes = ctx->slot_off_bytes = 0;
+    if (ctx->h_views) (void)hipHostFree(ctx->h_views);
+    ctx->h_views = nullptr;
+    gw_free(ctx, ctx->d_views, 0);
+    ctx->d_views = nullptr;
+}
+
+void gw_host_graph_free(gw_ctx* ctx) {
+    gw_stream_reset(ctx);
+    if (ctx->h_owned_off) (void)hipHostFree(ctx->h_owned_off);
+    if (ctx->h_owned_col) (void)hipHostFree(ctx->h_owned_col);
+    ctx->h_owned_off = ctx->h_owned_col = nullptr;
+    ctx->h_off = nullptr;
+    ctx->h_col = nullptr;
+    ctx->host_graph = false;
+    ctx->cache_bytes = 0;
+}
+
+int gw_stream_prepare(gw_ctx* ctx) {
+    if (!ctx->slot_mem.empty()) return GW_OK;
+    const uint32_t nb = ctx->nblocks;
+    uint64_t big = 0, big_off = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        uint64_t ob = 0;
+        const uint64_t bytes = block_bytes(ctx, b, &ob);
+        big = std::max(big, align_up(bytes, kSlotAlign));
+        big_off = std::max(big_off, ob);
+    }
+    // a slot holds the largest offsets part followed by the largest column part
+    uint64_t max_cols = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+        max_cols = std::max<uint64_t>(max_cols, ctx->h_off[ctx->block_first[b + 1]] - ctx->h_off[ctx->block_first[b]]);
+    const uint64_t slot = align_up(big_off + max_cols * 4, kSlotAlign);
+    const uint64_t nslots = std::min<uint64_t>(nb, ctx->cache_bytes / slot);
+    if (nslots == 0)
+        return gw_fail(ctx, GW_ENOMEM, "gw_run: the block cache (" + std::to_string(ctx->cache_bytes) +
+                                           " bytes) is smaller than one block slot (" + std::to_string(slot) + " bytes)");
+    ctx->slot_bytes = slot;
+    ctx->slot_off_bytes = big_off;
+    for (uint64_t i = 0; i < nslots; ++i) {
+        void* m = nullptr;
+        int rc = gw_alloc(ctx, &m, slot, "block cache slot");
+        if (rc) {
+            gw_stream_reset(ctx);
+            return rc;
+        }
+        ctx->slot_mem.push_back(m);
+    }
+    ctx->slot_block.assign(nslots, -1);
+    ctx->block_slot.assign(nb, -1);
+    GW_HIP(ctx, hipHostMalloc(&ctx->h_views, nb * 16ull, hipHostMallocDefault));
+    std::memset(ctx->h_views, 0, nb * 16ull);
+    int rc = gw_alloc(ctx, &ctx->d_views, nb * 16ull, "block views");
+    if (rc) return rc;
+    if (!ctx->copy_stream) GW_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    for (auto& e : ctx->copy_ev)
+        if (!e) GW_HIP(ctx, hipEventCreate(&e));
+    return GW_OK;
+}
+
+extern "C" {
+
+int gw_host_alloc(gw_ctx* ctx, uint64_t bytes, void** out) {
+    if (!ctx || !out || bytes == 0) return gw_fail(ctx, GW_EINVAL, "gw_host_alloc: bad arguments");
+    *out = nullptr;
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, "gw_host_alloc: hipHostMalloc(" + std::to_string(bytes) + ") failed: " +
+                                           hipGetErrorString(e));
+    }
+    ctx->pinned.emplace_back(*out, bytes);
+    return GW_OK;
+}
+
+int gw_host_free(gw_ctx* ctx, void* p) {
+    if (!ctx || !p) return gw_fail(ctx, GW_EINVAL, "gw_host_free: bad arguments");
+    auto it = std::find_if(ctx->pinned.begin(), ctx->pinned.end(), [&](const auto& a) { return a.first == p; });
+    if (it == ctx->pinned.end()) return gw_fail(ctx, GW_EINVAL, "gw_host_free: not a gw_host_alloc pointer");
+    const char* b = static_cast<const char*>(p);
+    auto inside = [&](const void* q) {
+        const char* c = static_cast<const char*>(q);
+        return q && c >= b && c < b + it->second;
+    };
+    if (ctx->host_graph && (inside(ctx->h_off) || inside(ctx->h_col)))
+        return gw_fail(ctx, GW_ESTATE, "gw_host_free: the host-resident graph still uses this memory");
+    GW_HIP(ctx, hipHostFree(p));
+    ctx->pinned.erase(it);
+    return GW_OK;
+}
+
+int gw_graph_set_host(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uint32_t n, uint64_t m,
+                      uint64_t cache_bytes) {
+    if (!ctx || !off || (m && !col) || n == 0 || n == 0xFFFFFFFFu || cache_bytes == 0)
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_set_host: bad arguments");
+    if (off[0] != 0 || off[n] != m) return gw_fail(ctx, GW_EINVAL, "gw_graph_set_host: off[0] must be 0 and off[n] == m");
+    for (uint32_t v = 0; v < n; ++v)
+        if (off[v + 1] < off[v]) return gw_fail(ctx, GW_EINVAL, "gw_graph_set_host: offsets not monotone");
+    for (uint64_t e = 0; e < m; ++e)
+        if (col[e] >= n) return gw_fail(ctx, GW_EINVAL, "gw_graph_set_host: column out of range");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_graph_free(ctx);
+    // pinned memory for the copies: the caller's arrays are used in place
+    // when gw_host_alloc made them
+    const uint64_t ob = (n + 1ull) * 8, cb = m * 4;
+    if (gw_is_pinned(ctx, off, ob)) {
+        ctx->h_off = off;
+    } else {
+        GW_HIP(ctx, hipHostMalloc(&ctx->h_owned_off, ob, hipHostMallocDefault));
+        std::memcpy(ctx->h_owned_off, off, ob);
+        ctx->h_off = static_cast<const uint64_t*>(ctx->h_owned_off);
+    }
+    if (m == 0 || gw_is_pinned(ctx, col, cb)) {
+        ctx->h_col = col;
+    } else {
+        hipError_t e = hipHostMalloc(&ctx->h_owned_col, cb, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            gw_host_graph_free(ctx);
+            return gw_hip_check(ctx, e, "hipHostMalloc(columns)");
+        }
+        std::memcpy(ctx->h_owned_col, col, cb);
+        ctx->h_col = static_cast<const uint32_t*>(ctx->h_owned_col);
+    }
+    ctx->host_graph = true;
+    ctx->n = n;
+    ctx->m = m;
+    ctx->cache_bytes = cache_bytes;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, n * 4ull, "vertex counts"))) {
+        gw_host_graph_free(ctx);
+        return rc;
+    }
+    // default block table: blocks of at most cache_bytes / 4 (at least four
+    // slots), grown until it has at most 2048 blocks
+    uint64_t target = std::max<uint64_t>(cache_bytes / 4, 64);
+    std::vector<uint32_t> bf = cut_blocks(ctx, target);
+    while (bf.size() - 1 > 2048) {
+        target *= 2;
+        bf = cut_blocks(ctx, target);
+    }
+    return gw_graph_blocks(ctx, bf.data(), static_cast<uint32_t>(bf.size() - 1));
+}
+
+}  // extern "C"

@@ -15,6 +15,15 @@
  *   gw_graph_set               loadSubGraph / findSubGraph: beg_pos + csr made
  *                              resident (graphwalker.hpp:168-234); block table
  *                              load_block_range (graphwalker.hpp:146-166)
+ *   gw_graph_set_host          the out-of-core engine itself: the graph stays in
+ *                              host memory and blocks are made resident in a
+ *                              device block cache of nmblocks-like capacity
+ *                              (findSubGraph / loadSubGraph / swapOut,
+ *                              graphwalker.hpp:168-251; block choice
+ *                              chooseBlock, walk.hpp:204-252)
+ *   gw_host_alloc / gw_host_free   pinned host memory for such graphs (the
+ *                              reference reads blocks with pread into malloc'd
+ *                              buffers, graphwalker.hpp:84-100, 184-213)
  *   gw_graph_rmat              (no counterpart: the reference reads a
  *                              preprocessed CSR, conversions.hpp:181-298; this
  *                              builds the same CSR layout on the device)
@@ -139,6 +148,10 @@ typedef struct gw_stats {
     double reduce_ms;      /* source-count aggregation after the step kernels (sort-based methods) */
     uint32_t source_method;    /* GW_SRC_* actually used (GW_REC_SOURCE_COUNT) */
     uint32_t flushed_sources;  /* GW_SRC_PARTITIONED: sources that went through the exact split sort */
+    uint64_t h2d_bytes;        /* host-resident graph: block bytes copied into the device cache */
+    uint32_t blocks_loaded;    /* host-resident graph: block loads */
+    uint32_t cache_slots;      /* host-resident graph: device block-cache slots */
+    double copy_ms;            /* host-resident graph: block-copy time on the copy stream */
 } gw_stats;
 
 int gw_create(gw_ctx **out, const gw_cfg *cfg);
@@ -155,6 +168,19 @@ int gw_graph_rmat(gw_ctx *ctx, uint32_t scale, uint64_t m, uint64_t seed,
 int gw_graph_blocks(gw_ctx *ctx, const uint32_t *block_first, uint32_t nblocks);
 int gw_graph_info(gw_ctx *ctx, uint32_t *n, uint64_t *m, uint32_t *nblocks);
 int gw_graph_read(gw_ctx *ctx, uint64_t *off, uint32_t *col);
+
+/* Out-of-core graph.  The CSR stays in host memory; gw_run steps it block by
+ * block through a device block cache of at most cache_bytes (slots sized for
+ * the largest block), copying blocks on a side stream while resident blocks
+ * step.  Arrays allocated with gw_host_alloc are used in place (they must
+ * outlive the graph); other host arrays are copied once into pinned memory
+ * the context owns.  Without a gw_graph_blocks call the vertices are cut into
+ * blocks of at most cache_bytes / 4 bytes each.  Runs use the BLOCKED
+ * schedule and the CSR layout; GW_REC_SOURCE_COUNT uses GW_SRC_HASH. */
+int gw_host_alloc(gw_ctx *ctx, uint64_t bytes, void **out);
+int gw_host_free(gw_ctx *ctx, void *p);
+int gw_graph_set_host(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uint32_t n, uint64_t m,
+                      uint64_t cache_bytes);
 
 /* Walks: records with ids < n (ids key the random streams). */
 int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
