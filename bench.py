@@ -202,6 +202,66 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
             "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]"}
 
 
+def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, queries: int, K: int,
+                   walks_lo: int = 1000, walks_hi: int = 100_000) -> dict:
+    """Out-of-core run on the device-built graph left resident by the last
+    workload: the same restart-PPR batch (p = 0.15, 10 hops, per-query
+    source counts in the hash table) first on the graph in HBM, then with the
+    graph moved to pinned host memory behind a device block cache of
+    cache_bytes.  Walks per query log-uniform in [walks_lo, walks_hi], uniform
+    sources."""
+    n = 1 << scale
+    rng = np.random.default_rng(seed)
+    src = gw.sources(seed + 11, n, queries)
+    wpq = np.exp(rng.uniform(np.log(walks_lo), np.log(walks_hi), queries)).astype(np.int64)
+    starts = np.repeat(src, wpq)
+    walks = np.zeros(starts.shape[0], dtype=gw.WALK_DTYPE)
+    walks["id"] = np.arange(starts.shape[0], dtype=np.uint32)
+    walks["source"] = starts
+    walks["vertex"] = starts
+    lg = int(np.ceil(np.log2(max(1, starts.shape[0]) * 10 * 2)))  # table for ~10 visits per walk at <= 1/2 load
+
+    def batch(sched: str, layout: str) -> tuple[float, dict]:
+        prog = gw.program(gw.RESTART, 10, 0.15, n, seed=seed, recorder=gw.REC_SOURCE_COUNT, schedule=sched,
+                          layout=layout, source_method=gw.SRC_HASH, table_log2=lg)
+        eng.walks_load(walks)
+        eng.run(prog)
+        t = time.perf_counter()
+        st = None
+        steps = 0
+        for _ in range(K):
+            st = eng.run(prog)
+            steps += st["steps"]
+        return steps / (time.perf_counter() - t), st
+
+    hbm_rate, hbm_st = batch("persistent", "ext")
+    hoff = eng.host_empty((eng.n + 1,), np.uint64)
+    hcol = eng.host_empty((eng.m,), np.uint32)
+    eng.graph_read(hoff, hcol)
+    graph_bytes = hoff.nbytes + hcol.nbytes
+    eng.graph_set_host(hoff, hcol, cache_bytes)
+    rate, st = batch("blocked", "csr")
+    h2d_ms = eng.h2d_probe(1 << 30)
+    out = {
+        "desc": f"R-MAT scale {scale} graph ({graph_bytes / 2**30:.2f} GiB CSR) in pinned host memory, device block "
+                f"cache {cache_bytes / 2**30:.2f} GiB; {queries} restart-PPR queries (p 0.15, 10 hops), walks per "
+                f"query log-uniform [{walks_lo}, {walks_hi}], hash source counts",
+        "walks": int(starts.shape[0]), "walk_steps_per_pass": st["steps"],
+        "walk_steps_per_s": rate, "in_hbm_walk_steps_per_s": hbm_rate,
+        "same_steps_as_in_hbm": st["steps"] == hbm_st["steps"] and st["visits"] == hbm_st["visits"],
+        "rounds": st["rounds"], "launches": st["launches"], "blocks": eng.graph_info()["nblocks"], "cache_slots": st["cache_slots"],
+        "blocks_loaded": st["blocks_loaded"], "h2d_bytes_per_pass": st["h2d_bytes"],
+        "copy_ms": st["copy_ms"], "step_ms": st["step_ms"], "bucket_ms": st["bucket_ms"], "total_ms": st["total_ms"],
+        "h2d_GBps_achieved": st["h2d_bytes"] / max(1e-9, st["copy_ms"] * 1e-3) / 1e9,
+        "h2d_GBps_probe": (1 << 30) / (h2d_ms * 1e-3) / 1e9,
+        "graph_bytes": graph_bytes, "cache_bytes": cache_bytes,
+    }
+    print(f"[bench] stream: {rate / 1e9:.3f} G steps/s ({hbm_rate / 1e9:.2f} in HBM), {st['rounds']} rounds, "
+          f"{st['blocks_loaded']} loads, {st['h2d_bytes'] / 2**30:.1f} GiB copied at "
+          f"{out['h2d_GBps_achieved']:.1f} GB/s (probe {out['h2d_GBps_probe']:.1f})", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +275,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--stream-cache-mib", type=int, default=1024,
+                    help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
+    ap.add_argument("--stream-queries", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -275,6 +338,12 @@ def main() -> int:
             print(f"[bench] serving {q:.0f} qps: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
                   f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
 
+    stream = {}
+    if args.stream_cache_mib:
+        wl_last = WORKLOADS[args.secondary] if workloads else wl
+        stream = stream_measure(eng, wl_last["scale"], wl_last["seed"], args.stream_cache_mib << 20,
+                                args.stream_queries, 2)
+
     out = {
         "metric": "walk-steps/s",
         "value": m["value"],
@@ -301,6 +370,7 @@ def main() -> int:
         "variants": m["variants"],
         "workloads": workloads,
         "serving": serving,
+        "out_of_core": stream,
     }
     if "speedup_vs_plain" in m:
         out["speedup_vs_plain"] = m["speedup_vs_plain"]

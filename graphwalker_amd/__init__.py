@@ -180,6 +180,7 @@ def lib() -> C.CDLL:
         "gw_source_counts_read": ([P, P, P, P, u64, P], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
+        "gw_h2d_probe": ([P, u64, P], i),
         "gw_sources": ([u64, u32, u32, P], i),
         "gw_serve_open": ([P, P, u64, u64, u32, u32], i),
         "gw_serve_submit": ([P, P, u32], i),
@@ -280,13 +281,24 @@ class Engine:
         self._check(self._L.gw_graph_rmat(self._ctx, scale, m, seed, ta, tab, tabc), "gw_graph_rmat")
         self.n, self.m = 1 << scale, m
 
+    def graph_info(self) -> dict:
+        n, m, nb = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        self._check(self._L.gw_graph_info(self._ctx, C.byref(n), C.byref(m), C.byref(nb)), "gw_graph_info")
+        return {"n": n.value, "m": m.value, "nblocks": nb.value}
+
     def graph_blocks(self, block_first) -> None:
         bf = np.ascontiguousarray(block_first, dtype=np.uint32)
         self._check(self._L.gw_graph_blocks(self._ctx, _ptr(bf), bf.shape[0] - 1), "gw_graph_blocks")
 
-    def graph_read(self) -> tuple[np.ndarray, np.ndarray]:
-        off = np.empty(self.n + 1, dtype=np.uint64)
-        col = np.empty(self.m, dtype=np.uint32)
+    def graph_read(self, off: np.ndarray | None = None, col: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """The resident graph's CSR, into new arrays or the given ones (e.g.
+        host_empty arrays, to move a device-built graph to host memory)."""
+        off = np.empty(self.n + 1, dtype=np.uint64) if off is None else off
+        col = np.empty(self.m, dtype=np.uint32) if col is None else col
+        if off.dtype != np.uint64 or off.shape != (self.n + 1,) or col.dtype != np.uint32 or col.shape != (self.m,):
+            raise GwError("graph_read: output arrays must be uint64[n + 1] and uint32[m]")
+        if not (off.flags.c_contiguous and col.flags.c_contiguous):
+            raise GwError("graph_read: output arrays must be contiguous")
         self._check(self._L.gw_graph_read(self._ctx, _ptr(off), _ptr(col)), "gw_graph_read")
         return off, col
 
@@ -341,6 +353,11 @@ class Engine:
         v = np.empty(self.nwalks, dtype=np.uint32)
         self._check(self._L.gw_summary_read(self._ctx, _ptr(h), _ptr(v), self.nwalks), "gw_summary_read")
         return h, v
+
+    def h2d_probe(self, nbytes: int) -> float:
+        ms = C.c_double()
+        self._check(self._L.gw_h2d_probe(self._ctx, nbytes, C.byref(ms)), "gw_h2d_probe")
+        return ms.value
 
     def copy_probe(self, nbytes: int) -> float:
         ms = C.c_double()

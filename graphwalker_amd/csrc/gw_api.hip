@@ -214,7 +214,7 @@ void gw_destroy(gw_ctx* ctx) {
     free_walks(ctx);
     gw_free(ctx, ctx->d_counters, kCounters * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
-    gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
+    gw_free(ctx, ctx->d_table, ctx->table_bytes);
     gw_free(ctx, ctx->d_keys, ctx->keys_cap * 8);
     gw_free(ctx, ctx->d_keys_alt, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_sort_temp, ctx->sort_temp_bytes);

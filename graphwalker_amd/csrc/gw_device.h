@@ -81,10 +81,11 @@ struct EventLog {
     uint32_t cap;
 };
 
-// Device-wide open-addressing hash of (source field, vertex) -> visit count.
-// A slot holds (key << cbits) | count with key = (source << vbits) | vertex;
-// 0 marks an empty slot (every stored entry has count >= 1).  Linear probing
-// from a multiplicative hash; a full table raises `overflow`.
+// Device-wide open-addressing hash of (source field, vertex) -> visit count,
+// key = (source << vbits) | vertex.  Packed (keys of at most 44 bits): a slot
+// holds (key << cbits) | count, 0 marks an empty slot (every stored entry has
+// count >= 1).  Wide: a slot is two words {key + 1, count}, 0 marks empty.
+// Linear probing from a multiplicative hash; a full table raises `overflow`.
 struct SourceTable {
     unsigned long long* __restrict__ tab;
     unsigned long long* __restrict__ overflow;
@@ -92,12 +93,26 @@ struct SourceTable {
     uint32_t shift;  // 64 - log2(slots)
     uint32_t vbits;
     uint32_t cbits;
+    uint32_t wide;
 };
 
 __device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source, uint32_t v) {
     const uint64_t key = (static_cast<uint64_t>(source) << t.vbits) | v;
-    const unsigned long long entry = (static_cast<unsigned long long>(key) << t.cbits) | 1ull;
     uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> t.shift;
+    if (t.wide) {
+        const unsigned long long tag = key + 1;
+        for (uint64_t probe = 0; probe <= t.mask; ++probe) {
+            const unsigned long long old = atomicCAS(&t.tab[2 * h], 0ull, tag);
+            if (old == 0ull || old == tag) {
+                atomicAdd(&t.tab[2 * h + 1], 1ull);
+                return;
+            }
+            h = (h + 1) & t.mask;
+        }
+        atomicAdd(t.overflow, 1ull);
+        return;
+    }
+    const unsigned long long entry = (static_cast<unsigned long long>(key) << t.cbits) | 1ull;
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
         const unsigned long long old = atomicCAS(&t.tab[h], 0ull, entry);
         if (old == 0ull) return;

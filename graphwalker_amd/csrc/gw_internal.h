@@ -60,7 +60,7 @@ struct gw_ctx {
     std::vector<int32_t> slot_block;    // block held by each cache slot, -1 = empty
     std::vector<int32_t> block_slot;    // cache slot of each block, -1 = not resident
     std::vector<void*> slot_mem;        // per slot: offsets then columns
-    uint64_t slot_bytes = 0, slot_off_bytes = 0;
+    uint64_t slot_bytes = 0;
     void* h_views = nullptr;            // pinned: per block {rebased offsets, rebased columns}
     void* d_views = nullptr;
     hipStream_t copy_stream = nullptr;
@@ -82,8 +82,10 @@ struct gw_ctx {
     uint64_t* d_events = nullptr;   // walk_cap x events_cap event slots
     uint32_t events_cap = 0;
     unsigned long long* d_table = nullptr;  // (source, vertex) -> count hash slots
-    uint64_t table_slots = 0;
-    uint32_t table_log2 = 0, table_vbits = 0, table_cbits = 0;
+    uint64_t table_slots = 0, table_bytes = 0;
+    uint32_t table_log2 = 0, table_vbits = 0;
+    uint32_t table_cbits = 0;                // 64 - key bits (count bits of a packed entry)
+    bool table_wide = false;                 // 16-byte entries {key + 1, count} for keys over 44 bits
     uint32_t src_method = 1;                 // GW_SRC_* of the last source-count run
     unsigned long long* d_keys = nullptr;    // sort-reduce key log
     uint64_t keys_cap = 0;

@@ -466,15 +466,19 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
 }
 
 // Host-resident graph: the blocked schedule over a device block cache.
-// Each round buckets the live walks by block, picks the blocks to step --
-// up to one per cache slot, by walk count, a resident block counting double
-// (the device analogue of chooseBlock's max-walks choice, walk.hpp:204-252,
-// with swapOut's fewest-walks eviction, graphwalker.hpp:236-251) -- copies
-// the missing ones into slots of unchosen blocks on the copy stream, steps
-// the buckets of the already-resident blocks while the copies run, then
-// (after an event wait) the buckets of the newly loaded ones.  Walks leave a
-// block exactly as in the in-memory blocked schedule, so trajectories do not
-// depend on the cache size.
+// Each pass buckets the live walks by block.  While the resident blocks hold
+// a worthwhile share of the walks (at least an eighth of the mean bucket of
+// the blocks with walks), a drain pass steps just them, with no copies.
+// Otherwise a load pass picks the blocks to step -- up to one per cache slot,
+// by walk count, a resident block counting double (the device analogue of
+// chooseBlock's max-walks choice, walk.hpp:204-252, with swapOut's
+// fewest-walks eviction, graphwalker.hpp:236-251) -- copies the missing ones
+// into slots of unchosen blocks on the copy stream, steps the buckets of the
+// already-resident blocks while the copies run, then (after an event wait)
+// the buckets of the newly loaded ones.  Block copies are the cost being
+// minimised (PCIe moves ~57 GB/s, the step kernels much more), so each load
+// is drained before the next.  Walks leave a block exactly as in the
+// in-memory blocked schedule, so trajectories do not depend on the cache.
 int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw_stats& st) {
     int rc = gw_stream_prepare(ctx);
     if (rc) return rc;
@@ -523,7 +527,28 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
         live = off[nb];
         std::swap(a, b);  // buckets now in `a`
         if (live == 0) break;
-        // choose the round's blocks
+        // drain pass: the resident blocks still hold enough walks
+        uint64_t resident_walks = 0, with_walks = 0;
+        chosen.clear();
+        for (uint32_t k = 0; k < nb; ++k) {
+            const uint64_t c = off[k + 1] - off[k];
+            with_walks += c > 0;
+            if (c > 0 && ctx->block_slot[k] >= 0) {
+                resident_walks += c;
+                chosen.push_back(k);
+            }
+        }
+        if (!chosen.empty() && resident_walks * 8 * with_walks >= live) {
+            GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+            if ((rc = step_blocks(chosen))) return rc;
+            GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            GW_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+            GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            step_ms += ms;
+            ++st.rounds;
+            continue;
+        }
+        // load pass: choose the blocks
         order.clear();
         for (uint32_t k = 0; k < nb; ++k)
             if (off[k + 1] > off[k]) order.push_back(k);
@@ -551,18 +576,19 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
             const uint32_t lo = ctx->block_first[k], hi = ctx->block_first[k + 1];
             const uint64_t e0 = ctx->h_off[lo], e1 = ctx->h_off[hi];
             char* mem = static_cast<char*>(ctx->slot_mem[sl]);
+            const uint64_t col_at = ((hi - lo + 1ull) * 8 + 255) / 256 * 256;  // columns after the offsets
             if (!copies) GW_HIP(ctx, hipEventRecord(ctx->copy_ev[1], ctx->copy_stream));
             copies = true;
             GW_HIP(ctx, hipMemcpyAsync(mem, ctx->h_off + lo, (hi - lo + 1ull) * 8, hipMemcpyHostToDevice,
                                        ctx->copy_stream));
             if (e1 > e0)
-                GW_HIP(ctx, hipMemcpyAsync(mem + ctx->slot_off_bytes, ctx->h_col + e0, (e1 - e0) * 4,
-                                           hipMemcpyHostToDevice, ctx->copy_stream));
+                GW_HIP(ctx, hipMemcpyAsync(mem + col_at, ctx->h_col + e0, (e1 - e0) * 4, hipMemcpyHostToDevice,
+                                           ctx->copy_stream));
             st.h2d_bytes += (hi - lo + 1ull) * 8 + (e1 - e0) * 4;
             ++st.blocks_loaded;
             // rebased so the kernels index with absolute vertex ids / edge positions
             hv[k].off = reinterpret_cast<const uint64_t*>(reinterpret_cast<uintptr_t>(mem) - uintptr_t{lo} * 8);
-            hv[k].col = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(mem + ctx->slot_off_bytes) -
+            hv[k].col = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(mem + col_at) -
                                                           static_cast<uintptr_t>(e0) * 4);
             loaded.push_back(k);
         }
@@ -640,8 +666,8 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     };
     const uint32_t vbits = std::max(1u, bits(ctx->n ? ctx->n - 1ull : 0ull));
     const uint32_t sbits = bits(ctx->walk_src_max);
-    if (vbits + sbits > 44)
-        return gw_fail(ctx, GW_EINVAL, "gw_run: source field and vertex ids need more than 44 key bits");
+    if (vbits + sbits > 63)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: source field and vertex ids need more than 63 key bits");
     uint32_t lg = prog->table_log2;
     if (lg == 0) lg = std::max(12u, std::min(31u, bits(std::max<uint64_t>(1, ctx->nwalks) * 4 - 1)));
     if (lg < 4 || lg > 34) return gw_fail(ctx, GW_EINVAL, "gw_run: table_log2 out of range [4, 34]");
@@ -649,19 +675,23 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     ctx->table_vbits = vbits;
     ctx->table_cbits = 64 - vbits - sbits;
     if (prog->source_method != GW_SRC_HASH && !ctx->host_graph) return GW_OK;  // key log, no hash table
+    // Keys of at most 44 bits pack with a 20-bit count into one word; longer
+    // keys (a restart program's vertex-id source fields on a large graph)
+    // take 16-byte entries {key + 1, count}, both words in one sector.
+    const bool wide = vbits + sbits > 44;
     const uint64_t slots = 1ull << lg;
-    if (slots != ctx->table_slots) {
-        gw_free(ctx, ctx->d_table, ctx->table_slots * 8);
+    const uint64_t bytes = slots * (wide ? 16 : 8);
+    if (bytes != ctx->table_bytes) {
+        gw_free(ctx, ctx->d_table, ctx->table_bytes);
         ctx->d_table = nullptr;
-        ctx->table_slots = 0;
-        int rc = gw_alloc(ctx, (void**)&ctx->d_table, slots * 8, "source-count table");
+        ctx->table_slots = ctx->table_bytes = 0;
+        int rc = gw_alloc(ctx, (void**)&ctx->d_table, bytes, "source-count table");
         if (rc) return rc;
-        ctx->table_slots = slots;
+        ctx->table_bytes = bytes;
     }
-    ctx->table_log2 = lg;
-    ctx->table_vbits = vbits;
-    ctx->table_cbits = 64 - vbits - sbits;
-    GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, slots * 8, ctx->stream));
+    ctx->table_slots = slots;
+    ctx->table_wide = wide;
+    GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, bytes, ctx->stream));
     return GW_OK;
 }
 
@@ -742,7 +772,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
     L.table = SourceTable{ctx->d_table, ctx->d_counters + kCntTableOverflow, ctx->table_slots - 1, 64u - ctx->table_log2,
-                          ctx->table_vbits, ctx->table_cbits};
+                          ctx->table_vbits, ctx->table_cbits, ctx->table_wide ? 1u : 0u};
     L.klog = KeyLog{ctx->d_keys, ctx->d_counters + kCntLogCursor, ctx->d_counters + kCntLogOverflow, ctx->keys_cap, ctx->keys_sentinel,
                     ctx->table_vbits, keylog_slab};
     L.tb = prog->block_threads ? prog->block_threads : 256u;

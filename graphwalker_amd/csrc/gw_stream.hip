@@ -62,7 +62,7 @@ void gw_stream_reset(gw_ctx* ctx) {
     ctx->slot_mem.clear();
     ctx->slot_block.clear();
     ctx->block_slot.clear();
-    ctx->slot_bytes = ctx->slot_off_bytes = 0;
+    ctx->slot_bytes = 0;
     if (ctx->h_views) (void)hipHostFree(ctx->h_views);
     ctx->h_views = nullptr;
     gw_free(ctx, ctx->d_views, 0);
@@ -83,24 +83,15 @@ void gw_host_graph_free(gw_ctx* ctx) {
 int gw_stream_prepare(gw_ctx* ctx) {
     if (!ctx->slot_mem.empty()) return GW_OK;
     const uint32_t nb = ctx->nblocks;
-    uint64_t big = 0, big_off = 0;
-    for (uint32_t b = 0; b < nb; ++b) {
-        uint64_t ob = 0;
-        const uint64_t bytes = block_bytes(ctx, b, &ob);
-        big = std::max(big, align_up(bytes, kSlotAlign));
-        big_off = std::max(big_off, ob);
-    }
-    // a slot holds the largest offsets part followed by the largest column part
-    uint64_t max_cols = 0;
-    for (uint32_t b = 0; b < nb; ++b)
-        max_cols = std::max<uint64_t>(max_cols, ctx->h_off[ctx->block_first[b + 1]] - ctx->h_off[ctx->block_first[b]]);
-    const uint64_t slot = align_up(big_off + max_cols * 4, kSlotAlign);
+    // a slot holds the largest block: its offsets, then (256-byte aligned)
+    // its columns
+    uint64_t slot = 0;
+    for (uint32_t b = 0; b < nb; ++b) slot = std::max(slot, align_up(block_bytes(ctx, b, nullptr), kSlotAlign));
     const uint64_t nslots = std::min<uint64_t>(nb, ctx->cache_bytes / slot);
     if (nslots == 0)
         return gw_fail(ctx, GW_ENOMEM, "gw_run: the block cache (" + std::to_string(ctx->cache_bytes) +
                                            " bytes) is smaller than one block slot (" + std::to_string(slot) + " bytes)");
     ctx->slot_bytes = slot;
-    ctx->slot_off_bytes = big_off;
     for (uint64_t i = 0; i < nslots; ++i) {
         void* m = nullptr;
         int rc = gw_alloc(ctx, &m, slot, "block cache slot");
@@ -204,6 +195,37 @@ int gw_graph_set_host(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uin
         bf = cut_blocks(ctx, target);
     }
     return gw_graph_blocks(ctx, bf.data(), static_cast<uint32_t>(bf.size() - 1));
+}
+
+// Host-to-device copy ceiling: `bytes` from pinned host memory into HBM on
+// the copy stream (the path block loads take); copy time in ms.
+int gw_h2d_probe(gw_ctx* ctx, uint64_t bytes, double* ms) {
+    if (!ctx || bytes == 0) return gw_fail(ctx, GW_EINVAL, "gw_h2d_probe: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->copy_stream) GW_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    for (auto& e : ctx->copy_ev)
+        if (!e) GW_HIP(ctx, hipEventCreate(&e));
+    void* h = nullptr;
+    void* d = nullptr;
+    GW_HIP(ctx, hipHostMalloc(&h, bytes, hipHostMallocDefault));
+    std::memset(h, 1, bytes);
+    int rc = gw_alloc(ctx, &d, bytes, "h2d probe");
+    if (rc) {
+        (void)hipHostFree(h);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->copy_stream);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->copy_ev[1], ctx->copy_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->copy_ev[2], ctx->copy_stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->copy_ev[2]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->copy_ev[1], ctx->copy_ev[2]);
+    gw_free(ctx, d, bytes);
+    (void)hipHostFree(h);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "h2d probe");
+    if (ms) *ms = t;
+    return GW_OK;
 }
 
 }  // extern "C"

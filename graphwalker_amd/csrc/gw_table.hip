@@ -20,11 +20,25 @@
 
 namespace {
 
-__global__ void compact_kernel(const unsigned long long* __restrict__ tab, uint64_t slots,
-                               unsigned long long* __restrict__ out, unsigned long long* __restrict__ n) {
+// Occupied slots of the hash table -> (key, count) arrays, one atomic per
+// wavefront (ballot ranking).  Packed slots hold (key << cbits) | count; wide
+// slots hold {key + 1, count}.
+__global__ void compact_kernel(const unsigned long long* __restrict__ tab, uint64_t slots, uint32_t wide,
+                               uint32_t cbits, unsigned long long* __restrict__ out_k, uint32_t* __restrict__ out_c,
+                               unsigned long long* __restrict__ n) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    const bool full = i < slots && tab[i] != 0ull;
-    // one atomic per wavefront: rank the full slots with a ballot
+    unsigned long long k = 0, c = 0;
+    if (i < slots) {
+        if (wide) {
+            k = tab[2 * i];
+            c = tab[2 * i + 1];
+        } else {
+            const unsigned long long x = tab[i];
+            k = x ? (x >> cbits) + 1 : 0ull;  // + 1: 0 stays "empty"
+            c = x & ((1ull << cbits) - 1);
+        }
+    }
+    const bool full = k != 0ull;
     const uint64_t mask = __ballot(full);
     if (mask == 0) return;
     const uint32_t lane = __lane_id();
@@ -35,19 +49,9 @@ __global__ void compact_kernel(const unsigned long long* __restrict__ tab, uint6
     if (full) {
         const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-        out[base + r] = tab[i];
+        out_k[base + r] = k - 1;
+        out_c[base + r] = static_cast<uint32_t>(c);
     }
-}
-
-__global__ void unpack_kernel(const unsigned long long* __restrict__ e, uint64_t n, uint32_t vbits, uint32_t cbits,
-                              uint32_t* __restrict__ s, uint32_t* __restrict__ v, uint32_t* __restrict__ c) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    const unsigned long long x = e[i];
-    const uint64_t key = x >> cbits;
-    c[i] = static_cast<uint32_t>(x & ((1ull << cbits) - 1));
-    v[i] = static_cast<uint32_t>(key & ((1ull << vbits) - 1));
-    s[i] = static_cast<uint32_t>(key >> vbits);
 }
 
 __global__ void unpack_runs_kernel(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ counts,
@@ -716,22 +720,29 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
     }
     if (!ctx->d_table) return gw_fail(ctx, GW_ESTATE, "gw_source_counts_read: no run with GW_REC_SOURCE_COUNT");
     const uint64_t slots = ctx->table_slots;
-    unsigned long long *a = nullptr, *b = nullptr;
+    unsigned long long *ka = nullptr, *kb = nullptr;
+    uint32_t *ca = nullptr, *cb = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
     uint32_t* d_out = nullptr;
     int rc;
-    if ((rc = gw_alloc(ctx, (void**)&a, slots * 8, "table compaction"))) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ka, slots * 8, "table compaction")) ||
+        (rc = gw_alloc(ctx, (void**)&ca, slots * 4, "table compaction counts"))) {
+        gw_free(ctx, ka, slots * 8);
+        return rc;
+    }
     auto release = [&]() {
-        gw_free(ctx, a, slots * 8);
-        gw_free(ctx, b, slots * 8);
+        gw_free(ctx, ka, slots * 8);
+        gw_free(ctx, ca, slots * 4);
+        gw_free(ctx, kb, slots * 8);
+        gw_free(ctx, cb, slots * 4);
         gw_free(ctx, temp, temp_bytes);
         gw_free(ctx, d_out, slots * 12);
     };
     hipError_t e = hipMemsetAsync(ctx->d_counters + kCntCompact, 0, 8, ctx->stream);
     if (e == hipSuccess) {
         compact_kernel<<<dim3(static_cast<uint32_t>((slots + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_table, slots, a, ctx->d_counters + kCntCompact);
+            ctx->d_table, slots, ctx->table_wide ? 1u : 0u, ctx->table_cbits, ka, ca, ctx->d_counters + kCntCompact);
         e = hipGetLastError();
     }
     unsigned long long n = 0;
@@ -745,18 +756,22 @@ extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* ve
         return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: output arrays missing or smaller than the pair count");
     }
     if (n == 0) { release(); return GW_OK; }
-    if ((rc = gw_alloc(ctx, (void**)&b, slots * 8, "table sort"))) { release(); return rc; }
-    // Sort the whole 64-bit entry: keys are unique per slot, so this orders by
-    // key.  (rocPRIM's onesweep sort of 64-bit keys with begin_bit > 0 returns
-    // a non-permutation on gfx950 with ROCm 7.2: measured, so it is avoided.)
-    const int begin_bit = 0;
-    e = hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, a, b, n, begin_bit, 64, ctx->stream);
+    if ((rc = gw_alloc(ctx, (void**)&kb, slots * 8, "table sort")) ||
+        (rc = gw_alloc(ctx, (void**)&cb, slots * 4, "table sort counts"))) {
+        release();
+        return rc;
+    }
+    // Keys are unique per slot.  Sorted from bit 0: rocPRIM's onesweep sort
+    // of 64-bit keys with begin_bit > 0 returns a non-permutation on gfx950
+    // with ROCm 7.2 (measured), so it is avoided.
+    const int end_bit = static_cast<int>(64 - ctx->table_cbits);
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
     if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "table sort temp"))) { release(); return rc; }
-    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, a, b, n, begin_bit, 64, ctx->stream);
+    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
     if (e == hipSuccess && (rc = gw_alloc(ctx, (void**)&d_out, slots * 12, "table unpack"))) { release(); return rc; }
     if (e == hipSuccess) {
-        unpack_kernel<<<dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            b, n, ctx->table_vbits, ctx->table_cbits, d_out, d_out + n, d_out + 2 * n);
+        unpack_runs_kernel<<<dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            kb, cb, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(source, d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream);

@@ -261,6 +261,9 @@ int gw_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);
 /* Streaming copy of `bytes` bytes between two fresh buffers (the known-size
  * kernel hardware byte counters are calibrated with); kernel time in ms. */
 int gw_copy_probe(gw_ctx *ctx, uint64_t bytes, double *ms);
+/* Host-to-device copy of `bytes` from pinned host memory on the block-copy
+ * stream (the path out-of-core block loads take); copy time in ms. */
+int gw_h2d_probe(gw_ctx *ctx, uint64_t bytes, double *ms);
 
 #ifdef __cplusplus
 }

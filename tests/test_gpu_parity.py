@@ -315,3 +315,22 @@ def test_source_table_overflow_is_an_error(engine, small_graph):
     with pytest.raises(gw.GwError, match="table full"):
         engine.run(gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT, table_log2=6,
                               source_method=gw.SRC_HASH))
+
+
+@pytest.mark.parametrize("method", [gw.SRC_HASH, gw.SRC_SORT])
+def test_source_counts_wide_keys(engine, method):
+    # fields >= 2^31 with 13-bit vertex ids: 45-bit keys, past what a packed
+    # hash entry holds, so the table switches to two-word entries
+    off, col = O.rmat_graph(13, 1 << 17, seed=8)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    sources = O.sources(4, n, 6)
+    field = (np.uint32(0x80000000) + np.arange(6, dtype=np.uint32) * np.uint32(0x1000000)).astype(np.uint32)
+    wps = 400
+    engine.walks_init_sources(sources, wps, source_field=field)
+    prog = gw.program(gw.STOP, 20, 0.15, n, seed=12, recorder=gw.REC_SOURCE_COUNT, source_method=method)
+    st = engine.run(prog)
+    s, v, c = engine.source_counts()
+    es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
+    assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
+    assert int(c.sum()) == st["visits"]

@@ -64,8 +64,9 @@ def test_streamed_explicit_blocks_two_slots(engine, graph):
     bf = np.array([0, 100, 700, 1024, 2500, 3000, 3500, 4095, n], np.uint32)
     # room for exactly two slots of the largest block
     sizes = [(bf[k + 1] - bf[k] + 1) * 8 + (off[bf[k + 1]] - off[bf[k]]) * 4 for k in range(len(bf) - 1)]
-    slot = ((max((bf[k + 1] - bf[k] + 1) * 8 for k in range(len(bf) - 1)) + 255) // 256 * 256 +
-            max(int(off[bf[k + 1]] - off[bf[k]]) for k in range(len(bf) - 1)) * 4 + 255) // 256 * 256
+    # a slot holds the largest block: offsets (256-byte aligned), then columns
+    slot = max((((int(bf[k + 1] - bf[k]) + 1) * 8 + 255) // 256 * 256 + int(off[bf[k + 1]] - off[bf[k]]) * 4 + 255)
+               // 256 * 256 for k in range(len(bf) - 1))
     engine.graph_set_host(off, col, 2 * slot + 100)
     engine.graph_blocks(bf)
     walks = _walks(10_000, n, gw.RESTART, seed=5)
