@@ -302,10 +302,13 @@ def main() -> int:
     copy_bytes = 1 << 30
     cms = eng.copy_probe(copy_bytes)
 
-    traffic = None
+    # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (profiles/), calibrated against probe kernels of known size
+    traffic = traffic_copy = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.schedule}_{args.layout}.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        t = json.load(open(tfile))
+        traffic, traffic_copy = t.get("bytes_per_launch"), t.get("bytes_per_launch_copy_calibrated")
 
     workloads = {}
     if args.secondary and args.secondary != args.workload:
@@ -361,6 +364,7 @@ def main() -> int:
                    "schedule": args.schedule, "layout": args.layout, "device": info["name"]},
         "roofline": {"bound": "hbm", "achieved": m["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": m["achieved"] / PEAK_HBM_GBPS, "traffic": traffic,
+                     "traffic_copy_calibrated": traffic_copy,
                      "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": m["kernel_ms"],
                      "algorithmic_bytes_per_launch": m["algorithmic_bytes"],
                      "bytes_per_step": m["algorithmic_bytes"] / max(1, m["steps"])},
