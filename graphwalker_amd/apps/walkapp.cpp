@@ -70,7 +70,8 @@ int run_app(const std::string& file, vid_t N, wid_t R, hid_t L, vid_t source, bo
     unsigned long long blocksize_kb = get_option_long("blocksize_kb", 0);
     if (blocksize_kb == 0) blocksize_kb = program.compBlockSize(R);
     const bid_t nblocks = convert_if_notexists(file, blocksize_kb);
-    graphwalker_engine engine(file, blocksize_kb, nblocks, 1, m);
+    const bid_t nmblocks = static_cast<bid_t>(get_option_int("nmblocks", 1));
+    graphwalker_engine engine(file, blocksize_kb, nblocks, nmblocks, m);
     engine.run(program, 0.2f);
     const std::string out = get_option_string("out", "");
     if (!out.empty()) {

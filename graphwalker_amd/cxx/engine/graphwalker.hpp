@@ -5,8 +5,12 @@
 //   graphwalker_engine engine(file, blocksize_kb, nblocks, nmblocks, m);
 //   engine.run(program, prob);
 //
-// The constructor loads the preprocessed CSR and block table and makes them
-// resident in HBM (the whole graph: the device block cache is the HBM).
+// The constructor loads the preprocessed CSR and block table.  In HBM by
+// default (the whole graph: the device block cache is the HBM); with option
+// outofcore 1 -- or by default when the CSR would not fit three quarters of
+// the device memory -- the CSR stays in pinned host memory and runs step it
+// through a device block cache of nmblocks slots, the reference's memory
+// budget (graphwalker.hpp:84-100, findSubGraph / swapOut :168-251).
 // run() calls the program's startWalks, gathers the per-(thread, block)
 // buckets into device walk records, runs the program's step semantics on the
 // GPU with a per-walk event log, and replays every visit to the program's
@@ -14,7 +18,7 @@
 // per step, so app recorders see the same (source, vertex, hop) visits.
 //
 // Options (get_option_*): execthreads, schedule (persistent|blocked|plain),
-// seed (Philox key), device.
+// seed (Philox key), device, outofcore (-1 auto, 0 in HBM, 1 host-resident).
 #ifndef DEF_GRAPHCHWALKER_ENGINE
 #define DEF_GRAPHCHWALKER_ENGINE
 
@@ -110,6 +114,7 @@ public:
         const std::string sched = get_option_string("schedule", "persistent");
         prog.schedule = sched == "plain" ? GW_SCHED_PLAIN : sched == "blocked" ? GW_SCHED_BLOCKED : GW_SCHED_PERSISTENT;
         if (prog.schedule == GW_SCHED_BLOCKED && !blocks_on_device_) prog.schedule = GW_SCHED_PERSISTENT;
+        if (host_resident_) prog.schedule = GW_SCHED_BLOCKED;  // the block cache steps block by block
         prog.event_cap = dp.kind == GW_PROB ? 64u : std::max<uint32_t>(1u, dp.L);
 
         m.start_time("2_findSubGraph");
@@ -132,6 +137,10 @@ public:
         if (!walks.empty()) check(gw_events_read(ctx_, events.data(), walks.size(), prog.event_cap), "gw_events_read");
         m.set("walk_steps", static_cast<size_t>(st.steps));
         m.set("device_ms", st.total_ms);
+        if (host_resident_) {
+            m.set("blocks_loaded", static_cast<size_t>(st.blocks_loaded));
+            m.set("h2d_bytes", static_cast<size_t>(st.h2d_bytes));
+        }
 
         // Replay every visit to updateInfo, walk by walk, hop by hop.
         m.start_time("z_replay_updateInfo");
@@ -155,6 +164,7 @@ public:
 private:
     gw_ctx* ctx_;
     bool blocks_on_device_ = false;
+    bool host_resident_ = false;
 
     void check(int rc, const char* what) {
         if (rc != GW_OK) logstream(LOG_FATAL) << what << " failed (" << rc << "): " << gw_last_error(ctx_) << std::endl;
@@ -172,26 +182,71 @@ private:
 
     void load_graph() {
         const std::string base = fidname(base_filename, 0);
-        std::vector<eid_t> off;
-        std::vector<vid_t> col;
-        read_file(base + ".beg_pos", off);
-        read_file(base + ".csr", col);
-        if (off.size() < 2) logstream(LOG_FATAL) << "empty beg_pos file " << base << ".beg_pos" << std::endl;
-        nvertices = static_cast<vid_t>(off.size() - 1);
         gw_cfg cfg;
         memset(&cfg, 0, sizeof cfg);
         cfg.device = get_option_int("device", 0);
         if (gw_create(&ctx_, &cfg) != GW_OK) logstream(LOG_FATAL) << "no usable HIP device" << std::endl;
-        m.start_time("g_loadSubGraph");
-        check(gw_graph_set(ctx_, off.data(), col.data(), nvertices, col.size()), "gw_graph_set");
-        m.stop_time("g_loadSubGraph");
+        const size_t off_bytes = file_bytes(base + ".beg_pos"), col_bytes = file_bytes(base + ".csr");
+        if (off_bytes < 2 * sizeof(eid_t)) logstream(LOG_FATAL) << "empty beg_pos file " << base << ".beg_pos" << std::endl;
+        nvertices = static_cast<vid_t>(off_bytes / sizeof(eid_t) - 1);
+        const uint64_t m_edges = col_bytes / sizeof(vid_t);
+        int ooc = get_option_int("outofcore", -1);
+        if (ooc < 0) {
+            uint64_t hbm = 0;
+            check(gw_device_info(ctx_, nullptr, 0, nullptr, &hbm), "gw_device_info");
+            ooc = off_bytes + col_bytes > hbm / 4 * 3 ? 1 : 0;
+        }
         // The block table of the reference cut rule may hold empty blocks;
         // the device block loop takes non-empty ones only.
         std::vector<uint32_t> bf;
         for (bid_t b = 0; b <= nblocks; ++b)
             if (bf.empty() || blocks[b] > bf.back()) bf.push_back(blocks[b]);
-        if (bf.back() == nvertices && bf.size() >= 2 && bf.size() - 1 <= 2048)
-            blocks_on_device_ = gw_graph_blocks(ctx_, bf.data(), static_cast<uint32_t>(bf.size() - 1)) == GW_OK;
+        m.start_time("g_loadSubGraph");
+        if (ooc) {
+            // read straight into pinned memory the engine uses in place
+            eid_t* off = nullptr;
+            vid_t* col = nullptr;
+            check(gw_host_alloc(ctx_, off_bytes, reinterpret_cast<void**>(&off)), "gw_host_alloc");
+            check(gw_host_alloc(ctx_, std::max<size_t>(col_bytes, 4), reinterpret_cast<void**>(&col)), "gw_host_alloc");
+            read_into(base + ".beg_pos", off, off_bytes);
+            read_into(base + ".csr", col, col_bytes);
+            // nmblocks slots of the largest block (offsets 256-byte aligned, then columns)
+            uint64_t slot = 0;
+            for (size_t b = 0; b + 1 < bf.size(); ++b) {
+                const uint64_t ob = ((bf[b + 1] - bf[b] + 1ull) * 8 + 255) / 256 * 256;
+                slot = std::max<uint64_t>(slot, (ob + (off[bf[b + 1]] - off[bf[b]]) * 4 + 255) / 256 * 256);
+            }
+            const uint64_t cache = std::max<uint64_t>(1, nmblocks) * std::max<uint64_t>(slot, 256);
+            check(gw_graph_set_host(ctx_, off, col, nvertices, m_edges, cache), "gw_graph_set_host");
+            host_resident_ = true;
+            if (bf.back() == nvertices && bf.size() >= 2 && bf.size() - 1 <= 2048)
+                check(gw_graph_blocks(ctx_, bf.data(), static_cast<uint32_t>(bf.size() - 1)), "gw_graph_blocks");
+            blocks_on_device_ = true;
+        } else {
+            std::vector<eid_t> off;
+            std::vector<vid_t> col;
+            read_file(base + ".beg_pos", off);
+            read_file(base + ".csr", col);
+            check(gw_graph_set(ctx_, off.data(), col.data(), nvertices, col.size()), "gw_graph_set");
+            if (bf.back() == nvertices && bf.size() >= 2 && bf.size() - 1 <= 2048)
+                blocks_on_device_ = gw_graph_blocks(ctx_, bf.data(), static_cast<uint32_t>(bf.size() - 1)) == GW_OK;
+        }
+        m.stop_time("g_loadSubGraph");
+    }
+
+    static size_t file_bytes(const std::string& f) {
+        int fd = open(f.c_str(), O_RDONLY);
+        if (fd < 0) logstream(LOG_FATAL) << "Could not load :" << f << ", error: " << strerror(errno) << std::endl;
+        const off_t bytes = lseek(fd, 0, SEEK_END);
+        close(fd);
+        return static_cast<size_t>(bytes);
+    }
+
+    static void read_into(const std::string& f, void* out, size_t bytes) {
+        int fd = open(f.c_str(), O_RDONLY);
+        if (fd < 0) logstream(LOG_FATAL) << "Could not load :" << f << ", error: " << strerror(errno) << std::endl;
+        if (bytes) preada(fd, static_cast<char*>(out), bytes, 0);
+        close(fd);
     }
 
     template <class T>

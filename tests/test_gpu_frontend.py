@@ -55,6 +55,24 @@ def test_walkapp_counts_bit_exact(graph, prog, schedule):
     assert f"walk_visits {ref['visits']}" in stdout
 
 
+@pytest.mark.parametrize("prog", ["restart", "jump"])
+@pytest.mark.parametrize("nmblocks", [1, 3])
+def test_walkapp_out_of_core(graph, prog, nmblocks):
+    # outofcore 1: the CSR stays in pinned host memory and runs step it
+    # through a device cache of nmblocks blocks (the reference's memory budget)
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    R, L, src0 = 4000, 10, 3
+    out = str(d / f"counts_ooc_{prog}_{nmblocks}.u32")
+    run([os.path.join(BIN, "walkapp"), "file", base, "program", prog, "R", str(R), "L", str(L), "source", str(src0),
+         "blocksize_kb", "16", "nmblocks", str(nmblocks), "outofcore", "1", "seed", "8", "out", out], d)
+    counts = np.fromfile(out, dtype=np.uint32)
+    starts = np.sort((src0 + np.arange(R)) % n).astype(np.uint32)
+    walks = O.make_walks(starts, sources_=starts, ids=np.arange(R))
+    ref = O.replay(O.program(KINDS[prog], L, 0.15, n, seed=8), off, col, walks)
+    assert np.array_equal(counts, ref["counts"])
+
+
 @pytest.mark.skipif(not os.access(os.path.join(COMPAT, "simrank"), os.X_OK), reason="compat apps not built")
 def test_reference_simrank_unchanged(graph):
     """apps/simrank.cpp of the reference, compiled unchanged against the
