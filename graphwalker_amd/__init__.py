@@ -161,6 +161,9 @@ def lib() -> C.CDLL:
         "gw_device_info": ([P, C.c_char_p, C.c_size_t, P, P], i),
         "gw_graph_set": ([P, P, P, u32, u64], i),
         "gw_graph_set_host": ([P, P, P, u32, u64, u64], i),
+        "gw_graph_transpose": ([P], i),
+        "gw_walks_init_pairs": ([P, P, P, u32, u32, u32], i),
+        "gw_meeting_counts": ([P, u32, u32, u32, P], i),
         "gw_host_alloc": ([P, u64, P], i),
         "gw_host_free": ([P, P], i),
         "gw_graph_rmat": ([P, u32, u64, u64, u32, u32, u32], i),
@@ -281,6 +284,10 @@ class Engine:
         self._check(self._L.gw_graph_rmat(self._ctx, scale, m, seed, ta, tab, tabc), "gw_graph_rmat")
         self.n, self.m = 1 << scale, m
 
+    def graph_transpose(self) -> None:
+        """Reverse every edge of the device-resident graph."""
+        self._check(self._L.gw_graph_transpose(self._ctx), "gw_graph_transpose")
+
     def graph_info(self) -> dict:
         n, m, nb = C.c_uint32(), C.c_uint64(), C.c_uint32()
         self._check(self._L.gw_graph_info(self._ctx, C.byref(n), C.byref(m), C.byref(nb)), "gw_graph_info")
@@ -318,6 +325,21 @@ class Engine:
         self._check(self._L.gw_walks_init_sources(self._ctx, _ptr(s), _ptr(f), s.shape[0], walks_per_source),
                     "gw_walks_init_sources")
         self.nwalks = s.shape[0] * walks_per_source
+
+    def walks_init_pairs(self, a, b, R: int, L: int) -> None:
+        """SimRank placement: R walks from a[k] then R from b[k] per pair, walk
+        i at hop i*L (ids 2kR + i and 2kR + R + i)."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        self._check(self._L.gw_walks_init_pairs(self._ctx, _ptr(a), _ptr(b), a.shape[0], R, L), "gw_walks_init_pairs")
+        self.nwalks = 2 * a.shape[0] * R
+
+    def meeting_counts(self, npairs: int, R: int, L: int) -> np.ndarray:
+        """First-meeting counts per pair and offset (npairs x L) of the last
+        GW_RESTART_JOINT run with GW_REC_EVENTS over walks_init_pairs walks."""
+        out = np.zeros((npairs, L), dtype=np.uint32)
+        self._check(self._L.gw_meeting_counts(self._ctx, npairs, R, L, _ptr(out)), "gw_meeting_counts")
+        return out
 
     def walks_final(self) -> np.ndarray:
         """Final walk records, indexed by walk id."""

@@ -152,6 +152,8 @@ __global__ void copy_probe_kernel(const uint4* __restrict__ a, uint4* __restrict
 
 }  // namespace
 
+int gw_reserve_walks(gw_ctx* ctx, uint64_t n) { return reserve_walks(ctx, n); }
+
 extern "C" {
 
 int gw_copy_probe(gw_ctx* ctx, uint64_t bytes, double* ms) {
@@ -281,6 +283,17 @@ int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     if (rc) return rc;
     if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, ctx->n * 4ull, "vertex counts"))) return rc;
     return single_block(ctx);
+}
+
+int gw_graph_transpose(gw_ctx* ctx) {
+    if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_transpose: needs a device-resident graph");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // the successor-header edges describe the old edges
+    gw_free(ctx, ctx->d_ext, std::max<uint64_t>(1, ctx->m) * 16);
+    ctx->d_ext = nullptr;
+    ctx->ext_failed = false;
+    return gw_transpose_build(ctx);
 }
 
 int gw_graph_blocks(gw_ctx* ctx, const uint32_t* block_first, uint32_t nblocks) {

@@ -129,6 +129,8 @@ struct gw_ctx {
 inline bool gw_has_graph(const gw_ctx* ctx) { return ctx->d_off != nullptr || ctx->host_graph; }
 // Frees the graph (device-resident or host-resident) and its block table.
 void gw_graph_free(gw_ctx* ctx);
+// Walk buffers for n walks (nwalks = n), reallocated only when they grow.
+int gw_reserve_walks(gw_ctx* ctx, uint64_t n);
 
 // Record an error message on the context and return `code`.
 int gw_fail(gw_ctx* ctx, int code, const std::string& msg);
@@ -145,6 +147,9 @@ void gw_free(gw_ctx* ctx, void* p, uint64_t bytes);
 // gw_rmat.hip
 int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
                   uint32_t tabc);
+
+// gw_rmat.hip: reverse every edge of the device-resident CSR in place
+int gw_transpose_build(gw_ctx* ctx);
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);

@@ -1,0 +1,148 @@
+// gw_meet.hip -- SimRank meeting counts on the device.
+//
+// The reference's simrank app (apps/simrank.cpp) places R walks from a and R
+// from b, walk i starting at hop i*L with the restart-with-joint program
+// (:30-59), records each walk's vertex per hop in updateInfo (:60-69), and
+// computeResult (:71-84) adds c^l / R^2 for every walk pair (i, j) whose paths
+// first coincide at offset l.  Here the walks of many pairs run in one device
+// run with the per-walk event log, and one workgroup per pair counts the first
+// meetings per offset: counts[k][l].  SimRank(a_k, b_k) = sum_l counts[k][l] c^l / R^2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "gw_device.h"
+#include "gw_internal.h"
+
+namespace {
+
+constexpr uint32_t kMeetThreads = 256;
+constexpr uint32_t kMaxPath = 31;        // visited offsets per walk (L - 1), registers per lane
+constexpr uint32_t kTileWords = 12288;   // LDS tile of b-side paths (48 KiB)
+
+__global__ void init_pairs_kernel(gw_walk* __restrict__ w, const uint32_t* __restrict__ a,
+                                  const uint32_t* __restrict__ b, uint64_t nw, uint32_t R, uint32_t L) {
+    const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (id >= nw) return;
+    const uint64_t k = id / (2ull * R);
+    const uint32_t r = static_cast<uint32_t>(id - k * 2ull * R);
+    const uint32_t v = r < R ? a[k] : b[k];
+    const uint32_t i = r < R ? r : r - R;
+    w[id] = gw_walk{static_cast<uint32_t>(id), v, v, i * L};
+}
+
+// One workgroup per pair k.  The b-side paths are staged in LDS tiles; each
+// lane holds one a-side path in registers and scans the tile's paths (the
+// same path for every lane, an LDS broadcast), recording the first offset at
+// which the two paths hold the same vertex.  Offsets never visited hold no
+// vertex and never meet (the reference's 0xffffffff fill).
+__global__ __launch_bounds__(kMeetThreads) void meeting_kernel(const unsigned long long* __restrict__ ev,
+                                                              uint32_t cap, uint32_t R, uint32_t P,
+                                                              const uint32_t* __restrict__ visits,
+                                                              uint32_t* __restrict__ counts, uint32_t L) {
+    __shared__ uint32_t s_b[kTileWords];
+    __shared__ uint32_t s_cnt[kMaxPath + 1];
+    const uint32_t k = blockIdx.x;
+    const uint64_t abase = 2ull * k * R, bbase = abase + R;
+    const uint32_t tile = kTileWords / P;  // b-side walks per tile
+    if (threadIdx.x <= kMaxPath) s_cnt[threadIdx.x] = 0;
+    uint32_t my[kMaxPath];
+#pragma unroll
+    for (uint32_t l = 0; l < kMaxPath; ++l) my[l] = 0;
+    for (uint32_t t0 = 0; t0 < R; t0 += tile) {
+        const uint32_t tn = std::min(tile, R - t0);
+        __syncthreads();  // the previous tile is consumed
+        for (uint32_t x = threadIdx.x; x < tn * P; x += blockDim.x) {
+            const uint32_t j = x / P, l = x - j * P;
+            const uint64_t id = bbase + t0 + j;
+            s_b[x] = l < visits[id] ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
+            const uint64_t id = abase + i;
+            const uint32_t nv = visits[id];
+            uint32_t a[kMaxPath];
+#pragma unroll
+            for (uint32_t l = 0; l < kMaxPath; ++l)
+                a[l] = (l < P && l < nv) ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFEu;  // never equal to an unvisited b offset
+            for (uint32_t j = 0; j < tn; ++j) {
+                const uint32_t* bp = s_b + j * P;
+                bool met = false;
+#pragma unroll
+                for (uint32_t l = 0; l < kMaxPath; ++l) {
+                    if (l < P && !met && a[l] == bp[l]) {
+                        ++my[l];
+                        met = true;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t l = 0; l < kMaxPath; ++l)
+        if (l < P && my[l]) atomicAdd(&s_cnt[l], my[l]);
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < L; l += blockDim.x) counts[static_cast<uint64_t>(k) * L + l] = l < P ? s_cnt[l] : 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_walks_init_pairs(gw_ctx* ctx, const uint32_t* a, const uint32_t* b, uint32_t npairs, uint32_t R, uint32_t L) {
+    if (!ctx || !a || !b || npairs == 0 || R == 0 || L == 0)
+        return gw_fail(ctx, GW_EINVAL, "gw_walks_init_pairs: bad arguments");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_pairs: no graph");
+    const uint64_t nw = 2ull * npairs * R;
+    if (nw >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_pairs: walk ids are 32-bit");
+    if (static_cast<uint64_t>(R) * L >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_pairs: R * L overflows the hop field");
+    uint32_t src_max = 0;
+    for (uint32_t k = 0; k < npairs; ++k) {
+        if (a[k] >= ctx->n || b[k] >= ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_pairs: vertex out of range");
+        src_max = std::max(src_max, std::max(a[k], b[k]));
+    }
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = gw_reserve_walks(ctx, nw);
+    if (rc) return rc;
+    uint32_t* d_ab = nullptr;
+    if ((rc = gw_alloc(ctx, (void**)&d_ab, npairs * 8ull, "pair table"))) return rc;
+    hipError_t e = hipMemcpyAsync(d_ab, a, npairs * 4ull, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ab + npairs, b, npairs * 4ull, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        init_pairs_kernel<<<dim3(static_cast<uint32_t>((nw + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+            ctx->d_walks, d_ab, d_ab + npairs, nw, R, L);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    gw_free(ctx, d_ab, npairs * 8ull);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "init_pairs_kernel");
+    ctx->walk_src_max = src_max;
+    ctx->field_walks.clear();
+    ctx->part_per_walk = 0;
+    return GW_OK;
+}
+
+int gw_meeting_counts(gw_ctx* ctx, uint32_t npairs, uint32_t R, uint32_t L, uint32_t* counts) {
+    if (!ctx || !counts || npairs == 0 || R == 0 || L < 2) return gw_fail(ctx, GW_EINVAL, "gw_meeting_counts: bad arguments");
+    const uint32_t P = L - 1;
+    if (P > kMaxPath) return gw_fail(ctx, GW_EINVAL, "gw_meeting_counts: L - 1 must be at most 31");
+    if (ctx->nwalks != 2ull * npairs * R)
+        return gw_fail(ctx, GW_ESTATE, "gw_meeting_counts: the loaded walks are not npairs x 2R pair walks");
+    if (!ctx->d_events || !ctx->d_visits || ctx->events_cap < P)
+        return gw_fail(ctx, GW_ESTATE, "gw_meeting_counts: needs a run with GW_REC_EVENTS and event_cap >= L - 1");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t* d_counts = nullptr;
+    int rc = gw_alloc(ctx, (void**)&d_counts, static_cast<uint64_t>(npairs) * L * 4, "meeting counts");
+    if (rc) return rc;
+    meeting_kernel<<<dim3(npairs), dim3(kMeetThreads), 0, ctx->stream>>>(
+        reinterpret_cast<const unsigned long long*>(ctx->d_events), ctx->events_cap, R, P, ctx->d_visits, d_counts, L);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts, d_counts, static_cast<uint64_t>(npairs) * L * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    gw_free(ctx, d_counts, static_cast<uint64_t>(npairs) * L * 4);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "meeting_kernel");
+    return GW_OK;
+}
+
+}  // extern "C"

@@ -9,8 +9,12 @@
 //      sort, off the walk path);
 //   3. col[e] = low `scale` bits of key e; off[v] = first e with src(e) >= v, written
 //      by the edge that starts each run (no atomics, deterministic).
+// gw_graph_transpose reverses every edge of the resident CSR with the same
+// sort + build steps (SimRank walks the in-links).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 
 #include "gw_device.h"
 #include "gw_internal.h"
@@ -36,9 +40,9 @@ __global__ void rmat_keys_kernel(uint64_t* __restrict__ keys, uint64_t m, uint32
     keys[e] = (static_cast<uint64_t>(s) << scale) | d;
 }
 
-__global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t scale,
+// Keys hold (src << bits) | dst, sorted; n vertices.
+__global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t n,
                                        uint64_t* __restrict__ off, uint32_t* __restrict__ col) {
-    const uint32_t n = 1u << scale;
     const uint64_t dmask = (1ull << scale) - 1;
     const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (e > m) return;
@@ -48,6 +52,26 @@ __global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64
     if (e < m) col[e] = static_cast<uint32_t>(keys[e] & dmask);
     for (int64_t v = prev + 1; v <= static_cast<int64_t>(s); ++v) off[v] = e;
 }
+
+// Row start markers: marker[off[v]] = v for every non-empty row v (rows are
+// sorted, so an inclusive max-scan of the markers gives each edge's source).
+__global__ void row_marker_kernel(const uint64_t* __restrict__ off, uint32_t n, uint32_t* __restrict__ marker) {
+    const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (v >= n) return;
+    if (off[v + 1] > off[v]) marker[off[v]] = static_cast<uint32_t>(v);
+}
+
+// Reversed-edge keys (dst << bits) | src.
+__global__ void transpose_keys_kernel(const uint32_t* __restrict__ col, const uint32_t* __restrict__ src, uint64_t m,
+                                      uint32_t bits, uint64_t* __restrict__ keys) {
+    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (e >= m) return;
+    keys[e] = (static_cast<uint64_t>(col[e]) << bits) | src[e];
+}
+
+struct MaxOp {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
 
 }  // namespace
 
@@ -90,10 +114,76 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     ctx->n = n;
     ctx->m = m;
     csr_from_sorted_kernel<<<dim3(static_cast<uint32_t>((m + 1 + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
-        k_out, m, scale, ctx->d_off, ctx->d_col);
+        k_out, m, scale, n, ctx->d_off, ctx->d_col);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     cleanup();
     if (e != hipSuccess) return gw_hip_check(ctx, e, "csr build");
+    return GW_OK;
+}
+
+// Reverses every edge of the device-resident CSR: keys (dst << bits) | src
+// from the row markers' max-scan, radix sort, CSR build.  Rows of the result
+// list in-neighbours in ascending order, with duplicates kept.
+int gw_transpose_build(gw_ctx* ctx) {
+    const uint32_t n = ctx->n;
+    const uint64_t m = ctx->m;
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < n) ++bits;
+    int rc;
+    uint32_t* src = nullptr;
+    uint64_t *k_in = nullptr, *k_out = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    auto cleanup = [&]() {
+        gw_free(ctx, src, m * 4);
+        gw_free(ctx, k_in, m * 8);
+        gw_free(ctx, k_out, m * 8);
+        gw_free(ctx, temp, temp_bytes);
+    };
+    const uint32_t tb = 256;
+    hipError_t e = hipSuccess;
+    if (m) {
+        if ((rc = gw_alloc(ctx, (void**)&src, m * 4, "transpose sources")) ||
+            (rc = gw_alloc(ctx, (void**)&k_in, m * 8, "transpose keys")) ||
+            (rc = gw_alloc(ctx, (void**)&k_out, m * 8, "transpose keys (sorted)"))) {
+            cleanup();
+            return rc;
+        }
+        e = hipMemsetAsync(src, 0, m * 4, ctx->stream);
+        if (e == hipSuccess) {
+            row_marker_kernel<<<dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream>>>(ctx->d_off, n, src);
+            e = hipGetLastError();
+        }
+        // edge sources: max-scan of the markers into the (still unused) sort
+        // output, reversed keys into the sort input, then the sort
+        uint32_t* esrc = reinterpret_cast<uint32_t*>(k_out);
+        size_t need_scan = 0, need_sort = 0;
+        if (e == hipSuccess) e = hipcub::DeviceScan::InclusiveScan(nullptr, need_scan, src, esrc, MaxOp(), m, ctx->stream);
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortKeys(nullptr, need_sort, k_in, k_out, m, 0, 2 * bits, ctx->stream);
+        temp_bytes = std::max(need_scan, need_sort);
+        if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "transpose temp"))) {
+            cleanup();
+            return rc;
+        }
+        size_t t = temp_bytes;
+        if (e == hipSuccess) e = hipcub::DeviceScan::InclusiveScan(temp, t, src, esrc, MaxOp(), m, ctx->stream);
+        if (e == hipSuccess) {
+            transpose_keys_kernel<<<dim3(static_cast<uint32_t>((m + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+                ctx->d_col, esrc, m, bits, k_in);
+            e = hipGetLastError();
+        }
+        t = temp_bytes;
+        if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(temp, t, k_in, k_out, m, 0, 2 * bits, ctx->stream);
+    }
+    if (e == hipSuccess) {
+        csr_from_sorted_kernel<<<dim3(static_cast<uint32_t>((m + 1 + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+            k_out, m, bits, n, ctx->d_off, ctx->d_col);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "transpose");
     return GW_OK;
 }

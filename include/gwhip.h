@@ -44,6 +44,14 @@
  *                              (msppr.cpp:71-76, discretedistribution.hpp:44-57), exact
  *   gw_events_read             the per-step updateInfo callback (randomwalk.hpp:32):
  *                              each walk's visits, replayed on the host in walk order
+ *   gw_graph_transpose         (no counterpart: the reference's SimRank reads a
+ *                              graph the user transposed; this reverses the
+ *                              resident CSR on the device)
+ *   gw_walks_init_pairs        simrank's placement: R walks from a and R from b,
+ *                              walk i at hop i*L (apps/simrank.cpp:30-59)
+ *   gw_meeting_counts          simrank's path recorder + computeResult
+ *                              (apps/simrank.cpp:60-84): first-meeting counts
+ *                              per walk-pair offset, for many pairs at once
  *   gw_walks_read / gw_summary_read   final walk records and per-walk visit
  *                              summaries (no reference counterpart; test access)
  */
@@ -166,6 +174,9 @@ int gw_graph_set(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uint32_t
 int gw_graph_rmat(gw_ctx *ctx, uint32_t scale, uint64_t m, uint64_t seed,
                   uint32_t ta, uint32_t tab, uint32_t tabc);
 int gw_graph_blocks(gw_ctx *ctx, const uint32_t *block_first, uint32_t nblocks);
+/* Reverse every edge of the device-resident graph (rows become in-neighbour
+ * lists, ascending, duplicates kept). */
+int gw_graph_transpose(gw_ctx *ctx);
 int gw_graph_info(gw_ctx *ctx, uint32_t *n, uint64_t *m, uint32_t *nblocks);
 int gw_graph_read(gw_ctx *ctx, uint64_t *off, uint32_t *col);
 
@@ -184,6 +195,15 @@ int gw_graph_set_host(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uin
 
 /* Walks: records with ids < n (ids key the random streams). */
 int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
+/* SimRank placement for npairs pairs (a[k], b[k]): walk ids 2kR + i (from
+ * a[k]) and 2kR + R + i (from b[k]), i < R, each starting at hop i*L with its
+ * start vertex as source field.  Run them with GW_RESTART_JOINT, period L,
+ * and GW_REC_EVENTS with event_cap >= L - 1, then call gw_meeting_counts:
+ * counts[k*L + l] = walk pairs (i, j) of pair k whose paths first hold the
+ * same vertex at offset l, so SimRank(a[k], b[k]) = sum_l counts[k*L+l] c^l / R^2.
+ * L - 1 <= 31. */
+int gw_walks_init_pairs(gw_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t npairs, uint32_t R, uint32_t L);
+int gw_meeting_counts(gw_ctx *ctx, uint32_t npairs, uint32_t R, uint32_t L, uint32_t *counts);
 /* count walks per vertex: walk i = vertex i / per_vertex, source = vertex, id = i */
 int gw_walks_init_per_vertex(gw_ctx *ctx, uint32_t per_vertex);
 /* walks_per_source walks per source: id = k * walks_per_source + j, source =

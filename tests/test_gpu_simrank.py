@@ -1,0 +1,107 @@
+"""SimRank on the device: gw_graph_transpose, gw_walks_init_pairs and
+gw_meeting_counts against the oracle's trajectories and a brute-force
+first-meeting count (apps/simrank.cpp:71-84)."""
+import numpy as np
+import pytest
+
+import graphwalker_amd as gw
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _transpose(off, col):
+    n = off.shape[0] - 1
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    order = np.lexsort((src, col))
+    toff = np.zeros(n + 1, dtype=np.uint64)
+    toff[1:] = np.cumsum(np.bincount(col, minlength=n))
+    return toff, src[order].astype(np.uint32)
+
+
+@pytest.mark.parametrize("scale,m,seed", [(4, 40, 3), (12, 1 << 16, 7), (16, 1 << 20, 2)])
+def test_transpose_equals_numpy(engine, scale, m, seed):
+    off, col = O.rmat_graph(scale, m, seed=seed)
+    engine.graph_set(off, col)
+    engine.graph_transpose()
+    toff, tcol = engine.graph_read()
+    eoff, ecol = _transpose(off, col)
+    assert np.array_equal(toff, eoff) and np.array_equal(tcol, ecol)
+    engine.graph_transpose()  # twice = the original graph
+    roff, rcol = engine.graph_read()
+    assert np.array_equal(roff, off) and np.array_equal(rcol, col)
+
+
+def _paths(prog, off, col, start, R, L, id0):
+    """Path by offset of walks id0.. id0+R-1 from `start` (hop i*L)."""
+    P = L - 1
+    out = np.full((R, P), 0xFFFFFFFF, dtype=np.uint64)
+    for i in range(R):
+        w = O.make_walks([start], sources_=[start], hops=[i * L], ids=[id0 + i])[0]
+        tv, th, _ = O.trace(prog, off, col, w)
+        out[i, (th - i * L).astype(np.int64)] = tv
+    return out
+
+
+def _first_meetings(A, B, L):
+    counts = np.zeros(L, dtype=np.uint64)
+    eq = (A[:, None, :] == B[None, :, :]) & (A[:, None, :] != 0xFFFFFFFF)
+    any_ = eq.any(axis=2)
+    first = eq.argmax(axis=2)
+    np.add.at(counts, first[any_], 1)
+    return counts
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+def test_meeting_counts_equal_brute_force(engine, transposed):
+    off, col = O.rmat_graph(10, 1 << 13, seed=4)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    if transposed:
+        engine.graph_transpose()
+        off, col = _transpose(off, col)
+    R, L = 60, 11
+    pairs = O.sources(5, n, 6).reshape(3, 2)
+    a, b = pairs[:, 0].copy(), pairs[:, 1].copy()
+    a[2] = b[2]  # a pair of one vertex meets at offset 0 every time
+    engine.walks_init_pairs(a, b, R, L)
+    prog = gw.program(gw.RESTART_JOINT, L, 0.15, n, seed=6, recorder=gw.REC_EVENTS, event_cap=L - 1)
+    st = engine.run(prog)
+    assert st["steps"] == 3 * 2 * R * (L - 1)
+    got = engine.meeting_counts(3, R, L)
+    op = O.Program(prog.kind, prog.L, prog.stop_thr, prog.nvertices, prog.stream, prog.seed)
+    for k in range(3):
+        A = _paths(op, off, col, a[k], R, L, 2 * k * R)
+        B = _paths(op, off, col, b[k], R, L, 2 * k * R + R)
+        assert np.array_equal(got[k], _first_meetings(A, B, L)), k
+    assert got[2, 0] == R * R
+
+
+def test_meeting_counts_large_tiles(engine):
+    # R large enough that the b-side paths take several LDS tiles
+    off, col = O.rmat_graph(9, 1 << 12, seed=2)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    R, L = 1500, 11
+    a, b = np.array([3], np.uint32), np.array([7], np.uint32)
+    engine.walks_init_pairs(a, b, R, L)
+    prog = gw.program(gw.RESTART_JOINT, L, 0.15, n, seed=9, recorder=gw.REC_EVENTS, event_cap=L - 1)
+    engine.run(prog)
+    got = engine.meeting_counts(1, R, L)
+    op = O.Program(prog.kind, prog.L, prog.stop_thr, prog.nvertices, prog.stream, prog.seed)
+    A = _paths(op, off, col, 3, R, L, 0)
+    B = _paths(op, off, col, 7, R, L, R)
+    assert np.array_equal(got[0], _first_meetings(A, B, L))
+
+
+def test_meeting_counts_invalid(engine):
+    off, col = O.rmat_graph(9, 1 << 12, seed=2)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.walks_init_pairs(np.array([1], np.uint32), np.array([2], np.uint32), 10, 11)
+    with pytest.raises(gw.GwError, match="npairs x 2R"):
+        engine.meeting_counts(2, 10, 11)
+    with pytest.raises(gw.GwError, match="at most 31"):
+        engine.meeting_counts(1, 10, 40)
+    with pytest.raises(gw.GwError, match="out of range"):
+        engine.walks_init_pairs(np.array([n], np.uint32), np.array([2], np.uint32), 10, 11)
