@@ -262,6 +262,47 @@ def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, quer
     return out
 
 
+def simrank_measure(eng: gw.Engine, scale: int, seed: int, npairs: int, R: int, L: int, c: float, K: int) -> dict:
+    """Config 4: SimRank meeting counts on a transposed R-MAT graph (edge
+    factor 16): npairs uniform pairs, R walks per endpoint with the
+    restart-with-joint program (p 0.15, period L), first-meeting counts per
+    offset joined on the device; SimRank = sum_l counts[l] c^l / R^2."""
+    n = 1 << scale
+    t0 = time.perf_counter()
+    eng.graph_rmat(scale, n * 16, seed)
+    eng.graph_transpose()
+    build_s = time.perf_counter() - t0
+    ab = gw.sources(seed, n, 2 * npairs).reshape(npairs, 2)
+    a, b = ab[:, 0].copy(), ab[:, 1].copy()
+    prog = gw.program(gw.RESTART_JOINT, L, 0.15, n, seed=seed, recorder=gw.REC_EVENTS, event_cap=L - 1,
+                      schedule="persistent", layout="ext")
+    eng.walks_init_pairs(a, b, R, L)
+    eng.run(prog)
+    eng.meeting_counts(npairs, R, L)
+    steps = 0
+    run_ms = []
+    t = time.perf_counter()
+    for _ in range(K):
+        st = eng.run(prog)
+        counts = eng.meeting_counts(npairs, R, L)
+        steps += st["steps"]
+        run_ms.append(st["total_ms"])
+    wall = time.perf_counter() - t
+    sim = (counts.astype(np.float64) * (c ** np.arange(L))).sum(axis=1) / (R * R)
+    out = {
+        "desc": f"R-MAT scale {scale} ef 16 seed {seed}, transposed; {npairs} uniform pairs x {R} walks per endpoint, "
+                f"restart-with-joint p 0.15, L {L}, c {c}; first-meeting counts joined on the device",
+        "walk_steps_per_pass": st["steps"], "walk_steps_per_s": steps / wall, "pairs_per_s": npairs * K / wall,
+        "step_ms": float(np.mean(run_ms)), "pass_ms": wall / K * 1e3,
+        "join_ms": wall / K * 1e3 - float(np.mean(run_ms)), "graph_build_s": build_s,
+        "simrank_mean": float(sim.mean()), "simrank_max": float(sim.max()),
+        "pairs_meeting": int((counts.sum(axis=1) > 0).sum()),
+    }
+    print(f"[bench] simrank: {out['walk_steps_per_s'] / 1e9:.2f} G steps/s, {out['pairs_per_s']:.0f} pairs/s "
+          f"({out['pass_ms']:.1f} ms per pass, graph built in {build_s:.1f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +319,7 @@ def main() -> int:
     ap.add_argument("--stream-cache-mib", type=int, default=1024,
                     help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
     ap.add_argument("--stream-queries", type=int, default=256)
+    ap.add_argument("--simrank-scale", type=int, default=28, help="config-4 SimRank line on R-MAT of this scale (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -347,6 +389,10 @@ def main() -> int:
         stream = stream_measure(eng, wl_last["scale"], wl_last["seed"], args.stream_cache_mib << 20,
                                 args.stream_queries, 2)
 
+    simrank = {}
+    if args.simrank_scale:
+        simrank = simrank_measure(eng, args.simrank_scale, 4, 1024, 1000, 11, 0.6, 3)
+
     out = {
         "metric": "walk-steps/s",
         "value": m["value"],
@@ -375,6 +421,7 @@ def main() -> int:
         "workloads": workloads,
         "serving": serving,
         "out_of_core": stream,
+        "workloads_simrank": simrank,
     }
     if "speedup_vs_plain" in m:
         out["speedup_vs_plain"] = m["speedup_vs_plain"]

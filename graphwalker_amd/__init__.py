@@ -181,6 +181,7 @@ def lib() -> C.CDLL:
         "gw_summary_read": ([P, P, P, u64], i),
         "gw_events_read": ([P, P, u64, u32], i),
         "gw_source_counts_read": ([P, P, P, P, u64, P], i),
+        "gw_source_topk": ([P, u32, P, P, P, u64, P], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
         "gw_h2d_probe": ([P, u64, P], i),
@@ -385,6 +386,18 @@ class Engine:
         ms = C.c_double()
         self._check(self._L.gw_copy_probe(self._ctx, nbytes, C.byref(ms)), "gw_copy_probe")
         return ms.value
+
+    def source_topk(self, k: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Each source's k most-visited vertices (count descending, ties to the
+        smaller vertex id), sources ascending: (source, vertex, count)."""
+        n = C.c_uint64()
+        self._check(self._L.gw_source_topk(self._ctx, k, None, None, None, 0, C.byref(n)), "gw_source_topk")
+        s = np.empty(n.value, dtype=np.uint32)
+        v = np.empty(n.value, dtype=np.uint32)
+        c = np.empty(n.value, dtype=np.uint32)
+        self._check(self._L.gw_source_topk(self._ctx, k, _ptr(s), _ptr(v), _ptr(c), n.value, C.byref(n)),
+                    "gw_source_topk")
+        return s, v, c
 
     def source_counts(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(source, vertex, count) of the last GW_REC_SOURCE_COUNT run, sorted by source then vertex."""

@@ -126,6 +126,14 @@ struct gw_ctx {
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
+// Workgroups for a grid-stride loop over n items: at most 2^20, so the
+// launch stays far below the 2^32 work-items a dimension may hold (graphs
+// reach 2^32 edges at R-MAT scale 28).
+inline uint32_t gw_grid(uint64_t n, uint32_t tb) {
+    const uint64_t g = (n + tb - 1) / tb;
+    return static_cast<uint32_t>(g == 0 ? 1 : g < (1u << 20) ? g : (1u << 20));
+}
+
 inline bool gw_has_graph(const gw_ctx* ctx) { return ctx->d_off != nullptr || ctx->host_graph; }
 // Frees the graph (device-resident or host-resident) and its block table.
 void gw_graph_free(gw_ctx* ctx);

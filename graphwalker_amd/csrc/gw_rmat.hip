@@ -23,34 +23,36 @@ namespace {
 
 __global__ void rmat_keys_kernel(uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t k0, uint32_t k1,
                                  uint32_t ta, uint32_t tab, uint32_t tabc) {
-    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (e >= m) return;
-    uint32_t s = 0, d = 0;
-    gw::u32x4 r{0, 0, 0, 0};
-    for (uint32_t lvl = 0; lvl < scale; ++lvl) {
-        if ((lvl & 7u) == 0)
-            r = gw::philox4x32_10(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32), gw::kTagRmat, lvl >> 3, k0, k1);
-        const uint32_t word = ((lvl & 7u) >> 1) == 0 ? r.x : ((lvl & 7u) >> 1) == 1 ? r.y : ((lvl & 7u) >> 1) == 2 ? r.z : r.w;
-        const uint32_t u = (lvl & 1u) ? (word >> 16) : (word & 0xFFFFu);
-        const uint32_t sb = u >= tab ? 1u : 0u;
-        const uint32_t db = ((u >= ta && u < tab) || u >= tabc) ? 1u : 0u;
-        s = (s << 1) | sb;
-        d = (d << 1) | db;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < m; e += stride) {
+        uint32_t s = 0, d = 0;
+        gw::u32x4 r{0, 0, 0, 0};
+        for (uint32_t lvl = 0; lvl < scale; ++lvl) {
+            if ((lvl & 7u) == 0)
+                r = gw::philox4x32_10(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32), gw::kTagRmat, lvl >> 3, k0, k1);
+            const uint32_t word = ((lvl & 7u) >> 1) == 0 ? r.x : ((lvl & 7u) >> 1) == 1 ? r.y : ((lvl & 7u) >> 1) == 2 ? r.z : r.w;
+            const uint32_t u = (lvl & 1u) ? (word >> 16) : (word & 0xFFFFu);
+            const uint32_t sb = u >= tab ? 1u : 0u;
+            const uint32_t db = ((u >= ta && u < tab) || u >= tabc) ? 1u : 0u;
+            s = (s << 1) | sb;
+            d = (d << 1) | db;
+        }
+        keys[e] = (static_cast<uint64_t>(s) << scale) | d;
     }
-    keys[e] = (static_cast<uint64_t>(s) << scale) | d;
 }
 
 // Keys hold (src << bits) | dst, sorted; n vertices.
 __global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t n,
                                        uint64_t* __restrict__ off, uint32_t* __restrict__ col) {
     const uint64_t dmask = (1ull << scale) - 1;
-    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (e > m) return;
-    // The run of source s starts at e: off[v] = e for every v in (prev_src, s].
-    const uint64_t s = e < m ? (keys[e] >> scale) : static_cast<uint64_t>(n);
-    const int64_t prev = e > 0 ? static_cast<int64_t>(keys[e - 1] >> scale) : -1;
-    if (e < m) col[e] = static_cast<uint32_t>(keys[e] & dmask);
-    for (int64_t v = prev + 1; v <= static_cast<int64_t>(s); ++v) off[v] = e;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e <= m; e += stride) {
+        // The run of source s starts at e: off[v] = e for every v in (prev_src, s].
+        const uint64_t s = e < m ? (keys[e] >> scale) : static_cast<uint64_t>(n);
+        const int64_t prev = e > 0 ? static_cast<int64_t>(keys[e - 1] >> scale) : -1;
+        if (e < m) col[e] = static_cast<uint32_t>(keys[e] & dmask);
+        for (int64_t v = prev + 1; v <= static_cast<int64_t>(s); ++v) off[v] = e;
+    }
 }
 
 // Row start markers: marker[off[v]] = v for every non-empty row v (rows are
@@ -64,9 +66,9 @@ __global__ void row_marker_kernel(const uint64_t* __restrict__ off, uint32_t n, 
 // Reversed-edge keys (dst << bits) | src.
 __global__ void transpose_keys_kernel(const uint32_t* __restrict__ col, const uint32_t* __restrict__ src, uint64_t m,
                                       uint32_t bits, uint64_t* __restrict__ keys) {
-    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (e >= m) return;
-    keys[e] = (static_cast<uint64_t>(col[e]) << bits) | src[e];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < m; e += stride)
+        keys[e] = (static_cast<uint64_t>(col[e]) << bits) | src[e];
 }
 
 struct MaxOp {
@@ -92,7 +94,7 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     const uint32_t tb = 256;
     hipError_t e = hipSuccess;
     if (m) {
-        rmat_keys_kernel<<<dim3(static_cast<uint32_t>((m + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+        rmat_keys_kernel<<<dim3(gw_grid(m, tb)), dim3(tb), 0, ctx->stream>>>(
             k_in, m, scale, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32), ta, tab, tabc);
         e = hipGetLastError();
         if (e == hipSuccess)
@@ -113,7 +115,7 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     if ((rc = gw_alloc(ctx, (void**)&ctx->d_col, m * 4, "columns"))) { cleanup(); return rc; }
     ctx->n = n;
     ctx->m = m;
-    csr_from_sorted_kernel<<<dim3(static_cast<uint32_t>((m + 1 + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+    csr_from_sorted_kernel<<<dim3(gw_grid(m + 1, tb)), dim3(tb), 0, ctx->stream>>>(
         k_out, m, scale, n, ctx->d_off, ctx->d_col);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -170,7 +172,7 @@ int gw_transpose_build(gw_ctx* ctx) {
         size_t t = temp_bytes;
         if (e == hipSuccess) e = hipcub::DeviceScan::InclusiveScan(temp, t, src, esrc, MaxOp(), m, ctx->stream);
         if (e == hipSuccess) {
-            transpose_keys_kernel<<<dim3(static_cast<uint32_t>((m + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+            transpose_keys_kernel<<<dim3(gw_grid(m, tb)), dim3(tb), 0, ctx->stream>>>(
                 ctx->d_col, esrc, m, bits, k_in);
             e = hipGetLastError();
         }
@@ -178,7 +180,7 @@ int gw_transpose_build(gw_ctx* ctx) {
         if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(temp, t, k_in, k_out, m, 0, 2 * bits, ctx->stream);
     }
     if (e == hipSuccess) {
-        csr_from_sorted_kernel<<<dim3(static_cast<uint32_t>((m + 1 + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+        csr_from_sorted_kernel<<<dim3(gw_grid(m + 1, tb)), dim3(tb), 0, ctx->stream>>>(
             k_out, m, bits, n, ctx->d_off, ctx->d_col);
         e = hipGetLastError();
     }

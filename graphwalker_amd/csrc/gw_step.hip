@@ -308,13 +308,14 @@ __global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t 
 
 __global__ void build_ext_kernel(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col, uint64_t m,
                                  Edge* __restrict__ ext, unsigned long long* __restrict__ overflow) {
-    const uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (e >= m) return;
-    const uint32_t d = col[e];
-    const uint64_t b = off[d];
-    const uint64_t deg = off[d + 1] - b;
-    if (deg > 0xFFFFFFFFull) atomicAdd(overflow, 1ull);
-    ext[e] = Edge{d, static_cast<uint32_t>(deg), b};
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < m; e += stride) {
+        const uint32_t d = col[e];
+        const uint64_t b = off[d];
+        const uint64_t deg = off[d + 1] - b;
+        if (deg > 0xFFFFFFFFull) atomicAdd(overflow, 1ull);
+        ext[e] = Edge{d, static_cast<uint32_t>(deg), b};
+    }
 }
 
 struct Launch {
@@ -403,7 +404,7 @@ bool ensure_ext(gw_ctx* ctx) {
     }
     hipError_t e = hipMemsetAsync(ctx->d_counters + kCntExtOverflow, 0, 8, ctx->stream);
     if (e == hipSuccess && ctx->m) {
-        build_ext_kernel<<<dim3(static_cast<uint32_t>((ctx->m + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+        build_ext_kernel<<<dim3(gw_grid(ctx->m, 256)), dim3(256), 0, ctx->stream>>>(
             ctx->d_off, ctx->d_col, ctx->m, reinterpret_cast<Edge*>(ctx->d_ext), ctx->d_counters + kCntExtOverflow);
         e = hipGetLastError();
     }

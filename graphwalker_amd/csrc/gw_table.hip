@@ -26,43 +26,48 @@ namespace {
 __global__ void compact_kernel(const unsigned long long* __restrict__ tab, uint64_t slots, uint32_t wide,
                                uint32_t cbits, unsigned long long* __restrict__ out_k, uint32_t* __restrict__ out_c,
                                unsigned long long* __restrict__ n) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    unsigned long long k = 0, c = 0;
-    if (i < slots) {
-        if (wide) {
-            k = tab[2 * i];
-            c = tab[2 * i + 1];
-        } else {
-            const unsigned long long x = tab[i];
-            k = x ? (x >> cbits) + 1 : 0ull;  // + 1: 0 stays "empty"
-            c = x & ((1ull << cbits) - 1);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    // workgroup-uniform trip count: every lane of a wavefront takes part in the ballots
+    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < slots; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        unsigned long long k = 0, c = 0;
+        if (i < slots) {
+            if (wide) {
+                k = tab[2 * i];
+                c = tab[2 * i + 1];
+            } else {
+                const unsigned long long x = tab[i];
+                k = x ? (x >> cbits) + 1 : 0ull;  // + 1: 0 stays "empty"
+                c = x & ((1ull << cbits) - 1);
+            }
         }
-    }
-    const bool full = k != 0ull;
-    const uint64_t mask = __ballot(full);
-    if (mask == 0) return;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(n, static_cast<unsigned long long>(__popcll(mask)));
-    base = __shfl(base, leader, 64);
-    if (full) {
-        const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-        out_k[base + r] = k - 1;
-        out_c[base + r] = static_cast<uint32_t>(c);
+        const bool full = k != 0ull;
+        const uint64_t mask = __ballot(full);
+        if (mask == 0) continue;
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(n, static_cast<unsigned long long>(__popcll(mask)));
+        base = __shfl(base, leader, 64);
+        if (full) {
+            const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+            out_k[base + r] = k - 1;
+            out_c[base + r] = static_cast<uint32_t>(c);
+        }
     }
 }
 
 __global__ void unpack_runs_kernel(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ counts,
                                    uint64_t n, uint32_t vbits, uint32_t* __restrict__ s, uint32_t* __restrict__ v,
                                    uint32_t* __restrict__ c) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    const unsigned long long k = keys[i];
-    v[i] = static_cast<uint32_t>(k & ((1ull << vbits) - 1));
-    s[i] = static_cast<uint32_t>(k >> vbits);
-    c[i] = counts[i];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+        const unsigned long long k = keys[i];
+        v[i] = static_cast<uint32_t>(k & ((1ull << vbits) - 1));
+        s[i] = static_cast<uint32_t>(k >> vbits);
+        c[i] = counts[i];
+    }
 }
 
 
@@ -436,6 +441,108 @@ __global__ __launch_bounds__(256) void gather_regions_kernel(const uint32_t* __r
     }
 }
 
+
+constexpr uint32_t kTopkThreads = 256;
+constexpr uint32_t kTopkMax = 1024;
+
+__global__ void source_of_kernel(const unsigned long long* __restrict__ keys, uint64_t n, uint32_t vbits,
+                                 uint64_t* __restrict__ src) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+        src[i] = keys[i] >> vbits;
+}
+
+// One workgroup per source segment [seg[g], seg[g+1]) of the sorted pairs.
+// Rank key = (count << 32) | ~vertex: larger count first, then smaller
+// vertex; keys are unique, so an 8-pass MSB radix select (LDS histograms)
+// finds the k-th largest exactly, the keys at or above it are gathered and
+// bitonic-sorted in LDS.
+__global__ __launch_bounds__(kTopkThreads) void topk_kernel(const unsigned long long* __restrict__ keys,
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint64_t* __restrict__ seg, uint32_t vbits,
+                                                           uint32_t k, uint32_t* __restrict__ out_s,
+                                                           uint32_t* __restrict__ out_v, uint32_t* __restrict__ out_c,
+                                                           uint32_t* __restrict__ out_n) {
+    __shared__ uint32_t hist[256];
+    __shared__ unsigned long long s_prefix;
+    __shared__ uint32_t s_remaining, s_fill;
+    __shared__ unsigned long long s_sel[kTopkMax];
+    const uint32_t g = blockIdx.x;
+    const uint64_t lo = seg[g], n = seg[g + 1] - lo;
+    const unsigned long long vmask = (1ull << vbits) - 1;
+    auto rank_key = [&](uint64_t i) {
+        const uint32_t v = static_cast<uint32_t>(keys[lo + i] & vmask);
+        return (static_cast<unsigned long long>(counts[lo + i]) << 32) | static_cast<uint32_t>(~v);
+    };
+    unsigned long long thr = 0;  // gather keys >= thr
+    if (n > k) {
+        if (threadIdx.x == 0) {
+            s_prefix = 0;
+            s_remaining = k;
+        }
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+            __syncthreads();
+            const unsigned long long prefix = s_prefix;
+            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const unsigned long long key = rank_key(i);
+                if (shift == 56 || (key >> (shift + 8)) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t above = 0;
+                for (int d = 255; d >= 0; --d) {
+                    if (above + hist[d] >= s_remaining) {
+                        s_prefix = (prefix << 8) | static_cast<unsigned long long>(d);
+                        s_remaining -= above;
+                        break;
+                    }
+                    above += hist[d];
+                }
+            }
+            __syncthreads();
+        }
+        thr = s_prefix;
+    }
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long key = rank_key(i);
+        if (key >= thr) s_sel[atomicAdd(&s_fill, 1u)] = key;
+    }
+    __syncthreads();
+    const uint32_t m = s_fill;  // min(n, k)
+    uint32_t size = 1;
+    while (size < m) size <<= 1;
+    for (uint32_t i = m + threadIdx.x; i < size; i += blockDim.x) s_sel[i] = 0;  // below every real key
+    __syncthreads();
+    // bitonic sort, descending
+    for (uint32_t len = 2; len <= size; len <<= 1) {
+        for (uint32_t j = len >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const bool desc = (i & len) == 0;
+                    const unsigned long long x = s_sel[i], y = s_sel[p];
+                    if (desc ? x < y : x > y) {
+                        s_sel[i] = y;
+                        s_sel[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t src = static_cast<uint32_t>(keys[lo] >> vbits);
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        const unsigned long long key = s_sel[r];
+        out_s[static_cast<uint64_t>(g) * k + r] = src;
+        out_v[static_cast<uint64_t>(g) * k + r] = ~static_cast<uint32_t>(key);
+        out_c[static_cast<uint64_t>(g) * k + r] = static_cast<uint32_t>(key >> 32);
+    }
+    if (threadIdx.x == 0) out_n[g] = m;
+}
+
 }  // namespace
 
 // Sort-reduce of the key log: radix sort of the n logged slots on the key
@@ -674,111 +781,216 @@ static int partitioned_sorted(gw_ctx* ctx, unsigned long long** keys_out, uint32
     return GW_OK;
 }
 
-extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* vertex, uint32_t* count, uint64_t cap,
-                                     uint64_t* n_out) {
-    if (!ctx || !n_out) return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: bad arguments");
-    GW_HIP(ctx, hipSetDevice(ctx->device));
-    if (ctx->src_method != GW_SRC_HASH) {
-        const uint64_t n = ctx->n_runs;
-        *n_out = n;
-        if (!source && !vertex && !count) return GW_OK;
-        if (!source || !vertex || !count || cap < n)
-            return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: output arrays missing or smaller than the pair count");
-        if (n == 0) return GW_OK;
-        const unsigned long long* keys = ctx->d_runs;
-        const uint32_t* counts = ctx->d_run_counts;
-        void* mem = nullptr;
-        uint64_t mem_bytes = 0;
-        if (ctx->src_method == GW_SRC_PARTITIONED) {
-            unsigned long long* k = nullptr;
-            uint32_t* c = nullptr;
-            int rp = partitioned_sorted(ctx, &k, &c, &mem, &mem_bytes);
-            if (rp) {
-                gw_free(ctx, mem, mem_bytes);
-                return rp;
-            }
-            keys = k;
-            counts = c;
-        }
-        uint32_t* d_out = nullptr;
-        int rc = gw_alloc(ctx, (void**)&d_out, n * 12, "runs unpack");
-        if (rc) {
-            gw_free(ctx, mem, mem_bytes);
-            return rc;
-        }
-        unpack_runs_kernel<<<dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            keys, counts, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(source, d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(vertex, d_out + n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(count, d_out + 2 * n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        gw_free(ctx, d_out, n * 12);
-        gw_free(ctx, mem, mem_bytes);
-        if (e != hipSuccess) return gw_hip_check(ctx, e, "runs read-out");
+// The last run's (source, vertex) counts on the device as keys
+// (source << vbits) | vertex sorted ascending, with their counts.  Owned
+// temporaries are released by release_pairs.  count_only: just n.
+struct SortedPairs {
+    const unsigned long long* keys = nullptr;
+    const uint32_t* counts = nullptr;
+    uint64_t n = 0;
+    void* mem = nullptr;
+    uint64_t mem_bytes = 0;
+};
+
+static void release_pairs(gw_ctx* ctx, SortedPairs& sp) {
+    gw_free(ctx, sp.mem, sp.mem_bytes);
+    sp = SortedPairs{};
+}
+
+static int sorted_pairs(gw_ctx* ctx, SortedPairs& sp, bool count_only) {
+    sp = SortedPairs{};
+    if (ctx->src_method == GW_SRC_GLOBAL) {
+        sp.keys = ctx->d_runs;
+        sp.counts = ctx->d_run_counts;
+        sp.n = ctx->n_runs;
         return GW_OK;
     }
-    if (!ctx->d_table) return gw_fail(ctx, GW_ESTATE, "gw_source_counts_read: no run with GW_REC_SOURCE_COUNT");
+    if (ctx->src_method == GW_SRC_PARTITIONED) {
+        sp.n = ctx->n_runs;
+        if (count_only || sp.n == 0) return GW_OK;
+        unsigned long long* k = nullptr;
+        uint32_t* c = nullptr;
+        int rc = partitioned_sorted(ctx, &k, &c, &sp.mem, &sp.mem_bytes);
+        if (rc) {
+            release_pairs(ctx, sp);
+            return rc;
+        }
+        sp.keys = k;
+        sp.counts = c;
+        return GW_OK;
+    }
+    if (!ctx->d_table) return gw_fail(ctx, GW_ESTATE, "no run with GW_REC_SOURCE_COUNT");
+    // hash table: compact the occupied slots, then sort by key (keys are
+    // unique per slot).  Sorted from bit 0: rocPRIM's onesweep sort of 64-bit
+    // keys with begin_bit > 0 returns a non-permutation on gfx950 with ROCm
+    // 7.2 (measured), so it is avoided.
     const uint64_t slots = ctx->table_slots;
-    unsigned long long *ka = nullptr, *kb = nullptr;
-    uint32_t *ca = nullptr, *cb = nullptr;
-    void* temp = nullptr;
+    const int end_bit = static_cast<int>(64 - ctx->table_cbits);
     size_t temp_bytes = 0;
-    uint32_t* d_out = nullptr;
-    int rc;
-    if ((rc = gw_alloc(ctx, (void**)&ka, slots * 8, "table compaction")) ||
-        (rc = gw_alloc(ctx, (void**)&ca, slots * 4, "table compaction counts"))) {
-        gw_free(ctx, ka, slots * 8);
+    {
+        unsigned long long* nk = nullptr;
+        uint32_t* nc = nullptr;
+        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, nk, nk, nc, nc, slots, 0, end_bit,
+                                                       ctx->stream));
+    }
+    sp.mem_bytes = slots * 24 + temp_bytes;
+    int rc = gw_alloc(ctx, &sp.mem, sp.mem_bytes, "table read-out");
+    if (rc) {
+        sp.mem_bytes = 0;
         return rc;
     }
-    auto release = [&]() {
-        gw_free(ctx, ka, slots * 8);
-        gw_free(ctx, ca, slots * 4);
-        gw_free(ctx, kb, slots * 8);
-        gw_free(ctx, cb, slots * 4);
-        gw_free(ctx, temp, temp_bytes);
-        gw_free(ctx, d_out, slots * 12);
-    };
+    unsigned long long* ka = static_cast<unsigned long long*>(sp.mem);
+    unsigned long long* kb = ka + slots;
+    uint32_t* ca = reinterpret_cast<uint32_t*>(kb + slots);
+    uint32_t* cb = ca + slots;
+    void* temp = cb + slots;
     hipError_t e = hipMemsetAsync(ctx->d_counters + kCntCompact, 0, 8, ctx->stream);
     if (e == hipSuccess) {
-        compact_kernel<<<dim3(static_cast<uint32_t>((slots + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+        compact_kernel<<<dim3(gw_grid(slots, 256)), dim3(256), 0, ctx->stream>>>(
             ctx->d_table, slots, ctx->table_wide ? 1u : 0u, ctx->table_cbits, ka, ca, ctx->d_counters + kCntCompact);
         e = hipGetLastError();
     }
     unsigned long long n = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&n, ctx->d_counters + kCntCompact, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) { release(); return gw_hip_check(ctx, e, "table compaction"); }
+    if (e == hipSuccess && !count_only && n)
+        e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
+    if (e != hipSuccess) {
+        release_pairs(ctx, sp);
+        return gw_hip_check(ctx, e, "table read-out");
+    }
+    sp.keys = kb;
+    sp.counts = cb;
+    sp.n = n;
+    return GW_OK;
+}
+
+extern "C" int gw_source_counts_read(gw_ctx* ctx, uint32_t* source, uint32_t* vertex, uint32_t* count, uint64_t cap,
+                                     uint64_t* n_out) {
+    if (!ctx || !n_out) return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    const bool size_only = !source && !vertex && !count;
+    SortedPairs sp;
+    int rc = sorted_pairs(ctx, sp, size_only);
+    if (rc) return rc;
+    const uint64_t n = sp.n;
     *n_out = n;
-    if (!source && !vertex && !count) { release(); return GW_OK; }
+    if (size_only || n == 0) {
+        release_pairs(ctx, sp);
+        return GW_OK;
+    }
     if (!source || !vertex || !count || cap < n) {
-        release();
+        release_pairs(ctx, sp);
         return gw_fail(ctx, GW_EINVAL, "gw_source_counts_read: output arrays missing or smaller than the pair count");
     }
-    if (n == 0) { release(); return GW_OK; }
-    if ((rc = gw_alloc(ctx, (void**)&kb, slots * 8, "table sort")) ||
-        (rc = gw_alloc(ctx, (void**)&cb, slots * 4, "table sort counts"))) {
-        release();
+    uint32_t* d_out = nullptr;
+    if ((rc = gw_alloc(ctx, (void**)&d_out, n * 12, "pairs unpack"))) {
+        release_pairs(ctx, sp);
         return rc;
     }
-    // Keys are unique per slot.  Sorted from bit 0: rocPRIM's onesweep sort
-    // of 64-bit keys with begin_bit > 0 returns a non-permutation on gfx950
-    // with ROCm 7.2 (measured), so it is avoided.
-    const int end_bit = static_cast<int>(64 - ctx->table_cbits);
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
-    if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "table sort temp"))) { release(); return rc; }
-    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
-    if (e == hipSuccess && (rc = gw_alloc(ctx, (void**)&d_out, slots * 12, "table unpack"))) { release(); return rc; }
-    if (e == hipSuccess) {
-        unpack_runs_kernel<<<dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            kb, cb, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
-        e = hipGetLastError();
-    }
+    unpack_runs_kernel<<<dim3(gw_grid(n, 256)), dim3(256), 0, ctx->stream>>>(
+        sp.keys, sp.counts, n, ctx->table_vbits, d_out, d_out + n, d_out + 2 * n);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(source, d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(vertex, d_out + n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(count, d_out + 2 * n, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    release();
-    if (e != hipSuccess) return gw_hip_check(ctx, e, "table read-out");
+    gw_free(ctx, d_out, n * 12);
+    release_pairs(ctx, sp);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "pairs read-out");
+    return GW_OK;
+}
+
+// Per-source top-k (DiscreteDistribution::getTop, discretedistribution.hpp:72-95):
+// the k most-visited vertices of each source, count descending; ties go to
+// the smaller vertex id (the reference's heap keeps whichever came first,
+// which depends on its racy insertion order).
+extern "C" int gw_source_topk(gw_ctx* ctx, uint32_t k, uint32_t* source, uint32_t* vertex, uint32_t* count,
+                              uint64_t cap, uint64_t* n_out) {
+    if (!ctx || !n_out || k == 0 || k > kTopkMax)
+        return gw_fail(ctx, GW_EINVAL, "gw_source_topk: bad arguments (1 <= k <= 1024)");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    SortedPairs sp;
+    int rc = sorted_pairs(ctx, sp, false);
+    if (rc) return rc;
+    const uint64_t n = sp.n;
+    if (n == 0) {
+        *n_out = 0;
+        release_pairs(ctx, sp);
+        return GW_OK;
+    }
+    // segments: one per source, from a run-length encode of the key's source part
+    const uint32_t vbits = ctx->table_vbits;
+    uint64_t* d_src = nullptr;  // per pair: source id
+    void* temp = nullptr;
+    size_t temp_bytes = 0, t_rle = 0, t_scan = 0;
+    const uint64_t bytes = (4 * n + 2) * 8;
+    if ((rc = gw_alloc(ctx, (void**)&d_src, bytes, "top-k segments"))) {
+        release_pairs(ctx, sp);
+        return rc;
+    }
+    uint64_t* d_uniq = d_src + n;     // sources of the runs
+    uint64_t* d_len = d_uniq + n;     // run lengths (n + 1 slots)
+    uint64_t* d_seg = d_len + n + 1;  // segment offsets (n + 1 slots)
+    auto cleanup = [&]() {
+        gw_free(ctx, d_src, bytes);
+        gw_free(ctx, temp, temp_bytes);
+        release_pairs(ctx, sp);
+    };
+    source_of_kernel<<<dim3(gw_grid(n, 256)), dim3(256), 0, ctx->stream>>>(sp.keys, n, vbits, d_src);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipcub::DeviceRunLengthEncode::Encode(nullptr, t_rle, d_src, d_uniq, d_len, ctx->d_counters + kCntCompact,
+                                                  n, ctx->stream);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, d_len, d_seg, n + 1, ctx->stream);
+    temp_bytes = std::max(t_rle, t_scan);
+    if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "top-k temp"))) {
+        cleanup();
+        return rc;
+    }
+    size_t tb = temp_bytes;
+    if (e == hipSuccess)
+        e = hipcub::DeviceRunLengthEncode::Encode(temp, tb, d_src, d_uniq, d_len, ctx->d_counters + kCntCompact, n,
+                                                  ctx->stream);
+    unsigned long long nseg = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nseg, ctx->d_counters + kCntCompact, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    // lengths -> offsets: a zero after the last run makes the scan's last slot n
+    if (e == hipSuccess) e = hipMemsetAsync(d_len + nseg, 0, 8, ctx->stream);
+    tb = temp_bytes;
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(temp, tb, d_len, d_seg, nseg + 1, ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return gw_hip_check(ctx, e, "top-k segments");
+    }
+    // top-k of every segment into [nseg x k] slots
+    const uint64_t slots = nseg * k;
+    uint32_t* d_top = nullptr;
+    if ((rc = gw_alloc(ctx, (void**)&d_top, slots * 12 + nseg * 4, "top-k output"))) {
+        cleanup();
+        return rc;
+    }
+    topk_kernel<<<dim3(static_cast<uint32_t>(nseg)), dim3(kTopkThreads), 0, ctx->stream>>>(
+        sp.keys, sp.counts, d_seg, vbits, k, d_top, d_top + slots, d_top + 2 * slots, d_top + 3 * slots);
+    e = hipGetLastError();
+    std::vector<uint32_t> top(slots * 3 + nseg);
+    if (e == hipSuccess) e = hipMemcpyAsync(top.data(), d_top, (slots * 3 + nseg) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    gw_free(ctx, d_top, slots * 12 + nseg * 4);
+    cleanup();
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "top-k");
+    uint64_t total = 0;
+    for (uint64_t g = 0; g < nseg; ++g) total += top[slots * 3 + g];
+    *n_out = total;
+    if (!source && !vertex && !count) return GW_OK;
+    if (!source || !vertex || !count || cap < total)
+        return gw_fail(ctx, GW_EINVAL, "gw_source_topk: output arrays missing or smaller than the result");
+    uint64_t o = 0;
+    for (uint64_t g = 0; g < nseg; ++g)
+        for (uint32_t r = 0; r < top[slots * 3 + g]; ++r, ++o) {
+            source[o] = top[g * k + r];
+            vertex[o] = top[slots + g * k + r];
+            count[o] = top[2 * slots + g * k + r];
+        }
     return GW_OK;
 }

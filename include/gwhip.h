@@ -42,6 +42,8 @@
  *                              (rwdomination.cpp:286-289 vertex_value[dstId]++)
  *   gw_source_counts_read      msppr's per-source DiscreteDistribution recorder
  *                              (msppr.cpp:71-76, discretedistribution.hpp:44-57), exact
+ *   gw_source_topk             DiscreteDistribution::getTop (discretedistribution.hpp:72-95),
+ *                              per source, on the device
  *   gw_events_read             the per-step updateInfo callback (randomwalk.hpp:32):
  *                              each walk's visits, replayed on the host in walk order
  *   gw_graph_transpose         (no counterpart: the reference's SimRank reads a
@@ -234,6 +236,11 @@ int gw_summary_read(gw_ctx *ctx, uint64_t *hash, uint32_t *visits, uint64_t n);
  * is returned; otherwise `cap` must be at least that number. */
 int gw_source_counts_read(gw_ctx *ctx, uint32_t *source, uint32_t *vertex, uint32_t *count, uint64_t cap,
                           uint64_t *n_out);
+/* Per-source top-k of the last GW_REC_SOURCE_COUNT run (1 <= k <= 1024): each
+ * source's k most-visited vertices, count descending, ties to the smaller
+ * vertex id; sources ascending.  NULL arrays: *n_out = entry count only. */
+int gw_source_topk(gw_ctx *ctx, uint32_t k, uint32_t *source, uint32_t *vertex, uint32_t *count, uint64_t cap,
+                   uint64_t *n_out);
 /* Event log of the last run with GW_REC_EVENTS: walk id w's visit k is
  * out[w * cap + k] = (hop << 32) | vertex for k < min(visits[w], cap).
  * `n_walks` and `cap` must equal the loaded walk count and the run's event_cap. */

@@ -334,3 +334,28 @@ def test_source_counts_wide_keys(engine, method):
     es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
     assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
     assert int(c.sum()) == st["visits"]
+
+
+def _topk(s, v, c, k):
+    order = np.lexsort((v, -c.astype(np.int64), s))  # source asc, count desc, vertex asc
+    s, v, c = s[order], v[order], c[order]
+    rank = np.arange(s.shape[0]) - np.searchsorted(s, s, side="left")
+    keep = rank < k
+    return s[keep], v[keep], c[keep]
+
+
+@pytest.mark.parametrize("method", SRC_METHODS, ids=["sort", "hash", "global", "partitioned"])
+@pytest.mark.parametrize("k", [1, 5, 1024])
+def test_source_topk(engine, small_graph, method, k):
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    sources = O.sources(3, n, 30)
+    field = np.arange(30, dtype=np.uint32)
+    wps = 300
+    engine.walks_init_sources(sources, wps, source_field=field)
+    prog = gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_SOURCE_COUNT, source_method=method)
+    engine.run(prog)
+    s, v, c = engine.source_topk(k)
+    es, ev, ec = _topk(*_expected_source_counts(prog, off, col, sources, field, wps), k)
+    assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
