@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "gw_device.h"
 #include "gw_internal.h"
@@ -31,58 +32,74 @@ __global__ void init_pairs_kernel(gw_walk* __restrict__ w, const uint32_t* __res
     w[id] = gw_walk{static_cast<uint32_t>(id), v, v, i * L};
 }
 
-// One workgroup per pair k.  The b-side paths are staged in LDS tiles; each
-// lane holds one a-side path in registers and scans the tile's paths (the
-// same path for every lane, an LDS broadcast), recording the first offset at
-// which the two paths hold the same vertex.  Offsets never visited hold no
-// vertex and never meet (the reference's 0xffffffff fill).
+// One workgroup per pair k, specialised on the path length P = L - 1.  The
+// b-side paths are staged in LDS tiles; each lane holds one a-side path in
+// registers and scans the tile's paths (the same path for every lane, an LDS
+// broadcast), comparing all P offsets at once into a bit mask; the first set
+// bit is the first meeting.  Meetings are rare, so they are counted with LDS
+// atomics.  Offsets never visited hold no vertex and never meet (the
+// reference's 0xffffffff fill).
+template <uint32_t P>
 __global__ __launch_bounds__(kMeetThreads) void meeting_kernel(const unsigned long long* __restrict__ ev,
-                                                              uint32_t cap, uint32_t R, uint32_t P,
+                                                              uint32_t cap, uint32_t R,
                                                               const uint32_t* __restrict__ visits,
                                                               uint32_t* __restrict__ counts, uint32_t L) {
-    __shared__ uint32_t s_b[kTileWords];
+    constexpr uint32_t kStride = (P + 3) / 4 * 4;  // a b-side path padded to whole uint4s
+    constexpr uint32_t kTile = kTileWords / kStride;
+    __shared__ __attribute__((aligned(16))) uint32_t s_b[kTileWords];
     __shared__ uint32_t s_cnt[kMaxPath + 1];
     const uint32_t k = blockIdx.x;
     const uint64_t abase = 2ull * k * R, bbase = abase + R;
-    const uint32_t tile = kTileWords / P;  // b-side walks per tile
     if (threadIdx.x <= kMaxPath) s_cnt[threadIdx.x] = 0;
-    uint32_t my[kMaxPath];
-#pragma unroll
-    for (uint32_t l = 0; l < kMaxPath; ++l) my[l] = 0;
-    for (uint32_t t0 = 0; t0 < R; t0 += tile) {
-        const uint32_t tn = std::min(tile, R - t0);
+    for (uint32_t t0 = 0; t0 < R; t0 += kTile) {
+        const uint32_t tn = std::min(kTile, R - t0);
         __syncthreads();  // the previous tile is consumed
-        for (uint32_t x = threadIdx.x; x < tn * P; x += blockDim.x) {
-            const uint32_t j = x / P, l = x - j * P;
+        for (uint32_t x = threadIdx.x; x < tn * kStride; x += blockDim.x) {
+            const uint32_t j = x / kStride, l = x - j * kStride;
             const uint64_t id = bbase + t0 + j;
-            s_b[x] = l < visits[id] ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFFu;
+            s_b[x] = l < P && l < visits[id] ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFFu;
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) {
             const uint64_t id = abase + i;
             const uint32_t nv = visits[id];
-            uint32_t a[kMaxPath];
+            uint32_t a[kStride];
 #pragma unroll
-            for (uint32_t l = 0; l < kMaxPath; ++l)
-                a[l] = (l < P && l < nv) ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFEu;  // never equal to an unvisited b offset
+            for (uint32_t l = 0; l < kStride; ++l)
+                a[l] = (l < P && l < nv) ? static_cast<uint32_t>(ev[id * cap + l]) : 0xFFFFFFFEu;  // never equal to a b fill
             for (uint32_t j = 0; j < tn; ++j) {
-                const uint32_t* bp = s_b + j * P;
-                bool met = false;
+                const uint4* bp = reinterpret_cast<const uint4*>(s_b + j * kStride);
+                uint32_t mask = 0;
 #pragma unroll
-                for (uint32_t l = 0; l < kMaxPath; ++l) {
-                    if (l < P && !met && a[l] == bp[l]) {
-                        ++my[l];
-                        met = true;
-                    }
+                for (uint32_t q = 0; q < kStride / 4; ++q) {
+                    const uint4 bq = bp[q];
+                    mask |= (a[4 * q] == bq.x ? 1u : 0u) << (4 * q);
+                    mask |= (a[4 * q + 1] == bq.y ? 1u : 0u) << (4 * q + 1);
+                    mask |= (a[4 * q + 2] == bq.z ? 1u : 0u) << (4 * q + 2);
+                    mask |= (a[4 * q + 3] == bq.w ? 1u : 0u) << (4 * q + 3);
                 }
+                if (mask) atomicAdd(&s_cnt[__ffs(mask) - 1], 1u);
             }
         }
     }
-#pragma unroll
-    for (uint32_t l = 0; l < kMaxPath; ++l)
-        if (l < P && my[l]) atomicAdd(&s_cnt[l], my[l]);
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < L; l += blockDim.x) counts[static_cast<uint64_t>(k) * L + l] = l < P ? s_cnt[l] : 0u;
+}
+
+template <uint32_t P>
+hipError_t launch_meeting(uint32_t npairs, hipStream_t st, const unsigned long long* ev, uint32_t cap, uint32_t R,
+                          const uint32_t* visits, uint32_t* counts, uint32_t L) {
+    meeting_kernel<P><<<dim3(npairs), dim3(kMeetThreads), 0, st>>>(ev, cap, R, visits, counts, L);
+    return hipGetLastError();
+}
+
+template <uint32_t... Ps>
+hipError_t dispatch_meeting(uint32_t P, std::integer_sequence<uint32_t, Ps...>, uint32_t npairs, hipStream_t st,
+                            const unsigned long long* ev, uint32_t cap, uint32_t R, const uint32_t* visits,
+                            uint32_t* counts, uint32_t L) {
+    hipError_t e = hipErrorInvalidValue;
+    ((P == Ps + 1 ? (e = launch_meeting<Ps + 1>(npairs, st, ev, cap, R, visits, counts, L), 0) : 0), ...);
+    return e;
 }
 
 }  // namespace
@@ -134,9 +151,9 @@ int gw_meeting_counts(gw_ctx* ctx, uint32_t npairs, uint32_t R, uint32_t L, uint
     uint32_t* d_counts = nullptr;
     int rc = gw_alloc(ctx, (void**)&d_counts, static_cast<uint64_t>(npairs) * L * 4, "meeting counts");
     if (rc) return rc;
-    meeting_kernel<<<dim3(npairs), dim3(kMeetThreads), 0, ctx->stream>>>(
-        reinterpret_cast<const unsigned long long*>(ctx->d_events), ctx->events_cap, R, P, ctx->d_visits, d_counts, L);
-    hipError_t e = hipGetLastError();
+    hipError_t e = dispatch_meeting(P, std::make_integer_sequence<uint32_t, kMaxPath>{}, npairs, ctx->stream,
+                                    reinterpret_cast<const unsigned long long*>(ctx->d_events), ctx->events_cap, R,
+                                    ctx->d_visits, d_counts, L);
     if (e == hipSuccess)
         e = hipMemcpyAsync(counts, d_counts, static_cast<uint64_t>(npairs) * L * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -52,15 +52,15 @@ def _first_meetings(A, B, L):
     return counts
 
 
-@pytest.mark.parametrize("transposed", [False, True])
-def test_meeting_counts_equal_brute_force(engine, transposed):
+@pytest.mark.parametrize("transposed,L", [(False, 11), (True, 11), (False, 2), (False, 5), (True, 32)])
+def test_meeting_counts_equal_brute_force(engine, transposed, L):
     off, col = O.rmat_graph(10, 1 << 13, seed=4)
     n = off.shape[0] - 1
     engine.graph_set(off, col)
     if transposed:
         engine.graph_transpose()
         off, col = _transpose(off, col)
-    R, L = 60, 11
+    R = 60
     pairs = O.sources(5, n, 6).reshape(3, 2)
     a, b = pairs[:, 0].copy(), pairs[:, 1].copy()
     a[2] = b[2]  # a pair of one vertex meets at offset 0 every time
