@@ -153,11 +153,12 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
 
 
 def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5, hops: int = 5,
-              walks_lo: int = 1000, walks_hi: int = 100_000) -> dict:
+              walks_lo: int = 1000, walks_hi: int = 100_000, topk: int = 100) -> dict:
     """Poisson arrivals of restart-PPR queries (p = 0.15, 10 hops) at `qps`,
     walk counts log-uniform in [walks_lo, walks_hi], uniform sources, served
-    by continuous batching.  Latency = completion (result on the host) minus
-    arrival, on the host clock."""
+    by continuous batching; each query returns its top-k vertices (0 = every
+    pair).  Latency = completion (result on the host) minus arrival, on the
+    host clock."""
     rng = np.random.default_rng(seed)
     nq = max(1, int(qps * seconds))
     arrivals = np.cumsum(rng.exponential(1.0 / qps, nq))
@@ -165,6 +166,8 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
     sources = gw.sources(seed, n, nq)
     eng.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=seed), pool_walks=1 << 24, arena_bytes=16 << 30,
                    max_queries=4096, hops_per_step=hops)
+    if topk:
+        eng.serve_topk(topk)
     lat = np.zeros(nq)
     steps = visits = nsteps = 0
     step_ms = 0.0
@@ -199,7 +202,7 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
             "walk_steps_per_s": steps / wall, "visits": visits, "serve_steps": nsteps,
             "gpu_busy_frac": step_ms * 1e-3 / wall, "hops_per_step": hops,
-            "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]"}
+            "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]", "result": f"top-{topk}" if topk else "all pairs"}
 
 
 def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, queries: int, K: int,
@@ -316,6 +319,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-topk", type=int, default=100, help="vertices returned per query (0 = every pair)")
     ap.add_argument("--stream-cache-mib", type=int, default=1024,
                     help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
     ap.add_argument("--stream-queries", type=int, default=256)
@@ -378,7 +382,7 @@ def main() -> int:
         # continuous batching on the graph left resident by the last workload
         n_last = 1 << (WORKLOADS[args.secondary]["scale"] if workloads else wl["scale"])
         for q in args.serve_qps:
-            r = serve_sim(eng, n_last, q, args.serve_seconds)
+            r = serve_sim(eng, n_last, q, args.serve_seconds, topk=args.serve_topk)
             serving[f"qps_{int(q)}"] = r
             print(f"[bench] serving {q:.0f} qps: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
                   f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)

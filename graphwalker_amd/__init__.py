@@ -191,6 +191,7 @@ def lib() -> C.CDLL:
         "gw_serve_step": ([P, P], i),
         "gw_serve_poll": ([P, P, P], i),
         "gw_serve_result": ([P, u64, P, P, u64], i),
+        "gw_serve_topk": ([P, u32], i),
         "gw_serve_close": ([P], i),
     }
     for name, (args, res) in sig.items():
@@ -445,6 +446,10 @@ class Engine:
             o = np.argsort(v, kind="stable")
             v, c = v[o], c[o]
         return v, c
+
+    def serve_topk(self, k: int) -> None:
+        """Completed queries keep their k most-visited vertices (0 = all pairs)."""
+        self._check(self._L.gw_serve_topk(self._ctx, k), "gw_serve_topk")
 
     def serve_close(self) -> None:
         self._check(self._L.gw_serve_close(self._ctx), "gw_serve_close")

@@ -316,4 +316,113 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
+// Block-wide top-k over n items with unique 64-bit rank keys (larger is
+// better; key 0 marks an empty item and is never selected), k <= kTopkMax.
+// An MSB radix select over LDS histograms finds the k-th largest key: one
+// pass counts the items and finds the largest key, the byte passes start at
+// its highest non-zero byte and stop as soon as the chosen bucket holds
+// exactly the keys still needed.  The keys at or above the threshold are
+// gathered into sel[] and bitonic-sorted descending.  Returns min(k,
+// non-empty items); every thread of the block calls it and gets the same
+// value.
+constexpr uint32_t kTopkMax = 1024;
+
+struct TopkScratch {
+    uint32_t hist[256];
+    unsigned long long prefix, maxkey;
+    uint32_t remaining, fill, valid;
+    int shift_done;  // the select stopped after the pass at this shift (-1: all passes)
+    unsigned long long sel[kTopkMax];
+};
+
+template <class KeyAt>
+__device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch& t) {
+    if (threadIdx.x == 0) {
+        t.valid = 0;
+        t.fill = 0;
+        t.prefix = 0;
+        t.maxkey = 0;
+        t.remaining = k;
+        t.shift_done = -1;
+    }
+    __syncthreads();
+    uint32_t v = 0;
+    unsigned long long mx = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long key = key_at(i);
+        v += key != 0ull;
+        mx = key > mx ? key : mx;
+    }
+    if (v) atomicAdd(&t.valid, v);
+    if (mx) atomicMax(&t.maxkey, mx);
+    __syncthreads();
+    const uint32_t valid = t.valid;
+    unsigned long long thr = 1;  // gather keys >= thr (every non-empty key when valid <= k)
+    if (valid > k) {
+        // bytes above the largest key's highest non-zero byte are zero in every key
+        int top = 56;
+        while (top > 0 && (t.maxkey >> top) == 0) top -= 8;
+        for (int shift = top; shift >= 0; shift -= 8) {
+            for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) t.hist[b] = 0;
+            __syncthreads();
+            const unsigned long long prefix = t.prefix;
+            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const unsigned long long key = key_at(i);
+                if (key != 0ull && (shift == top || (key >> (shift + 8)) == prefix))
+                    atomicAdd(&t.hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t above = 0;
+                for (int d = 255; d >= 0; --d) {
+                    if (above + t.hist[d] >= t.remaining) {
+                        t.prefix = (prefix << 8) | static_cast<unsigned long long>(d);
+                        t.remaining -= above;
+                        // the whole bucket is needed: its smallest possible key is the threshold
+                        if (t.hist[d] == t.remaining) t.shift_done = shift;
+                        break;
+                    }
+                    above += t.hist[d];
+                }
+            }
+            __syncthreads();
+            if (t.shift_done >= 0) break;
+        }
+        thr = t.shift_done >= 0 ? t.prefix << t.shift_done : t.prefix;
+        if (thr == 0) thr = 1;
+    }
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long key = key_at(i);
+        if (key != 0ull && key >= thr) t.sel[atomicAdd(&t.fill, 1u)] = key;
+    }
+    __syncthreads();
+    const uint32_t m = t.fill;
+    uint32_t size = 1;
+    while (size < m) size <<= 1;
+    for (uint32_t i = m + threadIdx.x; i < size; i += blockDim.x) t.sel[i] = 0;
+    __syncthreads();
+    for (uint32_t len = 2; len <= size; len <<= 1) {
+        for (uint32_t j = len >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
+                const uint32_t q = i ^ j;
+                if (q > i) {
+                    const bool desc = (i & len) == 0;
+                    const unsigned long long x = t.sel[i], y = t.sel[q];
+                    if (desc ? x < y : x > y) {
+                        t.sel[i] = y;
+                        t.sel[q] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    return m;
+}
+
+// Rank key of a (vertex, count) pair: more visits first, then the smaller vertex.
+__device__ __forceinline__ unsigned long long topk_key(uint32_t vertex, uint32_t count) {
+    return count ? (static_cast<unsigned long long>(count) << 32) | static_cast<uint32_t>(~vertex) : 0ull;
+}
+
 }  // namespace gw

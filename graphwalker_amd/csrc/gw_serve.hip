@@ -58,6 +58,9 @@ struct ServeState {
     std::deque<gw_query> queue;
     std::map<uint64_t, std::vector<unsigned long long>> done;  // qid -> packed (vertex << cbits | count)
     std::deque<uint64_t> done_order;
+    uint32_t topk = 0;                         // > 0: completed queries keep their top-k pairs only
+    unsigned long long* d_topk = nullptr;      // max_queries x topk packed pairs
+    uint32_t* d_topk_n = nullptr;              // max_queries
 };
 
 std::map<gw_ctx*, ServeState*>& states() {
@@ -269,6 +272,7 @@ __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __res
 struct Done {
     uint64_t base, size;   // region
     uint64_t out;          // first output index (after the scan)
+    uint32_t slot, source; // the query's slot (its source-vertex counter) and source
 };
 
 // Pass 1: non-empty slots per region (grid.y = region).
@@ -305,6 +309,37 @@ __global__ void serve_compact_kernel(unsigned long long* __restrict__ arena, con
             }
         }
     }
+}
+
+// Top-k of completed queries (gw_serve_topk): one workgroup per query over
+// its compacted pairs pairs[d.out .. d.out + n) (n = per-query pair count)
+// plus its source vertex (counted apart, in srccount).  Output packed
+// (vertex << cbits) | count, count descending, ties to the smaller vertex.
+__global__ __launch_bounds__(1024) void serve_topk_kernel(const unsigned long long* __restrict__ pairs,
+                                                        const Done* __restrict__ done,
+                                                        const unsigned long long* __restrict__ per_region,
+                                                        const uint32_t* __restrict__ srccount, uint32_t cbits,
+                                                        uint32_t k, unsigned long long* __restrict__ out,
+                                                        uint32_t* __restrict__ out_n) {
+    __shared__ TopkScratch t;
+    const Done d = done[blockIdx.x];
+    const uint64_t n = per_region[blockIdx.x];
+    const unsigned long long cmask = (1ull << cbits) - 1;
+    const uint32_t sc = srccount[d.slot];
+    const uint32_t m = block_topk(
+        n + 1, k,
+        [&](uint64_t i) {
+            if (i == n) return topk_key(d.source, sc);
+            const unsigned long long x = pairs[d.out + i];
+            return topk_key(static_cast<uint32_t>(x >> cbits), static_cast<uint32_t>(x & cmask));
+        },
+        t);
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        const unsigned long long key = t.sel[r];
+        out[static_cast<uint64_t>(blockIdx.x) * k + r] =
+            (static_cast<unsigned long long>(~static_cast<uint32_t>(key)) << cbits) | (key >> 32);
+    }
+    if (threadIdx.x == 0) out_n[blockIdx.x] = m;
 }
 
 template <int KIND>
@@ -464,7 +499,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         std::vector<Done> dn(fin.size());
         uint64_t maxsize = 0;
         for (size_t i = 0; i < fin.size(); ++i) {
-            dn[i] = Done{fin[i].base, fin[i].size, 0};
+            dn[i] = Done{fin[i].base, fin[i].size, 0, fin[i].slot, fin[i].source};
             maxsize = std::max(maxsize, fin[i].size);
         }
         const uint32_t nd = static_cast<uint32_t>(fin.size());
@@ -493,6 +528,33 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         serve_compact_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_done, S.d_region_n + nd,
                                                                           S.d_pairs);
         GW_HIP(ctx, hipGetLastError());
+        if (S.topk) {
+            // top-k per completed query on the device: k pairs per query cross PCIe
+            serve_topk_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_pairs, S.d_done, S.d_region_n,
+                                                                       S.d_srccount, S.cbits, S.topk, S.d_topk,
+                                                                       S.d_topk_n);
+            GW_HIP(ctx, hipGetLastError());
+            std::vector<unsigned long long> top(static_cast<uint64_t>(nd) * S.topk);
+            std::vector<uint32_t> tn(nd);
+            GW_HIP(ctx, hipMemcpyAsync(top.data(), S.d_topk, top.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+            GW_HIP(ctx, hipMemcpyAsync(tn.data(), S.d_topk_n, nd * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+            GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (uint32_t i = 0; i < nd; ++i) {
+                const auto& a = fin[i];
+                const auto first = top.begin() + static_cast<uint64_t>(i) * S.topk;
+                S.done[a.qid].assign(first, first + tn[i]);
+                S.done_order.push_back(a.qid);
+                arena_free(S, a.base, a.size);
+                S.free_slots.push_back(a.slot);
+            }
+            st.completed = nd;
+            S.active.swap(still);
+            st.live_walks = S.live;
+            st.active_queries = S.active.size();
+            st.queued = S.queue.size();
+            if (out) *out = st;
+            return GW_OK;
+        }
         std::vector<unsigned long long> all(total);
         if (total)
             GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -549,6 +611,28 @@ int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count
     return GW_OK;
 }
 
+int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
+    if (!ctx) return GW_EINVAL;
+    auto it = states().find(ctx);
+    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_topk: not open");
+    if (k > kTopkMax) return gw_fail(ctx, GW_EINVAL, "gw_serve_topk: k must be at most 1024");
+    ServeState& S = *it->second;
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    gw_free(ctx, S.d_topk, static_cast<uint64_t>(S.max_queries) * S.topk * 8);
+    gw_free(ctx, S.d_topk_n, S.max_queries * 4ull);
+    S.d_topk = nullptr;
+    S.d_topk_n = nullptr;
+    S.topk = 0;
+    if (k == 0) return GW_OK;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&S.d_topk, static_cast<uint64_t>(S.max_queries) * k * 8, "serve top-k")) ||
+        (rc = gw_alloc(ctx, (void**)&S.d_topk_n, S.max_queries * 4ull, "serve top-k counts")))
+        return rc;
+    S.topk = k;
+    return GW_OK;
+}
+
 int gw_serve_close(gw_ctx* ctx) {
     if (!ctx) return GW_EINVAL;
     auto it = states().find(ctx);
@@ -575,5 +659,7 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
     gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
     gw_free(ctx, S->d_region_n, S->max_queries * 16ull);
+    gw_free(ctx, S->d_topk, static_cast<uint64_t>(S->max_queries) * S->topk * 8);
+    gw_free(ctx, S->d_topk_n, S->max_queries * 4ull);
     delete S;
 }

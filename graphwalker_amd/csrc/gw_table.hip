@@ -442,8 +442,8 @@ __global__ __launch_bounds__(256) void gather_regions_kernel(const uint32_t* __r
 }
 
 
-constexpr uint32_t kTopkThreads = 256;
-constexpr uint32_t kTopkMax = 1024;
+constexpr uint32_t kTopkThreads = 1024;
+using gw::kTopkMax;
 
 __global__ void source_of_kernel(const unsigned long long* __restrict__ keys, uint64_t n, uint32_t vbits,
                                  uint64_t* __restrict__ src) {
@@ -452,90 +452,23 @@ __global__ void source_of_kernel(const unsigned long long* __restrict__ keys, ui
         src[i] = keys[i] >> vbits;
 }
 
-// One workgroup per source segment [seg[g], seg[g+1]) of the sorted pairs.
-// Rank key = (count << 32) | ~vertex: larger count first, then smaller
-// vertex; keys are unique, so an 8-pass MSB radix select (LDS histograms)
-// finds the k-th largest exactly, the keys at or above it are gathered and
-// bitonic-sorted in LDS.
+// One workgroup per source segment [seg[g], seg[g+1]) of the sorted pairs:
+// block_topk over the rank keys of the segment's (vertex, count) pairs.
 __global__ __launch_bounds__(kTopkThreads) void topk_kernel(const unsigned long long* __restrict__ keys,
                                                            const uint32_t* __restrict__ counts,
                                                            const uint64_t* __restrict__ seg, uint32_t vbits,
                                                            uint32_t k, uint32_t* __restrict__ out_s,
                                                            uint32_t* __restrict__ out_v, uint32_t* __restrict__ out_c,
                                                            uint32_t* __restrict__ out_n) {
-    __shared__ uint32_t hist[256];
-    __shared__ unsigned long long s_prefix;
-    __shared__ uint32_t s_remaining, s_fill;
-    __shared__ unsigned long long s_sel[kTopkMax];
+    __shared__ gw::TopkScratch t;
     const uint32_t g = blockIdx.x;
     const uint64_t lo = seg[g], n = seg[g + 1] - lo;
     const unsigned long long vmask = (1ull << vbits) - 1;
-    auto rank_key = [&](uint64_t i) {
-        const uint32_t v = static_cast<uint32_t>(keys[lo + i] & vmask);
-        return (static_cast<unsigned long long>(counts[lo + i]) << 32) | static_cast<uint32_t>(~v);
-    };
-    unsigned long long thr = 0;  // gather keys >= thr
-    if (n > k) {
-        if (threadIdx.x == 0) {
-            s_prefix = 0;
-            s_remaining = k;
-        }
-        for (int shift = 56; shift >= 0; shift -= 8) {
-            for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
-            __syncthreads();
-            const unsigned long long prefix = s_prefix;
-            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const unsigned long long key = rank_key(i);
-                if (shift == 56 || (key >> (shift + 8)) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t above = 0;
-                for (int d = 255; d >= 0; --d) {
-                    if (above + hist[d] >= s_remaining) {
-                        s_prefix = (prefix << 8) | static_cast<unsigned long long>(d);
-                        s_remaining -= above;
-                        break;
-                    }
-                    above += hist[d];
-                }
-            }
-            __syncthreads();
-        }
-        thr = s_prefix;
-    }
-    if (threadIdx.x == 0) s_fill = 0;
-    __syncthreads();
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const unsigned long long key = rank_key(i);
-        if (key >= thr) s_sel[atomicAdd(&s_fill, 1u)] = key;
-    }
-    __syncthreads();
-    const uint32_t m = s_fill;  // min(n, k)
-    uint32_t size = 1;
-    while (size < m) size <<= 1;
-    for (uint32_t i = m + threadIdx.x; i < size; i += blockDim.x) s_sel[i] = 0;  // below every real key
-    __syncthreads();
-    // bitonic sort, descending
-    for (uint32_t len = 2; len <= size; len <<= 1) {
-        for (uint32_t j = len >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const bool desc = (i & len) == 0;
-                    const unsigned long long x = s_sel[i], y = s_sel[p];
-                    if (desc ? x < y : x > y) {
-                        s_sel[i] = y;
-                        s_sel[p] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    const uint32_t m = gw::block_topk(
+        n, k, [&](uint64_t i) { return gw::topk_key(static_cast<uint32_t>(keys[lo + i] & vmask), counts[lo + i]); }, t);
     const uint32_t src = static_cast<uint32_t>(keys[lo] >> vbits);
     for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
-        const unsigned long long key = s_sel[r];
+        const unsigned long long key = t.sel[r];
         out_s[static_cast<uint64_t>(g) * k + r] = src;
         out_v[static_cast<uint64_t>(g) * k + r] = ~static_cast<uint32_t>(key);
         out_c[static_cast<uint64_t>(g) * k + r] = static_cast<uint32_t>(key >> 32);

@@ -280,6 +280,10 @@ int gw_serve_poll(gw_ctx *ctx, uint64_t *qid, uint64_t *npairs);
 /* Copies a completed query's (vertex, count) pairs, in no particular order,
  * and releases them. */
 int gw_serve_result(gw_ctx *ctx, uint64_t qid, uint32_t *vertex, uint32_t *count, uint64_t cap);
+/* Completed queries keep only their k most-visited vertices (1 <= k <= 1024;
+ * 0 = every pair, the default), selected on the device; gw_serve_result then
+ * returns them count descending, ties to the smaller vertex id. */
+int gw_serve_topk(gw_ctx *ctx, uint32_t k);
 int gw_serve_close(gw_ctx *ctx);
 
 /* Random-gather ceiling: `gathers` independent random 4-byte loads from a

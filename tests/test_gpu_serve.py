@@ -18,9 +18,11 @@ def expected(off, col, kind, L, qid, source, walks):
     return nz.astype(np.uint32), c[nz]
 
 
-def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64):
+def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64, topk=0):
     """arrivals: list of (step, qid, source, walks); submits each at its step."""
     engine.serve_open(gw.program(kind, L, 0.15, 0, seed=SEED), pool, arena, max_queries, hops)
+    if topk:
+        engine.serve_topk(topk)
     results, step, pending = {}, 0, sorted(arrivals)
     try:
         while pending or len(results) < len(arrivals):
@@ -31,7 +33,7 @@ def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64):
             st = engine.serve_step()
             assert st["live_walks"] <= pool
             while (r := engine.serve_poll()) is not None:
-                results[r[0]] = engine.serve_result(*r)
+                results[r[0]] = engine.serve_result(*r, sort=not topk)
             step += 1
             assert step < 10_000
     finally:
@@ -90,3 +92,19 @@ def test_serve_rejects_bad_queries(engine, g12):
             engine.serve_submit([(1, 5, 5000)])  # more walks than the pool
     finally:
         engine.serve_close()
+
+
+@pytest.mark.parametrize("k", [1, 10, 1024])
+def test_served_topk(engine, g12, k):
+    # completed queries keep their k most-visited vertices, count descending,
+    # ties to the smaller vertex -- the top of the exact counts
+    off, col = g12
+    rng = np.random.default_rng(k)
+    arrivals = [(int(rng.integers(0, 5)), 500 + i, int(rng.integers(0, 4096)), int(rng.integers(100, 4000)))
+                for i in range(20)]
+    res = serve_all(engine, arrivals, gw.RESTART, 10, 3, pool=100_000, arena=64 << 20, topk=k)
+    for _, qid, src, w in arrivals:
+        v, c = res[qid]
+        ev, ec = expected(off, col, gw.RESTART, 10, qid, src, w)
+        order = np.lexsort((ev, -ec.astype(np.int64)))[:k]
+        assert np.array_equal(v, ev[order]) and np.array_equal(c, ec[order]), qid
