@@ -176,6 +176,9 @@ def lib() -> C.CDLL:
         "gw_walks_read": ([P, P, u64], i),
         "gw_walks_current": ([P, P, u64], i),
         "gw_run": ([P, P, P], i),
+        "gw_run_capture": ([P, P], i),
+        "gw_run_replay": ([P, P], i),
+        "gw_autotune": ([P, P, u32, P, P], i),
         "gw_bucket": ([P, P], i),
         "gw_counts_read": ([P, P, u32], i),
         "gw_summary_read": ([P, P, P, u64], i),
@@ -354,6 +357,24 @@ class Engine:
         st = Stats()
         self._check(self._L.gw_run(self._ctx, C.byref(prog), C.byref(st)), "gw_run")
         return st.as_dict()
+
+    def run_capture(self, prog: Program) -> None:
+        """Run prog once, then capture its launch sequence in a hipGraph."""
+        self._check(self._L.gw_run_capture(self._ctx, C.byref(prog)), "gw_run_capture")
+
+    def run_replay(self) -> dict:
+        """Replay the captured run; stats as run() returns them (step_ms and
+        total_ms cover the whole graph)."""
+        st = Stats()
+        self._check(self._L.gw_run_replay(self._ctx, C.byref(st)), "gw_run_replay")
+        return st.as_dict()
+
+    def autotune(self, prog: Program, reps: int = 2) -> tuple[Program, float]:
+        """Fastest (block_threads, waves_per_simd) for prog: (program, ms)."""
+        best = Program()
+        ms = C.c_double()
+        self._check(self._L.gw_autotune(self._ctx, C.byref(prog), reps, C.byref(best), C.byref(ms)), "gw_autotune")
+        return best, ms.value
 
     def bucket(self, nblocks: int) -> np.ndarray:
         counts = np.zeros(nblocks, dtype=np.uint64)

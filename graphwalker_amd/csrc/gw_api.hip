@@ -210,6 +210,7 @@ int gw_create(gw_ctx** out, const gw_cfg* cfg) {
 void gw_destroy(gw_ctx* ctx) {
     if (!ctx) return;
     gw_serve_close(ctx);
+    gw_run_graph_free(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     gw_graph_free(ctx);

@@ -29,6 +29,8 @@ constexpr uint32_t kCntFlushed = 18;          // partitioned: flushed sources
 constexpr uint32_t kCntSplitRuns = 19;        // partitioned: sums of the split pairs
 constexpr uint32_t kCntAggQueue = 20;         // partitioned: next source for the aggregation workgroups
 
+struct gw_run_graph;  // gw_step.hip: a captured run (gw_run_capture)
+
 struct gw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -124,6 +126,8 @@ struct gw_ctx {
     unsigned long long* d_bucket_off = nullptr;  // nblocks + 2
 
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+    gw_run_graph* run_graph = nullptr;  // the last gw_run_capture
 };
 
 // Workgroups for a grid-stride loop over n items: at most 2^20, so the
@@ -161,6 +165,7 @@ int gw_transpose_build(gw_ctx* ctx);
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);
+void gw_run_graph_free(gw_ctx* ctx);
 // Builds the successor-header edges if they fit; false when the CSR is the only layout.
 bool gw_ensure_ext(gw_ctx* ctx);
 

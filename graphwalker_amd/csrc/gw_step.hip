@@ -692,8 +692,7 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     }
     ctx->table_slots = slots;
     ctx->table_wide = wide;
-    GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, bytes, ctx->stream));
-    return GW_OK;
+    return GW_OK;  // cleared by each run (a fill kernel, or a memset in the blocked schedule)
 }
 
 }  // namespace
@@ -701,9 +700,58 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
 
 bool gw_ensure_ext(gw_ctx* ctx) { return gw::ensure_ext(ctx); }
 
-int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
-    using namespace gw;
-    gw_stats st{};
+namespace gw {
+namespace {
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+}  // namespace
+
+// Everything gw_run decides before it enqueues work.
+struct RunPlan {
+    gw_program prog{};
+    uint32_t sched = 0, rec = 0;
+    bool ext = false, part = false;
+    Launch L{};
+    gw_stats st0{};  // schedule and layout
+};
+
+}  // namespace gw
+
+struct gw_run_graph {
+    gw::RunPlan plan;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> fp;             // what the graph's nodes point at and launch with
+    unsigned long long* h_cnt = nullptr;  // pinned: the counters the graph copies back
+};
+
+void gw_run_graph_free(gw_ctx* ctx) {
+    gw_run_graph* g = ctx->run_graph;
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->h_cnt) (void)hipHostFree(g->h_cnt);
+    delete g;
+    ctx->run_graph = nullptr;
+}
+
+namespace gw {
+namespace {
+
+// Validation and allocation: every buffer the run touches exists afterwards.
+int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
+    rp = RunPlan{};
+    rp.prog = *prog;
+    gw_stats& st = rp.st0;
     const uint32_t sched = prog->schedule != GW_SCHED_AUTO ? prog->schedule
                            : ctx->host_graph            ? GW_SCHED_BLOCKED
                                                         : GW_SCHED_PERSISTENT;
@@ -764,7 +812,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         int rc = gw_alloc(ctx, (void**)&ctx->d_work, ctx->walk_cap * sizeof(gw_walk), "blocked work buffer");
         if (rc) return rc;
     }
-    Launch L{};
+    Launch& L = rp.L;
     L.ctx = ctx;
     L.sched = sched;
     L.ext = ext;
@@ -779,55 +827,55 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
+    if (wps > 32) return gw_fail(ctx, GW_EINVAL, "gw_run: waves_per_simd must be at most 32");
     const uint64_t waves = static_cast<uint64_t>(std::max(1u, ctx->cu_count)) * 4u * wps;
     L.grid_persistent = static_cast<uint32_t>(std::max<uint64_t>(1, waves * 64 / L.tb));
-
     // Partitioned source counts: regions and pair buffers are sized before
     // the run so partition + aggregation follow the step kernel on the
     // stream with no host round trip in between.
-    bool part = false;
     if ((rec & kRecKeyLog) && prog->source_method != GW_SRC_GLOBAL && !ctx->field_walks.empty()) {
-        int rc = gw_partlog_prepare(ctx, keylog_per_walk(prog), &part);
+        int rc = gw_partlog_prepare(ctx, keylog_per_walk(prog), &rp.part);
         if (rc) return rc;
-        if (!part && prog->source_method == GW_SRC_PARTITIONED)
+        if (!rp.part && prog->source_method == GW_SRC_PARTITIONED)
             return gw_fail(ctx, GW_ENOMEM, "gw_run: the partitioned key log does not fit the HBM budget");
     }
-    {
-        int rc = gw_partlog_reset(ctx, part);
-        if (rc) return rc;
+    rp.sched = sched;
+    rp.rec = rec;
+    return GW_OK;
+}
+
+// The work of one run on the engine stream, for the persistent and plain
+// schedules: counter reset, recorder clears (fill kernels, so a captured
+// graph holds no memset nodes), the step kernel, and the partitioned
+// aggregation.  timed: HIP events around the step kernel and the aggregation.
+int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed) {
+    int rc = gw_partlog_reset(ctx, rp.part);
+    if (rc) return rc;
+    if (rp.rec & GW_REC_VERTEX_COUNT) {
+        fill_u32_kernel<<<dim3(gw_grid(ctx->n, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_counts, ctx->n);
+        GW_HIP(ctx, hipGetLastError());
     }
-    if (rec & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
-    float ms = 0.f;
-    if (sched == GW_SCHED_BLOCKED) {
-        GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        int rc = !ctx->nwalks     ? GW_OK
-                 : ctx->host_graph ? run_streamed(ctx, L, prog, rec, st)
-                                   : run_blocked(ctx, L, prog, rec, st);
-        if (rc) return rc;
-        GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-    } else {
-        GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        if (ctx->nwalks) {
-            hipError_t e = launch(L, prog->kind, rec);
-            if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
-            st.launches = 1;
-        }
-        GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        st.rounds = 1;
+    if (rp.rec & GW_REC_SOURCE_COUNT) {  // hash table
+        const uint64_t words = ctx->table_bytes / 8;
+        fill_u64_kernel<<<dim3(gw_grid(words, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_table, words);
+        GW_HIP(ctx, hipGetLastError());
     }
-    if (part) {
-        int rc = gw_partlog_enqueue(ctx);
-        if (rc) return rc;
+    if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if (ctx->nwalks) {
+        hipError_t e = launch(rp.L, rp.prog.kind, rp.rec);
+        if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
     }
-    GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-    unsigned long long cnt[kCounters];
-    GW_HIP(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-    st.total_ms = ms;
-    if (sched != GW_SCHED_BLOCKED) st.step_ms = ms;
+    if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    if (rp.part && (rc = gw_partlog_enqueue(ctx))) return rc;
+    if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    return GW_OK;
+}
+
+// Counters read back -> stats; source-count aggregation that needs the host.
+int finish_run(gw_ctx* ctx, RunPlan& rp, const unsigned long long* cnt, gw_stats& st, double reduce_ms) {
     st.steps = cnt[kCntSteps];
     st.visits = cnt[kCntVisits];
+    const uint32_t rec = rp.rec;
     if ((rec & GW_REC_SOURCE_COUNT) && cnt[kCntTableOverflow])
         return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[kCntTableOverflow]) +
                                            " visits not recorded); raise table_log2");
@@ -837,13 +885,12 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         double rms = 0;
         bool done = false;
         int rc;
-        if (part) {
-            GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]));
-            rms = ms;
+        if (rp.part) {
+            rms = reduce_ms;
             double extra = 0;
             if ((rc = gw_partlog_finish(ctx, cnt, &done, &extra))) return rc;
             rms += extra;
-            if (!done && prog->source_method == GW_SRC_PARTITIONED)
+            if (!done && rp.prog.source_method == GW_SRC_PARTITIONED)
                 return gw_fail(ctx, GW_ENOMEM, "gw_run: a source's visits outgrew its key-log region (GW_SRC_PARTITIONED)");
         }
         if (done) {
@@ -859,6 +906,174 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         st.reduce_ms = rms;
     }
     st.source_method = (rec & (GW_REC_SOURCE_COUNT | kRecKeyLog)) ? ctx->src_method : 0u;
+    return GW_OK;
+}
+
+// Every device pointer, size and launch parameter a captured run bakes into
+// its graph: a replay is valid while the freshly planned run has the same.
+std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
+    auto p = [](const void* x) { return static_cast<uint64_t>(reinterpret_cast<uintptr_t>(x)); };
+    return {p(ctx->d_walks), ctx->nwalks, p(ctx->d_final), p(ctx->d_hash), p(ctx->d_visits), p(ctx->d_counts),
+            ctx->n, p(ctx->d_off), p(ctx->d_col), p(ctx->d_ext), p(ctx->d_events), ctx->events_cap,
+            p(ctx->d_table), ctx->table_bytes, ctx->table_slots, ctx->table_log2, ctx->table_vbits, ctx->table_cbits,
+            ctx->table_wide, p(ctx->d_keys), ctx->keys_cap, ctx->keys_sentinel, p(ctx->d_part_keys),
+            p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), ctx->part_fields,
+            ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
+            rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count};
+}
+
+}  // namespace
+}  // namespace gw
+
+int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
+    using namespace gw;
+    RunPlan rp;
+    int rc = plan_run(ctx, prog, rp);
+    if (rc) return rc;
+    gw_stats st = rp.st0;
+    float ms = 0.f;
+    double reduce_ms = 0;
+    if (rp.sched == GW_SCHED_BLOCKED) {
+        if ((rc = gw_partlog_reset(ctx, rp.part))) return rc;
+        if (rp.rec & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
+        if (rp.rec & GW_REC_SOURCE_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, ctx->table_bytes, ctx->stream));
+        GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        rc = !ctx->nwalks     ? GW_OK
+             : ctx->host_graph ? run_streamed(ctx, rp.L, prog, rp.rec, st)
+                               : run_blocked(ctx, rp.L, prog, rp.rec, st);
+        if (rc) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        if (rp.part && (rc = gw_partlog_enqueue(ctx))) return rc;
+        GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    } else {
+        if ((rc = enqueue_run(ctx, rp, true))) return rc;
+        st.launches = ctx->nwalks ? 1 : 0;
+        st.rounds = 1;
+    }
+    unsigned long long cnt[kCounters];
+    GW_HIP(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    st.total_ms = ms;
+    if (rp.sched != GW_SCHED_BLOCKED) st.step_ms = ms;
+    if (rp.part) {
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]));
+        reduce_ms = ms;
+    }
+    if ((rc = finish_run(ctx, rp, cnt, st, reduce_ms))) return rc;
     if (stats) *stats = st;
     return GW_OK;
 }
+
+extern "C" {
+
+int gw_run_capture(gw_ctx* ctx, const gw_program* prog) {
+    using namespace gw;
+    if (!ctx || !prog) return gw_fail(ctx, GW_EINVAL, "gw_run_capture: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_run_graph_free(ctx);
+    // one ordinary run first: allocations settle and the run's own checks pass
+    gw_stats st0;
+    int rc = gw_run(ctx, prog, &st0);
+    if (rc) return rc;
+    RunPlan rp;
+    if ((rc = plan_run(ctx, prog, rp))) return rc;
+    if (rp.sched == GW_SCHED_BLOCKED)
+        return gw_fail(ctx, GW_EINVAL, "gw_run_capture: the blocked schedule decides rounds on the host");
+    if ((rp.rec & kRecKeyLog) && !rp.part)
+        return gw_fail(ctx, GW_EINVAL, "gw_run_capture: the global sort-reduce is not captured (use partitioned or hash counts)");
+    gw_run_graph* g = new gw_run_graph();
+    if (hipHostMalloc(reinterpret_cast<void**>(&g->h_cnt), kCounters * 8, hipHostMallocDefault) != hipSuccess) {
+        delete g;
+        return gw_fail(ctx, GW_ENOMEM, "gw_run_capture: pinned counter buffer");
+    }
+    ctx->run_graph = g;
+    g->plan = rp;
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GW_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    rc = enqueue_run(ctx, g->plan, false);
+    hipError_t e = rc ? hipSuccess : hipMemcpyAsync(g->h_cnt, ctx->d_counters, kCounters * 8, hipMemcpyDeviceToHost, ctx->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+    g->graph = graph;
+    if (rc || e != hipSuccess || e2 != hipSuccess) {
+        gw_run_graph_free(ctx);
+        if (rc) return rc;
+        return gw_hip_check(ctx, e != hipSuccess ? e : e2, "gw_run_capture: stream capture");
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        gw_run_graph_free(ctx);
+        return gw_hip_check(ctx, e, "gw_run_capture: graph instantiate");
+    }
+    g->fp = fingerprint(ctx, g->plan);
+    return GW_OK;
+}
+
+int gw_run_replay(gw_ctx* ctx, gw_stats* stats) {
+    using namespace gw;
+    if (!ctx) return GW_EINVAL;
+    gw_run_graph* g = ctx->run_graph;
+    if (!g || !g->exec) return gw_fail(ctx, GW_ESTATE, "gw_run_replay: nothing captured");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    // plan the run again: checks, allocations, region offsets of the current
+    // walks; the graph is valid while nothing it points at moved
+    RunPlan rp;
+    int rc = plan_run(ctx, &g->plan.prog, rp);
+    if (rc) return rc;
+    if (fingerprint(ctx, rp) != g->fp)
+        return gw_fail(ctx, GW_ESTATE, "gw_run_replay: buffers, graph or walk count changed since the capture");
+    GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    GW_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    gw_stats st = g->plan.st0;
+    st.total_ms = ms;
+    st.step_ms = ms;  // the whole graph: reset, step kernel, aggregation, counter copy
+    st.launches = ctx->nwalks ? 1 : 0;
+    st.rounds = 1;
+    if ((rc = finish_run(ctx, g->plan, g->h_cnt, st, 0.0))) return rc;
+    if (stats) *stats = st;
+    return GW_OK;
+}
+
+// Launch-shape autotuner: the same program at each (block_threads,
+// waves_per_simd) candidate, best of `reps` runs after one warm-up; results
+// do not depend on the launch shape (walks are keyed by id), only time does.
+int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* best, double* best_ms) {
+    if (!ctx || !prog || !best || reps == 0) return gw_fail(ctx, GW_EINVAL, "gw_autotune: bad arguments");
+    static const uint32_t kThreads[] = {64, 128, 256};
+    static const uint32_t kWaves[] = {2, 4, 6, 8, 12, 16};
+    const uint32_t sched = prog->schedule != GW_SCHED_AUTO ? prog->schedule
+                           : ctx->host_graph            ? GW_SCHED_BLOCKED
+                                                        : GW_SCHED_PERSISTENT;
+    double bms = 1e300;
+    gw_program bp = *prog;
+    for (uint32_t tb : kThreads) {
+        for (uint32_t wps : kWaves) {
+            if (sched == GW_SCHED_PLAIN && wps != kWaves[0]) continue;  // the plain grid has one walk per thread
+            gw_program p = *prog;
+            p.block_threads = tb;
+            p.waves_per_simd = sched == GW_SCHED_PLAIN ? 0 : wps;
+            gw_stats st;
+            int rc = gw_run(ctx, &p, &st);
+            if (rc) return rc;
+            double m = 1e300;
+            for (uint32_t r = 0; r < reps; ++r) {
+                if ((rc = gw_run(ctx, &p, &st))) return rc;
+                m = std::min(m, st.total_ms);
+            }
+            if (m < bms) {
+                bms = m;
+                bp = p;
+            }
+        }
+    }
+    *best = bp;
+    if (best_ms) *best_ms = bms;
+    return GW_OK;
+}
+
+}  // extern "C"

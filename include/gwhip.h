@@ -33,6 +33,8 @@
  *                              of rawrandomwalks.cpp:27-50 / rwdomination.cpp:264-284
  *                              (one walk per vertex) and msppr.cpp:151-179
  *                              (walkspersource walks per source)
+ *   gw_run_capture / gw_run_replay / gw_autotune   (no counterpart: hipGraph
+ *                              replay of a run and a launch-shape search)
  *   gw_run                     graphwalker_engine::run (graphwalker.hpp:272-311)
  *                              with exec_updates (graphwalker.hpp:257-270)
  *                              calling updateByWalk per walk (randomwalk.hpp:39)
@@ -224,6 +226,19 @@ int gw_sources(uint64_t seed, uint32_t n, uint32_t count, uint32_t *out);
 
 /* Run every loaded walk to completion. */
 int gw_run(gw_ctx *ctx, const gw_program *prog, gw_stats *stats);
+/* hipGraph of one run (persistent or plain schedule; vertex counts, walk
+ * summaries, events, hash or partitioned source counts): gw_run_capture runs
+ * the program once, then captures its launch sequence -- counter reset,
+ * recorder clears, step kernel, aggregation, counter read-back -- into a
+ * graph; gw_run_replay launches it and finishes the run as gw_run does
+ * (total_ms and step_ms: the whole graph).  A replay is refused once device
+ * buffers, the graph or the walks changed since the capture. */
+int gw_run_capture(gw_ctx *ctx, const gw_program *prog);
+int gw_run_replay(gw_ctx *ctx, gw_stats *stats);
+/* Launch-shape autotuner: runs prog at every (block_threads, waves_per_simd)
+ * candidate, `reps` timed runs each after a warm-up, and returns the fastest
+ * in *best (results do not depend on the launch shape). */
+int gw_autotune(gw_ctx *ctx, const gw_program *prog, uint32_t reps, gw_program *best, double *best_ms);
 
 /* Stable counting sort of the loaded walks by block of their vertex (out of
  * range vertices are dropped); bucket_counts[nblocks] receives the sizes. */
