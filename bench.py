@@ -11,7 +11,10 @@ seed 3), 4096 uniform sources x 2000 walks, stop probability 0.15, cap 20,
 exact sparse per-(source, vertex) u32 counts.
 
 A bench "step" is one pass of the engine over all walks (gw_run), graph and
-walk records resident in HBM before timing.  Prints one JSON line.  The CPU
+walk records resident in HBM before timing.  The launch shape (block threads x
+waves per SIMD) is picked by gw_autotune before the timed passes, or fixed
+with --shape; "graph_replay" reports the same passes replayed from one
+captured hipGraph.  Prints one JSON line.  The CPU
 baseline is the oracle's OpenMP replay of the same walk program on a bounded
 sample of the same walks, on the host's cores.
 """
@@ -79,9 +82,12 @@ def setup(eng: gw.Engine, wl: dict) -> int:
     return n
 
 
-def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int, source_method=gw.SRC_SORT):
+def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int, source_method=gw.SRC_SORT,
+          shape=None):
     prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
                       layout=layout, source_method=source_method)
+    if shape:
+        prog.block_threads, prog.waves_per_simd = shape
     for _ in range(W):
         eng.run(prog)
     t = time.perf_counter()
@@ -93,6 +99,21 @@ def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: 
         visits += st["visits"]
     wall = time.perf_counter() - t
     return prog, wall, steps, visits, kms, st
+
+
+def replayed(eng: gw.Engine, prog: gw.Program, K: int) -> dict:
+    """K passes of prog replayed from one captured hipGraph (one launch call
+    per pass instead of the run's host sequence)."""
+    eng.run_capture(prog)
+    eng.run_replay()
+    t = time.perf_counter()
+    steps, gms = 0, []
+    for _ in range(K):
+        st = eng.run_replay()
+        steps += st["steps"]
+        gms.append(st["total_ms"])
+    wall = time.perf_counter() - t
+    return {"walk_steps_per_s": steps / wall, "graph_ms": float(np.mean(gms)), "wall_s": wall}
 
 
 def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) -> dict:
@@ -124,10 +145,23 @@ def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) ->
                       f"same device-generated graph: {r['steps']} steps in {elapsed:.2f} s"}
 
 
-def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool) -> dict:
+def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool,
+            shape=None) -> dict:
     wl = WORKLOADS[name]
     n = setup(eng, wl)
     variants = {}
+    tuned = None
+    if shape:
+        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "run_ms": None}
+    else:
+        # launch shape (block size x waves per SIMD) picked by gw_autotune before the timed passes
+        base = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
+                          layout=layout)
+        best, best_ms = eng.autotune(base, reps=2)
+        shape = (best.block_threads, best.waves_per_simd)
+        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "run_ms": best_ms}
+        print(f"[bench] {name} autotune: {shape[0]} threads x {shape[1]} waves/SIMD, {best_ms:.2f} ms",
+              file=sys.stderr, flush=True)
     if compare:
         for s_, l_ in (("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "ext")):
             if (s_, l_) == (sched, layout):
@@ -136,7 +170,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
             variants[f"{s_}-{l_}"] = {"walk_steps_per_s": st_ / w_, "kernel_ms": float(np.mean(k_)),
                                       "rounds": last["rounds"], "bucket_ms": last["bucket_ms"]}
             print(f"[bench] {name} {s_}-{l_}: {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
-    prog, wall, steps, visits, kms, st = timed(eng, wl, n, sched, layout, K, W)
+    prog, wall, steps, visits, kms, st = timed(eng, wl, n, sched, layout, K, W, shape=shape)
     per_pass_steps, per_pass_visits = steps // K, visits // K
     kavg = float(np.mean(kms))
     abytes = algorithmic_bytes(per_pass_steps, per_pass_visits, wl["recorder"], st["layout"])
@@ -145,8 +179,13 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     res = {
         "value": steps / wall, "wall_s": wall, "steps": per_pass_steps, "visits": per_pass_visits, "prog": prog,
         "kernel_ms": kavg, "algorithmic_bytes": abytes, "achieved": achieved, "layout": st["layout"], "n": n,
-        "variants": variants, "wl": wl, "last": st,
+        "variants": variants, "wl": wl, "last": st, "tuned": tuned,
     }
+    if sched != "blocked":
+        r = replayed(eng, prog, K)
+        res["graph_replay"] = r
+        print(f"[bench] {name} {sched}-{layout} graph replay: {r['walk_steps_per_s'] / 1e9:.2f} G steps/s",
+              file=sys.stderr, flush=True)
     if "plain-csr" in variants:
         res["speedup_vs_plain"] = res["value"] / variants["plain-csr"]["walk_steps_per_s"]
     return res
@@ -315,6 +354,8 @@ def main() -> int:
     ap.add_argument("--secondary", default="ppr4096-rmat26", help="extra workload reported under 'workloads' ('' = none)")
     ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent", "blocked"])
     ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
+    ap.add_argument("--shape", default="", help="launch shape 'THREADS,WAVES' (block threads, waves per SIMD); "
+                                               "default: picked by the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
@@ -332,7 +373,8 @@ def main() -> int:
     info = eng.device_info()
     print(f"[bench] {info}", file=sys.stderr, flush=True)
 
-    m = measure(eng, args.workload, args.steps, args.warmup, args.schedule, args.layout, compare=True)
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+    m = measure(eng, args.workload, args.steps, args.warmup, args.schedule, args.layout, compare=True, shape=shape)
     wl = m["wl"]
     cpu = None
     if not args.no_cpu:
@@ -358,7 +400,8 @@ def main() -> int:
 
     workloads = {}
     if args.secondary and args.secondary != args.workload:
-        s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False)
+        s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False,
+                    shape=shape)
         _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_GLOBAL)
         _, wh_, sth_, _, _, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_HASH)
         _, wg_, stg_, _, _, lg_ = timed(eng, s["wl"], s["n"], args.schedule, args.layout, 2, 1,
@@ -375,6 +418,7 @@ def main() -> int:
                               gw.SRC_HASH: "hash"}.get(last["source_method"], "?"),
             "reduce_ms": last["reduce_ms"], "flushed_sources": last["flushed_sources"],
             "global_sort_walk_steps_per_s": stg_ / wg_, "global_sort_reduce_ms": lg_["reduce_ms"],
+            "launch_shape": s["tuned"], "graph_replay": s.get("graph_replay"),
         }
 
     serving = {}
@@ -422,6 +466,8 @@ def main() -> int:
                                   "frac_of_ceiling": m["achieved"] / rand_gbps},
         "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
         "variants": m["variants"],
+        "launch_shape": m["tuned"],
+        "graph_replay": m.get("graph_replay"),
         "workloads": workloads,
         "serving": serving,
         "out_of_core": stream,
