@@ -398,6 +398,17 @@ def main() -> int:
         t = json.load(open(tfile))
         traffic, traffic_copy = t.get("bytes_per_launch"), t.get("bytes_per_launch_copy_calibrated")
 
+    # post-processing on the headline graph: top-k of the per-vertex counts (the toplist caller)
+    post = {}
+    eng.run(gw.program(wl["kind"], wl["L"], wl["p"], m["n"], seed=wl["seed"], recorder=gw.REC_VERTEX_COUNT,
+                       schedule=args.schedule, layout=args.layout))
+    for k in (100, 1 << 20):
+        eng.counts_topk(k)
+        t = time.perf_counter()
+        eng.counts_topk(k)
+        post[f"counts_topk_{k}_ms"] = (time.perf_counter() - t) * 1e3
+    print(f"[bench] counts top-k: {post}", file=sys.stderr, flush=True)
+
     workloads = {}
     if args.secondary and args.secondary != args.workload:
         s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False,
@@ -472,6 +483,7 @@ def main() -> int:
         "serving": serving,
         "out_of_core": stream,
         "workloads_simrank": simrank,
+        "post_processing": post,
     }
     if "speedup_vs_plain" in m:
         out["speedup_vs_plain"] = m["speedup_vs_plain"]

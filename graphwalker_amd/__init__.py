@@ -185,6 +185,7 @@ def lib() -> C.CDLL:
         "gw_events_read": ([P, P, u64, u32], i),
         "gw_source_counts_read": ([P, P, P, P, u64, P], i),
         "gw_source_topk": ([P, u32, P, P, P, u64, P], i),
+        "gw_counts_topk": ([P, u64, P, P, P], i),
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
         "gw_h2d_probe": ([P, u64, P], i),
@@ -420,6 +421,16 @@ class Engine:
         self._check(self._L.gw_source_topk(self._ctx, k, _ptr(s), _ptr(v), _ptr(c), n.value, C.byref(n)),
                     "gw_source_topk")
         return s, v, c
+
+    def counts_topk(self, k: int) -> tuple[np.ndarray, np.ndarray]:
+        """The k most-visited vertices of the last per-vertex count run (count
+        descending, ties to the smaller vertex id): (vertex, count)."""
+        m = min(int(k), self.graph_info()["n"]) if k > 0 else 0
+        v = np.empty(max(m, 1), dtype=np.uint32)
+        c = np.empty(max(m, 1), dtype=np.uint32)
+        n = C.c_uint64()
+        self._check(self._L.gw_counts_topk(self._ctx, k, _ptr(v), _ptr(c), C.byref(n)), "gw_counts_topk")
+        return v[:n.value], c[:n.value]
 
     def source_counts(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(source, vertex, count) of the last GW_REC_SOURCE_COUNT run, sorted by source then vertex."""

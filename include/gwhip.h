@@ -46,6 +46,8 @@
  *                              (msppr.cpp:71-76, discretedistribution.hpp:44-57), exact
  *   gw_source_topk             DiscreteDistribution::getTop (discretedistribution.hpp:72-95),
  *                              per source, on the device
+ *   gw_counts_topk             get_top_vertices (util/toplist.hpp:86-183) over the
+ *                              per-vertex counts, on the device
  *   gw_events_read             the per-step updateInfo callback (randomwalk.hpp:32):
  *                              each walk's visits, replayed on the host in walk order
  *   gw_graph_transpose         (no counterpart: the reference's SimRank reads a
@@ -256,6 +258,11 @@ int gw_source_counts_read(gw_ctx *ctx, uint32_t *source, uint32_t *vertex, uint3
  * vertex id; sources ascending.  NULL arrays: *n_out = entry count only. */
 int gw_source_topk(gw_ctx *ctx, uint32_t k, uint32_t *source, uint32_t *vertex, uint32_t *count, uint64_t cap,
                    uint64_t *n_out);
+/* Top-k vertices of the per-vertex counts of the last GW_REC_VERTEX_COUNT run:
+ * count descending, ties to the smaller vertex id.  k is capped at the vertex
+ * count; vertex and count hold at least min(k, n) entries; *n_out = entries
+ * written. */
+int gw_counts_topk(gw_ctx *ctx, uint64_t k, uint32_t *vertex, uint32_t *count, uint64_t *n_out);
 /* Event log of the last run with GW_REC_EVENTS: walk id w's visit k is
  * out[w * cap + k] = (hop << 32) | vertex for k < min(visits[w], cap).
  * `n_walks` and `cap` must equal the loaded walk count and the run's event_cap. */

@@ -359,3 +359,45 @@ def test_source_topk(engine, small_graph, method, k):
     s, v, c = engine.source_topk(k)
     es, ev, ec = _topk(*_expected_source_counts(prog, off, col, sources, field, wps), k)
     assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
+
+
+def _np_topk(counts, k):
+    order = np.lexsort((np.arange(counts.shape[0]), -counts.astype(np.int64)))[:k]
+    return order.astype(np.uint32), counts[order]
+
+
+@pytest.mark.parametrize("L,k", [(10, 1), (10, 100), (10, 5000), (3, 3000), (3, 1 << 12), (3, 1 << 20)])
+def test_counts_topk_equals_numpy(engine, L, k):
+    # L = 3 leaves most counts small: many ties at the threshold, and k beyond
+    # the visited vertices (threshold 0); k above n returns all n vertices
+    off, col = O.rmat_graph(12, 1 << 15, seed=5)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.walks_init_per_vertex(3)
+    engine.run(gw.program(gw.STOP, L, 0.15, n, seed=2, recorder=gw.REC_VERTEX_COUNT))
+    counts = engine.counts()
+    v, c = engine.counts_topk(k)
+    ev, ec = _np_topk(counts, min(k, n))
+    assert np.array_equal(v, ev) and np.array_equal(c, ec)
+
+
+def test_counts_topk_large_counts(engine):
+    # counts spread over all four digits of the radix select (walks pile up on hubs)
+    off, col = O.rmat_graph(16, 1 << 20, seed=9)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.walks_init_per_vertex(64)
+    engine.run(gw.program(gw.JUMP, 40, 0.15, n, seed=4, recorder=gw.REC_VERTEX_COUNT, schedule="persistent"))
+    counts = engine.counts()
+    assert counts.max() >= 1 << 16
+    for k in (1, 777, 1 << 14, n):
+        v, c = engine.counts_topk(k)
+        ev, ec = _np_topk(counts, k)
+        assert np.array_equal(v, ev) and np.array_equal(c, ec), k
+
+
+def test_counts_topk_invalid(engine):
+    off, col = O.rmat_graph(8, 1 << 10, seed=1)
+    engine.graph_set(off, col)
+    with pytest.raises(gw.GwError, match="bad arguments"):
+        engine.counts_topk(0)
