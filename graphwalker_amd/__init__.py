@@ -32,6 +32,7 @@ LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_EXT = 0, 1, 2
 LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
 # per-source count aggregation
 SRC_SORT, SRC_HASH, SRC_GLOBAL, SRC_PARTITIONED = 0, 1, 2, 3
+POOL_KEEP_FINAL = 1
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
 
@@ -103,6 +104,8 @@ class Stats(C.Structure):
         ("blocks_loaded", C.c_uint32),
         ("cache_slots", C.c_uint32),
         ("copy_ms", C.c_double),
+        ("pool_windows", C.c_uint64),
+        ("pool_bytes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -171,6 +174,7 @@ def lib() -> C.CDLL:
         "gw_graph_info": ([P, P, P, P], i),
         "gw_graph_read": ([P, P, P], i),
         "gw_walks_load": ([P, P, u64], i),
+        "gw_walks_load_host": ([P, P, u64, u64, u32], i),
         "gw_walks_init_per_vertex": ([P, u32], i),
         "gw_walks_init_sources": ([P, P, P, u32, u32], i),
         "gw_walks_read": ([P, P, u64], i),
@@ -319,6 +323,17 @@ class Engine:
     def walks_load(self, walks: np.ndarray) -> None:
         walks = np.ascontiguousarray(walks, dtype=WALK_DTYPE)
         self._check(self._L.gw_walks_load(self._ctx, _ptr(walks), walks.shape[0]), "gw_walks_load")
+        self.nwalks = walks.shape[0]
+
+    def walks_load_host(self, walks: np.ndarray, window: int, keep_final: bool = True) -> None:
+        """Host walk pool: the records (walks[i].id == i) stay in host memory
+        -- an array from host_empty is copied from directly -- and runs step
+        them `window` walks at a time.  The array is kept alive here."""
+        if walks.dtype != WALK_DTYPE or not walks.flags.c_contiguous:
+            walks = np.ascontiguousarray(walks, dtype=WALK_DTYPE)
+        self._check(self._L.gw_walks_load_host(self._ctx, _ptr(walks), walks.shape[0], window,
+                                               POOL_KEEP_FINAL if keep_final else 0), "gw_walks_load_host")
+        self._pool = walks
         self.nwalks = walks.shape[0]
 
     def walks_init_per_vertex(self, per_vertex: int = 1) -> None:

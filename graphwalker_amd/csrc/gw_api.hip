@@ -84,6 +84,7 @@ void free_walks(gw_ctx* ctx) {
 }
 
 int reserve_walks(gw_ctx* ctx, uint64_t n) {
+    gw_pool_free(ctx);  // device walks replace a host pool
     if (n <= ctx->walk_cap) {
         ctx->nwalks = n;
         return GW_OK;
@@ -214,6 +215,7 @@ void gw_destroy(gw_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     gw_graph_free(ctx);
+    gw_pool_free(ctx);
     free_walks(ctx);
     gw_free(ctx, ctx->d_counters, kCounters * sizeof(unsigned long long));
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
@@ -343,17 +345,19 @@ int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
     if (!ctx || (n && !walks)) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: bad arguments");
     if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_load: no graph");
     if (n >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_load: walk ids are 32-bit");
-    uint32_t src_max = 0;
+    uint32_t src_max = 0, v_max = 0;
     for (uint64_t i = 0; i < n; ++i) {
         if (walks[i].id >= n || walks[i].vertex >= ctx->n)
             return gw_fail(ctx, GW_EINVAL, "gw_walks_load: walk " + std::to_string(i) + " has id >= n or vertex out of range");
         src_max = std::max(src_max, walks[i].source);
+        v_max = std::max(v_max, walks[i].vertex);
     }
     GW_HIP(ctx, hipSetDevice(ctx->device));
     int rc = reserve_walks(ctx, n);
     if (rc) return rc;
     if (n) GW_HIP(ctx, hipMemcpy(ctx->d_walks, walks, n * sizeof(gw_walk), hipMemcpyHostToDevice));
     ctx->walk_src_max = src_max;
+    ctx->walk_v_max = v_max;
     ctx->field_walks.clear();
     ctx->part_per_walk = 0;
     if (n && src_max < kPartFieldLimit) {
@@ -376,6 +380,7 @@ int gw_walks_init_per_vertex(gw_ctx* ctx, uint32_t per_vertex) {
     GW_HIP(ctx, hipGetLastError());
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->walk_src_max = ctx->n - 1;
+    ctx->walk_v_max = ctx->n - 1;
     ctx->field_walks.clear();
     ctx->part_per_walk = 0;
     if (ctx->n <= kPartFieldLimit) ctx->field_walks.assign(ctx->n, per_vertex);
@@ -387,10 +392,11 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     if (!ctx || !sources || nsources == 0 || walks_per_source == 0)
         return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: bad arguments");
     if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_walks_init_sources: no graph");
-    uint32_t src_max = 0;
+    uint32_t src_max = 0, v_max = 0;
     for (uint32_t k = 0; k < nsources; ++k) {
         if (sources[k] >= ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: source out of range");
         src_max = std::max(src_max, source_field ? source_field[k] : sources[k]);
+        v_max = std::max(v_max, sources[k]);
     }
     const uint64_t nw = static_cast<uint64_t>(nsources) * walks_per_source;
     if (nw >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_walks_init_sources: walk ids are 32-bit");
@@ -412,6 +418,7 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     gw_free(ctx, d_field, nsources * 4ull);
     if (e != hipSuccess) return gw_hip_check(ctx, e, "init_sources_kernel");
     ctx->walk_src_max = src_max;
+    ctx->walk_v_max = v_max;
     ctx->field_walks.clear();
     ctx->part_per_walk = 0;
     if (src_max < kPartFieldLimit) {
@@ -423,6 +430,13 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
 
 int gw_walks_read(gw_ctx* ctx, gw_walk* out, uint64_t n) {
     if (!ctx || (n && !out)) return gw_fail(ctx, GW_EINVAL, "gw_walks_read: bad arguments");
+    if (ctx->pool) {
+        const gw_pool& P = *ctx->pool;
+        if (!P.keep_final || !P.final_valid || n > P.n)
+            return gw_fail(ctx, GW_ESTATE, "gw_walks_read: the host pool keeps no final records (GW_POOL_KEEP_FINAL) or no run yet");
+        std::copy(P.h_final, P.h_final + n, out);
+        return GW_OK;
+    }
     if (n > ctx->nwalks) return gw_fail(ctx, GW_EINVAL, "gw_walks_read: more records requested than loaded");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -432,6 +446,7 @@ int gw_walks_read(gw_ctx* ctx, gw_walk* out, uint64_t n) {
 
 int gw_walks_current(gw_ctx* ctx, gw_walk* out, uint64_t n) {
     if (!ctx || (n && !out)) return gw_fail(ctx, GW_EINVAL, "gw_walks_current: bad arguments");
+    if (ctx->pool) return gw_fail(ctx, GW_ESTATE, "gw_walks_current: the walks are in a host pool");
     if (n > ctx->nwalks) return gw_fail(ctx, GW_EINVAL, "gw_walks_current: more records requested than loaded");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -460,6 +475,8 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     // An uncapped program must end: require a stop probability of at least 2^-12.
     if (prog->kind == GW_PROB && prog->stop_thr < (1ull << 20))
         return gw_fail(ctx, GW_EINVAL, "gw_run: PROB needs stop_thr >= 2^20 (mean walk length <= 4096)");
+    if (ctx->nwalks && ctx->walk_v_max >= ctx->n)
+        return gw_fail(ctx, GW_ESTATE, "gw_run: walks start at vertices outside the graph (load walks again after changing the graph)");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     return gw_step_run(ctx, prog, stats);
 }
@@ -475,6 +492,13 @@ int gw_counts_read(gw_ctx* ctx, uint32_t* counts, uint32_t n) {
 
 int gw_summary_read(gw_ctx* ctx, uint64_t* hash, uint32_t* visits, uint64_t n) {
     if (!ctx) return GW_EINVAL;
+    if (ctx->pool) {
+        const gw_pool& P = *ctx->pool;
+        if (!P.summary_valid || n > P.n) return gw_fail(ctx, GW_ESTATE, "gw_summary_read: no summary recorded for that many walks");
+        if (hash) std::copy(P.h_hash, P.h_hash + n, hash);
+        if (visits) std::copy(P.h_visits, P.h_visits + n, visits);
+        return GW_OK;
+    }
     if (!ctx->d_hash || n > ctx->nwalks) return gw_fail(ctx, GW_ESTATE, "gw_summary_read: no summary recorded for that many walks");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -495,6 +519,13 @@ int gw_sources(uint64_t seed, uint32_t n, uint32_t count, uint32_t* out) {
 
 int gw_events_read(gw_ctx* ctx, uint64_t* out, uint64_t n_walks, uint32_t cap) {
     if (!ctx || !out) return gw_fail(ctx, GW_EINVAL, "gw_events_read: bad arguments");
+    if (ctx->pool) {
+        const gw_pool& P = *ctx->pool;
+        if (!P.events_valid || cap != P.h_events_cap || n_walks > P.n)
+            return gw_fail(ctx, GW_ESTATE, "gw_events_read: no event log of that shape");
+        std::copy(P.h_events, P.h_events + n_walks * cap, out);
+        return GW_OK;
+    }
     if (!ctx->d_events || cap != ctx->events_cap || n_walks > ctx->walk_cap)
         return gw_fail(ctx, GW_ESTATE, "gw_events_read: no event log of that shape");
     GW_HIP(ctx, hipSetDevice(ctx->device));

@@ -267,6 +267,7 @@ int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std
 extern "C" int gw_bucket(gw_ctx* ctx, uint64_t* bucket_counts) {
     if (!ctx) return GW_EINVAL;
     if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_bucket: no graph");
+    if (ctx->pool) return gw_fail(ctx, GW_ESTATE, "gw_bucket: the walks are in a host pool");
     if (ctx->nwalks >= 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_bucket: too many walks");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<uint64_t> off;

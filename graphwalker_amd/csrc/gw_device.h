@@ -54,6 +54,10 @@ struct Prog {
     uint64_t stop_thr;
     uint32_t k0, k1;   // Philox key = seed
     uint32_t stream;
+    // Per-walk outputs (final record, summary, event log) of walk id w are at
+    // index w - id_base: a window of a host walk pool writes window-sized
+    // arrays.  The random stream stays keyed by the walk id itself.
+    uint32_t id_base;
 };
 
 // Successor-header edge (16 bytes): the column entry plus the destination's

@@ -31,6 +31,32 @@ constexpr uint32_t kCntAggQueue = 20;         // partitioned: next source for th
 
 struct gw_run_graph;  // gw_step.hip: a captured run (gw_run_capture)
 
+// Host walk pool (gw_walks_load_host): the records stay in host memory, walk
+// id == position, and a run steps them window by window through two device
+// slots.  Slot 0 is the context's own walk arrays (sized to the window); slot
+// 1 is held here.  Per-walk outputs are copied back by walk id.
+struct gw_pool {
+    const gw_walk* h_walks = nullptr;
+    uint64_t n = 0;
+    uint64_t window = 0;
+    bool keep_final = false;
+    gw_walk* d_in1 = nullptr;
+    gw_walk* d_final1 = nullptr;
+    uint64_t* d_hash1 = nullptr;
+    uint32_t* d_visits1 = nullptr;
+    uint64_t* d_events1 = nullptr;
+    uint32_t events1_cap = 0;
+    gw_walk* h_final = nullptr;    // pinned, n records (keep_final)
+    uint64_t* h_hash = nullptr;    // pinned, n
+    uint32_t* h_visits = nullptr;  // pinned, n
+    uint64_t* h_events = nullptr;  // pinned, n x h_events_cap
+    uint32_t h_events_cap = 0;
+    bool final_valid = false, summary_valid = false, events_valid = false;
+    unsigned long long* h_cnt = nullptr;  // pinned, 2 x kCounters: each slot's counters
+    hipEvent_t in_ev[2] = {nullptr, nullptr};
+    hipEvent_t run_ev[2] = {nullptr, nullptr};
+};
+
 struct gw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -72,6 +98,8 @@ struct gw_ctx {
     uint64_t nwalks = 0;
     uint64_t walk_cap = 0;
     uint32_t walk_src_max = 0;  // largest source field among loaded walks
+    uint32_t walk_v_max = 0;    // largest start vertex among loaded walks (checked against the graph per run)
+    gw_pool* pool = nullptr;    // host walk pool (gw_walks_load_host), else null
     gw_walk* d_walks = nullptr;
     gw_walk* d_walks_alt = nullptr;
     gw_walk* d_final = nullptr;      // final record of each walk, indexed by walk id
@@ -162,6 +190,12 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 
 // gw_rmat.hip: reverse every edge of the device-resident CSR in place
 int gw_transpose_build(gw_ctx* ctx);
+
+// gw_pool.hip: host walk pool.  prepare allocates the slot-1 device buffers
+// and the pinned host outputs a run with recorder flags `rec` needs (the
+// event log of event_cap entries per walk), and the copy stream and events.
+void gw_pool_free(gw_ctx* ctx);
+int gw_pool_prepare(gw_ctx* ctx, uint32_t rec, uint32_t event_cap);
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);

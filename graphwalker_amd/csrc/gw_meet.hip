@@ -134,6 +134,7 @@ int gw_walks_init_pairs(gw_ctx* ctx, const uint32_t* a, const uint32_t* b, uint3
     gw_free(ctx, d_ab, npairs * 8ull);
     if (e != hipSuccess) return gw_hip_check(ctx, e, "init_pairs_kernel");
     ctx->walk_src_max = src_max;
+    ctx->walk_v_max = src_max;
     ctx->field_walks.clear();
     ctx->part_per_walk = 0;
     return GW_OK;
@@ -143,7 +144,7 @@ int gw_meeting_counts(gw_ctx* ctx, uint32_t npairs, uint32_t R, uint32_t L, uint
     if (!ctx || !counts || npairs == 0 || R == 0 || L < 2) return gw_fail(ctx, GW_EINVAL, "gw_meeting_counts: bad arguments");
     const uint32_t P = L - 1;
     if (P > kMaxPath) return gw_fail(ctx, GW_EINVAL, "gw_meeting_counts: L - 1 must be at most 31");
-    if (ctx->nwalks != 2ull * npairs * R)
+    if (ctx->pool || ctx->nwalks != 2ull * npairs * R)
         return gw_fail(ctx, GW_ESTATE, "gw_meeting_counts: the loaded walks are not npairs x 2R pair walks");
     if (!ctx->d_events || !ctx->d_visits || ctx->events_cap < P)
         return gw_fail(ctx, GW_ESTATE, "gw_meeting_counts: needs a run with GW_REC_EVENTS and event_cap >= L - 1");

@@ -56,13 +56,14 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
         counts, log, table, klog, s_slab[threadIdx.x >> 6]);
     if (i < nw) {
         gw_walk w = walks[i];
-        rec.id = w.id;
+        const uint32_t o = w.id - p.id_base;
+        rec.id = o;
         rec.source = w.source;
         advance<KIND>(p, g, w.id, w.source, w.vertex, w.hop, 0u, g.n, rec);
-        final_walks[w.id] = w;
+        final_walks[o] = w;
         if (SUM) {
-            hash[w.id] = rec.hash;
-            visits[w.id] = rec.visits;
+            hash[o] = rec.hash;
+            visits[o] = rec.visits;
         }
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], threadIdx.x & 63u);
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             if (!active && r < take) {
                 const gw_walk w = walks[chunk_next + r];
                 s.id = w.id;
-                rec.id = w.id;
+                rec.id = w.id - p.id_base;
                 rec.source = w.source;
                 s.source = w.source;
                 s.v = w.vertex;
@@ -137,10 +138,10 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
         if (__ballot(active) == 0) break;
         if (active) {
             if (step_once<KIND, EXT>(p, g, s, rec)) {
-                final_walks[s.id] = gw_walk{s.id, s.source, s.v, s.hop};
+                final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
                 if (SUM) {
-                    hash[s.id] = rec.hash;
-                    visits[s.id] = rec.visits;
+                    hash[s.id - p.id_base] = rec.hash;
+                    visits[s.id - p.id_base] = rec.visits;
                 }
                 tot_steps += rec.steps;
                 tot_visits += rec.visits;
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 slot = chunk_next + r;
                 const gw_walk w = walks[slot];
                 s.id = w.id;
-                rec.id = w.id;
+                rec.id = w.id - p.id_base;
                 rec.source = w.source;
                 s.source = w.source;
                 s.v = w.vertex;
@@ -255,8 +256,8 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 }
                 load_row<EXT>(gl, s);
                 if (SUM) {
-                    rec.hash = hash[s.id];
-                    rec.visits = visits[s.id];
+                    rec.hash = hash[s.id - p.id_base];
+                    rec.visits = visits[s.id - p.id_base];
                 } else {
                     rec.visits = 0;
                 }
@@ -276,11 +277,11 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 left = true;
             }
             if (done || left) {
-                if (done) final_walks[s.id] = gw_walk{s.id, s.source, s.v, s.hop};
+                if (done) final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
                 walks[slot] = gw_walk{s.id, s.source, done ? kDead : s.v, s.hop};
                 if (SUM) {
-                    hash[s.id] = rec.hash;
-                    visits[s.id] = rec.visits;
+                    hash[s.id - p.id_base] = rec.hash;
+                    visits[s.id - p.id_base] = rec.visits;
                 }
                 tot_steps += rec.steps;
                 tot_visits += rec.visits - visits0;
@@ -297,11 +298,11 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     }
 }
 
-__global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t nw, uint64_t* __restrict__ hash,
-                                    uint32_t* __restrict__ visits) {
+__global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t nw, uint32_t id_base,
+                                    uint64_t* __restrict__ hash, uint32_t* __restrict__ visits) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= nw) return;
-    const uint32_t id = walks[i].id;
+    const uint32_t id = walks[i].id - id_base;
     hash[id] = kHashInit;
     visits[id] = 0;
 }
@@ -428,7 +429,7 @@ int run_blocked(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, gw
     GW_HIP(ctx, hipMemcpyAsync(a, ctx->d_walks, ctx->nwalks * sizeof(gw_walk), hipMemcpyDeviceToDevice, ctx->stream));
     if (rec & GW_REC_WALK_SUMMARY) {
         summary_init_kernel<<<dim3(static_cast<uint32_t>((ctx->nwalks + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_walks, ctx->nwalks, ctx->d_hash, ctx->d_visits);
+            ctx->d_walks, ctx->nwalks, L.p.id_base, ctx->d_hash, ctx->d_visits);
         GW_HIP(ctx, hipGetLastError());
     }
     const uint32_t nb = ctx->nblocks;
@@ -488,7 +489,7 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
     GW_HIP(ctx, hipMemcpyAsync(a, ctx->d_walks, ctx->nwalks * sizeof(gw_walk), hipMemcpyDeviceToDevice, ctx->stream));
     if (rec & GW_REC_WALK_SUMMARY) {
         summary_init_kernel<<<dim3(static_cast<uint32_t>((ctx->nwalks + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-            ctx->d_walks, ctx->nwalks, ctx->d_hash, ctx->d_visits);
+            ctx->d_walks, ctx->nwalks, L.p.id_base, ctx->d_hash, ctx->d_visits);
         GW_HIP(ctx, hipGetLastError());
     }
     const uint32_t nb = ctx->nblocks;
@@ -670,12 +671,14 @@ int prepare_source_table(gw_ctx* ctx, const gw_program* prog) {
     if (vbits + sbits > 63)
         return gw_fail(ctx, GW_EINVAL, "gw_run: source field and vertex ids need more than 63 key bits");
     uint32_t lg = prog->table_log2;
-    if (lg == 0) lg = std::max(12u, std::min(31u, bits(std::max<uint64_t>(1, ctx->nwalks) * 4 - 1)));
+    // a host pool's windows add to one table: size it for the whole pool
+    const uint64_t nw_all = ctx->pool ? ctx->pool->n : ctx->nwalks;
+    if (lg == 0) lg = std::max(12u, std::min(31u, bits(std::max<uint64_t>(1, nw_all) * 4 - 1)));
     if (lg < 4 || lg > 34) return gw_fail(ctx, GW_EINVAL, "gw_run: table_log2 out of range [4, 34]");
     ctx->table_log2 = lg;
     ctx->table_vbits = vbits;
     ctx->table_cbits = 64 - vbits - sbits;
-    if (prog->source_method != GW_SRC_HASH && !ctx->host_graph) return GW_OK;  // key log, no hash table
+    if (prog->source_method != GW_SRC_HASH && !ctx->host_graph && !ctx->pool) return GW_OK;  // key log, no hash table
     // Keys of at most 44 bits pack with a 20-bit count into one word; longer
     // keys (a restart program's vertex-id source fields on a large graph)
     // take 16-byte entries {key + 1, count}, both words in one sector.
@@ -758,6 +761,9 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     if (sched != GW_SCHED_PLAIN && sched != GW_SCHED_PERSISTENT && sched != GW_SCHED_BLOCKED)
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown schedule");
     if (prog->layout > GW_LAYOUT_EXT) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown layout");
+    if (ctx->pool && (prog->recorder & GW_REC_SOURCE_COUNT) &&
+        (prog->source_method == GW_SRC_GLOBAL || prog->source_method == GW_SRC_PARTITIONED))
+        return gw_fail(ctx, GW_EINVAL, "gw_run: a host walk pool counts sources with GW_SRC_HASH");
     if (ctx->host_graph) {
         if (sched != GW_SCHED_BLOCKED)
             return gw_fail(ctx, GW_EINVAL, "gw_run: a host-resident graph runs the blocked schedule");
@@ -792,7 +798,7 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     if (rec & GW_REC_SOURCE_COUNT) {
         int rc = prepare_source_table(ctx, prog);
         if (rc) return rc;
-        if (prog->source_method != GW_SRC_HASH && !ctx->host_graph) {
+        if (prog->source_method != GW_SRC_HASH && !ctx->host_graph && !ctx->pool) {
             if (prog->source_method == GW_SRC_PARTITIONED && ctx->field_walks.empty())
                 return gw_fail(ctx, GW_EINVAL, "gw_run: GW_SRC_PARTITIONED needs source fields below 2^20");
             const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -846,16 +852,17 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
 
 // The work of one run on the engine stream, for the persistent and plain
 // schedules: counter reset, recorder clears (fill kernels, so a captured
-// graph holds no memset nodes), the step kernel, and the partitioned
+// graph holds no memset nodes; skipped for the later windows of a host walk
+// pool, which add to the first's), the step kernel, and the partitioned
 // aggregation.  timed: HIP events around the step kernel and the aggregation.
-int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed) {
+int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed, bool clear = true) {
     int rc = gw_partlog_reset(ctx, rp.part);
     if (rc) return rc;
-    if (rp.rec & GW_REC_VERTEX_COUNT) {
+    if (clear && (rp.rec & GW_REC_VERTEX_COUNT)) {
         fill_u32_kernel<<<dim3(gw_grid(ctx->n, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_counts, ctx->n);
         GW_HIP(ctx, hipGetLastError());
     }
-    if (rp.rec & GW_REC_SOURCE_COUNT) {  // hash table
+    if (clear && (rp.rec & GW_REC_SOURCE_COUNT)) {  // hash table
         const uint64_t words = ctx->table_bytes / 8;
         fill_u64_kernel<<<dim3(gw_grid(words, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_table, words);
         GW_HIP(ctx, hipGetLastError());
@@ -922,11 +929,155 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count};
 }
 
+// Host walk pool: the run steps the pool window by window through two device
+// slots.  While the engine stream steps window k in slot k & 1, the copy
+// stream (after window k - 1 finished in the other slot) copies that window's
+// per-walk outputs out and window k + 1's records in.  Per-vertex counts and
+// the source-count hash add up over the windows (cleared once); the device
+// counters are reset per window and summed on the host one window behind, so
+// the host never holds the engine stream between windows.
+int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
+    gw_pool& P = *ctx->pool;
+    const uint64_t W = P.window, n = P.n;
+    const uint64_t nwin = (n + W - 1) / W;
+    P.final_valid = P.summary_valid = P.events_valid = false;
+    ctx->nwalks = W;
+    RunPlan rp;
+    int rc = plan_run(ctx, prog, rp);
+    if (rc) return rc;
+    const uint32_t rec = rp.rec;
+    const bool sum = (rec & GW_REC_WALK_SUMMARY) != 0;
+    const uint64_t ecap = (rec & GW_REC_EVENTS) ? prog->event_cap : 0u;
+    if ((rc = gw_pool_prepare(ctx, rec, static_cast<uint32_t>(ecap)))) return rc;
+    struct Slot {
+        gw_walk* in;
+        gw_walk* fin;
+        uint64_t* hash;
+        uint32_t* vis;
+        uint64_t* ev;
+    };
+    const Slot slot[2] = {{ctx->d_walks, ctx->d_final, ctx->d_hash, ctx->d_visits, ctx->d_events},
+                          {P.d_in1, P.d_final1, P.d_hash1, P.d_visits1, P.d_events1}};
+    struct Restore {  // the context's own arrays again on every exit
+        gw_ctx* c;
+        Slot s;
+        uint64_t nw;
+        ~Restore() {
+            c->d_walks = s.in;
+            c->d_final = s.fin;
+            c->d_hash = s.hash;
+            c->d_visits = s.vis;
+            c->d_events = s.ev;
+            c->nwalks = nw;
+        }
+    } restore{ctx, slot[0], W};
+    hipStream_t cs = ctx->copy_stream;
+    gw_stats st = rp.st0;
+    uint64_t bytes = 0;
+    auto count_of = [&](uint64_t k) { return std::min(W, n - k * W); };
+    auto copy_in = [&](uint64_t k) -> int {
+        const uint64_t c = count_of(k);
+        GW_HIP(ctx, hipMemcpyAsync(slot[k & 1].in, P.h_walks + k * W, c * sizeof(gw_walk), hipMemcpyHostToDevice, cs));
+        GW_HIP(ctx, hipEventRecord(P.in_ev[k & 1], cs));
+        bytes += c * sizeof(gw_walk);
+        return GW_OK;
+    };
+    auto absorb = [&](uint64_t k) -> int {
+        GW_HIP(ctx, hipEventSynchronize(P.run_ev[k & 1]));
+        const unsigned long long* cnt = P.h_cnt + (k & 1) * kCounters;
+        if ((rec & GW_REC_SOURCE_COUNT) && cnt[kCntTableOverflow])
+            return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[kCntTableOverflow]) +
+                                               " visits not recorded); raise table_log2");
+        st.steps += cnt[kCntSteps];
+        st.visits += cnt[kCntVisits];
+        return GW_OK;
+    };
+    const bool blocked = rp.sched == GW_SCHED_BLOCKED;
+    if (blocked) {  // the recorders are cleared once for all windows
+        if (rec & GW_REC_VERTEX_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_counts, 0, ctx->n * 4ull, ctx->stream));
+        if (rec & GW_REC_SOURCE_COUNT) GW_HIP(ctx, hipMemsetAsync(ctx->d_table, 0, ctx->table_bytes, ctx->stream));
+    }
+    GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    GW_HIP(ctx, hipStreamWaitEvent(cs, ctx->ev[4], 0));  // slot 0's arrays are free from here on
+    if ((rc = copy_in(0))) return rc;
+    if (nwin > 1 && (rc = copy_in(1))) return rc;
+    st.rounds = st.launches = 0;
+    for (uint64_t k = 0; k < nwin; ++k) {
+        const uint32_t s = static_cast<uint32_t>(k & 1);
+        const uint64_t c = count_of(k), base = k * W;
+        GW_HIP(ctx, hipStreamWaitEvent(ctx->stream, P.in_ev[s], 0));
+        ctx->d_walks = slot[s].in;
+        ctx->d_final = slot[s].fin;
+        ctx->d_hash = slot[s].hash;
+        ctx->d_visits = slot[s].vis;
+        ctx->d_events = slot[s].ev;
+        ctx->nwalks = c;
+        rp.L.p.id_base = static_cast<uint32_t>(base);
+        rp.L.log.ev = slot[s].ev;
+        if (blocked) {
+            if ((rc = gw_partlog_reset(ctx, false))) return rc;
+            gw_stats w = rp.st0;
+            rc = ctx->host_graph ? run_streamed(ctx, rp.L, prog, rec, w) : run_blocked(ctx, rp.L, prog, rec, w);
+            if (rc) return rc;
+            st.rounds += w.rounds;
+            st.launches += w.launches;
+            st.step_ms += w.step_ms;
+            st.bucket_ms += w.bucket_ms;
+            st.h2d_bytes += w.h2d_bytes;
+            st.blocks_loaded += w.blocks_loaded;
+            st.copy_ms += w.copy_ms;
+            st.cache_slots = w.cache_slots;
+        } else {
+            if ((rc = enqueue_run(ctx, rp, false, k == 0))) return rc;
+            st.launches += 1;
+            st.rounds += 1;
+        }
+        GW_HIP(ctx, hipMemcpyAsync(P.h_cnt + s * kCounters, ctx->d_counters, kCounters * 8, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        GW_HIP(ctx, hipEventRecord(P.run_ev[s], ctx->stream));
+        // copy stream: window k's per-walk outputs out, then window k + 2 into this slot
+        GW_HIP(ctx, hipStreamWaitEvent(cs, P.run_ev[s], 0));
+        if (P.keep_final) {
+            GW_HIP(ctx, hipMemcpyAsync(P.h_final + base, slot[s].fin, c * sizeof(gw_walk), hipMemcpyDeviceToHost, cs));
+            bytes += c * sizeof(gw_walk);
+        }
+        if (sum) {
+            GW_HIP(ctx, hipMemcpyAsync(P.h_hash + base, slot[s].hash, c * 8, hipMemcpyDeviceToHost, cs));
+            GW_HIP(ctx, hipMemcpyAsync(P.h_visits + base, slot[s].vis, c * 4, hipMemcpyDeviceToHost, cs));
+            bytes += c * 12;
+        }
+        if (ecap) {
+            GW_HIP(ctx, hipMemcpyAsync(P.h_events + base * ecap, slot[s].ev, c * ecap * 8, hipMemcpyDeviceToHost, cs));
+            bytes += c * ecap * 8;
+        }
+        if (k + 2 < nwin && (rc = copy_in(k + 2))) return rc;
+        if (k >= 1 && (rc = absorb(k - 1))) return rc;
+    }
+    if ((rc = absorb(nwin - 1))) return rc;
+    GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    GW_HIP(ctx, hipStreamSynchronize(cs));
+    GW_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
+    float ms = 0.f;
+    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    st.total_ms = ms;
+    if (!blocked) st.step_ms = ms;
+    st.pool_windows = nwin;
+    st.pool_bytes = bytes;
+    st.source_method = (rec & GW_REC_SOURCE_COUNT) ? GW_SRC_HASH : 0u;
+    ctx->src_method = GW_SRC_HASH;
+    P.final_valid = P.keep_final;
+    P.summary_valid = sum;
+    P.events_valid = ecap != 0;
+    if (stats) *stats = st;
+    return GW_OK;
+}
+
 }  // namespace
 }  // namespace gw
 
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     using namespace gw;
+    if (ctx->pool) return run_pool(ctx, prog, stats);
     RunPlan rp;
     int rc = plan_run(ctx, prog, rp);
     if (rc) return rc;
@@ -972,6 +1123,7 @@ int gw_run_capture(gw_ctx* ctx, const gw_program* prog) {
     if (!ctx || !prog) return gw_fail(ctx, GW_EINVAL, "gw_run_capture: bad arguments");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     gw_run_graph_free(ctx);
+    if (ctx->pool) return gw_fail(ctx, GW_EINVAL, "gw_run_capture: a host walk pool runs window by window");
     // one ordinary run first: allocations settle and the run's own checks pass
     gw_stats st0;
     int rc = gw_run(ctx, prog, &st0);

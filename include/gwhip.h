@@ -29,6 +29,9 @@
  *                              builds the same CSR layout on the device)
  *   gw_walks_load              the initial buckets written by startWalksbyApp
  *                              through WalkManager::moveWalk (walk.hpp:93-101)
+ *   gw_walks_load_host         the walk pool files (writeWalks2Disk /
+ *                              readWalksfromDisk, walk.hpp:103-157): a pool kept
+ *                              in host memory, stepped window by window
  *   gw_walks_init_*            device-side walk placement for the app patterns
  *                              of rawrandomwalks.cpp:27-50 / rwdomination.cpp:264-284
  *                              (one walk per vertex) and msppr.cpp:151-179
@@ -166,6 +169,8 @@ typedef struct gw_stats {
     uint32_t blocks_loaded;    /* host-resident graph: block loads */
     uint32_t cache_slots;      /* host-resident graph: device block-cache slots */
     double copy_ms;            /* host-resident graph: block-copy time on the copy stream */
+    uint64_t pool_windows;     /* host walk pool: windows stepped */
+    uint64_t pool_bytes;       /* host walk pool: walk records and per-walk outputs copied */
 } gw_stats;
 
 int gw_create(gw_ctx **out, const gw_cfg *cfg);
@@ -211,6 +216,17 @@ int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
 int gw_walks_init_pairs(gw_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t npairs, uint32_t R, uint32_t L);
 int gw_meeting_counts(gw_ctx *ctx, uint32_t npairs, uint32_t R, uint32_t L, uint32_t *counts);
 /* count walks per vertex: walk i = vertex i / per_vertex, source = vertex, id = i */
+/* Host walk pool: `walks` (n records, walks[i].id == i) stays in host memory
+ * -- gw_host_alloc memory is copied from directly -- and must outlive the
+ * pool.  gw_run steps it `window` walks at a time through two device windows,
+ * copying the next window in while one steps; per-vertex counts and source
+ * counts (GW_SRC_HASH) accumulate over the windows, per-walk outputs (summary,
+ * event log, and final records with GW_POOL_KEEP_FINAL) are copied back into
+ * pinned host arrays read by gw_walks_read / gw_summary_read /
+ * gw_events_read.  Loading device walks (gw_walks_load, gw_walks_init_*)
+ * drops the pool. */
+#define GW_POOL_KEEP_FINAL 1u
+int gw_walks_load_host(gw_ctx *ctx, const gw_walk *walks, uint64_t n, uint64_t window, uint32_t flags);
 int gw_walks_init_per_vertex(gw_ctx *ctx, uint32_t per_vertex);
 /* walks_per_source walks per source: id = k * walks_per_source + j, source =
  * source_field[k] (or sources[k] when source_field is NULL), vertex = sources[k] */
