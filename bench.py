@@ -116,6 +116,27 @@ def replayed(eng: gw.Engine, prog: gw.Program, K: int) -> dict:
     return {"walk_steps_per_s": steps / wall, "graph_ms": float(np.mean(gms)), "wall_s": wall}
 
 
+def pool_measure(eng: gw.Engine, wl: dict, n: int, prog: gw.Program, window: int, K: int,
+                 keep_final: bool) -> dict:
+    """The headline walks as a host walk pool (pinned memory, one record per
+    vertex, id == position) stepped `window` walks at a time."""
+    walks = eng.host_empty((n,), gw.WALK_DTYPE)
+    v = np.arange(n, dtype=np.uint32)
+    walks["id"], walks["source"], walks["vertex"], walks["hop"] = v, v, v, 0
+    eng.walks_load_host(walks, window, keep_final=keep_final)
+    eng.run(prog)
+    t = time.perf_counter()
+    steps, st = 0, None
+    for _ in range(K):
+        st = eng.run(prog)
+        steps += st["steps"]
+    wall = time.perf_counter() - t
+    eng.walks_init_per_vertex(1)  # device walks again
+    return {"walk_steps_per_s": steps / wall, "window": window, "windows": st["pool_windows"],
+            "pcie_bytes_per_pass": st["pool_bytes"], "pass_ms": wall / K * 1e3, "keep_final": keep_final,
+            "pcie_GBps": st["pool_bytes"] / (wall / K) / 1e9}
+
+
 def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) -> dict:
     import oracle as O
 
@@ -364,6 +385,8 @@ def main() -> int:
     ap.add_argument("--stream-cache-mib", type=int, default=1024,
                     help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
     ap.add_argument("--stream-queries", type=int, default=256)
+    ap.add_argument("--pool-window", type=int, default=1 << 21,
+                    help="walks per window of the host-walk-pool figure (0 = skip)")
     ap.add_argument("--simrank-scale", type=int, default=28, help="config-4 SimRank line on R-MAT of this scale (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -408,6 +431,16 @@ def main() -> int:
         eng.counts_topk(k)
         post[f"counts_topk_{k}_ms"] = (time.perf_counter() - t) * 1e3
     print(f"[bench] counts top-k: {post}", file=sys.stderr, flush=True)
+
+    # the same walks as a host walk pool (out-of-HBM walk pools), stepped in windows
+    pool = {}
+    if args.pool_window:
+        for keep in (False, True):
+            r = pool_measure(eng, wl, m["n"], m["prog"], args.pool_window, max(2, args.steps // 3), keep)
+            pool["with_finals" if keep else "counts_only"] = r
+            print(f"[bench] host walk pool (window {args.pool_window}, finals {keep}): "
+                  f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s, {r['pcie_GBps']:.1f} GB/s over PCIe",
+                  file=sys.stderr, flush=True)
 
     workloads = {}
     if args.secondary and args.secondary != args.workload:
@@ -484,6 +517,7 @@ def main() -> int:
         "out_of_core": stream,
         "workloads_simrank": simrank,
         "post_processing": post,
+        "host_walk_pool": pool,
     }
     if "speedup_vs_plain" in m:
         out["speedup_vs_plain"] = m["speedup_vs_plain"]

@@ -53,7 +53,9 @@ struct gw_pool {
     uint32_t h_events_cap = 0;
     bool final_valid = false, summary_valid = false, events_valid = false;
     unsigned long long* h_cnt = nullptr;  // pinned, 2 x kCounters: each slot's counters
+    hipStream_t out_stream = nullptr;     // device-to-host copies (the copy stream takes host-to-device)
     hipEvent_t in_ev[2] = {nullptr, nullptr};
+    hipEvent_t out_ev[2] = {nullptr, nullptr};
     hipEvent_t run_ev[2] = {nullptr, nullptr};
 };
 

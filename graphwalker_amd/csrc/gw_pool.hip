@@ -7,9 +7,9 @@
 // takes 16 B of records plus up to 60 B of working copies and per-walk outputs
 // in HBM, so pools beyond what the HBM budget leaves next to the graph stay in
 // (pinned) host memory here: gw_walks_load_host registers the caller's
-// records, and a run steps them window by window (gw_step.hip), copying each
-// window in on the copy stream while the previous one steps and copying the
-// per-walk outputs back by walk id.  Walk ids equal positions in the pool, so
+// records, and a run steps them window by window (gw_step.hip), copying the
+// next window in on the copy stream and the previous window's per-walk outputs
+// back (by walk id) on a stream of their own while one window steps.  Walk ids equal positions in the pool, so
 // a window is one contiguous id range and trajectories equal those of the
 // same records loaded into HBM.
 #include <hip/hip_runtime.h>
@@ -40,6 +40,7 @@ void gw_pool_free(gw_ctx* ctx) {
     if (!P) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    if (P->out_stream) (void)hipStreamSynchronize(P->out_stream);
     const uint64_t W = P->window;
     gw_free(ctx, P->d_in1, W * sizeof(gw_walk));
     gw_free(ctx, P->d_final1, W * sizeof(gw_walk));
@@ -51,10 +52,10 @@ void gw_pool_free(gw_ctx* ctx) {
     host_free(P->h_visits);
     host_free(P->h_events);
     host_free(P->h_cnt);
-    for (auto& e : P->in_ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto& e : P->run_ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto* set : {&P->in_ev, &P->out_ev, &P->run_ev})
+        for (auto& e : *set)
+            if (e) (void)hipEventDestroy(e);
+    if (P->out_stream) (void)hipStreamDestroy(P->out_stream);
     delete P;
     ctx->pool = nullptr;
 }
@@ -64,7 +65,8 @@ int gw_pool_prepare(gw_ctx* ctx, uint32_t rec, uint32_t event_cap) {
     const uint64_t W = P.window;
     int rc;
     if (!ctx->copy_stream) GW_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
-    for (auto* set : {&P.in_ev, &P.run_ev})
+    if (!P.out_stream) GW_HIP(ctx, hipStreamCreateWithFlags(&P.out_stream, hipStreamNonBlocking));
+    for (auto* set : {&P.in_ev, &P.out_ev, &P.run_ev})
         for (auto& e : *set)
             if (!e) GW_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (!P.h_cnt && (rc = host_alloc(ctx, reinterpret_cast<void**>(&P.h_cnt), 2 * kCounters * 8, "pool counters")))

@@ -931,8 +931,9 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
 
 // Host walk pool: the run steps the pool window by window through two device
 // slots.  While the engine stream steps window k in slot k & 1, the copy
-// stream (after window k - 1 finished in the other slot) copies that window's
-// per-walk outputs out and window k + 1's records in.  Per-vertex counts and
+// stream copies window k + 1's records into the other slot and the pool's
+// output stream copies window k - 1's per-walk outputs out of it (the two
+// directions of PCIe at once).  Window k + 2 waits for both.  Per-vertex counts and
 // the source-count hash add up over the windows (cleared once); the device
 // counters are reset per window and summed on the host one window behind, so
 // the host never holds the engine stream between windows.
@@ -971,7 +972,7 @@ int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
             c->nwalks = nw;
         }
     } restore{ctx, slot[0], W};
-    hipStream_t cs = ctx->copy_stream;
+    hipStream_t cs = ctx->copy_stream, os = P.out_stream;
     gw_stats st = rp.st0;
     uint64_t bytes = 0;
     auto count_of = [&](uint64_t k) { return std::min(W, n - k * W); };
@@ -999,6 +1000,7 @@ int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     }
     GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     GW_HIP(ctx, hipStreamWaitEvent(cs, ctx->ev[4], 0));  // slot 0's arrays are free from here on
+    const bool outputs = P.keep_final || sum || ecap;
     if ((rc = copy_in(0))) return rc;
     if (nwin > 1 && (rc = copy_in(1))) return rc;
     st.rounds = st.launches = 0;
@@ -1006,6 +1008,7 @@ int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         const uint32_t s = static_cast<uint32_t>(k & 1);
         const uint64_t c = count_of(k), base = k * W;
         GW_HIP(ctx, hipStreamWaitEvent(ctx->stream, P.in_ev[s], 0));
+        if (k >= 2 && outputs) GW_HIP(ctx, hipStreamWaitEvent(ctx->stream, P.out_ev[s], 0));
         ctx->d_walks = slot[s].in;
         ctx->d_final = slot[s].fin;
         ctx->d_hash = slot[s].hash;
@@ -1035,25 +1038,34 @@ int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         GW_HIP(ctx, hipMemcpyAsync(P.h_cnt + s * kCounters, ctx->d_counters, kCounters * 8, hipMemcpyDeviceToHost,
                                    ctx->stream));
         GW_HIP(ctx, hipEventRecord(P.run_ev[s], ctx->stream));
-        // copy stream: window k's per-walk outputs out, then window k + 2 into this slot
-        GW_HIP(ctx, hipStreamWaitEvent(cs, P.run_ev[s], 0));
-        if (P.keep_final) {
-            GW_HIP(ctx, hipMemcpyAsync(P.h_final + base, slot[s].fin, c * sizeof(gw_walk), hipMemcpyDeviceToHost, cs));
-            bytes += c * sizeof(gw_walk);
+        // window k's per-walk outputs out on the output stream; window k + 2
+        // into this slot on the copy stream
+        if (outputs) {
+            GW_HIP(ctx, hipStreamWaitEvent(os, P.run_ev[s], 0));
+            if (P.keep_final) {
+                GW_HIP(ctx, hipMemcpyAsync(P.h_final + base, slot[s].fin, c * sizeof(gw_walk), hipMemcpyDeviceToHost, os));
+                bytes += c * sizeof(gw_walk);
+            }
+            if (sum) {
+                GW_HIP(ctx, hipMemcpyAsync(P.h_hash + base, slot[s].hash, c * 8, hipMemcpyDeviceToHost, os));
+                GW_HIP(ctx, hipMemcpyAsync(P.h_visits + base, slot[s].vis, c * 4, hipMemcpyDeviceToHost, os));
+                bytes += c * 12;
+            }
+            if (ecap) {
+                GW_HIP(ctx, hipMemcpyAsync(P.h_events + base * ecap, slot[s].ev, c * ecap * 8, hipMemcpyDeviceToHost, os));
+                bytes += c * ecap * 8;
+            }
+            GW_HIP(ctx, hipEventRecord(P.out_ev[s], os));
         }
-        if (sum) {
-            GW_HIP(ctx, hipMemcpyAsync(P.h_hash + base, slot[s].hash, c * 8, hipMemcpyDeviceToHost, cs));
-            GW_HIP(ctx, hipMemcpyAsync(P.h_visits + base, slot[s].vis, c * 4, hipMemcpyDeviceToHost, cs));
-            bytes += c * 12;
+        if (k + 2 < nwin) {
+            GW_HIP(ctx, hipStreamWaitEvent(cs, P.run_ev[s], 0));
+            if ((rc = copy_in(k + 2))) return rc;
         }
-        if (ecap) {
-            GW_HIP(ctx, hipMemcpyAsync(P.h_events + base * ecap, slot[s].ev, c * ecap * 8, hipMemcpyDeviceToHost, cs));
-            bytes += c * ecap * 8;
-        }
-        if (k + 2 < nwin && (rc = copy_in(k + 2))) return rc;
         if (k >= 1 && (rc = absorb(k - 1))) return rc;
     }
     if ((rc = absorb(nwin - 1))) return rc;
+    // the run ends when the last outputs are on the host
+    if (outputs) GW_HIP(ctx, hipStreamWaitEvent(ctx->stream, P.out_ev[(nwin - 1) & 1], 0));
     GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(cs));
     GW_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
