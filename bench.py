@@ -304,6 +304,16 @@ def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, quer
     graph_bytes = hoff.nbytes + hcol.nbytes
     eng.graph_set_host(hoff, hcol, cache_bytes)
     rate, st = batch("blocked", "csr")
+    # the other load-pass block choices (results are the same; the loads differ)
+    policies = {}
+    for name, pol, prob in (("min_step_0.2", gw.BLOCKS_MIN_STEP, 0.2), ("max_weight", gw.BLOCKS_MAX_WEIGHT, 0.0)):
+        eng.stream_policy(pol, prob, seed=seed)
+        r_, st_ = batch("blocked", "csr")
+        policies[name] = {"walk_steps_per_s": r_, "blocks_loaded": st_["blocks_loaded"], "rounds": st_["rounds"],
+                          "h2d_bytes_per_pass": st_["h2d_bytes"], "same_steps": st_["steps"] == st["steps"]}
+        print(f"[bench] stream policy {name}: {r_ / 1e9:.3f} G steps/s, {st_['blocks_loaded']} loads",
+              file=sys.stderr, flush=True)
+    eng.stream_policy(gw.BLOCKS_MAX_WALKS)
     h2d_ms = eng.h2d_probe(1 << 30)
     out = {
         "desc": f"R-MAT scale {scale} graph ({graph_bytes / 2**30:.2f} GiB CSR) in pinned host memory, device block "
@@ -317,7 +327,7 @@ def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, quer
         "copy_ms": st["copy_ms"], "step_ms": st["step_ms"], "bucket_ms": st["bucket_ms"], "total_ms": st["total_ms"],
         "h2d_GBps_achieved": st["h2d_bytes"] / max(1e-9, st["copy_ms"] * 1e-3) / 1e9,
         "h2d_GBps_probe": (1 << 30) / (h2d_ms * 1e-3) / 1e9,
-        "graph_bytes": graph_bytes, "cache_bytes": cache_bytes,
+        "graph_bytes": graph_bytes, "cache_bytes": cache_bytes, "policy": "max_walks", "other_policies": policies,
     }
     print(f"[bench] stream: {rate / 1e9:.3f} G steps/s ({hbm_rate / 1e9:.2f} in HBM), {st['rounds']} rounds, "
           f"{st['blocks_loaded']} loads, {st['h2d_bytes'] / 2**30:.1f} GiB copied at "

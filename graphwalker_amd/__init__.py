@@ -33,6 +33,7 @@ LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
 # per-source count aggregation
 SRC_SORT, SRC_HASH, SRC_GLOBAL, SRC_PARTITIONED = 0, 1, 2, 3
 POOL_KEEP_FINAL = 1
+BLOCKS_MAX_WALKS, BLOCKS_MIN_STEP, BLOCKS_MAX_WEIGHT = 0, 1, 2
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
 
@@ -175,6 +176,7 @@ def lib() -> C.CDLL:
         "gw_graph_read": ([P, P, P], i),
         "gw_walks_load": ([P, P, u64], i),
         "gw_walks_load_host": ([P, P, u64, u64, u32], i),
+        "gw_stream_policy": ([P, u32, C.c_double, u64], i),
         "gw_walks_init_per_vertex": ([P, u32], i),
         "gw_walks_init_sources": ([P, P, P, u32, u32], i),
         "gw_walks_read": ([P, P, u64], i),
@@ -324,6 +326,11 @@ class Engine:
         walks = np.ascontiguousarray(walks, dtype=WALK_DTYPE)
         self._check(self._L.gw_walks_load(self._ctx, _ptr(walks), walks.shape[0]), "gw_walks_load")
         self.nwalks = walks.shape[0]
+
+    def stream_policy(self, policy: int, prob: float = 0.2, seed: int = 0) -> None:
+        """Block choice of out-of-core load passes: BLOCKS_MAX_WALKS (default),
+        BLOCKS_MIN_STEP (the reference's chooseBlock(prob)) or BLOCKS_MAX_WEIGHT."""
+        self._check(self._L.gw_stream_policy(self._ctx, policy, prob, seed), "gw_stream_policy")
 
     def walks_load_host(self, walks: np.ndarray, window: int, keep_final: bool = True) -> None:
         """Host walk pool: the records (walks[i].id == i) stay in host memory

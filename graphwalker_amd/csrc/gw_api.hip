@@ -232,6 +232,7 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
     gw_free(ctx, ctx->d_bucket_off, 0);
+    gw_free(ctx, ctx->d_block_minhop, ctx->block_minhop_cap * 4ull);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->copy_ev)

@@ -69,28 +69,38 @@ __device__ __forceinline__ void wave_group(uint32_t key, bool valid, uint32_t& r
     }
 }
 
+// minhop (optional): per-block minimum hop of the records, the minstep table
+// of the reference's block choice (walk.hpp:187-202), kept with atomicMin.
 __global__ __launch_bounds__(kThreads) void hist_kernel(const gw_walk* __restrict__ in, uint64_t n,
                                                         const uint32_t* __restrict__ block_first, uint32_t nb,
-                                                        uint32_t ntiles, uint32_t* __restrict__ hist) {
+                                                        uint32_t ntiles, uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ minhop) {
     __shared__ uint32_t s_bf[kMaxBlocks + 1];
     __shared__ uint32_t s_cnt[kMaxBlocks + 1];
+    __shared__ uint32_t s_min[kMaxBlocks + 1];
     for (uint32_t i = threadIdx.x; i <= nb; i += kThreads) {
         s_bf[i] = block_first[i];
         s_cnt[i] = 0;
+        s_min[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
     for (uint32_t it = 0; it < kPerThread; ++it) {
         const uint64_t i = base + it * kThreads + threadIdx.x;
         const bool valid = i < n;
-        const uint32_t b = valid ? find_block(s_bf, nb, in[i].vertex) : 0u;
+        const gw_walk w = valid ? in[i] : gw_walk{0, 0, 0, 0};
+        const uint32_t b = valid ? find_block(s_bf, nb, w.vertex) : 0u;
         uint32_t rank, count;
         bool first;
         wave_group(b, valid, rank, count, first);
         if (valid && first) atomicAdd(&s_cnt[b], count);
+        if (minhop && valid && w.hop < s_min[b]) atomicMin(&s_min[b], w.hop);
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b <= nb; b += kThreads) hist[static_cast<uint64_t>(b) * ntiles + blockIdx.x] = s_cnt[b];
+    for (uint32_t b = threadIdx.x; b <= nb; b += kThreads) {
+        hist[static_cast<uint64_t>(b) * ntiles + blockIdx.x] = s_cnt[b];
+        if (minhop && s_min[b] != 0xFFFFFFFFu) atomicMin(&minhop[b], s_min[b]);
+    }
 }
 
 // Exclusive scan, phase 1: each workgroup scans its chunk of kScanThreads*4
@@ -221,7 +231,8 @@ __global__ void gather_starts_kernel(const uint32_t* __restrict__ offs, uint32_t
 }  // namespace
 }  // namespace gw
 
-int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std::vector<uint64_t>& bucket_off) {
+int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std::vector<uint64_t>& bucket_off,
+                   std::vector<uint32_t>* minhop) {
     using namespace gw;
     const uint32_t nb = ctx->nblocks;
     if (nb == 0 || nb > kMaxBlocks) return gw_fail(ctx, GW_EINVAL, "bucketing supports 1..2048 blocks");
@@ -240,7 +251,20 @@ int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std
     }
     uint32_t* hist = ctx->d_tile_hist;
     uint32_t* sums = hist + entries;
-    hist_kernel<<<dim3(ntiles), dim3(kThreads), 0, ctx->stream>>>(in, n, ctx->d_block_first, nb, ntiles, hist);
+    uint32_t* d_min = nullptr;
+    if (minhop) {
+        if (ctx->block_minhop_cap < nb + 1) {
+            gw_free(ctx, ctx->d_block_minhop, ctx->block_minhop_cap * 4ull);
+            ctx->d_block_minhop = nullptr;
+            ctx->block_minhop_cap = 0;
+            int rc = gw_alloc(ctx, (void**)&ctx->d_block_minhop, (kMaxBlocks + 1) * 4ull, "block min hop");
+            if (rc) return rc;
+            ctx->block_minhop_cap = kMaxBlocks + 1;
+        }
+        d_min = ctx->d_block_minhop;
+        GW_HIP(ctx, hipMemsetAsync(d_min, 0xFF, (nb + 1) * 4ull, ctx->stream));
+    }
+    hist_kernel<<<dim3(ntiles), dim3(kThreads), 0, ctx->stream>>>(in, n, ctx->d_block_first, nb, ntiles, hist, d_min);
     GW_HIP(ctx, hipGetLastError());
     // block-major sizes: read back the per-bucket totals before the scan
     // overwrites the histogram, by scanning and reading bucket starts.
@@ -259,6 +283,10 @@ int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out, std
     GW_HIP(ctx, hipGetLastError());
     std::vector<uint32_t> starts(nb + 1);
     GW_HIP(ctx, hipMemcpyAsync(starts.data(), d_starts, (nb + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    if (minhop) {
+        minhop->resize(nb + 1);
+        GW_HIP(ctx, hipMemcpyAsync(minhop->data(), d_min, (nb + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    }
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     bucket_off.assign(starts.begin(), starts.end());
     return GW_OK;

@@ -95,6 +95,12 @@ struct gw_ctx {
     void* d_views = nullptr;
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ev[3] = {nullptr, nullptr, nullptr};
+    // which blocks a load pass brings in (gw_stream_policy)
+    uint32_t block_policy = 0;          // GW_BLOCKS_*
+    double block_prob = 0.2;            // GW_BLOCKS_MIN_STEP: chance of the min-step choice
+    uint64_t block_seed = 0;            // its coin stream
+    uint32_t* d_block_minhop = nullptr; // per-block minimum hop from the bucketing pass
+    uint32_t block_minhop_cap = 0;
 
     // walks
     uint64_t nwalks = 0;
@@ -230,5 +236,6 @@ bool gw_is_pinned(const gw_ctx* ctx, const void* p, uint64_t bytes);
 
 // gw_bucket.hip: stable counting sort of `in` (n records) by block into `out`;
 // bucket_off[b] = first index of bucket b (nblocks + 1 entries written, host).
+// minhop (optional): per-block minimum hop of the records (0xFFFFFFFF: none).
 int gw_bucket_sort(gw_ctx* ctx, const gw_walk* in, uint64_t n, gw_walk* out,
-                   std::vector<uint64_t>& bucket_off);
+                   std::vector<uint64_t>& bucket_off, std::vector<uint32_t>* minhop = nullptr);

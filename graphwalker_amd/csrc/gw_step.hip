@@ -500,6 +500,15 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
     std::vector<uint64_t> off;
     std::vector<uint32_t> order, chosen, loaded;
     std::vector<char> in_round(nb, 0);
+    const uint32_t policy = ctx->block_policy;
+    std::vector<uint32_t> minhop;
+    uint64_t coin_state = ctx->block_seed ^ 0x9E3779B97F4A7C15ull;
+    auto coin = [&]() {  // splitmix64 -> uniform [0, 1)
+        uint64_t z = (coin_state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return static_cast<double>((z ^ (z >> 31)) >> 11) * 0x1.0p-53;
+    };
     float ms = 0.f;
     double step_ms = 0, bucket_ms = 0, copy_ms = 0;
     // Steps the buckets of `blocks` (sorted), one launch per run of adjacent blocks.
@@ -521,7 +530,7 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
     };
     while (live > 0) {
         GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        if ((rc = gw_bucket_sort(ctx, a, live, b, off))) return rc;
+        if ((rc = gw_bucket_sort(ctx, a, live, b, off, policy != GW_BLOCKS_MAX_WALKS ? &minhop : nullptr))) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         GW_HIP(ctx, hipEventSynchronize(ctx->ev[3]));
         GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
@@ -554,8 +563,22 @@ int run_streamed(gw_ctx* ctx, Launch& L, const gw_program* prog, uint32_t rec, g
         order.clear();
         for (uint32_t k = 0; k < nb; ++k)
             if (off[k + 1] > off[k]) order.push_back(k);
-        auto score = [&](uint32_t k) { return (off[k + 1] - off[k]) * (ctx->block_slot[k] >= 0 ? 2u : 1u); };
+        // walks per block (MAX_WEIGHT: per hop of the block's least-advanced
+        // walk, a min hop of 0 counting as 1), resident blocks counting double
+        auto score = [&](uint32_t k) {
+            const double w = static_cast<double>(off[k + 1] - off[k]) * (ctx->block_slot[k] >= 0 ? 2.0 : 1.0);
+            return policy == GW_BLOCKS_MAX_WEIGHT ? w / std::max(1u, minhop[k]) : w;
+        };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return score(x) > score(y); });
+        if (policy == GW_BLOCKS_MIN_STEP && coin() < ctx->block_prob) {
+            // the block holding the least-advanced walk goes first (the lowest
+            // block id among equals, as blockWithMinStep scans)
+            uint32_t best = order[0];
+            for (uint32_t k : order)
+                if (minhop[k] < minhop[best] || (minhop[k] == minhop[best] && k < best)) best = k;
+            order.erase(std::find(order.begin(), order.end(), best));
+            order.insert(order.begin(), best);
+        }
         if (order.size() > nslots) order.resize(nslots);
         chosen.clear();
         loaded.clear();
@@ -1206,6 +1229,15 @@ int gw_run_replay(gw_ctx* ctx, gw_stats* stats) {
 // Launch-shape autotuner: the same program at each (block_threads,
 // waves_per_simd) candidate, best of `reps` runs after one warm-up; results
 // do not depend on the launch shape (walks are keyed by id), only time does.
+int gw_stream_policy(gw_ctx* ctx, uint32_t policy, double prob, uint64_t seed) {
+    if (!ctx || policy > GW_BLOCKS_MAX_WEIGHT || !(prob >= 0.0 && prob <= 1.0))
+        return gw_fail(ctx, GW_EINVAL, "gw_stream_policy: unknown policy or prob outside [0, 1]");
+    ctx->block_policy = policy;
+    ctx->block_prob = prob;
+    ctx->block_seed = seed;
+    return GW_OK;
+}
+
 int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* best, double* best_ms) {
     if (!ctx || !prog || !best || reps == 0) return gw_fail(ctx, GW_EINVAL, "gw_autotune: bad arguments");
     static const uint32_t kThreads[] = {64, 128, 256};

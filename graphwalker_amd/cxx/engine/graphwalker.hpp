@@ -87,11 +87,14 @@ public:
 
     gw_ctx* context() { return ctx_; }
 
-    // `prob` chose between the min-step and max-walks block heuristics
-    // (walk.hpp:245-252); with the graph resident in HBM every block is
-    // stepped each round, so it only affects the order of work.
+    // `prob` chooses between the min-step and max-walks block heuristics
+    // (walk.hpp:245-252).  With the graph resident in HBM every block is
+    // stepped each round, so it does not apply; out of core it picks the
+    // blocks each load pass brings into the device block cache.
     void run(RandomWalk& userprogram, float prob) {
-        (void)prob;
+        if (host_resident_)
+            check(gw_stream_policy(ctx_, GW_BLOCKS_MIN_STEP, prob, static_cast<uint64_t>(get_option_long("seed", 1))),
+                  "gw_stream_policy");
         m.start_time("0_startWalks");
         userprogram.startWalks(*walk_manager, nblocks, blocks, base_filename);
         m.stop_time("0_startWalks");

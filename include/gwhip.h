@@ -29,6 +29,8 @@
  *                              builds the same CSR layout on the device)
  *   gw_walks_load              the initial buckets written by startWalksbyApp
  *                              through WalkManager::moveWalk (walk.hpp:93-101)
+ *   gw_stream_policy           WalkManager::chooseBlock and its heuristics
+ *                              (walk.hpp:204-252) for out-of-core load passes
  *   gw_walks_load_host         the walk pool files (writeWalks2Disk /
  *                              readWalksfromDisk, walk.hpp:103-157): a pool kept
  *                              in host memory, stepped window by window
@@ -216,6 +218,20 @@ int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
 int gw_walks_init_pairs(gw_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t npairs, uint32_t R, uint32_t L);
 int gw_meeting_counts(gw_ctx *ctx, uint32_t npairs, uint32_t R, uint32_t L, uint32_t *counts);
 /* count walks per vertex: walk i = vertex i / per_vertex, source = vertex, id = i */
+/* Which blocks a load pass of an out-of-core run (gw_graph_set_host) brings
+ * into the device block cache, after the resident blocks' own walks:
+ *   GW_BLOCKS_MAX_WALKS  the blocks with the most walks, resident blocks
+ *                        counting double (default)
+ *   GW_BLOCKS_MIN_STEP   the reference's chooseBlock(prob) (walk.hpp:245-252):
+ *                        with probability `prob` the block holding the walk
+ *                        with the fewest hops first, then as MAX_WALKS
+ *   GW_BLOCKS_MAX_WEIGHT blockWithMaxWeight (walk.hpp:228-238): walks / min hop
+ * The coin stream is keyed by `seed`.  Results never depend on the policy. */
+#define GW_BLOCKS_MAX_WALKS 0u
+#define GW_BLOCKS_MIN_STEP 1u
+#define GW_BLOCKS_MAX_WEIGHT 2u
+int gw_stream_policy(gw_ctx *ctx, uint32_t policy, double prob, uint64_t seed);
+
 /* Host walk pool: `walks` (n records, walks[i].id == i) stays in host memory
  * -- gw_host_alloc memory is copied from directly -- and must outlive the
  * pool.  gw_run steps it `window` walks at a time through two device windows,

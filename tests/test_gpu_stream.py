@@ -151,3 +151,36 @@ def test_streamed_invalid_requests(engine, graph):
     engine.walks_load(_walks(100, n, gw.STOP, seed=1))
     with pytest.raises(gw.GwError, match="smaller than one block slot"):
         engine.run(gw.program(gw.STOP, 5, 0.15, n, seed=1))
+
+
+@pytest.mark.parametrize("policy,prob", [(gw.BLOCKS_MIN_STEP, 1.0), (gw.BLOCKS_MIN_STEP, 0.2),
+                                         (gw.BLOCKS_MAX_WEIGHT, 0.0)])
+def test_block_policies_keep_results(engine, graph, policy, prob):
+    # the load-pass block choice changes which blocks are copied when, never a trajectory
+    off, col = graph
+    n = off.shape[0] - 1
+    engine.graph_set_host(off, col, 40 << 10)
+    walks = _walks(20_000, n, gw.RESTART_JOINT, seed=6)
+    engine.walks_load(walks)
+    prog = gw.program(gw.RESTART_JOINT, 11, 0.15, n, seed=0xBEEF, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY)
+    ref = _expect(prog, off, col, walks)
+    engine.stream_policy(policy, prob, seed=3)
+    try:
+        st = engine.run(prog)
+        counts1 = engine.counts().copy()
+        # the second run starts with the first one's blocks still cached: other loads, same results
+        st2 = engine.run(prog)
+    finally:
+        engine.stream_policy(gw.BLOCKS_MAX_WALKS)
+    assert st["steps"] == ref["steps"] and np.array_equal(counts1, ref["counts"])
+    assert st2["steps"] == ref["steps"] and np.array_equal(engine.counts(), ref["counts"])
+    h, vis = engine.summary()
+    assert np.array_equal(h, ref["hash"]) and np.array_equal(vis, ref["walk_visits"])
+    assert st["blocks_loaded"] > st["cache_slots"]
+
+
+def test_block_policy_invalid(engine):
+    with pytest.raises(gw.GwError, match="unknown policy"):
+        engine.stream_policy(7)
+    with pytest.raises(gw.GwError, match="unknown policy"):
+        engine.stream_policy(gw.BLOCKS_MIN_STEP, 1.5)
