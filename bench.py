@@ -15,8 +15,10 @@ walk records resident in HBM before timing.  The launch shape (block threads x
 waves per SIMD) is picked by gw_autotune before the timed passes, or fixed
 with --shape; "graph_replay" reports the same passes replayed from one
 captured hipGraph.  Prints one JSON line.  The CPU
-baseline is the oracle's OpenMP replay of the same walk program on a bounded
-sample of the same walks, on the host's cores.
+baseline is the reference's own engine (its rawrandomwalks app, built from its
+sources into oracle/_ref/) on the same graph and walk count, on the host's
+cores; the oracle's OpenMP replay of the same walk program is reported beside
+it (and is the baseline where the reference app is not built).
 """
 from __future__ import annotations
 
@@ -161,9 +163,35 @@ def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) ->
         if elapsed >= target_s / 4 or nw >= total:
             break
         nw = min(total, int(nw * max(2.0, target_s / max(elapsed, 1e-3))))
-    return {"value": r["steps"] / elapsed, "unit": "walk-steps/s", "cores": threads, "kind": "port",
+    port = {"value": r["steps"] / elapsed, "unit": "walk-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle OpenMP replay (per-vertex counts) of the first {nw} walks of the same program on the "
                       f"same device-generated graph: {r['steps']} steps in {elapsed:.2f} s"}
+    if wl["sources"] or wl["kind"] != gw.JUMP or O.reference_app("rawrandomwalks") is None:
+        return port
+    # The reference engine itself (its rawrandomwalks app built from its own
+    # sources): the same graph in its on-disk layout, as many walks as the
+    # workload from uniform sources, L jump steps, its own 0.15 teleport.
+    import shutil
+    import tempfile
+
+    tmp = tempfile.mkdtemp(prefix="gw_ref_")
+    try:
+        base = os.path.join(tmp, "graph")
+        O.write_reference_graph(base, off, col)
+        blocksize_kb = max(1, -(-int(off[-1]) * 4 // 16 // 1024))  # ~16 blocks, as the 2^20-vertex blocks
+        R = total
+        t0 = time.perf_counter()
+        ref = O.run_reference_rawrandomwalks(base, n, R, wl["L"], threads, blocksize_kb, 16, timeout=300)
+        wall = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    steps = R * wl["L"]
+    return {"value": steps / ref["00_runtime"], "unit": "walk-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"reference rawrandomwalks (built from its sources, -O3 -fopenmp, execthreads {threads}) on the "
+                      f"same graph in its on-disk layout, {blocksize_kb} KB blocks all in memory: {R} walks x "
+                      f"{wl['L']} steps (uniform sources, its 0.15 teleport) in {ref['00_runtime']:.2f} s of its "
+                      f"00_runtime ({wall:.1f} s wall)",
+            "exec_phase_walk_steps_per_s": steps / ref["5_exec_updates"], "port": port}
 
 
 def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool,
@@ -199,7 +227,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     print(f"[bench] {name} {sched}-{layout}: {steps / wall / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
     res = {
         "value": steps / wall, "wall_s": wall, "steps": per_pass_steps, "visits": per_pass_visits, "prog": prog,
-        "kernel_ms": kavg, "algorithmic_bytes": abytes, "achieved": achieved, "layout": st["layout"], "n": n,
+        "kernel_ms": kavg, "kernel_ms_median": float(np.median(kms)), "algorithmic_bytes": abytes, "achieved": achieved, "layout": st["layout"], "n": n,
         "variants": variants, "wl": wl, "last": st, "tuned": tuned,
     }
     if sched != "blocked":
@@ -514,6 +542,7 @@ def main() -> int:
                      "frac": m["achieved"] / PEAK_HBM_GBPS, "traffic": traffic,
                      "traffic_copy_calibrated": traffic_copy,
                      "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": m["kernel_ms"],
+                     "kernel_ms_median": m["kernel_ms_median"],
                      "algorithmic_bytes_per_launch": m["algorithmic_bytes"],
                      "bytes_per_step": m["algorithmic_bytes"] / max(1, m["steps"])},
         "random_access_ceiling": {"gathers_per_s": gathers / (gms * 1e-3), "GBps_at_64B": rand_gbps,

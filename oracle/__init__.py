@@ -240,3 +240,64 @@ def set_threads(t: int) -> None:
 
 def num_threads() -> int:
     return lib().gwo_num_threads()
+
+
+# --------------------------------------------------------------- reference apps
+# The reference's own apps, compiled from its sources by build_ref.sh into
+# _ref/ (a CPU baseline only; never the checker of a GPU result).
+
+_REF_DIR = os.path.join(_HERE, "_ref")
+
+
+def reference_app(name: str) -> str | None:
+    """Path of a reference app built into oracle/_ref/, or None."""
+    p = os.path.join(_REF_DIR, name)
+    return p if os.access(p, os.X_OK) else None
+
+
+def write_reference_graph(base: str, off: np.ndarray, col: np.ndarray) -> None:
+    """A CSR in the reference's preprocessed layout, so its convert_if_notexists
+    finds it (conversions.hpp:371-385, filename.hpp:16-51): the u64 absolute
+    offsets <base>_GraphWalker/graphinfo/file_0.beg_pos (n + 1 entries), the u32
+    columns file_0.csr, the one-file range filesize_1024GB.filerange and
+    N.nvertices.  The reference computes its block range itself."""
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    n = off.shape[0] - 1
+    root = base + "_GraphWalker"
+    os.makedirs(os.path.join(root, "graphinfo"), exist_ok=True)
+    off.tofile(os.path.join(root, "graphinfo", "file_0.beg_pos"))
+    col[: int(off[-1])].tofile(os.path.join(root, "graphinfo", "file_0.csr"))
+    with open(os.path.join(root, "filesize_1024GB.filerange"), "w") as f:
+        f.write(f"0\n{n}\n")
+    with open(os.path.join(root, "N.nvertices"), "w") as f:
+        f.write(f"{n}\n")
+
+
+def run_reference_rawrandomwalks(base: str, n: int, R: int, L: int, threads: int, blocksize_kb: int,
+                                 nmblocks: int, timeout: float) -> dict:
+    """Runs the reference rawrandomwalks app (R walks from uniform sources,
+    L jump steps each) on a graph written by write_reference_graph, in the
+    graph's directory, and returns its own metrics (metrics.hpp timers):
+    00_runtime (the run after walk placement) and 5_exec_updates (the step
+    phase), in seconds."""
+    app = reference_app("rawrandomwalks")
+    if app is None:
+        raise FileNotFoundError("oracle/_ref/rawrandomwalks is not built")
+    cwd = os.path.dirname(base)
+    # the reference's option parser insists on a config file (configfile.hpp:81-90);
+    # every option here comes from the command line
+    os.makedirs(os.path.join(cwd, "conf"), exist_ok=True)
+    with open(os.path.join(cwd, "conf", "graphwalker.local.cnf"), "w") as f:
+        f.write("# options are given on the command line\n")
+    args = [app, "file", base, "N", str(n), "R", str(R), "L", str(L), "execthreads", str(threads),
+            "blocksize_kb", str(blocksize_kb), "nmblocks", str(nmblocks), "prob", "0.2"]
+    subprocess.run(args, cwd=cwd, check=True, timeout=timeout, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    out = {}
+    with open(os.path.join(cwd, "graphwalker_metrics.txt")) as f:
+        for line in f:
+            key, _, val = line.strip().partition("=")
+            for k in ("00_runtime", "5_exec_updates", "0_startWalks"):
+                if key.endswith("." + k):
+                    out[k] = float(val)
+    return out

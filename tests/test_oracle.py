@@ -353,3 +353,17 @@ def test_block_engine_equals_whole_graph_replay(kind):
     assert np.array_equal(counts, whole["counts"])
     assert (steps, visits) == (whole["steps"], whole["visits"])
     assert rounds > 5
+
+
+@pytest.mark.skipif(O.reference_app("rawrandomwalks") is None, reason="reference apps not built (oracle/build_ref.sh)")
+def test_reference_engine_reads_written_graph(tmp_path):
+    # the on-disk writer and the reference's reader agree: the reference's own
+    # rawrandomwalks runs on the written graph, computes its block range and
+    # reports its timers (the bench's reference CPU baseline takes this path)
+    off, col = O.rmat_graph(10, 1 << 13, seed=3)
+    base = str(tmp_path / "g")
+    O.write_reference_graph(base, off, col)
+    r = O.run_reference_rawrandomwalks(base, off.shape[0] - 1, 2000, 5, 2, 8, 4, timeout=60)
+    assert r["00_runtime"] > 0 and r["5_exec_updates"] > 0
+    blocks = np.loadtxt(base + "_GraphWalker/blocksize_8KB.blockrange", dtype=np.int64)
+    assert blocks[0] == 0 and blocks[-1] == off.shape[0] - 1 and np.all(np.diff(blocks) > 0)
