@@ -100,7 +100,8 @@ struct SourceTable {
     uint32_t wide;
 };
 
-__device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source, uint32_t v) {
+// Adds `inc` visits to (source, v).
+__device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source, uint32_t v, uint32_t inc = 1) {
     const uint64_t key = (static_cast<uint64_t>(source) << t.vbits) | v;
     uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> t.shift;
     if (t.wide) {
@@ -108,25 +109,25 @@ __device__ __forceinline__ void table_add(const SourceTable& t, uint32_t source,
         for (uint64_t probe = 0; probe <= t.mask; ++probe) {
             const unsigned long long old = atomicCAS(&t.tab[2 * h], 0ull, tag);
             if (old == 0ull || old == tag) {
-                atomicAdd(&t.tab[2 * h + 1], 1ull);
+                atomicAdd(&t.tab[2 * h + 1], static_cast<unsigned long long>(inc));
                 return;
             }
             h = (h + 1) & t.mask;
         }
-        atomicAdd(t.overflow, 1ull);
+        atomicAdd(t.overflow, static_cast<unsigned long long>(inc));
         return;
     }
-    const unsigned long long entry = (static_cast<unsigned long long>(key) << t.cbits) | 1ull;
+    const unsigned long long entry = (static_cast<unsigned long long>(key) << t.cbits) | inc;
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
         const unsigned long long old = atomicCAS(&t.tab[h], 0ull, entry);
         if (old == 0ull) return;
         if ((old >> t.cbits) == key) {
-            atomicAdd(&t.tab[h], 1ull);
+            atomicAdd(&t.tab[h], static_cast<unsigned long long>(inc));
             return;
         }
         h = (h + 1) & t.mask;
     }
-    atomicAdd(t.overflow, 1ull);
+    atomicAdd(t.overflow, static_cast<unsigned long long>(inc));
 }
 
 // Append-only log of (source, vertex) keys for sort-reduce aggregation.  Each
@@ -198,9 +199,30 @@ struct Recorder {
         if (kEvents && visits < log.cap)
             log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
         ++visits;
-        if (kVertexCount) atomicAdd(&counts[v], 1u);
+        if (kVertexCount || kSourceCount) {
+            // A walk standing on its own source field's vertex -- for restart
+            // programs the restart target, a sixth or more of all visits --
+            // would make every such lane of the wavefront hit one address.
+            // Those lanes combine: one atomic per (wavefront, source).
+            const bool at_src = v == source;
+            uint64_t todo = __ballot(at_src);
+            while (todo) {
+                const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+                const uint32_t ls = __shfl(source, leader, 64);
+                const uint64_t grp = __ballot(at_src && source == ls);
+                if (__lane_id() == leader) {
+                    const uint32_t c = static_cast<uint32_t>(__popcll(grp));
+                    if (kVertexCount) atomicAdd(&counts[ls], c);
+                    if (kSourceCount) table_add(table, ls, ls, c);
+                }
+                todo &= ~grp;
+            }
+            if (!at_src) {
+                if (kVertexCount) atomicAdd(&counts[v], 1u);
+                if (kSourceCount) table_add(table, source, v);
+            }
+        }
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
-        if (kSourceCount) table_add(table, source, v);
         if (kKeyLog) keylog_add(klog, slab, source, v);
     }
 };
@@ -274,7 +296,6 @@ template <int KIND, bool EXT, class Rec>
 __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
     constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
     if (capped && s.hop >= p.L) return true;
-    rec.visit(s.v, s.hop);
     const u32x4 r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
     uint64_t b, deg;
     if (EXT) {
@@ -284,18 +305,25 @@ __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& 
         b = g.off[s.v];
         deg = g.off[s.v + 1] - b;
     }
+    // The next edge's load is issued before the visit is recorded: a recorder
+    // that waits on memory (a hash probe) then overlaps the edge's latency.
+    // Counts commute and a walk's own visits keep their order.
     if (deg > 0 && static_cast<uint64_t>(r.x) >= p.stop_thr) {
         const uint64_t x = static_cast<uint64_t>(r.y) | (static_cast<uint64_t>(r.z) << 32);
         const uint64_t e = b + mulhi64(x, deg);
         if (EXT) {
             const Edge ed = g.ext[e];
+            rec.visit(s.v, s.hop);
             s.v = ed.dst;
             s.deg = ed.deg;
             s.row = ed.row;
         } else {
-            s.v = g.col[e];
+            const uint32_t nv = g.col[e];
+            rec.visit(s.v, s.hop);
+            s.v = nv;
         }
     } else {
+        rec.visit(s.v, s.hop);
         if (KIND == GW_JUMP) {
             s.v = mulhi32(r.w, p.nvertices);
             load_row<EXT>(g, s);

@@ -11,6 +11,8 @@
 
 // Source fields below this bound get a partitioned key log (one region per field).
 constexpr uint32_t kPartFieldLimit = 1u << 20;
+// GW_SRC_SORT picks the partitioned aggregation up to this mean region size (keys per source field).
+constexpr uint64_t kPartAutoRegionKeys = 1ull << 17;
 
 // Device counters (unsigned long long each).
 constexpr uint32_t kCounters = 32;

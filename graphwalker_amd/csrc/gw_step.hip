@@ -862,7 +862,16 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     // Partitioned source counts: regions and pair buffers are sized before
     // the run so partition + aggregation follow the step kernel on the
     // stream with no host round trip in between.
-    if ((rec & kRecKeyLog) && prog->source_method != GW_SRC_GLOBAL && !ctx->field_walks.empty()) {
+    // The automatic method takes the partitioned aggregation while the mean
+    // source region (walks x keys per walk) fits what the LDS tables absorb
+    // without flushing most sources; heavier sources sort-reduce globally.
+    bool part_ok = !ctx->field_walks.empty();
+    if (part_ok && prog->source_method == GW_SRC_SORT) {
+        uint64_t nonempty = 0;
+        for (uint64_t w : ctx->field_walks) nonempty += w != 0;
+        part_ok = ctx->nwalks * keylog_per_walk(prog) <= kPartAutoRegionKeys * std::max<uint64_t>(1, nonempty);
+    }
+    if ((rec & kRecKeyLog) && prog->source_method != GW_SRC_GLOBAL && part_ok) {
         int rc = gw_partlog_prepare(ctx, keylog_per_walk(prog), &rp.part);
         if (rc) return rc;
         if (!rp.part && prog->source_method == GW_SRC_PARTITIONED)
