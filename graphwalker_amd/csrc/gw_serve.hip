@@ -111,8 +111,8 @@ __device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
 // Per-visit recorder of a served walk.  Visits to the query's own source (a
 // quarter or more of a restart walk's visits) go to a per-query counter, one
 // atomic per (wavefront, query); other vertices go to the query's hash region:
-// a plain load finds an existing entry (one atomic add), an empty slot is
-// claimed with compare-and-swap.
+// one compare-and-swap either claims an empty slot or returns the entry
+// there (then an atomic add if it is this vertex's).
 struct ServeRec {
     unsigned long long* arena;
     uint32_t* srccount;
@@ -138,11 +138,8 @@ struct ServeRec {
         uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             unsigned long long* cell = arena + base + h;
-            unsigned long long cur = __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (cur == 0ull) {
-                cur = atomicCAS(cell, 0ull, entry);
-                if (cur == 0ull) return;
-            }
+            const unsigned long long cur = atomicCAS(cell, 0ull, entry);
+            if (cur == 0ull) return;
             if ((cur >> cbits) == v) {
                 atomicAdd(cell, 1ull);
                 return;
@@ -217,11 +214,18 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
             const uint32_t hop_before = s.hop;
             done = step_once<KIND, EXT>(q, g, s, rec);
             budget -= (s.hop != hop_before) ? 1u : 0u;
-            if (done) {
-                atomicSub(&remaining[slot], 1u);
-            } else if (budget == 0) {
-                leave = true;
-            }
+            leave = !done && budget == 0;
+        }
+        // walks that ended: one decrement per (wavefront, query)
+        uint64_t todo = __ballot(done);
+        while (todo) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+            const uint32_t ls = __shfl(slot, leader, 64);
+            const uint64_t grp = __ballot(done && slot == ls);
+            if (lane == leader) atomicSub(&remaining[ls], static_cast<uint32_t>(__popcll(grp)));
+            todo &= ~grp;
+        }
+        if (active) {
             if (done || leave) {
                 tot_steps += rec.steps;
                 tot_visits += rec.visits;
