@@ -278,7 +278,13 @@ struct State {
     uint32_t id, source, v, hop;
     uint64_t row;
     uint32_t deg;
+    u32x4 r;  // the Philox draw of (id, hop): set by start_walk, advanced by step_once
 };
+
+// A walk picked up by a kernel: its draw for the current hop.
+__device__ __forceinline__ void start_walk(const Prog& p, State& s) {
+    s.r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
+}
 
 template <bool EXT>
 __device__ __forceinline__ void load_row(const Graph& g, State& s) {
@@ -289,14 +295,16 @@ __device__ __forceinline__ void load_row(const Graph& g, State& s) {
     }
 }
 
-// One step of a walk.  Returns true when the walk has ended; the final
-// vertex/hop stay in the state, as the reference leaves them in the record it
-// drops.  Else-branches per kind follow advance() in gw_device.h.
+// One step of a walk (its state set up by start_walk).  Returns true when the
+// walk has ended; the final vertex/hop stay in the state, as the reference
+// leaves them in the record it drops.  Else-branches per kind follow advance()
+// in gw_device.h.  The draw of the next hop depends only on (id, hop + 1), so
+// it is computed while the edge load is in flight.
 template <int KIND, bool EXT, class Rec>
 __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
     constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
     if (capped && s.hop >= p.L) return true;
-    const u32x4 r = philox4x32_10(s.id, s.hop, kTagWalk, p.stream, p.k0, p.k1);
+    const u32x4 r = s.r;
     uint64_t b, deg;
     if (EXT) {
         b = s.row;
@@ -314,21 +322,23 @@ __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& 
         if (EXT) {
             const Edge ed = g.ext[e];
             rec.visit(s.v, s.hop);
+            s.r = philox4x32_10(s.id, s.hop + 1, kTagWalk, p.stream, p.k0, p.k1);
             s.v = ed.dst;
             s.deg = ed.deg;
             s.row = ed.row;
         } else {
             const uint32_t nv = g.col[e];
             rec.visit(s.v, s.hop);
+            s.r = philox4x32_10(s.id, s.hop + 1, kTagWalk, p.stream, p.k0, p.k1);
             s.v = nv;
         }
     } else {
         rec.visit(s.v, s.hop);
+        if (KIND == GW_STOP || KIND == GW_PROB) return true;
+        s.r = philox4x32_10(s.id, s.hop + 1, kTagWalk, p.stream, p.k0, p.k1);
         if (KIND == GW_JUMP) {
             s.v = mulhi32(r.w, p.nvertices);
             load_row<EXT>(g, s);
-        } else if (KIND == GW_STOP || KIND == GW_PROB) {
-            return true;
         } else {
             s.v = s.source;
             load_row<EXT>(g, s);

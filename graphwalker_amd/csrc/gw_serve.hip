@@ -196,6 +196,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
                 load_row<EXT>(g, s);
                 q.stream = si.qid_lo;
                 q.k1 = p.k1 ^ si.qid_hi;
+                start_walk(q, s);
                 rec.base = si.base;
                 rec.mask = si.mask;
                 rec.slot = slot;

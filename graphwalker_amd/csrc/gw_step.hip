@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 s.v = w.vertex;
                 s.hop = w.hop;
                 load_row<EXT>(g, s);
+                start_walk(p, s);
                 active = true;
                 rec.visits = 0;
                 rec.steps = 0;
@@ -255,6 +256,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                     gl.col = bv.col;
                 }
                 load_row<EXT>(gl, s);
+                start_walk(p, s);
                 if (SUM) {
                     rec.hash = hash[s.id - p.id_base];
                     rec.visits = visits[s.id - p.id_base];
