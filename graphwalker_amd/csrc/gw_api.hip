@@ -233,6 +233,7 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
     gw_free(ctx, ctx->d_bucket_off, 0);
     gw_free(ctx, ctx->d_block_minhop, ctx->block_minhop_cap * 4ull);
+    gw_free(ctx, ctx->d_meet, ctx->meet_bytes);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->copy_ev)

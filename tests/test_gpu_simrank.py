@@ -105,3 +105,26 @@ def test_meeting_counts_invalid(engine):
         engine.meeting_counts(1, 10, 40)
     with pytest.raises(gw.GwError, match="out of range"):
         engine.walks_init_pairs(np.array([n], np.uint32), np.array([2], np.uint32), 10, 11)
+
+
+@pytest.mark.parametrize("R,same", [(1024, True), (1, False), (333, False)])
+def test_meeting_counts_sparse_join_edges(engine, R, same):
+    # the sparse join's limits: R at its maximum with every pair meeting at
+    # offset 0 (a == b), a single walk per side, a hub-heavy small graph
+    off, col = O.rmat_graph(7, 1 << 11, seed=5)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    L = 6
+    a = np.array([3, 9], np.uint32)
+    b = np.array([3 if same else 17, 40], np.uint32)
+    engine.walks_init_pairs(a, b, R, L)
+    prog = gw.program(gw.RESTART_JOINT, L, 0.15, n, seed=13, recorder=gw.REC_EVENTS, event_cap=L - 1)
+    engine.run(prog)
+    got = engine.meeting_counts(2, R, L)
+    op = O.Program(prog.kind, prog.L, prog.stop_thr, prog.nvertices, prog.stream, prog.seed)
+    for k in range(2):
+        A = _paths(op, off, col, a[k], R, L, 2 * k * R)
+        B = _paths(op, off, col, b[k], R, L, 2 * k * R + R)
+        assert np.array_equal(got[k], _first_meetings(A, B, L)), k
+    if same:
+        assert got[0, 0] == R * R
