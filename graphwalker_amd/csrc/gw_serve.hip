@@ -41,6 +41,8 @@ struct ServeState {
     gw_walk* pool[2] = {nullptr, nullptr};
     int cur = 0;
     unsigned long long* arena = nullptr;
+    uint32_t* d_occ = nullptr;            // per arena slot: each region's claimed slots, in claim order
+    uint32_t* d_occn = nullptr;           // per query slot: length of its occupied list
     SlotInfo* d_slots = nullptr;
     uint32_t* d_remaining = nullptr;
     uint32_t* d_srccount = nullptr;       // visits of each query to its own source
@@ -50,7 +52,7 @@ struct ServeState {
     uint64_t pairs_cap = 0;
     Admit* d_admit = nullptr;               // max_queries admission records
     Done* d_done = nullptr;                 // max_queries completion records
-    unsigned long long* d_region_n = nullptr;  // 2 x max_queries: counts, cursors
+    unsigned long long* d_region_n = nullptr;  // max_queries: entries of each completed query
     struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; uint32_t source; };
     std::vector<Active> active;
     std::vector<uint32_t> free_slots;
@@ -112,10 +114,14 @@ __device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
 // quarter or more of a restart walk's visits) go to a per-query counter, one
 // atomic per (wavefront, query); other vertices go to the query's hash region:
 // one compare-and-swap either claims an empty slot or returns the entry
-// there (then an atomic add if it is this vertex's).
+// there (then an atomic add if it is this vertex's).  A claimed slot is
+// appended to the query's occupied list (one atomic per (wavefront, query)),
+// so completion reads the entries without scanning the region.
 struct ServeRec {
     unsigned long long* arena;
     uint32_t* srccount;
+    uint32_t* occ;
+    uint32_t* occn;
     uint64_t base;
     uint32_t mask;
     uint32_t cbits;
@@ -133,20 +139,41 @@ struct ServeRec {
             if (__lane_id() == leader) atomicAdd(&srccount[ls], static_cast<uint32_t>(__popcll(grp)));
             todo &= ~grp;
         }
-        if (at_src) return;
-        const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | 1ull;
-        uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            unsigned long long* cell = arena + base + h;
-            const unsigned long long cur = atomicCAS(cell, 0ull, entry);
-            if (cur == 0ull) return;
-            if ((cur >> cbits) == v) {
-                atomicAdd(cell, 1ull);
-                return;
+        constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+        uint32_t claimed = kNoSlot;
+        if (!at_src) {
+            const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | 1ull;
+            uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+            bool placed = false;
+            for (uint32_t probe = 0; probe <= mask; ++probe) {
+                unsigned long long* cell = arena + base + h;
+                const unsigned long long cur = atomicCAS(cell, 0ull, entry);
+                if (cur == 0ull) {
+                    claimed = h;
+                    placed = true;
+                    break;
+                }
+                if ((cur >> cbits) == v) {
+                    atomicAdd(cell, 1ull);
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & mask;
             }
-            h = (h + 1) & mask;
+            if (!placed) atomicAdd(overflow, 1ull);
         }
-        atomicAdd(overflow, 1ull);
+        const bool cl = claimed != kNoSlot;
+        uint64_t pend = __ballot(cl);
+        while (pend) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(pend)) - 1;
+            const uint32_t ls = __shfl(slot, leader, 64);
+            const uint64_t grp = __ballot(cl && slot == ls);
+            uint32_t b0 = 0;
+            if (__lane_id() == leader) b0 = atomicAdd(&occn[ls], static_cast<uint32_t>(__popcll(grp)));
+            b0 = __shfl(b0, leader, 64);
+            if (cl && slot == ls) occ[base + b0 + rank_of(grp)] = claimed;
+            pend &= ~grp;
+        }
     }
 };
 
@@ -159,8 +186,10 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
                                                         const SlotInfo* __restrict__ slots,
                                                         uint32_t* __restrict__ remaining,
                                                         uint32_t* __restrict__ srccount,
-                                                        unsigned long long* __restrict__ arena, uint32_t cbits,
-                                                        uint32_t hops, unsigned long long* __restrict__ cnt,
+                                                        unsigned long long* __restrict__ arena,
+                                                        uint32_t* __restrict__ occ, uint32_t* __restrict__ occn,
+                                                        uint32_t cbits, uint32_t hops,
+                                                        unsigned long long* __restrict__ cnt,
                                                         unsigned long long* __restrict__ cursor) {
     constexpr uint32_t kChunk = 256;
     const uint32_t lane = __lane_id();
@@ -168,7 +197,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
     bool exhausted = false, active = false;
     uint32_t slot = 0, budget = 0, tot_steps = 0, tot_visits = 0;
     Prog q = p;
-    ServeRec rec{arena, srccount, 0, 0, cbits, cnt + 3, 0, 0};
+    ServeRec rec{arena, srccount, occ, occn, 0, 0, cbits, cnt + 3, 0, 0};
     State s{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -262,12 +291,13 @@ struct Admit {
 // count, then its walk records (id = walk index, source field = slot).
 __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __restrict__ pool,
                                    SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining,
-                                   uint32_t* __restrict__ srccount) {
+                                   uint32_t* __restrict__ srccount, uint32_t* __restrict__ occn) {
     const Admit a = adm[blockIdx.x];
     if (threadIdx.x == 0) {
         slots[a.slot] = a.info;
         remaining[a.slot] = a.walks;
         srccount[a.slot] = 0;
+        occn[a.slot] = 0;
     }
     for (uint32_t i = threadIdx.x; i < a.walks; i += blockDim.x)
         pool[a.at + i] = gw_walk{i, a.slot, a.info.source, 0u};
@@ -280,39 +310,19 @@ struct Done {
     uint32_t slot, source; // the query's slot (its source-vertex counter) and source
 };
 
-// Pass 1: non-empty slots per region (grid.y = region).
-__global__ void serve_count_kernel(const unsigned long long* __restrict__ arena, const Done* __restrict__ done,
-                                   unsigned long long* __restrict__ per_region) {
+// A completed query's entries: its occupied list names the region slots it
+// claimed; each entry is copied to the query's output range and its slot
+// cleared for reuse (grid.y = query, `n` = list lengths).
+__global__ void serve_gather_kernel(unsigned long long* __restrict__ arena, const uint32_t* __restrict__ occ,
+                                    const Done* __restrict__ done, const unsigned long long* __restrict__ n,
+                                    unsigned long long* __restrict__ out) {
     const Done d = done[blockIdx.y];
-    uint32_t c = 0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < d.size;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        c += arena[d.base + i] != 0ull;
-    c = wave_sum(c);
-    if (__lane_id() == 0 && c) atomicAdd(&per_region[blockIdx.y], static_cast<unsigned long long>(c));
-}
-
-// Pass 2: compact each region's entries to its output range, then clear the
-// region for reuse.
-__global__ void serve_compact_kernel(unsigned long long* __restrict__ arena, const Done* __restrict__ done,
-                                     unsigned long long* __restrict__ cursor, unsigned long long* __restrict__ out) {
-    const Done d = done[blockIdx.y];
-    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < d.size;
-         i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t i = i0 + threadIdx.x;
-        const unsigned long long x = i < d.size ? arena[d.base + i] : 0ull;
-        const bool full = x != 0ull;
-        const uint64_t m = __ballot(full);
-        if (m) {
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-            unsigned long long b = 0;
-            if (__lane_id() == leader) b = atomicAdd(&cursor[blockIdx.y], static_cast<unsigned long long>(__popcll(m)));
-            b = __shfl(b, leader, 64);
-            if (full) {
-                out[d.out + b + rank_of(m)] = x;
-                arena[d.base + i] = 0ull;
-            }
-        }
+    const uint64_t m = n[blockIdx.y];
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t h = occ[d.base + i];
+        out[d.out + i] = arena[d.base + h];
+        arena[d.base + h] = 0ull;
     }
 }
 
@@ -352,11 +362,11 @@ hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, u
     const Graph g{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     if (S.ext)
         serve_step_kernel<KIND, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.cbits,
+            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
             S.hops, S.d_cnt, S.d_cnt + 5);
     else
         serve_step_kernel<KIND, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.cbits,
+            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
             S.hops, S.d_cnt, S.d_cnt + 5);
     return hipGetLastError();
 }
@@ -392,19 +402,16 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
     if ((rc = gw_alloc(ctx, (void**)&S->pool[0], pool_walks * sizeof(gw_walk), "serve pool")) ||
         (rc = gw_alloc(ctx, (void**)&S->pool[1], pool_walks * sizeof(gw_walk), "serve pool (alt)")) ||
         (rc = gw_alloc(ctx, (void**)&S->arena, S->arena_slots * 8, "serve count arena")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_occ, S->arena_slots * 4, "serve occupied lists")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_occn, max_queries * 4ull, "serve occupied list lengths")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_slots, max_queries * sizeof(SlotInfo), "serve slots")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_remaining, max_queries * 4ull, "serve remaining")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_srccount, max_queries * 4ull, "serve source counts")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
-        (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 16ull, "serve region counts"))) {
-        gw_free(ctx, S->pool[0], pool_walks * sizeof(gw_walk));
-        gw_free(ctx, S->pool[1], pool_walks * sizeof(gw_walk));
-        gw_free(ctx, S->arena, S->arena_slots * 8);
-        gw_free(ctx, S->d_slots, max_queries * sizeof(SlotInfo));
-        gw_free(ctx, S->d_remaining, max_queries * 4ull);
-        delete S;
+        (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 8ull, "serve region counts"))) {
+        gw_serve_close_state(ctx, S);
         return rc;
     }
     S->ext = prog->layout != GW_LAYOUT_CSR && gw_ensure_ext(ctx);
@@ -466,7 +473,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         GW_HIP(ctx, hipMemcpyAsync(S.d_admit, adm.data(), adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
                                    ctx->stream));
         serve_admit_kernel<<<dim3(static_cast<uint32_t>(adm.size())), dim3(256), 0, ctx->stream>>>(
-            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount);
+            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount, S.d_occn);
         GW_HIP(ctx, hipGetLastError());
         st.admitted = static_cast<uint32_t>(adm.size());
     }
@@ -485,8 +492,9 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     unsigned long long cnt[4];
     GW_HIP(ctx, hipMemcpyAsync(cnt, S.d_cnt, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<uint32_t> rem(S.max_queries);
+    std::vector<uint32_t> rem(S.max_queries), occn(S.max_queries);
     GW_HIP(ctx, hipMemcpyAsync(rem.data(), S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipMemcpyAsync(occn.data(), S.d_occn, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
@@ -496,31 +504,22 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     st.step_ms = ms;
     S.live = S.live ? cnt[0] : 0;
     S.cur ^= 1;
-    // 3. completed queries: count, scan and compact their regions in one pass
-    //    each (regions are cleared for reuse as they are compacted)
+    // 3. completed queries: their entries, named by their occupied lists, are
+    //    gathered back to back (and their slots cleared for reuse)
     std::vector<ServeState::Active> still, fin;
     for (const auto& a : S.active) (rem[a.slot] != 0 ? still : fin).push_back(a);
     if (!fin.empty()) {
-        std::vector<Done> dn(fin.size());
-        uint64_t maxsize = 0;
-        for (size_t i = 0; i < fin.size(); ++i) {
-            dn[i] = Done{fin[i].base, fin[i].size, 0, fin[i].slot, fin[i].source};
-            maxsize = std::max(maxsize, fin[i].size);
-        }
         const uint32_t nd = static_cast<uint32_t>(fin.size());
-        const uint32_t gx = static_cast<uint32_t>(std::min<uint64_t>(64, (maxsize + 255) / 256));
-        GW_HIP(ctx, hipMemcpyAsync(S.d_done, dn.data(), nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
-        GW_HIP(ctx, hipMemsetAsync(S.d_region_n, 0, nd * 16ull, ctx->stream));
-        serve_count_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_done, S.d_region_n);
-        GW_HIP(ctx, hipGetLastError());
+        std::vector<Done> dn(nd);
         std::vector<unsigned long long> per(nd);
-        GW_HIP(ctx, hipMemcpyAsync(per.data(), S.d_region_n, nd * 8ull, hipMemcpyDeviceToHost, ctx->stream));
-        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        uint64_t total = 0;
+        uint64_t total = 0, maxper = 0;
         for (uint32_t i = 0; i < nd; ++i) {
-            dn[i].out = total;
+            per[i] = occn[fin[i].slot];
+            dn[i] = Done{fin[i].base, fin[i].size, total, fin[i].slot, fin[i].source};
             total += per[i];
+            maxper = std::max<uint64_t>(maxper, per[i]);
         }
+        const uint32_t gx = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(64, (maxper + 255) / 256)));
         if (total > S.pairs_cap) {
             gw_free(ctx, S.d_pairs, S.pairs_cap * 8);
             S.d_pairs = nullptr;
@@ -530,8 +529,9 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
             S.pairs_cap = std::max<uint64_t>(total, 1 << 20);
         }
         GW_HIP(ctx, hipMemcpyAsync(S.d_done, dn.data(), nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
-        serve_compact_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_done, S.d_region_n + nd,
-                                                                          S.d_pairs);
+        GW_HIP(ctx, hipMemcpyAsync(S.d_region_n, per.data(), nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
+        serve_gather_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_done, S.d_region_n,
+                                                                         S.d_pairs);
         GW_HIP(ctx, hipGetLastError());
         if (S.topk) {
             // top-k per completed query on the device: k pairs per query cross PCIe
@@ -656,6 +656,8 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->pool[0], S->pool_cap * sizeof(gw_walk));
     gw_free(ctx, S->pool[1], S->pool_cap * sizeof(gw_walk));
     gw_free(ctx, S->arena, S->arena_slots * 8);
+    gw_free(ctx, S->d_occ, S->arena_slots * 4);
+    gw_free(ctx, S->d_occn, S->max_queries * 4ull);
     gw_free(ctx, S->d_slots, S->max_queries * sizeof(SlotInfo));
     gw_free(ctx, S->d_remaining, S->max_queries * 4ull);
     gw_free(ctx, S->d_srccount, S->max_queries * 4ull);
@@ -663,7 +665,7 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_pairs, S->pairs_cap * 8);
     gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
     gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
-    gw_free(ctx, S->d_region_n, S->max_queries * 16ull);
+    gw_free(ctx, S->d_region_n, S->max_queries * 8ull);
     gw_free(ctx, S->d_topk, static_cast<uint64_t>(S->max_queries) * S->topk * 8);
     gw_free(ctx, S->d_topk_n, S->max_queries * 4ull);
     delete S;
