@@ -408,11 +408,24 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
             for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) t.hist[b] = 0;
             __syncthreads();
             const unsigned long long prefix = t.prefix;
+            // Visit counts are skewed (most pairs hold 1 or 2), so most keys
+            // share a bin: a thread folds runs of one bin in registers and adds
+            // each run with one LDS atomic.
+            uint32_t cur = 0, run = 0;
             for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
                 const unsigned long long key = key_at(i);
-                if (key != 0ull && (shift == top || (key >> (shift + 8)) == prefix))
-                    atomicAdd(&t.hist[(key >> shift) & 255u], 1u);
+                if (key != 0ull && (shift == top || (key >> (shift + 8)) == prefix)) {
+                    const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
+                    if (d == cur) {
+                        ++run;
+                    } else {
+                        if (run) atomicAdd(&t.hist[cur], run);
+                        cur = d;
+                        run = 1;
+                    }
+                }
             }
+            if (run) atomicAdd(&t.hist[cur], run);
             __syncthreads();
             if (threadIdx.x == 0) {
                 uint32_t above = 0;
