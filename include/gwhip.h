@@ -307,7 +307,8 @@ int gw_events_read(gw_ctx *ctx, uint64_t *out, uint64_t n_walks, uint32_t cap);
  * counts accumulate in its own hash region of a count arena in HBM; queries
  * are admitted first come first served while walk slots, a query slot and
  * arena space (2 x min(walks * L, n) slots rounded up to a power of two) are
- * free.  The random stream of walk i of query q is Philox(counter {i, hop,
+ * free.  The arena takes arena_bytes of HBM plus half as much again for each
+ * region's list of occupied slots (completion reads only those).  The random stream of walk i of query q is Philox(counter {i, hop,
  * 'WALK', q.qid low word}, key {seed low, seed high ^ q.qid high word}), so a
  * query's result does not depend on the other queries in flight.  There is
  * no counterpart in the reference (it has no serving loop). */
