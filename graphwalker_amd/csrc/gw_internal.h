@@ -110,6 +110,7 @@ struct gw_ctx {
     uint32_t walk_src_max = 0;  // largest source field among loaded walks
     uint32_t walk_v_max = 0;    // largest start vertex among loaded walks (checked against the graph per run)
     gw_pool* pool = nullptr;    // host walk pool (gw_walks_load_host), else null
+    void* serve_state = nullptr; // the open server (gw_serve.hip), else null
     uint32_t* d_meet = nullptr; // gw_meeting_counts output, kept between calls
     uint64_t meet_bytes = 0;
     gw_walk* d_walks = nullptr;

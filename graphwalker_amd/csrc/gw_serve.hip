@@ -65,10 +65,8 @@ struct ServeState {
     uint32_t* d_topk_n = nullptr;              // max_queries
 };
 
-std::map<gw_ctx*, ServeState*>& states() {
-    static std::map<gw_ctx*, ServeState*> m;
-    return m;
-}
+// The context's serving state (null when no server is open on it).
+ServeState* state_of(gw_ctx* ctx) { return static_cast<ServeState*>(ctx->serve_state); }
 
 uint32_t bits_for(uint64_t x) {
     uint32_t b = 0;
@@ -388,7 +386,7 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
     if (prog->kind != GW_RESTART && prog->kind != GW_STOP && prog->kind != GW_JUMP)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: serving runs RESTART, STOP or JUMP programs");
     if (prog->L == 0 || prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad program");
-    if (states().count(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: already open");
+    if (state_of(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: already open");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     ServeState* S = new ServeState();
     S->prog = *prog;
@@ -423,15 +421,15 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t i = max_queries; i-- > 0;) S->free_slots.push_back(i);
     S->free_ranges.push_back({0, S->arena_slots});
-    states()[ctx] = S;
+    ctx->serve_state = S;
     return GW_OK;
 }
 
 int gw_serve_submit(gw_ctx* ctx, const gw_query* q, uint32_t n) {
     if (!ctx || (n && !q)) return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: bad arguments");
-    auto it = states().find(ctx);
-    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_submit: not open");
-    ServeState& S = *it->second;
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_submit: not open");
+    ServeState& S = *sp;
     for (uint32_t i = 0; i < n; ++i) {
         if (q[i].source >= ctx->n || q[i].walks == 0 || q[i].walks > S.pool_cap)
             return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) +
@@ -443,9 +441,9 @@ int gw_serve_submit(gw_ctx* ctx, const gw_query* q, uint32_t n) {
 
 int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     if (!ctx) return GW_EINVAL;
-    auto it = states().find(ctx);
-    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_step: not open");
-    ServeState& S = *it->second;
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_step: not open");
+    ServeState& S = *sp;
     GW_HIP(ctx, hipSetDevice(ctx->device));
     gw_serve_stats st{};
     // 1. admit queued queries in arrival order while walk slots, a query slot
@@ -587,9 +585,9 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
 
 int gw_serve_poll(gw_ctx* ctx, uint64_t* qid, uint64_t* npairs) {
     if (!ctx || !qid || !npairs) return gw_fail(ctx, GW_EINVAL, "gw_serve_poll: bad arguments");
-    auto it = states().find(ctx);
-    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: not open");
-    ServeState& S = *it->second;
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: not open");
+    ServeState& S = *sp;
     if (S.done_order.empty()) return 1;  // nothing completed
     *qid = S.done_order.front();
     S.done_order.pop_front();
@@ -599,9 +597,9 @@ int gw_serve_poll(gw_ctx* ctx, uint64_t* qid, uint64_t* npairs) {
 
 int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count, uint64_t cap) {
     if (!ctx) return GW_EINVAL;
-    auto it = states().find(ctx);
-    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_result: not open");
-    ServeState& S = *it->second;
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_result: not open");
+    ServeState& S = *sp;
     auto d = S.done.find(qid);
     if (d == S.done.end()) return gw_fail(ctx, GW_EINVAL, "gw_serve_result: no completed query with that id");
     std::vector<unsigned long long>& pairs = d->second;
@@ -618,10 +616,10 @@ int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count
 
 int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     if (!ctx) return GW_EINVAL;
-    auto it = states().find(ctx);
-    if (it == states().end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_topk: not open");
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_topk: not open");
     if (k > kTopkMax) return gw_fail(ctx, GW_EINVAL, "gw_serve_topk: k must be at most 1024");
-    ServeState& S = *it->second;
+    ServeState& S = *sp;
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     gw_free(ctx, S.d_topk, static_cast<uint64_t>(S.max_queries) * S.topk * 8);
@@ -640,10 +638,9 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
 
 int gw_serve_close(gw_ctx* ctx) {
     if (!ctx) return GW_EINVAL;
-    auto it = states().find(ctx);
-    if (it == states().end()) return GW_OK;
-    ServeState* S = it->second;
-    states().erase(it);
+    ServeState* S = state_of(ctx);
+    if (!S) return GW_OK;
+    ctx->serve_state = nullptr;
     gw_serve_close_state(ctx, S);
     return GW_OK;
 }
