@@ -219,6 +219,20 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
             variants[f"{s_}-{l_}"] = {"walk_steps_per_s": st_ / w_, "kernel_ms": float(np.mean(k_)),
                                       "rounds": last["rounds"], "bucket_ms": last["bucket_ms"]}
             print(f"[bench] {name} {s_}-{l_}: {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
+        # the blocked schedule with its block size and launch shape picked by gw_autotune
+        bprog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"],
+                           schedule="blocked", layout="ext")
+        bbest, _ = eng.autotune(bprog, reps=1)
+        nb = eng.graph_info()["nblocks"]
+        _, w_, st_, _, k_, last = timed(eng, wl, n, "blocked", "ext", max(2, K // 3), 1,
+                                        shape=(bbest.block_threads, bbest.waves_per_simd))
+        variants["blocked-ext-tuned"] = {"walk_steps_per_s": st_ / w_, "kernel_ms": float(np.mean(k_)),
+                                         "rounds": last["rounds"], "bucket_ms": last["bucket_ms"], "blocks": nb,
+                                         "block_vertices": -(-n // nb),
+                                         "launch_shape": [bbest.block_threads, bbest.waves_per_simd]}
+        print(f"[bench] {name} blocked-ext tuned ({nb} blocks): {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr,
+              flush=True)
+        eng.graph_blocks(np.arange(0, n + 1, wl["block_vertices"], dtype=np.uint32))
     prog, wall, steps, visits, kms, st = timed(eng, wl, n, sched, layout, K, W, shape=shape)
     per_pass_steps, per_pass_visits = steps // K, visits // K
     kavg = float(np.mean(kms))

@@ -1256,6 +1256,44 @@ int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* 
     const uint32_t sched = prog->schedule != GW_SCHED_AUTO ? prog->schedule
                            : ctx->host_graph            ? GW_SCHED_BLOCKED
                                                         : GW_SCHED_PERSISTENT;
+    auto time_of = [&](const gw_program& p, double& ms) -> int {
+        gw_stats st;
+        int rc = gw_run(ctx, &p, &st);
+        if (rc) return rc;
+        ms = 1e300;
+        for (uint32_t r = 0; r < reps; ++r) {
+            if ((rc = gw_run(ctx, &p, &st))) return rc;
+            ms = std::min(ms, st.total_ms);
+        }
+        return GW_OK;
+    };
+    // The blocked schedule on a device-resident graph: the block size first,
+    // over uniform tables of 2^12 .. 2^24 vertices per block (at most 2048
+    // blocks), at the program's launch shape; the fastest table stays
+    // installed (trajectories do not depend on it).
+    if (sched == GW_SCHED_BLOCKED && !ctx->host_graph && ctx->n > 0) {
+        std::vector<uint32_t> best_table = ctx->block_first;
+        double best_t = 1e300;
+        for (uint32_t k = 12; k <= 24; k += 2) {
+            const uint64_t bv = 1ull << k;
+            const uint64_t nb = (ctx->n + bv - 1) / bv;
+            if (nb > 2048) continue;
+            std::vector<uint32_t> table(nb + 1);
+            for (uint64_t b = 0; b < nb; ++b) table[b] = static_cast<uint32_t>(b * bv);
+            table[nb] = ctx->n;
+            int rc = gw_graph_blocks(ctx, table.data(), static_cast<uint32_t>(nb));
+            double t = 0;
+            if (!rc) rc = time_of(*prog, t);
+            if (rc) return rc;
+            if (t < best_t) {
+                best_t = t;
+                best_table = table;
+            }
+            if (nb == 1) break;  // larger blocks give the same one-block table
+        }
+        int rc = gw_graph_blocks(ctx, best_table.data(), static_cast<uint32_t>(best_table.size() - 1));
+        if (rc) return rc;
+    }
     double bms = 1e300;
     gw_program bp = *prog;
     for (uint32_t tb : kThreads) {
@@ -1264,14 +1302,9 @@ int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* 
             gw_program p = *prog;
             p.block_threads = tb;
             p.waves_per_simd = sched == GW_SCHED_PLAIN ? 0 : wps;
-            gw_stats st;
-            int rc = gw_run(ctx, &p, &st);
+            double m = 0;
+            int rc = time_of(p, m);
             if (rc) return rc;
-            double m = 1e300;
-            for (uint32_t r = 0; r < reps; ++r) {
-                if ((rc = gw_run(ctx, &p, &st))) return rc;
-                m = std::min(m, st.total_ms);
-            }
             if (m < bms) {
                 bms = m;
                 bp = p;

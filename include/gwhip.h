@@ -269,9 +269,12 @@ int gw_run(gw_ctx *ctx, const gw_program *prog, gw_stats *stats);
  * buffers, the graph or the walks changed since the capture. */
 int gw_run_capture(gw_ctx *ctx, const gw_program *prog);
 int gw_run_replay(gw_ctx *ctx, gw_stats *stats);
-/* Launch-shape autotuner: runs prog at every (block_threads, waves_per_simd)
- * candidate, `reps` timed runs each after a warm-up, and returns the fastest
- * in *best (results do not depend on the launch shape). */
+/* Autotuner: runs prog at every (block_threads, waves_per_simd) candidate,
+ * `reps` timed runs each after a warm-up, and returns the fastest in *best.
+ * For the blocked schedule on a device-resident graph it first picks the
+ * block size -- uniform tables of 2^12 .. 2^24 vertices per block, at most
+ * 2048 blocks -- and leaves the fastest table installed (gw_graph_info shows
+ * it).  Results depend on neither. */
 int gw_autotune(gw_ctx *ctx, const gw_program *prog, uint32_t reps, gw_program *best, double *best_ms);
 
 /* Stable counting sort of the loaded walks by block of their vertex (out of

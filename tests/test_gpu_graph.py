@@ -100,3 +100,22 @@ def test_autotune_keeps_results(engine, g12):
     assert ms > 0 and best.block_threads in (64, 128, 256) and best.waves_per_simd in (2, 4, 6, 8, 12, 16)
     engine.run(best)
     _same(_state(engine, prog.recorder), ref)
+
+
+def test_autotune_picks_block_size_for_blocked_runs(engine):
+    # the blocked schedule's block table is tuned too and left installed; the
+    # results are those of the eager run with any table
+    off, col = O.rmat_graph(16, 1 << 20, seed=3)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.graph_blocks([0, 30000, n])
+    engine.walks_init_sources(O.sources(5, n, 64), 200)
+    prog = gw.program(gw.RESTART, 10, 0.15, n, seed=12, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
+                      schedule="blocked", layout="ext")
+    engine.run(prog)
+    ref = _state(engine, prog.recorder)
+    best, ms = engine.autotune(prog, reps=1)
+    nb = engine.graph_info()["nblocks"]
+    assert ms > 0 and nb in (16, 4, 1)  # 2^12, 2^14 or 2^16 vertices per block
+    engine.run(best)
+    _same(_state(engine, prog.recorder), ref)
