@@ -45,12 +45,14 @@ WORKLOADS = {
     "rawwalks-rmat20": dict(scale=20, ef=16, seed=2, kind=gw.JUMP, L=10, p=0.0, recorder=gw.REC_NONE,
                             block_vertices=1 << 16, sources=0, wps=0,
                             desc="R-MAT scale 20 ef 16 seed 2, 1 walk/vertex, L=10, jump p=0 (reduced)"),
+    # PPR's output is the per-source counts (msppr keeps no walk records either): final walk records
+    # are not written (GW_REC_NO_FINAL), for the plain-kernel comparison as well
     "ppr4096-rmat26": dict(scale=26, ef=16, seed=3, kind=gw.STOP, L=20, p=0.15, recorder=gw.REC_SOURCE_COUNT,
-                           block_vertices=1 << 22, sources=4096, wps=2000,
+                           flags=gw.REC_NO_FINAL, block_vertices=1 << 22, sources=4096, wps=2000,
                            desc="R-MAT scale 26 ef 16 seed 3, 4096 sources x 2000 walks, stop 0.15, cap 20, "
-                                "sparse per-source counts"),
+                                "sparse per-source counts (no final walk records)"),
     "ppr256-rmat22": dict(scale=22, ef=16, seed=3, kind=gw.STOP, L=20, p=0.15, recorder=gw.REC_SOURCE_COUNT,
-                          block_vertices=1 << 20, sources=256, wps=2000,
+                          flags=gw.REC_NO_FINAL, block_vertices=1 << 20, sources=256, wps=2000,
                           desc="R-MAT scale 22 ef 16 seed 3, 256 sources x 2000 walks (reduced)"),
 }
 
@@ -86,8 +88,8 @@ def setup(eng: gw.Engine, wl: dict) -> int:
 
 def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: int, source_method=gw.SRC_SORT,
           shape=None):
-    prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
-                      layout=layout, source_method=source_method)
+    prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"] | wl.get("flags", 0),
+                      schedule=sched, layout=layout, source_method=source_method)
     if shape:
         prog.block_threads, prog.waves_per_simd = shape
     for _ in range(W):
@@ -204,7 +206,8 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
         tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "run_ms": None}
     else:
         # launch shape (block size x waves per SIMD) picked by gw_autotune before the timed passes
-        base = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"], schedule=sched,
+        base = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"] | wl.get("flags", 0),
+                          schedule=sched,
                           layout=layout)
         best, best_ms = eng.autotune(base, reps=2)
         shape = (best.block_threads, best.waves_per_simd)
@@ -220,7 +223,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
                                       "rounds": last["rounds"], "bucket_ms": last["bucket_ms"]}
             print(f"[bench] {name} {s_}-{l_}: {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
         # the blocked schedule with its block size and launch shape picked by gw_autotune
-        bprog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"],
+        bprog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"] | wl.get("flags", 0),
                            schedule="blocked", layout="ext")
         bbest, _ = eng.autotune(bprog, reps=1)
         nb = eng.graph_info()["nblocks"]

@@ -24,6 +24,7 @@ JUMP, STOP, PROB, RESTART_JOINT, RESTART = 0, 1, 2, 3, 4
 KIND_NAMES = {"jump": JUMP, "stop": STOP, "prob": PROB, "restart_joint": RESTART_JOINT, "restart": RESTART}
 # recorders
 REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY, REC_EVENTS, REC_SOURCE_COUNT = 0, 1, 2, 4, 8
+REC_NO_FINAL = 16  # skip the final walk records (flag, with any recorder)
 # schedules
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
 SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}

@@ -440,6 +440,7 @@ int gw_walks_read(gw_ctx* ctx, gw_walk* out, uint64_t n) {
         return GW_OK;
     }
     if (n > ctx->nwalks) return gw_fail(ctx, GW_EINVAL, "gw_walks_read: more records requested than loaded");
+    if (!ctx->finals_valid) return gw_fail(ctx, GW_ESTATE, "gw_walks_read: the last run kept no final records (GW_REC_NO_FINAL)");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (n) GW_HIP(ctx, hipMemcpy(out, ctx->d_final, n * sizeof(gw_walk), hipMemcpyDeviceToHost));
@@ -464,7 +465,7 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");
     if (prog->kind == GW_JUMP && prog->nvertices == 0) return gw_fail(ctx, GW_EINVAL, "gw_run: JUMP needs nvertices");
     if (prog->kind == GW_JUMP && prog->nvertices > ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_run: nvertices above graph size");
-    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS | GW_REC_SOURCE_COUNT))
+    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS | GW_REC_SOURCE_COUNT | GW_REC_NO_FINAL))
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
     if ((prog->recorder & GW_REC_SOURCE_COUNT) && (prog->recorder & (GW_REC_VERTEX_COUNT | GW_REC_EVENTS)))
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_SOURCE_COUNT combines with GW_REC_WALK_SUMMARY only");

@@ -116,6 +116,7 @@ struct gw_ctx {
     gw_walk* d_walks = nullptr;
     gw_walk* d_walks_alt = nullptr;
     gw_walk* d_final = nullptr;      // final record of each walk, indexed by walk id
+    bool finals_valid = true;        // false after a GW_REC_NO_FINAL run
     gw_walk* d_work = nullptr;       // blocked schedule: second bucket buffer
 
     // recorder outputs

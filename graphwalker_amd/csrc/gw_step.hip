@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
         rec.id = o;
         rec.source = w.source;
         advance<KIND>(p, g, w.id, w.source, w.vertex, w.hop, 0u, g.n, rec);
-        final_walks[o] = w;
+        if (final_walks) final_walks[o] = w;
         if (SUM) {
             hash[o] = rec.hash;
             visits[o] = rec.visits;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
         if (__ballot(active) == 0) break;
         if (active) {
             if (step_once<KIND, EXT>(p, g, s, rec)) {
-                final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
+                if (final_walks) final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
                 if (SUM) {
                     hash[s.id - p.id_base] = rec.hash;
                     visits[s.id - p.id_base] = rec.visits;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 left = true;
             }
             if (done || left) {
-                if (done) final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
+                if (done && final_walks) final_walks[s.id - p.id_base] = gw_walk{s.id, s.source, s.v, s.hop};
                 walks[slot] = gw_walk{s.id, s.source, done ? kDead : s.v, s.hop};
                 if (SUM) {
                     hash[s.id - p.id_base] = rec.hash;
@@ -336,6 +336,7 @@ struct Launch {
     gw_walk* bucket;
     uint64_t bucket_n;
     const BlockView* views;  // non-null: host-resident graph, step through the block cache
+    bool no_final;           // GW_REC_NO_FINAL: the kernels get no final-record array
 };
 
 template <int KIND, int REC, bool EXT>
@@ -344,17 +345,17 @@ hipError_t launch4(const Launch& L) {
     if (L.sched == GW_SCHED_PLAIN) {
         const uint32_t grid = static_cast<uint32_t>((c->nwalks + L.tb - 1) / L.tb);
         step_plain_kernel<KIND, REC><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
+            L.p, L.g, c->d_walks, L.no_final ? nullptr : c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else if (L.sched == GW_SCHED_PERSISTENT) {
         step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_walks, c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
+            L.p, L.g, c->d_walks, L.no_final ? nullptr : c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else if (L.views) {
         step_block_kernel<KIND, REC, false, true><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_block_first, c->nblocks, L.views, L.bucket, L.bucket_n, c->d_final, c->d_counts,
+            L.p, L.g, c->d_block_first, c->nblocks, L.views, L.bucket, L.bucket_n, L.no_final ? nullptr : c->d_final, c->d_counts,
             c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else {
         step_block_kernel<KIND, REC, EXT, false><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
-            L.p, L.g, c->d_block_first, c->nblocks, nullptr, L.bucket, L.bucket_n, c->d_final, c->d_counts,
+            L.p, L.g, c->d_block_first, c->nblocks, nullptr, L.bucket, L.bucket_n, L.no_final ? nullptr : c->d_final, c->d_counts,
             c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     }
     return hipGetLastError();
@@ -808,7 +809,7 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
             return gw_fail(ctx, GW_ENOMEM, "gw_run: successor-header layout does not fit (HBM budget or degree >= 2^32)");
     }
     st.layout = ext ? GW_LAYOUT_EXT : GW_LAYOUT_CSR;
-    uint32_t rec = prog->recorder;
+    uint32_t rec = prog->recorder & ~GW_REC_NO_FINAL;
     if (rec & GW_REC_EVENTS) rec |= GW_REC_WALK_SUMMARY;
     if ((rec & GW_REC_EVENTS) && ctx->events_cap != prog->event_cap) {
         gw_free(ctx, ctx->d_events, ctx->walk_cap * ctx->events_cap * 8);
@@ -855,6 +856,7 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
                           ctx->table_vbits, ctx->table_cbits, ctx->table_wide ? 1u : 0u};
     L.klog = KeyLog{ctx->d_keys, ctx->d_counters + kCntLogCursor, ctx->d_counters + kCntLogOverflow, ctx->keys_cap, ctx->keys_sentinel,
                     ctx->table_vbits, keylog_slab};
+    L.no_final = (prog->recorder & GW_REC_NO_FINAL) != 0;
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");
     const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
@@ -960,7 +962,7 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             ctx->table_wide, p(ctx->d_keys), ctx->keys_cap, ctx->keys_sentinel, p(ctx->d_part_keys),
             p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), ctx->part_fields,
             ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
-            rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count};
+            rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final};
 }
 
 // Host walk pool: the run steps the pool window by window through two device
@@ -1111,7 +1113,7 @@ int run_pool(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     st.pool_bytes = bytes;
     st.source_method = (rec & GW_REC_SOURCE_COUNT) ? GW_SRC_HASH : 0u;
     ctx->src_method = GW_SRC_HASH;
-    P.final_valid = P.keep_final;
+    P.final_valid = P.keep_final && !rp.L.no_final;
     P.summary_valid = sum;
     P.events_valid = ecap != 0;
     if (stats) *stats = st;
@@ -1158,6 +1160,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         reduce_ms = ms;
     }
     if ((rc = finish_run(ctx, rp, cnt, st, reduce_ms))) return rc;
+    ctx->finals_valid = !rp.L.no_final;
     if (stats) *stats = st;
     return GW_OK;
 }
@@ -1233,6 +1236,7 @@ int gw_run_replay(gw_ctx* ctx, gw_stats* stats) {
     st.launches = ctx->nwalks ? 1 : 0;
     st.rounds = 1;
     if ((rc = finish_run(ctx, g->plan, g->h_cnt, st, 0.0))) return rc;
+    ctx->finals_valid = !g->plan.L.no_final;
     if (stats) *stats = st;
     return GW_OK;
 }

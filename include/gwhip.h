@@ -100,6 +100,8 @@ typedef struct gw_ctx gw_ctx;
                                   on the host (randomwalk.hpp:32); implies GW_REC_WALK_SUMMARY */
 #define GW_REC_SOURCE_COUNT 8u /* sparse u32 visit counts per (source field, vertex): the per-source
                                   visit distributions of msppr (msppr.cpp:71-76), exact */
+#define GW_REC_NO_FINAL 16u    /* skip writing each walk's final record (the reference keeps none);
+                                  gw_walks_read then refuses until a run writes them again */
 
 /* schedules */
 #define GW_SCHED_AUTO 0u

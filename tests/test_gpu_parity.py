@@ -401,3 +401,25 @@ def test_counts_topk_invalid(engine):
     engine.graph_set(off, col)
     with pytest.raises(gw.GwError, match="bad arguments"):
         engine.counts_topk(0)
+
+
+@pytest.mark.parametrize("sched,layout", VARIANTS)
+def test_no_final_keeps_recorders(engine, small_graph, sched, layout):
+    # GW_REC_NO_FINAL drops only the final records: counts and summaries are unchanged
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    engine.walks_init_sources(O.sources(2, n, 30), 300)
+    rec = gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY
+    prog = gw.program(gw.STOP, 12, 0.15, n, seed=4, recorder=rec, schedule=sched, layout=layout)
+    st = engine.run(prog)
+    counts, (h, v), fin = engine.counts().copy(), engine.summary(), engine.walks_final()
+    prog_nf = gw.program(gw.STOP, 12, 0.15, n, seed=4, recorder=rec | gw.REC_NO_FINAL, schedule=sched, layout=layout)
+    st2 = engine.run(prog_nf)
+    assert st2["steps"] == st["steps"] and np.array_equal(engine.counts(), counts)
+    h2, v2 = engine.summary()
+    assert np.array_equal(h2, h) and np.array_equal(v2, v)
+    with pytest.raises(gw.GwError, match="no final records"):
+        engine.walks_final()
+    engine.run(prog)
+    assert np.array_equal(engine.walks_final(), fin)
