@@ -423,3 +423,46 @@ def test_no_final_keeps_recorders(engine, small_graph, sched, layout):
         engine.walks_final()
     engine.run(prog)
     assert np.array_equal(engine.walks_final(), fin)
+
+
+def _csr_from_edges(n, edges):
+    src = np.array([e[0] for e in edges], dtype=np.int64)
+    dst = np.array([e[1] for e in edges], dtype=np.uint32)
+    order = np.lexsort((dst, src))
+    off = np.zeros(n + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(np.bincount(src, minlength=n))
+    return off, dst[order].astype(np.uint32)
+
+
+UNUSUAL = {
+    "one-vertex-self-loop": (1, [(0, 0)]),
+    "star-1001": (1001, [(0, v) for v in range(1, 1001)] + [(v, 0) for v in range(1, 1001)]),
+    "dups-and-loops-37": (37, [(v, (v * 7) % 37) for v in range(37)] * 3 + [(v, v) for v in range(0, 37, 4)]
+                          + [(5, 6)] * 20),
+}
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=["-".join(v) for v in VARIANTS])
+@pytest.mark.parametrize("name", sorted(UNUSUAL))
+def test_unusual_graphs_all_kinds(engine, name, variant):
+    # a single vertex with a self-loop, a high-degree star with an odd vertex
+    # count, duplicate edges and self-loops: every kind equals the oracle
+    n, edges = UNUSUAL[name]
+    off, col = _csr_from_edges(n, edges)
+    engine.graph_set(off, col)
+    if n > 2:
+        engine.graph_blocks([0, n // 3, n])
+    v = np.repeat(np.arange(n, dtype=np.uint32), 3)
+    for kind in KINDS:
+        hops = (np.arange(v.shape[0]) % 3) * 7 if kind == gw.RESTART_JOINT else np.zeros(v.shape[0], np.uint32)
+        walks = O.make_walks(v, sources_=v, hops=hops)
+        engine.walks_load(walks)
+        prog = gw.program(kind, 7, 0.15, n, seed=kind + 40, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
+                          schedule=variant[0], layout=variant[1])
+        st = engine.run(prog)
+        ref = O.replay(oprog(prog), off, col, walks, counts=True, per_walk=True)
+        assert (st["steps"], st["visits"]) == (ref["steps"], ref["visits"]), (name, kind)
+        assert np.array_equal(engine.counts(), ref["counts"]), (name, kind)
+        assert np.array_equal(engine.walks_final(), ref["final"]), (name, kind)
+        h, vis = engine.summary()
+        assert np.array_equal(h, ref["hash"]) and np.array_equal(vis, ref["walk_visits"]), (name, kind)
