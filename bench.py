@@ -257,7 +257,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     return res
 
 
-def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5, hops: int = 5,
+def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5, hops: int = 10,
               walks_lo: int = 1000, walks_hi: int = 100_000, topk: int = 100) -> dict:
     """Poisson arrivals of restart-PPR queries (p = 0.15, 10 hops) at `qps`,
     walk counts log-uniform in [walks_lo, walks_hi], uniform sources, served
@@ -437,6 +437,8 @@ def main() -> int:
     ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-topk", type=int, default=100, help="vertices returned per query (0 = every pair)")
+    ap.add_argument("--serve-hops", type=int, default=10,
+                    help="hops per walk per serve step (10 = the program's L: a walk ends within one step)")
     ap.add_argument("--stream-cache-mib", type=int, default=1024,
                     help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
     ap.add_argument("--stream-queries", type=int, default=256)
@@ -525,7 +527,7 @@ def main() -> int:
         # continuous batching on the graph left resident by the last workload
         n_last = 1 << (WORKLOADS[args.secondary]["scale"] if workloads else wl["scale"])
         for q in args.serve_qps:
-            r = serve_sim(eng, n_last, q, args.serve_seconds, topk=args.serve_topk)
+            r = serve_sim(eng, n_last, q, args.serve_seconds, hops=args.serve_hops, topk=args.serve_topk)
             serving[f"qps_{int(q)}"] = r
             print(f"[bench] serving {q:.0f} qps: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
                   f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
