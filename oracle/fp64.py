@@ -16,6 +16,12 @@ programs run:
 H is L for the capped programs and L-1 per period for RESTART_JOINT (the walk
 records hop offsets 0..L-2 and stops when hop % L == L-1,
 randomwalkwithrestartwithjoint.hpp:52-54).
+
+first_meeting() gives SimRank's per-offset first-meeting probability of two
+independent RESTART_JOINT walks (simrank.cpp:71-84) from the product chain
+over vertex pairs.  Pins: closed forms on a two-leaves-and-a-hub graph, and
+the bit-exact replay's first meetings within the Monte-Carlo bound, tight
+enough that a chain without the restart coin fails it.
 """
 from __future__ import annotations
 
@@ -102,3 +108,32 @@ def l1_normalised(counts: np.ndarray, expected: np.ndarray) -> float:
     c = np.asarray(counts, dtype=np.float64)
     e = np.asarray(expected, dtype=np.float64)
     return float(np.abs(c / c.sum() - e / e.sum()).sum())
+
+
+def first_meeting(off, col, a: int, b: int, p: float, P: int) -> np.ndarray:
+    """Probability that a walk from ``a`` and an independent walk from ``b``,
+    both running the restart-with-joint chain over this CSR (follow a uniform
+    CSR slot with probability 1-p where the vertex has edges, otherwise return
+    to the walk's own start), stand on one vertex at offset t for the first
+    time, t = 0..P-1: the expectation of simrank.cpp:71-84's first-meeting
+    indicator per walk pair.  Exact product chain over vertex pairs, met pairs
+    absorbed (dense n x n: small graphs only)."""
+    Pm, _ = transition(off, col)
+    T = Pm.toarray()
+    n = T.shape[0]
+
+    def kernel(src: int) -> np.ndarray:
+        K = (1.0 - p) * T
+        K[:, src] += 1.0 - K.sum(axis=1)  # the coin or a dead end: back to the start
+        return K
+
+    KA, KB = kernel(a), kernel(b)
+    J = np.zeros((n, n))
+    J[a, b] = 1.0
+    out = np.zeros(P)
+    for t in range(P):
+        if t:
+            J = KA.T @ J @ KB
+        out[t] = np.trace(J)
+        np.fill_diagonal(J, 0.0)
+    return out

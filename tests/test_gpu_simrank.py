@@ -128,3 +128,42 @@ def test_meeting_counts_sparse_join_edges(engine, R, same):
         assert np.array_equal(got[k], _first_meetings(A, B, L)), k
     if same:
         assert got[0, 0] == R * R
+
+
+@pytest.mark.parametrize("R,copies", [(512, 64), (2048, 8)])
+def test_meeting_rates_match_fp64_chain(engine, R, copies):
+    # distributional parity: first-meeting counts / R^2 against the exact
+    # pair chain of two independent restart walks; R <= 1024 takes the sparse
+    # join, R > 1024 the brute-force one.  Each copy of a pair is an
+    # independent estimate; a copy's count sums R^2 indicators that share
+    # walks, so its variance is at most f/R^2 + 2f/R.
+    from oracle import fp64
+    off, col = O.rmat_graph(6, 1 << 9, seed=2)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    L = 9
+    pairs = [(3, 9), (0, 1), (5, 5)]
+    a = np.repeat(np.array([p[0] for p in pairs], np.uint32), copies)
+    b = np.repeat(np.array([p[1] for p in pairs], np.uint32), copies)
+    engine.walks_init_pairs(a, b, R, L)
+    prog = gw.program(gw.RESTART_JOINT, L, 0.15, n, seed=21, recorder=gw.REC_EVENTS, event_cap=L - 1)
+    engine.run(prog)
+    got = engine.meeting_counts(a.shape[0], R, L).astype(np.float64) / (R * R)
+    p = prog.stop_thr / 2.0 ** 32  # the coin restarts when the draw falls below stop_thr
+    for k, (x, y) in enumerate(pairs):
+        f = fp64.first_meeting(off, col, x, y, p, L - 1)
+        est = got[k * copies:(k + 1) * copies, :L - 1].mean(axis=0)
+        tol = 5.0 * np.sqrt((f / (R * R) + 2.0 * f / R) / copies) + 1e-9
+        assert np.all(np.abs(est - f) <= tol), (x, y, est, f, tol)
+        assert np.all(got[k * copies:(k + 1) * copies, L - 1] == 0)
+        if x == y:
+            assert np.all(est[1:] == 0.0) and est[0] == 1.0
+    # the estimate of SimRank itself, sum_t c^t f_t, within the summed bound
+    c = 0.8  # simrank.cpp:77 weighs a first meeting at offset l by 0.8^l
+    for k, (x, y) in enumerate(pairs[:2]):
+        f = fp64.first_meeting(off, col, x, y, p, L - 1)
+        w = c ** np.arange(L - 1)
+        est = (got[k * copies:(k + 1) * copies, :L - 1] @ w).mean()
+        tol = 5.0 * float(w @ np.sqrt((f / (R * R) + 2.0 * f / R) / copies)) + 1e-9
+        print(f"[simrank fp64] pair ({x},{y}) R={R}: estimate {est:.5f}, exact {float(w @ f):.5f}, bound {tol:.5f}")
+        assert abs(est - float(w @ f)) <= tol

@@ -367,3 +367,57 @@ def test_reference_engine_reads_written_graph(tmp_path):
     assert r["00_runtime"] > 0 and r["5_exec_updates"] > 0
     blocks = np.loadtxt(base + "_GraphWalker/blocksize_8KB.blockrange", dtype=np.int64)
     assert blocks[0] == 0 and blocks[-1] == off.shape[0] - 1 and np.all(np.diff(blocks) > 0)
+
+
+def test_fp64_first_meeting_closed_forms():
+    # leaves 0 and 1 each point at hub 2, which loops on itself: with q = 1-p,
+    # first meetings at offsets 0, 1, 2 are 0, q^2 and q^2 (1 - q^2); a == b
+    # meets at offset 0 for sure
+    off = np.array([0, 1, 2, 3], dtype=np.uint64)
+    col = np.array([2, 2, 2], dtype=np.uint32)
+    p = 0.15
+    q = 1 - p
+    f = fp64.first_meeting(off, col, 0, 1, p, 3)
+    assert np.allclose(f, [0.0, q * q, q * q * (1 - q * q)], rtol=0, atol=1e-15)
+    g = fp64.first_meeting(off, col, 1, 1, p, 4)
+    assert g[0] == 1.0 and np.all(g[1:] == 0.0)
+    # on any graph the first-meeting probabilities are a sub-distribution
+    roff, rcol = O.rmat_graph(6, 1 << 9, seed=2)
+    h = fp64.first_meeting(roff, rcol, 3, 9, p, 12)
+    assert np.all(h >= 0) and h.sum() <= 1.0 + 1e-12
+
+
+def test_fp64_first_meeting_matches_replayed_walks():
+    # the bit-exact replay's first meetings (walk i of a side at hop i*L,
+    # ids as gw_walks_init_pairs lays them out) agree with the chain within
+    # the Monte-Carlo bound of copies independent R x R estimates
+    roff, rcol = O.rmat_graph(6, 1 << 9, seed=2)
+    n = roff.shape[0] - 1
+    L, R, copies = 7, 100, 100
+    prog = O.Program(O.RESTART_JOINT, L, int(0.15 * 2 ** 32), n, 0, 5)
+    p = prog.stop_thr / 2.0 ** 32
+
+    def paths(start, id0):
+        out = np.full((R, L - 1), -1, dtype=np.int64)
+        for i in range(R):
+            w = O.make_walks([start], sources_=[start], hops=[i * L], ids=[id0 + i])[0]
+            tv, th, _ = O.trace(prog, roff, rcol, w)
+            out[i, (th - i * L).astype(np.int64)] = tv
+        return out
+
+    wrong = []
+    for x, y in [(3, 9), (0, 1)]:
+        f = fp64.first_meeting(roff, rcol, x, y, p, L - 1)
+        est = np.zeros(L - 1)
+        for c in range(copies):
+            A, B = paths(x, 2 * c * R), paths(y, 2 * c * R + R)
+            eq = A[:, None, :] == B[None, :, :]
+            hit = eq.any(axis=2)
+            np.add.at(est, eq.argmax(axis=2)[hit], 1.0)
+        est /= copies * R * R
+        tol = 5.0 * np.sqrt((f / (R * R) + 2.0 * f / R) / copies) + 1e-9
+        assert np.all(np.abs(est - f) <= tol), (x, y, est, f, tol)
+        assert f[1:].sum() > 0.05  # the pair meets often enough to test something
+        wrong.append(np.any(np.abs(fp64.first_meeting(roff, rcol, x, y, 0.0, L - 1) - est) > tol))
+    # the bound is tight enough that a chain without its restart coin fails
+    assert any(wrong)
