@@ -274,30 +274,65 @@ def write_reference_graph(base: str, off: np.ndarray, col: np.ndarray) -> None:
         f.write(f"{n}\n")
 
 
-def run_reference_rawrandomwalks(base: str, n: int, R: int, L: int, threads: int, blocksize_kb: int,
-                                 nmblocks: int, timeout: float) -> dict:
-    """Runs the reference rawrandomwalks app (R walks from uniform sources,
-    L jump steps each) on a graph written by write_reference_graph, in the
-    graph's directory, and returns its own metrics (metrics.hpp timers):
-    00_runtime (the run after walk placement) and 5_exec_updates (the step
-    phase), in seconds."""
-    app = reference_app("rawrandomwalks")
+def _run_reference(app_name: str, base: str, options: dict, timeout: float) -> str:
+    """Runs an app of oracle/_ref/ on a graph written by write_reference_graph,
+    in the graph's directory, with the given command-line options; returns
+    its stdout."""
+    app = reference_app(app_name)
     if app is None:
-        raise FileNotFoundError("oracle/_ref/rawrandomwalks is not built")
+        raise FileNotFoundError(f"oracle/_ref/{app_name} is not built")
     cwd = os.path.dirname(base)
     # the reference's option parser insists on a config file (configfile.hpp:81-90);
     # every option here comes from the command line
     os.makedirs(os.path.join(cwd, "conf"), exist_ok=True)
     with open(os.path.join(cwd, "conf", "graphwalker.local.cnf"), "w") as f:
         f.write("# options are given on the command line\n")
-    args = [app, "file", base, "N", str(n), "R", str(R), "L", str(L), "execthreads", str(threads),
-            "blocksize_kb", str(blocksize_kb), "nmblocks", str(nmblocks), "prob", "0.2"]
-    subprocess.run(args, cwd=cwd, check=True, timeout=timeout, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    args = [app, "file", base]
+    for k, v in options.items():
+        args += [k, str(v)]
+    r = subprocess.run(args, cwd=cwd, check=True, timeout=timeout, stdout=subprocess.PIPE,
+                       stderr=subprocess.DEVNULL, text=True)
+    return r.stdout
+
+
+def run_reference_rawrandomwalks(base: str, n: int, R: int, L: int, threads: int, blocksize_kb: int,
+                                 nmblocks: int, timeout: float) -> dict:
+    """Runs the reference rawrandomwalks app (R walks from uniform sources,
+    L jump steps each) and returns its own metrics (metrics.hpp timers):
+    00_runtime (the run after walk placement) and 5_exec_updates (the step
+    phase), in seconds."""
+    _run_reference("rawrandomwalks", base, {"N": n, "R": R, "L": L, "execthreads": threads,
+                                            "blocksize_kb": blocksize_kb, "nmblocks": nmblocks, "prob": 0.2},
+                   timeout)
     out = {}
-    with open(os.path.join(cwd, "graphwalker_metrics.txt")) as f:
+    with open(os.path.join(os.path.dirname(base), "graphwalker_metrics.txt")) as f:
         for line in f:
             key, _, val = line.strip().partition("=")
             for k in ("00_runtime", "5_exec_updates", "0_startWalks"):
                 if key.endswith("." + k):
                     out[k] = float(val)
     return out
+
+
+def run_reference_visit_counts(base: str, n: int, source: int, walks: int, L: int, threads: int,
+                               timeout: float) -> np.ndarray:
+    """The reference engine's single-source PPR visit vector: `walks` walks of
+    its RandomWalkwithStop program (stop 0.15, cap L) from `source`, counted
+    per vertex by oracle/ref_visit_counts.cpp (one block, all in memory)."""
+    out = base + ".visits"
+    _run_reference("visit_counts", base, {"N": n, "source": source, "walks": walks, "L": L, "out": out,
+                                          "execthreads": threads, "blocksize_kb": 1 << 20, "nmblocks": 1},
+                   timeout)
+    return np.fromfile(out, dtype=np.uint32)
+
+
+def run_reference_simrank(base: str, a: int, b: int, R: int, L: int, threads: int, timeout: float) -> float:
+    """The reference simrank app's score for (a, b): R walks of its
+    RandomWalkwithRestartwithJoint program from each vertex, first meetings
+    weighed by 0.8^offset (simrank.cpp:71-84), one block all in memory."""
+    text = _run_reference("simrank", base, {"a": a, "b": b, "R": R, "L": L, "execthreads": threads,
+                                            "blocksize_kb": 1 << 20, "nmblocks": 1, "prob": 0.2}, timeout)
+    for line in text.splitlines():
+        if line.startswith(f"SimRank for {a} and {b} = "):
+            return float(line.rsplit("=", 1)[1])
+    raise RuntimeError("the reference simrank app printed no score")

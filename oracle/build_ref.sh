@@ -19,6 +19,12 @@ for app in rawrandomwalks msppr simrank graphlet rwdomination kk_ppr; do
         g++ -O3 -fopenmp -w -I "$REF/src" "$src" -o "$OUT/$app" -lz
     fi
 done
+# A test-infrastructure app of this repository on the reference API: the
+# reference engine's single-source PPR visit vector (oracle/ref_visit_counts.cpp).
+HERE="$(cd "$(dirname "$0")" && pwd)"
+if [ ! -x "$OUT/visit_counts" ] || [ "$HERE/ref_visit_counts.cpp" -nt "$OUT/visit_counts" ]; then
+    g++ -O3 -fopenmp -w -I "$REF/src" "$HERE/ref_visit_counts.cpp" -o "$OUT/visit_counts" -lz
+fi
 echo "build_ref: built $(ls "$OUT" | tr '\n' ' ')"
 
 # The same reference apps compiled, unchanged, against this repository's

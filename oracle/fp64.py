@@ -22,6 +22,9 @@ independent RESTART_JOINT walks (simrank.cpp:71-84) from the product chain
 over vertex pairs.  Pins: closed forms on a two-leaves-and-a-hub graph, and
 the bit-exact replay's first meetings within the Monte-Carlo bound, tight
 enough that a chain without the restart coin fails it.
+
+expected_visits_sources() is expected_visits for many sources in one dense
+propagation; pinned by equality with expected_visits source by source.
 """
 from __future__ import annotations
 
@@ -71,6 +74,36 @@ def expected_visits(kind: int, off, col, source: int, walks: float, L: int, p: f
         if kind == PROB and x.sum() < 1e-12 * walks:
             break
     return acc
+
+
+def expected_visits_sources(kind: int, off, col, sources, walks: float, L: int, p: float) -> np.ndarray:
+    """expected_visits for many sources at once: row k holds the expected
+    visits of ``walks`` walks from sources[k] (the same chain, the columns of
+    one dense n x S state propagated together)."""
+    P, dangling = transition(off, col)
+    PT = P.T.tocsr()
+    n = P.shape[0]
+    src = np.asarray(sources, dtype=np.int64)
+    cols = np.arange(src.shape[0])
+    X = np.zeros((n, src.shape[0]))
+    X[src, cols] = float(walks)
+    H = L if kind != RESTART_JOINT else max(L - 1, 1)
+    if kind == PROB:
+        H = 10_000
+    keep = (~dangling).astype(np.float64)[:, None]
+    acc = np.zeros_like(X)
+    for _ in range(H):
+        acc += X
+        nxt = (1.0 - p) * (PT @ (X * keep))
+        lost = X.sum(axis=0) - nxt.sum(axis=0)
+        if kind == JUMP:
+            nxt += lost[None, :] / n
+        elif kind in (RESTART, RESTART_JOINT):
+            np.add.at(nxt, (src, cols), lost)
+        X = nxt
+        if kind == PROB and X.sum() < 1e-12 * walks * src.shape[0]:
+            break
+    return acc.T.copy()
 
 
 def expected_visits_start(kind: int, off, col, start: np.ndarray, L: int, p: float) -> np.ndarray:

@@ -111,6 +111,62 @@ def test_config1_l1_below_001_with_enough_walks(engine):
     assert l1 < 0.01
 
 
+def _per_source(s, v, c, nsrc, n):
+    dense = np.zeros((nsrc, n), dtype=np.uint64)
+    dense[s, v] = c
+    return dense
+
+
+def test_batched_ppr_per_source_fp64(engine):
+    """The batched multi-source PPR path (stop 0.15, cap 20, 2000 walks per
+    source, exact per-(source, vertex) counts by the partitioned method): each
+    source's normalised vector against its own FP64 expectation, within the
+    Monte-Carlo bound of 2000 walks, on the single-source PPR graph."""
+    engine.graph_rmat(16, 1 << 20, 1)
+    off, col = engine.graph_read()
+    n = off.shape[0] - 1
+    S, W, L = 128, 2000, 20
+    src = O.sources(31, n, S)
+    engine.walks_init_sources(src, W, source_field=np.arange(S, dtype=np.uint32))
+    prog = gw.program(gw.STOP, L, 0.15, n, seed=17, recorder=gw.REC_SOURCE_COUNT | gw.REC_NO_FINAL,
+                      source_method=gw.SRC_PARTITIONED)
+    st = engine.run(prog)
+    assert st["source_method"] == gw.SRC_PARTITIONED
+    dense = _per_source(*engine.source_counts(), S, n)
+    assert np.all(dense.sum(axis=1) > 0) and st["visits"] == int(dense.sum())
+    expect = fp64.expected_visits_sources(O.STOP, off, col, src, W, L, 0.15)
+    l1s, bounds = [], []
+    for k in range(S):
+        exp = expect[k]
+        l1s.append(fp64.l1_normalised(dense[k], exp))
+        bounds.append(fp64.mc_l1_bound(exp, W, exp.sum() / W, L))
+    l1s, bounds = np.array(l1s), np.array(bounds)
+    print(f"[ppr-batched] mean L1 {l1s.mean():.4f}, max L1/bound {np.max(l1s / bounds):.3f}")
+    assert np.all(l1s < bounds)
+
+
+def test_batched_ppr_l1_below_001_with_enough_walks(engine):
+    """Two sources that spread (several out-edges) need about 10^7 walks each
+    for a per-source L1 below 0.01 at cap 20; 2000 walks give 0.3-0.6."""
+    engine.graph_rmat(16, 1 << 20, 1)
+    off, col = engine.graph_read()
+    n = off.shape[0] - 1
+    deg = np.diff(off.astype(np.int64))
+    cand = O.sources(3, n, 64)
+    src = cand[deg[cand] >= 4][:2]
+    assert src.shape[0] == 2
+    W, L = 20_000_000, 20
+    engine.walks_init_sources(src, W, source_field=np.arange(2, dtype=np.uint32))
+    st = engine.run(gw.program(gw.STOP, L, 0.15, n, seed=23, recorder=gw.REC_SOURCE_COUNT | gw.REC_NO_FINAL))
+    dense = _per_source(*engine.source_counts(), 2, n)
+    assert st["visits"] == int(dense.sum())
+    for k in range(2):
+        exp = fp64.expected_visits(O.STOP, off, col, int(src[k]), W, L, 0.15)
+        l1 = fp64.l1_normalised(dense[k], exp)
+        print(f"[ppr-20M] source {src[k]} (out-degree {deg[src[k]]}): L1 = {l1:.5f}")
+        assert l1 < 0.01
+
+
 @pytest.mark.parametrize("tb,wps", [(64, 1), (128, 3), (256, 8), (192, 5)])
 def test_launch_shape_independence(engine, small_graph, tb, wps):
     off, col = small_graph

@@ -421,3 +421,71 @@ def test_fp64_first_meeting_matches_replayed_walks():
         wrong.append(np.any(np.abs(fp64.first_meeting(roff, rcol, x, y, 0.0, L - 1) - est) > tol))
     # the bound is tight enough that a chain without its restart coin fails
     assert any(wrong)
+
+
+_REF_SKIP = pytest.mark.skipif(O.reference_app("visit_counts") is None or O.reference_app("simrank") is None,
+                               reason="reference apps not built (oracle/build_ref.sh)")
+
+
+@_REF_SKIP
+@pytest.mark.parametrize("W", [100_000, 1_000_000])
+def test_reference_engine_ppr_within_mc_bound(tmp_path, W):
+    # both implementations against FP64 on the single-source PPR graph (R-MAT
+    # 2^16 / 2^20, seed 1, source 0, stop 0.15, cap 10): the reference CPU
+    # engine (its own RandomWalkwithStop and rand_r) and the Philox replay the
+    # device matches bit for bit
+    off, col = O.rmat_graph(16, 1 << 20, seed=1)
+    n = off.shape[0] - 1
+    base = str(tmp_path / "g")
+    O.write_reference_graph(base, off, col)
+    exp = fp64.expected_visits(O.STOP, off, col, 0, W, 10, 0.15)
+    bound = fp64.mc_l1_bound(exp, W, exp.sum() / W, 10)
+    ref = O.run_reference_visit_counts(base, n, 0, W, 10, 8, timeout=120)
+    assert ref.shape == (n,) and ref[0] >= W  # every walk visits its source first
+    l1_ref = fp64.l1_normalised(ref, exp)
+    prog = O.Program(O.STOP, 10, int(0.15 * 2 ** 32), n, 0, 0x5EED)
+    ours = O.replay(prog, off, col, O.make_walks(np.zeros(W, np.uint32)), counts=True)["counts"]
+    l1_ours = fp64.l1_normalised(ours, exp)
+    assert l1_ref < bound and l1_ours < bound, (l1_ref, l1_ours, bound)
+
+
+@_REF_SKIP
+def test_reference_engine_simrank_within_mc_bound(tmp_path):
+    # the reference simrank app's score (first meetings weighed 0.8^l) against
+    # the FP64 pair chain.  One pair's score averages R^2 walk-pair values in
+    # [0, 1] that share walks: Var <= S/R^2 + 2S/R; over K pairs the mean error
+    # has Var <= mean(S/R^2 + 2S/R) / K
+    off, col = O.rmat_graph(6, 1 << 9, seed=2)
+    base = str(tmp_path / "g")
+    O.write_reference_graph(base, off, col)
+    R, L = 1000, 11
+    w = 0.8 ** np.arange(L - 1)
+    rng = np.random.default_rng(4)
+    pairs = [(int(a), int(b)) for a, b in rng.integers(0, 64, (12, 2)) if a != b]
+    err, var = [], []
+    for a, b in pairs:
+        S = float(w @ fp64.first_meeting(off, col, a, b, 0.15, L - 1))
+        s = O.run_reference_simrank(base, a, b, R, L, 8, timeout=60)
+        v = S / (R * R) + 2.0 * S / R
+        assert abs(s - S) <= 5.0 * np.sqrt(v) + 1e-6, (a, b, s, S)
+        err.append(s - S)
+        var.append(v)
+    assert abs(np.mean(err)) <= 5.0 * np.sqrt(np.mean(var) / len(err))
+    # a == b: every walk carries source a, so the app records all paths on the
+    # a side (simrank.cpp:60-63) and reports 0, not the definition's 1 (the
+    # device join counts R^2 first meetings at offset 0 for such a pair)
+    assert O.run_reference_simrank(base, 5, 5, 100, L, 2, timeout=60) == 0.0
+
+
+def test_fp64_sources_equals_single_source():
+    # the batched FP64 expectation is the single-source one, source by source,
+    # for every kind (duplicate sources and a dead end included)
+    off, col = O.rmat_graph(8, 1 << 11, seed=5)
+    n = off.shape[0] - 1
+    deg = np.diff(off.astype(np.int64))
+    src = np.array([0, 7, 7, int(np.argmin(deg)), n - 1])
+    for kind in (O.JUMP, O.STOP, O.PROB, O.RESTART_JOINT, O.RESTART):
+        many = fp64.expected_visits_sources(kind, off, col, src, 300.0, 9, 0.2)
+        for k, s_ in enumerate(src):
+            one = fp64.expected_visits(kind, off, col, int(s_), 300.0, 9, 0.2)
+            assert np.allclose(many[k], one, rtol=1e-12, atol=1e-9), (kind, k)
