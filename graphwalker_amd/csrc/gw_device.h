@@ -143,6 +143,13 @@ struct KeyLog {
     uint64_t sentinel;
     uint32_t vbits;
     uint32_t slab;
+    // Direct regions (no log): visits of source field f go straight to
+    // region_out[region_off[f] + ...), claimed through region_n[f].
+    const uint64_t* __restrict__ region_off;
+    unsigned long long* __restrict__ region_n;
+    uint32_t* __restrict__ region_out;
+    unsigned long long* __restrict__ region_overflow;
+    uint32_t nf;
 };
 
 // Called by the lanes that visit; `slab` = this wavefront's {next, end} in LDS.
@@ -173,6 +180,100 @@ __device__ __forceinline__ void keylog_add(const KeyLog& k, uint64_t* slab, uint
     }
 }
 
+// Direct regions, staged per wavefront in LDS: two buffers of kRegionBuf
+// vertex ids, each holding visits of one source field.  A full buffer (or
+// one needed for a third field) is flushed to its field's region with one
+// cursor atomic for the whole buffer and coalesced stores, so the step
+// kernel waits on no atomic per visit and leaves no holes in a region.
+constexpr uint32_t kRegionBuf = 256;
+constexpr uint32_t kNoField = 0xFFFFFFFFu;
+struct alignas(16) RegionBuf {
+    uint32_t st[4];  // {field, count} of buffer 0, then of buffer 1: one 16-B LDS read
+    uint32_t key[2][kRegionBuf];
+};
+
+// One lane per wavefront, at kernel start.
+__device__ __forceinline__ void region_buf_init(RegionBuf* rb) {
+    rb->st[0] = rb->st[2] = kNoField;
+    rb->st[1] = rb->st[3] = 0;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Writes the n staged ids of field f to its region; the lanes in `active`
+// take part.  Ids past the region's end (impossible while regions hold every
+// visit a walk can make) count in region_overflow.  The caller records the
+// buffer's new state.
+__device__ __forceinline__ void region_flush(const KeyLog& k, const uint32_t* keys, uint32_t f, uint32_t n,
+                                             uint64_t active) {
+    if (n == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
+    unsigned long long at = 0;
+    if (lane == leader) at = atomicAdd(&k.region_n[f], static_cast<unsigned long long>(n));
+    at = __shfl(at, leader, 64);
+    const uint64_t base = k.region_off[f] + at, hi = k.region_off[f + 1];
+    const uint32_t m = __popcll(active);
+    for (uint32_t i = lanes_below(active); i < n; i += m)
+        if (base + i < hi) k.region_out[base + i] = keys[i];
+    if (lane == leader && base + n > hi)
+        atomicAdd(k.region_overflow, static_cast<unsigned long long>(base + n - (base > hi ? base : hi)));
+    __builtin_amdgcn_wave_barrier();  // every lane has read the ids before they are overwritten
+}
+
+// Called by the lanes that visit: each stages its vertex id in the buffer of
+// its own source field.  The visiting lanes are grouped by field with a
+// ballot loop (a wavefront's walks come from one stretch of the walk array,
+// so they name one or two fields).  A field >= nf counts in region_overflow.
+__device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint32_t source, uint32_t v) {
+    const uint64_t active = __ballot(1);
+    const uint32_t lane = __lane_id();
+    uint64_t todo = active;
+    while (todo) {  // wave-uniform
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+        const uint32_t ls = __shfl(source, leader, 64);
+        const uint64_t grp = __ballot(source == ls);
+        todo &= ~grp;
+        const uint32_t g = __popcll(grp);
+        if (ls >= k.nf) {
+            if (lane == leader) atomicAdd(k.region_overflow, static_cast<unsigned long long>(g));
+            continue;
+        }
+        const uint4 st = *reinterpret_cast<const uint4*>(rb->st);
+        uint32_t b, c;
+        if (st.x == ls) {
+            b = 0;
+            c = st.y;
+        } else if (st.z == ls) {
+            b = 1;
+            c = st.w;
+        } else {  // take an empty buffer, else flush the fuller one
+            b = st.y == 0 ? 0u : st.w == 0 ? 1u : st.y >= st.w ? 0u : 1u;
+            region_flush(k, rb->key[b], b ? st.z : st.x, b ? st.w : st.y, active);
+            c = 0;
+        }
+        if (c + g > kRegionBuf) {
+            region_flush(k, rb->key[b], ls, c, active);
+            c = 0;
+        }
+        if (source == ls) rb->key[b][c + lanes_below(grp)] = v;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == leader) *reinterpret_cast<uint2*>(&rb->st[2 * b]) = make_uint2(ls, c + g);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// At kernel exit (all lanes of the wavefront present): flush both buffers.
+__device__ __forceinline__ void region_close(const KeyLog& k, RegionBuf* rb) {
+    const uint64_t all = __ballot(1);
+    const uint4 st = *reinterpret_cast<const uint4*>(rb->st);
+    region_flush(k, rb->key[0], st.x, st.y, all);
+    region_flush(k, rb->key[1], st.z, st.w, all);
+}
+
 // At kernel exit (all lanes of the wavefront present): pad the open slab.
 __device__ __forceinline__ void keylog_close(const KeyLog& k, uint64_t* slab, uint32_t lane) {
     for (uint64_t i = slab[0] + lane; i < slab[1]; i += 64) k.keys[i] = k.sentinel;
@@ -180,7 +281,8 @@ __device__ __forceinline__ void keylog_close(const KeyLog& k, uint64_t* slab, ui
 
 // Recorder hooks, compile-time selected: the device replacements for the
 // per-step updateInfo callback (randomwalk.hpp:32).
-template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false, bool kKeyLog = false>
+template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false, bool kKeyLog = false,
+          bool kRegion = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
@@ -192,9 +294,11 @@ struct Recorder {
     uint32_t source;
     KeyLog klog;
     uint64_t* slab;
+    RegionBuf* rbuf;
     __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t, const KeyLog& kl,
-                                        uint64_t* sl)
-        : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl) {}
+                                        uint64_t* sl, RegionBuf* rb = nullptr)
+        : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl),
+          rbuf(rb) {}
     __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
         if (kEvents && visits < log.cap)
             log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
@@ -224,6 +328,7 @@ struct Recorder {
         }
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
         if (kKeyLog) keylog_add(klog, slab, source, v);
+        if (kRegion) region_add(klog, rbuf, source, v);
     }
 };
 

@@ -223,13 +223,14 @@ int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms);
 // regions for `per_walk` visits per walk and the pair buffers; *ok = false
 // when the fields or the memory do not allow it.  reset: zero the device
 // counters (and the region cursors when `part`) before a run.  enqueue:
-// partition the key log and aggregate every source, with no host sync.
+// partition the key log (when `partition`; otherwise the step kernel wrote
+// the regions directly) and aggregate every source, with no host sync.
 // finish: given the counters read back after the run, grow-and-repeat or
 // sort-reduce split pairs as needed; *done = false when a region overflowed
 // (the caller falls back to gw_keylog_reduce).  ms = extra device time.
 int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok);
 int gw_partlog_reset(gw_ctx* ctx, bool part);
-int gw_partlog_enqueue(gw_ctx* ctx);
+int gw_partlog_enqueue(gw_ctx* ctx, bool partition);
 int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, double* ms);
 
 // gw_stream.hip: device block cache of a host-resident graph.  prepare

@@ -26,6 +26,7 @@ namespace {
 
 constexpr uint32_t kChunk = 512;           // walk indices a wavefront claims per atomic
 constexpr int kRecKeyLog = 1 << 16;        // internal recorder bit: source counts by sort-reduce
+constexpr int kRecRegion = 1 << 17;        // internal recorder bit: visits straight into per-source regions
 constexpr uint32_t kDead = 0xFFFFFFFFu;    // vertex of a finished walk in the blocked schedule
 
 __device__ __forceinline__ uint32_t lane_id() {
@@ -47,13 +48,16 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
+    constexpr bool RG = (REC & kRecRegion) != 0;
     __shared__ uint64_t s_slab[4][2];
     if (KL && (threadIdx.x & 63u) == 0) {
         s_slab[threadIdx.x >> 6][0] = 0;
         s_slab[threadIdx.x >> 6][1] = 0;
     }
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
-        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
+    __shared__ RegionBuf s_rbuf[RG ? 4 : 1];
+    if (RG && (threadIdx.x & 63u) == 0) region_buf_init(&s_rbuf[threadIdx.x >> 6]);
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6], RG ? &s_rbuf[threadIdx.x >> 6] : nullptr);
     if (i < nw) {
         gw_walk w = walks[i];
         const uint32_t o = w.id - p.id_base;
@@ -67,6 +71,7 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
         }
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], threadIdx.x & 63u);
+    if (RG) region_close(klog, &s_rbuf[threadIdx.x >> 6]);
     const uint32_t st = wave_sum(rec.steps);
     const uint32_t vi = wave_sum(rec.visits);
     if ((threadIdx.x & 63u) == 0) {
@@ -91,13 +96,16 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
+    constexpr bool RG = (REC & kRecRegion) != 0;
     __shared__ uint64_t s_slab[4][2];
     if (KL && (threadIdx.x & 63u) == 0) {
         s_slab[threadIdx.x >> 6][0] = 0;
         s_slab[threadIdx.x >> 6][1] = 0;
     }
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
-        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
+    __shared__ RegionBuf s_rbuf[RG ? 4 : 1];
+    if (RG && (threadIdx.x & 63u) == 0) region_buf_init(&s_rbuf[threadIdx.x >> 6]);
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6], RG ? &s_rbuf[threadIdx.x >> 6] : nullptr);
 
     for (;;) {
         // Refill idle lanes from the wave's chunk; claim a new chunk when dry.
@@ -151,6 +159,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
         }
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
+    if (RG) region_close(klog, &s_rbuf[threadIdx.x >> 6]);
     const uint32_t st = wave_sum(tot_steps);
     const uint32_t vi = wave_sum(tot_visits);
     if (lane == 0) {
@@ -214,13 +223,16 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
+    constexpr bool RG = (REC & kRecRegion) != 0;
     __shared__ uint64_t s_slab[4][2];
     if (KL && (threadIdx.x & 63u) == 0) {
         s_slab[threadIdx.x >> 6][0] = 0;
         s_slab[threadIdx.x >> 6][1] = 0;
     }
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL> rec(
-        counts, log, table, klog, s_slab[threadIdx.x >> 6]);
+    __shared__ RegionBuf s_rbuf[RG ? 4 : 1];
+    if (RG && (threadIdx.x & 63u) == 0) region_buf_init(&s_rbuf[threadIdx.x >> 6]);
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG> rec(
+        counts, log, table, klog, s_slab[threadIdx.x >> 6], RG ? &s_rbuf[threadIdx.x >> 6] : nullptr);
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -292,6 +304,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
         }
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
+    if (RG) region_close(klog, &s_rbuf[threadIdx.x >> 6]);
     const uint32_t st = wave_sum(tot_steps);
     const uint32_t vi = wave_sum(tot_visits);
     if (lane == 0) {
@@ -383,6 +396,8 @@ hipError_t launch2(const Launch& L, uint32_t rec) {
     case SC | SU: return launch3<KIND, SC | SU>(L);
     case kRecKeyLog: return launch3<KIND, kRecKeyLog>(L);
     case kRecKeyLog | SU: return launch3<KIND, kRecKeyLog | SU>(L);
+    case kRecRegion: return launch3<KIND, kRecRegion>(L);
+    case kRecRegion | SU: return launch3<KIND, kRecRegion | SU>(L);
     default: return hipErrorInvalidValue;
     }
 }
@@ -749,6 +764,7 @@ struct RunPlan {
     gw_program prog{};
     uint32_t sched = 0, rec = 0;
     bool ext = false, part = false;
+    bool direct = false;  // partitioned counts written by the step kernel itself (no key log, no partition pass)
     Launch L{};
     gw_stats st0{};  // schedule and layout
 };
@@ -880,6 +896,19 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
         if (rc) return rc;
         if (!rp.part && prog->source_method == GW_SRC_PARTITIONED)
             return gw_fail(ctx, GW_ENOMEM, "gw_run: the partitioned key log does not fit the HBM budget");
+        // A capped walk makes at most L visits, so regions sized walks x L
+        // cannot overflow: the step kernel appends each visit to its source's
+        // region directly.  Uncapped (GW_PROB) walks keep the key log, whose
+        // global sort-reduce takes over when a region overflows.
+        if (rp.part && prog->kind != GW_PROB) {
+            rec = (rec & ~kRecKeyLog) | kRecRegion;
+            rp.direct = true;
+            L.klog.region_off = ctx->d_part_off;
+            L.klog.region_n = ctx->d_part_cursor;
+            L.klog.region_out = ctx->d_part_keys;
+            L.klog.region_overflow = ctx->d_counters + kCntPartOverflow;
+            L.klog.nf = ctx->part_fields;
+        }
     }
     rp.sched = sched;
     rp.rec = rec;
@@ -909,7 +938,7 @@ int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed, bool clear = true) {
         if (e != hipSuccess) return gw_hip_check(ctx, e, "step kernel launch");
     }
     if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-    if (rp.part && (rc = gw_partlog_enqueue(ctx))) return rc;
+    if (rp.part && (rc = gw_partlog_enqueue(ctx, !rp.direct))) return rc;
     if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     return GW_OK;
 }
@@ -922,7 +951,7 @@ int finish_run(gw_ctx* ctx, RunPlan& rp, const unsigned long long* cnt, gw_stats
     if ((rec & GW_REC_SOURCE_COUNT) && cnt[kCntTableOverflow])
         return gw_fail(ctx, GW_ENOMEM, "gw_run: source-count table full (" + std::to_string(cnt[kCntTableOverflow]) +
                                            " visits not recorded); raise table_log2");
-    if (rec & kRecKeyLog) {
+    if (rec & (kRecKeyLog | kRecRegion)) {
         if (cnt[kCntLogOverflow])
             return gw_fail(ctx, GW_ENOMEM, "gw_run: source key log full (walks longer than the log allows)");
         double rms = 0;
@@ -933,6 +962,8 @@ int finish_run(gw_ctx* ctx, RunPlan& rp, const unsigned long long* cnt, gw_stats
             double extra = 0;
             if ((rc = gw_partlog_finish(ctx, cnt, &done, &extra))) return rc;
             rms += extra;
+            if (!done && rp.direct)  // regions hold every visit a capped walk can make
+                return gw_fail(ctx, GW_ESTATE, "gw_run: a source's visits outgrew its region");
             if (!done && rp.prog.source_method == GW_SRC_PARTITIONED)
                 return gw_fail(ctx, GW_ENOMEM, "gw_run: a source's visits outgrew its key-log region (GW_SRC_PARTITIONED)");
         }
@@ -948,7 +979,7 @@ int finish_run(gw_ctx* ctx, RunPlan& rp, const unsigned long long* cnt, gw_stats
         st.total_ms += rms;
         st.reduce_ms = rms;
     }
-    st.source_method = (rec & (GW_REC_SOURCE_COUNT | kRecKeyLog)) ? ctx->src_method : 0u;
+    st.source_method = (rec & (GW_REC_SOURCE_COUNT | kRecKeyLog | kRecRegion)) ? ctx->src_method : 0u;
     return GW_OK;
 }
 
@@ -1142,7 +1173,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
                                : run_blocked(ctx, rp.L, prog, rp.rec, st);
         if (rc) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        if (rp.part && (rc = gw_partlog_enqueue(ctx))) return rc;
+        if (rp.part && (rc = gw_partlog_enqueue(ctx, !rp.direct))) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     } else {
         if ((rc = enqueue_run(ctx, rp, true))) return rc;

@@ -593,8 +593,9 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
         ctx->part_fields = nf;
         ctx->part_total = off[nf];
     }
-    // Split pairs (flushed sources) go to the key log, which the partition has
-    // consumed by then, with their counts here: at most one pair per logged key.
+    // Split pairs (flushed sources) go to the key log's memory -- consumed by
+    // the partition by then, or never written when the step kernel fills the
+    // regions directly -- with their counts here: at most one pair per visit.
     if (ctx->keys_cap > ctx->split_cap) {
         gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
         ctx->d_split_counts = nullptr;
@@ -615,13 +616,15 @@ int gw_partlog_reset(gw_ctx* ctx, bool part) {
     return GW_OK;
 }
 
-int gw_partlog_enqueue(gw_ctx* ctx) {
+int gw_partlog_enqueue(gw_ctx* ctx, bool partition) {
     unsigned long long* c = ctx->d_counters;
     const uint32_t cus = std::max(1u, ctx->cu_count);
-    partition_kernel<<<dim3(cus * 8u), dim3(256), 0, ctx->stream>>>(
-        ctx->d_keys, c + kCntLogCursor, ctx->keys_cap, ctx->keys_sentinel, ctx->table_vbits, ctx->part_fields,
-        ctx->d_part_off, ctx->d_part_cursor, ctx->d_part_keys, c + kCntPartOverflow);
-    GW_HIP(ctx, hipGetLastError());
+    if (partition) {  // else the step kernel filled the regions itself
+        partition_kernel<<<dim3(cus * 8u), dim3(256), 0, ctx->stream>>>(
+            ctx->d_keys, c + kCntLogCursor, ctx->keys_cap, ctx->keys_sentinel, ctx->table_vbits, ctx->part_fields,
+            ctx->d_part_off, ctx->d_part_cursor, ctx->d_part_keys, c + kCntPartOverflow);
+        GW_HIP(ctx, hipGetLastError());
+    }
     const uint32_t grid = std::min(ctx->part_fields, cus);
     part_aggregate_kernel<<<dim3(grid), dim3(kAggThreads), 0, ctx->stream>>>(
         ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->d_part_cursor,
