@@ -63,6 +63,9 @@ struct ServeState {
     uint32_t topk = 0;                         // > 0: completed queries keep their top-k pairs only
     unsigned long long* d_topk = nullptr;      // max_queries x topk packed pairs
     uint32_t* d_topk_n = nullptr;              // max_queries
+    unsigned long long* d_cand = nullptr;      // per-chunk top-k candidates of the completing queries
+    uint64_t cand_cap = 0;                     // keys
+    unsigned long long* d_cand_off = nullptr;  // max_queries: first candidate of each completing query
 };
 
 // The context's serving state (null when no server is open on it).
@@ -108,11 +111,36 @@ __device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// Per-wavefront LDS cache of (query slot, vertex) -> visits.  A restart
+// query's visits concentrate on a few hundred vertices near its source; the
+// cache absorbs their repeats so the arena sees one update per (window,
+// vertex) instead of one per visit.  Flushed to the arena when three
+// quarters full and at kernel end.
+constexpr uint32_t kServeCache = 256;
+constexpr uint32_t kServeCacheFlush = kServeCache * 3 / 4;
+constexpr uint32_t kServeCacheProbes = 4;
+constexpr unsigned long long kCacheEmpty = ~0ull;
+struct ServeCache {
+    unsigned long long key[kServeCache];  // (slot << 32) | vertex
+    uint32_t cnt[kServeCache];
+    uint32_t fill;
+};
+
+// One lane per wavefront, at kernel start.
+__device__ __forceinline__ void serve_cache_init(ServeCache* c, uint32_t lane) {
+    for (uint32_t i = lane; i < kServeCache; i += 64) {
+        c->key[i] = kCacheEmpty;
+        c->cnt[i] = 0;
+    }
+    if (lane == 0) c->fill = 0;
+}
+
 // Per-visit recorder of a served walk.  Visits to the query's own source (a
 // quarter or more of a restart walk's visits) go to a per-query counter, one
-// atomic per (wavefront, query); other vertices go to the query's hash region:
-// one compare-and-swap either claims an empty slot or returns the entry
-// there (then an atomic add if it is this vertex's).  A claimed slot is
+// atomic per (wavefront, query); lanes of one query on one vertex combine;
+// other visits go through the wavefront's LDS cache to the query's hash
+// region: one compare-and-swap either claims an empty slot or returns the
+// entry there (then an atomic add if it is this vertex's).  A claimed slot is
 // appended to the query's occupied list (one atomic per (wavefront, query)),
 // so completion reads the entries without scanning the region.
 struct ServeRec {
@@ -125,7 +153,84 @@ struct ServeRec {
     uint32_t cbits;
     unsigned long long* overflow;
     uint32_t slot, source;
+    const SlotInfo* slots;
+    ServeCache* cache;
     uint32_t visits = 0, steps = 0;
+
+    // Adds `inc` visits of vertex v to query slot `sl` (region at `rb`, `rm`
+    // slots - 1) for the lanes with `want`; every active lane calls it.
+    __device__ __forceinline__ void arena_add(bool want, uint32_t sl, uint64_t rb, uint32_t rm, uint32_t v,
+                                              uint32_t inc) {
+        constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+        uint32_t claimed = kNoSlot;
+        if (want) {
+            const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | inc;
+            uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm;
+            bool placed = false;
+            for (uint32_t probe = 0; probe <= rm; ++probe) {
+                unsigned long long* cell = arena + rb + h;
+                // compare-and-swap first: one request for a new vertex, two for a known one
+                const unsigned long long cur = atomicCAS(cell, 0ull, entry);
+                if (cur == 0ull) {
+                    claimed = h;
+                    placed = true;
+                    break;
+                }
+                if ((cur >> cbits) == v) {
+                    atomicAdd(cell, static_cast<unsigned long long>(inc));
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & rm;
+            }
+            if (!placed) atomicAdd(overflow, static_cast<unsigned long long>(inc));
+        }
+        const bool cl = claimed != kNoSlot;
+        uint64_t pend = __ballot(cl);
+        while (pend) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(pend)) - 1;
+            const uint32_t ls = __shfl(sl, leader, 64);
+            const uint64_t grp = __ballot(cl && sl == ls);
+            uint32_t b0 = 0;
+            if (__lane_id() == leader) b0 = atomicAdd(&occn[ls], static_cast<uint32_t>(__popcll(grp)));
+            b0 = __shfl(b0, leader, 64);
+            if (cl && sl == ls) occ[rb + b0 + rank_of(grp)] = claimed;
+            pend &= ~grp;
+        }
+    }
+
+    // Moves every cached entry to the arena; the lanes in `active` take part.
+    __device__ __forceinline__ void flush(uint64_t active) {
+        const uint32_t m = __popcll(active), r = rank_of(active);
+        for (uint32_t i0 = 0; i0 < kServeCache; i0 += m) {  // wave-uniform
+            const uint32_t i = i0 + r;
+            bool have = false;
+            uint32_t es = 0, ev = 0, ec = 0;
+            if (i < kServeCache) {
+                const unsigned long long k = cache->key[i];
+                if (k != kCacheEmpty) {
+                    have = true;
+                    es = static_cast<uint32_t>(k >> 32);
+                    ev = static_cast<uint32_t>(k);
+                    ec = cache->cnt[i];
+                    cache->key[i] = kCacheEmpty;
+                    cache->cnt[i] = 0;
+                }
+            }
+            uint64_t rb = 0;
+            uint32_t rm = 0;
+            if (have) {
+                const SlotInfo si = slots[es];
+                rb = si.base;
+                rm = si.mask;
+            }
+            arena_add(have, es, rb, rm, ev, ec);
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (__lane_id() == __ffsll(static_cast<unsigned long long>(active)) - 1) cache->fill = 0;
+        __builtin_amdgcn_wave_barrier();
+    }
+
     __device__ __forceinline__ void visit(uint32_t v, uint32_t) {
         ++visits;
         const bool at_src = v == source;
@@ -137,41 +242,44 @@ struct ServeRec {
             if (__lane_id() == leader) atomicAdd(&srccount[ls], static_cast<uint32_t>(__popcll(grp)));
             todo &= ~grp;
         }
-        constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-        uint32_t claimed = kNoSlot;
-        if (!at_src) {
-            const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | 1ull;
-            uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & mask;
-            bool placed = false;
-            for (uint32_t probe = 0; probe <= mask; ++probe) {
-                unsigned long long* cell = arena + base + h;
-                const unsigned long long cur = atomicCAS(cell, 0ull, entry);
-                if (cur == 0ull) {
-                    claimed = h;
-                    placed = true;
-                    break;
-                }
-                if ((cur >> cbits) == v) {
-                    atomicAdd(cell, 1ull);
-                    placed = true;
-                    break;
-                }
-                h = (h + 1) & mask;
+        // Lanes of one query standing on one vertex combine into one update.
+        // Up to three groups are folded per visit; the loop stops at the
+        // first group of one lane (the rest are likely distinct too).
+        bool mine = !at_src;
+        uint32_t inc = 1;
+        uint64_t fold = __ballot(mine);
+        for (uint32_t it = 0; it < 3 && fold; ++it) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(fold)) - 1;
+            const uint32_t v0 = __shfl(v, leader, 64);
+            const uint32_t s0 = __shfl(slot, leader, 64);
+            const bool in = mine && v == v0 && slot == s0;
+            const uint64_t grp = __ballot(in);
+            if (in) {
+                if (__lane_id() == leader) inc = static_cast<uint32_t>(__popcll(grp));
+                else mine = false;
             }
-            if (!placed) atomicAdd(overflow, 1ull);
+            fold &= ~grp;
+            if (__popcll(grp) == 1) break;
         }
-        const bool cl = claimed != kNoSlot;
-        uint64_t pend = __ballot(cl);
-        while (pend) {
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(pend)) - 1;
-            const uint32_t ls = __shfl(slot, leader, 64);
-            const uint64_t grp = __ballot(cl && slot == ls);
-            uint32_t b0 = 0;
-            if (__lane_id() == leader) b0 = atomicAdd(&occn[ls], static_cast<uint32_t>(__popcll(grp)));
-            b0 = __shfl(b0, leader, 64);
-            if (cl && slot == ls) occ[base + b0 + rank_of(grp)] = claimed;
-            pend &= ~grp;
+        // the wavefront's cache first; a lane that finds no room in a few
+        // probes goes to the arena directly
+        if (mine) {
+            const unsigned long long key = (static_cast<unsigned long long>(slot) << 32) | v;
+            uint32_t h = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> 56) & (kServeCache - 1);
+            for (uint32_t p = 0; p < kServeCacheProbes; ++p) {
+                const unsigned long long old = atomicCAS(&cache->key[h], kCacheEmpty, key);
+                if (old == kCacheEmpty || old == key) {
+                    atomicAdd(&cache->cnt[h], inc);
+                    if (old == kCacheEmpty) atomicAdd(&cache->fill, 1u);
+                    mine = false;
+                    break;
+                }
+                h = (h + 1) & (kServeCache - 1);
+            }
         }
+        arena_add(mine, slot, base, mask, v, inc);
+        const uint64_t active = __ballot(1);
+        if (cache->fill > kServeCacheFlush) flush(active);
     }
 };
 
@@ -188,25 +296,27 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
                                                         uint32_t* __restrict__ occ, uint32_t* __restrict__ occn,
                                                         uint32_t cbits, uint32_t hops,
                                                         unsigned long long* __restrict__ cnt,
-                                                        unsigned long long* __restrict__ cursor) {
-    constexpr uint32_t kChunk = 256;
+                                                        unsigned long long* __restrict__ cursor, uint32_t chunk) {
     const uint32_t lane = __lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;
     bool exhausted = false, active = false;
     uint32_t slot = 0, budget = 0, tot_steps = 0, tot_visits = 0;
     Prog q = p;
-    ServeRec rec{arena, srccount, occ, occn, 0, 0, cbits, cnt + 3, 0, 0};
+    __shared__ ServeCache s_cache[4];
+    ServeCache* cache = &s_cache[threadIdx.x >> 6];
+    serve_cache_init(cache, lane);
+    ServeRec rec{arena, srccount, occ, occn, 0, 0, cbits, cnt + 3, 0, 0, slots, cache};
     State s{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint64_t idle = __ballot(!active);
         while (idle && !exhausted) {
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(cursor, static_cast<unsigned long long>(kChunk));
+                if (lane == 0) base = atomicAdd(cursor, static_cast<unsigned long long>(chunk));
                 base = __shfl(base, 0, 64);
                 if (base >= n) { exhausted = true; break; }
                 chunk_next = base;
-                chunk_end = base + kChunk < n ? base + kChunk : n;
+                chunk_end = base + chunk < n ? base + chunk : n;
             }
             const uint32_t want = __popcll(idle);
             const uint64_t avail = chunk_end - chunk_next;
@@ -270,6 +380,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
             if (leave) pool_out[base + rank_of(lm)] = gw_walk{s.id, slot, s.v, s.hop};
         }
     }
+    rec.flush(__ballot(1));  // every lane of the wavefront is here
     const uint32_t st = wave_sum(tot_steps), vi = wave_sum(tot_visits);
     if (lane == 0) {
         atomicAdd(&cnt[1], static_cast<unsigned long long>(st));
@@ -324,48 +435,84 @@ __global__ void serve_gather_kernel(unsigned long long* __restrict__ arena, cons
     }
 }
 
-// Top-k of completed queries (gw_serve_topk): one workgroup per query over
-// its compacted pairs pairs[d.out .. d.out + n) (n = per-query pair count)
-// plus its source vertex (counted apart, in srccount).  Output packed
-// (vertex << cbits) | count, count descending, ties to the smaller vertex.
-__global__ __launch_bounds__(1024) void serve_topk_kernel(const unsigned long long* __restrict__ pairs,
-                                                        const Done* __restrict__ done,
-                                                        const unsigned long long* __restrict__ per_region,
-                                                        const uint32_t* __restrict__ srccount, uint32_t cbits,
-                                                        uint32_t k, unsigned long long* __restrict__ out,
-                                                        uint32_t* __restrict__ out_n) {
+// Top-k of completed queries (gw_serve_topk), in two levels so a query with
+// many pairs spreads over many CUs.  Items of query q: its compacted pairs
+// pairs[d.out .. d.out + n) (n = per-query pair count) and, as item n, its
+// source vertex (counted apart, in srccount).  Level 1: one workgroup per
+// kServeTopkChunk items writes that chunk's top-k keys to k candidate slots
+// (empty slots hold key 0).  Level 2: one workgroup per query selects the
+// top-k of its chunks' candidates -- the top-k of the union of every chunk's
+// top-k is the query's top-k -- and writes them packed (vertex << cbits) |
+// count, count descending, ties to the smaller vertex.
+constexpr uint64_t kServeTopkChunk = 32768;
+
+__device__ __forceinline__ uint64_t topk_chunks(uint64_t n) { return (n + 1 + kServeTopkChunk - 1) / kServeTopkChunk; }
+
+__global__ __launch_bounds__(1024) void serve_topk_chunk_kernel(const unsigned long long* __restrict__ pairs,
+                                                              const Done* __restrict__ done,
+                                                              const unsigned long long* __restrict__ per_region,
+                                                              const uint32_t* __restrict__ srccount,
+                                                              uint32_t cbits, uint32_t k,
+                                                              const unsigned long long* __restrict__ cand_off,
+                                                              unsigned long long* __restrict__ cand) {
     __shared__ TopkScratch t;
-    const Done d = done[blockIdx.x];
-    const uint64_t n = per_region[blockIdx.x];
+    const uint32_t q = blockIdx.y;
+    const uint64_t n = per_region[q];
+    const uint64_t c = blockIdx.x;
+    if (c >= topk_chunks(n)) return;  // workgroup-uniform
+    const Done d = done[q];
     const unsigned long long cmask = (1ull << cbits) - 1;
     const uint32_t sc = srccount[d.slot];
+    const uint64_t lo = c * kServeTopkChunk;
+    const uint64_t cnt = std::min<uint64_t>(kServeTopkChunk, n + 1 - lo);
     const uint32_t m = block_topk(
-        n + 1, k,
+        cnt, k,
         [&](uint64_t i) {
-            if (i == n) return topk_key(d.source, sc);
-            const unsigned long long x = pairs[d.out + i];
+            const uint64_t j = lo + i;
+            if (j == n) return topk_key(d.source, sc);
+            const unsigned long long x = pairs[d.out + j];
             return topk_key(static_cast<uint32_t>(x >> cbits), static_cast<uint32_t>(x & cmask));
         },
         t);
+    unsigned long long* o = cand + cand_off[q] + c * k;
+    for (uint32_t r = threadIdx.x; r < k; r += blockDim.x) o[r] = r < m ? t.sel[r] : 0ull;
+}
+
+__global__ __launch_bounds__(1024) void serve_topk_merge_kernel(const unsigned long long* __restrict__ per_region,
+                                                              const unsigned long long* __restrict__ cand_off,
+                                                              const unsigned long long* __restrict__ cand,
+                                                              uint32_t cbits, uint32_t k,
+                                                              unsigned long long* __restrict__ out,
+                                                              uint32_t* __restrict__ out_n) {
+    __shared__ TopkScratch t;
+    const uint32_t q = blockIdx.x;
+    const unsigned long long* c = cand + cand_off[q];
+    const uint32_t m = block_topk(topk_chunks(per_region[q]) * k, k, [&](uint64_t i) { return c[i]; }, t);
     for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
         const unsigned long long key = t.sel[r];
-        out[static_cast<uint64_t>(blockIdx.x) * k + r] =
+        out[static_cast<uint64_t>(q) * k + r] =
             (static_cast<unsigned long long>(~static_cast<uint32_t>(key)) << cbits) | (key >> 32);
     }
-    if (threadIdx.x == 0) out_n[blockIdx.x] = m;
+    if (threadIdx.x == 0) out_n[q] = m;
 }
 
 template <int KIND>
 hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, uint32_t grid) {
     const Graph g{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+    // Walks claimed per wavefront at a time: while the pool fits the grid at
+    // one walk per lane, 64, so every walk starts at once and a step lasts one
+    // walk's hops; larger pools take up to 256 (fewer cursor atomics).
+    const uint64_t waves = static_cast<uint64_t>(grid) * 4u;
+    const uint64_t per_wave = (n + waves - 1) / waves;
+    const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(256, std::max<uint64_t>(64, (per_wave + 63) / 64 * 64)));
     if (S.ext)
         serve_step_kernel<KIND, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
             p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
-            S.hops, S.d_cnt, S.d_cnt + 5);
+            S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     else
         serve_step_kernel<KIND, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
             p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
-            S.hops, S.d_cnt, S.d_cnt + 5);
+            S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     return hipGetLastError();
 }
 
@@ -409,6 +556,14 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 8ull, "serve region counts"))) {
+        gw_serve_close_state(ctx, S);
+        return rc;
+    }
+    // the completion buffer, sized here so a step rarely grows it (an
+    // allocation inside the serving loop would stall that step)
+    S->pairs_cap = std::max<uint64_t>(1, std::min<uint64_t>(S->arena_slots, 1ull << 24));
+    if ((rc = gw_alloc(ctx, (void**)&S->d_pairs, S->pairs_cap * 8, "serve compaction"))) {
+        S->pairs_cap = 0;
         gw_serve_close_state(ctx, S);
         return rc;
     }
@@ -533,9 +688,29 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         GW_HIP(ctx, hipGetLastError());
         if (S.topk) {
             // top-k per completed query on the device: k pairs per query cross PCIe
-            serve_topk_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_pairs, S.d_done, S.d_region_n,
-                                                                       S.d_srccount, S.cbits, S.topk, S.d_topk,
-                                                                       S.d_topk_n);
+            std::vector<unsigned long long> coff(nd);
+            uint64_t ncand = 0, maxc = 1;
+            for (uint32_t i = 0; i < nd; ++i) {
+                const uint64_t ch = (per[i] + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
+                coff[i] = ncand;
+                ncand += ch * S.topk;
+                maxc = std::max(maxc, ch);
+            }
+            if (ncand > S.cand_cap) {
+                gw_free(ctx, S.d_cand, S.cand_cap * 8);
+                S.d_cand = nullptr;
+                S.cand_cap = 0;
+                const uint64_t cap = std::max<uint64_t>(ncand, static_cast<uint64_t>(S.max_queries) * S.topk);
+                int rc = gw_alloc(ctx, (void**)&S.d_cand, cap * 8, "serve top-k candidates");
+                if (rc) return rc;
+                S.cand_cap = cap;
+            }
+            GW_HIP(ctx, hipMemcpyAsync(S.d_cand_off, coff.data(), nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
+            serve_topk_chunk_kernel<<<dim3(static_cast<uint32_t>(maxc), nd), dim3(1024), 0, ctx->stream>>>(
+                S.d_pairs, S.d_done, S.d_region_n, S.d_srccount, S.cbits, S.topk, S.d_cand_off, S.d_cand);
+            GW_HIP(ctx, hipGetLastError());
+            serve_topk_merge_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_region_n, S.d_cand_off, S.d_cand,
+                                                                            S.cbits, S.topk, S.d_topk, S.d_topk_n);
             GW_HIP(ctx, hipGetLastError());
             std::vector<unsigned long long> top(static_cast<uint64_t>(nd) * S.topk);
             std::vector<uint32_t> tn(nd);
@@ -624,14 +799,24 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     gw_free(ctx, S.d_topk, static_cast<uint64_t>(S.max_queries) * S.topk * 8);
     gw_free(ctx, S.d_topk_n, S.max_queries * 4ull);
+    gw_free(ctx, S.d_cand, S.cand_cap * 8);
+    gw_free(ctx, S.d_cand_off, S.max_queries * 8ull);
     S.d_topk = nullptr;
     S.d_topk_n = nullptr;
+    S.d_cand = S.d_cand_off = nullptr;
+    S.cand_cap = 0;
     S.topk = 0;
     if (k == 0) return GW_OK;
     int rc;
+    // candidates for a step whose completing queries average up to 8 chunks
+    // (256 K pairs each); a step that needs more grows the buffer
+    const uint64_t cand = static_cast<uint64_t>(S.max_queries) * 8 * k;
     if ((rc = gw_alloc(ctx, (void**)&S.d_topk, static_cast<uint64_t>(S.max_queries) * k * 8, "serve top-k")) ||
-        (rc = gw_alloc(ctx, (void**)&S.d_topk_n, S.max_queries * 4ull, "serve top-k counts")))
+        (rc = gw_alloc(ctx, (void**)&S.d_topk_n, S.max_queries * 4ull, "serve top-k counts")) ||
+        (rc = gw_alloc(ctx, (void**)&S.d_cand_off, S.max_queries * 8ull, "serve top-k candidate offsets")) ||
+        (rc = gw_alloc(ctx, (void**)&S.d_cand, cand * 8, "serve top-k candidates")))
         return rc;
+    S.cand_cap = cand;
     S.topk = k;
     return GW_OK;
 }
@@ -665,5 +850,7 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_region_n, S->max_queries * 8ull);
     gw_free(ctx, S->d_topk, static_cast<uint64_t>(S->max_queries) * S->topk * 8);
     gw_free(ctx, S->d_topk_n, S->max_queries * 4ull);
+    gw_free(ctx, S->d_cand, S->cand_cap * 8);
+    gw_free(ctx, S->d_cand_off, S->max_queries * 8ull);
     delete S;
 }

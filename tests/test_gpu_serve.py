@@ -108,3 +108,21 @@ def test_served_topk(engine, g12, k):
         ev, ec = expected(off, col, gw.RESTART, 10, qid, src, w)
         order = np.lexsort((ev, -ec.astype(np.int64)))[:k]
         assert np.array_equal(v, ev[order]) and np.array_equal(c, ec[order]), qid
+
+
+@pytest.mark.parametrize("k", [100, 1024])
+def test_served_topk_many_chunks(engine, k):
+    # queries with more pairs than one top-k chunk (32768 items): the chunks'
+    # candidates merge to the exact top-k of the whole query
+    off, col = O.rmat_graph(17, 1 << 21, seed=9)
+    engine.graph_set(off, col)
+    arrivals = [(0, 700, 11, 60_000), (0, 701, 5000, 40_000), (1, 702, 77, 2000), (1, 703, 123_456, 25_000)]
+    res = serve_all(engine, arrivals, gw.RESTART, 10, 10, pool=300_000, arena=256 << 20, topk=k)
+    most = 0
+    for _, qid, src, w in arrivals:
+        v, c = res[qid]
+        ev, ec = expected(off, col, gw.RESTART, 10, qid, src, w)
+        most = max(most, ev.shape[0])
+        order = np.lexsort((ev, -ec.astype(np.int64)))[:k]
+        assert np.array_equal(v, ev[order]) and np.array_equal(c, ec[order]), qid
+    assert most > 32768
