@@ -55,8 +55,9 @@ def test_rawwalks_rmat24_fullsize(engine):
 def test_ppr4096_rmat26_fullsize(engine):
     """R-MAT scale 26, edge factor 16, seed 3; 4096 uniform sources x 2000
     stop walks (p 0.15, cap 20): per-source counts identical by the
-    partitioned and the global sort-reduce methods, every source's counts
-    summing to its visits, and 6 sources replayed by the oracle."""
+    partitioned and the global sort-reduce methods, the counts summing to
+    the run's visits, sampled sources holding at least one visit of their
+    own vertex per walk, and 6 sources replayed by the oracle."""
     n = 1 << 26
     m = 16 * n
     engine.graph_rmat(26, m, 3)
