@@ -12,8 +12,8 @@ exact sparse per-(source, vertex) u32 counts.
 
 A bench "step" is one pass of the engine over all walks (gw_run), graph and
 walk records resident in HBM before timing.  The launch shape (block threads x
-waves per SIMD) is picked by gw_autotune before the timed passes, or fixed
-with --shape; "graph_replay" reports the same passes replayed from one
+waves per SIMD x walks a wavefront claims at a time) is picked by gw_autotune
+before the timed passes, or fixed with --shape; "graph_replay" reports the same passes replayed from one
 captured hipGraph.  Prints one JSON line.  The CPU
 baseline is the reference's own engine (its rawrandomwalks app, built from its
 sources into oracle/_ref/) on the same graph and walk count, on the host's
@@ -91,7 +91,8 @@ def timed(eng: gw.Engine, wl: dict, n: int, sched: str, layout: str, K: int, W: 
     prog = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"] | wl.get("flags", 0),
                       schedule=sched, layout=layout, source_method=source_method)
     if shape:
-        prog.block_threads, prog.waves_per_simd = shape
+        prog.block_threads, prog.waves_per_simd = shape[0], shape[1]
+        prog.walk_chunk = shape[2] if len(shape) > 2 else 0
     for _ in range(W):
         eng.run(prog)
     t = time.perf_counter()
@@ -203,16 +204,18 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     variants = {}
     tuned = None
     if shape:
-        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "run_ms": None}
+        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1],
+                 "walk_chunk": shape[2] if len(shape) > 2 else 512, "run_ms": None}
     else:
         # launch shape (block size x waves per SIMD) picked by gw_autotune before the timed passes
         base = gw.program(wl["kind"], wl["L"], wl["p"], n, seed=wl["seed"], recorder=wl["recorder"] | wl.get("flags", 0),
                           schedule=sched,
                           layout=layout)
         best, best_ms = eng.autotune(base, reps=2)
-        shape = (best.block_threads, best.waves_per_simd)
-        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "run_ms": best_ms}
-        print(f"[bench] {name} autotune: {shape[0]} threads x {shape[1]} waves/SIMD, {best_ms:.2f} ms",
+        shape = (best.block_threads, best.waves_per_simd, best.walk_chunk or 512)
+        tuned = {"block_threads": shape[0], "waves_per_simd": shape[1], "walk_chunk": shape[2], "run_ms": best_ms}
+        print(f"[bench] {name} autotune: {shape[0]} threads x {shape[1]} waves/SIMD, {shape[2]} walks per "
+              f"wavefront claim, {best_ms:.2f} ms",
               file=sys.stderr, flush=True)
     if compare:
         for s_, l_ in (("plain", "csr"), ("persistent", "csr"), ("persistent", "ext"), ("blocked", "ext")):
@@ -228,11 +231,12 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
         bbest, _ = eng.autotune(bprog, reps=1)
         nb = eng.graph_info()["nblocks"]
         _, w_, st_, _, k_, last = timed(eng, wl, n, "blocked", "ext", max(2, K // 3), 1,
-                                        shape=(bbest.block_threads, bbest.waves_per_simd))
+                                        shape=(bbest.block_threads, bbest.waves_per_simd, bbest.walk_chunk))
         variants["blocked-ext-tuned"] = {"walk_steps_per_s": st_ / w_, "kernel_ms": float(np.mean(k_)),
                                          "rounds": last["rounds"], "bucket_ms": last["bucket_ms"], "blocks": nb,
                                          "block_vertices": -(-n // nb),
-                                         "launch_shape": [bbest.block_threads, bbest.waves_per_simd]}
+                                         "launch_shape": [bbest.block_threads, bbest.waves_per_simd,
+                                                          bbest.walk_chunk or 512]}
         print(f"[bench] {name} blocked-ext tuned ({nb} blocks): {st_ / w_ / 1e9:.2f} G steps/s", file=sys.stderr,
               flush=True)
         eng.graph_blocks(np.arange(0, n + 1, wl["block_vertices"], dtype=np.uint32))
@@ -439,7 +443,8 @@ def main() -> int:
     ap.add_argument("--secondary", default="ppr4096-rmat26", help="extra workload reported under 'workloads' ('' = none)")
     ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent", "blocked"])
     ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
-    ap.add_argument("--shape", default="", help="launch shape 'THREADS,WAVES' (block threads, waves per SIMD); "
+    ap.add_argument("--shape", default="", help="launch shape 'THREADS,WAVES[,CHUNK]' (block threads, waves per "
+                                               "SIMD, walks a wavefront claims at a time, default 512); "
                                                "default: picked by the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -65,6 +65,7 @@ class Program(C.Structure):
         ("event_cap", C.c_uint32),
         ("table_log2", C.c_uint32),
         ("source_method", C.c_uint32),
+        ("walk_chunk", C.c_uint32),
     ]
 
 
@@ -124,7 +125,8 @@ def stop_threshold(p: float) -> int:
 
 def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, recorder: int = REC_NONE,
             schedule=SCHED_AUTO, stream: int = 0, block_threads: int = 0, waves_per_simd: int = 0,
-            layout=LAYOUT_AUTO, event_cap: int = 0, table_log2: int = 0, source_method: int = SRC_SORT) -> Program:
+            layout=LAYOUT_AUTO, event_cap: int = 0, table_log2: int = 0, source_method: int = SRC_SORT,
+            walk_chunk: int = 0) -> Program:
     if isinstance(kind, str):
         kind = KIND_NAMES[kind]
     if isinstance(schedule, str):
@@ -132,7 +134,7 @@ def program(kind, L: int, p: float = 0.15, nvertices: int = 0, seed: int = 0, re
     if isinstance(layout, str):
         layout = LAYOUT_NAMES[layout]
     return Program(kind, L, stop_threshold(p), nvertices, stream, seed, recorder, schedule, block_threads,
-                   waves_per_simd, layout, event_cap, table_log2, source_method)
+                   waves_per_simd, layout, event_cap, table_log2, source_method, walk_chunk)
 
 
 def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:

@@ -58,6 +58,7 @@ struct Prog {
     // index w - id_base: a window of a host walk pool writes window-sized
     // arrays.  The random stream stays keyed by the walk id itself.
     uint32_t id_base;
+    uint32_t chunk;  // walks a wavefront claims per atomic (persistent and blocked schedules)
 };
 
 // Successor-header edge (16 bytes): the column entry plus the destination's

@@ -24,7 +24,7 @@
 namespace gw {
 namespace {
 
-constexpr uint32_t kChunk = 512;           // walk indices a wavefront claims per atomic
+constexpr uint32_t kChunk = 512;           // default walk indices a wavefront claims per atomic
 constexpr int kRecKeyLog = 1 << 16;        // internal recorder bit: source counts by sort-reduce
 constexpr int kRecRegion = 1 << 17;        // internal recorder bit: visits straight into per-source regions
 constexpr uint32_t kDead = 0xFFFFFFFFu;    // vertex of a finished walk in the blocked schedule
@@ -113,14 +113,14 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
         while (idle && !exhausted) {
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(kChunk));
+                if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(p.chunk));
                 base = __shfl(base, 0, 64);
                 if (base >= nw) {
                     exhausted = true;
                     break;
                 }
                 chunk_next = base;
-                chunk_end = std::min<uint64_t>(base + kChunk, nw);
+                chunk_end = std::min<uint64_t>(base + p.chunk, nw);
             }
             const uint32_t want = __popcll(idle);
             const uint64_t avail = chunk_end - chunk_next;
@@ -239,14 +239,14 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
         while (idle && !exhausted) {
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(kChunk));
+                if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(p.chunk));
                 base = __shfl(base, 0, 64);
                 if (base >= nw) {
                     exhausted = true;
                     break;
                 }
                 chunk_next = base;
-                chunk_end = std::min<uint64_t>(base + kChunk, nw);
+                chunk_end = std::min<uint64_t>(base + p.chunk, nw);
             }
             const uint32_t want = __popcll(idle);
             const uint64_t avail = chunk_end - chunk_next;
@@ -864,8 +864,11 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     L.ctx = ctx;
     L.sched = sched;
     L.ext = ext;
+    const uint32_t chunk = prog->walk_chunk ? prog->walk_chunk : kChunk;
+    if (chunk % 64 != 0 || chunk > 8192)
+        return gw_fail(ctx, GW_EINVAL, "gw_run: walk_chunk must be a multiple of 64, at most 8192");
     L.p = Prog{prog->L, prog->nvertices, prog->stop_thr, static_cast<uint32_t>(prog->seed),
-               static_cast<uint32_t>(prog->seed >> 32), prog->stream};
+               static_cast<uint32_t>(prog->seed >> 32), prog->stream, 0u, chunk};
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
     L.table = SourceTable{ctx->d_table, ctx->d_counters + kCntTableOverflow, ctx->table_slots - 1, 64u - ctx->table_log2,
@@ -1337,6 +1340,23 @@ int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* 
             gw_program p = *prog;
             p.block_threads = tb;
             p.waves_per_simd = sched == GW_SCHED_PLAIN ? 0 : wps;
+            double m = 0;
+            int rc = time_of(p, m);
+            if (rc) return rc;
+            if (m < bms) {
+                bms = m;
+                bp = p;
+            }
+        }
+    }
+    // walks a wavefront claims per atomic, at the chosen shape (the plain
+    // grid claims none)
+    if (sched != GW_SCHED_PLAIN) {
+        static const uint32_t kChunks[] = {128, 512, 2048, 8192};
+        const gw_program shape = bp;
+        for (uint32_t c : kChunks) {
+            gw_program p = shape;
+            p.walk_chunk = c;
             double m = 0;
             int rc = time_of(p, m);
             if (rc) return rc;

@@ -140,6 +140,8 @@ typedef struct gw_program {
     uint32_t event_cap;       /* GW_REC_EVENTS: visits stored per walk (more are counted, not stored) */
     uint32_t table_log2;      /* GW_REC_SOURCE_COUNT with GW_SRC_HASH: log2 of hash slots, 0 = auto */
     uint32_t source_method;   /* GW_REC_SOURCE_COUNT: GW_SRC_* */
+    uint32_t walk_chunk;      /* walks a wavefront claims per atomic (persistent and blocked
+                                 schedules): a multiple of 64 in [64, 8192], 0 = 512 */
 } gw_program;
 
 /* How GW_REC_SOURCE_COUNT aggregates (source, vertex) visits.  The sort-based
@@ -147,9 +149,11 @@ typedef struct gw_program {
 #define GW_SRC_SORT 0u         /* GW_SRC_PARTITIONED when the source fields allow it, else GW_SRC_GLOBAL */
 #define GW_SRC_HASH 1u         /* device-wide open-addressing hash, one or two atomics per visit */
 #define GW_SRC_GLOBAL 2u       /* radix sort of the whole key log + run-length encode */
-#define GW_SRC_PARTITIONED 3u  /* scatter the log into one region per source field (fields < 2^20),
-                                  count each region in an LDS hash table; exact sort-reduce for
-                                  sources whose distinct vertices outgrow the table */
+#define GW_SRC_PARTITIONED 3u  /* one region per source field (fields < 2^20), filled by the step
+                                  kernel itself for capped walks (through per-wavefront LDS
+                                  buffers) or scattered from the key log for GW_PROB; each region
+                                  counted in an LDS hash table; exact sort-reduce for sources
+                                  whose distinct vertices outgrow the table */
 
 /* graph layouts the step kernels read (the CSR is always resident) */
 #define GW_LAYOUT_AUTO 0u
@@ -272,11 +276,13 @@ int gw_run(gw_ctx *ctx, const gw_program *prog, gw_stats *stats);
 int gw_run_capture(gw_ctx *ctx, const gw_program *prog);
 int gw_run_replay(gw_ctx *ctx, gw_stats *stats);
 /* Autotuner: runs prog at every (block_threads, waves_per_simd) candidate,
- * `reps` timed runs each after a warm-up, and returns the fastest in *best.
- * For the blocked schedule on a device-resident graph it first picks the
- * block size -- uniform tables of 2^12 .. 2^24 vertices per block, at most
- * 2048 blocks -- and leaves the fastest table installed (gw_graph_info shows
- * it).  Results depend on neither. */
+ * then at the best of those with walk_chunk in {128, 512, 2048, 8192}
+ * (persistent and blocked schedules), `reps` timed runs each after a
+ * warm-up, and returns the fastest in *best.  For the blocked schedule on a
+ * device-resident graph it first picks the block size -- uniform tables of
+ * 2^12 .. 2^24 vertices per block, at most 2048 blocks -- and leaves the
+ * fastest table installed (gw_graph_info shows it).  Results depend on none
+ * of these. */
 int gw_autotune(gw_ctx *ctx, const gw_program *prog, uint32_t reps, gw_program *best, double *best_ms);
 
 /* Stable counting sort of the loaded walks by block of their vertex (out of

@@ -49,7 +49,7 @@ def test_header_compiles_as_c():
 
 
 def test_struct_layouts_match_header():
-    assert ctypes.sizeof(gw.Program) == 64
+    assert ctypes.sizeof(gw.Program) == 72
     assert gw.WALK_DTYPE.itemsize == 16
     src = ('#include "gwhip.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){printf("%zu %zu %zu %zu",'
            ' sizeof(gw_program), sizeof(gw_walk), sizeof(gw_stats), offsetof(gw_program, recorder)); return 0;}\n')

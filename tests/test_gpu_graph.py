@@ -98,6 +98,7 @@ def test_autotune_keeps_results(engine, g12):
     ref = _state(engine, prog.recorder)
     best, ms = engine.autotune(prog, reps=1)
     assert ms > 0 and best.block_threads in (64, 128, 256) and best.waves_per_simd in (2, 4, 6, 8, 12, 16)
+    assert best.walk_chunk in (0, 128, 512, 2048, 8192)
     engine.run(best)
     _same(_state(engine, prog.recorder), ref)
 

@@ -167,21 +167,36 @@ def test_batched_ppr_l1_below_001_with_enough_walks(engine):
         assert l1 < 0.01
 
 
-@pytest.mark.parametrize("tb,wps", [(64, 1), (128, 3), (256, 8), (192, 5)])
-def test_launch_shape_independence(engine, small_graph, tb, wps):
+@pytest.mark.parametrize("tb,wps,chunk", [(64, 1, 64), (128, 3, 0), (256, 8, 8192), (192, 5, 2048)])
+def test_launch_shape_independence(engine, small_graph, tb, wps, chunk):
     off, col = small_graph
     n = off.shape[0] - 1
     engine.graph_set(off, col)
     walks = _walks(7_777, n, gw.STOP, seed=9)
     engine.walks_load(walks)
     prog = gw.program(gw.STOP, 20, 0.15, n, seed=3, recorder=gw.REC_VERTEX_COUNT | gw.REC_WALK_SUMMARY,
-                      schedule=gw.SCHED_PERSISTENT, block_threads=tb, waves_per_simd=wps)
+                      schedule=gw.SCHED_PERSISTENT, block_threads=tb, waves_per_simd=wps, walk_chunk=chunk)
     engine.run(prog)
     c1, (h1, _) = engine.counts(), engine.summary()
     prog.schedule = gw.SCHED_PLAIN
     engine.run(prog)
     assert np.array_equal(engine.counts(), c1)
     assert np.array_equal(engine.summary()[0], h1)
+    # the blocked schedule claims walks the same way
+    engine.graph_blocks([0, 700, 2500, n])
+    prog.schedule = gw.SCHED_BLOCKED
+    engine.run(prog)
+    assert np.array_equal(engine.counts(), c1)
+    assert np.array_equal(engine.summary()[0], h1)
+
+
+@pytest.mark.parametrize("chunk", [32, 100, 8256])
+def test_walk_chunk_invalid(engine, small_graph, chunk):
+    off, col = small_graph
+    engine.graph_set(off, col)
+    engine.walks_load(_walks(100, off.shape[0] - 1, gw.STOP, seed=1))
+    with pytest.raises(gw.GwError, match="walk_chunk"):
+        engine.run(gw.program(gw.STOP, 5, 0.15, off.shape[0] - 1, seed=1, walk_chunk=chunk))
 
 
 def test_edge_cases(engine):
