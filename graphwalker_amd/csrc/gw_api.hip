@@ -228,6 +228,8 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
     gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
     gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
+    gw_free(ctx, ctx->d_part_fvert, ctx->part_fields_cap * 4ull);
+    gw_free(ctx, ctx->d_part_fcount, ctx->part_fields_cap * 4ull);
     gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
     gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);
@@ -361,10 +363,14 @@ int gw_walks_load(gw_ctx* ctx, const gw_walk* walks, uint64_t n) {
     ctx->walk_src_max = src_max;
     ctx->walk_v_max = v_max;
     ctx->field_walks.clear();
+    ctx->field_vertex.clear();
     ctx->part_per_walk = 0;
     if (n && src_max < kPartFieldLimit) {
         ctx->field_walks.assign(src_max + 1ull, 0);
-        for (uint64_t i = 0; i < n; ++i) ctx->field_walks[walks[i].source]++;
+        ctx->field_vertex.assign(src_max + 1ull, 0xFFFFFFFFu);
+        for (uint64_t i = 0; i < n; ++i) {
+            if (ctx->field_walks[walks[i].source]++ == 0) ctx->field_vertex[walks[i].source] = walks[i].vertex;
+        }
     }
     return GW_OK;
 }
@@ -384,8 +390,13 @@ int gw_walks_init_per_vertex(gw_ctx* ctx, uint32_t per_vertex) {
     ctx->walk_src_max = ctx->n - 1;
     ctx->walk_v_max = ctx->n - 1;
     ctx->field_walks.clear();
+    ctx->field_vertex.clear();
     ctx->part_per_walk = 0;
-    if (ctx->n <= kPartFieldLimit) ctx->field_walks.assign(ctx->n, per_vertex);
+    if (ctx->n <= kPartFieldLimit) {
+        ctx->field_walks.assign(ctx->n, per_vertex);
+        ctx->field_vertex.resize(ctx->n);
+        for (uint32_t v = 0; v < ctx->n; ++v) ctx->field_vertex[v] = v;  // walk i starts at vertex i / per_vertex
+    }
     return GW_OK;
 }
 
@@ -422,10 +433,16 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     ctx->walk_src_max = src_max;
     ctx->walk_v_max = v_max;
     ctx->field_walks.clear();
+    ctx->field_vertex.clear();
     ctx->part_per_walk = 0;
     if (src_max < kPartFieldLimit) {
         ctx->field_walks.assign(src_max + 1ull, 0);
-        for (uint32_t k = 0; k < nsources; ++k) ctx->field_walks[source_field ? source_field[k] : sources[k]] += walks_per_source;
+        ctx->field_vertex.assign(src_max + 1ull, 0xFFFFFFFFu);
+        for (uint32_t k = 0; k < nsources; ++k) {
+            const uint32_t f = source_field ? source_field[k] : sources[k];
+            if (ctx->field_walks[f] == 0) ctx->field_vertex[f] = sources[k];
+            ctx->field_walks[f] += walks_per_source;
+        }
     }
     return GW_OK;
 }

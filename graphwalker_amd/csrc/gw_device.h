@@ -151,6 +151,11 @@ struct KeyLog {
     uint32_t* __restrict__ region_out;
     unsigned long long* __restrict__ region_overflow;
     uint32_t nf;
+    // a designated vertex per field (the first walk's start) whose visits are
+    // counted in field_count[f] instead of the region: every walk of a
+    // single-source field opens with one
+    const uint32_t* __restrict__ field_vertex;
+    uint32_t* __restrict__ field_count;
 };
 
 // Called by the lanes that visit; `slab` = this wavefront's {next, end} in LDS.
@@ -229,14 +234,26 @@ __device__ __forceinline__ void region_flush(const KeyLog& k, const uint32_t* ke
 // its own source field.  The visiting lanes are grouped by field with a
 // ballot loop (a wavefront's walks come from one stretch of the walk array,
 // so they name one or two fields).  A field >= nf counts in region_overflow.
-__device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint32_t source, uint32_t v) {
+__device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint32_t source, uint32_t v,
+                                           uint32_t field_v) {
     const uint64_t active = __ballot(1);
     const uint32_t lane = __lane_id();
-    uint64_t todo = active;
+    // visits of the field's designated vertex: one count per (wavefront, field)
+    const bool fvis = v == field_v;
+    uint64_t fdo = __ballot(fvis);
+    while (fdo) {  // wave-uniform
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(fdo)) - 1;
+        const uint32_t ls = __shfl(source, leader, 64);
+        const uint64_t grp = __ballot(fvis && source == ls);
+        if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], static_cast<uint32_t>(__popcll(grp)));
+        fdo &= ~grp;
+    }
+    uint64_t todo = __ballot(!fvis);
     while (todo) {  // wave-uniform
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
         const uint32_t ls = __shfl(source, leader, 64);
-        const uint64_t grp = __ballot(source == ls);
+        const bool in = !fvis && source == ls;
+        const uint64_t grp = __ballot(in);
         todo &= ~grp;
         const uint32_t g = __popcll(grp);
         if (ls >= k.nf) {
@@ -260,7 +277,7 @@ __device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint3
             region_flush(k, rb->key[b], ls, c, active);
             c = 0;
         }
-        if (source == ls) rb->key[b][c + lanes_below(grp)] = v;
+        if (in) rb->key[b][c + lanes_below(grp)] = v;
         __builtin_amdgcn_wave_barrier();
         if (lane == leader) *reinterpret_cast<uint2*>(&rb->st[2 * b]) = make_uint2(ls, c + g);
         __builtin_amdgcn_wave_barrier();
@@ -296,6 +313,7 @@ struct Recorder {
     KeyLog klog;
     uint64_t* slab;
     RegionBuf* rbuf;
+    uint32_t field_v = 0xFFFFFFFFu;  // the current walk's field's designated vertex (direct regions)
     __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t, const KeyLog& kl,
                                         uint64_t* sl, RegionBuf* rb = nullptr)
         : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl),
@@ -329,7 +347,7 @@ struct Recorder {
         }
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
         if (kKeyLog) keylog_add(klog, slab, source, v);
-        if (kRegion) region_add(klog, rbuf, source, v);
+        if (kRegion) region_add(klog, rbuf, source, v, field_v);
     }
 };
 

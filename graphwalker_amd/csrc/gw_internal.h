@@ -144,6 +144,9 @@ struct gw_ctx {
     // walks (empty when a field is kPartFieldLimit or more) size one region
     // of the partitioned key log per field.
     std::vector<uint64_t> field_walks;
+    // per source field: a designated vertex (the first loaded walk's start)
+    // whose visits a direct-region run counts apart; 0xFFFFFFFF for none
+    std::vector<uint32_t> field_vertex;
     uint32_t part_per_walk = 0;              // visits per walk the uploaded regions are sized for (0 = stale)
     uint32_t part_fields = 0, part_fields_cap = 0;
     uint64_t part_total = 0;                 // region slots in all (sum of the caps)
@@ -154,6 +157,8 @@ struct gw_ctx {
     uint64_t* d_part_off = nullptr;          // region start per field (fields + 1)
     unsigned long long* d_part_cursor = nullptr;  // keys written per field
     unsigned long long* d_part_pairs = nullptr;   // distinct vertices per field (0 for a flushed source)
+    uint32_t* d_part_fvert = nullptr;             // field_vertex on the device
+    uint32_t* d_part_fcount = nullptr;            // visits of each field's designated vertex
     uint32_t* d_split_counts = nullptr;      // 2 x split_cap: counts of the split pairs (their keys go to the key log)
     uint64_t split_cap = 0;
     uint64_t part_pairs = 0;                 // last run: pairs held in the regions

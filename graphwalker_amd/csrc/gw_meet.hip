@@ -250,6 +250,7 @@ int gw_walks_init_pairs(gw_ctx* ctx, const uint32_t* a, const uint32_t* b, uint3
     ctx->walk_src_max = src_max;
     ctx->walk_v_max = src_max;
     ctx->field_walks.clear();
+    ctx->field_vertex.clear();
     ctx->part_per_walk = 0;
     return GW_OK;
 }

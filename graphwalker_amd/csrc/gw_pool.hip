@@ -136,6 +136,7 @@ int gw_walks_load_host(gw_ctx* ctx, const gw_walk* walks, uint64_t n, uint64_t w
     ctx->walk_src_max = src_max;
     ctx->walk_v_max = v_max;
     ctx->field_walks.clear();
+    ctx->field_vertex.clear();
     ctx->part_per_walk = 0;
     return GW_OK;
 }

@@ -63,6 +63,7 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
         const uint32_t o = w.id - p.id_base;
         rec.id = o;
         rec.source = w.source;
+        if (RG) rec.field_v = w.source < klog.nf ? klog.field_vertex[w.source] : 0xFFFFFFFFu;
         advance<KIND>(p, g, w.id, w.source, w.vertex, w.hop, 0u, g.n, rec);
         if (final_walks) final_walks[o] = w;
         if (SUM) {
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 s.id = w.id;
                 rec.id = w.id - p.id_base;
                 rec.source = w.source;
+                if (RG) rec.field_v = w.source < klog.nf ? klog.field_vertex[w.source] : 0xFFFFFFFFu;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -258,6 +260,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                 s.id = w.id;
                 rec.id = w.id - p.id_base;
                 rec.source = w.source;
+                if (RG) rec.field_v = w.source < klog.nf ? klog.field_vertex[w.source] : 0xFFFFFFFFu;
                 s.source = w.source;
                 s.v = w.vertex;
                 s.hop = w.hop;
@@ -911,6 +914,8 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
             L.klog.region_out = ctx->d_part_keys;
             L.klog.region_overflow = ctx->d_counters + kCntPartOverflow;
             L.klog.nf = ctx->part_fields;
+            L.klog.field_vertex = ctx->d_part_fvert;
+            L.klog.field_count = ctx->d_part_fcount;
         }
     }
     rp.sched = sched;
@@ -994,7 +999,8 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             ctx->n, p(ctx->d_off), p(ctx->d_col), p(ctx->d_ext), p(ctx->d_events), ctx->events_cap,
             p(ctx->d_table), ctx->table_bytes, ctx->table_slots, ctx->table_log2, ctx->table_vbits, ctx->table_cbits,
             ctx->table_wide, p(ctx->d_keys), ctx->keys_cap, ctx->keys_sentinel, p(ctx->d_part_keys),
-            p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), ctx->part_fields,
+            p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), p(ctx->d_part_fvert),
+            p(ctx->d_part_fcount), ctx->part_fields,
             ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
             rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final};
 }

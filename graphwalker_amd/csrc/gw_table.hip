@@ -89,10 +89,14 @@ __device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
 // Zero the engine's device counters and the per-source region cursors before
 // a run, in one launch.
 __global__ void reset_counters_kernel(unsigned long long* __restrict__ counters, uint32_t ncounters,
-                                      unsigned long long* __restrict__ region_n, uint32_t nf) {
+                                      unsigned long long* __restrict__ region_n, uint32_t* __restrict__ fcount,
+                                      uint32_t nf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ncounters) counters[i] = 0ull;
-    if (i < nf) region_n[i] = 0ull;
+    if (i < nf) {
+        region_n[i] = 0ull;
+        fcount[i] = 0u;
+    }
 }
 
 // Scatter the key log into per-source regions: region f receives the vertex
@@ -289,13 +293,17 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ keys, uin
 // vertices and counts over the start of its own region (its keys are all in
 // registers or consumed by then) and its pair count to pairs[f], with no
 // atomic on the critical path; a flushed source emits partial pairs to the
-// split list, summed later by sort-reduce, and pairs[f] = 0.
+// split list, summed later by sort-reduce, and pairs[f] = 0.  With fcount
+// (direct-region runs), source f's designated vertex fvert[f] and its
+// fcount[f] visits, counted apart by the step kernel, enter the table before
+// the region's keys; a source with no region keys writes that one pair.
 __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
     uint32_t* __restrict__ keys, uint32_t* __restrict__ counts, unsigned long long* __restrict__ pairs,
     const uint64_t* __restrict__ region_off, const unsigned long long* __restrict__ region_n, uint32_t nf,
     uint32_t vbits, unsigned long long* __restrict__ total_pairs, unsigned long long* __restrict__ split_k,
     uint32_t* __restrict__ split_c, unsigned long long* __restrict__ split_n, uint64_t split_cap,
-    unsigned long long* __restrict__ flushed_sources, unsigned long long* __restrict__ queue) {
+    unsigned long long* __restrict__ flushed_sources, unsigned long long* __restrict__ queue,
+    const uint32_t* __restrict__ fvert, const uint32_t* __restrict__ fcount) {
     __shared__ uint32_t s_key[kAggSlots];
     __shared__ uint32_t s_cnt[kAggSlots];
     __shared__ uint32_t s_fill;
@@ -324,15 +332,34 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
     while (f < nf) {
         unsigned long long ticket = 0;
         if (threadIdx.x == 0) ticket = atomicAdd(queue, 1ull);  // consumed at the end of this source
+        // visits of the field's designated vertex, counted apart by the step kernel
+        const uint32_t fc = fcount ? fcount[f] : 0u;
+        const uint32_t fv = fc ? fvert[f] : kEmptyKey;
         if (n == 0) {
-            if (threadIdx.x == 0) pairs[f] = 0;
+            if (threadIdx.x == 0) {
+                if (fc) {
+                    keys[lo] = fv;
+                    counts[lo] = fc;
+                    atomicAdd(total_pairs, 1ull);
+                }
+                pairs[f] = fc ? 1u : 0u;
+            }
             load_tile(keys, nlo, nn, 0, pre);
         } else {
-            const uint32_t lg = agg_log2(n);
+            const uint32_t lg = agg_log2(n + (fc ? 1u : 0u));
             const uint32_t tslots = 1u << lg;
             const uint32_t max_fill = tslots / 4 * 3;
             const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
             bool flushed = false;
+            if (fc) {  // workgroup-uniform: the table is empty when a source starts
+                if (threadIdx.x == 0) {
+                    const uint32_t h = (fv * 0x9E3779B1u) >> (32 - lg);
+                    s_key[h] = fv;
+                    s_cnt[h] = fc;
+                    s_fill = 1;
+                }
+                __syncthreads();
+            }
             for (uint64_t base = 0;; base += kAggTile) {
                 const uint64_t next = base + kAggTile;
                 if (next < n) load_tile(keys, lo, n, next, pre);
@@ -564,12 +591,17 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
             gw_free(ctx, ctx->d_part_off, (ctx->part_fields_cap + 1ull) * 8);
             gw_free(ctx, ctx->d_part_cursor, ctx->part_fields_cap * 8ull);
             gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
+            gw_free(ctx, ctx->d_part_fvert, ctx->part_fields_cap * 4ull);
+            gw_free(ctx, ctx->d_part_fcount, ctx->part_fields_cap * 4ull);
             ctx->d_part_off = nullptr;
             ctx->d_part_cursor = ctx->d_part_pairs = nullptr;
+            ctx->d_part_fvert = ctx->d_part_fcount = nullptr;
             ctx->part_fields_cap = 0;
             if ((rc = gw_alloc(ctx, (void**)&ctx->d_part_off, (nf + 1ull) * 8, "source regions")) ||
                 (rc = gw_alloc(ctx, (void**)&ctx->d_part_cursor, nf * 8ull, "source region cursors")) ||
-                (rc = gw_alloc(ctx, (void**)&ctx->d_part_pairs, nf * 8ull, "source pair counts")))
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_pairs, nf * 8ull, "source pair counts")) ||
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_fvert, nf * 4ull, "source field vertices")) ||
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_fcount, nf * 4ull, "source field vertex counts")))
                 return rc;
             ctx->part_fields_cap = nf;
         }
@@ -589,6 +621,8 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
             ctx->part_keys_cap = need;
         }
         GW_HIP(ctx, hipMemcpy(ctx->d_part_off, off.data(), (nf + 1ull) * 8, hipMemcpyHostToDevice));
+        if (ctx->field_vertex.size() != nf) return gw_fail(ctx, GW_ESTATE, "gw_run: source field table out of date");
+        GW_HIP(ctx, hipMemcpy(ctx->d_part_fvert, ctx->field_vertex.data(), nf * 4ull, hipMemcpyHostToDevice));
         ctx->part_per_walk = per_walk;
         ctx->part_fields = nf;
         ctx->part_total = off[nf];
@@ -611,7 +645,8 @@ int gw_partlog_reset(gw_ctx* ctx, bool part) {
     const uint32_t nf = part ? ctx->part_fields : 0u;
     const uint32_t n = std::max<uint32_t>(kCounters, nf);
     reset_counters_kernel<<<dim3((n + 255) / 256), dim3(256), 0, ctx->stream>>>(ctx->d_counters, kCounters,
-                                                                                ctx->d_part_cursor, nf);
+                                                                                ctx->d_part_cursor,
+                                                                                ctx->d_part_fcount, nf);
     GW_HIP(ctx, hipGetLastError());
     return GW_OK;
 }
@@ -629,7 +664,8 @@ int gw_partlog_enqueue(gw_ctx* ctx, bool partition) {
     part_aggregate_kernel<<<dim3(grid), dim3(kAggThreads), 0, ctx->stream>>>(
         ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->d_part_cursor,
         ctx->part_fields, ctx->table_vbits, c + kCntOutPairs, ctx->d_keys, ctx->d_split_counts, c + kCntSplitPairs,
-        ctx->split_cap, c + kCntFlushed, c + kCntAggQueue);
+        ctx->split_cap, c + kCntFlushed, c + kCntAggQueue, partition ? nullptr : ctx->d_part_fvert,
+        partition ? nullptr : ctx->d_part_fcount);
     GW_HIP(ctx, hipGetLastError());
     return GW_OK;
 }
