@@ -537,3 +537,42 @@ def test_unusual_graphs_all_kinds(engine, name, variant):
         assert np.array_equal(engine.walks_final(), ref["final"]), (name, kind)
         h, vis = engine.summary()
         assert np.array_equal(h, ref["hash"]) and np.array_equal(vis, ref["walk_visits"]), (name, kind)
+
+
+def _pairs_of(walks, off, col, prog):
+    """Exact (field, vertex, count) of arbitrary loaded walks: the oracle's
+    counts per field, sorted by field then vertex."""
+    out = []
+    for f in np.unique(walks["source"]):
+        c = O.replay(oprog(prog), off, col, walks[walks["source"] == f])["counts"]
+        nz = np.nonzero(c)[0]
+        out.append((np.full(nz.shape[0], f, np.uint32), nz.astype(np.uint32), c[nz]))
+    return tuple(np.concatenate([o[i] for o in out]) for i in range(3))
+
+
+@pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("blocked", "ext")])
+@pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART, gw.RESTART_JOINT])
+def test_source_counts_mixed_starts(engine, small_graph, kind, variant):
+    # fields whose walks start at different vertices and hops: each field's
+    # designated vertex (its first walk's start) is counted apart by the
+    # direct-region path; every method gives the oracle's pairs
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    if variant[0] == "blocked":
+        engine.graph_blocks([0, 512, 1024, 3000, n])
+    rng = np.random.default_rng(kind + 40)
+    nw = 6000
+    field = rng.integers(0, 37, nw).astype(np.uint32)
+    start = np.where(rng.random(nw) < 0.5, (field * 97) % n, rng.integers(0, n, nw)).astype(np.uint32)
+    hops = (rng.integers(0, 3, nw) * (11 if kind == gw.RESTART_JOINT else 2)).astype(np.uint32)
+    walks = O.make_walks(start, sources_=field, hops=hops)
+    engine.walks_load(walks)
+    for method in (gw.SRC_PARTITIONED, gw.SRC_GLOBAL, gw.SRC_HASH):
+        prog = gw.program(kind, 11, 0.15, n, seed=91, recorder=gw.REC_SOURCE_COUNT, schedule=variant[0],
+                          layout=variant[1], source_method=method)
+        st = engine.run(prog)
+        assert st["source_method"] == method
+        es, ev, ec = _pairs_of(walks, off, col, prog)
+        s, v, c = engine.source_counts()
+        assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec), method
