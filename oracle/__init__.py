@@ -336,3 +336,15 @@ def run_reference_simrank(base: str, a: int, b: int, R: int, L: int, threads: in
         if line.startswith(f"SimRank for {a} and {b} = "):
             return float(line.rsplit("=", 1)[1])
     raise RuntimeError("the reference simrank app printed no score")
+
+
+def run_reference_graphlet(base: str, n: int, R: int, L: int, threads: int, timeout: float) -> int:
+    """The reference graphlet app's closed-walk count (cnt_ok): R stop walks
+    of cap L from rand() % N sources, counting visits to the source at hop
+    L-1 (graphlet.cpp:46-52), one block all in memory."""
+    text = _run_reference("graphlet", base, {"N": n, "R": R, "L": L, "execthreads": threads,
+                                             "blocksize_kb": 1 << 20, "nmblocks": 1, "prob": 0.2}, timeout)
+    for line in text.splitlines():
+        if line.startswith("cnt_ok:"):
+            return int(line.split()[-1])
+    raise RuntimeError("the reference graphlet app printed no count")

@@ -25,6 +25,10 @@ enough that a chain without the restart coin fails it.
 
 expected_visits_sources() is expected_visits for many sources in one dense
 propagation; pinned by equality with expected_visits source by source.
+
+return_at() gives graphlet's closed-walk probability per source; pinned by a
+directed triangle (q^3 at L = 4, 0 at L = 3), brute-force path enumeration
+on a small graph, and the bit-exact replay's counts within the binomial bound.
 """
 from __future__ import annotations
 
@@ -170,3 +174,19 @@ def first_meeting(off, col, a: int, b: int, p: float, P: int) -> np.ndarray:
         out[t] = np.trace(J)
         np.fill_diagonal(J, 0.0)
     return out
+
+
+def return_at(off, col, L: int, p: float) -> np.ndarray:
+    """Per source s, the probability that a STOP walk from s (follow a uniform
+    CSR slot with probability 1-p, die at a dead end or on the coin) stands
+    on s again at hop L-1: graphlet.cpp:46-52's closed-walk indicator (its
+    triangle count for L = 4).  diag(M^(L-1)) with M = (1-p) T."""
+    P, _ = transition(off, col)
+    M = ((1.0 - p) * P).tocsr()
+    n = M.shape[0]
+    if L <= 1:
+        return np.ones(n)
+    A = sp.identity(n, format="csr")
+    for _ in range(L - 2):
+        A = (A @ M).tocsr()
+    return np.asarray(A.multiply(M.T.tocsr()).sum(axis=1)).ravel()

@@ -111,3 +111,25 @@ def test_reference_rawrandomwalks_unchanged(graph):
                   "L", "10", "blocksize_kb", "16"], d)
     m = re.search(r"\[walk_steps\] (\d+)", stdout)
     assert m and int(m.group(1)) == 20000 * 10, stdout[-2000:]
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "graphlet"), os.X_OK), reason="compat apps not built")
+def test_reference_graphlet_unchanged(tmp_path):
+    """apps/graphlet.cpp of the reference, compiled unchanged against the
+    front end: its closed-walk ratio (rand() sources, so distributional) within
+    5 sigma of the FP64 probability averaged over uniform sources."""
+    from oracle import fp64
+    (tmp_path / "conf").mkdir()
+    (tmp_path / "conf" / "graphwalker.cnf").write_text("# empty\n")
+    off, col = O.rmat_graph(6, 1 << 10, seed=4)
+    n = off.shape[0] - 1
+    base = str(tmp_path / "rmat6")
+    write_graph(base, off, col, 1024)
+    R, L = 200_000, 4
+    stdout = run([os.path.join(COMPAT, "graphlet"), "file", base, "N", str(n), "R", str(R), "L", str(L),
+                  "blocksize_kb", "1024"], tmp_path)
+    m = re.search(r"cnt_ok:\s+(\d+)", stdout)
+    assert m, stdout[-2000:]
+    f = float(fp64.return_at(off, col, L, 0.15).mean())
+    got = int(m.group(1)) / R
+    assert abs(got - f) <= 5 * np.sqrt(f * (1 - f) / R), (got, f)

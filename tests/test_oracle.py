@@ -489,3 +489,66 @@ def test_fp64_sources_equals_single_source():
         for k, s_ in enumerate(src):
             one = fp64.expected_visits(kind, off, col, int(s_), 300.0, 9, 0.2)
             assert np.allclose(many[k], one, rtol=1e-12, atol=1e-9), (kind, k)
+
+
+def test_fp64_return_at_closed_forms():
+    # a directed triangle 0 -> 1 -> 2 -> 0: every walk is back at hop 3 with
+    # probability q^3, never at hop 2
+    off = np.array([0, 1, 2, 3], dtype=np.uint64)
+    col = np.array([1, 2, 0], dtype=np.uint32)
+    q = 0.85
+    assert np.allclose(fp64.return_at(off, col, 4, 0.15), q ** 3, rtol=0, atol=1e-15)
+    assert np.allclose(fp64.return_at(off, col, 3, 0.15), 0.0, atol=1e-15)
+    # brute force over every path of length 3 on a small random graph (CSR
+    # slots with multiplicity, dead ends stop the walk)
+    roff, rcol = O.rmat_graph(4, 60, seed=3)
+    n = roff.shape[0] - 1
+    deg = np.diff(roff.astype(np.int64))
+    exp = np.zeros(n)
+    for s in range(n):
+        def walk(v, k, prob):
+            if k == 3:
+                if v == s:
+                    exp[s] += prob
+                return
+            if deg[v] == 0:
+                return
+            for e in range(int(roff[v]), int(roff[v + 1])):
+                walk(int(rcol[e]), k + 1, prob * q / deg[v])
+        walk(s, 0, 1.0)
+    assert np.allclose(fp64.return_at(roff, rcol, 4, 0.15), exp, rtol=1e-12, atol=1e-15)
+
+
+def test_fp64_return_at_matches_replay():
+    # the bit-exact replay's closed walks (a visit to the source at hop L-1)
+    # within 5 sigma of the binomial bound, which a chain without the stop
+    # coin would miss
+    off, col = O.rmat_graph(6, 1 << 10, seed=4)
+    n = off.shape[0] - 1
+    L, W = 4, 40_000
+    prog = O.Program(O.STOP, L, int(0.15 * 2 ** 32), n, 0, 9)
+    p = prog.stop_thr / 2.0 ** 32
+    starts = (np.arange(W) % n).astype(np.uint32)
+    w = O.make_walks(starts, sources_=starts)
+    closed = 0
+    for i in range(W):
+        tv, th, _ = O.trace(prog, off, col, w[i])
+        closed += int(np.any((th == L - 1) & (tv == starts[i])))
+    f = float(fp64.return_at(off, col, L, p)[starts].mean())
+    tol = 5 * np.sqrt(f * (1 - f) / W)
+    assert abs(closed / W - f) <= tol, (closed / W, f)
+    assert abs(closed / W - float(fp64.return_at(off, col, L, 0.0)[starts].mean())) > tol
+
+
+@pytest.mark.skipif(O.reference_app("graphlet") is None, reason="reference apps not built (oracle/build_ref.sh)")
+def test_reference_engine_graphlet_within_mc_bound(tmp_path):
+    # the reference CPU engine's closed-walk ratio (its own rand() sources and
+    # rand_r coins) against the FP64 probability averaged over uniform sources
+    off, col = O.rmat_graph(6, 1 << 10, seed=4)
+    n = off.shape[0] - 1
+    base = str(tmp_path / "g")
+    O.write_reference_graph(base, off, col)
+    R, L = 200_000, 4
+    cnt = O.run_reference_graphlet(base, n, R, L, 8, timeout=120)
+    f = float(fp64.return_at(off, col, L, 0.15).mean())
+    assert abs(cnt / R - f) <= 5 * np.sqrt(f * (1 - f) / R), (cnt / R, f)
