@@ -133,3 +133,35 @@ def test_reference_graphlet_unchanged(tmp_path):
     f = float(fp64.return_at(off, col, L, 0.15).mean())
     got = int(m.group(1)) / R
     assert abs(got - f) <= 5 * np.sqrt(f * (1 - f) / R), (got, f)
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "rwdomination"), os.X_OK), reason="compat apps not built")
+def test_reference_rwdomination_unchanged(graph):
+    """apps/rwdomination.cpp unchanged: R jump walks of L steps from every
+    vertex, N x R x L steps exactly."""
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    R, L = 3, 6
+    stdout = run([os.path.join(COMPAT, "rwdomination"), "file", base, "N", str(n), "R", str(R), "L", str(L),
+                  "blocksize_kb", "16"], d)
+    m = re.search(r"\[walk_steps\] (\d+)", stdout)
+    assert m and int(m.group(1)) == n * R * L, stdout[-2000:]
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "kk_ppr"), os.X_OK), reason="compat apps not built")
+def test_reference_kk_ppr_unchanged(graph):
+    """apps/kk_ppr.cpp unchanged: walkspersource uncapped (prob) walks from
+    each of numsources consecutive sources; the step total is the oracle's
+    for the same walks (source index = field, ids in the packed records'
+    sorted order, seed 1)."""
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    first, ns, wps = 100, 40, 250
+    stdout = run([os.path.join(COMPAT, "kk_ppr"), "file", base, "firstsource", str(first), "numsources", str(ns),
+                  "walkspersource", str(wps), "maxwalklength", "30", "blocksize_kb", "1024"], d)
+    steps = re.search(r"\[walk_steps\] (\d+)", stdout)
+    assert steps, stdout[-2000:]
+    k = np.repeat(np.arange(ns, dtype=np.uint32), wps)
+    walks = O.make_walks(first + k, sources_=k, ids=np.arange(ns * wps))
+    ref = O.replay(O.program(O.PROB, 30, 0.15, n, seed=1), off, col, walks)
+    assert int(steps.group(1)) == ref["steps"]
