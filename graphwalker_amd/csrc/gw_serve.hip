@@ -529,6 +529,8 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
                   uint32_t max_queries, uint32_t hops_per_step) {
     if (!ctx || !prog || pool_walks == 0 || max_queries == 0 || hops_per_step == 0 || arena_bytes < 4096)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad arguments");
+    if (max_queries > 65535)  // completion launches one grid row per query
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_open: max_queries must be at most 65535");
     if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, ctx->host_graph ? "gw_serve_open: serving needs a device-resident graph" : "gw_serve_open: no graph");
     if (prog->kind != GW_RESTART && prog->kind != GW_STOP && prog->kind != GW_JUMP)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: serving runs RESTART, STOP or JUMP programs");

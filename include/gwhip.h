@@ -336,6 +336,8 @@ typedef struct gw_serve_stats {
     double step_ms;           /* step kernel time, HIP events on the engine stream */
 } gw_serve_stats;
 
+/* max_queries: queries in flight at once, 1 .. 65535 (completed queries are
+ * processed one workgroup row per query). */
 int gw_serve_open(gw_ctx *ctx, const gw_program *prog, uint64_t pool_walks, uint64_t arena_bytes,
                   uint32_t max_queries, uint32_t hops_per_step);
 int gw_serve_submit(gw_ctx *ctx, const gw_query *queries, uint32_t n);

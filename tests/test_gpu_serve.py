@@ -92,6 +92,8 @@ def test_serve_rejects_bad_queries(engine, g12):
             engine.serve_submit([(1, 5, 5000)])  # more walks than the pool
     finally:
         engine.serve_close()
+    with pytest.raises(gw.GwError, match="at most 65535"):
+        engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), 1000, 1 << 20, 70_000, 4)
 
 
 @pytest.mark.parametrize("k", [1, 10, 1024])
