@@ -227,7 +227,7 @@ struct ServeRec {
             arena_add(have, es, rb, rm, ev, ec);
         }
         __builtin_amdgcn_wave_barrier();
-        if (__lane_id() == __ffsll(static_cast<unsigned long long>(active)) - 1) cache->fill = 0;
+        if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(active)) - 1)) cache->fill = 0;
         __builtin_amdgcn_wave_barrier();
     }
 
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
     ServeCache* cache = &s_cache[threadIdx.x >> 6];
     serve_cache_init(cache, lane);
     ServeRec rec{arena, srccount, occ, occn, 0, 0, cbits, cnt + 3, 0, 0, slots, cache};
-    State s{0, 0, 0, 0, 0, 0};
+    State s{};
     for (;;) {
         uint64_t idle = __ballot(!active);
         while (idle && !exhausted) {
@@ -637,7 +637,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (S.live) {
         const Prog p{L, S.prog.nvertices, S.prog.stop_thr, static_cast<uint32_t>(S.prog.seed),
-                     static_cast<uint32_t>(S.prog.seed >> 32), 0u};
+                     static_cast<uint32_t>(S.prog.seed >> 32), 0u, 0u, 0u};
         const uint32_t grid = std::max(1u, ctx->cu_count) * 4u * 8u * 64u / 256u;
         hipError_t e = S.prog.kind == GW_RESTART ? launch_serve<GW_RESTART>(ctx, S, p, S.live, grid)
                      : S.prog.kind == GW_STOP    ? launch_serve<GW_STOP>(ctx, S, p, S.live, grid)

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
     bool active = false;
-    State s{0, 0, 0, 0, 0, 0};
+    State s{};
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     bool active = false;
     uint64_t slot = 0;
     uint32_t lo = 0, hi = 0, visits0 = 0;
-    State s{0, 0, 0, 0, 0, 0};
+    State s{};
     Graph gl = g;  // the graph as this lane's walk sees it (its block's view when streaming)
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
@@ -876,8 +876,14 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
     L.table = SourceTable{ctx->d_table, ctx->d_counters + kCntTableOverflow, ctx->table_slots - 1, 64u - ctx->table_log2,
                           ctx->table_vbits, ctx->table_cbits, ctx->table_wide ? 1u : 0u};
-    L.klog = KeyLog{ctx->d_keys, ctx->d_counters + kCntLogCursor, ctx->d_counters + kCntLogOverflow, ctx->keys_cap, ctx->keys_sentinel,
-                    ctx->table_vbits, keylog_slab};
+    L.klog = KeyLog{};  // region fields stay null unless the run fills regions directly
+    L.klog.keys = ctx->d_keys;
+    L.klog.cursor = ctx->d_counters + kCntLogCursor;
+    L.klog.overflow = ctx->d_counters + kCntLogOverflow;
+    L.klog.cap = ctx->keys_cap;
+    L.klog.sentinel = ctx->keys_sentinel;
+    L.klog.vbits = ctx->table_vbits;
+    L.klog.slab = keylog_slab;
     L.no_final = (prog->recorder & GW_REC_NO_FINAL) != 0;
     L.tb = prog->block_threads ? prog->block_threads : 256u;
     if (L.tb % 64 != 0 || L.tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run: block_threads must be a multiple of 64, at most 256");

@@ -77,7 +77,6 @@ constexpr uint32_t kAggThreads = 1024;
 constexpr uint32_t kAggWaves = kAggThreads / 64;
 constexpr uint32_t kAggPerThread = 4;
 constexpr uint32_t kAggTile = kAggThreads * kAggPerThread;   // keys loaded per iteration
-constexpr uint32_t kAggMaxFill = kAggSlots * 3 / 4;          // the table is never fuller than this
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;                   // never a vertex id (n <= 2^32 - 1)
 constexpr uint32_t kPartPerLane = 4;                         // log slots per lane per partition sweep
 
