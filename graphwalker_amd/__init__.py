@@ -233,6 +233,7 @@ class Engine:
         self.m = 0
         self.nwalks = 0
         self._host_graph = None
+        self._meet_buf = None  # pinned output of meeting_counts, grown as needed
 
     def close(self) -> None:
         if self._ctx:
@@ -368,9 +369,12 @@ class Engine:
     def meeting_counts(self, npairs: int, R: int, L: int) -> np.ndarray:
         """First-meeting counts per pair and offset (npairs x L) of the last
         GW_RESTART_JOINT run with GW_REC_EVENTS over walks_init_pairs walks."""
-        out = np.zeros((npairs, L), dtype=np.uint32)
-        self._check(self._L.gw_meeting_counts(self._ctx, npairs, R, L, _ptr(out)), "gw_meeting_counts")
-        return out
+        n = npairs * L
+        if self._meet_buf is None or self._meet_buf.shape[0] < n:
+            self._meet_buf = self.host_empty((n,), np.uint32)  # pinned: the copy runs at link speed
+        buf = self._meet_buf[:n]
+        self._check(self._L.gw_meeting_counts(self._ctx, npairs, R, L, _ptr(buf)), "gw_meeting_counts")
+        return buf.reshape(npairs, L).copy()
 
     def walks_final(self) -> np.ndarray:
         """Final walk records, indexed by walk id."""
