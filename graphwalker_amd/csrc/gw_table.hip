@@ -77,6 +77,8 @@ constexpr uint32_t kAggThreads = 1024;
 constexpr uint32_t kAggWaves = kAggThreads / 64;
 constexpr uint32_t kAggPerThread = 4;
 constexpr uint32_t kAggTile = kAggThreads * kAggPerThread;   // keys loaded per iteration
+// A flushed table takes a whole tile plus the designated vertex below its 3/4 fill bound.
+static_assert(kAggTile + 1 <= kAggSlots / 4 * 3, "a tile must fit an emptied table");
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;                   // never a vertex id (n <= 2^32 - 1)
 constexpr uint32_t kPartPerLane = 4;                         // log slots per lane per partition sweep
 
