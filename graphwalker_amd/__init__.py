@@ -187,6 +187,7 @@ def lib() -> C.CDLL:
         "gw_run": ([P, P, P], i),
         "gw_run_capture": ([P, P], i),
         "gw_run_replay": ([P, P], i),
+        "gw_run_noalloc": ([P, P, P, u64, P, P, P], i),
         "gw_autotune": ([P, P, u32, P, P], i),
         "gw_bucket": ([P, P], i),
         "gw_counts_read": ([P, P, u32], i),

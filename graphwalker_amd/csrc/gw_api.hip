@@ -204,6 +204,11 @@ int gw_create(gw_ctx** out, const gw_cfg* cfg) {
         gw_destroy(ctx);
         return GW_ENOMEM;
     }
+    // the device address of gw_run_noalloc's static scratch, with this device current
+    if (gw_noalloc_init(ctx) != GW_OK) {
+        gw_destroy(ctx);
+        return GW_EHIP;
+    }
     *out = ctx;
     return GW_OK;
 }

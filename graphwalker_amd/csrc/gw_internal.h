@@ -30,6 +30,7 @@ constexpr uint32_t kCntSplitPairs = 17;       // partitioned: partial pairs of f
 constexpr uint32_t kCntFlushed = 18;          // partitioned: flushed sources
 constexpr uint32_t kCntSplitRuns = 19;        // partitioned: sums of the split pairs
 constexpr uint32_t kCntAggQueue = 20;         // partitioned: next source for the aggregation workgroups
+constexpr uint32_t kCntBadWalks = 21;         // persistent schedule: walks skipped at pickup (vertex or restart source not in the graph)
 
 struct gw_run_graph;  // gw_step.hip: a captured run (gw_run_capture)
 
@@ -175,6 +176,10 @@ struct gw_ctx {
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
     gw_run_graph* run_graph = nullptr;  // the last gw_run_capture
+
+    // gw_run_noalloc: device address of this device's static scratch array
+    // (hipGetSymbolAddress at gw_create)
+    unsigned long long* d_scratch = nullptr;
 };
 
 // Workgroups for a grid-stride loop over n items: at most 2^20, so the
@@ -218,6 +223,9 @@ int gw_pool_prepare(gw_ctx* ctx, uint32_t rec, uint32_t event_cap);
 
 // gw_step.hip
 int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats);
+// gw_step.hip: looks up (once per device) the device address of the static
+// scratch array gw_run_noalloc uses and keeps it in ctx->d_scratch.
+int gw_noalloc_init(gw_ctx* ctx);
 void gw_run_graph_free(gw_ctx* ctx);
 // Builds the successor-header edges if they fit; false when the CSR is the only layout.
 bool gw_ensure_ext(gw_ctx* ctx);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "gw_device.h"
@@ -127,8 +128,18 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             const uint64_t avail = chunk_end - chunk_next;
             const uint32_t take = avail < want ? static_cast<uint32_t>(avail) : want;
             const uint32_t r = rank_in(idle);
+            // A walk whose vertex (or restart target) is not in the graph is
+            // skipped and counted: gw_run checks walks on the host, but the
+            // walks of gw_run_noalloc are device data it never reads.
+            constexpr bool kRestarts = KIND == GW_RESTART || KIND == GW_RESTART_JOINT;
+            gw_walk w{};
+            bool bad = false;
             if (!active && r < take) {
-                const gw_walk w = walks[chunk_next + r];
+                w = walks[chunk_next + r];
+                bad = w.vertex >= g.n || (kRestarts && w.source >= g.n);
+                if (bad) atomicAdd(&counters[kCntBadWalks], 1ull);
+            }
+            if (!active && r < take && !bad) {
                 s.id = w.id;
                 rec.id = w.id - p.id_base;
                 rec.source = w.source;
@@ -1384,3 +1395,138 @@ int gw_autotune(gw_ctx* ctx, const gw_program* prog, uint32_t reps, gw_program* 
 }
 
 }  // extern "C"
+
+// ---- gw_run_noalloc: the persistent schedule, allocating nothing ---------
+//
+// The run's device counters (steps, visits, the walk-claim cursor, skipped
+// walks) live in a file-scope static array instead of the context's
+// allocation.  Host code never passes the array's name to a kernel or a copy
+// (in host code it is a host-side stand-in); it uses the device address
+// hipGetSymbolAddress returns, looked up once per device by gw_create with
+// that device current.  Each device holds its own copy; every call on one
+// device shares it, so calls that overlap on one device race.
+namespace gw {
+namespace {
+
+constexpr uint32_t kScratchWords = kCounters;  // 32 x u64 = 256 bytes per device
+constexpr int kScratchDevices = 64;
+
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kScratchDevices] = {};
+
+// totals[0..2] = steps, visits, skipped walks of the run.
+__global__ void noalloc_totals_kernel(const unsigned long long* __restrict__ cnt, unsigned long long* __restrict__ totals) {
+    if (threadIdx.x == 0) {
+        totals[0] = cnt[kCntSteps];
+        totals[1] = cnt[kCntVisits];
+        totals[2] = cnt[kCntBadWalks];
+    }
+}
+
+template <int KIND>
+hipError_t launch_noalloc(const Prog& p, const Graph& g, bool ext, bool vc, uint32_t grid, uint32_t tb, hipStream_t st,
+                          const gw_walk* walks, uint64_t nw, uint32_t* counts, unsigned long long* cnt) {
+    const EventLog log{nullptr, 0};
+    const SourceTable table{};
+    const KeyLog klog{};
+    constexpr int VC = GW_REC_VERTEX_COUNT;
+    if (ext && vc)
+        step_persistent_kernel<KIND, VC, true><<<dim3(grid), dim3(tb), 0, st>>>(p, g, walks, nullptr, nw, counts, nullptr,
+                                                                            nullptr, cnt, log, table, klog);
+    else if (ext)
+        step_persistent_kernel<KIND, 0, true><<<dim3(grid), dim3(tb), 0, st>>>(p, g, walks, nullptr, nw, nullptr, nullptr,
+                                                                           nullptr, cnt, log, table, klog);
+    else if (vc)
+        step_persistent_kernel<KIND, VC, false><<<dim3(grid), dim3(tb), 0, st>>>(p, g, walks, nullptr, nw, counts, nullptr,
+                                                                             nullptr, cnt, log, table, klog);
+    else
+        step_persistent_kernel<KIND, 0, false><<<dim3(grid), dim3(tb), 0, st>>>(p, g, walks, nullptr, nw, nullptr, nullptr,
+                                                                            nullptr, cnt, log, table, klog);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace gw
+
+static __device__ __align__(256) unsigned long long scratch[gw::kScratchWords];
+
+int gw_noalloc_init(gw_ctx* ctx) {
+    using namespace gw;
+    if (ctx->device < 0 || ctx->device >= kScratchDevices)
+        return gw_fail(ctx, GW_EINVAL, "gw_create: device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return gw_hip_check(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[ctx->device] = static_cast<unsigned long long*>(ptr);
+    }
+    ctx->d_scratch = g_scratch_ptr[ctx->device];
+    return GW_OK;
+}
+
+extern "C" int gw_run_noalloc(gw_ctx* ctx, const gw_program* prog, const gw_walk* walks, uint64_t nwalks,
+                              uint32_t* counts, unsigned long long* totals, void* stream) {
+    using namespace gw;
+    if (!ctx || !prog || !totals || (nwalks && !walks)) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: bad arguments");
+    if (!ctx->d_scratch) return gw_fail(ctx, GW_ESTATE, "gw_run_noalloc: no scratch address for this device");
+    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_run_noalloc: needs a device-resident graph");
+    if (nwalks > 0xFFFFFFFFull) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: at most 2^32 - 1 walks per call");
+    if (prog->kind > GW_RESTART) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: unknown program kind");
+    if (prog->L == 0 && prog->kind != GW_PROB) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: L must be >= 1");
+    if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: stop_thr above 2^32");
+    if (prog->kind == GW_PROB && prog->stop_thr < (1ull << 20))
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: PROB needs stop_thr >= 2^20");
+    if (prog->kind == GW_JUMP && (prog->nvertices == 0 || prog->nvertices > ctx->n))
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: JUMP needs nvertices in [1, n]");
+    const uint32_t rec = prog->recorder & ~GW_REC_NO_FINAL;
+    if (rec & ~GW_REC_VERTEX_COUNT)
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: records vertex counts only (GW_REC_VERTEX_COUNT or none)");
+    const bool vc = rec == GW_REC_VERTEX_COUNT;
+    if (vc != (counts != nullptr))
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: counts is required with GW_REC_VERTEX_COUNT and only then");
+    if (prog->schedule != GW_SCHED_AUTO && prog->schedule != GW_SCHED_PERSISTENT)
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: runs the persistent schedule");
+    // the successor-header layout only when an earlier run built it (building allocates)
+    if (prog->layout == GW_LAYOUT_EXT && !ctx->d_ext)
+        return gw_fail(ctx, GW_ESTATE, "gw_run_noalloc: successor-header edges not built (run gw_run with GW_LAYOUT_EXT first)");
+    if (prog->layout > GW_LAYOUT_EXT) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: unknown layout");
+    const bool ext = prog->layout != GW_LAYOUT_CSR && ctx->d_ext != nullptr;
+    const uint32_t chunk = prog->walk_chunk ? prog->walk_chunk : kChunk;
+    if (chunk % 64 != 0 || chunk > 8192)
+        return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: walk_chunk must be a multiple of 64, at most 8192");
+    const uint32_t tb = prog->block_threads ? prog->block_threads : 256u;
+    if (tb % 64 != 0 || tb > 256) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: block_threads must be a multiple of 64, at most 256");
+    const uint32_t wps = prog->waves_per_simd ? prog->waves_per_simd : 8u;
+    if (wps > 32) return gw_fail(ctx, GW_EINVAL, "gw_run_noalloc: waves_per_simd must be at most 32");
+    const uint64_t waves = static_cast<uint64_t>(std::max(1u, ctx->cu_count)) * 4u * wps;
+    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, waves * 64 / tb));
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    unsigned long long* cnt = ctx->d_scratch;
+    // the counters start at zero; fill kernels (not memset nodes) keep the
+    // sequence capturable into a HIP graph
+    fill_u64_kernel<<<dim3(1), dim3(64), 0, st>>>(cnt, kScratchWords);
+    GW_HIP(ctx, hipGetLastError());
+    if (vc) {
+        fill_u32_kernel<<<dim3(gw_grid(ctx->n, 256)), dim3(256), 0, st>>>(counts, ctx->n);
+        GW_HIP(ctx, hipGetLastError());
+    }
+    if (nwalks) {
+        const Prog p{prog->L, prog->nvertices, prog->stop_thr, static_cast<uint32_t>(prog->seed),
+                     static_cast<uint32_t>(prog->seed >> 32), prog->stream, 0u, chunk};
+        const Graph g{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+        hipError_t e;
+        switch (prog->kind) {
+        case GW_JUMP: e = launch_noalloc<GW_JUMP>(p, g, ext, vc, grid, tb, st, walks, nwalks, counts, cnt); break;
+        case GW_STOP: e = launch_noalloc<GW_STOP>(p, g, ext, vc, grid, tb, st, walks, nwalks, counts, cnt); break;
+        case GW_PROB: e = launch_noalloc<GW_PROB>(p, g, ext, vc, grid, tb, st, walks, nwalks, counts, cnt); break;
+        case GW_RESTART_JOINT: e = launch_noalloc<GW_RESTART_JOINT>(p, g, ext, vc, grid, tb, st, walks, nwalks, counts, cnt); break;
+        default: e = launch_noalloc<GW_RESTART>(p, g, ext, vc, grid, tb, st, walks, nwalks, counts, cnt); break;
+        }
+        if (e != hipSuccess) return gw_hip_check(ctx, e, "gw_run_noalloc: step kernel launch");
+    }
+    noalloc_totals_kernel<<<dim3(1), dim3(64), 0, st>>>(cnt, totals);
+    GW_HIP(ctx, hipGetLastError());
+    return GW_OK;
+}

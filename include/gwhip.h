@@ -285,6 +285,24 @@ int gw_run_replay(gw_ctx *ctx, gw_stats *stats);
  * of these. */
 int gw_autotune(gw_ctx *ctx, const gw_program *prog, uint32_t reps, gw_program *best, double *best_ms);
 
+/* gw_run without allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  It runs `prog` with the persistent
+ * schedule over `nwalks` walk records the caller holds in device memory, on
+ * the context's device-resident graph (the CSR, or the successor-header
+ * edges when an earlier gw_run built them), and enqueues everything on
+ * `stream` (a hipStream_t; NULL = the context's stream) without waiting:
+ *   counts  device u32[n], zeroed then filled (GW_REC_VERTEX_COUNT); NULL
+ *           with GW_REC_NONE.  No other recorder, and no final records.
+ *   totals  device u64[3]: steps, visits, walks skipped because their vertex
+ *           (or a restart program's source field) is not a graph vertex.
+ * Results equal gw_run's on the same walks (persistent schedule).  Its
+ * counters live in a static 256-byte device array per device, shared by
+ * every call on that device: calls that overlap on one device race.
+ * nwalks above 2^32 - 1 is rejected (GW_EINVAL). */
+int gw_run_noalloc(gw_ctx *ctx, const gw_program *prog, const gw_walk *walks, uint64_t nwalks,
+                   uint32_t *counts, unsigned long long *totals, void *stream);
+
 /* Stable counting sort of the loaded walks by block of their vertex (out of
  * range vertices are dropped); bucket_counts[nblocks] receives the sizes. */
 int gw_bucket(gw_ctx *ctx, uint64_t *bucket_counts);
