@@ -83,6 +83,11 @@ class ServeStats(C.Structure):
         ("active_queries", C.c_uint32),
         ("queued", C.c_uint32),
         ("step_ms", C.c_double),
+        ("h2d_bytes", C.c_uint64),
+        ("pages_loaded", C.c_uint32),
+        ("pages_resident", C.c_uint32),
+        ("sampled_reads", C.c_uint64),
+        ("sampled_resident", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -174,6 +179,7 @@ def lib() -> C.CDLL:
         "gw_host_alloc": ([P, u64, P], i),
         "gw_host_free": ([P, P], i),
         "gw_graph_rmat": ([P, u32, u64, u64, u32, u32, u32], i),
+        "gw_graph_rmat_host": ([P, u32, u64, u64, u32, u32, u32, u64, u64, P], i),
         "gw_graph_blocks": ([P, P, u32], i),
         "gw_graph_info": ([P, P, P, P], i),
         "gw_graph_read": ([P, P, P], i),
@@ -207,6 +213,7 @@ def lib() -> C.CDLL:
         "gw_serve_result": ([P, u64, P, P, u64], i),
         "gw_serve_topk": ([P, u32], i),
         "gw_serve_close": ([P], i),
+        "gw_serve_cache": ([P, u32, u32], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -300,6 +307,18 @@ class Engine:
         ta, tab, tabc = rmat_thresholds(a, b, c)
         self._check(self._L.gw_graph_rmat(self._ctx, scale, m, seed, ta, tab, tabc), "gw_graph_rmat")
         self.n, self.m = 1 << scale, m
+
+    def graph_rmat_host(self, scale: int, m: int, seed: int, cache_bytes: int, work_bytes: int = 0,
+                        a: float = 0.57, b: float = 0.19, c: float = 0.19) -> float:
+        """The graph_rmat graph built into pinned host memory (host-resident,
+        device block cache of cache_bytes); returns the build time in ms."""
+        ta, tab, tabc = rmat_thresholds(a, b, c)
+        ms = C.c_double()
+        self._check(self._L.gw_graph_rmat_host(self._ctx, scale, m, seed, ta, tab, tabc, cache_bytes, work_bytes,
+                                                C.byref(ms)), "gw_graph_rmat_host")
+        self.n, self.m = 1 << scale, m
+        self._host_graph = None
+        return ms.value
 
     def graph_transpose(self) -> None:
         """Reverse every edge of the device-resident graph."""
@@ -512,6 +531,11 @@ class Engine:
     def serve_topk(self, k: int) -> None:
         """Completed queries keep their k most-visited vertices (0 = all pairs)."""
         self._check(self._L.gw_serve_topk(self._ctx, k), "gw_serve_topk")
+
+    def serve_cache(self, page_log2: int = 0, max_loads: int = 0) -> None:
+        """Page size and page copies in flight of a server over a host-resident
+        graph (call before serve_open; 0 = defaults)."""
+        self._check(self._L.gw_serve_cache(self._ctx, page_log2, max_loads), "gw_serve_cache")
 
     def serve_close(self) -> None:
         self._check(self._L.gw_serve_close(self._ctx), "gw_serve_close")

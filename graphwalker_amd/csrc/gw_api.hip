@@ -297,6 +297,15 @@ int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
     return single_block(ctx);
 }
 
+int gw_graph_rmat_host(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
+                       uint32_t tabc, uint64_t cache_bytes, uint64_t work_bytes, double* build_ms) {
+    if (!ctx || scale == 0 || scale > 31 || cache_bytes == 0 || !(ta <= tab && tab <= tabc && tabc <= 65536))
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_rmat_host: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_graph_free(ctx);
+    return gw_rmat_build_host(ctx, scale, m, seed, ta, tab, tabc, cache_bytes, work_bytes, build_ms);
+}
+
 int gw_graph_transpose(gw_ctx* ctx) {
     if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_transpose: needs a device-resident graph");
     GW_HIP(ctx, hipSetDevice(ctx->device));

@@ -357,7 +357,7 @@ struct Recorder {
 //   GW_STOP/GW_PROB  die                                randomwalkwithstop.hpp:39-41
 //   GW_RESTART_JOINT restart at source; end when hop % L == L-1
 //                                                       randomwalkwithrestartwithjoint.hpp:47-53
-//   GW_RESTART       restart at source, cap L           randomwalkwithrestart.hpp:163-176
+//   GW_RESTART       restart at source, cap L           randomwalkwithrestart.hpp:33-63
 // The loop guard (vertex in block, hop < L) is the while-condition of the
 // reference loops (randomwalkwithjump.hpp:42).
 template <int KIND, class Rec>
@@ -419,13 +419,36 @@ __device__ __forceinline__ void load_row(const Graph& g, State& s) {
     }
 }
 
+// Where the CSR layout's column of edge e is read from: the resident column
+// array (PlainCols), or a page table for a graph whose columns stay in host
+// memory (PagedCols: page e >> shift is a device cache slot or the pinned
+// host array itself, each pointer rebased so that it takes the absolute edge
+// position).  PagedCols also counts a 1-in-64 sample of the reads per page,
+// the demand the host's page cache policy follows.
+struct PlainCols {
+    __device__ __forceinline__ uint32_t operator()(const Graph& g, uint64_t e, uint32_t, uint32_t) const {
+        return g.col[e];
+    }
+};
+
+struct PagedCols {
+    const uint32_t* const* __restrict__ pages;
+    uint32_t* __restrict__ hits;
+    uint32_t shift;
+    __device__ __forceinline__ uint32_t operator()(const Graph&, uint64_t e, uint32_t id, uint32_t hop) const {
+        const uint64_t pg = e >> shift;
+        if (((id + hop * 0x9E3779B9u) >> 26) == 0) atomicAdd(&hits[pg], 1u);
+        return pages[pg][e];
+    }
+};
+
 // One step of a walk (its state set up by start_walk).  Returns true when the
 // walk has ended; the final vertex/hop stay in the state, as the reference
 // leaves them in the record it drops.  Else-branches per kind follow advance()
 // in gw_device.h.  The draw of the next hop depends only on (id, hop + 1), so
 // it is computed while the edge load is in flight.
-template <int KIND, bool EXT, class Rec>
-__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec) {
+template <int KIND, bool EXT, class Rec, class Cols = PlainCols>
+__device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& s, Rec& rec, const Cols& cols = Cols{}) {
     constexpr bool capped = (KIND == GW_JUMP || KIND == GW_STOP || KIND == GW_RESTART);
     if (capped && s.hop >= p.L) return true;
     const u32x4 r = s.r;
@@ -451,7 +474,7 @@ __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& 
             s.deg = ed.deg;
             s.row = ed.row;
         } else {
-            const uint32_t nv = g.col[e];
+            const uint32_t nv = cols(g, e, s.id, s.hop);
             rec.visit(s.v, s.hop);
             s.r = philox4x32_10(s.id, s.hop + 1, kTagWalk, p.stream, p.k0, p.k1);
             s.v = nv;

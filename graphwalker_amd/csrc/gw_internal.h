@@ -112,6 +112,8 @@ struct gw_ctx {
     uint32_t walk_v_max = 0;    // largest start vertex among loaded walks (checked against the graph per run)
     gw_pool* pool = nullptr;    // host walk pool (gw_walks_load_host), else null
     void* serve_state = nullptr; // the open server (gw_serve.hip), else null
+    uint32_t serve_page_log2 = 0;   // gw_serve_cache: column page size of a server over a host graph (0 = auto)
+    uint32_t serve_max_loads = 0;   // gw_serve_cache: page loads in flight at once (0 = 4)
     uint32_t* d_meet = nullptr; // gw_meeting_counts output, kept between calls
     uint64_t meet_bytes = 0;
     gw_walk* d_walks = nullptr;
@@ -212,6 +214,12 @@ void gw_free(gw_ctx* ctx, void* p, uint64_t bytes);
 int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
                   uint32_t tabc);
 
+// gw_rmat.hip: the same R-MAT graph built range by range into pinned host
+// memory the context owns, with at most work_bytes of device memory (0 =
+// most of what is free); the graph becomes host-resident.
+int gw_rmat_build_host(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
+                       uint32_t tabc, uint64_t cache_bytes, uint64_t work_bytes, double* build_ms);
+
 // gw_rmat.hip: reverse every edge of the device-resident CSR in place
 int gw_transpose_build(gw_ctx* ctx);
 
@@ -251,6 +259,9 @@ int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, do
 int gw_stream_prepare(gw_ctx* ctx);
 void gw_stream_reset(gw_ctx* ctx);
 void gw_host_graph_free(gw_ctx* ctx);
+// The CSR already in ctx->h_off / ctx->h_col becomes the context's
+// host-resident graph (vertex counts, default block table).
+int gw_host_graph_adopt(gw_ctx* ctx, uint32_t n, uint64_t m, uint64_t cache_bytes);
 // true when the pointer range lies inside one gw_host_alloc allocation
 bool gw_is_pinned(const gw_ctx* ctx, const void* p, uint64_t bytes);
 

@@ -9,11 +9,23 @@
 // slots and arena space (the HBM budget), first come first served.  The
 // random stream of a walk is keyed by (seed, query id; walk index, hop), so a
 // query's result does not depend on what else is in flight.
+//
+// Over a host-resident graph (gw_graph_set_host, gw_graph_rmat_host) the
+// offsets are copied to HBM and the columns stay in pinned host memory,
+// read through a page table: pages of 2^k edges that the policy keeps in a
+// device page cache (the graph's cache_bytes, offsets included), the others
+// read from host memory directly.  Between serve steps the policy follows
+// the kernel's sampled per-page demand and streams the most-read pages in
+// with hipMemcpyAsync on the copy stream while steps run (the reference's
+// block cache keeps the blocks with the most walks, graphwalker.hpp:168-251);
+// no walk waits for a page.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <deque>
 #include <map>
+#include <numeric>
+#include <set>
 #include <vector>
 
 #include "gw_device.h"
@@ -66,6 +78,31 @@ struct ServeState {
     unsigned long long* d_cand = nullptr;      // per-chunk top-k candidates of the completing queries
     uint64_t cand_cap = 0;                     // keys
     unsigned long long* d_cand_off = nullptr;  // max_queries: first candidate of each completing query
+    std::set<uint64_t> qids;                   // queued, in flight, or completed and not yet read
+    bool failed = false;                       // a step failed after its kernel ran: only close is allowed
+    // host-resident graph: offsets in HBM, columns paged
+    bool ooc = false;
+    uint64_t* d_off = nullptr;                 // n + 1 offsets
+    uint32_t page_log2 = 0;
+    uint64_t npages = 0;
+    const uint32_t* host_col = nullptr;        // device address of the pinned column array
+    const uint32_t** h_pages = nullptr;        // pinned page table: rebased slot pointer or host_col
+    const uint32_t** d_pages = nullptr;
+    uint32_t* d_page_hits = nullptr;           // sampled column reads per page in the last step
+    std::vector<uint32_t> hits;                // read back after each step
+    std::vector<double> score;                 // decayed demand
+    std::vector<int32_t> page_slot;            // slot holding the page, -1 = read from host memory
+    std::vector<int64_t> slot_page;            // page in (or loading into) the slot, -1 = free
+    std::vector<char> loading;                 // per page
+    uint32_t* d_cache = nullptr;               // slots x page edges
+    uint64_t nslots = 0;
+    uint32_t max_loads = 0;
+    struct Load { uint32_t page, slot; hipEvent_t done; };
+    std::vector<Load> inflight;
+    std::vector<hipEvent_t> free_events;
+    hipEvent_t table_ev = nullptr;
+    bool table_dirty = true;
+    uint64_t h2d_bytes = 0, pages_loaded = 0;
 };
 
 // The context's serving state (null when no server is open on it).
@@ -286,8 +323,8 @@ struct ServeRec {
 // One serve step: every walk of pool_in advances at most `hops` hops; walks
 // still alive are appended to pool_out; ended walks decrement their query's
 // remaining count.  Persistent wavefronts take chunks of the pool.
-template <int KIND, bool EXT>
-__global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const gw_walk* __restrict__ pool_in,
+template <int KIND, bool EXT, bool PAGED>
+__global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedCols pc, const gw_walk* __restrict__ pool_in,
                                                         uint64_t n, gw_walk* __restrict__ pool_out,
                                                         const SlotInfo* __restrict__ slots,
                                                         uint32_t* __restrict__ remaining,
@@ -350,7 +387,10 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, const 
         bool leave = false, done = false;
         if (active) {
             const uint32_t hop_before = s.hop;
-            done = step_once<KIND, EXT>(q, g, s, rec);
+            if constexpr (PAGED)
+                done = step_once<KIND, false>(q, g, s, rec, pc);
+            else
+                done = step_once<KIND, EXT>(q, g, s, rec);
             budget -= (s.hop != hop_before) ? 1u : 0u;
             leave = !done && budget == 0;
         }
@@ -498,22 +538,214 @@ __global__ __launch_bounds__(1024) void serve_topk_merge_kernel(const unsigned l
 
 template <int KIND>
 hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, uint32_t grid) {
-    const Graph g{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+    const Graph g = S.ooc ? Graph{S.d_off, nullptr, nullptr, ctx->n}
+                          : Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
+    const PagedCols pc{S.d_pages, S.d_page_hits, S.page_log2};
     // Walks claimed per wavefront at a time: while the pool fits the grid at
     // one walk per lane, 64, so every walk starts at once and a step lasts one
     // walk's hops; larger pools take up to 256 (fewer cursor atomics).
     const uint64_t waves = static_cast<uint64_t>(grid) * 4u;
     const uint64_t per_wave = (n + waves - 1) / waves;
     const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(256, std::max<uint64_t>(64, (per_wave + 63) / 64 * 64)));
-    if (S.ext)
-        serve_step_kernel<KIND, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
-            S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+    if (S.ooc)
+        serve_step_kernel<KIND, false, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
+            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+    else if (S.ext)
+        serve_step_kernel<KIND, true, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
+            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     else
-        serve_step_kernel<KIND, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn, S.cbits,
-            S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+        serve_step_kernel<KIND, false, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
+            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     return hipGetLastError();
+}
+
+// Count-region slots of a query: twice its most distinct vertices, rounded
+// up to a power of two.
+uint64_t region_slots(uint64_t walks, uint32_t L, uint32_t n) {
+    const uint64_t distinct = std::min<uint64_t>(walks * L, n);
+    uint64_t size = 64;
+    while (size < 2 * distinct) size <<= 1;
+    return size;
+}
+
+// ---- page cache of a server over a host-resident graph ------------------
+
+const uint32_t* page_base(const ServeState& S, uint64_t slot, uint64_t page) {
+    // rebased so that base[e] is column e for e in the page
+    const uintptr_t slot_addr = reinterpret_cast<uintptr_t>(S.d_cache) + (slot << S.page_log2) * 4;
+    return reinterpret_cast<const uint32_t*>(slot_addr - (page << S.page_log2) * 4);
+}
+
+uint64_t page_edges(const gw_ctx* ctx, const ServeState& S, uint64_t page) {
+    const uint64_t lo = page << S.page_log2;
+    return std::min<uint64_t>(1ull << S.page_log2, ctx->m - lo);
+}
+
+int ooc_open(gw_ctx* ctx, ServeState& S) {
+    S.ooc = true;
+    int unified = 0;
+    GW_HIP(ctx, hipDeviceGetAttribute(&unified, hipDeviceAttributeUnifiedAddressing, ctx->device));
+    if (!unified) return gw_fail(ctx, GW_EHIP, "gw_serve_open: the device cannot read pinned host memory in place");
+    const uint64_t off_bytes = (ctx->n + 1ull) * 8;
+    if (ctx->cache_bytes < off_bytes)
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: the graph's cache_bytes (" + std::to_string(ctx->cache_bytes) +
+                                           ") does not hold its offsets (" + std::to_string(off_bytes) + " bytes)");
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&S.d_off, off_bytes, "serve offsets"))) return rc;
+    GW_HIP(ctx, hipMemcpy(S.d_off, ctx->h_off, off_bytes, hipMemcpyHostToDevice));
+    uint32_t lg = ctx->serve_page_log2;
+    if (lg == 0) {  // pages of at least 2^10 edges, at most 4096 pages
+        lg = 10;
+        while (lg < 26 && (ctx->m >> lg) >= 4096) ++lg;
+    }
+    S.page_log2 = lg;
+    S.npages = (ctx->m + (1ull << lg) - 1) >> lg;
+    const uint64_t page_bytes = (1ull << lg) * 4;
+    S.nslots = std::min<uint64_t>(S.npages, (ctx->cache_bytes - off_bytes) / page_bytes);
+    S.max_loads = ctx->serve_max_loads ? ctx->serve_max_loads : 4u;
+    S.host_col = ctx->h_col;  // pinned memory, read in place over the host link
+    const uint64_t np = std::max<uint64_t>(1, S.npages);
+    GW_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&S.h_pages), np * sizeof(void*), hipHostMallocDefault));
+    if ((rc = gw_alloc(ctx, (void**)&S.d_pages, np * sizeof(void*), "serve page table")) ||
+        (rc = gw_alloc(ctx, (void**)&S.d_page_hits, np * 4, "serve page demand")))
+        return rc;
+    if (S.nslots && (rc = gw_alloc(ctx, (void**)&S.d_cache, S.nslots * page_bytes, "serve page cache"))) return rc;
+    S.hits.assign(S.npages, 0);
+    S.score.assign(S.npages, 0.0);
+    S.page_slot.assign(S.npages, -1);
+    S.loading.assign(S.npages, 0);
+    S.slot_page.assign(S.nslots, -1);
+    // the cache starts with the first pages (the rows of the lowest vertex
+    // ids); demand decides from the first step on
+    for (uint64_t pg = 0; pg < S.npages; ++pg) S.h_pages[pg] = S.host_col;
+    for (uint64_t sl = 0; sl < S.nslots; ++sl) {
+        GW_HIP(ctx, hipMemcpy(S.d_cache + (sl << lg), ctx->h_col + (sl << lg), page_edges(ctx, S, sl) * 4,
+                              hipMemcpyHostToDevice));
+        S.page_slot[sl] = static_cast<int32_t>(sl);
+        S.slot_page[sl] = static_cast<int64_t>(sl);
+        S.h_pages[sl] = page_base(S, sl, sl);
+    }
+    GW_HIP(ctx, hipMemcpy(S.d_pages, S.h_pages, np * sizeof(void*), hipMemcpyHostToDevice));
+    GW_HIP(ctx, hipMemset(S.d_page_hits, 0, np * 4));
+    S.table_dirty = false;
+    if (!ctx->copy_stream) GW_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    GW_HIP(ctx, hipEventCreateWithFlags(&S.table_ev, hipEventDisableTiming));
+    return GW_OK;
+}
+
+// Before a serve step's kernel (the engine stream is idle): loads that
+// finished enter the table; demand decays and the most-read pages that are
+// read from host memory replace the least-read resident ones (or fill free
+// slots), at most max_loads copies in flight; the table goes to the device
+// and the copies start once it is there, so no kernel reads a slot while it
+// is overwritten.  *started = bytes of the loads started.
+int ooc_before_step(gw_ctx* ctx, ServeState& S, uint64_t* started, uint32_t* nstarted) {
+    *started = 0;
+    *nstarted = 0;
+    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < S.inflight.size();) {
+        const hipError_t q = hipEventQuery(S.inflight[i].done);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        if (q != hipSuccess) return gw_hip_check(ctx, q, "serve page load");
+        const ServeState::Load ld = S.inflight[i];
+        S.page_slot[ld.page] = static_cast<int32_t>(ld.slot);
+        S.loading[ld.page] = 0;
+        S.h_pages[ld.page] = page_base(S, ld.slot, ld.page);
+        S.table_dirty = true;
+        S.free_events.push_back(ld.done);
+        S.inflight.erase(S.inflight.begin() + i);
+    }
+    for (uint64_t pg = 0; pg < S.npages; ++pg) S.score[pg] = S.score[pg] * 0.75 + S.hits[pg];
+    std::vector<ServeState::Load> start;
+    if (S.nslots && S.inflight.size() < S.max_loads) {
+        std::vector<uint32_t> cand, res;
+        for (uint64_t pg = 0; pg < S.npages; ++pg) {
+            if (S.loading[pg]) continue;
+            if (S.page_slot[pg] < 0) {
+                if (S.score[pg] > 0) cand.push_back(static_cast<uint32_t>(pg));
+            } else {
+                res.push_back(static_cast<uint32_t>(pg));
+            }
+        }
+        const size_t want = std::min<size_t>(cand.size(), S.max_loads - S.inflight.size());
+        std::partial_sort(cand.begin(), cand.begin() + want, cand.end(),
+                          [&](uint32_t a, uint32_t b) { return S.score[a] > S.score[b]; });
+        std::sort(res.begin(), res.end(), [&](uint32_t a, uint32_t b) { return S.score[a] < S.score[b]; });
+        std::vector<uint64_t> free_slots;
+        for (uint64_t sl = 0; sl < S.nslots; ++sl)
+            if (S.slot_page[sl] < 0) free_slots.push_back(sl);
+        size_t vi = 0;
+        for (size_t ci = 0; ci < want; ++ci) {
+            const uint32_t c = cand[ci];
+            uint64_t slot;
+            if (!free_slots.empty()) {
+                slot = free_slots.back();
+                free_slots.pop_back();
+            } else {
+                if (vi >= res.size()) break;
+                const uint32_t v = res[vi];
+                if (S.score[c] <= 1.5 * S.score[v] + 1.0) break;  // hysteresis: no swap for a small gain
+                ++vi;
+                slot = static_cast<uint64_t>(S.page_slot[v]);
+                S.page_slot[v] = -1;
+                S.h_pages[v] = S.host_col;
+                S.table_dirty = true;
+            }
+            S.slot_page[slot] = c;
+            S.loading[c] = 1;
+            start.push_back({c, static_cast<uint32_t>(slot), nullptr});
+        }
+    }
+    if (S.table_dirty) {
+        GW_HIP(ctx, hipMemcpyAsync(S.d_pages, S.h_pages, S.npages * sizeof(void*), hipMemcpyHostToDevice, ctx->stream));
+        S.table_dirty = false;
+    }
+    if (!start.empty()) {
+        GW_HIP(ctx, hipEventRecord(S.table_ev, ctx->stream));
+        GW_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, S.table_ev, 0));
+        for (auto& ld : start) {
+            if (S.free_events.empty()) {
+                hipEvent_t ev = nullptr;
+                GW_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                S.free_events.push_back(ev);
+            }
+            ld.done = S.free_events.back();
+            S.free_events.pop_back();
+            const uint64_t bytes = page_edges(ctx, S, ld.page) * 4;
+            GW_HIP(ctx, hipMemcpyAsync(S.d_cache + (static_cast<uint64_t>(ld.slot) << S.page_log2),
+                                       ctx->h_col + (static_cast<uint64_t>(ld.page) << S.page_log2), bytes,
+                                       hipMemcpyHostToDevice, ctx->copy_stream));
+            GW_HIP(ctx, hipEventRecord(ld.done, ctx->copy_stream));
+            S.inflight.push_back(ld);
+            *started += bytes;
+            ++*nstarted;
+        }
+        S.h2d_bytes += *started;
+        S.pages_loaded += *nstarted;
+    }
+    if (S.npages) GW_HIP(ctx, hipMemsetAsync(S.d_page_hits, 0, S.npages * 4, ctx->stream));
+    return GW_OK;
+}
+
+void ooc_close(gw_ctx* ctx, ServeState& S) {
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    for (auto& ld : S.inflight) (void)hipEventDestroy(ld.done);
+    for (auto ev : S.free_events) (void)hipEventDestroy(ev);
+    S.inflight.clear();
+    S.free_events.clear();
+    if (S.table_ev) (void)hipEventDestroy(S.table_ev);
+    if (S.h_pages) (void)hipHostFree(S.h_pages);
+    const uint64_t np = std::max<uint64_t>(1, S.npages);
+    gw_free(ctx, S.d_pages, np * sizeof(void*));
+    gw_free(ctx, S.d_page_hits, np * 4);
+    gw_free(ctx, S.d_cache, S.nslots * (1ull << S.page_log2) * 4);
+    gw_free(ctx, S.d_off, (ctx->n + 1ull) * 8);
 }
 
 }  // namespace
@@ -531,7 +763,9 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad arguments");
     if (max_queries > 65535)  // completion launches one grid row per query
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: max_queries must be at most 65535");
-    if (!ctx->d_off) return gw_fail(ctx, GW_ESTATE, ctx->host_graph ? "gw_serve_open: serving needs a device-resident graph" : "gw_serve_open: no graph");
+    if (!gw_has_graph(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_serve_open: no graph");
+    if (ctx->host_graph && prog->layout == GW_LAYOUT_EXT)
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_open: a host-resident graph is read in the CSR layout");
     if (prog->kind != GW_RESTART && prog->kind != GW_STOP && prog->kind != GW_JUMP)
         return gw_fail(ctx, GW_EINVAL, "gw_serve_open: serving runs RESTART, STOP or JUMP programs");
     if (prog->L == 0 || prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_serve_open: bad program");
@@ -569,7 +803,11 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         gw_serve_close_state(ctx, S);
         return rc;
     }
-    S->ext = prog->layout != GW_LAYOUT_CSR && gw_ensure_ext(ctx);
+    if (ctx->host_graph && (rc = ooc_open(ctx, *S))) {
+        gw_serve_close_state(ctx, S);
+        return rc;
+    }
+    S->ext = !ctx->host_graph && prog->layout != GW_LAYOUT_CSR && gw_ensure_ext(ctx);
     if (prog->layout == GW_LAYOUT_EXT && !S->ext) {
         gw_serve_close_state(ctx, S);
         return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: successor-header layout does not fit");
@@ -587,12 +825,24 @@ int gw_serve_submit(gw_ctx* ctx, const gw_query* q, uint32_t n) {
     ServeState* sp = state_of(ctx);
     if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_submit: not open");
     ServeState& S = *sp;
+    if (S.failed) return gw_fail(ctx, GW_ESTATE, "gw_serve_submit: the server failed; close it");
+    std::set<uint64_t> batch;
     for (uint32_t i = 0; i < n; ++i) {
         if (q[i].source >= ctx->n || q[i].walks == 0 || q[i].walks > S.pool_cap)
             return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) +
                                                " has a source out of range or a walk count outside [1, pool]");
+        if (region_slots(q[i].walks, S.prog.L, ctx->n) > S.arena_slots)
+            return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) + " needs a count region of " +
+                                               std::to_string(region_slots(q[i].walks, S.prog.L, ctx->n)) +
+                                               " slots, more than the whole arena (" + std::to_string(S.arena_slots) + ")");
+        if (S.qids.count(q[i].qid) || !batch.insert(q[i].qid).second)
+            return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query id " + std::to_string(q[i].qid) +
+                                               " is already queued, in flight or waiting to be read");
     }
-    for (uint32_t i = 0; i < n; ++i) S.queue.push_back(q[i]);
+    for (uint32_t i = 0; i < n; ++i) {
+        S.queue.push_back(q[i]);
+        S.qids.insert(q[i].qid);
+    }
     return GW_OK;
 }
 
@@ -601,8 +851,17 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     ServeState* sp = state_of(ctx);
     if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_step: not open");
     ServeState& S = *sp;
+    if (S.failed) return gw_fail(ctx, GW_ESTATE, "gw_serve_step: the server failed; close it");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     gw_serve_stats st{};
+    // 0. over a host-resident graph: the page cache's table and loads
+    if (S.ooc) {
+        int rc = ooc_before_step(ctx, S, &st.h2d_bytes, &st.pages_loaded);
+        if (rc) {
+            S.failed = true;
+            return rc;
+        }
+    }
     // 1. admit queued queries in arrival order while walk slots, a query slot
     //    and arena space for their count region are free
     const uint32_t L = S.prog.L;
@@ -610,9 +869,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     while (!S.queue.empty()) {
         const gw_query& q = S.queue.front();
         if (S.live + q.walks > S.pool_cap || S.free_slots.empty()) break;
-        const uint64_t distinct = std::min<uint64_t>(static_cast<uint64_t>(q.walks) * L, ctx->n);
-        uint64_t size = 64;
-        while (size < 2 * distinct) size <<= 1;
+        const uint64_t size = region_slots(q.walks, L, ctx->n);
         uint64_t base = 0;
         if (size > S.arena_slots || !arena_alloc(S, size, base)) break;
         const uint32_t slot = S.free_slots.back();
@@ -650,10 +907,22 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     std::vector<uint32_t> rem(S.max_queries), occn(S.max_queries);
     GW_HIP(ctx, hipMemcpyAsync(rem.data(), S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipMemcpyAsync(occn.data(), S.d_occn, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    if (S.ooc && S.npages)
+        GW_HIP(ctx, hipMemcpyAsync(S.hits.data(), S.d_page_hits, S.npages * 4, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    if (cnt[3]) return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed");
+    if (cnt[3]) {  // the walks have moved on: the server cannot be stepped again consistently
+        S.failed = true;
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+    }
+    if (S.ooc) {
+        for (uint64_t pg = 0; pg < S.npages; ++pg) {
+            st.sampled_reads += S.hits[pg];
+            if (S.page_slot[pg] >= 0) st.sampled_resident += S.hits[pg];
+        }
+        for (uint64_t pg = 0; pg < S.npages; ++pg) st.pages_resident += S.page_slot[pg] >= 0;
+    }
     st.steps = cnt[1];
     st.visits = cnt[2];
     st.step_ms = ms;
@@ -766,9 +1035,12 @@ int gw_serve_poll(gw_ctx* ctx, uint64_t* qid, uint64_t* npairs) {
     if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: not open");
     ServeState& S = *sp;
     if (S.done_order.empty()) return 1;  // nothing completed
-    *qid = S.done_order.front();
+    const uint64_t id = S.done_order.front();
     S.done_order.pop_front();
-    *npairs = S.done[*qid].size();
+    auto d = S.done.find(id);
+    if (d == S.done.end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: completed query " + std::to_string(id) + " lost");
+    *qid = id;
+    *npairs = d->second.size();
     return GW_OK;
 }
 
@@ -788,6 +1060,7 @@ int gw_serve_result(gw_ctx* ctx, uint64_t qid, uint32_t* vertex, uint32_t* count
         count[i] = static_cast<uint32_t>(pairs[i] & cmask);
     }
     S.done.erase(d);
+    S.qids.erase(qid);
     return GW_OK;
 }
 
@@ -823,6 +1096,15 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     return GW_OK;
 }
 
+int gw_serve_cache(gw_ctx* ctx, uint32_t page_log2, uint32_t max_loads) {
+    if (!ctx || (page_log2 != 0 && (page_log2 < 10 || page_log2 > 26)) || max_loads > 64)
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_cache: page_log2 in [10, 26] (or 0), max_loads at most 64");
+    if (state_of(ctx)) return gw_fail(ctx, GW_ESTATE, "gw_serve_cache: call it before gw_serve_open");
+    ctx->serve_page_log2 = page_log2;
+    ctx->serve_max_loads = max_loads;
+    return GW_OK;
+}
+
 int gw_serve_close(gw_ctx* ctx) {
     if (!ctx) return GW_EINVAL;
     ServeState* S = state_of(ctx);
@@ -837,6 +1119,7 @@ int gw_serve_close(gw_ctx* ctx) {
 static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (S->ooc) gw::ooc_close(ctx, *S);
     gw_free(ctx, S->pool[0], S->pool_cap * sizeof(gw_walk));
     gw_free(ctx, S->pool[1], S->pool_cap * sizeof(gw_walk));
     gw_free(ctx, S->arena, S->arena_slots * 8);

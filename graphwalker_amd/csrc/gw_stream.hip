@@ -113,6 +113,29 @@ int gw_stream_prepare(gw_ctx* ctx) {
     return GW_OK;
 }
 
+// The CSR in ctx->h_off / ctx->h_col (pinned) becomes the context's graph:
+// vertex counts and the default block table of an out-of-core run.
+int gw_host_graph_adopt(gw_ctx* ctx, uint32_t n, uint64_t m, uint64_t cache_bytes) {
+    ctx->host_graph = true;
+    ctx->n = n;
+    ctx->m = m;
+    ctx->cache_bytes = cache_bytes;
+    int rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, n * 4ull, "vertex counts"))) {
+        gw_host_graph_free(ctx);
+        return rc;
+    }
+    // default block table: blocks of at most cache_bytes / 4 (at least four
+    // slots), grown until it has at most 2048 blocks
+    uint64_t target = std::max<uint64_t>(cache_bytes / 4, 64);
+    std::vector<uint32_t> bf = cut_blocks(ctx, target);
+    while (bf.size() - 1 > 2048) {
+        target *= 2;
+        bf = cut_blocks(ctx, target);
+    }
+    return gw_graph_blocks(ctx, bf.data(), static_cast<uint32_t>(bf.size() - 1));
+}
+
 extern "C" {
 
 int gw_host_alloc(gw_ctx* ctx, uint64_t bytes, void** out) {
@@ -177,24 +200,7 @@ int gw_graph_set_host(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uin
         std::memcpy(ctx->h_owned_col, col, cb);
         ctx->h_col = static_cast<const uint32_t*>(ctx->h_owned_col);
     }
-    ctx->host_graph = true;
-    ctx->n = n;
-    ctx->m = m;
-    ctx->cache_bytes = cache_bytes;
-    int rc;
-    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, n * 4ull, "vertex counts"))) {
-        gw_host_graph_free(ctx);
-        return rc;
-    }
-    // default block table: blocks of at most cache_bytes / 4 (at least four
-    // slots), grown until it has at most 2048 blocks
-    uint64_t target = std::max<uint64_t>(cache_bytes / 4, 64);
-    std::vector<uint32_t> bf = cut_blocks(ctx, target);
-    while (bf.size() - 1 > 2048) {
-        target *= 2;
-        bf = cut_blocks(ctx, target);
-    }
-    return gw_graph_blocks(ctx, bf.data(), static_cast<uint32_t>(bf.size() - 1));
+    return gw_host_graph_adopt(ctx, n, m, cache_bytes);
 }
 
 // Host-to-device copy ceiling: `bytes` from pinned host memory into HBM on

@@ -27,6 +27,9 @@
  *   gw_graph_rmat              (no counterpart: the reference reads a
  *                              preprocessed CSR, conversions.hpp:181-298; this
  *                              builds the same CSR layout on the device)
+ *   gw_graph_rmat_host         the same, into pinned host memory for graphs
+ *                              larger than the device (the reference's graph
+ *                              lives on disk and is read block by block)
  *   gw_walks_load              the initial buckets written by startWalksbyApp
  *                              through WalkManager::moveWalk (walk.hpp:93-101)
  *   gw_stream_policy           WalkManager::chooseBlock and its heuristics
@@ -35,8 +38,8 @@
  *                              readWalksfromDisk, walk.hpp:103-157): a pool kept
  *                              in host memory, stepped window by window
  *   gw_walks_init_*            device-side walk placement for the app patterns
- *                              of rawrandomwalks.cpp:27-50 / rwdomination.cpp:264-284
- *                              (one walk per vertex) and msppr.cpp:151-179
+ *                              of rawrandomwalks.cpp:27-50 / rwdomination.cpp:41-61
+ *                              (one walk per vertex) and msppr.cpp:41-76
  *                              (walkspersource walks per source)
  *   gw_run_capture / gw_run_replay / gw_autotune   (no counterpart: hipGraph
  *                              replay of a run and a launch-shape search)
@@ -46,7 +49,7 @@
  *   gw_bucket                  the re-bucketing of walks by block done by
  *                              moveWalk + getCurrentWalks (walk.hpp:93-137)
  *   gw_counts_read             the per-vertex recorder of updateInfo
- *                              (rwdomination.cpp:286-289 vertex_value[dstId]++)
+ *                              (rwdomination.cpp:63-66 vertex_value[dstId]++)
  *   gw_source_counts_read      msppr's per-source DiscreteDistribution recorder
  *                              (msppr.cpp:71-76, discretedistribution.hpp:44-57), exact
  *   gw_source_topk             DiscreteDistribution::getTop (discretedistribution.hpp:72-95),
@@ -94,7 +97,7 @@ typedef struct gw_ctx gw_ctx;
 
 /* recorders (bit flags): device replacements for the per-step updateInfo */
 #define GW_REC_NONE 0u
-#define GW_REC_VERTEX_COUNT 1u /* u32 visits per vertex (rwdomination.cpp:288) */
+#define GW_REC_VERTEX_COUNT 1u /* u32 visits per vertex (rwdomination.cpp:65) */
 #define GW_REC_WALK_SUMMARY 2u /* per-walk visit count + trajectory hash */
 #define GW_REC_EVENTS 4u       /* per-walk event log of (vertex, hop) visits, replayed to updateInfo
                                   on the host (randomwalk.hpp:32); implies GW_REC_WALK_SUMMARY */
@@ -211,6 +214,16 @@ int gw_host_alloc(gw_ctx *ctx, uint64_t bytes, void **out);
 int gw_host_free(gw_ctx *ctx, void *p);
 int gw_graph_set_host(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uint32_t n, uint64_t m,
                       uint64_t cache_bytes);
+/* The gw_graph_rmat graph, bit for bit, built straight into pinned host
+ * memory the context owns (for graphs larger than the device) and made the
+ * host-resident graph, as gw_graph_set_host with cache_bytes.  It is built
+ * in source ranges whose edges fit `work_bytes` of device memory (0 = most
+ * of what is free): every edge is regenerated per range and the range's are
+ * sorted on the device, their CSR fragment copied to the host while the next
+ * range is generated.  build_ms (may be NULL): wall time on the device.
+ * gw_graph_read copies the arrays out. */
+int gw_graph_rmat_host(gw_ctx *ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
+                       uint32_t tabc, uint64_t cache_bytes, uint64_t work_bytes, double *build_ms);
 
 /* Walks: records with ids < n (ids key the random streams). */
 int gw_walks_load(gw_ctx *ctx, const gw_walk *walks, uint64_t n);
@@ -352,10 +365,31 @@ typedef struct gw_serve_stats {
     uint64_t live_walks;      /* walks still in flight after the step */
     uint32_t admitted, completed, active_queries, queued;
     double step_ms;           /* step kernel time, HIP events on the engine stream */
+    /* host-resident graph only (zero otherwise): */
+    uint64_t h2d_bytes;       /* column pages whose copy into the page cache started before this step */
+    uint32_t pages_loaded;    /* how many */
+    uint32_t pages_resident;  /* pages in the cache during the step */
+    uint64_t sampled_reads;   /* 1-in-64 sample of the step's column reads */
+    uint64_t sampled_resident;/* of those, reads served by the page cache (the rest crossed the host link) */
 } gw_serve_stats;
 
-/* max_queries: queries in flight at once, 1 .. 65535 (completed queries are
- * processed one workgroup row per query). */
+/* Over a host-resident graph (gw_graph_set_host, gw_graph_rmat_host) the
+ * server copies the offsets to HBM and keeps the columns in host memory, read
+ * through a device page cache of pages of 2^page_log2 edges: the graph's
+ * cache_bytes, offsets included, bound it.  Columns of pages not in the cache
+ * are read from the pinned host arrays in place; between serve steps the
+ * pages read most (a 1-in-64 sample of each step's reads, decayed) are
+ * copied into the cache on the copy stream while steps run, replacing the
+ * least-read ones.  Results do not depend on which pages are resident.
+ * gw_serve_cache (before gw_serve_open) sets the page size (0 = pages of at
+ * least 2^10 edges, at most 4096 pages) and the page copies in flight at once
+ * (0 = 4).  Host-resident graphs are served in the CSR layout.
+ * max_queries: queries in flight at once, 1 .. 65535 (completed queries are
+ * processed one workgroup row per query).  gw_serve_submit refuses a query
+ * whose count region cannot fit the arena, and an id already queued, in
+ * flight or completed but not yet read by gw_serve_result.  A step that fails
+ * after its kernel ran leaves the server failed: only gw_serve_close works. */
+int gw_serve_cache(gw_ctx *ctx, uint32_t page_log2, uint32_t max_loads);
 int gw_serve_open(gw_ctx *ctx, const gw_program *prog, uint64_t pool_walks, uint64_t arena_bytes,
                   uint32_t max_queries, uint32_t hops_per_step);
 int gw_serve_submit(gw_ctx *ctx, const gw_query *queries, uint32_t n);

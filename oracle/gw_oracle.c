@@ -208,7 +208,7 @@ static inline void record(gwo_rec *rc, uint32_t v, uint32_t hop) {
  *   PROB    die, no hop cap                  (randomwalkwithprob.hpp:30,39-42)
  *   JOINT   restart at the source vertex; finish when hop % L == L-1
  *                                            (randomwalkwithrestartwithjoint.hpp:47-53)
- *   RESTART restart at the source vertex, cap L (randomwalkwithrestart.hpp:163-176) */
+ *   RESTART restart at the source vertex, cap L (randomwalkwithrestart.hpp:33-63) */
 static int advance(const gwo_program *p, const gwo_graph *g, gwo_walk *w,
                    uint32_t lo, uint32_t hi, gwo_rec *rc) {
     const uint32_t key[2] = {(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};

@@ -92,6 +92,26 @@ def test_serve_rejects_bad_queries(engine, g12):
             engine.serve_submit([(1, 5, 5000)])  # more walks than the pool
     finally:
         engine.serve_close()
+    # a query whose count region can never fit the arena is refused at once
+    # (it would otherwise block the queue), and so is an id already in use
+    engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), 100_000, 32 << 10, 8, 4)
+    try:
+        with pytest.raises(gw.GwError, match="whole arena"):
+            engine.serve_submit([(1, 5, 5000)])
+        engine.serve_submit([(2, 5, 100)])
+        with pytest.raises(gw.GwError, match="already"):
+            engine.serve_submit([(2, 6, 100)])
+        with pytest.raises(gw.GwError, match="already"):
+            engine.serve_submit([(3, 6, 100), (3, 7, 100)])
+        while (r := engine.serve_poll()) is None:
+            engine.serve_step()
+        assert r[0] == 2
+        with pytest.raises(gw.GwError, match="already"):
+            engine.serve_submit([(2, 6, 100)])  # completed, not yet read
+        engine.serve_result(*r)
+        engine.serve_submit([(2, 6, 100)])      # read: the id is free again
+    finally:
+        engine.serve_close()
     with pytest.raises(gw.GwError, match="at most 65535"):
         engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), 1000, 1 << 20, 70_000, 4)
 
