@@ -1,24 +1,32 @@
 """Walk-steps/s of the walk-update hot path on one MI355X.
 
-Default workload (`rawwalks-rmat24`): raw random walks on R-MAT scale 24
-(16,777,216 vertices, edge factor 16 = 268,435,456 edges, a/b/c/d =
-.57/.19/.19/.05, seed 2), one walk per vertex, fixed length 10, no restart
-(dangling vertices teleport): 167,772,160 walk-steps per pass.
+Headline workload (`ppr4096-rmat26`): batched multi-source PPR on R-MAT
+scale 26 (67,108,864 vertices, edge factor 16 = 1,073,741,824 edges, a/b/c/d =
+.57/.19/.19/.05, seed 3), 4096 uniform sources x 2000 walks, stop probability
+0.15, cap 20, exact sparse per-(source, vertex) u32 counts, one persistent
+step kernel per pass.
 
-Secondary workload (`ppr4096-rmat26`, reported under "workloads"): batched
-multi-source PPR on R-MAT scale 26 (67,108,864 vertices, 1,073,741,824 edges,
-seed 3), 4096 uniform sources x 2000 walks, stop probability 0.15, cap 20,
-exact sparse per-(source, vertex) u32 counts.
+Secondary workloads (under "workloads"): raw random walks on R-MAT scale 24
+(`rawwalks-rmat24`, one walk per vertex, fixed length 10, no restart); SimRank
+on the transposed scale-28 graph; continuous-batching restart-PPR serving on
+the scale-26 graph in HBM; and serving over a host-resident R-MAT graph of
+scale 31, edge factor 24 (51.5 G edges in pinned host memory, a 64 GB
+device cache), or the largest scale the host's memory holds.
 
 A bench "step" is one pass of the engine over all walks (gw_run), graph and
 walk records resident in HBM before timing.  The launch shape (block threads x
 waves per SIMD x walks a wavefront claims at a time) is picked by gw_autotune
-before the timed passes, or fixed with --shape; "graph_replay" reports the same passes replayed from one
-captured hipGraph.  Prints one JSON line.  The CPU
-baseline is the reference's own engine (its rawrandomwalks app, built from its
-sources into oracle/_ref/) on the same graph and walk count, on the host's
-cores; the oracle's OpenMP replay of the same walk program is reported beside
-it (and is the baseline where the reference app is not built).
+before the timed passes, or fixed with --shape; "graph_replay" reports the
+same passes replayed from one captured hipGraph.  Prints one JSON line.
+
+roofline.achieved uses the path's algorithmic bytes per walk-step,
+B_step = 64 B x (2 + r): one random 64-B sector for the offset pair, one for
+the column, and r = 1 more when the workload records a per-step count (the
+per-source counts here; raw walks have r = 0), divided by the step kernel's
+mean duration (HIP events on the engine stream).  The CPU baseline is the
+reference's own engine (an app on its API with msppr's recorder and the same
+source list, built from its sources into oracle/_ref/) on the same graph; the
+oracle's OpenMP replay is reported beside it.
 """
 from __future__ import annotations
 
@@ -57,19 +65,31 @@ WORKLOADS = {
 }
 
 
-def random_sectors_per_step(layout: int) -> int:
-    """Random reads per walk-step at 64-B sector granularity: the CSR layout
-    reads one sector for the offset pair and one for the column; the
-    successor-header layout reads one 16-B edge record."""
-    return 2 if layout == gw.LAYOUT_CSR else 1
+def bytes_per_step(recorder: int) -> int:
+    """Algorithmic bytes per walk-step, B_step = 64 x (2 + r): the offset
+    pair's sector, the column's sector, and one more (r = 1) when the walk
+    program records a per-step count."""
+    r = 1 if recorder & (gw.REC_VERTEX_COUNT | gw.REC_SOURCE_COUNT) else 0
+    return SECTOR * (2 + r)
 
 
-def algorithmic_bytes(steps: int, visits: int, recorder: int, layout: int) -> int:
-    """Random-access bytes one pass needs: the step reads, plus one sector
-    read-modify-write per visit for a count recorder (vertex atomics or the
-    source-count hash)."""
-    rec = 1 if recorder & (gw.REC_VERTEX_COUNT | gw.REC_SOURCE_COUNT) else 0
-    return SECTOR * (steps * random_sectors_per_step(layout) + visits * rec)
+def algorithmic_bytes(steps: int, recorder: int) -> int:
+    return steps * bytes_per_step(recorder)
+
+
+def csrc_digest() -> str:
+    """sha256 over the library's sources: ties a committed counter profile to
+    the kernels it was measured on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for d in (os.path.join(ROOT, "graphwalker_amd", "csrc"), os.path.join(ROOT, "include")):
+        for f in sorted(os.listdir(d)):
+            if f.endswith((".hip", ".h")):
+                h.update(f.encode())
+                with open(os.path.join(d, f), "rb") as fh:
+                    h.update(fh.read())
+    return h.hexdigest()
 
 
 def setup(eng: gw.Engine, wl: dict) -> int:
@@ -142,7 +162,7 @@ def pool_measure(eng: gw.Engine, wl: dict, n: int, prog: gw.Program, window: int
             "pcie_GBps": st["pool_bytes"] / (wall / K) / 1e9}
 
 
-def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) -> dict:
+def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float, steps_per_visit: float) -> dict:
     import oracle as O
 
     off, col = eng.graph_read()
@@ -169,7 +189,11 @@ def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) ->
     port = {"value": r["steps"] / elapsed, "unit": "walk-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle OpenMP replay (per-vertex counts) of the first {nw} walks of the same program on the "
                       f"same device-generated graph: {r['steps']} steps in {elapsed:.2f} s"}
-    if wl["sources"] or wl["kind"] != gw.JUMP or O.reference_app("rawrandomwalks") is None:
+    if wl["sources"]:
+        if O.reference_app("msppr_sources") is None:
+            return port
+        return cpu_reference_multisource(eng, wl, off, col, threads, target_s, steps_per_visit, port)
+    if wl["kind"] != gw.JUMP or O.reference_app("rawrandomwalks") is None:
         return port
     # The reference engine itself (its rawrandomwalks app built from its own
     # sources): the same graph in its on-disk layout, as many walks as the
@@ -195,6 +219,48 @@ def cpu_baseline(eng: gw.Engine, wl: dict, prog: gw.Program, target_s: float) ->
                       f"{wl['L']} steps (uniform sources, its 0.15 teleport) in {ref['00_runtime']:.2f} s of its "
                       f"00_runtime ({wall:.1f} s wall)",
             "exec_phase_walk_steps_per_s": steps / ref["5_exec_updates"], "port": port}
+
+
+def cpu_reference_multisource(eng: gw.Engine, wl: dict, off: np.ndarray, col: np.ndarray, threads: int,
+                              target_s: float, steps_per_visit: float, port: dict) -> dict:
+    """The reference engine on the batched PPR workload: its RandomWalkwithStop
+    program and msppr's recorder (oracle/ref_msppr_sources.cpp) on the same
+    graph in its on-disk layout, with the workload's own sources (a leading
+    share of them, grown until the run takes about target_s), one in-memory
+    block, execthreads = the host's cores; timed by its 00_runtime."""
+    import shutil
+    import tempfile
+
+    import oracle as O
+
+    n = off.shape[0] - 1
+    src = gw.sources(wl["seed"], n, wl["sources"])
+    tmp = tempfile.mkdtemp(prefix="gw_ref_")
+    try:
+        base = os.path.join(tmp, "graph")
+        O.write_reference_graph(base, off, col)
+        blocksize_kb = -(-int(off[-1]) * 4 // 1024) + 1  # the whole column file in one block
+        k = max(1, wl["sources"] // 16)
+        while True:
+            t0 = time.perf_counter()
+            ref = O.run_reference_msppr_sources(base, src[:k], wl["wps"], wl["L"], threads, timeout=600,
+                                                blocksize_kb=blocksize_kb)
+            wall = time.perf_counter() - t0
+            if ref["00_runtime"] >= target_s / 3 or k >= wl["sources"]:
+                break
+            k = min(wl["sources"], int(k * max(2.0, target_s / max(ref["00_runtime"], 1e-3))))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    visits_per_s = ref["visits"] / ref["00_runtime"]
+    return {"value": visits_per_s * steps_per_visit, "unit": "walk-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"reference engine (built from its sources, -O3 -fopenmp, execthreads {threads}) running an app "
+                      f"on its API with msppr's recorder (a DiscreteDistribution per source and thread) on the same "
+                      f"graph in its on-disk layout, one in-memory block: the workload's first {k} of "
+                      f"{wl['sources']} sources x {wl['wps']} walks, stop 0.15, cap {wl['L']}: {ref['visits']} visits "
+                      f"in {ref['00_runtime']:.2f} s of its 00_runtime ({wall:.1f} s wall); walk-steps = visits x "
+                      f"{steps_per_visit:.4f}, the device pass's exact steps per visit",
+            "visits_per_s": visits_per_s, "exec_phase_visits_per_s": ref["visits"] / ref["5_exec_updates"],
+            "port": port}
 
 
 def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool,
@@ -243,7 +309,7 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
     prog, wall, steps, visits, kms, st = timed(eng, wl, n, sched, layout, K, W, shape=shape)
     per_pass_steps, per_pass_visits = steps // K, visits // K
     kavg = float(np.mean(kms))
-    abytes = algorithmic_bytes(per_pass_steps, per_pass_visits, wl["recorder"], st["layout"])
+    abytes = algorithmic_bytes(per_pass_steps, wl["recorder"])
     achieved = abytes / (kavg * 1e-3) / 1e9
     print(f"[bench] {name} {sched}-{layout}: {steps / wall / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
     res = {
@@ -258,22 +324,27 @@ def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, 
               file=sys.stderr, flush=True)
     if "plain-csr" in variants:
         res["speedup_vs_plain"] = res["value"] / variants["plain-csr"]["walk_steps_per_s"]
+    res["reduce_ms"] = st["reduce_ms"]
     return res
 
 
 def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5, hops: int = 10,
-              walks_lo: int = 1000, walks_hi: int = 100_000, topk: int = 100) -> dict:
+              walks_lo: int = 1000, walks_hi: int = 100_000, topk: int = 100, warm_seconds: float = 0.0,
+              pool_walks: int = 1 << 24, arena_bytes: int = 16 << 30) -> dict:
     """Poisson arrivals of restart-PPR queries (p = 0.15, 10 hops) at `qps`,
     walk counts log-uniform in [walks_lo, walks_hi], uniform sources, served
     by continuous batching; each query returns its top-k vertices (0 = every
     pair).  Latency = completion (result on the host) minus arrival, on the
-    host clock.  One warm-up query is served before the clock starts."""
+    host clock.  One warm-up query is served before the clock starts; with
+    warm_seconds, the queries arriving in that first stretch of the stream
+    are served but left out of the figures (a host-resident graph's page
+    cache adapts to the demand meanwhile)."""
     rng = np.random.default_rng(seed)
-    nq = max(1, int(qps * seconds))
+    nq = max(1, int(qps * (seconds + warm_seconds)))
     arrivals = np.cumsum(rng.exponential(1.0 / qps, nq))
     walks = np.exp(rng.uniform(np.log(walks_lo), np.log(walks_hi), nq)).astype(np.int64)
     sources = gw.sources(seed, n, nq)
-    eng.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=seed), pool_walks=1 << 24, arena_bytes=16 << 30,
+    eng.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=seed), pool_walks=pool_walks, arena_bytes=arena_bytes,
                    max_queries=4096, hops_per_step=hops)
     if topk:
         eng.serve_topk(topk)
@@ -287,10 +358,14 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
             eng.serve_result(*r, sort=False)
             break
     lat = np.zeros(nq)
-    steps = visits = nsteps = 0
-    step_ms = 0.0
+    measured = arrivals >= warm_seconds
+    keys = ("steps", "visits", "step_ms", "h2d_bytes", "pages_loaded", "sampled_reads", "sampled_resident")
+    acc = {k: 0 for k in keys}  # over the serve steps after warm_seconds
+    nsteps = 0
+    resident = 0
     done = nxt = 0
     t0 = time.perf_counter()
+    t_meas = None
     try:
         while done < nq:
             now = time.perf_counter() - t0
@@ -300,12 +375,16 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
             if k > nxt:
                 eng.serve_submit([(i, int(sources[i]), int(walks[i])) for i in range(nxt, k)])
                 nxt = k
-            if done + 0 < nxt:
+            if done < nxt:
+                in_window = now >= warm_seconds
+                if in_window and t_meas is None:
+                    t_meas = now
                 st = eng.serve_step()
-                steps += st["steps"]
-                visits += st["visits"]
-                step_ms += st["step_ms"]
-                nsteps += 1
+                if in_window:
+                    for k_ in keys:
+                        acc[k_] += st[k_]
+                    nsteps += 1
+                    resident = st["pages_resident"]
                 t_done = time.perf_counter() - t0
                 while (r := eng.serve_poll()) is not None:
                     eng.serve_result(*r, sort=False)
@@ -315,12 +394,98 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
                 time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
     finally:
         eng.serve_close()
-    wall = time.perf_counter() - t0
-    return {"qps_offered": qps, "queries": nq, "seconds": wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
-            "walk_steps_per_s": steps / wall, "visits": visits, "serve_steps": nsteps,
-            "gpu_busy_frac": step_ms * 1e-3 / wall, "hops_per_step": hops,
-            "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]", "result": f"top-{topk}" if topk else "all pairs"}
+    wall = time.perf_counter() - t0 - (t_meas if t_meas is not None else 0.0)
+    lm = lat[measured] if measured.any() else lat
+    out = {"qps_offered": qps, "queries": int(measured.sum()), "seconds": wall,
+           "p50_ms": float(np.percentile(lm, 50) * 1e3), "p99_ms": float(np.percentile(lm, 99) * 1e3),
+           "mean_ms": float(lm.mean() * 1e3), "walk_steps_per_s": acc["steps"] / wall, "visits": acc["visits"],
+           "serve_steps": nsteps, "gpu_busy_frac": acc["step_ms"] * 1e-3 / wall, "hops_per_step": hops,
+           "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]", "result": f"top-{topk}" if topk else "all pairs"}
+    if warm_seconds:
+        out["warmup_queries"] = int((~measured).sum())
+        out["warmup_seconds"] = warm_seconds
+    if acc["sampled_reads"]:
+        out["page_cache"] = {
+            "pages_resident": resident, "pages_loaded": acc["pages_loaded"], "h2d_bytes": acc["h2d_bytes"],
+            "h2d_GBps": acc["h2d_bytes"] / wall / 1e9,
+            "column_reads_from_cache_frac": acc["sampled_resident"] / acc["sampled_reads"],
+            # 1-in-64 sample of the column reads, the rest crossed the host link as 4-byte reads
+            "host_column_reads_per_s": (acc["sampled_reads"] - acc["sampled_resident"]) * 64 / wall,
+        }
+    return out
+
+
+def serve_ooc_measure(scale_max: int, ef: int, seed: int, cache_frac_of_graph: float, cache_cap: int,
+                      qps: float, seconds: float, warm_seconds: float, hops: int, topk: int) -> dict:
+    """Serving over a host-resident graph: R-MAT (edge factor `ef`, seed) built
+    range by range into pinned host memory at the largest scale <= scale_max
+    whose CSR fits the host's available memory with room to spare, served
+    through a device page cache of cache_cap bytes (offsets included) -- or,
+    below scale_max, the same fraction of the graph.  Its own engine, so
+    earlier workloads' host memory is released first."""
+    avail = 0
+    with open("/proc/meminfo") as f:
+        for line in f:
+            if line.startswith("MemAvailable:"):
+                avail = int(line.split()[1]) * 1024
+    total = 0
+    with open("/proc/meminfo") as f:
+        for line in f:
+            if line.startswith("MemTotal:"):
+                total = int(line.split()[1]) * 1024
+    room = min(int(avail * 0.85), 240 << 30)  # the CSR plus the process, under the host's limit
+    eng = gw.Engine(0)
+    tried = []
+    try:
+        # the largest scale whose CSR the host can pin (pinned memory may be
+        # capped below the available memory: a refused allocation steps down)
+        for scale in range(scale_max, 19, -1):
+            n, m = 1 << scale, ef << scale
+            graph_bytes = (n + 1) * 8 + m * 4
+            if graph_bytes > room:
+                tried.append(f"scale {scale}: {graph_bytes / 2**30:.0f} GiB of CSR above the available host memory")
+                continue
+            cache = cache_cap if scale == scale_max else int(graph_bytes * cache_frac_of_graph)
+            try:
+                t0 = time.perf_counter()
+                build_ms = eng.graph_rmat_host(scale, m, seed, cache_bytes=cache)
+                build_s = time.perf_counter() - t0
+                break
+            except gw.GwError as exc:
+                if "pinned host memory" not in str(exc):
+                    raise
+                tried.append(f"scale {scale}: pinning {graph_bytes / 2**30:.0f} GiB refused ({exc})")
+                print(f"[bench] {tried[-1]}", file=sys.stderr, flush=True)
+        else:
+            raise gw.GwError("no scale's CSR could be pinned: " + "; ".join(tried))
+        print(f"[bench] host-resident R-MAT scale {scale} ef {ef}: {graph_bytes / 2**30:.1f} GiB built in "
+              f"{build_s:.1f} s", file=sys.stderr, flush=True)
+        probe_gathers = 1 << 24
+        host_read_ms = eng.host_gather_probe(4 << 30, probe_gathers)
+        h2d_ms = eng.h2d_probe(1 << 30)
+        t0 = time.perf_counter()
+        r = serve_sim(eng, n, qps, seconds, seed=seed + 1, hops=hops, topk=topk, warm_seconds=warm_seconds,
+                      pool_walks=1 << 25, arena_bytes=32 << 30)
+        serve_wall = time.perf_counter() - t0
+    finally:
+        eng.close()
+    host_ceiling = probe_gathers / (host_read_ms * 1e-3)
+    h2d_ceiling = (1 << 30) / (h2d_ms * 1e-3) / 1e9
+    pc = r.get("page_cache", {})
+    r.update({
+        "desc": f"R-MAT scale {scale} ef {ef} seed {seed}: {n} vertices, {m} edges, {graph_bytes / 1e9:.1f} GB of CSR in "
+                f"pinned host memory; offsets copied to HBM, columns through a {cache / 1e9:.1f} GB device page cache "
+                f"(offsets included, pages chosen by sampled demand); restart-PPR queries (p 0.15, 10 hops)",
+        "scale": scale, "scale_asked": scale_max, "larger_scales": tried, "edge_factor": ef, "graph_bytes": graph_bytes,
+        "cache_bytes": cache,
+        "host_mem_total": total, "host_mem_available": avail,
+        "graph_build_s": build_s, "graph_build_device_ms": build_ms, "serve_wall_s": serve_wall,
+        "host_read_ceiling_per_s": host_ceiling, "h2d_ceiling_GBps": h2d_ceiling,
+    })
+    if pc:
+        pc["h2d_frac_of_link"] = pc["h2d_GBps"] / h2d_ceiling
+        pc["host_reads_frac_of_ceiling"] = pc["host_column_reads_per_s"] / host_ceiling
+    return r
 
 
 def stream_measure(eng: gw.Engine, scale: int, seed: int, cache_bytes: int, queries: int, K: int,
@@ -439,8 +604,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rawwalks-rmat24", choices=sorted(WORKLOADS))
-    ap.add_argument("--secondary", default="ppr4096-rmat26", help="extra workload reported under 'workloads' ('' = none)")
+    ap.add_argument("--workload", default="ppr4096-rmat26", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="rawwalks-rmat24", help="extra workload reported under 'workloads' ('' = none)")
     ap.add_argument("--schedule", default="persistent", choices=["plain", "persistent", "blocked"])
     ap.add_argument("--layout", default="ext", choices=["csr", "ext"])
     ap.add_argument("--shape", default="", help="launch shape 'THREADS,WAVES[,CHUNK]' (block threads, waves per "
@@ -454,11 +619,18 @@ def main() -> int:
     ap.add_argument("--serve-hops", type=int, default=10,
                     help="hops per walk per serve step (10 = the program's L: a walk ends within one step)")
     ap.add_argument("--stream-cache-mib", type=int, default=1024,
-                    help="out-of-core run: device block cache for the last workload's graph (0 = skip)")
+                    help="out-of-core run: device block cache for the headline's graph (0 = skip)")
     ap.add_argument("--stream-queries", type=int, default=256)
     ap.add_argument("--pool-window", type=int, default=1 << 21,
-                    help="walks per window of the host-walk-pool figure (0 = skip)")
-    ap.add_argument("--simrank-scale", type=int, default=28, help="config-4 SimRank line on R-MAT of this scale (0 = skip)")
+                    help="walks per window of the host-walk-pool figure on the raw-walk workload (0 = skip)")
+    ap.add_argument("--simrank-scale", type=int, default=28, help="SimRank line on R-MAT of this scale (0 = skip)")
+    ap.add_argument("--ooc-scale", type=int, default=31,
+                    help="serving over a host-resident R-MAT of this scale, edge factor 24 (or the largest the host "
+                         "memory holds; 0 = skip)")
+    ap.add_argument("--ooc-cache-gb", type=float, default=64.0, help="its device page cache, offsets included")
+    ap.add_argument("--ooc-qps", type=float, default=2000.0)
+    ap.add_argument("--ooc-seconds", type=float, default=6.0)
+    ap.add_argument("--ooc-warm-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -466,14 +638,57 @@ def main() -> int:
     eng = gw.Engine(0)
     info = eng.device_info()
     print(f"[bench] {info}", file=sys.stderr, flush=True)
-
     shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+
+    # ---- secondary workloads first (the headline's graph then stays for serving)
+    workloads = {}
+    pool = {}
+    post = {}
+    if args.secondary and args.secondary != args.workload:
+        s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False,
+                    shape=shape)
+        _, w_, st_, _, _, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1)
+        workloads[args.secondary] = {
+            "desc": s["wl"]["desc"], "walk_steps_per_s": s["value"], "walk_steps_per_pass": s["steps"],
+            "kernel_ms": s["kernel_ms"], "roofline_achieved_GBps": s["achieved"],
+            "roofline_frac": s["achieved"] / PEAK_HBM_GBPS, "bytes_per_step": bytes_per_step(s["wl"]["recorder"]),
+            "plain_csr_walk_steps_per_s": st_ / w_, "speedup_vs_plain": s["value"] / (st_ / w_),
+            "schedule": f"{args.schedule}-{args.layout}", "launch_shape": s["tuned"], "graph_replay": s.get("graph_replay"),
+        }
+        # post-processing on this graph: top-k of the per-vertex counts (the toplist caller)
+        wl2 = s["wl"]
+        eng.run(gw.program(wl2["kind"], wl2["L"], wl2["p"], s["n"], seed=wl2["seed"], recorder=gw.REC_VERTEX_COUNT,
+                           schedule=args.schedule, layout=args.layout))
+        for k in (100, 1 << 20):
+            eng.counts_topk(k)
+            t = time.perf_counter()
+            eng.counts_topk(k)
+            post[f"counts_topk_{k}_ms"] = (time.perf_counter() - t) * 1e3
+        print(f"[bench] counts top-k: {post}", file=sys.stderr, flush=True)
+        # the same walks as a host walk pool (out-of-HBM walk pools), stepped in windows
+        if args.pool_window and not wl2["sources"]:
+            for keep in (False, True):
+                r = pool_measure(eng, wl2, s["n"], s["prog"], args.pool_window, max(2, args.steps // 3), keep)
+                pool["with_finals" if keep else "counts_only"] = r
+                print(f"[bench] host walk pool (window {args.pool_window}, finals {keep}): "
+                      f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s, {r['pcie_GBps']:.1f} GB/s over PCIe",
+                      file=sys.stderr, flush=True)
+
+    # ---- the headline
     m = measure(eng, args.workload, args.steps, args.warmup, args.schedule, args.layout, compare=True, shape=shape)
     wl = m["wl"]
+    plain_global = plain_hash = global_sort = None
+    if wl["sources"]:
+        _, w_, st_, _, _, _ = timed(eng, wl, m["n"], "plain", "csr", 2, 1, source_method=gw.SRC_GLOBAL)
+        plain_global = st_ / w_
+        _, w_, st_, _, _, _ = timed(eng, wl, m["n"], "plain", "csr", 2, 1, source_method=gw.SRC_HASH)
+        plain_hash = st_ / w_
+        _, w_, st_, _, _, lg_ = timed(eng, wl, m["n"], args.schedule, args.layout, 2, 1, source_method=gw.SRC_GLOBAL)
+        global_sort = {"walk_steps_per_s": st_ / w_, "reduce_ms": lg_["reduce_ms"]}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(eng, wl, m["prog"], args.cpu_seconds)
+        cpu = cpu_baseline(eng, wl, m["prog"], args.cpu_seconds, m["steps"] / max(1, m["visits"]))
 
     # random-gather ceiling: independent 4-byte gathers over 4 GiB
     gathers = 1 << 30
@@ -484,78 +699,59 @@ def main() -> int:
     copy_bytes = 1 << 30
     cms = eng.copy_probe(copy_bytes)
 
-    # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # (profiles/), calibrated against probe kernels of known size
+    # HBM bytes per launch of the step kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (profiles/), valid only for the library sources they were measured on
     traffic = traffic_copy = None
+    traffic_note = "no counter profile for this workload"
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.schedule}_{args.layout}.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        traffic, traffic_copy = t.get("bytes_per_launch"), t.get("bytes_per_launch_copy_calibrated")
-
-    # post-processing on the headline graph: top-k of the per-vertex counts (the toplist caller)
-    post = {}
-    eng.run(gw.program(wl["kind"], wl["L"], wl["p"], m["n"], seed=wl["seed"], recorder=gw.REC_VERTEX_COUNT,
-                       schedule=args.schedule, layout=args.layout))
-    for k in (100, 1 << 20):
-        eng.counts_topk(k)
-        t = time.perf_counter()
-        eng.counts_topk(k)
-        post[f"counts_topk_{k}_ms"] = (time.perf_counter() - t) * 1e3
-    print(f"[bench] counts top-k: {post}", file=sys.stderr, flush=True)
-
-    # the same walks as a host walk pool (out-of-HBM walk pools), stepped in windows
-    pool = {}
-    if args.pool_window:
-        for keep in (False, True):
-            r = pool_measure(eng, wl, m["n"], m["prog"], args.pool_window, max(2, args.steps // 3), keep)
-            pool["with_finals" if keep else "counts_only"] = r
-            print(f"[bench] host walk pool (window {args.pool_window}, finals {keep}): "
-                  f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s, {r['pcie_GBps']:.1f} GB/s over PCIe",
-                  file=sys.stderr, flush=True)
-
-    workloads = {}
-    if args.secondary and args.secondary != args.workload:
-        s = measure(eng, args.secondary, max(3, args.steps // 2), 1, args.schedule, args.layout, compare=False,
-                    shape=shape)
-        _, w_, st_, vi_, k_, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_GLOBAL)
-        _, wh_, sth_, _, _, _ = timed(eng, s["wl"], s["n"], "plain", "csr", 2, 1, source_method=gw.SRC_HASH)
-        _, wg_, stg_, _, _, lg_ = timed(eng, s["wl"], s["n"], args.schedule, args.layout, 2, 1,
-                                        source_method=gw.SRC_GLOBAL)
-        last = s["last"]
-        workloads[args.secondary] = {
-            "desc": s["wl"]["desc"], "walk_steps_per_s": s["value"], "visits_per_s": s["visits"] * (s["value"] / s["steps"]),
-            "walk_steps_per_pass": s["steps"], "visits_per_pass": s["visits"], "kernel_ms": s["kernel_ms"],
-            "roofline_achieved_GBps": s["achieved"], "roofline_frac": s["achieved"] / PEAK_HBM_GBPS,
-            "plain_csr_walk_steps_per_s": st_ / w_, "speedup_vs_plain": s["value"] / (st_ / w_),
-            "plain_csr_hash_walk_steps_per_s": sth_ / wh_, "speedup_vs_plain_hash": s["value"] / (sth_ / wh_),
-            "schedule": f"{args.schedule}-{args.layout}",
-            "source_method": {gw.SRC_PARTITIONED: "partitioned", gw.SRC_GLOBAL: "global-sort",
-                              gw.SRC_HASH: "hash"}.get(last["source_method"], "?"),
-            "reduce_ms": last["reduce_ms"], "flushed_sources": last["flushed_sources"],
-            "global_sort_walk_steps_per_s": stg_ / wg_, "global_sort_reduce_ms": lg_["reduce_ms"],
-            "launch_shape": s["tuned"], "graph_replay": s.get("graph_replay"),
-        }
+        if t.get("csrc_sha256") == csrc_digest():
+            traffic, traffic_copy = t.get("bytes_per_launch"), t.get("bytes_per_launch_copy_calibrated")
+            traffic_note = f"{os.path.relpath(tfile, ROOT)} (same library sources)"
+        else:
+            traffic_note = f"{os.path.relpath(tfile, ROOT)} was measured on other library sources: not used"
 
     serving = {}
     if args.serve_qps:
-        # continuous batching on the graph left resident by the last workload
-        n_last = 1 << (WORKLOADS[args.secondary]["scale"] if workloads else wl["scale"])
+        # continuous batching on the headline's graph, resident in HBM
         for q in args.serve_qps:
-            r = serve_sim(eng, n_last, q, args.serve_seconds, hops=args.serve_hops, topk=args.serve_topk)
+            r = serve_sim(eng, m["n"], q, args.serve_seconds, hops=args.serve_hops, topk=args.serve_topk)
             serving[f"qps_{int(q)}"] = r
             print(f"[bench] serving {q:.0f} qps: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
                   f"{r['walk_steps_per_s'] / 1e9:.2f} G steps/s", file=sys.stderr, flush=True)
 
     stream = {}
     if args.stream_cache_mib:
-        wl_last = WORKLOADS[args.secondary] if workloads else wl
-        stream = stream_measure(eng, wl_last["scale"], wl_last["seed"], args.stream_cache_mib << 20,
-                                args.stream_queries, 2)
+        stream = stream_measure(eng, wl["scale"], wl["seed"], args.stream_cache_mib << 20, args.stream_queries, 2)
 
     simrank = {}
     if args.simrank_scale:
         simrank = simrank_measure(eng, args.simrank_scale, 4, 1024, 1000, 11, 0.6, 3)
+    eng.close()
 
+    serve_ooc = {}
+    if args.ooc_scale:
+        try:
+            serve_ooc = serve_ooc_measure(args.ooc_scale, 24, 5, 64.0 / 223.0, int(args.ooc_cache_gb * 1e9),
+                                          args.ooc_qps, args.ooc_seconds, args.ooc_warm_seconds, args.serve_hops,
+                                          args.serve_topk)
+            print(f"[bench] serving over host-resident scale {serve_ooc['scale']}: p50 {serve_ooc['p50_ms']:.1f} ms "
+                  f"p99 {serve_ooc['p99_ms']:.1f} ms, {serve_ooc['walk_steps_per_s'] / 1e9:.3f} G steps/s",
+                  file=sys.stderr, flush=True)
+        except gw.GwError as exc:
+            serve_ooc = {"error": str(exc)}
+            print(f"[bench] serving over a host-resident graph failed: {exc}", file=sys.stderr, flush=True)
+
+    frac = m["achieved"] / PEAK_HBM_GBPS
+    roofline = {"bound": "hbm", "achieved": m["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": frac,
+                "traffic": traffic, "traffic_copy_calibrated": traffic_copy, "traffic_source": traffic_note,
+                "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": m["kernel_ms"],
+                "kernel_ms_median": m["kernel_ms_median"], "algorithmic_bytes_per_launch": m["algorithmic_bytes"],
+                "bytes_per_step": bytes_per_step(wl["recorder"]),
+                "pass_frac": m["algorithmic_bytes"] / ((m["kernel_ms"] + m["reduce_ms"]) * 1e-3) / 1e9 / PEAK_HBM_GBPS}
+    if traffic:
+        roofline["traffic_frac"] = traffic / (m["kernel_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBPS
     out = {
         "metric": "walk-steps/s",
         "value": m["value"],
@@ -570,14 +766,9 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic (R-MAT generated on the device)",
         "config": {"workload": args.workload, "desc": wl["desc"], "walk_steps_per_pass": m["steps"],
-                   "schedule": args.schedule, "layout": args.layout, "device": info["name"]},
-        "roofline": {"bound": "hbm", "achieved": m["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": m["achieved"] / PEAK_HBM_GBPS, "traffic": traffic,
-                     "traffic_copy_calibrated": traffic_copy,
-                     "kernel": f"step_{args.schedule}_kernel ({args.layout})", "kernel_ms": m["kernel_ms"],
-                     "kernel_ms_median": m["kernel_ms_median"],
-                     "algorithmic_bytes_per_launch": m["algorithmic_bytes"],
-                     "bytes_per_step": m["algorithmic_bytes"] / max(1, m["steps"])},
+                   "visits_per_pass": m["visits"], "schedule": args.schedule, "layout": args.layout,
+                   "device": info["name"]},
+        "roofline": roofline,
         "random_access_ceiling": {"gathers_per_s": gathers / (gms * 1e-3), "GBps_at_64B": rand_gbps,
                                   "frac_of_ceiling": m["achieved"] / rand_gbps},
         "copy_GBps": 2 * copy_bytes / (cms * 1e-3) / 1e9,
@@ -586,16 +777,25 @@ def main() -> int:
         "graph_replay": m.get("graph_replay"),
         "workloads": workloads,
         "serving": serving,
+        "serving_out_of_core": serve_ooc,
         "out_of_core": stream,
         "workloads_simrank": simrank,
         "post_processing": post,
         "host_walk_pool": pool,
     }
     if "speedup_vs_plain" in m:
-        out["speedup_vs_plain"] = m["speedup_vs_plain"]
+        out["speedup_vs_plain_same_recorder"] = m["speedup_vs_plain"]
+    if plain_global:
+        out["speedup_vs_plain_global_sort"] = m["value"] / plain_global
+        out["speedup_vs_plain_hash"] = m["value"] / plain_hash
+        out["plain_recorders_walk_steps_per_s"] = {"global_sort": plain_global, "hash": plain_hash}
+        out["global_sort"] = global_sort
+        out["source_method"] = {gw.SRC_PARTITIONED: "partitioned", gw.SRC_GLOBAL: "global-sort",
+                                gw.SRC_HASH: "hash"}.get(m["last"]["source_method"], "?")
+        out["reduce_ms"] = m["reduce_ms"]
+        out["flushed_sources"] = m["last"]["flushed_sources"]
     if cpu:
         out["cpu_baseline"] = cpu
-    eng.close()
     print(json.dumps(out), flush=True)
     return 0
 

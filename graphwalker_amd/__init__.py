@@ -205,6 +205,7 @@ def lib() -> C.CDLL:
         "gw_gather_probe": ([P, u64, u64, P], i),
         "gw_copy_probe": ([P, u64, P], i),
         "gw_h2d_probe": ([P, u64, P], i),
+        "gw_host_gather_probe": ([P, u64, u64, P], i),
         "gw_sources": ([u64, u32, u32, P], i),
         "gw_serve_open": ([P, P, u64, u64, u32, u32], i),
         "gw_serve_submit": ([P, P, u32], i),
@@ -452,6 +453,12 @@ class Engine:
     def h2d_probe(self, nbytes: int) -> float:
         ms = C.c_double()
         self._check(self._L.gw_h2d_probe(self._ctx, nbytes, C.byref(ms)), "gw_h2d_probe")
+        return ms.value
+
+    def host_gather_probe(self, nbytes: int, gathers: int) -> float:
+        """Kernel ms for `gathers` random 4-byte reads of pinned host memory."""
+        ms = C.c_double()
+        self._check(self._L.gw_host_gather_probe(self._ctx, nbytes, gathers, C.byref(ms)), "gw_host_gather_probe")
         return ms.value
 
     def copy_probe(self, nbytes: int) -> float:

@@ -311,6 +311,7 @@ int gw_rmat_build_host(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, u
     hipError_t e = hipHostMalloc(&h_off, (n + 1ull) * 8, hipHostMallocDefault);
     if (e == hipSuccess && m) e = hipHostMalloc(&h_col, m * 4, hipHostMallocDefault);
     if (e != hipSuccess) {
+        (void)hipGetLastError();  // a refused allocation must not surface at the next launch check
         if (h_off) (void)hipHostFree(h_off);
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);

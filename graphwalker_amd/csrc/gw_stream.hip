@@ -14,6 +14,7 @@
 #include <cstring>
 #include <vector>
 
+#include "gw_device.h"
 #include "gw_internal.h"
 
 namespace {
@@ -43,6 +44,21 @@ std::vector<uint32_t> cut_blocks(const gw_ctx* ctx, uint64_t target) {
         lo = hi;
     }
     return bf;
+}
+
+// Independent random 4-byte reads of pinned host memory by a kernel: index
+// i's address comes from Philox(i), so a lane's reads are all in flight
+// together (the column reads of pages a server reads from host memory).
+__global__ void host_gather_kernel(const uint32_t* __restrict__ buf, uint32_t nwords, uint64_t gathers,
+                                   uint32_t* __restrict__ out) {
+    const uint64_t tid = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    uint32_t acc = 0;
+    for (uint64_t i = tid; i < gathers; i += stride) {
+        const gw::u32x4 r = gw::philox4x32_10(static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0x484F5354u, 0, 3, 4);
+        acc ^= buf[gw::mulhi32(r.x, nwords)];
+    }
+    out[tid] = acc;
 }
 
 }  // namespace
@@ -144,6 +160,7 @@ int gw_host_alloc(gw_ctx* ctx, uint64_t bytes, void** out) {
     GW_HIP(ctx, hipSetDevice(ctx->device));
     hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
     if (e != hipSuccess) {
+        (void)hipGetLastError();  // a refused allocation must not surface at the next launch check
         *out = nullptr;
         return gw_fail(ctx, GW_ENOMEM, "gw_host_alloc: hipHostMalloc(" + std::to_string(bytes) + ") failed: " +
                                            hipGetErrorString(e));
@@ -201,6 +218,42 @@ int gw_graph_set_host(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uin
         ctx->h_col = static_cast<const uint32_t*>(ctx->h_owned_col);
     }
     return gw_host_graph_adopt(ctx, n, m, cache_bytes);
+}
+
+// Random-read ceiling of pinned host memory from a kernel: `gathers`
+// independent 4-byte reads over a pinned buffer of `bytes` bytes (kernel time
+// in ms, after a warm-up of an eighth as many).
+int gw_host_gather_probe(gw_ctx* ctx, uint64_t bytes, uint64_t gathers, double* ms) {
+    if (!ctx || bytes < 4096 || gathers == 0 || bytes / 4 > 0xFFFFFFFFull)
+        return gw_fail(ctx, GW_EINVAL, "gw_host_gather_probe: bytes in [4096, 16 GiB], gathers > 0");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    void* h = nullptr;
+    GW_HIP(ctx, hipHostMalloc(&h, bytes, hipHostMallocDefault));
+    std::memset(h, 1, bytes);
+    uint32_t* out = nullptr;
+    const uint32_t tb = 256, grid = std::max(1u, ctx->cu_count) * 64;
+    int rc = gw_alloc(ctx, (void**)&out, static_cast<uint64_t>(grid) * tb * 4, "host gather out");
+    if (rc) {
+        (void)hipHostFree(h);
+        return rc;
+    }
+    const uint32_t nwords = static_cast<uint32_t>(bytes / 4);
+    host_gather_kernel<<<dim3(grid), dim3(tb), 0, ctx->stream>>>(static_cast<const uint32_t*>(h), nwords, gathers / 8 + 1, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess) {
+        host_gather_kernel<<<dim3(grid), dim3(tb), 0, ctx->stream>>>(static_cast<const uint32_t*>(h), nwords, gathers, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    gw_free(ctx, out, static_cast<uint64_t>(grid) * tb * 4);
+    (void)hipHostFree(h);
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "host gather probe");
+    if (ms) *ms = t;
+    return GW_OK;
 }
 
 // Host-to-device copy ceiling: `bytes` from pinned host memory into HBM on

@@ -412,6 +412,11 @@ int gw_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);
 /* Streaming copy of `bytes` bytes between two fresh buffers (the known-size
  * kernel hardware byte counters are calibrated with); kernel time in ms. */
 int gw_copy_probe(gw_ctx *ctx, uint64_t bytes, double *ms);
+/* Random-read ceiling of pinned host memory from a kernel (how a server over
+ * a host-resident graph reads the columns of pages not in its cache):
+ * `gathers` independent random 4-byte reads over a pinned buffer of `bytes`
+ * bytes (at most 16 GiB); kernel time in ms. */
+int gw_host_gather_probe(gw_ctx *ctx, uint64_t bytes, uint64_t gathers, double *ms);
 /* Host-to-device copy of `bytes` from pinned host memory on the block-copy
  * stream (the path out-of-core block loads take); copy time in ms. */
 int gw_h2d_probe(gw_ctx *ctx, uint64_t bytes, double *ms);

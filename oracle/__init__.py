@@ -348,3 +348,33 @@ def run_reference_graphlet(base: str, n: int, R: int, L: int, threads: int, time
         if line.startswith("cnt_ok:"):
             return int(line.split()[-1])
     raise RuntimeError("the reference graphlet app printed no count")
+
+
+def run_reference_msppr_sources(base: str, sources: np.ndarray, wps: int, L: int, threads: int, timeout: float,
+                                blocksize_kb: int, nmblocks: int = 1) -> dict:
+    """The reference engine on batched multi-source PPR: walks_per_source
+    stop walks (0.15, cap L) from each vertex of `sources`, recorded as msppr
+    records them (oracle/ref_msppr_sources.cpp).  Returns its own timers
+    (00_runtime, 5_exec_updates, 0_startWalks, in seconds), the visit total,
+    and source 0's (vertex, count) distribution.  blocksize_kb: the engine's
+    block size (a block at least the whole column file keeps the graph in
+    one in-memory block)."""
+    src_path = base + ".sources"
+    np.ascontiguousarray(sources, dtype=np.uint32).tofile(src_path)
+    out = base + ".top"
+    text = _run_reference("msppr_sources", base, {"sources": src_path, "walkspersource": wps, "L": L, "out": out,
+                                                  "execthreads": threads, "blocksize_kb": blocksize_kb,
+                                                  "nmblocks": nmblocks, "prob": 0.2}, timeout)
+    res = {}
+    with open(os.path.join(os.path.dirname(base), "graphwalker_metrics.txt")) as f:
+        for line in f:
+            key, _, val = line.strip().partition("=")
+            for k in ("00_runtime", "5_exec_updates", "0_startWalks"):
+                if key.endswith("." + k):
+                    res[k] = float(val)
+    for line in text.splitlines():
+        if line.startswith("visits:"):
+            res["visits"] = int(line.split()[-1])
+    pairs = np.fromfile(out, dtype=np.uint32).reshape(-1, 2)
+    res["source0_vertex"], res["source0_count"] = pairs[:, 0].copy(), pairs[:, 1].copy()
+    return res

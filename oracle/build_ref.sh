@@ -25,6 +25,10 @@ HERE="$(cd "$(dirname "$0")" && pwd)"
 if [ ! -x "$OUT/visit_counts" ] || [ "$HERE/ref_visit_counts.cpp" -nt "$OUT/visit_counts" ]; then
     g++ -O3 -fopenmp -w -I "$REF/src" "$HERE/ref_visit_counts.cpp" -o "$OUT/visit_counts" -lz
 fi
+# ... and its batched multi-source PPR on a list of sources (oracle/ref_msppr_sources.cpp).
+if [ ! -x "$OUT/msppr_sources" ] || [ "$HERE/ref_msppr_sources.cpp" -nt "$OUT/msppr_sources" ]; then
+    g++ -O3 -fopenmp -w -I "$REF/src" "$HERE/ref_msppr_sources.cpp" -o "$OUT/msppr_sources" -lz
+fi
 echo "build_ref: built $(ls "$OUT" | tr '\n' ' ')"
 
 # The same reference apps compiled, unchanged, against this repository's

@@ -449,6 +449,33 @@ def test_reference_engine_ppr_within_mc_bound(tmp_path, W):
     assert l1_ref < bound and l1_ours < bound, (l1_ref, l1_ours, bound)
 
 
+@pytest.mark.skipif(O.reference_app("msppr_sources") is None, reason="reference apps not built (oracle/build_ref.sh)")
+def test_reference_engine_multisource_ppr_within_mc_bound(tmp_path):
+    # the CPU-baseline app for the batched PPR workload (the reference engine,
+    # msppr's recorder, a list of uniform sources): it places and steps every
+    # walk (visit total within the MC bound of the FP64 expectation over all
+    # sources), and source 0's distribution -- exact below the recorder's
+    # 1000-entry capacity on a 1024-vertex graph -- is within the MC bound of
+    # the FP64 single-source vector
+    off, col = O.rmat_graph(10, 1 << 14, seed=3)
+    n = off.shape[0] - 1
+    base = str(tmp_path / "g")
+    O.write_reference_graph(base, off, col)
+    src = O.sources(3, n, 16)
+    W, L = 4000, 20
+    ref = O.run_reference_msppr_sources(base, src, W, L, 8, timeout=120, blocksize_kb=1024)
+    assert ref["00_runtime"] > 0
+    exp_all = sum(fp64.expected_visits(O.STOP, off, col, int(s), W, L, 0.15).sum() for s in src)
+    sd = np.sqrt(len(src) * W * L * L)  # a walk makes at most L visits
+    assert abs(ref["visits"] - exp_all) <= 5 * sd / 4, (ref["visits"], exp_all)
+    c = np.zeros(n, np.uint64)
+    c[ref["source0_vertex"]] = ref["source0_count"]
+    assert ref["source0_vertex"].shape[0] < 1000
+    exp = fp64.expected_visits(O.STOP, off, col, int(src[0]), W, L, 0.15)
+    bound = fp64.mc_l1_bound(exp, W, exp.sum() / W, L)
+    assert fp64.l1_normalised(c, exp) < bound
+
+
 @_REF_SKIP
 def test_reference_engine_simrank_within_mc_bound(tmp_path):
     # the reference simrank app's score (first meetings weighed 0.8^l) against
