@@ -45,6 +45,17 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1
 }
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+
+// Lane `lane`'s value of x, for a wave-uniform lane index (v_readlane into a
+// scalar register: no LDS crossbar round trip as a shuffle would take).
+__device__ __forceinline__ uint32_t lane_value(uint32_t x, uint32_t lane) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), static_cast<int>(lane)));
+}
+__device__ __forceinline__ unsigned long long lane_value(unsigned long long x, uint32_t lane) {
+    const uint32_t lo = lane_value(static_cast<uint32_t>(x), lane);
+    const uint32_t hi = lane_value(static_cast<uint32_t>(x >> 32), lane);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
 
 // Program parameters as the kernels see them (kind is a template argument).
@@ -175,7 +186,7 @@ __device__ __forceinline__ void keylog_add(const KeyLog& k, uint64_t* slab, uint
     }
     unsigned long long b = 0;
     if (rank == 0) b = atomicAdd(k.cursor, static_cast<unsigned long long>(k.slab));
-    b = __shfl(b, leader, 64);
+    b = lane_value(b, leader);
     const bool ok = b + k.slab <= k.cap;
     if (rank < room) k.keys[next + rank] = key;
     else if (ok) k.keys[b + (rank - room)] = key;
@@ -220,7 +231,7 @@ __device__ __forceinline__ void region_flush(const KeyLog& k, const uint32_t* ke
     const uint32_t leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
     unsigned long long at = 0;
     if (lane == leader) at = atomicAdd(&k.region_n[f], static_cast<unsigned long long>(n));
-    at = __shfl(at, leader, 64);
+    at = lane_value(at, leader);
     const uint64_t base = k.region_off[f] + at, hi = k.region_off[f + 1];
     const uint32_t m = __popcll(active);
     for (uint32_t i = lanes_below(active); i < n; i += m)
@@ -243,7 +254,7 @@ __device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint3
     uint64_t fdo = __ballot(fvis);
     while (fdo) {  // wave-uniform
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(fdo)) - 1;
-        const uint32_t ls = __shfl(source, leader, 64);
+        const uint32_t ls = lane_value(source, leader);
         const uint64_t grp = __ballot(fvis && source == ls);
         if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], static_cast<uint32_t>(__popcll(grp)));
         fdo &= ~grp;
@@ -251,7 +262,7 @@ __device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint3
     uint64_t todo = __ballot(!fvis);
     while (todo) {  // wave-uniform
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-        const uint32_t ls = __shfl(source, leader, 64);
+        const uint32_t ls = lane_value(source, leader);
         const bool in = !fvis && source == ls;
         const uint64_t grp = __ballot(in);
         todo &= ~grp;
@@ -284,6 +295,83 @@ __device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint3
     }
 }
 
+// The same direct regions for a caller that records in wave-uniform control
+// flow (every lane of the wavefront present, `want` marking the lanes with a
+// visit): the two buffers' {field, count} state lives in wave-uniform
+// registers instead of LDS, and group leaders broadcast with readlane.
+struct RegionState {
+    uint32_t f0 = kNoField, c0 = 0, f1 = kNoField, c1 = 0;
+};
+
+__device__ __forceinline__ void region_flush_all(const KeyLog& k, const uint32_t* keys, uint32_t f, uint32_t n) {
+    if (n == 0) return;
+    const uint32_t lane = __lane_id();
+    unsigned long long at = 0;
+    if (lane == 0) at = atomicAdd(&k.region_n[f], static_cast<unsigned long long>(n));
+    at = lane_value(at, 0u);
+    const uint64_t base = k.region_off[f] + at, hi = k.region_off[f + 1];
+    for (uint32_t i = lane; i < n; i += 64)
+        if (base + i < hi) k.region_out[base + i] = keys[i];
+    if (lane == 0 && base + n > hi)
+        atomicAdd(k.region_overflow, static_cast<unsigned long long>(base + n - (base > hi ? base : hi)));
+    __builtin_amdgcn_wave_barrier();  // every lane has read the ids before they are overwritten
+}
+
+__device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* rb, RegionState& st, bool want,
+                                                   uint32_t source, uint32_t v, uint32_t field_v) {
+    const uint32_t lane = __lane_id();
+    // visits of the field's designated vertex: one count per (wavefront, field)
+    const bool fvis = want && v == field_v;
+    uint64_t fdo = __ballot(fvis);
+    while (fdo) {
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(fdo)) - 1;
+        const uint32_t ls = lane_value(source, leader);
+        const uint64_t grp = __ballot(fvis && source == ls);
+        if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], static_cast<uint32_t>(__popcll(grp)));
+        fdo &= ~grp;
+    }
+    const bool rest = want && !fvis;
+    uint64_t todo = __ballot(rest);
+    while (todo) {
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+        const uint32_t ls = lane_value(source, leader);
+        const bool in = rest && source == ls;
+        const uint64_t grp = __ballot(in);
+        todo &= ~grp;
+        const uint32_t g = __popcll(grp);
+        if (ls >= k.nf) {
+            if (lane == leader) atomicAdd(k.region_overflow, static_cast<unsigned long long>(g));
+            continue;
+        }
+        bool second;
+        if (st.f0 == ls) {
+            second = false;
+        } else if (st.f1 == ls) {
+            second = true;
+        } else {  // take an empty buffer, else flush the fuller one
+            second = st.c0 != 0 && (st.c1 == 0 || st.c1 > st.c0);
+            if (second) {
+                region_flush_all(k, rb->key[1], st.f1, st.c1);
+                st.f1 = ls;
+                st.c1 = 0;
+            } else {
+                region_flush_all(k, rb->key[0], st.f0, st.c0);
+                st.f0 = ls;
+                st.c0 = 0;
+            }
+        }
+        uint32_t* keys = rb->key[second ? 1 : 0];
+        uint32_t c = second ? st.c1 : st.c0;
+        if (c + g > kRegionBuf) {
+            region_flush_all(k, keys, ls, c);
+            c = 0;
+        }
+        if (in) keys[c + lanes_below(grp)] = v;
+        __builtin_amdgcn_wave_barrier();
+        if (second) st.c1 = c + g; else st.c0 = c + g;
+    }
+}
+
 // At kernel exit (all lanes of the wavefront present): flush both buffers.
 __device__ __forceinline__ void region_close(const KeyLog& k, RegionBuf* rb) {
     const uint64_t all = __ballot(1);
@@ -300,7 +388,7 @@ __device__ __forceinline__ void keylog_close(const KeyLog& k, uint64_t* slab, ui
 // Recorder hooks, compile-time selected: the device replacements for the
 // per-step updateInfo callback (randomwalk.hpp:32).
 template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false, bool kKeyLog = false,
-          bool kRegion = false>
+          bool kRegion = false, bool kDeferRegion = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
@@ -314,6 +402,11 @@ struct Recorder {
     uint64_t* slab;
     RegionBuf* rbuf;
     uint32_t field_v = 0xFFFFFFFFu;  // the current walk's field's designated vertex (direct regions)
+    // kDeferRegion: the step's visit, handed to the regions by commit() in
+    // wave-uniform control flow, and the regions' buffer state
+    uint32_t pend_v = 0;
+    bool pend = false;
+    RegionState rst;
     __device__ __forceinline__ Recorder(uint32_t* c, const EventLog& l, const SourceTable& t, const KeyLog& kl,
                                         uint64_t* sl, RegionBuf* rb = nullptr)
         : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl),
@@ -331,7 +424,7 @@ struct Recorder {
             uint64_t todo = __ballot(at_src);
             while (todo) {
                 const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-                const uint32_t ls = __shfl(source, leader, 64);
+                const uint32_t ls = lane_value(source, leader);
                 const uint64_t grp = __ballot(at_src && source == ls);
                 if (__lane_id() == leader) {
                     const uint32_t c = static_cast<uint32_t>(__popcll(grp));
@@ -347,7 +440,29 @@ struct Recorder {
         }
         if (kSummary) hash = (hash ^ ((static_cast<uint64_t>(v) << 32) | hop)) * kHashPrime;
         if (kKeyLog) keylog_add(klog, slab, source, v);
-        if (kRegion) region_add(klog, rbuf, source, v, field_v);
+        if (kRegion && kDeferRegion) {
+            pend_v = v;
+            pend = true;
+        } else if (kRegion) {
+            region_add(klog, rbuf, source, v, field_v);
+        }
+    }
+    // kDeferRegion: every lane of the wavefront calls it once per step, after
+    // the step (each step makes at most one visit per lane).
+    __device__ __forceinline__ void commit() {
+        if (kRegion && kDeferRegion) {
+            region_add_uniform(klog, rbuf, rst, pend, source, pend_v, field_v);
+            pend = false;
+        }
+    }
+    // At kernel exit, every lane present.
+    __device__ __forceinline__ void close_regions() {
+        if (kRegion && kDeferRegion) {
+            region_flush_all(klog, rbuf->key[0], rst.f0, rst.c0);
+            region_flush_all(klog, rbuf->key[1], rst.f1, rst.c1);
+        } else if (kRegion) {
+            region_close(klog, rbuf);
+        }
     }
 };
 

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void rmat_range_keys_kernel(uint64_t m, uint32
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
             unsigned long long at = 0;
             if (lane == leader) at = atomicAdd(cursor, static_cast<unsigned long long>(__popcll(mask)));
-            at = __shfl(at, leader, 64);
+            at = gw::lane_value(at, leader);
             const uint64_t pos = at + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
             if (keep && pos < cap) keys[pos] = key;

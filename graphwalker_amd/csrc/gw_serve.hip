@@ -226,11 +226,11 @@ struct ServeRec {
         uint64_t pend = __ballot(cl);
         while (pend) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(pend)) - 1;
-            const uint32_t ls = __shfl(sl, leader, 64);
+            const uint32_t ls = lane_value(sl, leader);
             const uint64_t grp = __ballot(cl && sl == ls);
             uint32_t b0 = 0;
             if (__lane_id() == leader) b0 = atomicAdd(&occn[ls], static_cast<uint32_t>(__popcll(grp)));
-            b0 = __shfl(b0, leader, 64);
+            b0 = lane_value(b0, leader);
             if (cl && sl == ls) occ[rb + b0 + rank_of(grp)] = claimed;
             pend &= ~grp;
         }
@@ -274,7 +274,7 @@ struct ServeRec {
         uint64_t todo = __ballot(at_src);
         while (todo) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-            const uint32_t ls = __shfl(slot, leader, 64);
+            const uint32_t ls = lane_value(slot, leader);
             const uint64_t grp = __ballot(at_src && slot == ls);
             if (__lane_id() == leader) atomicAdd(&srccount[ls], static_cast<uint32_t>(__popcll(grp)));
             todo &= ~grp;
@@ -287,8 +287,8 @@ struct ServeRec {
         uint64_t fold = __ballot(mine);
         for (uint32_t it = 0; it < 3 && fold; ++it) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(fold)) - 1;
-            const uint32_t v0 = __shfl(v, leader, 64);
-            const uint32_t s0 = __shfl(slot, leader, 64);
+            const uint32_t v0 = lane_value(v, leader);
+            const uint32_t s0 = lane_value(slot, leader);
             const bool in = mine && v == v0 && slot == s0;
             const uint64_t grp = __ballot(in);
             if (in) {
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(cursor, static_cast<unsigned long long>(chunk));
-                base = __shfl(base, 0, 64);
+                base = lane_value(base, 0u);
                 if (base >= n) { exhausted = true; break; }
                 chunk_next = base;
                 chunk_end = base + chunk < n ? base + chunk : n;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
         uint64_t todo = __ballot(done);
         while (todo) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-            const uint32_t ls = __shfl(slot, leader, 64);
+            const uint32_t ls = lane_value(slot, leader);
             const uint64_t grp = __ballot(done && slot == ls);
             if (lane == leader) atomicSub(&remaining[ls], static_cast<uint32_t>(__popcll(grp)));
             todo &= ~grp;
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(lm)) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(&cnt[0], static_cast<unsigned long long>(__popcll(lm)));
-            base = __shfl(base, leader, 64);
+            base = lane_value(base, leader);
             if (leave) pool_out[base + rank_of(lm)] = gw_walk{s.id, slot, s.v, s.hop};
         }
     }

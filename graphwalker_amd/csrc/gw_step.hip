@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     }
     __shared__ RegionBuf s_rbuf[RG ? 4 : 1];
     if (RG && (threadIdx.x & 63u) == 0) region_buf_init(&s_rbuf[threadIdx.x >> 6]);
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG> rec(
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG, RG> rec(
         counts, log, table, klog, s_slab[threadIdx.x >> 6], RG ? &s_rbuf[threadIdx.x >> 6] : nullptr);
 
     for (;;) {
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(p.chunk));
-                base = __shfl(base, 0, 64);
+                base = lane_value(base, 0u);
                 if (base >= nw) {
                     exhausted = true;
                     break;
@@ -170,9 +170,11 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 active = false;
             }
         }
+        // the step's visit into its source's region, with every lane present
+        if constexpr (RG) rec.commit();
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
-    if (RG) region_close(klog, &s_rbuf[threadIdx.x >> 6]);
+    if (RG) rec.close_regions();
     const uint32_t st = wave_sum(tot_steps);
     const uint32_t vi = wave_sum(tot_visits);
     if (lane == 0) {
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(&counters[2], static_cast<unsigned long long>(p.chunk));
-                base = __shfl(base, 0, 64);
+                base = lane_value(base, 0u);
                 if (base >= nw) {
                     exhausted = true;
                     break;

@@ -48,7 +48,7 @@ __global__ void compact_kernel(const unsigned long long* __restrict__ tab, uint6
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(n, static_cast<unsigned long long>(__popcll(mask)));
-        base = __shfl(base, leader, 64);
+        base = gw::lane_value(base, leader);
         if (full) {
             const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void partition_kernel(const unsigned long long
             }
             unsigned long long at = 0;
             if (lane == 0) at = atomicAdd(&region_n[fl], static_cast<unsigned long long>(tot));
-            at = __shfl(at, 0, 64);
+            at = gw::lane_value(at, 0u);
             const uint64_t lo = region_off[fl], hi = region_off[fl + 1];
 #pragma unroll
             for (uint32_t j = 0; j < kPartPerLane; ++j) {
