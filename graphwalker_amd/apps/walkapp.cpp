@@ -8,7 +8,7 @@
 //
 // Walk i starts at vertex (source + i) mod N (all at `source` with
 // samesource 1) with hop 0; its source field is its start vertex.  Every
-// visit increments a per-vertex u32 count (rwdomination.cpp:286-289); the
+// visit increments a per-vertex u32 count (rwdomination.cpp:63-66); the
 // counts are written to `out` as raw uint32.
 #include <string>
 #include <type_traits>

@@ -67,6 +67,34 @@ def build_apps() -> list[str]:
     return out
 
 
+REFERENCE_APPS = ("rawrandomwalks", "msppr", "simrank", "graphlet", "rwdomination", "kk_ppr")
+
+
+def build_compat_apps(reference: str = "/root/reference") -> list[str]:
+    """The reference's own apps, compiled unchanged from where they lie
+    against this package's GraphWalker API headers (cxx/) and linked to
+    lib/libgwhip.so -> bin/compat/ (the drop-in boundary).  Skipped where the
+    reference tree is absent; nothing of it is copied."""
+    apps_dir = os.path.join(reference, "apps")
+    if not os.path.isdir(apps_dir):
+        return []
+    out_dir = os.path.join(BIN, "compat")
+    os.makedirs(out_dir, exist_ok=True)
+    out = []
+    for app in REFERENCE_APPS:
+        src = os.path.join(apps_dir, app + ".cpp")
+        if not os.path.exists(src):
+            continue
+        exe = os.path.join(out_dir, app)
+        cmd = ["g++", "-std=c++17", "-O2", "-fopenmp", "-w", "-I", CXX_INC, "-I", C_INC, src, "-L", LIB_DIR, "-lgwhip",
+               "-Wl,-rpath,$ORIGIN/../../lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compat app build failed: {' '.join(cmd)}\n{r.stderr}")
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
@@ -80,8 +108,9 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     apps = build_apps()
+    compat = build_compat_apps()
     if verbose:
-        print(f"built {LIB} and {len(apps)} apps")
+        print(f"built {LIB}, {len(apps)} apps and {len(compat)} unchanged reference apps")
     return LIB
 
 

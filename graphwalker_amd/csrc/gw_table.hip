@@ -10,7 +10,10 @@
 //    at read-out.
 // Read-out unpacks (source, vertex, count) triples sorted by source, vertex.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -543,19 +546,17 @@ int gw_keylog_reduce(gw_ctx* ctx, uint64_t n, double* ms) {
     int rc0 = ensure_runs(ctx, n);
     if (rc0) return rc0;
     size_t need_sort = 0, need_rle = 0;
-    GW_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, n, 0, end_bit,
+    GW_HIP(ctx, rocprim::radix_sort_keys(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, n, 0, end_bit,
                                                   ctx->stream));
-    GW_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(nullptr, need_rle, ctx->d_keys_alt, ctx->d_runs,
-                                                      ctx->d_run_counts, ctx->d_counters + kCntRuns, n, ctx->stream));
+    GW_HIP(ctx, rocprim::run_length_encode(nullptr, need_rle, ctx->d_keys_alt, n, ctx->d_runs, ctx->d_run_counts, ctx->d_counters + kCntRuns, ctx->stream));
     if ((rc0 = ensure_sort_temp(ctx, std::max(need_sort, need_rle)))) return rc0;
     size_t tb = ctx->sort_temp_bytes;
     GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (n) {
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_keys_alt, n, 0, end_bit,
+        GW_HIP(ctx, rocprim::radix_sort_keys(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_keys_alt, n, 0, end_bit,
                                                       ctx->stream));
         tb = ctx->sort_temp_bytes;
-        GW_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(ctx->d_sort_temp, tb, ctx->d_keys_alt, ctx->d_runs,
-                                                          ctx->d_run_counts, ctx->d_counters + kCntRuns, n, ctx->stream));
+        GW_HIP(ctx, rocprim::run_length_encode(ctx->d_sort_temp, tb, ctx->d_keys_alt, n, ctx->d_runs, ctx->d_run_counts, ctx->d_counters + kCntRuns, ctx->stream));
     } else {
         GW_HIP(ctx, hipMemsetAsync(ctx->d_counters + kCntRuns, 0, 8, ctx->stream));
     }
@@ -686,20 +687,16 @@ int gw_partlog_finish(gw_ctx* ctx, const unsigned long long* cnt, bool* done, do
         uint32_t* cnt_in = ctx->d_split_counts;
         uint32_t* cnt_sorted = ctx->d_split_counts + ctx->split_cap;
         size_t need_sort = 0, need_red = 0;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, cnt_in,
+        GW_HIP(ctx, rocprim::radix_sort_pairs(nullptr, need_sort, ctx->d_keys, ctx->d_keys_alt, cnt_in,
                                                        cnt_sorted, split, 0, end_bit, ctx->stream));
-        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(nullptr, need_red, ctx->d_keys_alt, ctx->d_runs, cnt_sorted,
-                                                      ctx->d_run_counts, ctx->d_counters + kCntSplitRuns,
-                                                      hipcub::Sum(), split, ctx->stream));
+        GW_HIP(ctx, rocprim::reduce_by_key(nullptr, need_red, ctx->d_keys_alt, cnt_sorted, split, ctx->d_runs, ctx->d_run_counts, ctx->d_counters + kCntSplitRuns, rocprim::plus<uint32_t>(), rocprim::equal_to<unsigned long long>(), ctx->stream));
         if ((rc = ensure_sort_temp(ctx, std::max(need_sort, need_red)))) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
         size_t tb = ctx->sort_temp_bytes;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_keys_alt, cnt_in,
+        GW_HIP(ctx, rocprim::radix_sort_pairs(ctx->d_sort_temp, tb, ctx->d_keys, ctx->d_keys_alt, cnt_in,
                                                        cnt_sorted, split, 0, end_bit, ctx->stream));
         tb = ctx->sort_temp_bytes;
-        GW_HIP(ctx, hipcub::DeviceReduce::ReduceByKey(ctx->d_sort_temp, tb, ctx->d_keys_alt, ctx->d_runs, cnt_sorted,
-                                                      ctx->d_run_counts, ctx->d_counters + kCntSplitRuns,
-                                                      hipcub::Sum(), split, ctx->stream));
+        GW_HIP(ctx, rocprim::reduce_by_key(ctx->d_sort_temp, tb, ctx->d_keys_alt, cnt_sorted, split, ctx->d_runs, ctx->d_run_counts, ctx->d_counters + kCntSplitRuns, rocprim::plus<uint32_t>(), rocprim::equal_to<unsigned long long>(), ctx->stream));
         GW_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         unsigned long long r = 0;
         GW_HIP(ctx, hipMemcpyAsync(&r, ctx->d_counters + kCntSplitRuns, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -745,10 +742,10 @@ static int partitioned_sorted(gw_ctx* ctx, unsigned long long** keys_out, uint32
     }
     const int end_bit = static_cast<int>(64 - ctx->table_cbits);
     size_t need = 0;
-    GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
+    GW_HIP(ctx, rocprim::radix_sort_pairs(nullptr, need, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
     if ((rc = ensure_sort_temp(ctx, need))) return rc;
     size_t tb = ctx->sort_temp_bytes;
-    GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_temp, tb, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
+    GW_HIP(ctx, rocprim::radix_sort_pairs(ctx->d_sort_temp, tb, k1, k2, c1, c2, n, 0, end_bit, ctx->stream));
     *keys_out = k2;
     *counts_out = c2;
     return GW_OK;
@@ -803,7 +800,7 @@ static int sorted_pairs(gw_ctx* ctx, SortedPairs& sp, bool count_only) {
     {
         unsigned long long* nk = nullptr;
         uint32_t* nc = nullptr;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, nk, nk, nc, nc, slots, 0, end_bit,
+        GW_HIP(ctx, rocprim::radix_sort_pairs(nullptr, temp_bytes, nk, nk, nc, nc, slots, 0, end_bit,
                                                        ctx->stream));
     }
     sp.mem_bytes = slots * 24 + temp_bytes;
@@ -827,7 +824,7 @@ static int sorted_pairs(gw_ctx* ctx, SortedPairs& sp, bool count_only) {
     if (e == hipSuccess) e = hipMemcpyAsync(&n, ctx->d_counters + kCntCompact, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess && !count_only && n)
-        e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
+        e = rocprim::radix_sort_pairs(temp, temp_bytes, ka, kb, ca, cb, n, 0, end_bit, ctx->stream);
     if (e != hipSuccess) {
         release_pairs(ctx, sp);
         return gw_hip_check(ctx, e, "table read-out");
@@ -913,9 +910,8 @@ extern "C" int gw_source_topk(gw_ctx* ctx, uint32_t k, uint32_t* source, uint32_
     source_of_kernel<<<dim3(gw_grid(n, 256)), dim3(256), 0, ctx->stream>>>(sp.keys, n, vbits, d_src);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
-        e = hipcub::DeviceRunLengthEncode::Encode(nullptr, t_rle, d_src, d_uniq, d_len, ctx->d_counters + kCntCompact,
-                                                  n, ctx->stream);
-    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, d_len, d_seg, n + 1, ctx->stream);
+        e = rocprim::run_length_encode(nullptr, t_rle, d_src, n, d_uniq, d_len, ctx->d_counters + kCntCompact, ctx->stream);
+    if (e == hipSuccess) e = rocprim::exclusive_scan(nullptr, t_scan, d_len, d_seg, 0ull, n + 1, rocprim::plus<unsigned long long>(), ctx->stream);
     temp_bytes = std::max(t_rle, t_scan);
     if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "top-k temp"))) {
         cleanup();
@@ -923,15 +919,14 @@ extern "C" int gw_source_topk(gw_ctx* ctx, uint32_t k, uint32_t* source, uint32_
     }
     size_t tb = temp_bytes;
     if (e == hipSuccess)
-        e = hipcub::DeviceRunLengthEncode::Encode(temp, tb, d_src, d_uniq, d_len, ctx->d_counters + kCntCompact, n,
-                                                  ctx->stream);
+        e = rocprim::run_length_encode(temp, tb, d_src, n, d_uniq, d_len, ctx->d_counters + kCntCompact, ctx->stream);
     unsigned long long nseg = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&nseg, ctx->d_counters + kCntCompact, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     // lengths -> offsets: a zero after the last run makes the scan's last slot n
     if (e == hipSuccess) e = hipMemsetAsync(d_len + nseg, 0, 8, ctx->stream);
     tb = temp_bytes;
-    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(temp, tb, d_len, d_seg, nseg + 1, ctx->stream);
+    if (e == hipSuccess) e = rocprim::exclusive_scan(temp, tb, d_len, d_seg, 0ull, nseg + 1, rocprim::plus<unsigned long long>(), ctx->stream);
     if (e != hipSuccess) {
         cleanup();
         return gw_hip_check(ctx, e, "top-k segments");

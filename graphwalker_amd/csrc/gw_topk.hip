@@ -16,7 +16,10 @@
 // The order is total (count descending, then vertex id ascending); the
 // reference's quickSort leaves ties among equal values in no fixed order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -181,10 +184,10 @@ int compact_ordered(gw_ctx* ctx, TopkBufs& B, uint32_t T, uint32_t ties, uint64_
     chunk_count_kernel<<<dim3(grid), dim3(kChunkThreads), 0, ctx->stream>>>(ctx->d_counts, n, T, ties, cnt, nchunks);
     GW_HIP(ctx, hipGetLastError());
     size_t need = 0;
-    GW_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, nchunks, ctx->stream));
+    GW_HIP(ctx, rocprim::exclusive_scan(nullptr, need, cnt, off, 0ull, nchunks, rocprim::plus<unsigned long long>(), ctx->stream));
     void* temp = nullptr;
     if ((rc = B.get(&temp, need, "top-k scan temp"))) return rc;
-    GW_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(temp, need, cnt, off, nchunks, ctx->stream));
+    GW_HIP(ctx, rocprim::exclusive_scan(temp, need, cnt, off, 0ull, nchunks, rocprim::plus<unsigned long long>(), ctx->stream));
     chunk_write_kernel<<<dim3(grid), dim3(kChunkThreads), 0, ctx->stream>>>(ctx->d_counts, n, T, ties, off, nchunks,
                                                                            limit, out);
     GW_HIP(ctx, hipGetLastError());
@@ -245,10 +248,10 @@ extern "C" int gw_counts_topk(gw_ctx* ctx, uint64_t k, uint32_t* vertex, uint32_
         topk_keys_kernel<<<dim3(gw_grid(g, tb)), dim3(tb), 0, ctx->stream>>>(ctx->d_counts, gv, g, keys);
         GW_HIP(ctx, hipGetLastError());
         size_t need = 0;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, need, keys, keys_out, g, 0, 64, ctx->stream));
+        GW_HIP(ctx, rocprim::radix_sort_keys(nullptr, need, keys, keys_out, g, 0, 64, ctx->stream));
         void* temp = nullptr;
         if ((rc = B.get(&temp, need, "top-k sort temp"))) return rc;
-        GW_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(temp, need, keys, keys_out, g, 0, 64, ctx->stream));
+        GW_HIP(ctx, rocprim::radix_sort_keys(temp, need, keys, keys_out, g, 0, 64, ctx->stream));
     }
     topk_unpack_kernel<<<dim3(gw_grid(k, tb)), dim3(tb), 0, ctx->stream>>>(keys_out, g, T, tv, remaining, ov, oc);
     GW_HIP(ctx, hipGetLastError());

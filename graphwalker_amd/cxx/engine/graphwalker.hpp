@@ -18,7 +18,23 @@
 // per step, so app recorders see the same (source, vertex, hop) visits.
 //
 // Options (get_option_*): execthreads, schedule (persistent|blocked|plain),
-// seed (Philox key), device, outofcore (-1 auto, 0 in HBM, 1 host-resident).
+// seed (Philox key), device, outofcore (-1 auto, 0 in HBM, 1 host-resident),
+// and the recorder the app's updateInfo is fed from:
+//   recorder events  (default) the per-walk event log: every visit replayed
+//                    with its (source, vertex, hop), walk by walk;
+//   recorder source  per-(source, vertex) counts on the device
+//                    (GW_REC_SOURCE_COUNT), replayed as that many
+//                    updateInfo(source, vertex, t, 0) calls -- for apps whose
+//                    updateInfo depends on the source and the vertex only
+//                    (msppr's visit distributions);
+//   recorder vertex  per-vertex counts on the device (GW_REC_VERTEX_COUNT),
+//                    replayed as updateInfo(vertex, vertex, t, 0) calls -- for
+//                    apps that count visits per vertex (rwdomination);
+//   recorder none    no replay (an app whose updateInfo does nothing, such as
+//                    rawrandomwalks).
+// replaythreads (default 1) replays on that many threads, each with its own
+// threadid (< execthreads) and a contiguous range of sources (or vertices),
+// so per-thread recorders see disjoint sources; 1 replays in walk order.
 #ifndef DEF_GRAPHCHWALKER_ENGINE
 #define DEF_GRAPHCHWALKER_ENGINE
 
@@ -113,7 +129,13 @@ public:
         prog.stop_thr = static_cast<uint64_t>(llround(static_cast<double>(dp.p) * 4294967296.0));
         prog.nvertices = dp.kind == GW_JUMP ? (dp.N ? dp.N : nvertices) : 0;
         prog.seed = get_option_long("seed", 1);
-        prog.recorder = GW_REC_EVENTS | GW_REC_WALK_SUMMARY;
+        const std::string recm = get_option_string("recorder", "events");
+        if (recm != "events" && recm != "source" && recm != "vertex" && recm != "none")
+            logstream(LOG_FATAL) << "recorder must be events, source, vertex or none (got " << recm << ")" << std::endl;
+        prog.recorder = recm == "events" ? (GW_REC_EVENTS | GW_REC_WALK_SUMMARY)
+                      : recm == "source" ? (GW_REC_SOURCE_COUNT | GW_REC_NO_FINAL)
+                      : recm == "vertex" ? (GW_REC_VERTEX_COUNT | GW_REC_NO_FINAL)
+                                         : GW_REC_NO_FINAL;
         const std::string sched = get_option_string("schedule", "persistent");
         prog.schedule = sched == "plain" ? GW_SCHED_PLAIN : sched == "blocked" ? GW_SCHED_BLOCKED : GW_SCHED_PERSISTENT;
         if (prog.schedule == GW_SCHED_BLOCKED && !blocks_on_device_) prog.schedule = GW_SCHED_PERSISTENT;
@@ -124,37 +146,83 @@ public:
         check(gw_walks_load(ctx_, walks.data(), walks.size()), "gw_walks_load");
         m.stop_time("2_findSubGraph");
 
-        std::vector<uint64_t> events;
-        std::vector<uint32_t> visits(walks.size());
         gw_stats st;
-        for (;;) {
+        const int rthreads = std::max(1, std::min<int>(get_option_int("replaythreads", 1), exec_threads));
+        if (recm == "events") {
+            std::vector<uint64_t> events;
+            std::vector<uint32_t> visits(walks.size());
+            for (;;) {
+                m.start_time("5_exec_updates");
+                check(gw_run(ctx_, &prog, &st), "gw_run");
+                m.stop_time("5_exec_updates");
+                check(gw_summary_read(ctx_, nullptr, visits.data(), visits.size()), "gw_summary_read");
+                const uint32_t vmax = visits.empty() ? 0 : *std::max_element(visits.begin(), visits.end());
+                if (vmax <= prog.event_cap) break;
+                // an uncapped walk outgrew the log: rerun (the same walks) with a
+                // log that holds the longest, if the device holds it
+                uint64_t hbm = 0;
+                check(gw_device_info(ctx_, nullptr, 0, nullptr, &hbm), "gw_device_info");
+                const uint64_t need = static_cast<uint64_t>(walks.size()) * vmax * 8;
+                if (need > hbm / 2)
+                    logstream(LOG_FATAL) << "the event log for walks of up to " << vmax << " visits needs " << need
+                                         << " bytes, over half the device memory; run with recorder source, vertex "
+                                            "or none" << std::endl;
+                prog.event_cap = vmax;
+            }
+            events.resize(walks.size() * static_cast<size_t>(prog.event_cap));
+            if (!walks.empty())
+                check(gw_events_read(ctx_, events.data(), walks.size(), prog.event_cap), "gw_events_read");
+            // Replay every visit to updateInfo, walk by walk, hop by hop; with
+            // replaythreads > 1 each thread takes a contiguous range of the
+            // walks (sorted by source) cut at source boundaries.
+            m.start_time("z_replay_updateInfo");
+            const std::vector<size_t> cut = split_by_key(walks.size(), rthreads, [&](size_t i) { return walks[i].source; });
+#pragma omp parallel for num_threads(rthreads) schedule(static, 1)
+            for (int t = 0; t < rthreads; ++t) {
+                for (size_t i = cut[t]; i < cut[t + 1]; ++i) {
+                    const gw_walk& w = walks[i];
+                    const uint64_t* ev = events.data() + static_cast<size_t>(w.id) * prog.event_cap;
+                    for (uint32_t k = 0; k < visits[w.id]; ++k)
+                        userprogram.updateInfo(w.source, static_cast<vid_t>(ev[k] & 0xffffffffu), static_cast<tid_t>(t),
+                                               static_cast<hid_t>(ev[k] >> 32));
+                }
+            }
+            m.stop_time("z_replay_updateInfo");
+        } else {
             m.start_time("5_exec_updates");
             check(gw_run(ctx_, &prog, &st), "gw_run");
             m.stop_time("5_exec_updates");
-            check(gw_summary_read(ctx_, nullptr, visits.data(), visits.size()), "gw_summary_read");
-            const uint32_t vmax = visits.empty() ? 0 : *std::max_element(visits.begin(), visits.end());
-            if (vmax <= prog.event_cap) break;
-            prog.event_cap = vmax;  // deterministic: the rerun takes the same walks
+            m.start_time("z_replay_updateInfo");
+            if (recm == "vertex") {
+                std::vector<uint32_t> counts(nvertices);
+                check(gw_counts_read(ctx_, counts.data(), nvertices), "gw_counts_read");
+                const std::vector<size_t> cut = split_by_key(nvertices, rthreads, [](size_t i) { return i; });
+#pragma omp parallel for num_threads(rthreads) schedule(static, 1)
+                for (int t = 0; t < rthreads; ++t)
+                    for (size_t v = cut[t]; v < cut[t + 1]; ++v)
+                        for (uint32_t c = 0; c < counts[v]; ++c)
+                            userprogram.updateInfo(static_cast<vid_t>(v), static_cast<vid_t>(v), static_cast<tid_t>(t), 0);
+            } else if (recm == "source") {
+                uint64_t np = 0;
+                check(gw_source_counts_read(ctx_, nullptr, nullptr, nullptr, 0, &np), "gw_source_counts_read");
+                std::vector<uint32_t> ps(np), pv(np), pc(np);
+                check(gw_source_counts_read(ctx_, ps.data(), pv.data(), pc.data(), np, &np), "gw_source_counts_read");
+                const std::vector<size_t> cut = split_by_key(np, rthreads, [&](size_t i) { return ps[i]; });
+#pragma omp parallel for num_threads(rthreads) schedule(static, 1)
+                for (int t = 0; t < rthreads; ++t)
+                    for (size_t i = cut[t]; i < cut[t + 1]; ++i)
+                        for (uint32_t c = 0; c < pc[i]; ++c)
+                            userprogram.updateInfo(ps[i], pv[i], static_cast<tid_t>(t), 0);
+            }
+            m.stop_time("z_replay_updateInfo");
         }
-        events.resize(walks.size() * static_cast<size_t>(prog.event_cap));
-        if (!walks.empty()) check(gw_events_read(ctx_, events.data(), walks.size(), prog.event_cap), "gw_events_read");
         m.set("walk_steps", static_cast<size_t>(st.steps));
+        m.set("walk_visits_device", static_cast<size_t>(st.visits));
         m.set("device_ms", st.total_ms);
         if (host_resident_) {
             m.set("blocks_loaded", static_cast<size_t>(st.blocks_loaded));
             m.set("h2d_bytes", static_cast<size_t>(st.h2d_bytes));
         }
-
-        // Replay every visit to updateInfo, walk by walk, hop by hop.
-        m.start_time("z_replay_updateInfo");
-        for (size_t i = 0; i < walks.size(); ++i) {
-            const gw_walk& w = walks[i];
-            const uint64_t* ev = events.data() + static_cast<size_t>(w.id) * prog.event_cap;
-            for (uint32_t k = 0; k < visits[w.id]; ++k)
-                userprogram.updateInfo(w.source, static_cast<vid_t>(ev[k] & 0xffffffffu), 0,
-                                       static_cast<hid_t>(ev[k] >> 32));
-        }
-        m.stop_time("z_replay_updateInfo");
 
         walk_manager->walksum = 0;
         for (bid_t b = 0; b < nblocks; ++b) {
@@ -166,6 +234,21 @@ public:
 
 private:
     gw_ctx* ctx_;
+
+    // Cut [0, n) into `parts` contiguous ranges of about equal size whose
+    // boundaries fall where key(i) changes (key non-decreasing in i), so no
+    // key spans two ranges.  Returns parts + 1 boundaries.
+    template <class Key>
+    static std::vector<size_t> split_by_key(size_t n, int parts, Key key) {
+        std::vector<size_t> cut(parts + 1, n);
+        cut[0] = 0;
+        for (int t = 1; t < parts; ++t) {
+            size_t i = std::max(cut[t - 1], n * t / parts);
+            while (i > cut[t - 1] && i < n && key(i) == key(i - 1)) ++i;
+            cut[t] = i;
+        }
+        return cut;
+    }
     bool blocks_on_device_ = false;
     bool host_resident_ = false;
 

@@ -74,7 +74,9 @@ public:
         logstream(LOG_INFO) << "getTop " << ntop << " from size = " << size << std::endl;
         const std::vector<IdCount> t = top(ntop);
         logstream(LOG_INFO) << "Top " << ntop << " visitfrequencies - " << std::endl;
-        for (size_t i = 0; i < t.size(); ++i) logstream(LOG_INFO) << i << "-\t" << t[i].id << ":\t " << t[i].count << std::endl;
+        // listed as the reference lists them, fewest visits first
+        for (size_t i = 0; i < t.size(); ++i)
+            logstream(LOG_INFO) << i << "-\t" << t[t.size() - 1 - i].id << ":\t " << t[t.size() - 1 - i].count << std::endl;
     }
 };
 

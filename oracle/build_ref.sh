@@ -30,18 +30,3 @@ if [ ! -x "$OUT/msppr_sources" ] || [ "$HERE/ref_msppr_sources.cpp" -nt "$OUT/ms
     g++ -O3 -fopenmp -w -I "$REF/src" "$HERE/ref_msppr_sources.cpp" -o "$OUT/msppr_sources" -lz
 fi
 echo "build_ref: built $(ls "$OUT" | tr '\n' ' ')"
-
-# The same reference apps compiled, unchanged, against this repository's
-# GraphWalker API headers (graphwalker_amd/cxx) and linked to libgwhip.so:
-# the drop-in check.  Outputs to oracle/_ref/compat/.
-ROOT="$(cd "$(dirname "$0")/.." && pwd)"
-if [ -f "$ROOT/graphwalker_amd/lib/libgwhip.so" ]; then
-    mkdir -p "$OUT/compat"
-    # (msppr is compile-checked only: its main() runs `killall top`)
-    for app in rawrandomwalks simrank graphlet rwdomination kk_ppr; do
-        g++ -std=c++17 -O2 -fopenmp -w -I "$ROOT/graphwalker_amd/cxx" -I "$ROOT/include" "$REF/apps/$app.cpp" \
-            -L "$ROOT/graphwalker_amd/lib" -lgwhip -Wl,-rpath,'$ORIGIN/../../../graphwalker_amd/lib' \
-            -o "$OUT/compat/$app"
-    done
-    echo "build_ref: compat apps $(ls "$OUT/compat" | tr '\n' ' ')"
-fi

@@ -1,6 +1,6 @@
 """GPU runs through the C++ front end: an app on the GraphWalker API and
-reference apps compiled unchanged against it (oracle/_ref/compat/, built by
-oracle/build_ref.sh) -- results checked against the oracle."""
+reference apps compiled unchanged against it (graphwalker_amd/bin/compat/, built
+by graphwalker_amd/build.py) -- results checked against the oracle."""
 import os
 import re
 import subprocess
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "graphwalker_amd", "bin")
-COMPAT = os.path.join(ROOT, "oracle", "_ref", "compat")
+COMPAT = os.path.join(ROOT, "graphwalker_amd", "bin", "compat")
 KINDS = {"jump": O.JUMP, "stop": O.STOP, "prob": O.PROB, "joint": O.RESTART_JOINT, "restart": O.RESTART}
 
 
@@ -31,10 +31,10 @@ def graph(tmp_path_factory):
     return d, base, off, col
 
 
-def run(cmd, cwd, timeout=120):
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd, timeout=timeout)
+def run(cmd, cwd, timeout=120, env=None, both=False):
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd, timeout=timeout, env=env)
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-3000:])
-    return r.stdout
+    return r.stdout + r.stderr if both else r.stdout
 
 
 @pytest.mark.parametrize("prog", sorted(KINDS))
@@ -165,3 +165,66 @@ def test_reference_kk_ppr_unchanged(graph):
     walks = O.make_walks(first + k, sources_=k, ids=np.arange(ns * wps))
     ref = O.replay(O.program(O.PROB, 30, 0.15, n, seed=1), off, col, walks)
     assert int(steps.group(1)) == ref["steps"]
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "msppr"), os.X_OK), reason="compat apps not built")
+@pytest.mark.parametrize("recorder,threads", [("events", 1), ("events", 4), ("source", 4)])
+def test_reference_msppr_unchanged(graph, tmp_path, recorder, threads):
+    """apps/msppr.cpp unchanged (its `killall top` answered by a no-op on
+    PATH): the top-20 of source 0's visit distribution it logs equals the
+    oracle's counts for the same walks (index 0's walks: ids 0..W-1, seed 1),
+    exact because the source reaches fewer vertices than the recorder's
+    1000-entry capacity; with every recorder path of the front end."""
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    wps, L = 600, 10
+    prog = O.program(O.STOP, L, 0.15, n, seed=1)
+    first, ref = None, None
+    for v in range(100, 4000, 37):
+        walks = O.make_walks(np.full(wps, v, np.uint32), sources_=np.zeros(wps, np.uint32), ids=np.arange(wps))
+        c = O.replay(prog, off, col, walks)["counts"]
+        if 40 < np.count_nonzero(c) < 900:
+            first, ref = v, c
+            break
+    assert first is not None
+    bindir = tmp_path / "bin"
+    bindir.mkdir()
+    (bindir / "killall").write_text("#!/bin/sh\nexit 0\n")
+    (bindir / "killall").chmod(0o755)
+    env = dict(os.environ, PATH=f"{bindir}:{os.environ.get('PATH', '')}")
+    out = run([os.path.join(COMPAT, "msppr"), "file", base, "firstsource", str(first), "numsources", "2",
+               "walkspersource", str(wps), "maxwalklength", str(L), "blocksize_kb", "1024", "recorder", recorder,
+               "replaythreads", str(threads), "execthreads", "4"], d, env=env, both=True)
+    rows = re.findall(r"(\d+)-\t(\d+):\t (\d+)", out)
+    assert len(rows) == 20, out[-3000:]
+    assert [int(r[0]) for r in rows] == list(range(20))
+    counts = [int(r[2]) for r in rows]
+    assert counts == sorted(counts)  # the reference's order: fewest visits first
+    for _, vtx, cnt in rows:
+        assert ref[int(vtx)] == int(cnt), (vtx, cnt, ref[int(vtx)])
+    assert sorted(counts) == sorted(np.sort(ref)[-20:].tolist())
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "rwdomination"), os.X_OK), reason="compat apps not built")
+def test_reference_rwdomination_vertex_recorder(graph):
+    """rwdomination unchanged with the device per-vertex recorder: the same
+    step total as the event-log path."""
+    d, base, off, col = graph
+    n = off.shape[0] - 1
+    outs = []
+    for rec in ("events", "vertex"):
+        stdout = run([os.path.join(COMPAT, "rwdomination"), "file", base, "N", str(n), "R", "2", "L", "6",
+                      "blocksize_kb", "16", "recorder", rec, "replaythreads", "4"], d)
+        m = re.search(r"\[walk_steps\] (\d+)", stdout)
+        assert m, stdout[-2000:]
+        outs.append(int(m.group(1)))
+    assert outs[0] == outs[1] == n * 2 * 6
+
+
+@pytest.mark.skipif(not os.access(os.path.join(COMPAT, "rawrandomwalks"), os.X_OK), reason="compat apps not built")
+def test_reference_rawrandomwalks_no_recorder(graph):
+    d, base, off, col = graph
+    stdout = run([os.path.join(COMPAT, "rawrandomwalks"), "file", base, "N", str(off.shape[0] - 1), "R", "20000",
+                  "L", "10", "blocksize_kb", "16", "recorder", "none"], d)
+    m = re.search(r"\[walk_steps\] (\d+)", stdout)
+    assert m and int(m.group(1)) == 20000 * 10, stdout[-2000:]
