@@ -263,6 +263,60 @@ def cpu_reference_multisource(eng: gw.Engine, wl: dict, off: np.ndarray, col: np
             "port": port}
 
 
+def frontend_measure(eng: gw.Engine, wl: dict, n: int, timeout: float = 300) -> dict:
+    """The reference's rawrandomwalks app, compiled unchanged against this
+    package's GraphWalker API headers (graphwalker_amd/bin/compat/), on the
+    graph resident in `eng` written in the on-disk layout: R = n walks of L
+    steps from rand() % N sources (its fixed 0.15 teleport), one block, with
+    the front end's recorder `none` (the app's updateInfo is empty) and with
+    the event log replayed to updateInfo.  Time from its own 00_runtime (walk
+    gathering, upload, the device run, the replay)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    app = os.path.join(ROOT, "graphwalker_amd", "bin", "compat", "rawrandomwalks")
+    if not os.access(app, os.X_OK):
+        return {}
+    off, col = eng.graph_read()
+    tmp = tempfile.mkdtemp(prefix="gw_fe_")
+    out = {}
+    try:
+        base = os.path.join(tmp, "graph")
+        kb = -(-int(off[-1]) * 4 // 1024) + 1  # one block
+        gw.write_graph_files(base, off, col, kb)
+        del off, col
+        os.makedirs(os.path.join(tmp, "conf"), exist_ok=True)
+        with open(os.path.join(tmp, "conf", "graphwalker.cnf"), "w") as f:
+            f.write("# options are given on the command line\n")
+        for rec in ("none", "events"):
+            t0 = time.perf_counter()
+            r = subprocess.run([app, "file", base, "N", str(n), "R", str(n), "L", str(wl["L"]), "blocksize_kb", str(kb),
+                                "recorder", rec], cwd=tmp, capture_output=True, text=True, timeout=timeout)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                out[rec] = {"error": f"exit {r.returncode}: {r.stderr[-500:]}"}
+                continue
+            def timer(key):
+                m = re.search(r"\[" + key + r"\] time=([0-9.eE+-]+)s", r.stdout)
+                return float(m.group(1)) if m else None
+            steps = int(re.search(r"\[walk_steps\] (\d+)", r.stdout).group(1))
+            rt = timer("00_runtime")
+            out[rec] = {"walk_steps": steps, "runtime_s": rt, "walk_steps_per_s": steps / rt if rt else None,
+                        "gather_s": timer("3_getCurrentWalks"), "upload_s": timer("2_findSubGraph"),
+                        "device_run_s": timer("5_exec_updates"), "replay_s": timer("z_replay_updateInfo"),
+                        "process_wall_s": wall}
+            print(f"[bench] front end (unchanged rawrandomwalks, recorder {rec}): {steps / rt / 1e9:.3f} G steps/s "
+                  f"({rt:.2f} s of its 00_runtime)", file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out["desc"] = (f"the reference's apps/rawrandomwalks.cpp compiled unchanged against graphwalker_amd/cxx, on the "
+                   f"{wl['desc'].split(',')[0]} graph in the on-disk layout (one block): {n} walks x {wl['L']} steps; "
+                   f"00_runtime = gathering the packed walk records, their upload, the device run and the replay")
+    return out
+
+
 def measure(eng: gw.Engine, name: str, K: int, W: int, sched: str, layout: str, compare: bool,
             shape=None) -> dict:
     wl = WORKLOADS[name]
@@ -624,6 +678,7 @@ def main() -> int:
     ap.add_argument("--pool-window", type=int, default=1 << 21,
                     help="walks per window of the host-walk-pool figure on the raw-walk workload (0 = skip)")
     ap.add_argument("--simrank-scale", type=int, default=28, help="SimRank line on R-MAT of this scale (0 = skip)")
+    ap.add_argument("--frontend", type=int, default=1, help="time the unchanged rawrandomwalks app (1) or not (0)")
     ap.add_argument("--ooc-scale", type=int, default=31,
                     help="serving over a host-resident R-MAT of this scale, edge factor 24 (or the largest the host "
                          "memory holds; 0 = skip)")
@@ -665,6 +720,9 @@ def main() -> int:
             eng.counts_topk(k)
             post[f"counts_topk_{k}_ms"] = (time.perf_counter() - t) * 1e3
         print(f"[bench] counts top-k: {post}", file=sys.stderr, flush=True)
+        # the same workload through the C++ front end: the reference's app, unchanged
+        if not wl2["sources"] and args.frontend:
+            workloads[args.secondary]["frontend"] = frontend_measure(eng, wl2, s["n"])
         # the same walks as a host walk pool (out-of-HBM walk pools), stepped in windows
         if args.pool_window and not wl2["sources"]:
             for keep in (False, True):

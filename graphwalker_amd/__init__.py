@@ -150,6 +150,56 @@ def rmat_thresholds(a: float, b: float, c: float) -> tuple[int, int, int]:
 _lib = None
 
 
+def block_cut(off: np.ndarray, blocksize_kb: int) -> np.ndarray:
+    """Block table of a CSR by the reference's cut rule (compute_block,
+    preprocess/conversions.hpp:300-364): a block closes before the vertex
+    whose row would take it past blocksize_kb KB of columns, and a row larger
+    than a block gets a block of its own."""
+    off = np.asarray(off, dtype=np.uint64)
+    n = off.shape[0] - 1
+    mneb = blocksize_kb * 1024 // 4
+    blocks = [0]
+    start, v = 0, 1
+    while v <= n:
+        # first vertex v' >= v whose row end passes start + mneb
+        w = int(np.searchsorted(off, np.uint64(start + mneb), side="right"))
+        if w < v:
+            w = v
+        if w > n:
+            break
+        if int(off[w]) - int(off[w - 1]) > mneb:
+            blocks += [w - 1, w]
+            start = int(off[w])
+        else:
+            blocks.append(w - 1)
+            start = int(off[w - 1])
+        v = w + 1
+    blocks.append(n)
+    return np.asarray(blocks, dtype=np.uint32)
+
+
+def write_graph_files(base: str, off: np.ndarray, col: np.ndarray, blocksize_kb: int) -> int:
+    """A CSR in the preprocessed on-disk layout the GraphWalker engine reads
+    (filename.hpp:16-51, conversions.hpp:181-298): <base>_GraphWalker/
+    graphinfo/file_0.beg_pos (u64 absolute offsets) and file_0.csr (u32
+    columns), N.nvertices, filesize_1024GB.filerange and the block table
+    blocksize_<KB>KB.blockrange.  Returns the number of blocks."""
+    d = base + "_GraphWalker"
+    os.makedirs(os.path.join(d, "graphinfo"), exist_ok=True)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    np.ascontiguousarray(col, dtype=np.uint32).tofile(os.path.join(d, "graphinfo", "file_0.csr"))
+    off.tofile(os.path.join(d, "graphinfo", "file_0.beg_pos"))
+    n = off.shape[0] - 1
+    with open(os.path.join(d, "filesize_1024GB.filerange"), "w") as f:
+        f.write(f"0\n{n}\n")
+    with open(os.path.join(d, "N.nvertices"), "w") as f:
+        f.write(f"{n}\n")
+    blocks = block_cut(off, blocksize_kb)
+    with open(os.path.join(d, f"blocksize_{blocksize_kb}KB.blockrange"), "w") as f:
+        f.write("".join(f"{int(b)}\n" for b in blocks))
+    return blocks.shape[0] - 1
+
+
 def sources(seed: int, n: int, count: int) -> np.ndarray:
     """Uniform source vertices reproducible from (seed, index) (gw_sources)."""
     out = np.empty(count, dtype=np.uint32)

@@ -47,6 +47,7 @@
 
 #include <algorithm>
 #include <fstream>
+#include <parallel/algorithm>
 #include <string>
 #include <vector>
 
@@ -350,17 +351,24 @@ private:
     // thread placed which walk.
     std::vector<gw_walk> gather_walks() {
         struct rec { WalkDataType w; bid_t p; };
+        size_t total = 0;
+        for (bid_t p = 0; p < nblocks; ++p)
+            for (tid_t t = 0; t < exec_threads; ++t) total += walk_manager->pwalks[t][p].size_w;
         std::vector<rec> all;
+        all.reserve(total);
         for (bid_t p = 0; p < nblocks; ++p)
             for (tid_t t = 0; t < exec_threads; ++t) {
                 WalkBuffer& b = walk_manager->pwalks[t][p];
                 for (wid_t k = 0; k < b.size_w; ++k) all.push_back({b[k], p});
                 b.size_w = 0;
             }
-        std::stable_sort(all.begin(), all.end(), [](const rec& a, const rec& b) {
+        // (record, block) pairs are unique up to identical walks, so any sort
+        // by that pair gives the same ids: sorted in parallel on the host's cores
+        __gnu_parallel::sort(all.begin(), all.end(), [](const rec& a, const rec& b) {
             return a.w != b.w ? a.w < b.w : a.p < b.p;
         });
         std::vector<gw_walk> out(all.size());
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < all.size(); ++i) {
             const WalkDataType w = all[i].w;
             out[i].id = static_cast<uint32_t>(i);

@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
         L.gwo_trace.restype = u64
         L.gwo_block_round.argtypes = [P, P, P, u32, P, u32, u32, P, u64, P, P, P]
         L.gwo_bucket_stable.argtypes = [P, u64, P, u32, P, P]
+        L.gwo_fp64_stop_visits.argtypes = [u32, P, P, P, u32, P, C.c_double, u32, C.c_double, P]
         L.gwo_num_threads.restype = C.c_int
         L.gwo_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -109,6 +110,25 @@ def rmat_edges(scale: int, m: int, seed: int, a=0.57, b=0.19, c=0.19, e_begin: i
     dst = np.empty(m, dtype=np.uint32)
     lib().gwo_rmat_edges(scale, e_begin, e_begin + m, seed, ta, tab, tabc, _p(src), _p(dst))
     return src, dst
+
+
+def fp64_stop_visits(off: np.ndarray, toff: np.ndarray, tcol: np.ndarray, sources, walks: float, L: int,
+                     p: float) -> np.ndarray:
+    """FP64 expected visits of `walks` STOP walks (cap L) from each source:
+    row k for sources[k] (at most 64), pulled over the transposed CSR (toff,
+    tcol) with the forward offsets' out-degrees; fp64.expected_visits_sources
+    for graphs too large for scipy."""
+    src = np.ascontiguousarray(sources, dtype=np.uint32)
+    K = src.shape[0]
+    if not 1 <= K <= 64:
+        raise ValueError("1..64 sources")
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    toff = np.ascontiguousarray(toff, dtype=np.uint64)
+    tcol = np.ascontiguousarray(tcol, dtype=np.uint32)
+    n = off.shape[0] - 1
+    out = np.empty((n, K), dtype=np.float64)
+    lib().gwo_fp64_stop_visits(n, _p(toff), _p(tcol), _p(off), K, _p(src), float(walks), L, float(p), _p(out))
+    return out.T.copy()
 
 
 def csr_build(n: int, src: np.ndarray, dst: np.ndarray):

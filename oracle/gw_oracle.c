@@ -348,6 +348,59 @@ void gwo_bucket_stable(const gwo_walk *in, uint64_t n, const uint32_t *block_fir
     free(bk);
 }
 
+/* ------------------------------------------------ FP64 stop-walk visits */
+/* Expected visits of `walks` STOP walks (follow a uniform CSR slot with
+ * probability 1 - p where the vertex has edges, else die; cap L hops,
+ * randomwalkwithstop.hpp:20-52) from each of K sources: out[v * K + k] =
+ * sum_{h < L} x_h[v] for source k, x_0 = walks at the source and
+ * x_{h+1}[d] = (1 - p) sum over the in-edges (u -> d) of x_h[u] / deg(u),
+ * computed by pulling over the transposed CSR (toff, tcol: the in-neighbours
+ * of each vertex, one entry per edge) with deg the forward out-degrees.
+ * The same chain as fp64.expected_visits for a graph too large for a sparse
+ * matrix in Python; K columns share each edge's read. */
+void gwo_fp64_stop_visits(uint32_t n, const uint64_t *toff, const uint32_t *tcol, const uint64_t *off,
+                          uint32_t K, const uint32_t *sources, double walks, uint32_t L, double p, double *out) {
+    const size_t nk = (size_t)n * K;
+    double *x = (double *)calloc(nk ? nk : 1, sizeof(double));
+    double *y = (double *)calloc(nk ? nk : 1, sizeof(double));
+    double *inv = (double *)malloc(sizeof(double) * (n ? n : 1));
+    #pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < (int64_t)n; ++v) {
+        const uint64_t d = off[v + 1] - off[v];
+        inv[v] = d ? (1.0 - p) / (double)d : 0.0;
+    }
+    memset(out, 0, sizeof(double) * nk);
+    for (uint32_t k = 0; k < K; ++k) x[(size_t)sources[k] * K + k] += walks;
+    for (uint32_t h = 0; h < L; ++h) {
+        #pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t v = 0; v < (int64_t)n; ++v) {
+            double *o = out + (size_t)v * K;
+            const double *xv = x + (size_t)v * K;
+            for (uint32_t k = 0; k < K; ++k) o[k] += xv[k];
+        }
+        if (h + 1 == L) break;
+        #pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t d = 0; d < (int64_t)n; ++d) {
+            double acc[64];
+            for (uint32_t k = 0; k < K; ++k) acc[k] = 0.0;
+            for (uint64_t e = toff[d]; e < toff[d + 1]; ++e) {
+                const uint32_t u = tcol[e];
+                const double w = inv[u];
+                const double *xu = x + (size_t)u * K;
+                for (uint32_t k = 0; k < K; ++k) acc[k] += xu[k] * w;
+            }
+            double *yd = y + (size_t)d * K;
+            for (uint32_t k = 0; k < K; ++k) yd[k] = acc[k];
+        }
+        double *t = x;
+        x = y;
+        y = t;
+    }
+    free(x);
+    free(y);
+    free(inv);
+}
+
 int gwo_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -100,3 +100,16 @@ def test_block_table_rule():
         assert y >= x
         if y - x > 1:
             assert int(o[y]) - int(o[x]) <= 16 * 256 + int(o[y] - o[y - 1])
+
+
+def test_block_cut_matches_the_reference_rule():
+    # the product's vectorised cut against the rule written out vertex by vertex
+    import graphwalker_amd as gw
+    from gwfiles import block_table
+
+    rng = np.random.default_rng(3)
+    for n, kb in ((1, 1), (50, 1), (3000, 1), (3000, 4), (20000, 16)):
+        deg = rng.geometric(0.05, n) - 1
+        deg[rng.integers(0, n, max(1, n // 500))] += 3000  # rows larger than a block
+        off = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+        assert gw.block_cut(off, kb).tolist() == block_table(off, kb), (n, kb)

@@ -92,6 +92,33 @@ def test_ppr4096_rmat26_fullsize(engine):
         nz = np.nonzero(cnt)[0]
         lo, hi = bounds[k], bounds[k + 1]
         assert np.array_equal(v[lo:hi], nz.astype(np.uint32)) and np.array_equal(c[lo:hi], cnt[nz]), k
+    # FP64 on the configuration's own graph: the expected visit vectors of the
+    # first 8 sources with out-edges (20-hop stop chain, pulled over the
+    # device-transposed CSR).  Per source: the normalised counts within the
+    # Monte-Carlo L1 bound of its 2000 walks, the visit total, and each of the
+    # 50 most-visited vertices within 5 sigma (a vertex's count sums 2000
+    # per-walk counts in [0, L], so its variance is at most L x expected)
+    from oracle import fp64
+    engine.graph_transpose()
+    toff, tcol = engine.graph_read()
+    deg = off[src.astype(np.int64) + 1].astype(np.int64) - off[src.astype(np.int64)].astype(np.int64)
+    ks = [k for k in range(S) if deg[k] > 0][:8]
+    assert len(ks) == 8
+    E = O.fp64_stop_visits(off, toff, tcol, src[ks], float(W), L, prog.stop_thr / 2.0 ** 32)
+    del toff, tcol
+    for i, k in enumerate(ks):
+        cnt = np.zeros(n, dtype=np.float64)
+        lo, hi = bounds[k], bounds[k + 1]
+        cnt[v[lo:hi]] = c[lo:hi]
+        exp = E[i]
+        assert abs(cnt.sum() - exp.sum()) <= 5 * np.sqrt(L * exp.sum()), k
+        l1, bound = fp64.l1_normalised(cnt, exp), fp64.mc_l1_bound(exp, W, exp.sum() / W, L)
+        top = np.argsort(exp)[-50:]
+        z = np.abs(cnt[top] - exp[top]) / np.sqrt(L * exp[top])
+        print(f"[ppr fp64] source {k} (vertex {src[k]}, out-degree {deg[k]}): L1 {l1:.4f} (MC bound {bound:.4f}), "
+              f"top-50 vertices max |z| {z.max():.2f}, own vertex {cnt[src[k]]:.0f} vs {exp[src[k]]:.1f}")
+        assert l1 < bound, (k, l1, bound)
+        assert np.all(z <= 5.0), (k, z.max())
 
 
 def _transposed_np(off, col):
@@ -145,3 +172,46 @@ def test_simrank_rmat28_fullsize(engine):
         exp = np.zeros(L, dtype=np.uint64)
         np.add.at(exp, eq.argmax(axis=2)[hit], 1)
         assert np.array_equal(got[k].astype(np.uint64), exp), k
+    # FP64 SimRank on a sampled subgraph of this graph: the vertices within
+    # two in-link hops of 4 uniform pairs (at most 10 in-neighbours taken per
+    # vertex) and every edge among them; the device's first-meeting rates of
+    # 8 copies x 1024 walks per endpoint against the exact pair chain there
+    from oracle import fp64
+    cand = gw.sources(44, n, 4096)
+    ends = cand[(toff[cand.astype(np.int64) + 1] - toff[cand.astype(np.int64)]) >= 3][:8]  # endpoints with in-links
+    assert ends.shape[0] == 8
+    ball, frontier = set(int(x) for x in ends), [int(x) for x in ends]
+    for _ in range(2):
+        nxt = []
+        for u in frontier:
+            for w in tcol[int(toff[u]):min(int(toff[u + 1]), int(toff[u]) + 10)].tolist():
+                if w not in ball:
+                    ball.add(w)
+                    nxt.append(w)
+        frontier = nxt
+    S = np.array(sorted(ball), dtype=np.uint32)
+    sub_src, sub_dst = [], []
+    for i, u in enumerate(S.tolist()):
+        r = tcol[int(toff[u]):int(toff[u + 1])]
+        r = r[np.isin(r, S)]
+        sub_dst.append(np.searchsorted(S, r).astype(np.uint32))
+        sub_src.append(np.full(r.shape[0], i, np.uint32))
+    del tcol
+    local = {int(u): i for i, u in enumerate(S.tolist())}
+    soff, scol = O.csr_build(S.shape[0], np.concatenate(sub_src), np.concatenate(sub_dst))
+    engine.graph_set(soff, scol)
+    R2, copies = 1024, 8
+    la = np.repeat(np.array([local[int(x)] for x in ends[0::2]], np.uint32), copies)
+    lb = np.repeat(np.array([local[int(x)] for x in ends[1::2]], np.uint32), copies)
+    engine.walks_init_pairs(la, lb, R2, L)
+    sprog = gw.program(gw.RESTART_JOINT, L, 0.15, S.shape[0], seed=41, recorder=gw.REC_EVENTS, event_cap=L - 1)
+    engine.run(sprog)
+    rates = engine.meeting_counts(la.shape[0], R2, L).astype(np.float64) / (R2 * R2)
+    pr = sprog.stop_thr / 2.0 ** 32
+    for j in range(4):
+        f = fp64.first_meeting(soff, scol, int(la[j * copies]), int(lb[j * copies]), pr, L - 1)
+        est = rates[j * copies:(j + 1) * copies, :L - 1].mean(axis=0)
+        tol = 5.0 * np.sqrt((f / (R2 * R2) + 2.0 * f / R2) / copies) + 1e-9
+        print(f"[simrank fp64] subgraph of {S.shape[0]} vertices, pair {j}: SimRank estimate "
+              f"{float(est @ 0.8 ** np.arange(L - 1)):.5f}, exact {float(f @ 0.8 ** np.arange(L - 1)):.5f}")
+        assert np.all(np.abs(est - f) <= tol), (j, est, f, tol)

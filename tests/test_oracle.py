@@ -579,3 +579,18 @@ def test_reference_engine_graphlet_within_mc_bound(tmp_path):
     cnt = O.run_reference_graphlet(base, n, R, L, 8, timeout=120)
     f = float(fp64.return_at(off, col, L, 0.15).mean())
     assert abs(cnt / R - f) <= 5 * np.sqrt(f * (1 - f) / R), (cnt / R, f)
+
+
+def test_fp64_stop_visits_pull_equals_sparse_chain():
+    # the C pull over the transposed CSR against fp64.expected_visits_sources
+    # (scipy), on a graph with dead ends, duplicate edges and self-loops
+    off, col = O.rmat_graph(10, 1 << 13, seed=9)
+    n = off.shape[0] - 1
+    deg = np.diff(off.astype(np.int64))
+    src_of = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    toff, tcol = O.csr_build(n, col, src_of)  # in-neighbour lists
+    sources = np.array([0, 1, 5, 77, int(np.argmax(deg == 0)), 1023], dtype=np.uint32)
+    got = O.fp64_stop_visits(off, toff, tcol, sources, 2000.0, 20, 0.15)
+    exp = fp64.expected_visits_sources(O.STOP, off, col, sources, 2000.0, 20, 0.15)
+    assert np.allclose(got, exp, rtol=1e-12, atol=1e-9)
+    assert got[4, sources[4]] == 2000.0 and got[4].sum() == 2000.0  # a dead end: one visit per walk
