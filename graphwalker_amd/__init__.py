@@ -34,6 +34,7 @@ LAYOUT_NAMES = {"auto": LAYOUT_AUTO, "csr": LAYOUT_CSR, "ext": LAYOUT_EXT}
 # per-source count aggregation
 SRC_SORT, SRC_HASH, SRC_GLOBAL, SRC_PARTITIONED = 0, 1, 2, 3
 POOL_KEEP_FINAL = 1
+EDGES_DROP_SELF_LOOPS, EDGES_SORT_ROWS = 1, 2
 BLOCKS_MAX_WALKS, BLOCKS_MIN_STEP, BLOCKS_MAX_WEIGHT = 0, 1, 2
 
 WALK_DTYPE = np.dtype([("id", "<u4"), ("source", "<u4"), ("vertex", "<u4"), ("hop", "<u4")])
@@ -229,6 +230,7 @@ def lib() -> C.CDLL:
         "gw_host_alloc": ([P, u64, P], i),
         "gw_host_free": ([P, P], i),
         "gw_graph_rmat": ([P, u32, u64, u64, u32, u32, u32], i),
+        "gw_graph_set_edges": ([P, P, P, u64, u32, u32], i),
         "gw_graph_rmat_host": ([P, u32, u64, u64, u32, u32, u32, u64, u64, P], i),
         "gw_graph_blocks": ([P, P, u32], i),
         "gw_graph_info": ([P, P, P, P], i),
@@ -332,6 +334,20 @@ class Engine:
         n = off.shape[0] - 1
         self._check(self._L.gw_graph_set(self._ctx, _ptr(off), _ptr(col), n, col.shape[0]), "gw_graph_set")
         self.n, self.m = n, col.shape[0]
+
+    def graph_set_edges(self, src: np.ndarray, dst: np.ndarray, n: int = 0, drop_self_loops: bool = False,
+                        sort_rows: bool = False) -> None:
+        """CSR built on the device from an edge list (rows keep the list's
+        order unless sort_rows); n = 0: the largest vertex id + 1."""
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        if src.shape != dst.shape:
+            raise GwError("graph_set_edges: src and dst differ in length")
+        flags = (EDGES_DROP_SELF_LOOPS if drop_self_loops else 0) | (EDGES_SORT_ROWS if sort_rows else 0)
+        self._check(self._L.gw_graph_set_edges(self._ctx, _ptr(src), _ptr(dst), src.shape[0], n, flags),
+                    "gw_graph_set_edges")
+        info = self.graph_info()
+        self.n, self.m = info["n"], info["m"]
 
     def host_empty(self, shape, dtype) -> np.ndarray:
         """A numpy array in pinned host memory (gw_host_alloc), freed with the

@@ -285,6 +285,19 @@ int gw_graph_set(gw_ctx* ctx, const uint64_t* off, const uint32_t* col, uint32_t
     return single_block(ctx);
 }
 
+int gw_graph_set_edges(gw_ctx* ctx, const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t n,
+                       uint32_t flags) {
+    if (!ctx || (m && (!src || !dst)) || (flags & ~(GW_EDGES_DROP_SELF_LOOPS | GW_EDGES_SORT_ROWS)) ||
+        n == 0xFFFFFFFFu)
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_set_edges: bad arguments");
+    GW_HIP(ctx, hipSetDevice(ctx->device));
+    gw_graph_free(ctx);
+    int rc = gw_edges_build(ctx, src, dst, m, n, flags);
+    if (rc) return rc;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_counts, ctx->n * 4ull, "vertex counts"))) return rc;
+    return single_block(ctx);
+}
+
 int gw_graph_rmat(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
                   uint32_t tabc) {
     if (!ctx || scale == 0 || scale > 31 || !(ta <= tab && tab <= tabc && tabc <= 65536))

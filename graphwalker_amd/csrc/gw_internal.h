@@ -220,6 +220,9 @@ int gw_rmat_build(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32
 int gw_rmat_build_host(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, uint32_t ta, uint32_t tab,
                        uint32_t tabc, uint64_t cache_bytes, uint64_t work_bytes, double* build_ms);
 
+// gw_rmat.hip: CSR on the device from a host edge list (GW_EDGES_* flags)
+int gw_edges_build(gw_ctx* ctx, const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t n, uint32_t flags);
+
 // gw_rmat.hip: reverse every edge of the device-resident CSR in place
 int gw_transpose_build(gw_ctx* ctx);
 

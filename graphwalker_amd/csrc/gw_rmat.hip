@@ -141,6 +141,48 @@ __global__ void csr_fragment_kernel(const uint64_t* __restrict__ keys, uint64_t 
     }
 }
 
+// Edge list -> sort keys.  The source is the key (stably sorted, so a row
+// keeps the list's order) or, sorting rows too, (src << 32) | dst; a dropped
+// self-loop gets the key that sorts past every edge and is counted.
+__global__ void edge_keys_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t m,
+                                 bool drop_loops, bool sort_rows, uint64_t* __restrict__ keys,
+                                 uint32_t* __restrict__ vals, unsigned long long* __restrict__ dropped,
+                                 unsigned long long* __restrict__ vmax) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    uint32_t mx = 0, nd = 0;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < m; e += stride) {
+        const uint32_t s = src[e], d = dst[e];
+        const bool loop = drop_loops && s == d;
+        keys[e] = loop ? ~0ull : sort_rows ? (static_cast<uint64_t>(s) << 32) | d : s;
+        vals[e] = d;
+        nd += loop ? 1u : 0u;
+        if (!loop) mx = s > mx ? (d > s ? d : s) : (d > mx ? d : mx);
+    }
+    const uint32_t tot = gw::wave_sum(nd);
+    uint32_t m2 = mx;
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(m2, o, 64);
+        m2 = y > m2 ? y : m2;
+    }
+    if (__lane_id() == 0) {
+        if (tot) atomicAdd(dropped, static_cast<unsigned long long>(tot));
+        atomicMax(vmax, static_cast<unsigned long long>(m2));
+    }
+}
+
+// CSR from edges sorted by source: off[v] = first edge whose source >= v,
+// written by the edge that starts each run (sources are the keys' high or
+// only bits).
+__global__ void csr_from_sources_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t shift, uint32_t n,
+                                        uint64_t* __restrict__ off) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e <= m; e += stride) {
+        const uint64_t s = e < m ? (keys[e] >> shift) : static_cast<uint64_t>(n);
+        const int64_t prev = e > 0 ? static_cast<int64_t>(keys[e - 1] >> shift) : -1;
+        for (int64_t v = prev + 1; v <= static_cast<int64_t>(s); ++v) off[v] = e;
+    }
+}
+
 // Keys hold (src << bits) | dst, sorted; n vertices.
 __global__ void csr_from_sorted_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t n,
                                        uint64_t* __restrict__ off, uint32_t* __restrict__ col) {
@@ -491,4 +533,103 @@ int gw_rmat_build_host(gw_ctx* ctx, uint32_t scale, uint64_t m, uint64_t seed, u
     ctx->h_off = hoff;
     ctx->h_col = hcol;
     return gw_host_graph_adopt(ctx, n, m, cache_bytes);
+}
+
+// A CSR built on the device from a host edge list (the job of the
+// reference's convert_to_csr, conversions.hpp:181-298): edges grouped by
+// source with a stable radix sort, so a row keeps the list's edge order
+// unless rows are sorted too; self-loops dropped on request (the converter
+// drops them, conversions.hpp:237); n = the largest vertex + 1 when 0 (the
+// converter's vertex count).
+int gw_edges_build(gw_ctx* ctx, const uint32_t* src, const uint32_t* dst, uint64_t m, uint32_t n, uint32_t flags) {
+    const bool drop = (flags & GW_EDGES_DROP_SELF_LOOPS) != 0, sort_rows = (flags & GW_EDGES_SORT_ROWS) != 0;
+    int rc;
+    uint32_t *d_src = nullptr, *d_dst = nullptr, *v_in = nullptr, *v_out = nullptr;
+    uint64_t *k_in = nullptr, *k_out = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    const uint64_t mm = std::max<uint64_t>(1, m);
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);
+        gw_free(ctx, d_src, mm * 4);
+        gw_free(ctx, d_dst, mm * 4);
+        gw_free(ctx, v_in, mm * 4);
+        gw_free(ctx, v_out, mm * 4);
+        gw_free(ctx, k_in, mm * 8);
+        gw_free(ctx, k_out, mm * 8);
+        gw_free(ctx, d_cnt, 16);
+        gw_free(ctx, temp, temp_bytes);
+    };
+    if ((rc = gw_alloc(ctx, (void**)&d_src, mm * 4, "edge sources")) ||
+        (rc = gw_alloc(ctx, (void**)&d_dst, mm * 4, "edge targets")) ||
+        (rc = gw_alloc(ctx, (void**)&v_in, mm * 4, "edge sort values")) ||
+        (rc = gw_alloc(ctx, (void**)&v_out, mm * 4, "edge sort values (sorted)")) ||
+        (rc = gw_alloc(ctx, (void**)&k_in, mm * 8, "edge sort keys")) ||
+        (rc = gw_alloc(ctx, (void**)&k_out, mm * 8, "edge sort keys (sorted)")) ||
+        (rc = gw_alloc(ctx, (void**)&d_cnt, 16, "edge counters"))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(d_cnt, 0, 16, ctx->stream);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(d_src, src, m * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(d_dst, dst, m * 4, hipMemcpyHostToDevice, ctx->stream);
+    const uint32_t tb = 256;
+    if (e == hipSuccess && m) {
+        edge_keys_kernel<<<dim3(gw_grid(m, tb)), dim3(tb), 0, ctx->stream>>>(d_src, d_dst, m, drop, sort_rows, k_in, v_in,
+                                                                             d_cnt, d_cnt + 1);
+        e = hipGetLastError();
+    }
+    unsigned long long cnt[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt, d_cnt, 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return gw_hip_check(ctx, e, "edge list upload");
+    }
+    const uint64_t kept = m - cnt[0];
+    const uint64_t nv = n ? n : (kept ? cnt[1] + 1 : 1);
+    if (kept && cnt[1] >= nv) {
+        cleanup();
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_set_edges: vertex " + std::to_string(cnt[1]) + " not below n = " +
+                                           std::to_string(nv));
+    }
+    if (nv >= 0xFFFFFFFFull) {
+        cleanup();
+        return gw_fail(ctx, GW_EINVAL, "gw_graph_set_edges: vertex ids must stay below 2^32 - 1");
+    }
+    // the keys sort on every bit (a begin bit above 0 is not used here)
+    const uint32_t end_bit = sort_rows ? 64u : (drop && cnt[0] ? 64u : 32u);
+    if (m) {
+        e = rocprim::radix_sort_pairs(nullptr, temp_bytes, k_in, k_out, v_in, v_out, m, 0u, end_bit, ctx->stream);
+        if (e == hipSuccess && (rc = gw_alloc(ctx, &temp, temp_bytes, "edge sort temp"))) {
+            cleanup();
+            return rc;
+        }
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(temp, temp_bytes, k_in, k_out, v_in, v_out, m, 0u, end_bit, ctx->stream);
+        if (e != hipSuccess) {
+            cleanup();
+            return gw_hip_check(ctx, e, "edge sort");
+        }
+    }
+    gw_free(ctx, k_in, mm * 8);
+    k_in = nullptr;
+    gw_free(ctx, d_src, mm * 4);
+    d_src = nullptr;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_off, (nv + 1) * 8, "offsets")) ||
+        (rc = gw_alloc(ctx, (void**)&ctx->d_col, std::max<uint64_t>(1, kept) * 4, "columns"))) {
+        cleanup();
+        return rc;
+    }
+    ctx->n = static_cast<uint32_t>(nv);
+    ctx->m = kept;
+    csr_from_sources_kernel<<<dim3(gw_grid(kept + 1, tb)), dim3(tb), 0, ctx->stream>>>(k_out, kept, sort_rows ? 32u : 0u,
+                                                                                        ctx->n, ctx->d_off);
+    e = hipGetLastError();
+    if (e == hipSuccess && kept) e = hipMemcpyAsync(ctx->d_col, v_out, kept * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return gw_hip_check(ctx, e, "edge CSR build");
+    return GW_OK;
 }

@@ -27,6 +27,8 @@
  *   gw_graph_rmat              (no counterpart: the reference reads a
  *                              preprocessed CSR, conversions.hpp:181-298; this
  *                              builds the same CSR layout on the device)
+ *   gw_graph_set_edges         convert_to_csr (preprocess/conversions.hpp:181-298):
+ *                              the CSR of an edge list, built on the device
  *   gw_graph_rmat_host         the same, into pinned host memory for graphs
  *                              larger than the device (the reference's graph
  *                              lives on disk and is read block by block)
@@ -195,6 +197,17 @@ int gw_device_info(gw_ctx *ctx, char *name, size_t name_len, uint32_t *cu_count,
 int gw_graph_set(gw_ctx *ctx, const uint64_t *off, const uint32_t *col, uint32_t n, uint64_t m);
 int gw_graph_rmat(gw_ctx *ctx, uint32_t scale, uint64_t m, uint64_t seed,
                   uint32_t ta, uint32_t tab, uint32_t tabc);
+/* CSR built on the device from a host edge list src[e] -> dst[e] (e < m):
+ * edges grouped by source with a stable sort, so each row keeps the list's
+ * edge order -- the rows convert_to_csr writes from a list grouped by source
+ * (conversions.hpp:181-298) -- or with GW_EDGES_SORT_ROWS its columns
+ * ascending; GW_EDGES_DROP_SELF_LOOPS drops u -> u edges as the converter
+ * does (conversions.hpp:237).  n = 0: the largest vertex id + 1 (the
+ * converter's vertex count); otherwise every id must be below n. */
+#define GW_EDGES_DROP_SELF_LOOPS 1u
+#define GW_EDGES_SORT_ROWS 2u
+int gw_graph_set_edges(gw_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint64_t m, uint32_t n,
+                       uint32_t flags);
 int gw_graph_blocks(gw_ctx *ctx, const uint32_t *block_first, uint32_t nblocks);
 /* Reverse every edge of the device-resident graph (rows become in-neighbour
  * lists, ascending, duplicates kept). */
