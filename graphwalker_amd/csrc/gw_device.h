@@ -620,6 +620,29 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
+// Adds a and b, summed over the workgroup, to d0 and d1: one atomic per
+// workgroup and counter (none for a zero sum) instead of one per wavefront --
+// a persistent grid's wavefronts all add to the same two addresses, where
+// device-scope atomics serialise.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void workgroup_add2(unsigned long long* d0, unsigned long long* d1, uint32_t a, uint32_t b) {
+    __shared__ unsigned long long s_tot[2];
+    if (threadIdx.x == 0) {
+        s_tot[0] = 0;
+        s_tot[1] = 0;
+    }
+    __syncthreads();
+    const uint32_t wa = wave_sum(a), wb = wave_sum(b);
+    if (__lane_id() == 0) {
+        if (wa) atomicAdd(&s_tot[0], static_cast<unsigned long long>(wa));
+        if (wb) atomicAdd(&s_tot[1], static_cast<unsigned long long>(wb));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_tot[0]) atomicAdd(d0, s_tot[0]);
+        if (s_tot[1]) atomicAdd(d1, s_tot[1]);
+    }
+}
+
 // Block-wide top-k over n items with unique 64-bit rank keys (larger is
 // better; key 0 marks an empty item and is never selected), k <= kTopkMax.
 // An MSB radix select over LDS histograms finds the k-th largest key: one

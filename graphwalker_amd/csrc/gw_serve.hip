@@ -103,6 +103,7 @@ struct ServeState {
     hipEvent_t table_ev = nullptr;
     bool table_dirty = true;
     uint64_t h2d_bytes = 0, pages_loaded = 0;
+    uint32_t grid_max = 0;                     // serve-step workgroups resident on the device at once
 };
 
 // The context's serving state (null when no server is open on it).
@@ -157,8 +158,11 @@ constexpr uint32_t kServeCache = 512;  // 24 KiB per 256-thread workgroup: 6 wor
 constexpr uint32_t kServeCacheFlush = kServeCache * 3 / 4;
 constexpr uint32_t kServeCacheProbes = 4;
 constexpr unsigned long long kCacheEmpty = ~0ull;
+// Marks a cache key that counts visits to the query's own source; slots stay
+// below 2^16, so a marked key is never kCacheEmpty.
+constexpr unsigned long long kCacheSource = 1ull << 63;
 struct ServeCache {
-    unsigned long long key[kServeCache];  // (slot << 32) | vertex
+    unsigned long long key[kServeCache];  // [kCacheSource |] (slot << 32) | vertex
     uint32_t cnt[kServeCache];
     uint32_t fill;
 };
@@ -172,14 +176,17 @@ __device__ __forceinline__ void serve_cache_init(ServeCache* c, uint32_t lane) {
     if (lane == 0) c->fill = 0;
 }
 
-// Per-visit recorder of a served walk.  Visits to the query's own source (a
-// quarter or more of a restart walk's visits) go to a per-query counter, one
-// atomic per (wavefront, query); lanes of one query on one vertex combine;
-// other visits go through the wavefront's LDS cache to the query's hash
+// Per-visit recorder of a served walk.  Lanes of one query on one vertex
+// combine, then visits go through the wavefront's LDS cache.  A flush sends
+// the cached visits to the query's own source (a quarter or more of a restart
+// walk's visits) to a per-query counter, and the others to the query's hash
 // region: one compare-and-swap either claims an empty slot or returns the
 // entry there (then an atomic add if it is this vertex's).  A claimed slot is
 // appended to the query's occupied list (one atomic per (wavefront, query)),
-// so completion reads the entries without scanning the region.
+// so completion reads the entries without scanning the region.  Counting the
+// source through the cache matters: every wavefront of a query would
+// otherwise update the same counter on nearly every hop, and device-scope
+// atomics on one address serialize.
 struct ServeRec {
     unsigned long long* arena;
     uint32_t* srccount;
@@ -198,11 +205,18 @@ struct ServeRec {
     // slots - 1) for the lanes with `want`; every active lane calls it.
     __device__ __forceinline__ void arena_add(bool want, uint32_t sl, uint64_t rb, uint32_t rm, uint32_t v,
                                               uint32_t inc) {
+        arena_add_from(want, sl, rb, rm, v, inc,
+                       static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm);
+    }
+
+    // The same, probing from slot h0 of the region.
+    __device__ __forceinline__ void arena_add_from(bool want, uint32_t sl, uint64_t rb, uint32_t rm, uint32_t v,
+                                                   uint32_t inc, uint32_t h0) {
         constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
         uint32_t claimed = kNoSlot;
         if (want) {
             const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | inc;
-            uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm;
+            uint32_t h = h0;
             bool placed = false;
             for (uint32_t probe = 0; probe <= rm; ++probe) {
                 unsigned long long* cell = arena + rb + h;
@@ -246,12 +260,13 @@ struct ServeRec {
             if (i < kServeCache) {
                 const unsigned long long k = cache->key[i];
                 if (k != kCacheEmpty) {
-                    have = true;
-                    es = static_cast<uint32_t>(k >> 32);
+                    es = static_cast<uint32_t>(k >> 32) & 0xFFFFu;
                     ev = static_cast<uint32_t>(k);
                     ec = cache->cnt[i];
                     cache->key[i] = kCacheEmpty;
                     cache->cnt[i] = 0;
+                    if (k & kCacheSource) atomicAdd(&srccount[es], ec);
+                    else have = true;
                 }
             }
             uint64_t rb = 0;
@@ -271,25 +286,30 @@ struct ServeRec {
     __device__ __forceinline__ void visit(uint32_t v, uint32_t) {
         ++visits;
         const bool at_src = v == source;
+        // Lanes of one query standing on one vertex combine into one update:
+        // first those at their query's source (one pass per query there),
+        // then up to three groups of the others; that loop stops at the
+        // first group of one lane (the rest are likely distinct too).
+        bool mine = true;
+        uint32_t inc = 1;
         uint64_t todo = __ballot(at_src);
-        while (todo) {
+        while (todo) {  // wave-uniform
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
             const uint32_t ls = lane_value(slot, leader);
-            const uint64_t grp = __ballot(at_src && slot == ls);
-            if (__lane_id() == leader) atomicAdd(&srccount[ls], static_cast<uint32_t>(__popcll(grp)));
+            const bool in = at_src && slot == ls;
+            const uint64_t grp = __ballot(in);
+            if (in) {
+                if (__lane_id() == leader) inc = static_cast<uint32_t>(__popcll(grp));
+                else mine = false;
+            }
             todo &= ~grp;
         }
-        // Lanes of one query standing on one vertex combine into one update.
-        // Up to three groups are folded per visit; the loop stops at the
-        // first group of one lane (the rest are likely distinct too).
-        bool mine = !at_src;
-        uint32_t inc = 1;
-        uint64_t fold = __ballot(mine);
+        uint64_t fold = __ballot(!at_src);
         for (uint32_t it = 0; it < 3 && fold; ++it) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(fold)) - 1;
             const uint32_t v0 = lane_value(v, leader);
             const uint32_t s0 = lane_value(slot, leader);
-            const bool in = mine && v == v0 && slot == s0;
+            const bool in = !at_src && v == v0 && slot == s0;
             const uint64_t grp = __ballot(in);
             if (in) {
                 if (__lane_id() == leader) inc = static_cast<uint32_t>(__popcll(grp));
@@ -300,19 +320,29 @@ struct ServeRec {
         }
         // the wavefront's cache first; a lane that finds no room in a few
         // probes goes to the arena directly
+        bool fresh = false;
         if (mine) {
-            const unsigned long long key = (static_cast<unsigned long long>(slot) << 32) | v;
+            const unsigned long long key = (at_src ? kCacheSource : 0ull) | (static_cast<unsigned long long>(slot) << 32) | v;
             uint32_t h = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> 56) & (kServeCache - 1);
             for (uint32_t p = 0; p < kServeCacheProbes; ++p) {
                 const unsigned long long old = atomicCAS(&cache->key[h], kCacheEmpty, key);
                 if (old == kCacheEmpty || old == key) {
                     atomicAdd(&cache->cnt[h], inc);
-                    if (old == kCacheEmpty) atomicAdd(&cache->fill, 1u);
+                    fresh = old == kCacheEmpty;
                     mine = false;
                     break;
                 }
                 h = (h + 1) & (kServeCache - 1);
             }
+        }
+        {  // entries claimed in the cache: one LDS atomic per wavefront
+            const uint64_t fm = __ballot(fresh);
+            if (fm && __lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(fm)) - 1))
+                atomicAdd(&cache->fill, static_cast<uint32_t>(__popcll(fm)));
+        }
+        if (mine && at_src) {  // no room in the cache
+            atomicAdd(&srccount[slot], inc);
+            mine = false;
         }
         arena_add(mine, slot, base, mask, v, inc);
         const uint64_t active = __ballot(1);
@@ -421,11 +451,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
         }
     }
     rec.flush(__ballot(1));  // every lane of the wavefront is here
-    const uint32_t st = wave_sum(tot_steps), vi = wave_sum(tot_visits);
-    if (lane == 0) {
-        atomicAdd(&cnt[1], static_cast<unsigned long long>(st));
-        atomicAdd(&cnt[2], static_cast<unsigned long long>(vi));
-    }
+    workgroup_add2(&cnt[1], &cnt[2], tot_steps, tot_visits);
 }
 
 // A query admitted in this step: its slot record, walk count and where its
@@ -812,6 +838,14 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         gw_serve_close_state(ctx, S);
         return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: successor-header layout does not fit");
     }
+    {  // workgroups of the serve-step kernel resident per CU (its LDS cache bounds them)
+        int per_cu = 0;
+        hipError_t oe = S->ooc   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_step_kernel<GW_RESTART, false, true>, 256, 0)
+                      : S->ext ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_step_kernel<GW_RESTART, true, false>, 256, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_step_kernel<GW_RESTART, false, false>, 256, 0);
+        if (oe != hipSuccess || per_cu <= 0) per_cu = 4;
+        S->grid_max = std::max(1u, ctx->cu_count) * static_cast<uint32_t>(std::min(per_cu, 8));
+    }
     GW_HIP(ctx, hipMemsetAsync(S->arena, 0, S->arena_slots * 8, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t i = max_queries; i-- > 0;) S->free_slots.push_back(i);
@@ -895,7 +929,11 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     if (S.live) {
         const Prog p{L, S.prog.nvertices, S.prog.stop_thr, static_cast<uint32_t>(S.prog.seed),
                      static_cast<uint32_t>(S.prog.seed >> 32), 0u, 0u, 0u};
-        const uint32_t grid = std::max(1u, ctx->cu_count) * 4u * 8u * 64u / 256u;
+        // one walk per lane while the pool fits the resident grid: no more
+        // workgroups than the walks need (each wavefront that finds no work
+        // still pays a cursor atomic), no more than stay resident at once
+        const uint64_t need = (S.live + 255) / 256;
+        const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(S.grid_max, need)));
         hipError_t e = S.prog.kind == GW_RESTART ? launch_serve<GW_RESTART>(ctx, S, p, S.live, grid)
                      : S.prog.kind == GW_STOP    ? launch_serve<GW_STOP>(ctx, S, p, S.live, grid)
                                                  : launch_serve<GW_JUMP>(ctx, S, p, S.live, grid);

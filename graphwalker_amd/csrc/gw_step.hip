@@ -175,12 +175,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     }
     if (KL) keylog_close(klog, s_slab[threadIdx.x >> 6], lane);
     if (RG) rec.close_regions();
-    const uint32_t st = wave_sum(tot_steps);
-    const uint32_t vi = wave_sum(tot_visits);
-    if (lane == 0) {
-        atomicAdd(&counters[0], static_cast<unsigned long long>(st));
-        atomicAdd(&counters[1], static_cast<unsigned long long>(vi));
-    }
+    workgroup_add2(&counters[0], &counters[1], tot_steps, tot_visits);
 }
 
 constexpr uint32_t kMaxBlocks = 2048;
