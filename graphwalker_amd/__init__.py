@@ -25,6 +25,7 @@ KIND_NAMES = {"jump": JUMP, "stop": STOP, "prob": PROB, "restart_joint": RESTART
 # recorders
 REC_NONE, REC_VERTEX_COUNT, REC_WALK_SUMMARY, REC_EVENTS, REC_SOURCE_COUNT = 0, 1, 2, 4, 8
 REC_NO_FINAL = 16  # skip the final walk records (flag, with any recorder)
+REC_EVENTS_LAST = 32  # with REC_EVENTS: visits past event_cap overwrite the log's last slot
 # schedules
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT, SCHED_BLOCKED = 0, 1, 2, 3
 SCHED_NAMES = {"auto": SCHED_AUTO, "plain": SCHED_PLAIN, "persistent": SCHED_PERSISTENT, "blocked": SCHED_BLOCKED}

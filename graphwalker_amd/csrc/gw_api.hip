@@ -509,8 +509,11 @@ int gw_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
     if (prog->stop_thr > (1ull << 32)) return gw_fail(ctx, GW_EINVAL, "gw_run: stop_thr above 2^32");
     if (prog->kind == GW_JUMP && prog->nvertices == 0) return gw_fail(ctx, GW_EINVAL, "gw_run: JUMP needs nvertices");
     if (prog->kind == GW_JUMP && prog->nvertices > ctx->n) return gw_fail(ctx, GW_EINVAL, "gw_run: nvertices above graph size");
-    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS | GW_REC_SOURCE_COUNT | GW_REC_NO_FINAL))
+    if (prog->recorder & ~(GW_REC_VERTEX_COUNT | GW_REC_WALK_SUMMARY | GW_REC_EVENTS | GW_REC_SOURCE_COUNT | GW_REC_NO_FINAL |
+                           GW_REC_EVENTS_LAST))
         return gw_fail(ctx, GW_EINVAL, "gw_run: unknown recorder");
+    if ((prog->recorder & GW_REC_EVENTS_LAST) && !(prog->recorder & GW_REC_EVENTS))
+        return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_EVENTS_LAST needs GW_REC_EVENTS");
     if ((prog->recorder & GW_REC_SOURCE_COUNT) && (prog->recorder & (GW_REC_VERTEX_COUNT | GW_REC_EVENTS)))
         return gw_fail(ctx, GW_EINVAL, "gw_run: GW_REC_SOURCE_COUNT combines with GW_REC_WALK_SUMMARY only");
     if (prog->source_method > GW_SRC_PARTITIONED) return gw_fail(ctx, GW_EINVAL, "gw_run: unknown source_method");

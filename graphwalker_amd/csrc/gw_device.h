@@ -95,6 +95,7 @@ enum : int { kMoved = 0, kFinished = 1 };
 struct EventLog {
     uint64_t* __restrict__ ev;
     uint32_t cap;
+    uint32_t keep_last;  // visits past cap overwrite the last slot (cap >= 1)
 };
 
 // Device-wide open-addressing hash of (source field, vertex) -> visit count,
@@ -412,8 +413,9 @@ struct Recorder {
         : counts(c), visits(0), steps(0), hash(kHashInit), log(l), id(0), table(t), source(0), klog(kl), slab(sl),
           rbuf(rb) {}
     __device__ __forceinline__ void visit(uint32_t v, uint32_t hop) {
-        if (kEvents && visits < log.cap)
-            log.ev[static_cast<uint64_t>(id) * log.cap + visits] = (static_cast<uint64_t>(hop) << 32) | v;
+        if (kEvents && (visits < log.cap || log.keep_last))
+            log.ev[static_cast<uint64_t>(id) * log.cap + (visits < log.cap ? visits : log.cap - 1)] =
+                (static_cast<uint64_t>(hop) << 32) | v;
         ++visits;
         if (kVertexCount || kSourceCount) {
             // A walk standing on its own source field's vertex -- for restart

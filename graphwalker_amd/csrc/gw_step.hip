@@ -836,7 +836,7 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
             return gw_fail(ctx, GW_ENOMEM, "gw_run: successor-header layout does not fit (HBM budget or degree >= 2^32)");
     }
     st.layout = ext ? GW_LAYOUT_EXT : GW_LAYOUT_CSR;
-    uint32_t rec = prog->recorder & ~GW_REC_NO_FINAL;
+    uint32_t rec = prog->recorder & ~(GW_REC_NO_FINAL | GW_REC_EVENTS_LAST);
     if (rec & GW_REC_EVENTS) rec |= GW_REC_WALK_SUMMARY;
     if ((rec & GW_REC_EVENTS) && ctx->events_cap != prog->event_cap) {
         gw_free(ctx, ctx->d_events, ctx->walk_cap * ctx->events_cap * 8);
@@ -881,7 +881,8 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
     L.p = Prog{prog->L, prog->nvertices, prog->stop_thr, static_cast<uint32_t>(prog->seed),
                static_cast<uint32_t>(prog->seed >> 32), prog->stream, 0u, chunk};
     L.g = Graph{ctx->d_off, ctx->d_col, reinterpret_cast<const Edge*>(ctx->d_ext), ctx->n};
-    L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u};
+    L.log = EventLog{ctx->d_events, (rec & GW_REC_EVENTS) ? prog->event_cap : 0u,
+                     (rec & GW_REC_EVENTS) && (prog->recorder & GW_REC_EVENTS_LAST) ? 1u : 0u};
     L.table = SourceTable{ctx->d_table, ctx->d_counters + kCntTableOverflow, ctx->table_slots - 1, 64u - ctx->table_log2,
                           ctx->table_vbits, ctx->table_cbits, ctx->table_wide ? 1u : 0u};
     L.klog = KeyLog{};  // region fields stay null unless the run fills regions directly

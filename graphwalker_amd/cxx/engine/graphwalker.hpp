@@ -30,6 +30,11 @@
 //   recorder vertex  per-vertex counts on the device (GW_REC_VERTEX_COUNT),
 //                    replayed as updateInfo(vertex, vertex, t, 0) calls -- for
 //                    apps that count visits per vertex (rwdomination);
+//   recorder last    each walk's last visit only (GW_REC_EVENTS_LAST with a
+//                    one-visit log), replayed with its (source, vertex, hop) --
+//                    for apps whose updateInfo returns early below the final
+//                    hop (graphlet counts walks back at their source at hop
+//                    L-1);
 //   recorder none    no replay (an app whose updateInfo does nothing, such as
 //                    rawrandomwalks).
 // replaythreads (default 1) replays on that many threads, each with its own
@@ -131,9 +136,12 @@ public:
         prog.nvertices = dp.kind == GW_JUMP ? (dp.N ? dp.N : nvertices) : 0;
         prog.seed = get_option_long("seed", 1);
         const std::string recm = get_option_string("recorder", "events");
-        if (recm != "events" && recm != "source" && recm != "vertex" && recm != "none")
-            logstream(LOG_FATAL) << "recorder must be events, source, vertex or none (got " << recm << ")" << std::endl;
+        if (recm != "events" && recm != "last" && recm != "source" && recm != "vertex" && recm != "none")
+            logstream(LOG_FATAL) << "recorder must be events, last, source, vertex or none (got " << recm << ")"
+                                 << std::endl;
+        const bool last_only = recm == "last";
         prog.recorder = recm == "events" ? (GW_REC_EVENTS | GW_REC_WALK_SUMMARY)
+                      : last_only        ? (GW_REC_EVENTS | GW_REC_WALK_SUMMARY | GW_REC_EVENTS_LAST)
                       : recm == "source" ? (GW_REC_SOURCE_COUNT | GW_REC_NO_FINAL)
                       : recm == "vertex" ? (GW_REC_VERTEX_COUNT | GW_REC_NO_FINAL)
                                          : GW_REC_NO_FINAL;
@@ -141,7 +149,7 @@ public:
         prog.schedule = sched == "plain" ? GW_SCHED_PLAIN : sched == "blocked" ? GW_SCHED_BLOCKED : GW_SCHED_PERSISTENT;
         if (prog.schedule == GW_SCHED_BLOCKED && !blocks_on_device_) prog.schedule = GW_SCHED_PERSISTENT;
         if (host_resident_) prog.schedule = GW_SCHED_BLOCKED;  // the block cache steps block by block
-        prog.event_cap = dp.kind == GW_PROB ? 64u : std::max<uint32_t>(1u, dp.L);
+        prog.event_cap = last_only ? 1u : dp.kind == GW_PROB ? 64u : std::max<uint32_t>(1u, dp.L);
 
         m.start_time("2_findSubGraph");
         check(gw_walks_load(ctx_, walks.data(), walks.size()), "gw_walks_load");
@@ -149,7 +157,7 @@ public:
 
         gw_stats st;
         const int rthreads = std::max(1, std::min<int>(get_option_int("replaythreads", 1), exec_threads));
-        if (recm == "events") {
+        if (recm == "events" || last_only) {
             std::vector<uint64_t> events;
             std::vector<uint32_t> visits(walks.size());
             for (;;) {
@@ -158,7 +166,7 @@ public:
                 m.stop_time("5_exec_updates");
                 check(gw_summary_read(ctx_, nullptr, visits.data(), visits.size()), "gw_summary_read");
                 const uint32_t vmax = visits.empty() ? 0 : *std::max_element(visits.begin(), visits.end());
-                if (vmax <= prog.event_cap) break;
+                if (last_only || vmax <= prog.event_cap) break;  // the last-visit log never outgrows its slot
                 // an uncapped walk outgrew the log: rerun (the same walks) with a
                 // log that holds the longest, if the device holds it
                 uint64_t hbm = 0;
@@ -173,7 +181,8 @@ public:
             events.resize(walks.size() * static_cast<size_t>(prog.event_cap));
             if (!walks.empty())
                 check(gw_events_read(ctx_, events.data(), walks.size(), prog.event_cap), "gw_events_read");
-            // Replay every visit to updateInfo, walk by walk, hop by hop; with
+            // Replay every logged visit (with recorder last, each walk's last
+            // one) to updateInfo, walk by walk, hop by hop; with
             // replaythreads > 1 each thread takes a contiguous range of the
             // walks (sorted by source) cut at source boundaries.
             m.start_time("z_replay_updateInfo");
@@ -183,7 +192,8 @@ public:
                 for (size_t i = cut[t]; i < cut[t + 1]; ++i) {
                     const gw_walk& w = walks[i];
                     const uint64_t* ev = events.data() + static_cast<size_t>(w.id) * prog.event_cap;
-                    for (uint32_t k = 0; k < visits[w.id]; ++k)
+                    const uint32_t nk = last_only ? std::min<uint32_t>(visits[w.id], 1u) : visits[w.id];
+                    for (uint32_t k = 0; k < nk; ++k)
                         userprogram.updateInfo(w.source, static_cast<vid_t>(ev[k] & 0xffffffffu), static_cast<tid_t>(t),
                                                static_cast<hid_t>(ev[k] >> 32));
                 }

@@ -107,6 +107,11 @@ typedef struct gw_ctx gw_ctx;
                                   visit distributions of msppr (msppr.cpp:71-76), exact */
 #define GW_REC_NO_FINAL 16u    /* skip writing each walk's final record (the reference keeps none);
                                   gw_walks_read then refuses until a run writes them again */
+#define GW_REC_EVENTS_LAST 32u /* with GW_REC_EVENTS: a walk with more visits than event_cap keeps its
+                                  latest visit in the last slot of its log.  With event_cap 1 the log
+                                  is each walk's last visit: an updateInfo that looks only at the
+                                  final hop (graphlet.cpp:46-52, hop == L-1 and dstId == source) is
+                                  fed one visit per walk */
 
 /* schedules */
 #define GW_SCHED_AUTO 0u

@@ -98,3 +98,13 @@ def test_sources_match_oracle():
 
     for seed, n, k in ((3, 1 << 26, 4096), (1, 1000, 77), (2**40 + 5, 7, 10)):
         assert np.array_equal(gw.sources(seed, n, k), O.sources(seed, n, k))
+
+
+def test_python_constants_match_header():
+    """Every GW_<NAME> integer #define the binding mirrors as <NAME> has the header's value."""
+    defs = dict((k, int(v, 0)) for k, v in re.findall(r"#define GW_([A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)u?\b",
+                                                       open(HEADER).read()))
+    mirrored = [k for k in defs if hasattr(gw, k) and isinstance(getattr(gw, k), int)]
+    assert {"REC_EVENTS", "REC_EVENTS_LAST", "REC_NO_FINAL"} <= set(mirrored)
+    for k in mirrored:
+        assert getattr(gw, k) == defs[k], k

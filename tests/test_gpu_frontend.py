@@ -114,10 +114,13 @@ def test_reference_rawrandomwalks_unchanged(graph):
 
 
 @pytest.mark.skipif(not os.access(os.path.join(COMPAT, "graphlet"), os.X_OK), reason="compat apps not built")
-def test_reference_graphlet_unchanged(tmp_path):
+@pytest.mark.parametrize("recorder", ["events", "last"])
+def test_reference_graphlet_unchanged(tmp_path, recorder):
     """apps/graphlet.cpp of the reference, compiled unchanged against the
     front end: its closed-walk ratio (rand() sources, so distributional) within
-    5 sigma of the FP64 probability averaged over uniform sources."""
+    5 sigma of the FP64 probability averaged over uniform sources -- with every
+    visit replayed, and with only each walk's last visit recorded on the device
+    and replayed (the hop L-1 visit graphlet's updateInfo counts)."""
     from oracle import fp64
     (tmp_path / "conf").mkdir()
     (tmp_path / "conf" / "graphwalker.cnf").write_text("# empty\n")
@@ -127,7 +130,7 @@ def test_reference_graphlet_unchanged(tmp_path):
     write_graph(base, off, col, 1024)
     R, L = 200_000, 4
     stdout = run([os.path.join(COMPAT, "graphlet"), "file", base, "N", str(n), "R", str(R), "L", str(L),
-                  "blocksize_kb", "1024"], tmp_path)
+                  "blocksize_kb", "1024", "recorder", recorder], tmp_path)
     m = re.search(r"cnt_ok:\s+(\d+)", stdout)
     assert m, stdout[-2000:]
     f = float(fp64.return_at(off, col, L, 0.15).mean())

@@ -282,6 +282,47 @@ def test_event_log_is_the_trajectory(engine, small_graph, kind, variant):
         assert np.array_equal(row & 0xFFFFFFFF, tv[:k]) and np.array_equal(row >> 32, th[:k])
 
 
+@pytest.mark.parametrize("variant", [("plain", "csr"), ("persistent", "ext"), ("blocked", "ext")])
+@pytest.mark.parametrize("kind", [gw.STOP, gw.RESTART, gw.JUMP, gw.PROB])
+@pytest.mark.parametrize("cap", [1, 3])
+def test_event_log_keeps_last_visit(engine, small_graph, kind, variant, cap):
+    """REC_EVENTS_LAST: the first cap - 1 visits, then the latest visit in the
+    last slot -- with cap 1 each walk's last (vertex, hop), the visit an
+    updateInfo that looks only at hop L - 1 needs (graphlet.cpp:46-52)."""
+    off, col = small_graph
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    if variant[0] == "blocked":
+        engine.graph_blocks([0, 512, 1024, 3000, n])
+    walks = _walks(3000, n, kind, seed=12)
+    engine.walks_load(walks)
+    prog = gw.program(kind, 4, 0.15, n, seed=78, recorder=gw.REC_EVENTS | gw.REC_EVENTS_LAST, schedule=variant[0],
+                      layout=variant[1], event_cap=cap)
+    engine.run(prog)
+    ev = engine.events(cap)
+    _, vis = engine.summary()
+    op = oprog(prog)
+    for i in range(0, 3000, 5):
+        tv, th, _ = O.trace(op, off, col, walks[i])
+        wid = walks[i]["id"]
+        assert vis[wid] == len(tv)
+        if len(tv) == 0:
+            continue
+        k = min(len(tv), cap)
+        exp_v = np.concatenate([tv[:k - 1], tv[-1:]])
+        exp_h = np.concatenate([th[:k - 1], th[-1:]])
+        row = ev[wid, :k]
+        assert np.array_equal(row & 0xFFFFFFFF, exp_v) and np.array_equal(row >> 32, exp_h), (i, kind, cap)
+
+
+def test_events_last_needs_events(engine, small_graph):
+    off, col = small_graph
+    engine.graph_set(off, col)
+    engine.walks_load(_walks(10, off.shape[0] - 1, gw.STOP, seed=1))
+    with pytest.raises(gw.GwError, match="GW_REC_EVENTS_LAST needs GW_REC_EVENTS"):
+        engine.run(gw.program(gw.STOP, 4, 0.15, recorder=gw.REC_EVENTS_LAST))
+
+
 def _expected_source_counts(prog, off, col, sources, field, wps):
     out_s, out_v, out_c = [], [], []
     for k, (s, f) in enumerate(zip(sources, field)):
