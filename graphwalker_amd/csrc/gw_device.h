@@ -203,7 +203,7 @@ __device__ __forceinline__ void keylog_add(const KeyLog& k, uint64_t* slab, uint
 // one needed for a third field) is flushed to its field's region with one
 // cursor atomic for the whole buffer and coalesced stores, so the step
 // kernel waits on no atomic per visit and leaves no holes in a region.
-constexpr uint32_t kRegionBuf = 256;
+constexpr uint32_t kRegionBuf = 1024;
 constexpr uint32_t kNoField = 0xFFFFFFFFu;
 struct alignas(16) RegionBuf {
     uint32_t st[4];  // {field, count} of buffer 0, then of buffer 1: one 16-B LDS read
@@ -299,9 +299,14 @@ __device__ __forceinline__ void region_add(const KeyLog& k, RegionBuf* rb, uint3
 // The same direct regions for a caller that records in wave-uniform control
 // flow (every lane of the wavefront present, `want` marking the lanes with a
 // visit): the two buffers' {field, count} state lives in wave-uniform
-// registers instead of LDS, and group leaders broadcast with readlane.
+// registers instead of LDS, and group leaders broadcast with readlane.  Visits
+// of a buffered field's designated vertex are only counted, in d0 / d1, and
+// reach field_count when the field leaves its buffer (or at kernel exit):
+// nearly every step of a short-walk batch starts walks at their source.  A
+// buffer whose field has only such counts (c == 0) counts as empty: giving
+// it to another field evicts just the count.
 struct RegionState {
-    uint32_t f0 = kNoField, c0 = 0, f1 = kNoField, c1 = 0;
+    uint32_t f0 = kNoField, c0 = 0, f1 = kNoField, c1 = 0, d0 = 0, d1 = 0;
 };
 
 __device__ __forceinline__ void region_flush_all(const KeyLog& k, const uint32_t* keys, uint32_t f, uint32_t n) {
@@ -318,17 +323,29 @@ __device__ __forceinline__ void region_flush_all(const KeyLog& k, const uint32_t
     __builtin_amdgcn_wave_barrier();  // every lane has read the ids before they are overwritten
 }
 
+// A buffered field leaves its buffer: its staged ids to the region, its
+// designated-vertex count to field_count.
+__device__ __forceinline__ void region_evict(const KeyLog& k, const uint32_t* keys, uint32_t f, uint32_t n,
+                                             uint32_t d) {
+    region_flush_all(k, keys, f, n);
+    if (d && __lane_id() == 0) atomicAdd(&k.field_count[f], d);
+}
+
 __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* rb, RegionState& st, bool want,
                                                    uint32_t source, uint32_t v, uint32_t field_v) {
     const uint32_t lane = __lane_id();
-    // visits of the field's designated vertex: one count per (wavefront, field)
+    // visits of the field's designated vertex: counted in the buffer state
+    // while their field holds a buffer, else one atomic per (wavefront, field)
     const bool fvis = want && v == field_v;
     uint64_t fdo = __ballot(fvis);
     while (fdo) {
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(fdo)) - 1;
         const uint32_t ls = lane_value(source, leader);
         const uint64_t grp = __ballot(fvis && source == ls);
-        if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], static_cast<uint32_t>(__popcll(grp)));
+        const uint32_t d = __popcll(grp);
+        if (st.f0 == ls) st.d0 += d;
+        else if (st.f1 == ls) st.d1 += d;
+        else if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], d);
         fdo &= ~grp;
     }
     const bool rest = want && !fvis;
@@ -349,16 +366,16 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
             second = false;
         } else if (st.f1 == ls) {
             second = true;
-        } else {  // take an empty buffer, else flush the fuller one
+        } else {  // take an empty buffer, else evict the fuller one
             second = st.c0 != 0 && (st.c1 == 0 || st.c1 > st.c0);
             if (second) {
-                region_flush_all(k, rb->key[1], st.f1, st.c1);
+                region_evict(k, rb->key[1], st.f1, st.c1, st.d1);
                 st.f1 = ls;
-                st.c1 = 0;
+                st.c1 = st.d1 = 0;
             } else {
-                region_flush_all(k, rb->key[0], st.f0, st.c0);
+                region_evict(k, rb->key[0], st.f0, st.c0, st.d0);
                 st.f0 = ls;
-                st.c0 = 0;
+                st.c0 = st.d0 = 0;
             }
         }
         uint32_t* keys = rb->key[second ? 1 : 0];
@@ -460,8 +477,8 @@ struct Recorder {
     // At kernel exit, every lane present.
     __device__ __forceinline__ void close_regions() {
         if (kRegion && kDeferRegion) {
-            region_flush_all(klog, rbuf->key[0], rst.f0, rst.c0);
-            region_flush_all(klog, rbuf->key[1], rst.f1, rst.c1);
+            region_evict(klog, rbuf->key[0], rst.f0, rst.c0, rst.d0);
+            region_evict(klog, rbuf->key[1], rst.f1, rst.c1, rst.d1);
         } else if (kRegion) {
             region_close(klog, rbuf);
         }

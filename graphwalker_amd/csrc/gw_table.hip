@@ -77,7 +77,6 @@ __global__ void unpack_runs_kernel(const unsigned long long* __restrict__ keys, 
 constexpr uint32_t kAggLog2 = 14;
 constexpr uint32_t kAggSlots = 1u << kAggLog2;               // LDS hash slots per workgroup (key + count: 128 KiB)
 constexpr uint32_t kAggThreads = 1024;
-constexpr uint32_t kAggWaves = kAggThreads / 64;
 constexpr uint32_t kAggPerThread = 4;
 constexpr uint32_t kAggTile = kAggThreads * kAggPerThread;   // keys loaded per iteration
 // A flushed table takes a whole tile plus the designated vertex below its 3/4 fill bound.
@@ -188,49 +187,28 @@ __device__ __forceinline__ uint32_t agg_log2(uint64_t n) {
     return lg;
 }
 
-// Occupied slots of each wavefront's contiguous share of the first `tslots`
-// slots, exclusive-scanned into s_wave[0..kAggWaves); s_wave[kAggWaves] = total.
-// All threads of the workgroup; ends synchronised.
-__device__ __forceinline__ void agg_count(const uint32_t* s_key, uint32_t tslots, uint32_t* s_wave) {
-    const uint32_t share = tslots / kAggWaves;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    uint32_t c = 0;
-    for (uint32_t i = wave * share + lane; i < (wave + 1) * share; i += 64)
-        c += __popcll(__ballot(s_key[i] != kEmptyKey));
-    if (lane == 0) s_wave[wave] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
-        for (uint32_t w = 0; w < kAggWaves; ++w) {
-            const uint32_t x = s_wave[w];
-            s_wave[w] = total;
-            total += x;
-        }
-        s_wave[kAggWaves] = total;
-    }
-    __syncthreads();
-}
-
-// Write the occupied slots of the table (first `tslots`) in slot order from
-// position `at0` on, then clear them.  kSplit: as 64-bit keys
-// (fbase | vertex) + counts into the split list (only if `fits`); else as
-// vertex + count into the source's own region.  Ends synchronised.
+// Write the occupied slots of the table (first `tslots`) from position `at0`
+// on, then clear them; `total` = the occupied slots (the table's fill).  The
+// pairs of a source are in no particular order (read-out sorts them, top-k
+// breaks ties by vertex): each wavefront reserves its places row by row with
+// one LDS atomic on s_cur, which starts at 0 and is left at 0.  kSplit: as
+// 64-bit keys (fbase | vertex) + counts into the split list (only if `fits`);
+// else as vertex + count into the source's own region.  Ends synchronised.
 template <bool kSplit>
-__device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, const uint32_t* s_wave,
+__device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t* s_cur,
                                           uint64_t at0, bool fits, uint64_t fbase, uint32_t* __restrict__ seg_v,
                                           uint32_t* __restrict__ seg_c, unsigned long long* __restrict__ split_k,
                                           uint32_t* __restrict__ split_c) {
-    const uint32_t share = tslots / kAggWaves;
     const uint32_t lane = __lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    uint64_t at = at0 + s_wave[wave];
-    for (uint32_t i = wave * share + lane; i < (wave + 1) * share; i += 64) {
+    for (uint32_t i = threadIdx.x; i < tslots; i += kAggThreads) {  // tslots >= 1024: every lane is present
         const uint32_t k = s_key[i];
         const bool full = k != kEmptyKey;
         const uint64_t mask = __ballot(full);
+        uint32_t b = 0;
+        if (lane == 0 && mask) b = atomicAdd(s_cur, static_cast<uint32_t>(__popcll(mask)));
+        b = gw::lane_value(b, 0u);
         if (full && fits) {
-            const uint64_t pos = at + rank_in(mask);
+            const uint64_t pos = at0 + b + rank_in(mask);
             const uint32_t c = s_cnt[i];
             if (kSplit) {
                 split_k[pos] = fbase | k;
@@ -240,41 +218,50 @@ __device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint
                 seg_c[pos] = c;
             }
         }
-        at += __popcll(mask);
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
     __syncthreads();
+    if (threadIdx.x == 0) *s_cur = 0;
 }
 
-// Flush the table to the split list: one returning atomic reserves the
-// pairs (counted even when they do not fit `cap`; the host checks).
-__device__ __forceinline__ void agg_flush(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t* s_wave,
-                                          unsigned long long* s_base, uint64_t fbase,
+// Flush the table (`total` occupied slots) to the split list: one returning
+// atomic reserves the pairs (counted even when they do not fit `cap`; the
+// host checks).
+__device__ __forceinline__ void agg_flush(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t total,
+                                          uint32_t* s_cur, unsigned long long* s_base, uint64_t fbase,
                                           unsigned long long* __restrict__ split_k, uint32_t* __restrict__ split_c,
                                           unsigned long long* __restrict__ split_n, uint64_t cap) {
-    agg_count(s_key, tslots, s_wave);
-    if (threadIdx.x == 0) {
-        const uint32_t total = s_wave[kAggWaves];
-        *s_base = total ? atomicAdd(split_n, static_cast<unsigned long long>(total)) : 0ull;
-    }
+    if (threadIdx.x == 0) *s_base = total ? atomicAdd(split_n, static_cast<unsigned long long>(total)) : 0ull;
     __syncthreads();
     const uint64_t b = *s_base;
-    agg_write<true>(s_key, s_cnt, tslots, s_wave, b, b + s_wave[kAggWaves] <= cap, fbase, nullptr, nullptr, split_k,
-                    split_c);
+    agg_write<true>(s_key, s_cnt, tslots, s_cur, b, b + total <= cap, fbase, nullptr, nullptr, split_k, split_c);
 }
 
-__device__ __forceinline__ void region_meta(const uint64_t* __restrict__ region_off,
-                                            const unsigned long long* __restrict__ region_n, uint32_t nf, uint32_t f,
-                                            uint64_t& lo, uint64_t& n) {
-    lo = 0;
-    n = 0;
+// Source f's region [lo, lo + n) and, in direct-region runs (fcount set), its
+// designated vertex's count fc and id fv (fv = kEmptyKey when fc is 0).
+struct AggMeta {
+    uint64_t lo = 0, n = 0;
+    uint32_t fc = 0, fv = kEmptyKey;
+};
+
+__device__ __forceinline__ AggMeta region_meta(const uint64_t* __restrict__ region_off,
+                                               const unsigned long long* __restrict__ region_n, uint32_t nf,
+                                               uint32_t f, const uint32_t* __restrict__ fvert,
+                                               const uint32_t* __restrict__ fcount) {
+    AggMeta m;
     if (f < nf) {
-        lo = region_off[f];
+        m.lo = region_off[f];
         // clamp to the region: an overflowed partition is discarded by the
         // host, but must still be read in bounds
-        n = std::min<uint64_t>(region_n[f], region_off[f + 1] - lo);
+        m.n = std::min<uint64_t>(region_n[f], region_off[f + 1] - m.lo);
+        if (fcount) {  // both loads issued together, a source ahead of their use
+            const uint32_t c = fcount[f], v = fvert[f];
+            m.fc = c;
+            m.fv = c ? v : kEmptyKey;
+        }
     }
+    return m;
 }
 
 __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ keys, uint64_t lo, uint64_t n, uint64_t first,
@@ -311,7 +298,7 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
     __shared__ uint32_t s_key[kAggSlots];
     __shared__ uint32_t s_cnt[kAggSlots];
     __shared__ uint32_t s_fill;
-    __shared__ uint32_t s_wave[kAggWaves + 1];
+    __shared__ uint32_t s_cur;
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_next;
     const uint32_t G = gridDim.x;
@@ -325,20 +312,22 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
-    if (threadIdx.x == 0) s_fill = 0;
+    if (threadIdx.x == 0) {
+        s_fill = 0;
+        s_cur = 0;
+    }
     __syncthreads();
     uint32_t fn = s_next;
-    uint64_t lo, n, nlo, nn;
-    region_meta(region_off, region_n, nf, f, lo, n);
-    region_meta(region_off, region_n, nf, fn, nlo, nn);
+    AggMeta cm = region_meta(region_off, region_n, nf, f, fvert, fcount);
+    AggMeta nm = region_meta(region_off, region_n, nf, fn, fvert, fcount);
+    uint64_t lo = cm.lo, n = cm.n, nlo = nm.lo, nn = nm.n;
     uint32_t cur[kAggPerThread], pre[kAggPerThread];
     load_tile(keys, lo, n, 0, cur);
     while (f < nf) {
         unsigned long long ticket = 0;
         if (threadIdx.x == 0) ticket = atomicAdd(queue, 1ull);  // consumed at the end of this source
         // visits of the field's designated vertex, counted apart by the step kernel
-        const uint32_t fc = fcount ? fcount[f] : 0u;
-        const uint32_t fv = fc ? fvert[f] : kEmptyKey;
+        const uint32_t fc = cm.fc, fv = cm.fv;
         if (n == 0) {
             if (threadIdx.x == 0) {
                 if (fc) {
@@ -364,6 +353,7 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
                 }
                 __syncthreads();
             }
+            uint32_t fill = 0;
             for (uint64_t base = 0;; base += kAggTile) {
                 const uint64_t next = base + kAggTile;
                 if (next < n) load_tile(keys, lo, n, next, pre);
@@ -406,11 +396,12 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
                 added = gw::wave_sum(added);
                 if (lane == 0 && added) atomicAdd(&s_fill, added);
                 __syncthreads();
-                const uint32_t fill = s_fill;
+                fill = s_fill;
                 __syncthreads();  // every thread has read the fill before it is reset
                 if (next >= n) break;
                 if (fill + std::min<uint64_t>(kAggTile, n - next) > max_fill) {  // workgroup-uniform
-                    agg_flush(s_key, s_cnt, tslots, s_wave, &s_base, fbase, split_k, split_c, split_n, split_cap);
+                    agg_flush(s_key, s_cnt, tslots, fill, &s_cur, &s_base, fbase, split_k, split_c, split_n,
+                              split_cap);
                     if (threadIdx.x == 0) s_fill = 0;
                     flushed = true;
                     __syncthreads();
@@ -419,15 +410,14 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
                 for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
             }
             if (flushed) {
-                agg_flush(s_key, s_cnt, tslots, s_wave, &s_base, fbase, split_k, split_c, split_n, split_cap);
+                agg_flush(s_key, s_cnt, tslots, fill, &s_cur, &s_base, fbase, split_k, split_c, split_n, split_cap);
                 if (threadIdx.x == 0) {
                     pairs[f] = 0;
                     atomicAdd(flushed_sources, 1ull);
                 }
             } else {
-                agg_count(s_key, tslots, s_wave);
-                const uint32_t total = s_wave[kAggWaves];
-                agg_write<false>(s_key, s_cnt, tslots, s_wave, lo, true, 0, keys, counts, nullptr, nullptr);
+                const uint32_t total = fill;
+                agg_write<false>(s_key, s_cnt, tslots, &s_cur, lo, true, 0, keys, counts, nullptr, nullptr);
                 if (threadIdx.x == 0) {
                     pairs[f] = total;
                     atomicAdd(total_pairs, static_cast<unsigned long long>(total));
@@ -442,9 +432,12 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
         for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
         f = fn;
         fn = s_next;
+        cm = nm;
         lo = nlo;
         n = nn;
-        region_meta(region_off, region_n, nf, fn, nlo, nn);
+        nm = region_meta(region_off, region_n, nf, fn, fvert, fcount);
+        nlo = nm.lo;
+        nn = nm.n;
     }
 }
 

@@ -144,9 +144,7 @@ def test_streamed_invalid_requests(engine, graph):
     with pytest.raises(gw.GwError, match="GW_SRC_HASH"):
         engine.run(gw.program(gw.STOP, 5, 0.15, n, seed=1, recorder=gw.REC_SOURCE_COUNT,
                               source_method=gw.SRC_PARTITIONED))
-    with pytest.raises(gw.GwError, match="device-resident"):
-        engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), pool_walks=1024, arena_bytes=1 << 20,
-                          max_queries=4, hops_per_step=2)
+    # (serving over a host-resident graph is supported: tests/test_gpu_serve_ooc.py)
     engine.graph_set_host(off, col, 1000)  # blocks cut to 250 bytes; the biggest vertex row does not fit
     engine.walks_load(_walks(100, n, gw.STOP, seed=1))
     with pytest.raises(gw.GwError, match="smaller than one block slot"):
