@@ -731,17 +731,34 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
             }
             if (run) atomicAdd(&t.hist[cur], run);
             __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t above = 0;
-                for (int d = 255; d >= 0; --d) {
-                    if (above + t.hist[d] >= t.remaining) {
-                        t.prefix = (prefix << 8) | static_cast<unsigned long long>(d);
-                        t.remaining -= above;
+            // The bin that holds the remaining-th largest key: the first wavefront
+            // scans the 256 bins from the top, four per lane, and the one lane
+            // whose bin crosses `remaining` records it.
+            if (threadIdx.x < 64) {
+                const uint32_t l = threadIdx.x;
+                uint32_t h[4], s = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    h[j] = t.hist[255 - (4 * l + j)];
+                    s += h[j];
+                }
+                uint32_t incl = s;
+#pragma unroll
+                for (uint32_t o = 1; o < 64; o <<= 1) {
+                    const uint32_t y = __shfl_up(incl, o, 64);
+                    if (l >= o) incl += y;
+                }
+                const uint32_t rem = t.remaining;
+                uint32_t above = incl - s;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    if (above < rem && above + h[j] >= rem) {
+                        t.prefix = (prefix << 8) | static_cast<unsigned long long>(255 - (4 * l + j));
+                        t.remaining = rem - above;
                         // the whole bucket is needed: its smallest possible key is the threshold
-                        if (t.hist[d] == t.remaining) t.shift_done = shift;
-                        break;
+                        if (h[j] == rem - above) t.shift_done = shift;
                     }
-                    above += t.hist[d];
+                    above += h[j];
                 }
             }
             __syncthreads();

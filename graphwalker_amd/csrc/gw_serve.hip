@@ -510,7 +510,7 @@ __global__ void serve_gather_kernel(unsigned long long* __restrict__ arena, cons
 // top-k of its chunks' candidates -- the top-k of the union of every chunk's
 // top-k is the query's top-k -- and writes them packed (vertex << cbits) |
 // count, count descending, ties to the smaller vertex.
-constexpr uint64_t kServeTopkChunk = 32768;
+constexpr uint64_t kServeTopkChunk = 8192;
 
 __device__ __forceinline__ uint64_t topk_chunks(uint64_t n) { return (n + 1 + kServeTopkChunk - 1) / kServeTopkChunk; }
 
@@ -1122,7 +1122,7 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     if (k == 0) return GW_OK;
     int rc;
     // candidates for a step whose completing queries average up to 8 chunks
-    // (256 K pairs each); a step that needs more grows the buffer
+    // (64 K pairs each); a step that needs more grows the buffer
     const uint64_t cand = static_cast<uint64_t>(S.max_queries) * 8 * k;
     if ((rc = gw_alloc(ctx, (void**)&S.d_topk, static_cast<uint64_t>(S.max_queries) * k * 8, "serve top-k")) ||
         (rc = gw_alloc(ctx, (void**)&S.d_topk_n, S.max_queries * 4ull, "serve top-k counts")) ||
