@@ -45,6 +45,7 @@ void gw_free(gw_ctx* ctx, void* p, uint64_t bytes) {
 }
 
 void gw_graph_free(gw_ctx* ctx) {
+    ++ctx->graph_epoch;
     gw_host_graph_free(ctx);
     gw_free(ctx, ctx->d_off, (ctx->n + 1ull) * 8);
     gw_free(ctx, ctx->d_col, ctx->m * 4);
@@ -65,7 +66,23 @@ void gw_graph_free(gw_ctx* ctx) {
 
 namespace {
 
+}  // namespace
+
+// The walk array no longer holds the walks of gw_walks_init_sources.
+void gw_gen_clear(gw_ctx* ctx) {
+    gw_free(ctx, ctx->d_gen_src, ctx->gen_nsrc * 4ull);
+    gw_free(ctx, ctx->d_gen_field, ctx->gen_nsrc * 4ull);
+    gw_free(ctx, ctx->d_gen_table, ctx->gen_nsrc * 32ull);
+    ctx->d_gen_src = ctx->d_gen_field = nullptr;
+    ctx->d_gen_table = nullptr;
+    ctx->gen_nsrc = ctx->gen_wps = 0;
+    ctx->gen_built[0] = ~0ull;
+}
+
+namespace {
+
 void free_walks(gw_ctx* ctx) {
+    gw_gen_clear(ctx);
     gw_free(ctx, ctx->d_walks, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_walks_alt, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_final, ctx->walk_cap * sizeof(gw_walk));
@@ -85,6 +102,7 @@ void free_walks(gw_ctx* ctx) {
 
 int reserve_walks(gw_ctx* ctx, uint64_t n) {
     gw_pool_free(ctx);  // device walks replace a host pool
+    gw_gen_clear(ctx);
     if (n <= ctx->walk_cap) {
         ctx->nwalks = n;
         return GW_OK;
@@ -323,6 +341,7 @@ int gw_graph_transpose(gw_ctx* ctx) {
     if (!ctx || !ctx->d_off) return gw_fail(ctx, GW_ESTATE, "gw_graph_transpose: needs a device-resident graph");
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ++ctx->graph_epoch;
     // the successor-header edges describe the old edges
     gw_free(ctx, ctx->d_ext, std::max<uint64_t>(1, ctx->m) * 16);
     ctx->d_ext = nullptr;
@@ -443,20 +462,28 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     GW_HIP(ctx, hipSetDevice(ctx->device));
     int rc = reserve_walks(ctx, nw);
     if (rc) return rc;
-    uint32_t *d_src = nullptr, *d_field = nullptr;
-    if ((rc = gw_alloc(ctx, (void**)&d_src, nsources * 4ull, "sources"))) return rc;
-    GW_HIP(ctx, hipMemcpy(d_src, sources, nsources * 4ull, hipMemcpyHostToDevice));
-    if (source_field) {
-        if ((rc = gw_alloc(ctx, (void**)&d_field, nsources * 4ull, "source field"))) { gw_free(ctx, d_src, nsources * 4ull); return rc; }
-        GW_HIP(ctx, hipMemcpy(d_field, source_field, nsources * 4ull, hipMemcpyHostToDevice));
+    // the sources stay on the device: walk starts are generated from them
+    // (the walk records are written too, for every other path)
+    ctx->gen_nsrc = nsources;
+    if ((rc = gw_alloc(ctx, (void**)&ctx->d_gen_src, nsources * 4ull, "sources")) ||
+        (source_field && (rc = gw_alloc(ctx, (void**)&ctx->d_gen_field, nsources * 4ull, "source field"))) ||
+        (rc = gw_alloc(ctx, &ctx->d_gen_table, nsources * 32ull, "source start table"))) {
+        gw_gen_clear(ctx);
+        return rc;
     }
+    GW_HIP(ctx, hipMemcpy(ctx->d_gen_src, sources, nsources * 4ull, hipMemcpyHostToDevice));
+    if (source_field)
+        GW_HIP(ctx, hipMemcpy(ctx->d_gen_field, source_field, nsources * 4ull, hipMemcpyHostToDevice));
     const uint32_t tb = 256;
-    init_sources_kernel<<<dim3(static_cast<uint32_t>((nw + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(ctx->d_walks, nw, d_src, d_field, walks_per_source);
+    init_sources_kernel<<<dim3(static_cast<uint32_t>((nw + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
+        ctx->d_walks, nw, ctx->d_gen_src, ctx->d_gen_field, walks_per_source);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    gw_free(ctx, d_src, nsources * 4ull);
-    gw_free(ctx, d_field, nsources * 4ull);
-    if (e != hipSuccess) return gw_hip_check(ctx, e, "init_sources_kernel");
+    if (e != hipSuccess) {
+        gw_gen_clear(ctx);
+        return gw_hip_check(ctx, e, "init_sources_kernel");
+    }
+    ctx->gen_wps = walks_per_source;
     ctx->walk_src_max = src_max;
     ctx->walk_v_max = v_max;
     ctx->field_walks.clear();

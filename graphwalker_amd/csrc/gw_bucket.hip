@@ -302,6 +302,7 @@ extern "C" int gw_bucket(gw_ctx* ctx, uint64_t* bucket_counts) {
     int rc = gw_bucket_sort(ctx, ctx->d_walks, ctx->nwalks, ctx->d_walks_alt, off);
     if (rc) return rc;
     std::swap(ctx->d_walks, ctx->d_walks_alt);
+    gw_gen_clear(ctx);  // the walks are in block order now
     const uint32_t nb = ctx->nblocks;
     const uint64_t kept = off[nb];
     for (uint32_t b = 0; b < nb; ++b) {

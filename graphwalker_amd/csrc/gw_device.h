@@ -331,9 +331,22 @@ __device__ __forceinline__ void region_evict(const KeyLog& k, const uint32_t* ke
     if (d && __lane_id() == 0) atomicAdd(&k.field_count[f], d);
 }
 
+// The state is wave-uniform by construction; readfirstlane tells the compiler
+// so, which keeps it in scalar registers and its branches scalar (otherwise a
+// divergent atomic's join turns the whole decision chain into exec-masked code).
+__device__ __forceinline__ uint32_t uniform(uint32_t x) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+
 __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* rb, RegionState& st, bool want,
                                                    uint32_t source, uint32_t v, uint32_t field_v) {
     const uint32_t lane = __lane_id();
+    st.f0 = uniform(st.f0);
+    st.c0 = uniform(st.c0);
+    st.f1 = uniform(st.f1);
+    st.c1 = uniform(st.c1);
+    st.d0 = uniform(st.d0);
+    st.d1 = uniform(st.d1);
     // visits of the field's designated vertex: counted in the buffer state
     // while their field holds a buffer, else one atomic per (wavefront, field)
     const bool fvis = want && v == field_v;
@@ -343,9 +356,10 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
         const uint32_t ls = lane_value(source, leader);
         const uint64_t grp = __ballot(fvis && source == ls);
         const uint32_t d = __popcll(grp);
-        if (st.f0 == ls) st.d0 += d;
-        else if (st.f1 == ls) st.d1 += d;
-        else if (lane == leader && ls < k.nf) atomicAdd(&k.field_count[ls], d);
+        const bool in0 = st.f0 == ls, in1 = st.f1 == ls;
+        st.d0 += in0 ? d : 0u;
+        st.d1 += in1 ? d : 0u;
+        if (!in0 && !in1 && ls < k.nf && lane == leader) atomicAdd(&k.field_count[ls], d);
         fdo &= ~grp;
     }
     const bool rest = want && !fvis;
@@ -359,34 +373,38 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
         const uint32_t g = __popcll(grp);
         if (ls >= k.nf) {
             if (lane == leader) atomicAdd(k.region_overflow, static_cast<unsigned long long>(g));
-            continue;
-        }
-        bool second;
-        if (st.f0 == ls) {
-            second = false;
-        } else if (st.f1 == ls) {
-            second = true;
-        } else {  // take an empty buffer, else evict the fuller one
-            second = st.c0 != 0 && (st.c1 == 0 || st.c1 > st.c0);
-            if (second) {
-                region_evict(k, rb->key[1], st.f1, st.c1, st.d1);
-                st.f1 = ls;
-                st.c1 = st.d1 = 0;
-            } else {
-                region_evict(k, rb->key[0], st.f0, st.c0, st.d0);
-                st.f0 = ls;
-                st.c0 = st.d0 = 0;
+        } else {
+            // the field's buffer: its own, else an empty one, else the fuller
+            // one evicted; the state changes by selects, not branches
+            const bool in0 = st.f0 == ls, in1 = st.f1 == ls, take = !in0 && !in1;
+            const bool second = in1 || (take && st.c0 != 0 && (st.c1 == 0 || st.c1 > st.c0));
+            if (take) {
+                if (second) region_evict(k, rb->key[1], st.f1, st.c1, st.d1);
+                else region_evict(k, rb->key[0], st.f0, st.c0, st.d0);
             }
+            const bool new0 = take && !second, new1 = take && second;
+            st.f0 = new0 ? ls : st.f0;
+            st.c0 = new0 ? 0u : st.c0;
+            st.d0 = new0 ? 0u : st.d0;
+            st.f1 = new1 ? ls : st.f1;
+            st.c1 = new1 ? 0u : st.c1;
+            st.d1 = new1 ? 0u : st.d1;
+            uint32_t* keys = rb->key[second ? 1 : 0];
+            uint32_t c = second ? st.c1 : st.c0;
+            const bool full = c + g > kRegionBuf;
+            if (full) region_flush_all(k, keys, ls, c);
+            c = full ? 0u : c;
+            if (in) keys[c + lanes_below(grp)] = v;
+            __builtin_amdgcn_wave_barrier();
+            st.c0 = second ? st.c0 : c + g;
+            st.c1 = second ? c + g : st.c1;
         }
-        uint32_t* keys = rb->key[second ? 1 : 0];
-        uint32_t c = second ? st.c1 : st.c0;
-        if (c + g > kRegionBuf) {
-            region_flush_all(k, keys, ls, c);
-            c = 0;
-        }
-        if (in) keys[c + lanes_below(grp)] = v;
-        __builtin_amdgcn_wave_barrier();
-        if (second) st.c1 = c + g; else st.c0 = c + g;
+        st.f0 = uniform(st.f0);
+        st.c0 = uniform(st.c0);
+        st.f1 = uniform(st.f1);
+        st.c1 = uniform(st.c1);
+        st.d0 = uniform(st.d0);
+        st.d1 = uniform(st.d1);
     }
 }
 
@@ -528,6 +546,22 @@ __device__ __forceinline__ int advance(const Prog& p, const Graph& g, uint32_t i
     if (capped && hop >= p.L) return kFinished;
     return kMoved;
 }
+
+// Where a generated walk starts (walks placed per source: walk i belongs to
+// source entry i / wps): its vertex, source field, the field's designated
+// vertex, and the vertex's row and degree.  32 bytes, read by a wavefront
+// with one scalar load per entry.
+struct alignas(32) SourceStart {
+    uint32_t vertex, field, fvertex, deg;
+    uint64_t row;
+    uint64_t pad;
+};
+
+struct WalkGen {
+    const SourceStart* __restrict__ tab;  // null: the kernel reads walk records
+    uint32_t wps;                          // walks per source entry
+    uint32_t nsrc;                         // entries
+};
 
 // Walk state carried in registers.  With the successor-header layout the
 // current vertex's row start and degree ride along, fetched with the edge

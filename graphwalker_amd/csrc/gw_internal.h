@@ -73,6 +73,7 @@ struct gw_ctx {
     // graph (resident in HBM)
     uint32_t n = 0;
     uint64_t m = 0;
+    uint64_t graph_epoch = 0;  // bumped whenever the resident graph is freed or rewritten
     uint64_t* d_off = nullptr;
     uint32_t* d_col = nullptr;
     uint32_t nblocks = 0;
@@ -121,6 +122,16 @@ struct gw_ctx {
     gw_walk* d_final = nullptr;      // final record of each walk, indexed by walk id
     bool finals_valid = true;        // false after a GW_REC_NO_FINAL run
     gw_walk* d_work = nullptr;       // blocked schedule: second bucket buffer
+    // Walks placed by gw_walks_init_sources are a function of their index
+    // (walk i: id i, source entry i / gen_wps, hop 0), so the persistent
+    // kernel can start them from one entry per source (SourceStart, built per
+    // run into d_gen_table) instead of a record per walk and a row read per
+    // start.  gen_wps = 0: the walk array holds other walks.
+    uint32_t* d_gen_src = nullptr;    // source vertices [gen_nsrc]
+    uint32_t* d_gen_field = nullptr;  // source fields [gen_nsrc], null: the vertex is the field
+    void* d_gen_table = nullptr;      // SourceStart[gen_nsrc]
+    uint32_t gen_nsrc = 0, gen_wps = 0;
+    uint64_t gen_built[3] = {~0ull, 0, 0};  // {graph_epoch, field-vertex array, fields} the table was built for
 
     // recorder outputs
     uint32_t* d_counts = nullptr;   // n
@@ -230,6 +241,7 @@ int gw_transpose_build(gw_ctx* ctx);
 // and the pinned host outputs a run with recorder flags `rec` needs (the
 // event log of event_cap entries per walk), and the copy stream and events.
 void gw_pool_free(gw_ctx* ctx);
+void gw_gen_clear(gw_ctx* ctx);  // the walks are no longer those of gw_walks_init_sources
 int gw_pool_prepare(gw_ctx* ctx, uint32_t rec, uint32_t event_cap);
 
 // gw_step.hip

@@ -82,19 +82,28 @@ __global__ __launch_bounds__(256) void step_plain_kernel(Prog p, Graph g, const 
     }
 }
 
-template <int KIND, int REC, bool EXT>
+// GEN: the walks are those of gw_walks_init_sources, started from gen.tab
+// (walk i: id i, hop 0, source entry i / gen.wps) instead of walk records.  A
+// wavefront keeps the entries of two consecutive sources in scalar registers
+// -- its walk indices [seg_lo, seg_hi), split at seg_mid -- so a walk start
+// waits on no load: a walk-record read and the row read of its vertex would
+// put two dependent memory latencies in front of the first step, and batched
+// multi-source walks end every two or three steps.
+template <int KIND, int REC, bool EXT, bool GEN = false>
 __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, const gw_walk* __restrict__ walks,
                                                              gw_walk* __restrict__ final_walks, uint64_t nw,
                                                              uint32_t* __restrict__ counts,
                                                              uint64_t* __restrict__ hash,
                                                              uint32_t* __restrict__ visits,
                                                              unsigned long long* __restrict__ counters, EventLog log,
-                                                        SourceTable table, KeyLog klog) {
+                                                        SourceTable table, KeyLog klog, WalkGen gen = WalkGen{}) {
     const uint32_t lane = lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     bool exhausted = false;                  // wave-uniform
     bool active = false;
     State s{};
+    uint32_t seg_lo = 0, seg_mid = 0, seg_hi = 0;  // wave-uniform (GEN)
+    SourceStart e0{}, e1{};                         // wave-uniform (GEN)
     uint32_t tot_steps = 0, tot_visits = 0;
     constexpr bool SUM = (REC & GW_REC_WALK_SUMMARY) != 0;
     constexpr bool KL = (REC & kRecKeyLog) != 0;
@@ -124,10 +133,45 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 chunk_next = base;
                 chunk_end = std::min<uint64_t>(base + p.chunk, nw);
             }
+            if (GEN && (chunk_next < seg_lo || chunk_next >= seg_hi)) {
+                // the entries of the source holding chunk_next and the next one
+                const uint32_t f = static_cast<uint32_t>(chunk_next) / gen.wps;
+                seg_lo = f * gen.wps;
+                seg_mid = seg_lo + gen.wps;
+                seg_hi = f + 1 < gen.nsrc ? seg_mid + gen.wps : seg_mid;
+                e0 = gen.tab[f];
+                e1 = gen.tab[f + 1 < gen.nsrc ? f + 1 : f];
+            }
             const uint32_t want = __popcll(idle);
-            const uint64_t avail = chunk_end - chunk_next;
+            uint64_t avail = chunk_end - chunk_next;
+            if (GEN) avail = std::min<uint64_t>(avail, seg_hi - chunk_next);
             const uint32_t take = avail < want ? static_cast<uint32_t>(avail) : want;
             const uint32_t r = rank_in(idle);
+            if (GEN) {
+                if (!active && r < take) {
+                    const uint32_t i = static_cast<uint32_t>(chunk_next) + r;
+                    const bool hi = i >= seg_mid;
+                    s.id = i;
+                    s.source = hi ? e1.field : e0.field;
+                    s.v = hi ? e1.vertex : e0.vertex;
+                    s.hop = 0;
+                    if (EXT) {
+                        s.row = hi ? e1.row : e0.row;
+                        s.deg = hi ? e1.deg : e0.deg;
+                    }
+                    rec.id = i - p.id_base;
+                    rec.source = s.source;
+                    if (RG) rec.field_v = hi ? e1.fvertex : e0.fvertex;
+                    start_walk(p, s);
+                    active = true;
+                    rec.visits = 0;
+                    rec.steps = 0;
+                    rec.hash = kHashInit;
+                }
+                chunk_next += take;
+                idle = __ballot(!active);
+                continue;
+            }
             // A walk whose vertex (or restart target) is not in the graph is
             // skipped and counted: gw_run checks walks on the host, but the
             // walks of gw_run_noalloc are device data it never reads.
@@ -324,6 +368,17 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
     }
 }
 
+__global__ void source_start_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ field,
+                                    uint32_t nsrc, const uint64_t* __restrict__ off,
+                                    const uint32_t* __restrict__ fvert, uint32_t nf, SourceStart* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsrc) return;
+    const uint32_t v = src[k];
+    const uint32_t f = field ? field[k] : v;
+    const uint64_t b = off[v];
+    out[k] = SourceStart{v, f, f < nf ? fvert[f] : 0xFFFFFFFFu, static_cast<uint32_t>(off[v + 1] - b), b, 0};
+}
+
 __global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t nw, uint32_t id_base,
                                     uint64_t* __restrict__ hash, uint32_t* __restrict__ visits) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
@@ -361,6 +416,7 @@ struct Launch {
     uint64_t bucket_n;
     const BlockView* views;  // non-null: host-resident graph, step through the block cache
     bool no_final;           // GW_REC_NO_FINAL: the kernels get no final-record array
+    WalkGen gen;             // tab non-null: persistent walk starts from the source table
 };
 
 template <int KIND, int REC, bool EXT>
@@ -371,6 +427,16 @@ hipError_t launch4(const Launch& L) {
         step_plain_kernel<KIND, REC><<<dim3(grid), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_walks, L.no_final ? nullptr : c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else if (L.sched == GW_SCHED_PERSISTENT) {
+        // generated walk starts: instantiated for the batched multi-source
+        // recorder (direct regions) on the successor-header layout only
+        if constexpr (REC == kRecRegion && EXT) {
+            if (L.gen.tab) {
+                step_persistent_kernel<KIND, REC, EXT, true><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
+                    L.p, L.g, c->d_walks, L.no_final ? nullptr : c->d_final, c->nwalks, c->d_counts, c->d_hash,
+                    c->d_visits, c->d_counters, L.log, L.table, L.klog, L.gen);
+                return hipGetLastError();
+            }
+        }
         step_persistent_kernel<KIND, REC, EXT><<<dim3(L.grid_persistent), dim3(L.tb), 0, c->stream>>>(
             L.p, L.g, c->d_walks, L.no_final ? nullptr : c->d_final, c->nwalks, c->d_counts, c->d_hash, c->d_visits, c->d_counters, L.log, L.table, L.klog);
     } else if (L.views) {
@@ -933,6 +999,10 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
             L.klog.field_count = ctx->d_part_fcount;
         }
     }
+    // walks of gw_walks_init_sources start from the per-source table
+    L.gen = WalkGen{};
+    if (sched == GW_SCHED_PERSISTENT && ext && rec == kRecRegion && ctx->gen_wps && !ctx->pool && !ctx->host_graph)
+        L.gen = WalkGen{static_cast<const SourceStart*>(ctx->d_gen_table), ctx->gen_wps, ctx->gen_nsrc};
     rp.sched = sched;
     rp.rec = rec;
     return GW_OK;
@@ -954,6 +1024,15 @@ int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed, bool clear = true) {
         const uint64_t words = ctx->table_bytes / 8;
         fill_u64_kernel<<<dim3(gw_grid(words, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_table, words);
         GW_HIP(ctx, hipGetLastError());
+    }
+    // the sources' start entries, built when the graph or the fields changed
+    const uint64_t built[3] = {ctx->graph_epoch, reinterpret_cast<uintptr_t>(rp.L.klog.field_vertex), rp.L.klog.nf};
+    if (rp.L.gen.tab && !std::equal(built, built + 3, ctx->gen_built)) {
+        source_start_kernel<<<dim3((ctx->gen_nsrc + 255) / 256), dim3(256), 0, ctx->stream>>>(
+            ctx->d_gen_src, ctx->d_gen_field, ctx->gen_nsrc, ctx->d_off, rp.L.klog.field_vertex, rp.L.klog.nf,
+            const_cast<SourceStart*>(rp.L.gen.tab));
+        GW_HIP(ctx, hipGetLastError());
+        std::copy(built, built + 3, ctx->gen_built);
     }
     if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (ctx->nwalks) {
@@ -1017,7 +1096,8 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), p(ctx->d_part_fvert),
             p(ctx->d_part_fcount), ctx->part_fields,
             ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
-            rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final};
+            rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final,
+            p(rp.L.gen.tab), rp.L.gen.wps, rp.L.gen.nsrc, p(ctx->d_gen_src), p(ctx->d_gen_field), ctx->graph_epoch};
 }
 
 // Host walk pool: the run steps the pool window by window through two device
