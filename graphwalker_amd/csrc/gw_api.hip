@@ -68,21 +68,27 @@ namespace {
 
 }  // namespace
 
-// The walk array no longer holds the walks of gw_walks_init_sources.
+// The walk array no longer holds the walks of gw_walks_init_sources (the
+// source arrays stay allocated for the next placement).
 void gw_gen_clear(gw_ctx* ctx) {
-    gw_free(ctx, ctx->d_gen_src, ctx->gen_nsrc * 4ull);
-    gw_free(ctx, ctx->d_gen_field, ctx->gen_nsrc * 4ull);
-    gw_free(ctx, ctx->d_gen_table, ctx->gen_nsrc * 32ull);
-    ctx->d_gen_src = ctx->d_gen_field = nullptr;
-    ctx->d_gen_table = nullptr;
     ctx->gen_nsrc = ctx->gen_wps = 0;
     ctx->gen_built[0] = ~0ull;
+}
+
+void gw_gen_free(gw_ctx* ctx) {
+    gw_gen_clear(ctx);
+    gw_free(ctx, ctx->d_gen_src, ctx->gen_cap * 4ull);
+    gw_free(ctx, ctx->d_gen_field, ctx->gen_cap * 4ull);
+    gw_free(ctx, ctx->d_gen_table, ctx->gen_cap * sizeof(gw::SourceStart));
+    ctx->d_gen_src = ctx->d_gen_field = nullptr;
+    ctx->d_gen_table = nullptr;
+    ctx->gen_cap = 0;
 }
 
 namespace {
 
 void free_walks(gw_ctx* ctx) {
-    gw_gen_clear(ctx);
+    gw_gen_free(ctx);
     gw_free(ctx, ctx->d_walks, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_walks_alt, ctx->walk_cap * sizeof(gw_walk));
     gw_free(ctx, ctx->d_final, ctx->walk_cap * sizeof(gw_walk));
@@ -464,16 +470,21 @@ int gw_walks_init_sources(gw_ctx* ctx, const uint32_t* sources, const uint32_t* 
     if (rc) return rc;
     // the sources stay on the device: walk starts are generated from them
     // (the walk records are written too, for every other path)
-    ctx->gen_nsrc = nsources;
-    if ((rc = gw_alloc(ctx, (void**)&ctx->d_gen_src, nsources * 4ull, "sources")) ||
-        (source_field && (rc = gw_alloc(ctx, (void**)&ctx->d_gen_field, nsources * 4ull, "source field"))) ||
-        (rc = gw_alloc(ctx, &ctx->d_gen_table, nsources * 32ull, "source start table"))) {
-        gw_gen_clear(ctx);
-        return rc;
+    // (kept allocated across placements, so a captured run can replay new sources)
+    if (nsources > ctx->gen_cap) {
+        gw_gen_free(ctx);
+        if ((rc = gw_alloc(ctx, (void**)&ctx->d_gen_src, nsources * 4ull, "sources")) ||
+            (rc = gw_alloc(ctx, (void**)&ctx->d_gen_field, nsources * 4ull, "source fields")) ||
+            (rc = gw_alloc(ctx, &ctx->d_gen_table, nsources * sizeof(gw::SourceStart), "source start table"))) {
+            gw_gen_free(ctx);
+            return rc;
+        }
+        ctx->gen_cap = nsources;
     }
+    ctx->gen_nsrc = nsources;
     GW_HIP(ctx, hipMemcpy(ctx->d_gen_src, sources, nsources * 4ull, hipMemcpyHostToDevice));
-    if (source_field)
-        GW_HIP(ctx, hipMemcpy(ctx->d_gen_field, source_field, nsources * 4ull, hipMemcpyHostToDevice));
+    GW_HIP(ctx, hipMemcpy(ctx->d_gen_field, source_field ? source_field : sources, nsources * 4ull,
+                          hipMemcpyHostToDevice));
     const uint32_t tb = 256;
     init_sources_kernel<<<dim3(static_cast<uint32_t>((nw + tb - 1) / tb)), dim3(tb), 0, ctx->stream>>>(
         ctx->d_walks, nw, ctx->d_gen_src, ctx->d_gen_field, walks_per_source);

@@ -157,8 +157,10 @@ struct KeyLog {
     uint32_t vbits;
     uint32_t slab;
     // Direct regions (no log): visits of source field f go straight to
-    // region_out[region_off[f] + ...), claimed through region_n[f].
+    // region_out[region_off[f] + ...), claimed through region_n[f], up to
+    // region_end[f] (region_off + 1 where the regions are back to back).
     const uint64_t* __restrict__ region_off;
+    const uint64_t* __restrict__ region_end;
     unsigned long long* __restrict__ region_n;
     uint32_t* __restrict__ region_out;
     unsigned long long* __restrict__ region_overflow;
@@ -233,7 +235,7 @@ __device__ __forceinline__ void region_flush(const KeyLog& k, const uint32_t* ke
     unsigned long long at = 0;
     if (lane == leader) at = atomicAdd(&k.region_n[f], static_cast<unsigned long long>(n));
     at = lane_value(at, leader);
-    const uint64_t base = k.region_off[f] + at, hi = k.region_off[f + 1];
+    const uint64_t base = k.region_off[f] + at, hi = k.region_end[f];
     const uint32_t m = __popcll(active);
     for (uint32_t i = lanes_below(active); i < n; i += m)
         if (base + i < hi) k.region_out[base + i] = keys[i];
@@ -315,7 +317,7 @@ __device__ __forceinline__ void region_flush_all(const KeyLog& k, const uint32_t
     unsigned long long at = 0;
     if (lane == 0) at = atomicAdd(&k.region_n[f], static_cast<unsigned long long>(n));
     at = lane_value(at, 0u);
-    const uint64_t base = k.region_off[f] + at, hi = k.region_off[f + 1];
+    const uint64_t base = k.region_off[f] + at, hi = k.region_end[f];
     for (uint32_t i = lane; i < n; i += 64)
         if (base + i < hi) k.region_out[base + i] = keys[i];
     if (lane == 0 && base + n > hi)
@@ -549,13 +551,17 @@ __device__ __forceinline__ int advance(const Prog& p, const Graph& g, uint32_t i
 
 // Where a generated walk starts (walks placed per source: walk i belongs to
 // source entry i / wps): its vertex, source field, the field's designated
-// vertex, and the vertex's row and degree.  32 bytes, read by a wavefront
-// with one scalar load per entry.
-struct alignas(32) SourceStart {
+// vertex, the vertex's row and degree, and the row and degree of the field as
+// a vertex (the restart target of the restart programs; zero when the field
+// is not a vertex).  48 bytes, read by a wavefront with scalar loads.
+struct alignas(16) SourceStart {
     uint32_t vertex, field, fvertex, deg;
     uint64_t row;
-    uint64_t pad;
+    uint64_t srow;
+    uint32_t sdeg, pad0;
+    uint64_t pad1;
 };
+static_assert(sizeof(SourceStart) == 48, "SourceStart is 48 bytes");
 
 struct WalkGen {
     const SourceStart* __restrict__ tab;  // null: the kernel reads walk records
@@ -571,6 +577,12 @@ struct State {
     uint64_t row;
     uint32_t deg;
     u32x4 r;  // the Philox draw of (id, hop): set by start_walk, advanced by step_once
+    // restart programs on the successor-header layout: the restart target's
+    // row and degree, fetched once per walk (set_source_row) -- a restart
+    // then needs no load, and in a wavefront of restart walks some lane
+    // restarts on nearly every hop
+    uint64_t srow;
+    uint32_t sdeg;
 };
 
 // A walk picked up by a kernel: its draw for the current hop.
@@ -584,6 +596,20 @@ __device__ __forceinline__ void load_row(const Graph& g, State& s) {
         const uint64_t b = g.off[s.v];
         s.row = b;
         s.deg = static_cast<uint32_t>(g.off[s.v + 1] - b);
+    }
+}
+
+template <int KIND>
+constexpr bool restarts() { return KIND == GW_RESTART || KIND == GW_RESTART_JOINT; }
+
+// At a walk's start: the restart target's row and degree (restart programs
+// on the successor-header layout; s.source must be a vertex of g).
+template <int KIND, bool EXT>
+__device__ __forceinline__ void set_source_row(const Graph& g, State& s) {
+    if (EXT && restarts<KIND>()) {
+        const uint64_t b = g.off[s.source];
+        s.srow = b;
+        s.sdeg = static_cast<uint32_t>(g.off[s.source + 1] - b);
     }
 }
 
@@ -656,7 +682,8 @@ __device__ __forceinline__ bool step_once(const Prog& p, const Graph& g, State& 
             load_row<EXT>(g, s);
         } else {
             s.v = s.source;
-            load_row<EXT>(g, s);
+            s.row = s.srow;
+            s.deg = s.sdeg;
         }
     }
     ++s.hop;

@@ -127,10 +127,10 @@ struct gw_ctx {
     // kernel can start them from one entry per source (SourceStart, built per
     // run into d_gen_table) instead of a record per walk and a row read per
     // start.  gen_wps = 0: the walk array holds other walks.
-    uint32_t* d_gen_src = nullptr;    // source vertices [gen_nsrc]
-    uint32_t* d_gen_field = nullptr;  // source fields [gen_nsrc], null: the vertex is the field
-    void* d_gen_table = nullptr;      // SourceStart[gen_nsrc]
-    uint32_t gen_nsrc = 0, gen_wps = 0;
+    uint32_t* d_gen_src = nullptr;    // source vertices [gen_cap]
+    uint32_t* d_gen_field = nullptr;  // source fields [gen_cap] (the vertices when none are given)
+    void* d_gen_table = nullptr;      // SourceStart[gen_cap]
+    uint32_t gen_cap = 0, gen_nsrc = 0, gen_wps = 0;
     uint64_t gen_built[3] = {~0ull, 0, 0};  // {graph_epoch, field-vertex array, fields} the table was built for
 
     // recorder outputs
@@ -242,6 +242,7 @@ int gw_transpose_build(gw_ctx* ctx);
 // event log of event_cap entries per walk), and the copy stream and events.
 void gw_pool_free(gw_ctx* ctx);
 void gw_gen_clear(gw_ctx* ctx);  // the walks are no longer those of gw_walks_init_sources
+void gw_gen_free(gw_ctx* ctx);   // and the source arrays are released
 int gw_pool_prepare(gw_ctx* ctx, uint32_t rec, uint32_t event_cap);
 
 // gw_step.hip

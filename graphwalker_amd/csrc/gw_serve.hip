@@ -40,6 +40,9 @@ struct SlotInfo {
     uint32_t source;   // source vertex: start and restart target
     uint32_t qid_lo;   // Philox counter word 3
     uint32_t qid_hi;   // mixed into Philox key word 1
+    uint64_t srow;     // the source's row and degree, written at admission:
+    uint32_t sdeg;     // a restart needs no load
+    uint32_t pad;
 };
 
 struct Admit;
@@ -65,7 +68,24 @@ struct ServeState {
     Admit* d_admit = nullptr;               // max_queries admission records
     Done* d_done = nullptr;                 // max_queries completion records
     unsigned long long* d_region_n = nullptr;  // max_queries: entries of each completed query
-    struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size; uint32_t source; };
+    // per query slot: its key region in the arena (u32 index range) and cursor
+    uint64_t* d_koff = nullptr;
+    uint64_t* d_kend = nullptr;
+    unsigned long long* d_kcur = nullptr;
+    // arena range of an active query: [base, base + size) its hash region,
+    // then `total - size` slots holding its key region (walks x L u32 keys)
+    struct Active { uint64_t qid; uint32_t slot, walks; uint64_t base, size, total; uint32_t source; };
+    // pinned host staging of the per-step copies (a copy from pageable memory
+    // is staged by the runtime before the call returns)
+    void* h_block = nullptr;
+    Admit* h_adm = nullptr;
+    Done* h_done = nullptr;
+    unsigned long long* h_cnt = nullptr;   // 8 counters
+    uint32_t* h_rem = nullptr;             // max_queries
+    unsigned long long* h_kcur = nullptr;  // max_queries
+    unsigned long long* h_coff = nullptr;  // max_queries
+    uint32_t* h_tn = nullptr;              // max_queries
+    unsigned long long* h_top = nullptr;   // max_queries x topk (gw_serve_topk)
     std::vector<Active> active;
     std::vector<uint32_t> free_slots;
     std::vector<std::pair<uint64_t, uint64_t>> free_ranges;  // arena [start, size), sorted by start
@@ -149,219 +169,88 @@ __device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Per-wavefront LDS cache of (query slot, vertex) -> visits.  A restart
-// query's visits concentrate on a few hundred vertices near its source; the
-// cache absorbs their repeats so the arena sees one update per (window,
-// vertex) instead of one per visit.  Flushed to the arena when three
-// quarters full and at kernel end.
-constexpr uint32_t kServeCache = 512;  // 24 KiB per 256-thread workgroup: 6 workgroups per CU (256 and 1024 measured slower)
-constexpr uint32_t kServeCacheFlush = kServeCache * 3 / 4;
-constexpr uint32_t kServeCacheProbes = 4;
-constexpr unsigned long long kCacheEmpty = ~0ull;
-// Marks a cache key that counts visits to the query's own source; slots stay
-// below 2^16, so a marked key is never kCacheEmpty.
-constexpr unsigned long long kCacheSource = 1ull << 63;
-struct ServeCache {
-    unsigned long long key[kServeCache];  // [kCacheSource |] (slot << 32) | vertex
-    uint32_t cnt[kServeCache];
-    uint32_t fill;
-};
-
-// One lane per wavefront, at kernel start.
-__device__ __forceinline__ void serve_cache_init(ServeCache* c, uint32_t lane) {
-    for (uint32_t i = lane; i < kServeCache; i += 64) {
-        c->key[i] = kCacheEmpty;
-        c->cnt[i] = 0;
+// Adds `inc` visits of vertex v to a query's hash region (arena slots [rb,
+// rb + rm]) for the lanes with `want`; every active lane calls it, and the
+// lanes with `want` belong to query slot `sl`.  One compare-and-swap either
+// claims an empty slot or returns the entry there (then an atomic add if it
+// is this vertex's).  A claimed slot is appended to the query's occupied list
+// (one atomic per wavefront), so completion reads the entries without
+// scanning the region.
+__device__ __forceinline__ void arena_add(unsigned long long* __restrict__ arena, uint32_t* __restrict__ occ,
+                                          uint32_t* __restrict__ occn, uint32_t cbits,
+                                          unsigned long long* __restrict__ overflow, bool want, uint32_t sl,
+                                          uint64_t rb, uint32_t rm, uint32_t v, uint32_t inc) {
+    constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+    uint32_t claimed = kNoSlot;
+    if (want) {
+        const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | inc;
+        uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm;
+        bool placed = false;
+        for (uint32_t probe = 0; probe <= rm; ++probe) {
+            unsigned long long* cell = arena + rb + h;
+            // compare-and-swap first: one request for a new vertex, two for a known one
+            const unsigned long long cur = atomicCAS(cell, 0ull, entry);
+            if (cur == 0ull) {
+                claimed = h;
+                placed = true;
+                break;
+            }
+            if ((cur >> cbits) == v) {
+                atomicAdd(cell, static_cast<unsigned long long>(inc));
+                placed = true;
+                break;
+            }
+            h = (h + 1) & rm;
+        }
+        if (!placed) atomicAdd(overflow, static_cast<unsigned long long>(inc));
     }
-    if (lane == 0) c->fill = 0;
+    const uint64_t cm = __ballot(claimed != kNoSlot);
+    if (cm) {
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(cm)) - 1;
+        uint32_t b0 = 0;
+        if (__lane_id() == leader) b0 = atomicAdd(&occn[sl], static_cast<uint32_t>(__popcll(cm)));
+        b0 = lane_value(b0, leader);
+        if (claimed != kNoSlot) occ[rb + b0 + rank_of(cm)] = claimed;
+    }
 }
 
-// Per-visit recorder of a served walk.  Lanes of one query on one vertex
-// combine, then visits go through the wavefront's LDS cache.  A flush sends
-// the cached visits to the query's own source (a quarter or more of a restart
-// walk's visits) to a per-query counter, and the others to the query's hash
-// region: one compare-and-swap either claims an empty slot or returns the
-// entry there (then an atomic add if it is this vertex's).  A claimed slot is
-// appended to the query's occupied list (one atomic per (wavefront, query)),
-// so completion reads the entries without scanning the region.  Counting the
-// source through the cache matters: every wavefront of a query would
-// otherwise update the same counter on nearly every hop, and device-scope
-// atomics on one address serialize.
-struct ServeRec {
-    unsigned long long* arena;
-    uint32_t* srccount;
-    uint32_t* occ;
-    uint32_t* occn;
-    uint64_t base;
-    uint32_t mask;
-    uint32_t cbits;
-    unsigned long long* overflow;
-    uint32_t slot, source;
-    const SlotInfo* slots;
-    ServeCache* cache;
+// Per-visit recorder of a served walk: the step's visit is held, then staged
+// by region_add_uniform after the step with the whole wavefront present
+// (the batched path's direct regions, one field per query slot): a query's
+// visits go to its key region in the arena through per-wavefront LDS buffers
+// -- no atomic per visit -- and visits to the query's own source are only
+// counted.  A completing query's keys are folded into its hash region by
+// serve_fold_kernel.  Aggregating in the step instead (a per-wavefront LDS
+// hash in front of the arena) put an LDS round trip and the combining of
+// equal keys into every hop of every walk.
+struct ServeVisits {
     uint32_t visits = 0, steps = 0;
-
-    // Adds `inc` visits of vertex v to query slot `sl` (region at `rb`, `rm`
-    // slots - 1) for the lanes with `want`; every active lane calls it.
-    __device__ __forceinline__ void arena_add(bool want, uint32_t sl, uint64_t rb, uint32_t rm, uint32_t v,
-                                              uint32_t inc) {
-        arena_add_from(want, sl, rb, rm, v, inc,
-                       static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm);
-    }
-
-    // The same, probing from slot h0 of the region.
-    __device__ __forceinline__ void arena_add_from(bool want, uint32_t sl, uint64_t rb, uint32_t rm, uint32_t v,
-                                                   uint32_t inc, uint32_t h0) {
-        constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-        uint32_t claimed = kNoSlot;
-        if (want) {
-            const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | inc;
-            uint32_t h = h0;
-            bool placed = false;
-            for (uint32_t probe = 0; probe <= rm; ++probe) {
-                unsigned long long* cell = arena + rb + h;
-                // compare-and-swap first: one request for a new vertex, two for a known one
-                const unsigned long long cur = atomicCAS(cell, 0ull, entry);
-                if (cur == 0ull) {
-                    claimed = h;
-                    placed = true;
-                    break;
-                }
-                if ((cur >> cbits) == v) {
-                    atomicAdd(cell, static_cast<unsigned long long>(inc));
-                    placed = true;
-                    break;
-                }
-                h = (h + 1) & rm;
-            }
-            if (!placed) atomicAdd(overflow, static_cast<unsigned long long>(inc));
-        }
-        const bool cl = claimed != kNoSlot;
-        uint64_t pend = __ballot(cl);
-        while (pend) {
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(pend)) - 1;
-            const uint32_t ls = lane_value(sl, leader);
-            const uint64_t grp = __ballot(cl && sl == ls);
-            uint32_t b0 = 0;
-            if (__lane_id() == leader) b0 = atomicAdd(&occn[ls], static_cast<uint32_t>(__popcll(grp)));
-            b0 = lane_value(b0, leader);
-            if (cl && sl == ls) occ[rb + b0 + rank_of(grp)] = claimed;
-            pend &= ~grp;
-        }
-    }
-
-    // Moves every cached entry to the arena; the lanes in `active` take part.
-    __device__ __forceinline__ void flush(uint64_t active) {
-        const uint32_t m = __popcll(active), r = rank_of(active);
-        for (uint32_t i0 = 0; i0 < kServeCache; i0 += m) {  // wave-uniform
-            const uint32_t i = i0 + r;
-            bool have = false;
-            uint32_t es = 0, ev = 0, ec = 0;
-            if (i < kServeCache) {
-                const unsigned long long k = cache->key[i];
-                if (k != kCacheEmpty) {
-                    es = static_cast<uint32_t>(k >> 32) & 0xFFFFu;
-                    ev = static_cast<uint32_t>(k);
-                    ec = cache->cnt[i];
-                    cache->key[i] = kCacheEmpty;
-                    cache->cnt[i] = 0;
-                    if (k & kCacheSource) atomicAdd(&srccount[es], ec);
-                    else have = true;
-                }
-            }
-            uint64_t rb = 0;
-            uint32_t rm = 0;
-            if (have) {
-                const SlotInfo si = slots[es];
-                rb = si.base;
-                rm = si.mask;
-            }
-            arena_add(have, es, rb, rm, ev, ec);
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(active)) - 1)) cache->fill = 0;
-        __builtin_amdgcn_wave_barrier();
-    }
-
+    uint32_t pend_v = 0;
+    bool pend = false;
     __device__ __forceinline__ void visit(uint32_t v, uint32_t) {
         ++visits;
-        const bool at_src = v == source;
-        // Lanes of one query standing on one vertex combine into one update:
-        // first those at their query's source (one pass per query there),
-        // then up to three groups of the others; that loop stops at the
-        // first group of one lane (the rest are likely distinct too).
-        bool mine = true;
-        uint32_t inc = 1;
-        uint64_t todo = __ballot(at_src);
-        while (todo) {  // wave-uniform
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-            const uint32_t ls = lane_value(slot, leader);
-            const bool in = at_src && slot == ls;
-            const uint64_t grp = __ballot(in);
-            if (in) {
-                if (__lane_id() == leader) inc = static_cast<uint32_t>(__popcll(grp));
-                else mine = false;
-            }
-            todo &= ~grp;
-        }
-        uint64_t fold = __ballot(!at_src);
-        for (uint32_t it = 0; it < 3 && fold; ++it) {
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(fold)) - 1;
-            const uint32_t v0 = lane_value(v, leader);
-            const uint32_t s0 = lane_value(slot, leader);
-            const bool in = !at_src && v == v0 && slot == s0;
-            const uint64_t grp = __ballot(in);
-            if (in) {
-                if (__lane_id() == leader) inc = static_cast<uint32_t>(__popcll(grp));
-                else mine = false;
-            }
-            fold &= ~grp;
-            if (__popcll(grp) == 1) break;
-        }
-        // the wavefront's cache first; a lane that finds no room in a few
-        // probes goes to the arena directly
-        bool fresh = false;
-        if (mine) {
-            const unsigned long long key = (at_src ? kCacheSource : 0ull) | (static_cast<unsigned long long>(slot) << 32) | v;
-            uint32_t h = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> 56) & (kServeCache - 1);
-            for (uint32_t p = 0; p < kServeCacheProbes; ++p) {
-                const unsigned long long old = atomicCAS(&cache->key[h], kCacheEmpty, key);
-                if (old == kCacheEmpty || old == key) {
-                    atomicAdd(&cache->cnt[h], inc);
-                    fresh = old == kCacheEmpty;
-                    mine = false;
-                    break;
-                }
-                h = (h + 1) & (kServeCache - 1);
-            }
-        }
-        {  // entries claimed in the cache: one LDS atomic per wavefront
-            const uint64_t fm = __ballot(fresh);
-            if (fm && __lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(fm)) - 1))
-                atomicAdd(&cache->fill, static_cast<uint32_t>(__popcll(fm)));
-        }
-        if (mine && at_src) {  // no room in the cache
-            atomicAdd(&srccount[slot], inc);
-            mine = false;
-        }
-        arena_add(mine, slot, base, mask, v, inc);
-        const uint64_t active = __ballot(1);
-        if (cache->fill > kServeCacheFlush) flush(active);
+        pend_v = v;
+        pend = true;
     }
 };
 
 // One serve step: every walk of pool_in advances at most `hops` hops; walks
 // still alive are appended to pool_out; ended walks decrement their query's
 // remaining count.  Persistent wavefronts take chunks of the pool.
+// Query slot q's visits (other than to its source) go to its key region
+// keys[koff[q], kend[q]) through the cursor kcur[q]; visits to its source
+// count in srccount[q].
 template <int KIND, bool EXT, bool PAGED>
 __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedCols pc, const gw_walk* __restrict__ pool_in,
                                                         uint64_t n, gw_walk* __restrict__ pool_out,
                                                         const SlotInfo* __restrict__ slots,
                                                         uint32_t* __restrict__ remaining,
                                                         uint32_t* __restrict__ srccount,
-                                                        unsigned long long* __restrict__ arena,
-                                                        uint32_t* __restrict__ occ, uint32_t* __restrict__ occn,
-                                                        uint32_t cbits, uint32_t hops,
+                                                        uint32_t* __restrict__ keys,
+                                                        const uint64_t* __restrict__ koff,
+                                                        const uint64_t* __restrict__ kend,
+                                                        unsigned long long* __restrict__ kcur, uint32_t nslots,
+                                                        uint32_t hops,
                                                         unsigned long long* __restrict__ cnt,
                                                         unsigned long long* __restrict__ cursor, uint32_t chunk) {
     const uint32_t lane = __lane_id();
@@ -369,10 +258,18 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
     bool exhausted = false, active = false;
     uint32_t slot = 0, budget = 0, tot_steps = 0, tot_visits = 0;
     Prog q = p;
-    __shared__ ServeCache s_cache[4];
-    ServeCache* cache = &s_cache[threadIdx.x >> 6];
-    serve_cache_init(cache, lane);
-    ServeRec rec{arena, srccount, occ, occn, 0, 0, cbits, cnt + 3, 0, 0, slots, cache};
+    KeyLog kl{};
+    kl.region_off = koff;
+    kl.region_end = kend;
+    kl.region_n = kcur;
+    kl.region_out = keys;
+    kl.region_overflow = cnt + 3;
+    kl.nf = nslots;
+    kl.field_count = srccount;
+    __shared__ RegionBuf s_rbuf[4];
+    RegionBuf* rbuf = &s_rbuf[threadIdx.x >> 6];
+    RegionState rst;
+    ServeVisits rec;
     State s{};
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -398,13 +295,11 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
                 s.v = w.vertex;
                 s.hop = w.hop;
                 load_row<EXT>(g, s);
+                s.srow = si.srow;
+                s.sdeg = si.sdeg;
                 q.stream = si.qid_lo;
                 q.k1 = p.k1 ^ si.qid_hi;
                 start_walk(q, s);
-                rec.base = si.base;
-                rec.mask = si.mask;
-                rec.slot = slot;
-                rec.source = si.source;
                 rec.visits = 0;
                 rec.steps = 0;
                 budget = hops;
@@ -424,6 +319,10 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
             budget -= (s.hop != hop_before) ? 1u : 0u;
             leave = !done && budget == 0;
         }
+        // the step's visit into its query's key region, every lane present
+        // (the query's source is the field's counted-apart vertex)
+        region_add_uniform(kl, rbuf, rst, rec.pend, slot, rec.pend_v, s.source);
+        rec.pend = false;
         // walks that ended: one decrement per (wavefront, query)
         uint64_t todo = __ballot(done);
         while (todo) {
@@ -450,7 +349,8 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
             if (leave) pool_out[base + rank_of(lm)] = gw_walk{s.id, slot, s.v, s.hop};
         }
     }
-    rec.flush(__ballot(1));  // every lane of the wavefront is here
+    region_evict(kl, rbuf->key[0], rst.f0, rst.c0, rst.d0);  // every lane of the wavefront is here
+    region_evict(kl, rbuf->key[1], rst.f1, rst.c1, rst.d1);
     workgroup_add2(&cnt[1], &cnt[2], tot_steps, tot_visits);
 }
 
@@ -460,16 +360,25 @@ struct Admit {
     SlotInfo info;
     uint64_t at;
     uint32_t slot, walks;
+    uint64_t koff, kend;  // its key region, as u32 indexes into the arena
 };
 
 // One workgroup per admitted query: writes its slot record and remaining
 // count, then its walk records (id = walk index, source field = slot).
 __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __restrict__ pool,
                                    SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining,
-                                   uint32_t* __restrict__ srccount, uint32_t* __restrict__ occn) {
+                                   uint32_t* __restrict__ srccount, uint32_t* __restrict__ occn,
+                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ koff,
+                                   uint64_t* __restrict__ kend, unsigned long long* __restrict__ kcur) {
     const Admit a = adm[blockIdx.x];
     if (threadIdx.x == 0) {
-        slots[a.slot] = a.info;
+        SlotInfo si = a.info;
+        si.srow = off[si.source];
+        si.sdeg = static_cast<uint32_t>(off[si.source + 1] - si.srow);
+        slots[a.slot] = si;
+        koff[a.slot] = a.koff;
+        kend[a.slot] = a.kend;
+        kcur[a.slot] = 0;
         remaining[a.slot] = a.walks;
         srccount[a.slot] = 0;
         occn[a.slot] = 0;
@@ -480,19 +389,143 @@ __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __res
 
 // A completed query's count region and where its pairs go.
 struct Done {
-    uint64_t base, size;   // region
-    uint64_t out;          // first output index (after the scan)
+    uint64_t base, size;   // hash region
+    uint64_t out;          // first output index: room for nkeys pairs from there
     uint32_t slot, source; // the query's slot (its source-vertex counter) and source
+    uint64_t kbase, nkeys; // its key region (u32 index into the arena) and the keys in it
 };
 
-// A completed query's entries: its occupied list names the region slots it
-// claimed; each entry is copied to the query's output range and its slot
-// cleared for reuse (grid.y = query, `n` = list lengths).
-__global__ void serve_gather_kernel(unsigned long long* __restrict__ arena, const uint32_t* __restrict__ occ,
-                                    const Done* __restrict__ done, const unsigned long long* __restrict__ n,
-                                    unsigned long long* __restrict__ out) {
+// A completed query's keys, folded into its hash region: one workgroup per
+// kFoldKeys keys counts them in an LDS table, then adds each distinct vertex
+// once to the hash region (grid.x = chunk, grid.y = query).  The keys are
+// zeroed as they are read, so the arena range is clear again for the next
+// query once the gather has cleared the hash slots.
+constexpr uint32_t kFoldKeys = 4096;
+constexpr uint32_t kFoldSlots = 8192;  // load at most one half
+constexpr uint32_t kFoldEmpty = 0xFFFFFFFFu;  // never a vertex (vertex ids are below 2^32 - 1)
+
+__global__ __launch_bounds__(512) void serve_fold_kernel(unsigned long long* __restrict__ arena,
+                                                        uint32_t* __restrict__ occ, uint32_t* __restrict__ occn,
+                                                        const Done* __restrict__ done, uint32_t cbits,
+                                                        unsigned long long* __restrict__ overflow) {
+    __shared__ uint32_t tkey[kFoldSlots];
+    __shared__ uint32_t tcnt[kFoldSlots];
+    __shared__ uint32_t s_n, s_claims, s_at;
     const Done d = done[blockIdx.y];
-    const uint64_t m = n[blockIdx.y];
+    const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kFoldKeys;
+    if (lo >= d.nkeys) return;  // workgroup-uniform
+    const uint64_t hi = std::min<uint64_t>(lo + kFoldKeys, d.nkeys);
+    for (uint32_t i = threadIdx.x; i < kFoldSlots; i += blockDim.x) {
+        tkey[i] = kFoldEmpty;
+        tcnt[i] = 0;
+    }
+    if (threadIdx.x == 0) s_n = s_claims = 0;
+    __syncthreads();
+    uint32_t* keys = reinterpret_cast<uint32_t*>(arena) + d.kbase;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint32_t v = keys[i];
+        keys[i] = 0;
+        uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 51);
+        for (;;) {  // at most kFoldKeys keys in kFoldSlots slots: an empty slot exists
+            const uint32_t old = atomicCAS(&tkey[h], kFoldEmpty, v);
+            if (old == kFoldEmpty || old == v) {
+                atomicAdd(&tcnt[h], 1u);
+                break;
+            }
+            h = (h + 1) & (kFoldSlots - 1);
+        }
+    }
+    __syncthreads();
+    // the distinct vertices, compacted to the front of the table (one LDS
+    // atomic per wavefront): every thread reads its slots into registers
+    // before any thread writes (the barrier between)
+    uint32_t mv[kFoldSlots / 512], mc[kFoldSlots / 512];
+#pragma unroll
+    for (uint32_t j = 0; j < kFoldSlots / 512; ++j) {
+        mv[j] = tkey[j * 512 + threadIdx.x];
+        mc[j] = tcnt[j * 512 + threadIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kFoldSlots / 512; ++j) {
+        const bool have = mv[j] != kFoldEmpty;
+        const uint64_t bm = __ballot(have);
+        uint32_t at = 0;
+        if (bm) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(bm)) - 1;
+            if (__lane_id() == leader) at = atomicAdd(&s_n, static_cast<uint32_t>(__popcll(bm)));
+            at = lane_value(at, leader);
+        }
+        if (have) {
+            tkey[at + rank_of(bm)] = mv[j];
+            tcnt[at + rank_of(bm)] = mc[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t nd = s_n;
+    // each distinct vertex into the hash region: a thread's compare-and-swaps
+    // are issued back to back, then resolved; collisions probe on
+    const uint32_t rm = static_cast<uint32_t>(d.size - 1);
+    constexpr uint32_t kPer = kFoldKeys / 512;  // distinct vertices per thread at most
+    uint32_t cv[kPer], cc[kPer], ch[kPer];
+    unsigned long long cur[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t i = j * 512 + threadIdx.x;
+        cv[j] = i < nd ? tkey[i] : kFoldEmpty;
+        cc[j] = i < nd ? tcnt[i] : 0u;
+        ch[j] = static_cast<uint32_t>((static_cast<uint64_t>(cv[j]) * 0x9E3779B97F4A7C15ull) >> 32) & rm;
+        cur[j] = cv[j] != kFoldEmpty
+                     ? atomicCAS(arena + d.base + ch[j], 0ull, (static_cast<unsigned long long>(cv[j]) << cbits) | cc[j])
+                     : 1ull;
+    }
+    uint32_t claimed_mask = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        if (cv[j] == kFoldEmpty) continue;
+        const unsigned long long entry = (static_cast<unsigned long long>(cv[j]) << cbits) | cc[j];
+        unsigned long long c = cur[j];
+        uint32_t h = ch[j];
+        bool placed = false;
+        for (uint32_t probe = 0; probe <= rm; ++probe) {
+            if (c == 0ull) {
+                claimed_mask |= 1u << j;
+                placed = true;
+                break;
+            }
+            if ((c >> cbits) == cv[j]) {
+                atomicAdd(arena + d.base + h, static_cast<unsigned long long>(cc[j]));
+                placed = true;
+                break;
+            }
+            h = (h + 1) & rm;
+            c = atomicCAS(arena + d.base + h, 0ull, entry);
+        }
+        ch[j] = h;
+        if (!placed) atomicAdd(overflow, static_cast<unsigned long long>(cc[j]));
+    }
+    // claimed slots join the query's occupied list: one global atomic per workgroup
+    uint32_t mine = __popc(claimed_mask), rank0 = 0;
+    if (mine) rank0 = atomicAdd(&s_claims, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_claims) s_at = atomicAdd(&occn[d.slot], s_claims);
+    __syncthreads();
+    const uint32_t base = s_at;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j)
+        if (claimed_mask & (1u << j)) occ[d.base + base + rank0++] = ch[j];
+}
+
+// A completed query's entries: its occupied list names the hash slots the
+// fold claimed; each entry is copied to the query's output range and its
+// slot cleared for reuse (grid.y = query).  Writes each query's pair count
+// to n[q] for the top-k kernels.
+__global__ void serve_gather_kernel(unsigned long long* __restrict__ arena, const uint32_t* __restrict__ occ,
+                                    const uint32_t* __restrict__ occn, const Done* __restrict__ done,
+                                    unsigned long long* __restrict__ n, unsigned long long* __restrict__ out) {
+    const Done d = done[blockIdx.y];
+    const uint64_t m = occn[d.slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) n[blockIdx.y] = m;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
         const uint32_t h = occ[d.base + i];
@@ -573,18 +606,19 @@ hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, u
     const uint64_t waves = static_cast<uint64_t>(grid) * 4u;
     const uint64_t per_wave = (n + waves - 1) / waves;
     const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(256, std::max<uint64_t>(64, (per_wave + 63) / 64 * 64)));
+    uint32_t* keys = reinterpret_cast<uint32_t*>(S.arena);
     if (S.ooc)
         serve_step_kernel<KIND, false, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
-            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     else if (S.ext)
         serve_step_kernel<KIND, true, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
-            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     else
         serve_step_kernel<KIND, false, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
-            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, S.arena, S.d_occ, S.d_occn,
-            S.cbits, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
     return hipGetLastError();
 }
 
@@ -595,6 +629,12 @@ uint64_t region_slots(uint64_t walks, uint32_t L, uint32_t n) {
     uint64_t size = 64;
     while (size < 2 * distinct) size <<= 1;
     return size;
+}
+
+// Arena slots of a query: its hash region, then its key region (a visit per
+// hop of every walk at most, two u32 keys per slot).
+uint64_t query_slots(uint64_t walks, uint32_t L, uint32_t n) {
+    return region_slots(walks, L, n) + (walks * L + 1) / 2;
 }
 
 // ---- page cache of a server over a host-resident graph ------------------
@@ -817,7 +857,10 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
-        (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 8ull, "serve region counts"))) {
+        (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 8ull, "serve region counts")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_koff, max_queries * 8ull, "serve key regions")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_kend, max_queries * 8ull, "serve key region ends")) ||
+        (rc = gw_alloc(ctx, (void**)&S->d_kcur, max_queries * 8ull, "serve key cursors"))) {
         gw_serve_close_state(ctx, S);
         return rc;
     }
@@ -828,6 +871,28 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         S->pairs_cap = 0;
         gw_serve_close_state(ctx, S);
         return rc;
+    }
+    {  // pinned staging of the per-step copies
+        const uint64_t mq = max_queries;
+        const uint64_t sizes[] = {mq * sizeof(Admit), mq * sizeof(Done), 8 * 8, mq * 4, mq * 8, mq * 8, mq * 4};
+        uint64_t at[7], total = 0;
+        for (int i = 0; i < 7; ++i) {
+            at[i] = total;
+            total += (sizes[i] + 63) / 64 * 64;
+        }
+        if (hipHostMalloc(&S->h_block, total, hipHostMallocDefault) != hipSuccess) {
+            S->h_block = nullptr;
+            gw_serve_close_state(ctx, S);
+            return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: pinned staging buffer");
+        }
+        char* b = static_cast<char*>(S->h_block);
+        S->h_adm = reinterpret_cast<Admit*>(b + at[0]);
+        S->h_done = reinterpret_cast<Done*>(b + at[1]);
+        S->h_cnt = reinterpret_cast<unsigned long long*>(b + at[2]);
+        S->h_rem = reinterpret_cast<uint32_t*>(b + at[3]);
+        S->h_kcur = reinterpret_cast<unsigned long long*>(b + at[4]);
+        S->h_coff = reinterpret_cast<unsigned long long*>(b + at[5]);
+        S->h_tn = reinterpret_cast<uint32_t*>(b + at[6]);
     }
     if (ctx->host_graph && (rc = ooc_open(ctx, *S))) {
         gw_serve_close_state(ctx, S);
@@ -865,9 +930,9 @@ int gw_serve_submit(gw_ctx* ctx, const gw_query* q, uint32_t n) {
         if (q[i].source >= ctx->n || q[i].walks == 0 || q[i].walks > S.pool_cap)
             return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) +
                                                " has a source out of range or a walk count outside [1, pool]");
-        if (region_slots(q[i].walks, S.prog.L, ctx->n) > S.arena_slots)
+        if (query_slots(q[i].walks, S.prog.L, ctx->n) > S.arena_slots)
             return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query " + std::to_string(q[i].qid) + " needs a count region of " +
-                                               std::to_string(region_slots(q[i].walks, S.prog.L, ctx->n)) +
+                                               std::to_string(query_slots(q[i].walks, S.prog.L, ctx->n)) +
                                                " slots, more than the whole arena (" + std::to_string(S.arena_slots) + ")");
         if (S.qids.count(q[i].qid) || !batch.insert(q[i].qid).second)
             return gw_fail(ctx, GW_EINVAL, "gw_serve_submit: query id " + std::to_string(q[i].qid) +
@@ -903,23 +968,26 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     while (!S.queue.empty()) {
         const gw_query& q = S.queue.front();
         if (S.live + q.walks > S.pool_cap || S.free_slots.empty()) break;
-        const uint64_t size = region_slots(q.walks, L, ctx->n);
+        const uint64_t size = region_slots(q.walks, L, ctx->n), total = query_slots(q.walks, L, ctx->n);
         uint64_t base = 0;
-        if (size > S.arena_slots || !arena_alloc(S, size, base)) break;
+        if (total > S.arena_slots || !arena_alloc(S, total, base)) break;
         const uint32_t slot = S.free_slots.back();
         S.free_slots.pop_back();
         const SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
                           static_cast<uint32_t>(q.qid >> 32)};
-        adm.push_back(Admit{si, S.live, slot, q.walks});
+        const uint64_t koff = 2 * (base + size);
+        adm.push_back(Admit{si, S.live, slot, q.walks, koff, koff + static_cast<uint64_t>(q.walks) * L});
         S.live += q.walks;
-        S.active.push_back({q.qid, slot, q.walks, base, size, q.source});
+        S.active.push_back({q.qid, slot, q.walks, base, size, total, q.source});
         S.queue.pop_front();
     }
     if (!adm.empty()) {
-        GW_HIP(ctx, hipMemcpyAsync(S.d_admit, adm.data(), adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
+        std::copy(adm.begin(), adm.end(), S.h_adm);
+        GW_HIP(ctx, hipMemcpyAsync(S.d_admit, S.h_adm, adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
                                    ctx->stream));
         serve_admit_kernel<<<dim3(static_cast<uint32_t>(adm.size())), dim3(256), 0, ctx->stream>>>(
-            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount, S.d_occn);
+            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount, S.d_occn, S.ooc ? S.d_off : ctx->d_off,
+            S.d_koff, S.d_kend, S.d_kcur);
         GW_HIP(ctx, hipGetLastError());
         st.admitted = static_cast<uint32_t>(adm.size());
     }
@@ -940,11 +1008,12 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         if (e != hipSuccess) return gw_hip_check(ctx, e, "serve step launch");
     }
     GW_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    unsigned long long cnt[4];
-    GW_HIP(ctx, hipMemcpyAsync(cnt, S.d_cnt, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<uint32_t> rem(S.max_queries), occn(S.max_queries);
-    GW_HIP(ctx, hipMemcpyAsync(rem.data(), S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-    GW_HIP(ctx, hipMemcpyAsync(occn.data(), S.d_occn, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    const unsigned long long* cnt = S.h_cnt;
+    const uint32_t* rem = S.h_rem;
+    const unsigned long long* nkeys = S.h_kcur;
+    GW_HIP(ctx, hipMemcpyAsync(S.h_cnt, S.d_cnt, 4 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipMemcpyAsync(S.h_rem, S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    GW_HIP(ctx, hipMemcpyAsync(S.h_kcur, S.d_kcur, S.max_queries * 8ull, hipMemcpyDeviceToHost, ctx->stream));
     if (S.ooc && S.npages)
         GW_HIP(ctx, hipMemcpyAsync(S.hits.data(), S.d_page_hits, S.npages * 4, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -966,22 +1035,23 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     st.step_ms = ms;
     S.live = S.live ? cnt[0] : 0;
     S.cur ^= 1;
-    // 3. completed queries: their entries, named by their occupied lists, are
-    //    gathered back to back (and their slots cleared for reuse)
+    // 3. completed queries: their keys are folded into their hash regions,
+    //    whose entries (named by the occupied lists) are gathered back to
+    //    back, with their slots cleared for reuse.  A query's pair count is
+    //    only known on the device: its output range is sized by its key count.
     std::vector<ServeState::Active> still, fin;
     for (const auto& a : S.active) (rem[a.slot] != 0 ? still : fin).push_back(a);
     if (!fin.empty()) {
         const uint32_t nd = static_cast<uint32_t>(fin.size());
         std::vector<Done> dn(nd);
-        std::vector<unsigned long long> per(nd);
-        uint64_t total = 0, maxper = 0;
+        uint64_t total = 0, maxkeys = 0;
         for (uint32_t i = 0; i < nd; ++i) {
-            per[i] = occn[fin[i].slot];
-            dn[i] = Done{fin[i].base, fin[i].size, total, fin[i].slot, fin[i].source};
-            total += per[i];
-            maxper = std::max<uint64_t>(maxper, per[i]);
+            const auto& a = fin[i];
+            const uint64_t nk = nkeys[a.slot];
+            dn[i] = Done{a.base, a.size, total, a.slot, a.source, 2 * (a.base + a.size), nk};
+            total += nk;
+            maxkeys = std::max<uint64_t>(maxkeys, nk);
         }
-        const uint32_t gx = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(64, (maxper + 255) / 256)));
         if (total > S.pairs_cap) {
             gw_free(ctx, S.d_pairs, S.pairs_cap * 8);
             S.d_pairs = nullptr;
@@ -990,17 +1060,25 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
             if (rc) return rc;
             S.pairs_cap = std::max<uint64_t>(total, 1 << 20);
         }
-        GW_HIP(ctx, hipMemcpyAsync(S.d_done, dn.data(), nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
-        GW_HIP(ctx, hipMemcpyAsync(S.d_region_n, per.data(), nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
-        serve_gather_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_done, S.d_region_n,
-                                                                         S.d_pairs);
+        std::copy(dn.begin(), dn.end(), S.h_done);
+        GW_HIP(ctx, hipMemcpyAsync(S.d_done, S.h_done, nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
+        if (maxkeys) {
+            const uint32_t fx = static_cast<uint32_t>((maxkeys + kFoldKeys - 1) / kFoldKeys);
+            serve_fold_kernel<<<dim3(fx, nd), dim3(512), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_occn, S.d_done, S.cbits,
+                                                                         S.d_cnt + 4);
+            GW_HIP(ctx, hipGetLastError());
+        }
+        const uint32_t gx = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(64, (maxkeys + 255) / 256)));
+        serve_gather_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_occn, S.d_done,
+                                                                         S.d_region_n, S.d_pairs);
         GW_HIP(ctx, hipGetLastError());
+        unsigned long long fold_overflow = 0;
         if (S.topk) {
             // top-k per completed query on the device: k pairs per query cross PCIe
             std::vector<unsigned long long> coff(nd);
             uint64_t ncand = 0, maxc = 1;
             for (uint32_t i = 0; i < nd; ++i) {
-                const uint64_t ch = (per[i] + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
+                const uint64_t ch = (dn[i].nkeys + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
                 coff[i] = ncand;
                 ncand += ch * S.topk;
                 maxc = std::max(maxc, ch);
@@ -1014,48 +1092,58 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                 if (rc) return rc;
                 S.cand_cap = cap;
             }
-            GW_HIP(ctx, hipMemcpyAsync(S.d_cand_off, coff.data(), nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
+            std::copy(coff.begin(), coff.end(), S.h_coff);
+            GW_HIP(ctx, hipMemcpyAsync(S.d_cand_off, S.h_coff, nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
             serve_topk_chunk_kernel<<<dim3(static_cast<uint32_t>(maxc), nd), dim3(1024), 0, ctx->stream>>>(
                 S.d_pairs, S.d_done, S.d_region_n, S.d_srccount, S.cbits, S.topk, S.d_cand_off, S.d_cand);
             GW_HIP(ctx, hipGetLastError());
             serve_topk_merge_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_region_n, S.d_cand_off, S.d_cand,
                                                                             S.cbits, S.topk, S.d_topk, S.d_topk_n);
             GW_HIP(ctx, hipGetLastError());
-            std::vector<unsigned long long> top(static_cast<uint64_t>(nd) * S.topk);
-            std::vector<uint32_t> tn(nd);
-            GW_HIP(ctx, hipMemcpyAsync(top.data(), S.d_topk, top.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-            GW_HIP(ctx, hipMemcpyAsync(tn.data(), S.d_topk_n, nd * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+            const unsigned long long* top = S.h_top;
+            const uint32_t* tn = S.h_tn;
+            GW_HIP(ctx, hipMemcpyAsync(S.h_top, S.d_topk, static_cast<uint64_t>(nd) * S.topk * 8, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            GW_HIP(ctx, hipMemcpyAsync(S.h_tn, S.d_topk_n, nd * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+            GW_HIP(ctx, hipMemcpyAsync(S.h_cnt + 4, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
             GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            fold_overflow = S.h_cnt[4];
+            if (fold_overflow) {
+                S.failed = true;
+                return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+            }
             for (uint32_t i = 0; i < nd; ++i) {
                 const auto& a = fin[i];
-                const auto first = top.begin() + static_cast<uint64_t>(i) * S.topk;
+                const auto* first = top + static_cast<uint64_t>(i) * S.topk;
                 S.done[a.qid].assign(first, first + tn[i]);
                 S.done_order.push_back(a.qid);
-                arena_free(S, a.base, a.size);
+                arena_free(S, a.base, a.total);
                 S.free_slots.push_back(a.slot);
             }
-            st.completed = nd;
-            S.active.swap(still);
-            st.live_walks = S.live;
-            st.active_queries = S.active.size();
-            st.queued = S.queue.size();
-            if (out) *out = st;
-            return GW_OK;
-        }
-        std::vector<unsigned long long> all(total);
-        if (total)
-            GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
-        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        std::vector<uint32_t> srcc(S.max_queries);
-        GW_HIP(ctx, hipMemcpy(srcc.data(), S.d_srccount, S.max_queries * 4ull, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < nd; ++i) {
-            const auto& a = fin[i];
-            S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
-            if (srcc[a.slot])
-                S.done[a.qid].push_back((static_cast<unsigned long long>(a.source) << S.cbits) | srcc[a.slot]);
-            S.done_order.push_back(a.qid);
-            arena_free(S, a.base, a.size);
-            S.free_slots.push_back(a.slot);
+        } else {
+            std::vector<unsigned long long> per(nd);
+            std::vector<unsigned long long> all(total);
+            std::vector<uint32_t> srcc(S.max_queries);
+            GW_HIP(ctx, hipMemcpyAsync(per.data(), S.d_region_n, nd * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+            if (total)
+                GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
+            GW_HIP(ctx, hipMemcpyAsync(srcc.data(), S.d_srccount, S.max_queries * 4ull, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            GW_HIP(ctx, hipMemcpyAsync(&fold_overflow, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
+            GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (fold_overflow) {
+                S.failed = true;
+                return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+            }
+            for (uint32_t i = 0; i < nd; ++i) {
+                const auto& a = fin[i];
+                S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
+                if (srcc[a.slot])
+                    S.done[a.qid].push_back((static_cast<unsigned long long>(a.source) << S.cbits) | srcc[a.slot]);
+                S.done_order.push_back(a.qid);
+                arena_free(S, a.base, a.total);
+                S.free_slots.push_back(a.slot);
+            }
         }
         st.completed = nd;
     }
@@ -1117,10 +1205,17 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     S.d_topk = nullptr;
     S.d_topk_n = nullptr;
     S.d_cand = S.d_cand_off = nullptr;
+    if (S.h_top) (void)hipHostFree(S.h_top);
+    S.h_top = nullptr;
     S.cand_cap = 0;
     S.topk = 0;
     if (k == 0) return GW_OK;
     int rc;
+    if (hipHostMalloc(reinterpret_cast<void**>(&S.h_top), static_cast<uint64_t>(S.max_queries) * k * 8,
+                      hipHostMallocDefault) != hipSuccess) {
+        S.h_top = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_topk: pinned result buffer");
+    }
     // candidates for a step whose completing queries average up to 8 chunks
     // (64 K pairs each); a step that needs more grows the buffer
     const uint64_t cand = static_cast<uint64_t>(S.max_queries) * 8 * k;
@@ -1171,6 +1266,11 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
     gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
     gw_free(ctx, S->d_region_n, S->max_queries * 8ull);
+    if (S->h_block) (void)hipHostFree(S->h_block);
+    if (S->h_top) (void)hipHostFree(S->h_top);
+    gw_free(ctx, S->d_koff, S->max_queries * 8ull);
+    gw_free(ctx, S->d_kend, S->max_queries * 8ull);
+    gw_free(ctx, S->d_kcur, S->max_queries * 8ull);
     gw_free(ctx, S->d_topk, static_cast<uint64_t>(S->max_queries) * S->topk * 8);
     gw_free(ctx, S->d_topk_n, S->max_queries * 4ull);
     gw_free(ctx, S->d_cand, S->cand_cap * 8);

@@ -158,6 +158,10 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                     if (EXT) {
                         s.row = hi ? e1.row : e0.row;
                         s.deg = hi ? e1.deg : e0.deg;
+                        if (restarts<KIND>()) {
+                            s.srow = hi ? e1.srow : e0.srow;
+                            s.sdeg = hi ? e1.sdeg : e0.sdeg;
+                        }
                     }
                     rec.id = i - p.id_base;
                     rec.source = s.source;
@@ -192,6 +196,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
                 s.v = w.vertex;
                 s.hop = w.hop;
                 load_row<EXT>(g, s);
+                set_source_row<KIND, EXT>(g, s);
                 start_walk(p, s);
                 active = true;
                 rec.visits = 0;
@@ -323,6 +328,7 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
                     gl.col = bv.col;
                 }
                 load_row<EXT>(gl, s);
+                set_source_row<KIND, EXT>(gl, s);
                 start_walk(p, s);
                 if (SUM) {
                     rec.hash = hash[s.id - p.id_base];
@@ -369,14 +375,24 @@ __global__ __launch_bounds__(256) void step_block_kernel(Prog p, Graph g, const 
 }
 
 __global__ void source_start_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ field,
-                                    uint32_t nsrc, const uint64_t* __restrict__ off,
+                                    uint32_t nsrc, const uint64_t* __restrict__ off, uint32_t n,
                                     const uint32_t* __restrict__ fvert, uint32_t nf, SourceStart* __restrict__ out) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsrc) return;
     const uint32_t v = src[k];
     const uint32_t f = field ? field[k] : v;
     const uint64_t b = off[v];
-    out[k] = SourceStart{v, f, f < nf ? fvert[f] : 0xFFFFFFFFu, static_cast<uint32_t>(off[v + 1] - b), b, 0};
+    SourceStart e{};
+    e.vertex = v;
+    e.field = f;
+    e.fvertex = f < nf ? fvert[f] : 0xFFFFFFFFu;
+    e.deg = static_cast<uint32_t>(off[v + 1] - b);
+    e.row = b;
+    if (f < n) {  // the field as a restart target
+        e.srow = off[f];
+        e.sdeg = static_cast<uint32_t>(off[f + 1] - e.srow);
+    }
+    out[k] = e;
 }
 
 __global__ void summary_init_kernel(const gw_walk* __restrict__ walks, uint64_t nw, uint32_t id_base,
@@ -991,6 +1007,7 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
             rec = (rec & ~kRecKeyLog) | kRecRegion;
             rp.direct = true;
             L.klog.region_off = ctx->d_part_off;
+            L.klog.region_end = ctx->d_part_off + 1;
             L.klog.region_n = ctx->d_part_cursor;
             L.klog.region_out = ctx->d_part_keys;
             L.klog.region_overflow = ctx->d_counters + kCntPartOverflow;
@@ -1005,6 +1022,21 @@ int plan_run(gw_ctx* ctx, const gw_program* prog, RunPlan& rp) {
         L.gen = WalkGen{static_cast<const SourceStart*>(ctx->d_gen_table), ctx->gen_wps, ctx->gen_nsrc};
     rp.sched = sched;
     rp.rec = rec;
+    return GW_OK;
+}
+
+// The per-source start entries of generated walks, rebuilt on the engine
+// stream when the walks, the graph or the source fields changed since the
+// last build (outside any captured graph: a replay calls it before launching).
+int gen_table_ready(gw_ctx* ctx, const RunPlan& rp) {
+    if (!rp.L.gen.tab) return GW_OK;
+    const uint64_t built[3] = {ctx->graph_epoch, reinterpret_cast<uintptr_t>(rp.L.klog.field_vertex), rp.L.klog.nf};
+    if (std::equal(built, built + 3, ctx->gen_built)) return GW_OK;
+    source_start_kernel<<<dim3((ctx->gen_nsrc + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        ctx->d_gen_src, ctx->d_gen_field, ctx->gen_nsrc, ctx->d_off, ctx->n, rp.L.klog.field_vertex, rp.L.klog.nf,
+        const_cast<SourceStart*>(rp.L.gen.tab));
+    GW_HIP(ctx, hipGetLastError());
+    std::copy(built, built + 3, ctx->gen_built);
     return GW_OK;
 }
 
@@ -1024,15 +1056,6 @@ int enqueue_run(gw_ctx* ctx, RunPlan& rp, bool timed, bool clear = true) {
         const uint64_t words = ctx->table_bytes / 8;
         fill_u64_kernel<<<dim3(gw_grid(words, 256)), dim3(256), 0, ctx->stream>>>(ctx->d_table, words);
         GW_HIP(ctx, hipGetLastError());
-    }
-    // the sources' start entries, built when the graph or the fields changed
-    const uint64_t built[3] = {ctx->graph_epoch, reinterpret_cast<uintptr_t>(rp.L.klog.field_vertex), rp.L.klog.nf};
-    if (rp.L.gen.tab && !std::equal(built, built + 3, ctx->gen_built)) {
-        source_start_kernel<<<dim3((ctx->gen_nsrc + 255) / 256), dim3(256), 0, ctx->stream>>>(
-            ctx->d_gen_src, ctx->d_gen_field, ctx->gen_nsrc, ctx->d_off, rp.L.klog.field_vertex, rp.L.klog.nf,
-            const_cast<SourceStart*>(rp.L.gen.tab));
-        GW_HIP(ctx, hipGetLastError());
-        std::copy(built, built + 3, ctx->gen_built);
     }
     if (timed) GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (ctx->nwalks) {
@@ -1097,7 +1120,7 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             p(ctx->d_part_fcount), ctx->part_fields,
             ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
             rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final,
-            p(rp.L.gen.tab), rp.L.gen.wps, rp.L.gen.nsrc, p(ctx->d_gen_src), p(ctx->d_gen_field), ctx->graph_epoch};
+            p(rp.L.gen.tab), rp.L.gen.wps, rp.L.gen.nsrc};
 }
 
 // Host walk pool: the run steps the pool window by window through two device
@@ -1280,7 +1303,7 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         if (rp.part && (rc = gw_partlog_enqueue(ctx, !rp.direct))) return rc;
         GW_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     } else {
-        if ((rc = enqueue_run(ctx, rp, true))) return rc;
+        if ((rc = gen_table_ready(ctx, rp)) || (rc = enqueue_run(ctx, rp, true))) return rc;
         st.launches = ctx->nwalks ? 1 : 0;
         st.rounds = 1;
     }
@@ -1325,6 +1348,7 @@ int gw_run_capture(gw_ctx* ctx, const gw_program* prog) {
     }
     ctx->run_graph = g;
     g->plan = rp;
+    if ((rc = gen_table_ready(ctx, rp))) return rc;
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     GW_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     rc = enqueue_run(ctx, g->plan, false);
@@ -1359,6 +1383,7 @@ int gw_run_replay(gw_ctx* ctx, gw_stats* stats) {
     if (rc) return rc;
     if (fingerprint(ctx, rp) != g->fp)
         return gw_fail(ctx, GW_ESTATE, "gw_run_replay: buffers, graph or walk count changed since the capture");
+    if ((rc = gen_table_ready(ctx, rp))) return rc;
     GW_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     GW_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
     GW_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
