@@ -252,7 +252,8 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
                                                         unsigned long long* __restrict__ kcur, uint32_t nslots,
                                                         uint32_t hops,
                                                         unsigned long long* __restrict__ cnt,
-                                                        unsigned long long* __restrict__ cursor, uint32_t chunk) {
+                                                        unsigned long long* __restrict__ cursor, uint32_t chunk,
+                                                        uint64_t nchunks, uint64_t stride) {
     const uint32_t lane = __lane_id();
     uint64_t chunk_next = 0, chunk_end = 0;
     bool exhausted = false, active = false;
@@ -276,9 +277,14 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
         while (idle && !exhausted) {
             if (chunk_next == chunk_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(cursor, static_cast<unsigned long long>(chunk));
+                if (lane == 0) base = atomicAdd(cursor, 1ull);
                 base = lane_value(base, 0u);
-                if (base >= n) { exhausted = true; break; }
+                if (base >= nchunks) { exhausted = true; break; }
+                // chunks in a permuted order (stride coprime to nchunks): the
+                // pool holds each query's walks back to back, and taken in
+                // order the chunks in flight would come from a few large
+                // queries, whose walks all read the same few lines
+                base = (base * stride % nchunks) * chunk;  // both below nchunks < 2^32
                 chunk_next = base;
                 chunk_end = base + chunk < n ? base + chunk : n;
             }
@@ -606,19 +612,23 @@ hipError_t launch_serve(gw_ctx* ctx, ServeState& S, const Prog& p, uint64_t n, u
     const uint64_t waves = static_cast<uint64_t>(grid) * 4u;
     const uint64_t per_wave = (n + waves - 1) / waves;
     const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(256, std::max<uint64_t>(64, (per_wave + 63) / 64 * 64)));
+    const uint64_t nchunks = (n + chunk - 1) / chunk;
+    uint64_t stride = std::max<uint64_t>(1, static_cast<uint64_t>(static_cast<double>(nchunks) * 0.6180339887) | 1);
+    while (std::gcd(stride, nchunks) != 1) stride += 2;
+    stride %= nchunks;
     uint32_t* keys = reinterpret_cast<uint32_t*>(S.arena);
     if (S.ooc)
         serve_step_kernel<KIND, false, true><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
             p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
-            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk, nchunks, stride);
     else if (S.ext)
         serve_step_kernel<KIND, true, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
             p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
-            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk, nchunks, stride);
     else
         serve_step_kernel<KIND, false, false><<<dim3(grid), dim3(256), 0, ctx->stream>>>(
             p, g, pc, S.pool[S.cur], n, S.pool[S.cur ^ 1], S.d_slots, S.d_remaining, S.d_srccount, keys, S.d_koff,
-            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk);
+            S.d_kend, S.d_kcur, S.max_queries, S.hops, S.d_cnt, S.d_cnt + 5, chunk, nchunks, stride);
     return hipGetLastError();
 }
 
