@@ -453,7 +453,10 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
     out = {"qps_offered": qps, "queries": int(measured.sum()), "seconds": wall,
            "p50_ms": float(np.percentile(lm, 50) * 1e3), "p99_ms": float(np.percentile(lm, 99) * 1e3),
            "mean_ms": float(lm.mean() * 1e3), "walk_steps_per_s": acc["steps"] / wall, "visits": acc["visits"],
-           "serve_steps": nsteps, "gpu_busy_frac": acc["step_ms"] * 1e-3 / wall, "hops_per_step": hops,
+           "serve_steps": nsteps, "gpu_busy_frac": acc["step_ms"] * 1e-3 / wall,
+           # walk-steps per second of serve-step kernel time (HIP events around the kernel)
+           "in_kernel_walk_steps_per_s": acc["steps"] / (acc["step_ms"] * 1e-3) if acc["step_ms"] else 0.0,
+           "hops_per_step": hops,
            "walks_per_query": f"log-uniform [{walks_lo}, {walks_hi}]", "result": f"top-{topk}" if topk else "all pairs"}
     if warm_seconds:
         out["warmup_queries"] = int((~measured).sum())
@@ -667,7 +670,7 @@ def main() -> int:
                                                "default: picked by the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0])
+    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0, 40000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-topk", type=int, default=100, help="vertices returned per query (0 = every pair)")
     ap.add_argument("--serve-hops", type=int, default=10,
