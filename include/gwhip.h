@@ -282,7 +282,9 @@ int gw_stream_policy(gw_ctx *ctx, uint32_t policy, double prob, uint64_t seed);
 int gw_walks_load_host(gw_ctx *ctx, const gw_walk *walks, uint64_t n, uint64_t window, uint32_t flags);
 int gw_walks_init_per_vertex(gw_ctx *ctx, uint32_t per_vertex);
 /* walks_per_source walks per source: id = k * walks_per_source + j, source =
- * source_field[k] (or sources[k] when source_field is NULL), vertex = sources[k] */
+ * source_field[k] (or sources[k] when source_field is NULL), vertex = sources[k].
+ * Persistent runs start these walks from one entry per source instead of a
+ * record per walk, until other walks are placed or gw_bucket reorders them. */
 int gw_walks_init_sources(gw_ctx *ctx, const uint32_t *sources, const uint32_t *source_field,
                           uint32_t nsources, uint32_t walks_per_source);
 /* Final record of every walk after the last gw_run, indexed by walk id. */
@@ -363,12 +365,14 @@ int gw_events_read(gw_ctx *ctx, uint64_t *out, uint64_t n_walks, uint32_t cap);
 /* ---- serving: continuous batching of PPR-style queries -------------------
  * Admitted queries share one device pool of walk records; each gw_serve_step
  * advances every live walk by at most `hops_per_step` hops, so queries that
- * arrive while others are in flight join at the next step.  A query's visit
- * counts accumulate in its own hash region of a count arena in HBM; queries
+ * arrive while others are in flight join at the next step.  A query's visits
+ * are appended to its own key region of a count arena in HBM while its walks
+ * run, and counted into its own hash region when its last walk ends; queries
  * are admitted first come first served while walk slots, a query slot and
- * arena space (2 x min(walks * L, n) slots rounded up to a power of two) are
- * free.  The arena takes arena_bytes of HBM plus half as much again for each
- * region's list of occupied slots (completion reads only those).  The random stream of walk i of query q is Philox(counter {i, hop,
+ * arena space are free (8-byte slots: 2 x min(walks * L, n) rounded up to a
+ * power of two for the hash region, plus walks * L / 2 for the keys).  The
+ * arena takes arena_bytes of HBM plus half as much again for each region's
+ * list of occupied slots (completion reads only those).  The random stream of walk i of query q is Philox(counter {i, hop,
  * 'WALK', q.qid low word}, key {seed low, seed high ^ q.qid high word}), so a
  * query's result does not depend on the other queries in flight.  There is
  * no counterpart in the reference (it has no serving loop). */
