@@ -440,10 +440,17 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
                     nsteps += 1
                     resident = st["pages_resident"]
                 t_done = time.perf_counter() - t0
-                while (r := eng.serve_poll()) is not None:
-                    eng.serve_result(*r, sort=False)
-                    lat[r[0]] = t_done - arrivals[r[0]]
-                    done += 1
+                if topk:
+                    # every completed query's top-k in one call
+                    while (q := eng.serve_results(4096)[0]).shape[0]:
+                        qi = q.astype(np.int64)
+                        lat[qi] = t_done - arrivals[qi]
+                        done += qi.shape[0]
+                else:
+                    while (r := eng.serve_poll()) is not None:
+                        eng.serve_result(*r, sort=False)
+                        lat[r[0]] = t_done - arrivals[r[0]]
+                        done += 1
             elif nxt < nq:
                 time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
     finally:

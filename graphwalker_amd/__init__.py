@@ -265,6 +265,7 @@ def lib() -> C.CDLL:
         "gw_serve_step": ([P, P], i),
         "gw_serve_poll": ([P, P, P], i),
         "gw_serve_result": ([P, u64, P, P, u64], i),
+        "gw_serve_results": ([P, u32, P, P, P, P, u64, P], i),
         "gw_serve_topk": ([P, u32], i),
         "gw_serve_close": ([P], i),
         "gw_serve_cache": ([P, u32, u32], i),
@@ -571,6 +572,7 @@ class Engine:
                    hops_per_step: int = 4) -> None:
         self._check(self._L.gw_serve_open(self._ctx, C.byref(prog), pool_walks, arena_bytes, max_queries,
                                           hops_per_step), "gw_serve_open")
+        self._serve_k = 0
 
     def serve_submit(self, queries) -> None:
         """queries: iterable of (qid, source, walks)."""
@@ -602,9 +604,31 @@ class Engine:
             v, c = v[o], c[o]
         return v, c
 
+    def serve_results(self, max_q: int = 4096):
+        """Every completed query not yet read, at most max_q, in one call:
+        (qids, npairs, vertices, counts) with the queries' pairs back to back
+        in completion order.  Needs serve_topk(k > 0) (a query then holds at
+        most k pairs); without it, read queries with serve_poll/serve_result."""
+        k = getattr(self, "_serve_k", 0)
+        if k <= 0:
+            raise GwError("serve_results needs serve_topk(k > 0)")
+        buf = getattr(self, "_serve_buf", None)
+        if buf is None or buf[0].shape[0] < max_q:
+            buf = (np.empty(max_q, np.uint64), np.empty(max_q, np.uint64),
+                   np.empty(max_q * k, np.uint32), np.empty(max_q * k, np.uint32))
+            self._serve_buf = buf
+        q, npairs, v, c = buf
+        n = C.c_uint32()
+        self._check(self._L.gw_serve_results(self._ctx, max_q, _ptr(q), _ptr(npairs), _ptr(v), _ptr(c),
+                                             max_q * k, C.byref(n)), "gw_serve_results")
+        m = n.value
+        tot = int(npairs[:m].sum())
+        return q[:m].copy(), npairs[:m].copy(), v[:tot].copy(), c[:tot].copy()
+
     def serve_topk(self, k: int) -> None:
         """Completed queries keep their k most-visited vertices (0 = all pairs)."""
         self._check(self._L.gw_serve_topk(self._ctx, k), "gw_serve_topk")
+        self._serve_k = k
 
     def serve_cache(self, page_log2: int = 0, max_loads: int = 0) -> None:
         """Page size and page copies in flight of a server over a host-resident
@@ -613,6 +637,7 @@ class Engine:
 
     def serve_close(self) -> None:
         self._check(self._L.gw_serve_close(self._ctx), "gw_serve_close")
+        self._serve_k = 0
 
     def events(self, cap: int) -> np.ndarray:
         """Event log of the last run: (nwalks, cap) u64, (hop << 32) | vertex."""

@@ -425,7 +425,7 @@ __global__ __launch_bounds__(512) void serve_fold_kernel(unsigned long long* __r
         tkey[i] = kFoldEmpty;
         tcnt[i] = 0;
     }
-    if (threadIdx.x == 0) s_n = s_claims = 0;
+    if (threadIdx.x == 0) s_n = s_claims = s_at = 0;
     __syncthreads();
     uint32_t* keys = reinterpret_cast<uint32_t*>(arena) + d.kbase;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
@@ -1057,7 +1057,8 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         uint64_t total = 0, maxkeys = 0;
         for (uint32_t i = 0; i < nd; ++i) {
             const auto& a = fin[i];
-            const uint64_t nk = nkeys[a.slot];
+            // the cursor never passes the region (an overflow fails the step above)
+            const uint64_t nk = std::min<uint64_t>(nkeys[a.slot], static_cast<uint64_t>(a.walks) * L);
             dn[i] = Done{a.base, a.size, total, a.slot, a.source, 2 * (a.base + a.size), nk};
             total += nk;
             maxkeys = std::max<uint64_t>(maxkeys, nk);
@@ -1170,13 +1171,52 @@ int gw_serve_poll(gw_ctx* ctx, uint64_t* qid, uint64_t* npairs) {
     ServeState* sp = state_of(ctx);
     if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: not open");
     ServeState& S = *sp;
+    // queries read by id through gw_serve_result before they were polled are skipped
+    while (!S.done_order.empty() && !S.done.count(S.done_order.front())) S.done_order.pop_front();
     if (S.done_order.empty()) return 1;  // nothing completed
     const uint64_t id = S.done_order.front();
     S.done_order.pop_front();
     auto d = S.done.find(id);
-    if (d == S.done.end()) return gw_fail(ctx, GW_ESTATE, "gw_serve_poll: completed query " + std::to_string(id) + " lost");
     *qid = id;
     *npairs = d->second.size();
+    return GW_OK;
+}
+
+int gw_serve_results(gw_ctx* ctx, uint32_t max_q, uint64_t* qid, uint64_t* npairs, uint32_t* vertex, uint32_t* count,
+                     uint64_t cap, uint32_t* n_out) {
+    if (!ctx || !n_out || (max_q && (!qid || !npairs)) || (cap && (!vertex || !count)))
+        return gw_fail(ctx, GW_EINVAL, "gw_serve_results: bad arguments");
+    ServeState* sp = state_of(ctx);
+    if (!sp) return gw_fail(ctx, GW_ESTATE, "gw_serve_results: not open");
+    ServeState& S = *sp;
+    const unsigned long long cmask = (1ull << S.cbits) - 1;
+    uint32_t nq = 0;
+    uint64_t at = 0;
+    while (nq < max_q && !S.done_order.empty()) {
+        const uint64_t id = S.done_order.front();
+        auto d = S.done.find(id);
+        if (d == S.done.end()) {  // read already through gw_serve_result
+            S.done_order.pop_front();
+            continue;
+        }
+        const std::vector<unsigned long long>& pairs = d->second;
+        if (at + pairs.size() > cap) {
+            if (nq == 0) return gw_fail(ctx, GW_EINVAL, "gw_serve_results: the next query's pairs exceed cap");
+            break;
+        }
+        for (size_t i = 0; i < pairs.size(); ++i) {
+            vertex[at + i] = static_cast<uint32_t>(pairs[i] >> S.cbits);
+            count[at + i] = static_cast<uint32_t>(pairs[i] & cmask);
+        }
+        qid[nq] = id;
+        npairs[nq] = pairs.size();
+        at += pairs.size();
+        ++nq;
+        S.done.erase(d);
+        S.qids.erase(id);
+        S.done_order.pop_front();
+    }
+    *n_out = nq;
     return GW_OK;
 }
 

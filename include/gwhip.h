@@ -422,6 +422,14 @@ int gw_serve_poll(gw_ctx *ctx, uint64_t *qid, uint64_t *npairs);
 /* Copies a completed query's (vertex, count) pairs, in no particular order,
  * and releases them. */
 int gw_serve_result(gw_ctx *ctx, uint64_t qid, uint32_t *vertex, uint32_t *count, uint64_t cap);
+/* The completed queries not yet read, in completion order, at most max_q of
+ * them: qid[i] and npairs[i], their pairs back to back in vertex / count (at
+ * most cap entries; a query that does not fit stays for the next call).
+ * *n_out = queries returned, each released.  One call per serve step in place
+ * of a gw_serve_poll + gw_serve_result pair per query (no counterpart in the
+ * reference). */
+int gw_serve_results(gw_ctx *ctx, uint32_t max_q, uint64_t *qid, uint64_t *npairs, uint32_t *vertex, uint32_t *count,
+                     uint64_t cap, uint32_t *n_out);
 /* Completed queries keep only their k most-visited vertices (1 <= k <= 1024;
  * 0 = every pair, the default), selected on the device; gw_serve_result then
  * returns them count descending, ties to the smaller vertex id. */

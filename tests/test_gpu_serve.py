@@ -18,8 +18,10 @@ def expected(off, col, kind, L, qid, source, walks):
     return nz.astype(np.uint32), c[nz]
 
 
-def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64, topk=0):
-    """arrivals: list of (step, qid, source, walks); submits each at its step."""
+def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64, topk=0, batch=0):
+    """arrivals: list of (step, qid, source, walks); submits each at its step.
+    batch > 0 reads the results through serve_results, at most batch queries
+    a call; otherwise one query at a time through serve_poll / serve_result."""
     engine.serve_open(gw.program(kind, L, 0.15, 0, seed=SEED), pool, arena, max_queries, hops)
     if topk:
         engine.serve_topk(topk)
@@ -32,8 +34,16 @@ def serve_all(engine, arrivals, kind, L, hops, pool, arena, max_queries=64, topk
                 engine.serve_submit([(q, s, w) for _, q, s, w in now])
             st = engine.serve_step()
             assert st["live_walks"] <= pool
-            while (r := engine.serve_poll()) is not None:
-                results[r[0]] = engine.serve_result(*r, sort=not topk)
+            if batch:
+                while (got := engine.serve_results(batch))[0].shape[0]:
+                    q, npairs, v, c = got
+                    assert q.shape[0] <= batch and int(npairs.sum()) == v.shape[0] == c.shape[0]
+                    ends = np.cumsum(npairs.astype(np.int64)).tolist()
+                    for qid, e, m in zip(q.tolist(), ends, npairs.tolist()):
+                        results[qid] = v[e - m:e], c[e - m:e]
+            else:
+                while (r := engine.serve_poll()) is not None:
+                    results[r[0]] = engine.serve_result(*r, sort=not topk)
             step += 1
             assert step < 10_000
     finally:
@@ -116,15 +126,17 @@ def test_serve_rejects_bad_queries(engine, g12):
         engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=1), 1000, 1 << 20, 70_000, 4)
 
 
-@pytest.mark.parametrize("k", [1, 10, 1024])
-def test_served_topk(engine, g12, k):
+@pytest.mark.parametrize("k,batch", [(1, 0), (10, 0), (1024, 0), (10, 3), (1024, 64)])
+def test_served_topk(engine, g12, k, batch):
     # completed queries keep their k most-visited vertices, count descending,
-    # ties to the smaller vertex -- the top of the exact counts
+    # ties to the smaller vertex -- the top of the exact counts; read one by
+    # one or in batches of at most `batch` queries
     off, col = g12
     rng = np.random.default_rng(k)
     arrivals = [(int(rng.integers(0, 5)), 500 + i, int(rng.integers(0, 4096)), int(rng.integers(100, 4000)))
                 for i in range(20)]
-    res = serve_all(engine, arrivals, gw.RESTART, 10, 3, pool=100_000, arena=64 << 20, topk=k)
+    res = serve_all(engine, arrivals, gw.RESTART, 10, 3, pool=100_000, arena=64 << 20, topk=k, batch=batch)
+    assert len(res) == len(arrivals)
     for _, qid, src, w in arrivals:
         v, c = res[qid]
         ev, ec = expected(off, col, gw.RESTART, 10, qid, src, w)
@@ -148,3 +160,28 @@ def test_served_topk_many_chunks(engine, k):
         order = np.lexsort((ev, -ec.astype(np.int64)))[:k]
         assert np.array_equal(v, ev[order]) and np.array_equal(c, ec[order]), qid
     assert most > 32768
+
+
+def test_batched_results_skip_queries_read_by_id(engine, g12):
+    # a query read through serve_result by its id before any poll is not
+    # handed out again by serve_poll or serve_results; serve_results needs a
+    # top-k (its output holds at most k pairs a query)
+    off, col = g12
+    engine.graph_set(off, col)  # an earlier test may have set another graph
+    engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=SEED), 100_000, 64 << 20, 64, 10)
+    try:
+        with pytest.raises(gw.GwError, match="serve_topk"):
+            engine.serve_results()
+        engine.serve_topk(5)
+        engine.serve_submit([(1, 10, 300), (2, 20, 300), (3, 30, 300)])
+        for _ in range(100):
+            engine.serve_step()
+        v, c = engine.serve_result(2, 5, sort=False)
+        ev, ec = expected(off, col, gw.RESTART, 10, 2, 20, 300)
+        order = np.lexsort((ev, -ec.astype(np.int64)))[:5]
+        assert np.array_equal(v, ev[order]) and np.array_equal(c, ec[order])
+        q, npairs, v, c = engine.serve_results(10)
+        assert sorted(q.tolist()) == [1, 3] and list(npairs) == [5, 5]
+        assert engine.serve_poll() is None and engine.serve_results()[0].shape[0] == 0
+    finally:
+        engine.serve_close()
