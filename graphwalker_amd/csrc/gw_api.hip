@@ -259,6 +259,7 @@ void gw_destroy(gw_ctx* ctx) {
     gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
     gw_free(ctx, ctx->d_part_fvert, ctx->part_fields_cap * 4ull);
     gw_free(ctx, ctx->d_part_fcount, ctx->part_fields_cap * 4ull);
+    gw_free(ctx, ctx->d_part_items, ctx->part_fields_cap * 32ull);
     gw_free(ctx, ctx->d_split_counts, ctx->split_cap * 8);
     gw_free(ctx, ctx->d_runs, ctx->runs_cap * 8);
     gw_free(ctx, ctx->d_run_counts, ctx->runs_cap * 4);

@@ -29,8 +29,8 @@ constexpr uint32_t kCntOutPairs = 16;         // partitioned: exact pairs emitte
 constexpr uint32_t kCntSplitPairs = 17;       // partitioned: partial pairs of flushed sources
 constexpr uint32_t kCntFlushed = 18;          // partitioned: flushed sources
 constexpr uint32_t kCntSplitRuns = 19;        // partitioned: sums of the split pairs
-constexpr uint32_t kCntAggQueue = 20;         // partitioned: next source for the aggregation workgroups
 constexpr uint32_t kCntBadWalks = 21;         // persistent schedule: walks skipped at pickup (vertex or restart source not in the graph)
+constexpr uint32_t kCntAggItems = 22;         // partitioned: sources with region keys (aggregation work list)
 
 struct gw_run_graph;  // gw_step.hip: a captured run (gw_run_capture)
 
@@ -173,6 +173,7 @@ struct gw_ctx {
     unsigned long long* d_part_pairs = nullptr;   // distinct vertices per field (0 for a flushed source)
     uint32_t* d_part_fvert = nullptr;             // field_vertex on the device
     uint32_t* d_part_fcount = nullptr;            // visits of each field's designated vertex
+    void* d_part_items = nullptr;                 // aggregation work list, 32 B per field
     uint32_t* d_split_counts = nullptr;      // 2 x split_cap: counts of the split pairs (their keys go to the key log)
     uint64_t split_cap = 0;
     uint64_t part_pairs = 0;                 // last run: pairs held in the regions

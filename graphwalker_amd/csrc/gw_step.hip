@@ -1117,7 +1117,7 @@ std::vector<uint64_t> fingerprint(const gw_ctx* ctx, const RunPlan& rp) {
             p(ctx->d_table), ctx->table_bytes, ctx->table_slots, ctx->table_log2, ctx->table_vbits, ctx->table_cbits,
             ctx->table_wide, p(ctx->d_keys), ctx->keys_cap, ctx->keys_sentinel, p(ctx->d_part_keys),
             p(ctx->d_part_counts), p(ctx->d_part_off), p(ctx->d_part_cursor), p(ctx->d_part_pairs), p(ctx->d_part_fvert),
-            p(ctx->d_part_fcount), ctx->part_fields,
+            p(ctx->d_part_fcount), p(ctx->d_part_items), ctx->part_fields,
             ctx->part_total, p(ctx->d_split_counts), ctx->split_cap, p(ctx->d_counters), rp.part, rp.rec, rp.sched,
             rp.ext, rp.L.tb, rp.L.grid_persistent, rp.L.klog.slab, ctx->cu_count, rp.L.no_final,
             p(rp.L.gen.tab), rp.L.gen.wps, rp.L.gen.nsrc};

@@ -79,6 +79,10 @@ constexpr uint32_t kAggSlots = 1u << kAggLog2;               // LDS hash slots p
 constexpr uint32_t kAggThreads = 1024;
 constexpr uint32_t kAggPerThread = 4;
 constexpr uint32_t kAggTile = kAggThreads * kAggPerThread;   // keys loaded per iteration
+constexpr uint32_t kAggWaves = kAggThreads / 64;
+constexpr uint32_t kAggHeld = 12;                             // keys a thread holds for a whole source
+constexpr uint32_t kAggHeldKeys = kAggHeld * kAggThreads;
+static_assert(kAggHeldKeys <= kAggSlots / 4 * 3, "held keys must fit a table that never flushes");
 // A flushed table takes a whole tile plus the designated vertex below its 3/4 fill bound.
 static_assert(kAggTile + 1 <= kAggSlots / 4 * 3, "a tile must fit an emptied table");
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;                   // never a vertex id (n <= 2^32 - 1)
@@ -188,27 +192,33 @@ __device__ __forceinline__ uint32_t agg_log2(uint64_t n) {
 }
 
 // Write the occupied slots of the table (first `tslots`) from position `at0`
-// on, then clear them; `total` = the occupied slots (the table's fill).  The
-// pairs of a source are in no particular order (read-out sorts them, top-k
-// breaks ties by vertex): each wavefront reserves its places row by row with
-// one LDS atomic on s_cur, which starts at 0 and is left at 0.  kSplit: as
-// 64-bit keys (fbase | vertex) + counts into the split list (only if `fits`);
-// else as vertex + count into the source's own region.  Ends synchronised.
+// on, then clear them.  Wavefront w owns the slots [w * tslots / 16, (w + 1)
+// * tslots / 16): it counts its occupied slots, the counts meet in s_wtot,
+// and each wavefront writes its pairs in slot order after those of the
+// wavefronts before it (no atomics).  kSplit: as 64-bit keys (fbase |
+// vertex) + counts into the split list (only if `fits`); else as vertex +
+// count into the source's own region.  Starts after the inserts have been
+// synchronised; ends synchronised.
 template <bool kSplit>
-__device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t* s_cur,
+__device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t* s_wtot,
                                           uint64_t at0, bool fits, uint64_t fbase, uint32_t* __restrict__ seg_v,
                                           uint32_t* __restrict__ seg_c, unsigned long long* __restrict__ split_k,
                                           uint32_t* __restrict__ split_c) {
-    const uint32_t lane = __lane_id();
-    for (uint32_t i = threadIdx.x; i < tslots; i += kAggThreads) {  // tslots >= 1024: every lane is present
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
+    const uint32_t spw = tslots / kAggWaves;  // tslots >= 1024: a multiple of 64, every lane present
+    const uint32_t s0 = w * spw;
+    uint32_t occ = 0;
+    for (uint32_t s = lane; s < spw; s += 64) occ += __popcll(__ballot(s_key[s0 + s] != kEmptyKey));
+    if (lane == 0) s_wtot[w] = occ;
+    __syncthreads();
+    uint64_t at = at0 + gw::wave_sum(lane < w ? s_wtot[lane] : 0u);
+    for (uint32_t s = lane; s < spw; s += 64) {
+        const uint32_t i = s0 + s;
         const uint32_t k = s_key[i];
         const bool full = k != kEmptyKey;
         const uint64_t mask = __ballot(full);
-        uint32_t b = 0;
-        if (lane == 0 && mask) b = atomicAdd(s_cur, static_cast<uint32_t>(__popcll(mask)));
-        b = gw::lane_value(b, 0u);
         if (full && fits) {
-            const uint64_t pos = at0 + b + rank_in(mask);
+            const uint64_t pos = at + rank_in(mask);
             const uint32_t c = s_cnt[i];
             if (kSplit) {
                 split_k[pos] = fbase | k;
@@ -218,24 +228,24 @@ __device__ __forceinline__ void agg_write(uint32_t* s_key, uint32_t* s_cnt, uint
                 seg_c[pos] = c;
             }
         }
+        at += __popcll(mask);
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *s_cur = 0;
 }
 
 // Flush the table (`total` occupied slots) to the split list: one returning
 // atomic reserves the pairs (counted even when they do not fit `cap`; the
 // host checks).
 __device__ __forceinline__ void agg_flush(uint32_t* s_key, uint32_t* s_cnt, uint32_t tslots, uint32_t total,
-                                          uint32_t* s_cur, unsigned long long* s_base, uint64_t fbase,
+                                          uint32_t* s_wtot, unsigned long long* s_base, uint64_t fbase,
                                           unsigned long long* __restrict__ split_k, uint32_t* __restrict__ split_c,
                                           unsigned long long* __restrict__ split_n, uint64_t cap) {
     if (threadIdx.x == 0) *s_base = total ? atomicAdd(split_n, static_cast<unsigned long long>(total)) : 0ull;
     __syncthreads();
     const uint64_t b = *s_base;
-    agg_write<true>(s_key, s_cnt, tslots, s_cur, b, b + total <= cap, fbase, nullptr, nullptr, split_k, split_c);
+    agg_write<true>(s_key, s_cnt, tslots, s_wtot, b, b + total <= cap, fbase, nullptr, nullptr, split_k, split_c);
 }
 
 // Source f's region [lo, lo + n) and, in direct-region runs (fcount set), its
@@ -255,13 +265,75 @@ __device__ __forceinline__ AggMeta region_meta(const uint64_t* __restrict__ regi
         // clamp to the region: an overflowed partition is discarded by the
         // host, but must still be read in bounds
         m.n = std::min<uint64_t>(region_n[f], region_off[f + 1] - m.lo);
-        if (fcount) {  // both loads issued together, a source ahead of their use
+        if (fcount) {
             const uint32_t c = fcount[f], v = fvert[f];
             m.fc = c;
             m.fv = c ? v : kEmptyKey;
         }
     }
     return m;
+}
+
+// A source with region keys, as the aggregation workgroups take it.
+struct alignas(16) AggItem {
+    uint64_t lo, n;
+    uint32_t f, fc, fv, pad;
+};
+
+// One thread per source.  A source with no region keys is finished here --
+// its designated vertex's pair (if it has visits) written over the start of
+// its region, pairs[f] = 0 or 1 -- and every other source is appended to the
+// work list of part_aggregate_kernel (one atomic per wavefront).  Most
+// sources of a batch can be of the first kind (walks from a vertex with no
+// out-edges end at once), and they then cost the workgroups nothing.
+__global__ __launch_bounds__(256) void part_classify_kernel(
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ counts, unsigned long long* __restrict__ pairs,
+    const uint64_t* __restrict__ region_off, const unsigned long long* __restrict__ region_n, uint32_t nf,
+    unsigned long long* __restrict__ total_pairs, AggItem* __restrict__ items, unsigned long long* __restrict__ nitems,
+    const uint32_t* __restrict__ fvert, const uint32_t* __restrict__ fcount) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const AggMeta m = region_meta(region_off, region_n, nf, f, fvert, fcount);
+    const bool work = f < nf && m.n > 0;
+    uint32_t single = 0;
+    if (f < nf && !work) {
+        if (m.fc) {
+            keys[m.lo] = m.fv;
+            counts[m.lo] = m.fc;
+            single = 1;
+        }
+        pairs[f] = single;
+    }
+    const uint64_t mask = __ballot(work);
+    if (mask) {
+        unsigned long long at = 0;
+        if (lane == 0) at = atomicAdd(nitems, static_cast<unsigned long long>(__popcll(mask)));
+        at = gw::lane_value(at, 0u);
+        if (work) items[at + rank_in(mask)] = AggItem{m.lo, m.n, f, m.fc, m.fv, 0u};
+    }
+    single = gw::wave_sum(single);
+    if (lane == 0 && single) atomicAdd(total_pairs, static_cast<unsigned long long>(single));
+}
+
+__device__ __forceinline__ AggItem agg_item(const AggItem* __restrict__ items, uint64_t nitems, uint64_t i) {
+    return i < nitems ? items[i] : AggItem{0, 0, 0, 0, kEmptyKey, 0};
+}
+
+// A source whose keys all fit in registers and in a table it never has to
+// flush (kAggHeld keys a thread; with its designated vertex at most 3/4 of
+// kAggSlots).
+__device__ __forceinline__ bool agg_held(const AggItem& m) {
+    return m.n + (m.fc ? 1u : 0u) <= kAggHeldKeys;
+}
+
+__device__ __forceinline__ void load_held(const uint32_t* __restrict__ keys, const AggItem& m,
+                                          uint32_t (&t)[kAggHeld]) {
+    const bool held = agg_held(m);
+#pragma unroll
+    for (uint32_t j = 0; j < kAggHeld; ++j) {
+        const uint64_t idx = j * kAggThreads + threadIdx.x;
+        t[j] = held && idx < m.n ? keys[m.lo + idx] : kEmptyKey;
+    }
 }
 
 __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ keys, uint64_t lo, uint64_t n, uint64_t first,
@@ -273,126 +345,133 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ keys, uin
     }
 }
 
-// Persistent workgroups, one per CU, each counting one source at a time in
-// an LDS open-addressing table (linear probing, multiplicative hash) sized to
-// the source's key count.  Keys are
-// loaded kAggTile at a time, the next tile -- or the next source's first tile
-// -- in flight while the current one is inserted.  Runs of one vertex across
-// a wavefront are combined before the LDS atomics.  Before a tile would lift
-// the fill above 3/4 of the table it is flushed to the split list, so every
-// probe sequence ends.  A source that never flushed writes its distinct
-// vertices and counts over the start of its own region (its keys are all in
-// registers or consumed by then) and its pair count to pairs[f], with no
-// atomic on the critical path; a flushed source emits partial pairs to the
-// split list, summed later by sort-reduce, and pairs[f] = 0.  With fcount
-// (direct-region runs), source f's designated vertex fvert[f] and its
-// fcount[f] visits, counted apart by the step kernel, enter the table before
-// the region's keys; a source with no region keys writes that one pair.
+__device__ __forceinline__ uint32_t agg_home(uint32_t v, uint32_t lg) { return (v * 0x9E3779B1u) >> (32 - lg); }
+
+// Adds `inc` visits of v to the table (linear probing from slot h, v's home
+// slot or a later one of its probe sequence); 1 if v took an empty slot.
+__device__ __forceinline__ uint32_t agg_put(uint32_t* s_key, uint32_t* s_cnt, uint32_t lg, uint32_t v, uint32_t inc,
+                                            uint32_t h) {
+    for (;;) {
+        const uint32_t old = atomicCAS(&s_key[h], kEmptyKey, v);
+        if (old == kEmptyKey || old == v) {
+            atomicAdd(&s_cnt[h], inc);
+            return old == kEmptyKey;
+        }
+        h = (h + 1) & ((1u << lg) - 1);
+    }
+}
+
+// One key a lane (kEmptyKey = none).  Runs of one vertex across the wavefront
+// (the first hops of a source's walks) are combined before touching LDS: the
+// lowest lane holding the first valid lane's vertex adds the whole group.
+// (Measured alternatives that were slower: probing the lane's keys
+// together, 4-slot buckets read with one 128-bit load, and 64-bit key|count
+// slots read before the atomic; plain stores of every key to its home slot
+// read back after a barrier, with compare-and-swap only for the keys that
+// lost their slot; a lane's keys probed in batches of compare-and-swaps
+// issued back to back.)
+__device__ __forceinline__ uint32_t agg_insert(uint32_t* s_key, uint32_t* s_cnt, uint32_t lg, uint32_t v,
+                                               uint32_t h) {
+    const uint64_t valid = __ballot(v != kEmptyKey);
+    if (!valid) return 0;  // wavefront-uniform
+    uint32_t inc = 1;
+    const uint32_t v0 = __shfl(v, static_cast<int>(__ffsll(static_cast<unsigned long long>(valid)) - 1), 64);
+    const uint64_t same = __ballot(v == v0);
+    if (v == v0) {
+        if (rank_in(same) == 0) inc = __popcll(same);
+        else v = kEmptyKey;
+    }
+    return v == kEmptyKey ? 0u : agg_put(s_key, s_cnt, lg, v, inc, h);
+}
+
+// Persistent workgroups, one per CU, each counting one source of the work
+// list at a time in an LDS open-addressing table sized to the source's key
+// count; workgroup b takes the items b, b + G, b + 2G, ... (G workgroups:
+// the sources of a batch are alike in size, and a fixed order needs no
+// queue atomics, whose returns would stall the pipeline below).  Right
+// after a source's keys are in the table, the workgroup issues the loads of
+// the next source's keys (into registers) and of the list entry after that,
+// so they are in flight while the table is written out, and a source's
+// inserts wait only on loads issued a phase earlier.  A source with more
+// keys than a workgroup holds (kAggHeldKeys) goes tile by tile: before a
+// tile would lift the fill above 3/4 of the table, the table is flushed to
+// the split list, so every probe sequence ends.  A source that never flushed
+// writes its distinct vertices and counts over the start of its own region
+// (its keys are all in registers or consumed by then) and its pair count to
+// pairs[f]; a flushed source emits partial pairs to the split list, summed
+// later by sort-reduce, and pairs[f] = 0.  A source's designated vertex
+// (direct-region runs) and its count fc, counted apart by the step kernel,
+// go into the table with the region's keys.
 __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ counts, unsigned long long* __restrict__ pairs,
-    const uint64_t* __restrict__ region_off, const unsigned long long* __restrict__ region_n, uint32_t nf,
-    uint32_t vbits, unsigned long long* __restrict__ total_pairs, unsigned long long* __restrict__ split_k,
+    const AggItem* __restrict__ items, const unsigned long long* __restrict__ nitems_p, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ counts, unsigned long long* __restrict__ pairs, uint32_t vbits,
+    unsigned long long* __restrict__ total_pairs, unsigned long long* __restrict__ split_k,
     uint32_t* __restrict__ split_c, unsigned long long* __restrict__ split_n, uint64_t split_cap,
-    unsigned long long* __restrict__ flushed_sources, unsigned long long* __restrict__ queue,
-    const uint32_t* __restrict__ fvert, const uint32_t* __restrict__ fcount) {
+    unsigned long long* __restrict__ flushed_sources) {
     __shared__ uint32_t s_key[kAggSlots];
     __shared__ uint32_t s_cnt[kAggSlots];
+    __shared__ uint32_t s_wtot[kAggWaves];
     __shared__ uint32_t s_fill;
-    __shared__ uint32_t s_cur;
     __shared__ unsigned long long s_base;
-    __shared__ uint32_t s_next;
-    const uint32_t G = gridDim.x;
+    const uint64_t nitems = *nitems_p;
+    if (blockIdx.x >= nitems) return;  // workgroup-uniform, before any barrier
+    const uint64_t G = gridDim.x;
     const uint32_t lane = __lane_id();
-    // Sources 0..G-1 go one per workgroup; the others are taken from a queue
-    // (one atomic per source, issued a source ahead), so workgroups that draw
-    // light sources take more of them.
-    uint32_t f = blockIdx.x;
-    if (threadIdx.x == 0) s_next = G + static_cast<uint32_t>(atomicAdd(queue, 1ull));
+    if (threadIdx.x == 0) s_fill = 0;
     for (uint32_t i = threadIdx.x; i < kAggSlots; i += kAggThreads) {
         s_key[i] = kEmptyKey;
         s_cnt[i] = 0;
     }
-    if (threadIdx.x == 0) {
-        s_fill = 0;
-        s_cur = 0;
-    }
     __syncthreads();
-    uint32_t fn = s_next;
-    AggMeta cm = region_meta(region_off, region_n, nf, f, fvert, fcount);
-    AggMeta nm = region_meta(region_off, region_n, nf, fn, fvert, fcount);
-    uint64_t lo = cm.lo, n = cm.n, nlo = nm.lo, nn = nm.n;
-    uint32_t cur[kAggPerThread], pre[kAggPerThread];
-    load_tile(keys, lo, n, 0, cur);
-    while (f < nf) {
-        unsigned long long ticket = 0;
-        if (threadIdx.x == 0) ticket = atomicAdd(queue, 1ull);  // consumed at the end of this source
-        // visits of the field's designated vertex, counted apart by the step kernel
-        const uint32_t fc = cm.fc, fv = cm.fv;
-        if (n == 0) {
+    uint64_t x = blockIdx.x;
+    AggItem cm = agg_item(items, nitems, x);
+    AggItem nm = agg_item(items, nitems, x + G);
+    uint32_t cur[kAggHeld], nxt[kAggHeld];
+    load_held(keys, cm, cur);
+    while (x < nitems) {
+        const uint64_t lo = cm.lo, n = cm.n;
+        const uint32_t f = cm.f, fc = cm.fc, fv = cm.fv;
+        const uint32_t lg = agg_log2(n + (fc ? 1u : 0u));
+        const uint32_t tslots = 1u << lg;
+        AggItem nnm;
+        if (agg_held(cm)) {
+            uint32_t added = 0;
+            if (threadIdx.x == 0 && fc) added = agg_put(s_key, s_cnt, lg, fv, fc, agg_home(fv, lg));
+#pragma unroll
+            for (uint32_t j = 0; j < kAggHeld; ++j)
+                if (j * kAggThreads < n) added += agg_insert(s_key, s_cnt, lg, cur[j], agg_home(cur[j], lg));  // uniform
+            added = gw::wave_sum(added);
+            if (lane == 0 && added) atomicAdd(&s_fill, added);
+            __syncthreads();
+            const uint32_t total = s_fill;
+            nnm = agg_item(items, nitems, x + 2 * G);
+            load_held(keys, nm, nxt);
+            agg_write<false>(s_key, s_cnt, tslots, s_wtot, lo, true, 0, keys, counts, nullptr, nullptr);
             if (threadIdx.x == 0) {
-                if (fc) {
-                    keys[lo] = fv;
-                    counts[lo] = fc;
-                    atomicAdd(total_pairs, 1ull);
-                }
-                pairs[f] = fc ? 1u : 0u;
+                pairs[f] = total;
+                atomicAdd(total_pairs, static_cast<unsigned long long>(total));
+                s_fill = 0;
             }
-            load_tile(keys, nlo, nn, 0, pre);
         } else {
-            const uint32_t lg = agg_log2(n + (fc ? 1u : 0u));
-            const uint32_t tslots = 1u << lg;
             const uint32_t max_fill = tslots / 4 * 3;
             const uint64_t fbase = static_cast<uint64_t>(f) << vbits;
             bool flushed = false;
             if (fc) {  // workgroup-uniform: the table is empty when a source starts
                 if (threadIdx.x == 0) {
-                    const uint32_t h = (fv * 0x9E3779B1u) >> (32 - lg);
-                    s_key[h] = fv;
-                    s_cnt[h] = fc;
+                    agg_put(s_key, s_cnt, lg, fv, fc, agg_home(fv, lg));
                     s_fill = 1;
                 }
                 __syncthreads();
             }
+            uint32_t tc[kAggPerThread], tp[kAggPerThread];
+            load_tile(keys, lo, n, 0, tc);
             uint32_t fill = 0;
             for (uint64_t base = 0;; base += kAggTile) {
                 const uint64_t next = base + kAggTile;
-                if (next < n) load_tile(keys, lo, n, next, pre);
-                else load_tile(keys, nlo, nn, 0, pre);
-                // Runs of one vertex across the wavefront (a source's own vertex
-                // opens every one of its walks) are combined before touching
-                // LDS: the lowest lane holding the first valid lane's vertex
-                // adds the whole group.  Then linear probing, one key of the
-                // lane at a time.  (Measured alternatives that were slower:
-                // probing the lane's keys together, 4-slot buckets read with
-                // one 128-bit load, and 64-bit key|count slots read before
-                // the atomic.)
+                if (next < n) load_tile(keys, lo, n, next, tp);
                 uint32_t added = 0;
 #pragma unroll
-                for (uint32_t j = 0; j < kAggPerThread; ++j) {
-                    uint32_t v = cur[j];
-                    uint32_t inc = 1;
-                    const uint64_t valid = __ballot(v != kEmptyKey);
-                    if (valid) {
-                        const uint32_t v0 =
-                            __shfl(v, static_cast<int>(__ffsll(static_cast<unsigned long long>(valid)) - 1), 64);
-                        const uint64_t same = __ballot(v == v0);
-                        if (v == v0) {
-                            if (rank_in(same) == 0) inc = __popcll(same);
-                            else v = kEmptyKey;
-                        }
-                    }
-                    if (v == kEmptyKey) continue;
-                    uint32_t h = (v * 0x9E3779B1u) >> (32 - lg);
-                    for (;;) {
-                        const uint32_t old = atomicCAS(&s_key[h], kEmptyKey, v);
-                        if (old == kEmptyKey || old == v) {
-                            atomicAdd(&s_cnt[h], inc);
-                            added += old == kEmptyKey;
-                            break;
-                        }
-                        h = (h + 1) & (tslots - 1);
-                    }
-                }
+                for (uint32_t j = 0; j < kAggPerThread; ++j) added += agg_insert(s_key, s_cnt, lg, tc[j], agg_home(tc[j], lg));
                 added = gw::wave_sum(added);
                 if (lane == 0 && added) atomicAdd(&s_fill, added);
                 __syncthreads();
@@ -400,44 +479,37 @@ __global__ __launch_bounds__(kAggThreads) void part_aggregate_kernel(
                 __syncthreads();  // every thread has read the fill before it is reset
                 if (next >= n) break;
                 if (fill + std::min<uint64_t>(kAggTile, n - next) > max_fill) {  // workgroup-uniform
-                    agg_flush(s_key, s_cnt, tslots, fill, &s_cur, &s_base, fbase, split_k, split_c, split_n,
+                    agg_flush(s_key, s_cnt, tslots, fill, s_wtot, &s_base, fbase, split_k, split_c, split_n,
                               split_cap);
                     if (threadIdx.x == 0) s_fill = 0;
                     flushed = true;
                     __syncthreads();
                 }
 #pragma unroll
-                for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
+                for (uint32_t j = 0; j < kAggPerThread; ++j) tc[j] = tp[j];
             }
             if (flushed) {
-                agg_flush(s_key, s_cnt, tslots, fill, &s_cur, &s_base, fbase, split_k, split_c, split_n, split_cap);
+                agg_flush(s_key, s_cnt, tslots, fill, s_wtot, &s_base, fbase, split_k, split_c, split_n, split_cap);
                 if (threadIdx.x == 0) {
                     pairs[f] = 0;
                     atomicAdd(flushed_sources, 1ull);
                 }
             } else {
-                const uint32_t total = fill;
-                agg_write<false>(s_key, s_cnt, tslots, &s_cur, lo, true, 0, keys, counts, nullptr, nullptr);
+                agg_write<false>(s_key, s_cnt, tslots, s_wtot, lo, true, 0, keys, counts, nullptr, nullptr);
                 if (threadIdx.x == 0) {
-                    pairs[f] = total;
-                    atomicAdd(total_pairs, static_cast<unsigned long long>(total));
+                    pairs[f] = fill;
+                    atomicAdd(total_pairs, static_cast<unsigned long long>(fill));
                 }
             }
             if (threadIdx.x == 0) s_fill = 0;
+            nnm = agg_item(items, nitems, x + 2 * G);
+            load_held(keys, nm, nxt);
         }
-        __syncthreads();  // every thread has read the previous s_next
-        if (threadIdx.x == 0) s_next = G + static_cast<uint32_t>(ticket);
-        __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < kAggPerThread; ++j) cur[j] = pre[j];
-        f = fn;
-        fn = s_next;
+        x += G;
         cm = nm;
-        lo = nlo;
-        n = nn;
-        nm = region_meta(region_off, region_n, nf, fn, fvert, fcount);
-        nlo = nm.lo;
-        nn = nm.n;
+        nm = nnm;
+#pragma unroll
+        for (uint32_t j = 0; j < kAggHeld; ++j) cur[j] = nxt[j];
     }
 }
 
@@ -588,15 +660,18 @@ int gw_partlog_prepare(gw_ctx* ctx, uint32_t per_walk, bool* ok) {
             gw_free(ctx, ctx->d_part_pairs, ctx->part_fields_cap * 8ull);
             gw_free(ctx, ctx->d_part_fvert, ctx->part_fields_cap * 4ull);
             gw_free(ctx, ctx->d_part_fcount, ctx->part_fields_cap * 4ull);
+            gw_free(ctx, ctx->d_part_items, ctx->part_fields_cap * 32ull);
             ctx->d_part_off = nullptr;
             ctx->d_part_cursor = ctx->d_part_pairs = nullptr;
             ctx->d_part_fvert = ctx->d_part_fcount = nullptr;
+            ctx->d_part_items = nullptr;
             ctx->part_fields_cap = 0;
             if ((rc = gw_alloc(ctx, (void**)&ctx->d_part_off, (nf + 1ull) * 8, "source regions")) ||
                 (rc = gw_alloc(ctx, (void**)&ctx->d_part_cursor, nf * 8ull, "source region cursors")) ||
                 (rc = gw_alloc(ctx, (void**)&ctx->d_part_pairs, nf * 8ull, "source pair counts")) ||
                 (rc = gw_alloc(ctx, (void**)&ctx->d_part_fvert, nf * 4ull, "source field vertices")) ||
-                (rc = gw_alloc(ctx, (void**)&ctx->d_part_fcount, nf * 4ull, "source field vertex counts")))
+                (rc = gw_alloc(ctx, (void**)&ctx->d_part_fcount, nf * 4ull, "source field vertex counts")) ||
+                (rc = gw_alloc(ctx, &ctx->d_part_items, nf * 32ull, "aggregation work list")))
                 return rc;
             ctx->part_fields_cap = nf;
         }
@@ -655,12 +730,17 @@ int gw_partlog_enqueue(gw_ctx* ctx, bool partition) {
             ctx->d_part_off, ctx->d_part_cursor, ctx->d_part_keys, c + kCntPartOverflow);
         GW_HIP(ctx, hipGetLastError());
     }
-    const uint32_t grid = std::min(ctx->part_fields, cus);
-    part_aggregate_kernel<<<dim3(grid), dim3(kAggThreads), 0, ctx->stream>>>(
-        ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->d_part_cursor,
-        ctx->part_fields, ctx->table_vbits, c + kCntOutPairs, ctx->d_keys, ctx->d_split_counts, c + kCntSplitPairs,
-        ctx->split_cap, c + kCntFlushed, c + kCntAggQueue, partition ? nullptr : ctx->d_part_fvert,
+    const uint32_t nf = ctx->part_fields;
+    AggItem* items = static_cast<AggItem*>(ctx->d_part_items);
+    part_classify_kernel<<<dim3((nf + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->d_part_off, ctx->d_part_cursor, nf,
+        c + kCntOutPairs, items, c + kCntAggItems, partition ? nullptr : ctx->d_part_fvert,
         partition ? nullptr : ctx->d_part_fcount);
+    GW_HIP(ctx, hipGetLastError());
+    // workgroups beyond the work list's length return at once
+    part_aggregate_kernel<<<dim3(std::min(nf, cus)), dim3(kAggThreads), 0, ctx->stream>>>(
+        items, c + kCntAggItems, ctx->d_part_keys, ctx->d_part_counts, ctx->d_part_pairs, ctx->table_vbits,
+        c + kCntOutPairs, ctx->d_keys, ctx->d_split_counts, c + kCntSplitPairs, ctx->split_cap, c + kCntFlushed);
     GW_HIP(ctx, hipGetLastError());
     return GW_OK;
 }

@@ -390,6 +390,31 @@ def test_source_counts_partitioned_flush_is_exact(engine):
     assert np.array_equal(s2, es) and np.array_equal(v2, ev) and np.array_equal(c2, ec)
 
 
+@pytest.mark.parametrize("schedule,layout", [("persistent", "ext"), ("plain", "csr")])
+def test_source_counts_partitioned_large_regions_exact(engine, schedule, layout):
+    # Regions with more keys than an aggregation workgroup holds in registers
+    # (12288) but few distinct vertices are counted tile by tile without a
+    # flush; sources on sinks have no region keys and are finished before the
+    # workgroups start; the rest take the register path.
+    off, col = O.rmat_graph(10, 1 << 14, seed=6)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    sinks = np.nonzero(np.diff(off) == 0)[0][:3].astype(np.uint32)
+    sources = np.r_[O.sources(2, n, 5), sinks].astype(np.uint32)
+    field = np.arange(sources.shape[0], dtype=np.uint32)
+    wps = 8000
+    engine.walks_init_sources(sources, wps, source_field=field)
+    prog = gw.program(gw.STOP, 20, 0.15, n, seed=7, recorder=gw.REC_SOURCE_COUNT | gw.REC_NO_FINAL,
+                      schedule=schedule, layout=layout, source_method=gw.SRC_PARTITIONED)
+    st = engine.run(prog)
+    assert st["source_method"] == gw.SRC_PARTITIONED and st["flushed_sources"] == 0
+    s, v, c = engine.source_counts()
+    es, ev, ec = _expected_source_counts(prog, off, col, sources, field, wps)
+    assert np.array_equal(s, es) and np.array_equal(v, ev) and np.array_equal(c, ec)
+    region = np.bincount(es, weights=np.where(ev == sources[es], 0, ec), minlength=sources.shape[0])
+    assert region.max() > 12288 and sinks.shape[0] > 0 and (region[5:] == 0).all()
+
+
 def test_source_counts_region_overflow_falls_back(engine):
     # PROB walks are uncapped; regions are sized for 64 visits per walk.  Source
     # field 0 starts on a sink-free cycle (about 128 visits per walk at p = 1/128),
