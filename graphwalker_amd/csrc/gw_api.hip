@@ -247,6 +247,7 @@ void gw_destroy(gw_ctx* ctx) {
     gw_pool_free(ctx);
     free_walks(ctx);
     gw_free(ctx, ctx->d_counters, kCounters * sizeof(unsigned long long));
+    if (ctx->h_run_cnt) (void)hipHostFree(ctx->h_run_cnt);
     gw_free(ctx, ctx->d_tile_hist, ctx->tile_hist_cap * 4);
     gw_free(ctx, ctx->d_table, ctx->table_bytes);
     gw_free(ctx, ctx->d_keys, ctx->keys_cap * 8);

@@ -182,6 +182,7 @@ struct gw_ctx {
 
     // device counters (kCounters of them, see the kCnt* indices)
     unsigned long long* d_counters = nullptr;
+    unsigned long long* h_run_cnt = nullptr;  // pinned: a run's counters read back (kCounters)
     // bucketing scratch
     uint32_t* d_tile_hist = nullptr;
     uint64_t tile_hist_cap = 0;

@@ -1307,8 +1307,15 @@ int gw_step_run(gw_ctx* ctx, const gw_program* prog, gw_stats* stats) {
         st.launches = ctx->nwalks ? 1 : 0;
         st.rounds = 1;
     }
-    unsigned long long cnt[kCounters];
-    GW_HIP(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    // through pinned memory: a copy to pageable memory is staged by the
+    // runtime before it returns
+    if (!ctx->h_run_cnt &&
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_run_cnt), kCounters * 8, hipHostMallocDefault) != hipSuccess) {
+        ctx->h_run_cnt = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, "gw_run: pinned counter buffer");
+    }
+    const unsigned long long* cnt = ctx->h_run_cnt;
+    GW_HIP(ctx, hipMemcpyAsync(ctx->h_run_cnt, ctx->d_counters, kCounters * 8, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
     st.total_ms = ms;
