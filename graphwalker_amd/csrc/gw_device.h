@@ -350,7 +350,11 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
     st.d0 = uniform(st.d0);
     st.d1 = uniform(st.d1);
     // visits of the field's designated vertex: counted in the buffer state
-    // while their field holds a buffer, else one atomic per (wavefront, field)
+    // while their field holds a buffer -- a field without one takes a buffer
+    // that holds no staged ids (evicting just its count) -- else one atomic
+    // per (wavefront, field).  A field whose walks only ever revisit that
+    // vertex (a restart walk from a vertex with no out-edges) would otherwise
+    // add to one address from every wavefront at every step.
     const bool fvis = want && v == field_v;
     uint64_t fdo = __ballot(fvis);
     while (fdo) {
@@ -358,11 +362,28 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
         const uint32_t ls = lane_value(source, leader);
         const uint64_t grp = __ballot(fvis && source == ls);
         const uint32_t d = __popcll(grp);
-        const bool in0 = st.f0 == ls, in1 = st.f1 == ls;
+        bool in0 = st.f0 == ls, in1 = st.f1 == ls;
+        if (!in0 && !in1 && ls < k.nf) {  // wavefront-uniform
+            if (st.c0 == 0) {
+                if (st.d0 && lane == 0) atomicAdd(&k.field_count[st.f0], st.d0);
+                st.f0 = ls;
+                st.d0 = 0;
+                in0 = true;
+            } else if (st.c1 == 0) {
+                if (st.d1 && lane == 0) atomicAdd(&k.field_count[st.f1], st.d1);
+                st.f1 = ls;
+                st.d1 = 0;
+                in1 = true;
+            }
+        }
         st.d0 += in0 ? d : 0u;
         st.d1 += in1 ? d : 0u;
         if (!in0 && !in1 && ls < k.nf && lane == leader) atomicAdd(&k.field_count[ls], d);
         fdo &= ~grp;
+        st.f0 = uniform(st.f0);
+        st.d0 = uniform(st.d0);
+        st.f1 = uniform(st.f1);
+        st.d1 = uniform(st.d1);
     }
     const bool rest = want && !fvis;
     uint64_t todo = __ballot(rest);
