@@ -340,6 +340,7 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x) {
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
 }
 
+template <bool kCountsTakeBuffers = false>
 __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* rb, RegionState& st, bool want,
                                                    uint32_t source, uint32_t v, uint32_t field_v) {
     const uint32_t lane = __lane_id();
@@ -350,11 +351,13 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
     st.d0 = uniform(st.d0);
     st.d1 = uniform(st.d1);
     // visits of the field's designated vertex: counted in the buffer state
-    // while their field holds a buffer -- a field without one takes a buffer
-    // that holds no staged ids (evicting just its count) -- else one atomic
-    // per (wavefront, field).  A field whose walks only ever revisit that
-    // vertex (a restart walk from a vertex with no out-edges) would otherwise
-    // add to one address from every wavefront at every step.
+    // while their field holds a buffer -- with kCountsTakeBuffers, a field
+    // without one takes a buffer that holds no staged ids (evicting just its
+    // count) -- else one atomic per (wavefront, field).  Serving takes
+    // buffers: a query whose walks only ever revisit that vertex (restart
+    // walks from a vertex with no out-edges) would otherwise add to one
+    // address from every wavefront at every hop.  The batched path does not:
+    // its walks visit the designated vertex once, at their start.
     const bool fvis = want && v == field_v;
     uint64_t fdo = __ballot(fvis);
     while (fdo) {
@@ -363,7 +366,7 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
         const uint64_t grp = __ballot(fvis && source == ls);
         const uint32_t d = __popcll(grp);
         bool in0 = st.f0 == ls, in1 = st.f1 == ls;
-        if (!in0 && !in1 && ls < k.nf) {  // wavefront-uniform
+        if (kCountsTakeBuffers && !in0 && !in1 && ls < k.nf) {  // wavefront-uniform
             if (st.c0 == 0) {
                 if (st.d0 && lane == 0) atomicAdd(&k.field_count[st.f0], st.d0);
                 st.f0 = ls;
@@ -380,10 +383,12 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
         st.d1 += in1 ? d : 0u;
         if (!in0 && !in1 && ls < k.nf && lane == leader) atomicAdd(&k.field_count[ls], d);
         fdo &= ~grp;
-        st.f0 = uniform(st.f0);
-        st.d0 = uniform(st.d0);
-        st.f1 = uniform(st.f1);
-        st.d1 = uniform(st.d1);
+        if (kCountsTakeBuffers) {
+            st.f0 = uniform(st.f0);
+            st.d0 = uniform(st.d0);
+            st.f1 = uniform(st.f1);
+            st.d1 = uniform(st.d1);
+        }
     }
     const bool rest = want && !fvis;
     uint64_t todo = __ballot(rest);

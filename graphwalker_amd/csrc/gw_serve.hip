@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void serve_step_kernel(Prog p, Graph g, PagedC
         }
         // the step's visit into its query's key region, every lane present
         // (the query's source is the field's counted-apart vertex)
-        region_add_uniform(kl, rbuf, rst, rec.pend, slot, rec.pend_v, s.source);
+        region_add_uniform<true>(kl, rbuf, rst, rec.pend, slot, rec.pend_v, s.source);
         rec.pend = false;
         // walks that ended: one decrement per (wavefront, query)
         uint64_t todo = __ballot(done);
