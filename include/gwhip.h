@@ -417,15 +417,16 @@ int gw_serve_open(gw_ctx *ctx, const gw_program *prog, uint64_t pool_walks, uint
 int gw_serve_submit(gw_ctx *ctx, const gw_query *queries, uint32_t n);
 int gw_serve_step(gw_ctx *ctx, gw_serve_stats *stats);
 /* Pops the next completed query: returns 0 and its id and pair count, or 1
- * when none is waiting. */
+ * when none is waiting (queries already read by id are skipped). */
 int gw_serve_poll(gw_ctx *ctx, uint64_t *qid, uint64_t *npairs);
 /* Copies a completed query's (vertex, count) pairs, in no particular order,
  * and releases them. */
 int gw_serve_result(gw_ctx *ctx, uint64_t qid, uint32_t *vertex, uint32_t *count, uint64_t cap);
 /* The completed queries not yet read, in completion order, at most max_q of
  * them: qid[i] and npairs[i], their pairs back to back in vertex / count (at
- * most cap entries; a query that does not fit stays for the next call).
- * *n_out = queries returned, each released.  One call per serve step in place
+ * most cap entries; a query that does not fit stays for the next call, and
+ * GW_EINVAL if the first one does not).  *n_out = queries returned, each
+ * released.  One call per serve step in place
  * of a gw_serve_poll + gw_serve_result pair per query (no counterpart in the
  * reference). */
 int gw_serve_results(gw_ctx *ctx, uint32_t max_q, uint64_t *qid, uint64_t *npairs, uint32_t *vertex, uint32_t *count,
