@@ -356,8 +356,9 @@ __device__ __forceinline__ void region_add_uniform(const KeyLog& k, RegionBuf* r
     // count) -- else one atomic per (wavefront, field).  Serving takes
     // buffers: a query whose walks only ever revisit that vertex (restart
     // walks from a vertex with no out-edges) would otherwise add to one
-    // address from every wavefront at every hop.  The batched path does not:
-    // its walks visit the designated vertex once, at their start.
+    // address from every wavefront at every hop.  Batched restart walks do
+    // the same; batched walks of the other kinds visit the designated vertex
+    // once, at their start, and keep the plain rule.
     const bool fvis = want && v == field_v;
     uint64_t fdo = __ballot(fvis);
     while (fdo) {
@@ -452,7 +453,7 @@ __device__ __forceinline__ void keylog_close(const KeyLog& k, uint64_t* slab, ui
 // Recorder hooks, compile-time selected: the device replacements for the
 // per-step updateInfo callback (randomwalk.hpp:32).
 template <bool kVertexCount, bool kSummary, bool kEvents = false, bool kSourceCount = false, bool kKeyLog = false,
-          bool kRegion = false, bool kDeferRegion = false>
+          bool kRegion = false, bool kDeferRegion = false, bool kCountsTakeBuffers = false>
 struct Recorder {
     uint32_t* __restrict__ counts;
     uint32_t visits;
@@ -516,7 +517,7 @@ struct Recorder {
     // the step (each step makes at most one visit per lane).
     __device__ __forceinline__ void commit() {
         if (kRegion && kDeferRegion) {
-            region_add_uniform(klog, rbuf, rst, pend, source, pend_v, field_v);
+            region_add_uniform<kCountsTakeBuffers>(klog, rbuf, rst, pend, source, pend_v, field_v);
             pend = false;
         }
     }

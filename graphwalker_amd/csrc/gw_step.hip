@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(Prog p, Graph g, c
     }
     __shared__ RegionBuf s_rbuf[RG ? 4 : 1];
     if (RG && (threadIdx.x & 63u) == 0) region_buf_init(&s_rbuf[threadIdx.x >> 6]);
-    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG, RG> rec(
+    Recorder<(REC & GW_REC_VERTEX_COUNT) != 0, SUM, (REC & GW_REC_EVENTS) != 0, (REC & GW_REC_SOURCE_COUNT) != 0, KL, RG, RG,
+             RG && restarts<KIND>()> rec(
         counts, log, table, klog, s_slab[threadIdx.x >> 6], RG ? &s_rbuf[threadIdx.x >> 6] : nullptr);
 
     for (;;) {
