@@ -400,6 +400,8 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
     arrivals = np.cumsum(rng.exponential(1.0 / qps, nq))
     walks = np.exp(rng.uniform(np.log(walks_lo), np.log(walks_hi), nq)).astype(np.int64)
     sources = gw.sources(seed, n, nq)
+    qarr = np.empty(nq, gw.QUERY_DTYPE)
+    qarr["qid"], qarr["source"], qarr["walks"] = np.arange(nq), sources, walks
     eng.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=seed), pool_walks=pool_walks, arena_bytes=arena_bytes,
                    max_queries=4096, hops_per_step=hops)
     if topk:
@@ -425,11 +427,9 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
     try:
         while done < nq:
             now = time.perf_counter() - t0
-            k = nxt
-            while k < nq and arrivals[k] <= now:
-                k += 1
+            k = int(np.searchsorted(arrivals, now, side="right"))
             if k > nxt:
-                eng.serve_submit([(i, int(sources[i]), int(walks[i])) for i in range(nxt, k)])
+                eng.serve_submit(qarr[nxt:k])
                 nxt = k
             if done < nxt:
                 in_window = now >= warm_seconds

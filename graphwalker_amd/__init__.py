@@ -71,6 +71,10 @@ class Program(C.Structure):
     ]
 
 
+# gw_query as a numpy record: (qid, source, walks)
+QUERY_DTYPE = np.dtype([("qid", np.uint64), ("source", np.uint32), ("walks", np.uint32)])
+
+
 class Query(C.Structure):
     _fields_ = [("qid", C.c_uint64), ("source", C.c_uint32), ("walks", C.c_uint32)]
 
@@ -575,7 +579,14 @@ class Engine:
         self._serve_k = 0
 
     def serve_submit(self, queries) -> None:
-        """queries: iterable of (qid, source, walks)."""
+        """queries: iterable of (qid, source, walks), or a numpy array of
+        QUERY_DTYPE (one call, no per-query Python work)."""
+        if isinstance(queries, np.ndarray):
+            if queries.dtype != QUERY_DTYPE:
+                raise GwError(f"serve_submit: expected dtype {QUERY_DTYPE}, got {queries.dtype}")
+            a = np.ascontiguousarray(queries)
+            self._check(self._L.gw_serve_submit(self._ctx, _ptr(a), a.shape[0]), "gw_serve_submit")
+            return
         q = list(queries)
         arr = (Query * max(1, len(q)))(*[Query(int(a), int(b), int(c)) for a, b, c in q])
         self._check(self._L.gw_serve_submit(self._ctx, arr, len(q)), "gw_serve_submit")

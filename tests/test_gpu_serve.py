@@ -185,3 +185,28 @@ def test_batched_results_skip_queries_read_by_id(engine, g12):
         assert engine.serve_poll() is None and engine.serve_results()[0].shape[0] == 0
     finally:
         engine.serve_close()
+
+
+def test_submit_numpy_queries(engine, g12):
+    # queries handed over as one QUERY_DTYPE array are served like tuples
+    off, col = g12
+    engine.graph_set(off, col)  # an earlier test may have set another graph
+    engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=SEED), 100_000, 64 << 20, 64, 10)
+    try:
+        qids, srcs, walks = [7, 8, 9], [10, 20, 30], [200, 300, 400]
+        q = np.zeros(3, gw.QUERY_DTYPE)
+        q["qid"], q["source"], q["walks"] = qids, srcs, walks
+        engine.serve_submit(q)
+        for _ in range(50):
+            engine.serve_step()
+        got = {}
+        while (r := engine.serve_poll()) is not None:
+            got[r[0]] = engine.serve_result(*r)
+        assert sorted(got) == qids
+        for qid, src, w in zip(qids, srcs, walks):
+            ev, ec = expected(off, col, gw.RESTART, 10, qid, src, w)
+            assert np.array_equal(got[qid][0], ev) and np.array_equal(got[qid][1], ec)
+        with pytest.raises(gw.GwError, match="dtype"):
+            engine.serve_submit(np.zeros(2, np.uint64))
+    finally:
+        engine.serve_close()
