@@ -624,14 +624,15 @@ class Engine:
         if k <= 0:
             raise GwError("serve_results needs serve_topk(k > 0)")
         buf = getattr(self, "_serve_buf", None)
-        if buf is None or buf[0].shape[0] < max_q:
+        if buf is None or buf[0].shape[0] < max_q or buf[2].shape[0] < max_q * k:
             buf = (np.empty(max_q, np.uint64), np.empty(max_q, np.uint64),
                    np.empty(max_q * k, np.uint32), np.empty(max_q * k, np.uint32))
             self._serve_buf = buf
         q, npairs, v, c = buf
         n = C.c_uint32()
+        # capacities from the arrays themselves: never more than they hold
         self._check(self._L.gw_serve_results(self._ctx, max_q, _ptr(q), _ptr(npairs), _ptr(v), _ptr(c),
-                                             max_q * k, C.byref(n)), "gw_serve_results")
+                                             v.shape[0], C.byref(n)), "gw_serve_results")
         m = n.value
         tot = int(npairs[:m].sum())
         return q[:m].copy(), npairs[:m].copy(), v[:tot].copy(), c[:tot].copy()
