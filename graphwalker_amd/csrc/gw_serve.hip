@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <deque>
 #include <map>
 #include <numeric>
@@ -87,7 +88,7 @@ struct ServeState {
     uint32_t* h_tn = nullptr;              // max_queries
     unsigned long long* h_top = nullptr;   // max_queries x topk (gw_serve_topk)
     std::vector<Active> active;
-    std::vector<uint32_t> free_slots;
+    std::vector<uint32_t> free_slots;  // a min-heap: the lowest free slot is taken first
     std::vector<std::pair<uint64_t, uint64_t>> free_ranges;  // arena [start, size), sorted by start
     std::deque<gw_query> queue;
     std::map<uint64_t, std::vector<unsigned long long>> done;  // qid -> packed (vertex << cbits | count)
@@ -923,7 +924,7 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
     }
     GW_HIP(ctx, hipMemsetAsync(S->arena, 0, S->arena_slots * 8, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (uint32_t i = max_queries; i-- > 0;) S->free_slots.push_back(i);
+    for (uint32_t i = 0; i < max_queries; ++i) S->free_slots.push_back(i);  // ascending: already a min-heap
     S->free_ranges.push_back({0, S->arena_slots});
     ctx->serve_state = S;
     return GW_OK;
@@ -981,6 +982,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         const uint64_t size = region_slots(q.walks, L, ctx->n), total = query_slots(q.walks, L, ctx->n);
         uint64_t base = 0;
         if (total > S.arena_slots || !arena_alloc(S, total, base)) break;
+        std::pop_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
         const uint32_t slot = S.free_slots.back();
         S.free_slots.pop_back();
         const SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
@@ -1021,9 +1023,16 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     const unsigned long long* cnt = S.h_cnt;
     const uint32_t* rem = S.h_rem;
     const unsigned long long* nkeys = S.h_kcur;
+    // the remaining walks and key cursors of the slots in use only: slots are
+    // handed out lowest first, so they sit below a small bound (the copies'
+    // latency grows with their size)
+    uint32_t hi = 0;
+    for (const auto& a : S.active) hi = std::max(hi, a.slot + 1);
     GW_HIP(ctx, hipMemcpyAsync(S.h_cnt, S.d_cnt, 4 * 8, hipMemcpyDeviceToHost, ctx->stream));
-    GW_HIP(ctx, hipMemcpyAsync(S.h_rem, S.d_remaining, S.max_queries * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-    GW_HIP(ctx, hipMemcpyAsync(S.h_kcur, S.d_kcur, S.max_queries * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    if (hi) {
+        GW_HIP(ctx, hipMemcpyAsync(S.h_rem, S.d_remaining, hi * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+        GW_HIP(ctx, hipMemcpyAsync(S.h_kcur, S.d_kcur, hi * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    }
     if (S.ooc && S.npages)
         GW_HIP(ctx, hipMemcpyAsync(S.hits.data(), S.d_page_hits, S.npages * 4, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1130,6 +1139,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                 S.done_order.push_back(a.qid);
                 arena_free(S, a.base, a.total);
                 S.free_slots.push_back(a.slot);
+                std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
             }
         } else {
             std::vector<unsigned long long> per(nd);
@@ -1154,6 +1164,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                 S.done_order.push_back(a.qid);
                 arena_free(S, a.base, a.total);
                 S.free_slots.push_back(a.slot);
+                std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
             }
         }
         st.completed = nd;

@@ -415,6 +415,24 @@ def test_source_counts_partitioned_large_regions_exact(engine, schedule, layout)
     assert region.max() > 12288 and sinks.shape[0] > 0 and (region[5:] == 0).all()
 
 
+def test_source_counts_partitioned_all_sources_without_keys(engine):
+    # every source is a vertex with no out-edges: no source has region keys,
+    # the classify kernel writes each one's single pair and the aggregation
+    # workgroups find an empty work list
+    off, col = O.rmat_graph(10, 1 << 14, seed=6)
+    n = off.shape[0] - 1
+    engine.graph_set(off, col)
+    sources = np.nonzero(np.diff(off) == 0)[0][:40].astype(np.uint32)
+    field = np.arange(sources.shape[0], dtype=np.uint32)
+    engine.walks_init_sources(sources, 50, source_field=field)
+    prog = gw.program(gw.STOP, 20, 0.15, n, seed=7, recorder=gw.REC_SOURCE_COUNT | gw.REC_NO_FINAL,
+                      schedule="persistent", layout="ext", source_method=gw.SRC_PARTITIONED)
+    st = engine.run(prog)
+    s_, v_, c_ = engine.source_counts()
+    assert np.array_equal(s_, field) and np.array_equal(v_, sources) and (c_ == 50).all()
+    assert st["visits"] == 50 * sources.shape[0] and st["flushed_sources"] == 0
+
+
 def test_source_counts_region_overflow_falls_back(engine):
     # PROB walks are uncapped; regions are sized for 64 visits per walk.  Source
     # field 0 starts on a sink-free cycle (about 128 visits per walk at p = 1/128),
