@@ -66,7 +66,6 @@ struct ServeState {
     unsigned long long* d_cnt = nullptr;  // [0] out cursor [1] steps [2] visits [3] overflow [4] compact n
     unsigned long long* d_pairs = nullptr;  // compaction output
     uint64_t pairs_cap = 0;
-    Admit* d_admit = nullptr;               // max_queries admission records
     Done* d_done = nullptr;                 // max_queries completion records
     unsigned long long* d_region_n = nullptr;  // max_queries: entries of each completed query
     // per query slot: its key region in the arena (u32 index range) and cursor
@@ -84,9 +83,15 @@ struct ServeState {
     unsigned long long* h_cnt = nullptr;   // 8 counters
     uint32_t* h_rem = nullptr;             // max_queries
     unsigned long long* h_kcur = nullptr;  // max_queries
-    unsigned long long* h_coff = nullptr;  // max_queries
     uint32_t* h_tn = nullptr;              // max_queries
     unsigned long long* h_top = nullptr;   // max_queries x topk (gw_serve_topk)
+    // the device addresses of the pinned buffers the kernels read and write
+    Admit* dh_adm = nullptr;
+    unsigned long long* dh_cnt = nullptr;
+    uint32_t* dh_rem = nullptr;
+    unsigned long long* dh_kcur = nullptr;
+    uint32_t* dh_tn = nullptr;
+    unsigned long long* dh_top = nullptr;
     std::vector<Active> active;
     std::vector<uint32_t> free_slots;  // a min-heap: the lowest free slot is taken first
     std::vector<std::pair<uint64_t, uint64_t>> free_ranges;  // arena [start, size), sorted by start
@@ -94,11 +99,8 @@ struct ServeState {
     std::map<uint64_t, std::vector<unsigned long long>> done;  // qid -> packed (vertex << cbits | count)
     std::deque<uint64_t> done_order;
     uint32_t topk = 0;                         // > 0: completed queries keep their top-k pairs only
-    unsigned long long* d_topk = nullptr;      // max_queries x topk packed pairs
-    uint32_t* d_topk_n = nullptr;              // max_queries
     unsigned long long* d_cand = nullptr;      // per-chunk top-k candidates of the completing queries
     uint64_t cand_cap = 0;                     // keys
-    unsigned long long* d_cand_off = nullptr;  // max_queries: first candidate of each completing query
     std::set<uint64_t> qids;                   // queued, in flight, or completed and not yet read
     bool failed = false;                       // a step failed after its kernel ran: only close is allowed
     // host-resident graph: offsets in HBM, columns paged
@@ -372,6 +374,7 @@ struct Admit {
 
 // One workgroup per admitted query: writes its slot record and remaining
 // count, then its walk records (id = walk index, source field = slot).
+// Reads the admission records from pinned host memory (no copy to enqueue).
 __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __restrict__ pool,
                                    SlotInfo* __restrict__ slots, uint32_t* __restrict__ remaining,
                                    uint32_t* __restrict__ srccount, uint32_t* __restrict__ occn,
@@ -394,12 +397,30 @@ __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __res
         pool[a.at + i] = gw_walk{i, a.slot, a.info.source, 0u};
 }
 
+// After the walks of a serve step: the counters, and the remaining walks
+// and key cursors of the slots below `hi`, into pinned host memory (one
+// launch in place of a copy each).
+__global__ __launch_bounds__(256) void serve_export_kernel(const unsigned long long* __restrict__ cnt,
+                                                           const uint32_t* __restrict__ rem,
+                                                           const unsigned long long* __restrict__ kcur, uint32_t hi,
+                                                           unsigned long long* __restrict__ o_cnt,
+                                                           uint32_t* __restrict__ o_rem,
+                                                           unsigned long long* __restrict__ o_kcur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) o_cnt[i] = cnt[i];
+    for (uint32_t j = i; j < hi; j += gridDim.x * blockDim.x) {
+        o_rem[j] = rem[j];
+        o_kcur[j] = kcur[j];
+    }
+}
+
 // A completed query's count region and where its pairs go.
 struct Done {
     uint64_t base, size;   // hash region
     uint64_t out;          // first output index: room for nkeys pairs from there
     uint32_t slot, source; // the query's slot (its source-vertex counter) and source
     uint64_t kbase, nkeys; // its key region (u32 index into the arena) and the keys in it
+    uint64_t cand;         // top-k: its first chunk candidate (the chunks' candidates back to back)
 };
 
 // A completed query's keys, folded into its hash region: one workgroup per
@@ -559,7 +580,6 @@ __global__ __launch_bounds__(1024) void serve_topk_chunk_kernel(const unsigned l
                                                               const unsigned long long* __restrict__ per_region,
                                                               const uint32_t* __restrict__ srccount,
                                                               uint32_t cbits, uint32_t k,
-                                                              const unsigned long long* __restrict__ cand_off,
                                                               unsigned long long* __restrict__ cand) {
     __shared__ TopkScratch t;
     const uint32_t q = blockIdx.y;
@@ -580,19 +600,25 @@ __global__ __launch_bounds__(1024) void serve_topk_chunk_kernel(const unsigned l
             return topk_key(static_cast<uint32_t>(x >> cbits), static_cast<uint32_t>(x & cmask));
         },
         t);
-    unsigned long long* o = cand + cand_off[q] + c * k;
+    unsigned long long* o = cand + d.cand + c * k;
     for (uint32_t r = threadIdx.x; r < k; r += blockDim.x) o[r] = r < m ? t.sel[r] : 0ull;
 }
 
+// The results go straight into pinned host memory (out, out_n, *fold_copy):
+// the host reads them after its synchronisation, with no copy to enqueue.
 __global__ __launch_bounds__(1024) void serve_topk_merge_kernel(const unsigned long long* __restrict__ per_region,
-                                                              const unsigned long long* __restrict__ cand_off,
+                                                              const Done* __restrict__ done,
                                                               const unsigned long long* __restrict__ cand,
                                                               uint32_t cbits, uint32_t k,
                                                               unsigned long long* __restrict__ out,
-                                                              uint32_t* __restrict__ out_n) {
+                                                              uint32_t* __restrict__ out_n,
+                                                              const unsigned long long* __restrict__ fold_over,
+                                                              unsigned long long* __restrict__ fold_copy) {
     __shared__ TopkScratch t;
     const uint32_t q = blockIdx.x;
-    const unsigned long long* c = cand + cand_off[q];
+    // the fold ran before this kernel: its overflow count travels with the results
+    if (q == 0 && threadIdx.x == 0) *fold_copy = *fold_over;
+    const unsigned long long* c = cand + done[q].cand;
     const uint32_t m = block_topk(topk_chunks(per_region[q]) * k, k, [&](uint64_t i) { return c[i]; }, t);
     for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
         const unsigned long long key = t.sel[r];
@@ -866,7 +892,6 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         (rc = gw_alloc(ctx, (void**)&S->d_remaining, max_queries * 4ull, "serve remaining")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_srccount, max_queries * 4ull, "serve source counts")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_cnt, 8 * 8, "serve counters")) ||
-        (rc = gw_alloc(ctx, (void**)&S->d_admit, max_queries * sizeof(Admit), "serve admissions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_done, max_queries * sizeof(Done), "serve completions")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_region_n, max_queries * 8ull, "serve region counts")) ||
         (rc = gw_alloc(ctx, (void**)&S->d_koff, max_queries * 8ull, "serve key regions")) ||
@@ -885,13 +910,15 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
     }
     {  // pinned staging of the per-step copies
         const uint64_t mq = max_queries;
-        const uint64_t sizes[] = {mq * sizeof(Admit), mq * sizeof(Done), 8 * 8, mq * 4, mq * 8, mq * 8, mq * 4};
-        uint64_t at[7], total = 0;
-        for (int i = 0; i < 7; ++i) {
+        const uint64_t sizes[] = {mq * sizeof(Admit), mq * sizeof(Done), 8 * 8, mq * 4, mq * 8, mq * 4};
+        uint64_t at[6], total = 0;
+        for (int i = 0; i < 6; ++i) {
             at[i] = total;
             total += (sizes[i] + 63) / 64 * 64;
         }
-        if (hipHostMalloc(&S->h_block, total, hipHostMallocDefault) != hipSuccess) {
+        // coherent: kernels read the admissions and write the counters here,
+        // and the host reads the kernels' writes after synchronising
+        if (hipHostMalloc(&S->h_block, total, hipHostMallocCoherent) != hipSuccess) {
             S->h_block = nullptr;
             gw_serve_close_state(ctx, S);
             return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: pinned staging buffer");
@@ -902,8 +929,17 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         S->h_cnt = reinterpret_cast<unsigned long long*>(b + at[2]);
         S->h_rem = reinterpret_cast<uint32_t*>(b + at[3]);
         S->h_kcur = reinterpret_cast<unsigned long long*>(b + at[4]);
-        S->h_coff = reinterpret_cast<unsigned long long*>(b + at[5]);
-        S->h_tn = reinterpret_cast<uint32_t*>(b + at[6]);
+        S->h_tn = reinterpret_cast<uint32_t*>(b + at[5]);
+        char* db = nullptr;
+        if (hipHostGetDevicePointer(reinterpret_cast<void**>(&db), S->h_block, 0) != hipSuccess || !db) {
+            gw_serve_close_state(ctx, S);
+            return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: device address of the pinned staging buffer");
+        }
+        S->dh_adm = reinterpret_cast<Admit*>(db + at[0]);
+        S->dh_cnt = reinterpret_cast<unsigned long long*>(db + at[2]);
+        S->dh_rem = reinterpret_cast<uint32_t*>(db + at[3]);
+        S->dh_kcur = reinterpret_cast<unsigned long long*>(db + at[4]);
+        S->dh_tn = reinterpret_cast<uint32_t*>(db + at[5]);
     }
     if (ctx->host_graph && (rc = ooc_open(ctx, *S))) {
         gw_serve_close_state(ctx, S);
@@ -994,11 +1030,11 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         S.queue.pop_front();
     }
     if (!adm.empty()) {
+        // the admit kernel reads the records from pinned memory; the previous
+        // step's kernels are done with it (the step ended synchronised)
         std::copy(adm.begin(), adm.end(), S.h_adm);
-        GW_HIP(ctx, hipMemcpyAsync(S.d_admit, S.h_adm, adm.size() * sizeof(Admit), hipMemcpyHostToDevice,
-                                   ctx->stream));
         serve_admit_kernel<<<dim3(static_cast<uint32_t>(adm.size())), dim3(256), 0, ctx->stream>>>(
-            S.d_admit, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount, S.d_occn, S.ooc ? S.d_off : ctx->d_off,
+            S.dh_adm, S.pool[S.cur], S.d_slots, S.d_remaining, S.d_srccount, S.d_occn, S.ooc ? S.d_off : ctx->d_off,
             S.d_koff, S.d_kend, S.d_kcur);
         GW_HIP(ctx, hipGetLastError());
         st.admitted = static_cast<uint32_t>(adm.size());
@@ -1028,11 +1064,9 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     // latency grows with their size)
     uint32_t hi = 0;
     for (const auto& a : S.active) hi = std::max(hi, a.slot + 1);
-    GW_HIP(ctx, hipMemcpyAsync(S.h_cnt, S.d_cnt, 4 * 8, hipMemcpyDeviceToHost, ctx->stream));
-    if (hi) {
-        GW_HIP(ctx, hipMemcpyAsync(S.h_rem, S.d_remaining, hi * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-        GW_HIP(ctx, hipMemcpyAsync(S.h_kcur, S.d_kcur, hi * 8ull, hipMemcpyDeviceToHost, ctx->stream));
-    }
+    serve_export_kernel<<<dim3(std::max(1u, std::min(64u, (hi + 255) / 256))), dim3(256), 0, ctx->stream>>>(
+        S.d_cnt, S.d_remaining, S.d_kcur, hi, S.dh_cnt, S.dh_rem, S.dh_kcur);
+    GW_HIP(ctx, hipGetLastError());
     if (S.ooc && S.npages)
         GW_HIP(ctx, hipMemcpyAsync(S.hits.data(), S.d_page_hits, S.npages * 4, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1063,14 +1097,18 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     if (!fin.empty()) {
         const uint32_t nd = static_cast<uint32_t>(fin.size());
         std::vector<Done> dn(nd);
-        uint64_t total = 0, maxkeys = 0;
+        uint64_t total = 0, maxkeys = 0, ncand = 0, maxc = 1;
         for (uint32_t i = 0; i < nd; ++i) {
             const auto& a = fin[i];
             // the cursor never passes the region (an overflow fails the step above)
             const uint64_t nk = std::min<uint64_t>(nkeys[a.slot], static_cast<uint64_t>(a.walks) * L);
-            dn[i] = Done{a.base, a.size, total, a.slot, a.source, 2 * (a.base + a.size), nk};
+            // top-k: the query's chunks' candidates follow the previous query's
+            const uint64_t ch = (nk + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
+            dn[i] = Done{a.base, a.size, total, a.slot, a.source, 2 * (a.base + a.size), nk, ncand};
             total += nk;
             maxkeys = std::max<uint64_t>(maxkeys, nk);
+            ncand += ch * S.topk;
+            maxc = std::max(maxc, ch);
         }
         if (total > S.pairs_cap) {
             gw_free(ctx, S.d_pairs, S.pairs_cap * 8);
@@ -1095,14 +1133,6 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         unsigned long long fold_overflow = 0;
         if (S.topk) {
             // top-k per completed query on the device: k pairs per query cross PCIe
-            std::vector<unsigned long long> coff(nd);
-            uint64_t ncand = 0, maxc = 1;
-            for (uint32_t i = 0; i < nd; ++i) {
-                const uint64_t ch = (dn[i].nkeys + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
-                coff[i] = ncand;
-                ncand += ch * S.topk;
-                maxc = std::max(maxc, ch);
-            }
             if (ncand > S.cand_cap) {
                 gw_free(ctx, S.d_cand, S.cand_cap * 8);
                 S.d_cand = nullptr;
@@ -1112,20 +1142,16 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                 if (rc) return rc;
                 S.cand_cap = cap;
             }
-            std::copy(coff.begin(), coff.end(), S.h_coff);
-            GW_HIP(ctx, hipMemcpyAsync(S.d_cand_off, S.h_coff, nd * 8ull, hipMemcpyHostToDevice, ctx->stream));
             serve_topk_chunk_kernel<<<dim3(static_cast<uint32_t>(maxc), nd), dim3(1024), 0, ctx->stream>>>(
-                S.d_pairs, S.d_done, S.d_region_n, S.d_srccount, S.cbits, S.topk, S.d_cand_off, S.d_cand);
+                S.d_pairs, S.d_done, S.d_region_n, S.d_srccount, S.cbits, S.topk, S.d_cand);
             GW_HIP(ctx, hipGetLastError());
-            serve_topk_merge_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_region_n, S.d_cand_off, S.d_cand,
-                                                                            S.cbits, S.topk, S.d_topk, S.d_topk_n);
+            // results, pair counts and the fold's overflow count land in pinned memory
+            serve_topk_merge_kernel<<<dim3(nd), dim3(1024), 0, ctx->stream>>>(S.d_region_n, S.d_done, S.d_cand,
+                                                                            S.cbits, S.topk, S.dh_top, S.dh_tn,
+                                                                            S.d_cnt + 4, S.dh_cnt + 4);
             GW_HIP(ctx, hipGetLastError());
             const unsigned long long* top = S.h_top;
             const uint32_t* tn = S.h_tn;
-            GW_HIP(ctx, hipMemcpyAsync(S.h_top, S.d_topk, static_cast<uint64_t>(nd) * S.topk * 8, hipMemcpyDeviceToHost,
-                                       ctx->stream));
-            GW_HIP(ctx, hipMemcpyAsync(S.h_tn, S.d_topk_n, nd * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-            GW_HIP(ctx, hipMemcpyAsync(S.h_cnt + 4, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
             GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
             fold_overflow = S.h_cnt[4];
             if (fold_overflow) {
@@ -1259,32 +1285,29 @@ int gw_serve_topk(gw_ctx* ctx, uint32_t k) {
     ServeState& S = *sp;
     GW_HIP(ctx, hipSetDevice(ctx->device));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    gw_free(ctx, S.d_topk, static_cast<uint64_t>(S.max_queries) * S.topk * 8);
-    gw_free(ctx, S.d_topk_n, S.max_queries * 4ull);
     gw_free(ctx, S.d_cand, S.cand_cap * 8);
-    gw_free(ctx, S.d_cand_off, S.max_queries * 8ull);
-    S.d_topk = nullptr;
-    S.d_topk_n = nullptr;
-    S.d_cand = S.d_cand_off = nullptr;
+    S.d_cand = nullptr;
     if (S.h_top) (void)hipHostFree(S.h_top);
-    S.h_top = nullptr;
+    S.h_top = S.dh_top = nullptr;
     S.cand_cap = 0;
     S.topk = 0;
     if (k == 0) return GW_OK;
     int rc;
+    // coherent: the merge kernel writes the results here
     if (hipHostMalloc(reinterpret_cast<void**>(&S.h_top), static_cast<uint64_t>(S.max_queries) * k * 8,
-                      hipHostMallocDefault) != hipSuccess) {
+                      hipHostMallocCoherent) != hipSuccess) {
         S.h_top = nullptr;
         return gw_fail(ctx, GW_ENOMEM, "gw_serve_topk: pinned result buffer");
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&S.dh_top), S.h_top, 0) != hipSuccess || !S.dh_top) {
+        (void)hipHostFree(S.h_top);
+        S.h_top = S.dh_top = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_topk: device address of the pinned result buffer");
     }
     // candidates for a step whose completing queries average up to 8 chunks
     // (64 K pairs each); a step that needs more grows the buffer
     const uint64_t cand = static_cast<uint64_t>(S.max_queries) * 8 * k;
-    if ((rc = gw_alloc(ctx, (void**)&S.d_topk, static_cast<uint64_t>(S.max_queries) * k * 8, "serve top-k")) ||
-        (rc = gw_alloc(ctx, (void**)&S.d_topk_n, S.max_queries * 4ull, "serve top-k counts")) ||
-        (rc = gw_alloc(ctx, (void**)&S.d_cand_off, S.max_queries * 8ull, "serve top-k candidate offsets")) ||
-        (rc = gw_alloc(ctx, (void**)&S.d_cand, cand * 8, "serve top-k candidates")))
-        return rc;
+    if ((rc = gw_alloc(ctx, (void**)&S.d_cand, cand * 8, "serve top-k candidates"))) return rc;
     S.cand_cap = cand;
     S.topk = k;
     return GW_OK;
@@ -1324,7 +1347,6 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_srccount, S->max_queries * 4ull);
     gw_free(ctx, S->d_cnt, 64);
     gw_free(ctx, S->d_pairs, S->pairs_cap * 8);
-    gw_free(ctx, S->d_admit, S->max_queries * sizeof(Admit));
     gw_free(ctx, S->d_done, S->max_queries * sizeof(Done));
     gw_free(ctx, S->d_region_n, S->max_queries * 8ull);
     if (S->h_block) (void)hipHostFree(S->h_block);
@@ -1332,9 +1354,6 @@ static void gw_serve_close_state(gw_ctx* ctx, gw::ServeState* S) {
     gw_free(ctx, S->d_koff, S->max_queries * 8ull);
     gw_free(ctx, S->d_kend, S->max_queries * 8ull);
     gw_free(ctx, S->d_kcur, S->max_queries * 8ull);
-    gw_free(ctx, S->d_topk, static_cast<uint64_t>(S->max_queries) * S->topk * 8);
-    gw_free(ctx, S->d_topk_n, S->max_queries * 4ull);
     gw_free(ctx, S->d_cand, S->cand_cap * 8);
-    gw_free(ctx, S->d_cand_off, S->max_queries * 8ull);
     delete S;
 }
