@@ -399,8 +399,9 @@ __global__ void serve_admit_kernel(const Admit* __restrict__ adm, gw_walk* __res
 
 // After the walks of a serve step: the counters, and the remaining walks
 // and key cursors of the slots below `hi`, into pinned host memory (one
-// launch in place of a copy each).
-__global__ __launch_bounds__(256) void serve_export_kernel(const unsigned long long* __restrict__ cnt,
+// launch in place of a copy each); the 8 counters are then zeroed for the
+// next step (the fold adds its overflow count to counter 4 afterwards).
+__global__ __launch_bounds__(256) void serve_export_kernel(unsigned long long* __restrict__ cnt,
                                                            const uint32_t* __restrict__ rem,
                                                            const unsigned long long* __restrict__ kcur, uint32_t hi,
                                                            unsigned long long* __restrict__ o_cnt,
@@ -408,6 +409,7 @@ __global__ __launch_bounds__(256) void serve_export_kernel(const unsigned long l
                                                            unsigned long long* __restrict__ o_kcur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 4) o_cnt[i] = cnt[i];
+    if (i < 8) cnt[i] = 0;
     for (uint32_t j = i; j < hi; j += gridDim.x * blockDim.x) {
         o_rem[j] = rem[j];
         o_kcur[j] = kcur[j];
@@ -959,6 +961,7 @@ int gw_serve_open(gw_ctx* ctx, const gw_program* prog, uint64_t pool_walks, uint
         S->grid_max = std::max(1u, ctx->cu_count) * static_cast<uint32_t>(std::min(per_cu, 8));
     }
     GW_HIP(ctx, hipMemsetAsync(S->arena, 0, S->arena_slots * 8, ctx->stream));
+    GW_HIP(ctx, hipMemsetAsync(S->d_cnt, 0, 8 * 8, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t i = 0; i < max_queries; ++i) S->free_slots.push_back(i);  // ascending: already a min-heap
     S->free_ranges.push_back({0, S->arena_slots});
@@ -1040,7 +1043,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         st.admitted = static_cast<uint32_t>(adm.size());
     }
     // 2. one step over every live walk
-    GW_HIP(ctx, hipMemsetAsync(S.d_cnt, 0, 8 * 8, ctx->stream));
+    // the counters are zero: cleared at open, then by each step's export kernel
     GW_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (S.live) {
         const Prog p{L, S.prog.nvertices, S.prog.stop_thr, static_cast<uint32_t>(S.prog.seed),
