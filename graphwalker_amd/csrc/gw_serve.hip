@@ -112,7 +112,8 @@ struct ServeState {
     const uint32_t** h_pages = nullptr;        // pinned page table: rebased slot pointer or host_col
     const uint32_t** d_pages = nullptr;
     uint32_t* d_page_hits = nullptr;           // sampled column reads per page in the last step
-    std::vector<uint32_t> hits;                // read back after each step
+    uint32_t* hits = nullptr;                  // pinned (coherent): written by each step's export kernel
+    uint32_t* dh_hits = nullptr;               // its device address
     std::vector<double> score;                 // decayed demand
     std::vector<int32_t> page_slot;            // slot holding the page, -1 = read from host memory
     std::vector<int64_t> slot_page;            // page in (or loading into) the slot, -1 = free
@@ -406,13 +407,21 @@ __global__ __launch_bounds__(256) void serve_export_kernel(unsigned long long* _
                                                            const unsigned long long* __restrict__ kcur, uint32_t hi,
                                                            unsigned long long* __restrict__ o_cnt,
                                                            uint32_t* __restrict__ o_rem,
-                                                           unsigned long long* __restrict__ o_kcur) {
+                                                           unsigned long long* __restrict__ o_kcur,
+                                                           uint32_t* __restrict__ hits, uint32_t* __restrict__ o_hits,
+                                                           uint32_t npages) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
     if (i < 4) o_cnt[i] = cnt[i];
     if (i < 8) cnt[i] = 0;
-    for (uint32_t j = i; j < hi; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = i; j < hi; j += stride) {
         o_rem[j] = rem[j];
         o_kcur[j] = kcur[j];
+    }
+    // host-resident graph: the step's sampled reads per page, then cleared
+    for (uint32_t j = i; j < npages; j += stride) {
+        o_hits[j] = hits[j];
+        hits[j] = 0;
     }
 }
 
@@ -718,7 +727,13 @@ int ooc_open(gw_ctx* ctx, ServeState& S) {
         (rc = gw_alloc(ctx, (void**)&S.d_page_hits, np * 4, "serve page demand")))
         return rc;
     if (S.nslots && (rc = gw_alloc(ctx, (void**)&S.d_cache, S.nslots * page_bytes, "serve page cache"))) return rc;
-    S.hits.assign(S.npages, 0);
+    if (hipHostMalloc(reinterpret_cast<void**>(&S.hits), np * 4, hipHostMallocCoherent) != hipSuccess) {
+        S.hits = nullptr;
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: pinned page demand buffer");
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&S.dh_hits), S.hits, 0) != hipSuccess || !S.dh_hits)
+        return gw_fail(ctx, GW_ENOMEM, "gw_serve_open: device address of the page demand buffer");
+    std::fill(S.hits, S.hits + np, 0u);
     S.score.assign(S.npages, 0.0);
     S.page_slot.assign(S.npages, -1);
     S.loading.assign(S.npages, 0);
@@ -834,7 +849,7 @@ int ooc_before_step(gw_ctx* ctx, ServeState& S, uint64_t* started, uint32_t* nst
         S.h2d_bytes += *started;
         S.pages_loaded += *nstarted;
     }
-    if (S.npages) GW_HIP(ctx, hipMemsetAsync(S.d_page_hits, 0, S.npages * 4, ctx->stream));
+    // the demand counts were read out and cleared by the last step's export kernel
     return GW_OK;
 }
 
@@ -846,6 +861,8 @@ void ooc_close(gw_ctx* ctx, ServeState& S) {
     S.free_events.clear();
     if (S.table_ev) (void)hipEventDestroy(S.table_ev);
     if (S.h_pages) (void)hipHostFree(S.h_pages);
+    if (S.hits) (void)hipHostFree(S.hits);
+    S.hits = S.dh_hits = nullptr;
     const uint64_t np = std::max<uint64_t>(1, S.npages);
     gw_free(ctx, S.d_pages, np * sizeof(void*));
     gw_free(ctx, S.d_page_hits, np * 4);
@@ -1067,11 +1084,11 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     // latency grows with their size)
     uint32_t hi = 0;
     for (const auto& a : S.active) hi = std::max(hi, a.slot + 1);
-    serve_export_kernel<<<dim3(std::max(1u, std::min(64u, (hi + 255) / 256))), dim3(256), 0, ctx->stream>>>(
-        S.d_cnt, S.d_remaining, S.d_kcur, hi, S.dh_cnt, S.dh_rem, S.dh_kcur);
+    const uint32_t np = S.ooc ? static_cast<uint32_t>(S.npages) : 0u;
+    serve_export_kernel<<<dim3(std::max(1u, std::min(64u, (std::max(hi, np) + 255) / 256))), dim3(256), 0,
+                          ctx->stream>>>(S.d_cnt, S.d_remaining, S.d_kcur, hi, S.dh_cnt, S.dh_rem, S.dh_kcur,
+                                         S.d_page_hits, S.dh_hits, np);
     GW_HIP(ctx, hipGetLastError());
-    if (S.ooc && S.npages)
-        GW_HIP(ctx, hipMemcpyAsync(S.hits.data(), S.d_page_hits, S.npages * 4, hipMemcpyDeviceToHost, ctx->stream));
     GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
