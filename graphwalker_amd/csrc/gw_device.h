@@ -756,9 +756,9 @@ __device__ __forceinline__ void workgroup_add2(unsigned long long* d0, unsigned 
 // pass counts the items and finds the largest key, the byte passes start at
 // its highest non-zero byte and stop as soon as the chosen bucket holds
 // exactly the keys still needed.  The keys at or above the threshold are
-// gathered into sel[] and bitonic-sorted descending.  Returns min(k,
-// non-empty items); every thread of the block calls it and gets the same
-// value.
+// gathered into sel[] and (kSorted) bitonic-sorted descending.  Returns
+// min(k, non-empty items); every thread of the block calls it and gets the
+// same value.
 constexpr uint32_t kTopkMax = 1024;
 
 struct TopkScratch {
@@ -769,8 +769,30 @@ struct TopkScratch {
     unsigned long long sel[kTopkMax];
 };
 
-template <class KeyAt>
+template <bool kSorted = true, uint32_t kCache = 0, class KeyAt>
 __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch& t) {
+    // With kCache, a block of at most kCache x blockDim items keeps its keys in
+    // registers: one read per item instead of one per pass.  `each` visits a
+    // thread's items in index order either way.
+    constexpr uint32_t kC = kCache ? kCache : 1;
+    unsigned long long kc[kC];
+    const bool cached = kCache && n <= static_cast<uint64_t>(kCache) * blockDim.x;
+    if (cached) {
+#pragma unroll
+        for (uint32_t j = 0; j < kC; ++j) {
+            const uint64_t i = threadIdx.x + static_cast<uint64_t>(j) * blockDim.x;
+            kc[j] = i < n ? key_at(i) : 0ull;
+        }
+    }
+    auto each = [&](auto&& body) {
+        if (cached) {
+#pragma unroll
+            for (uint32_t j = 0; j < kC; ++j)
+                if (threadIdx.x + static_cast<uint64_t>(j) * blockDim.x < n) body(kc[j]);
+        } else {
+            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) body(key_at(i));
+        }
+    };
     if (threadIdx.x == 0) {
         t.valid = 0;
         t.fill = 0;
@@ -782,11 +804,10 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
     __syncthreads();
     uint32_t v = 0;
     unsigned long long mx = 0;
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const unsigned long long key = key_at(i);
+    each([&](unsigned long long key) {
         v += key != 0ull;
         mx = key > mx ? key : mx;
-    }
+    });
     if (v) atomicAdd(&t.valid, v);
     if (mx) atomicMax(&t.maxkey, mx);
     __syncthreads();
@@ -804,8 +825,7 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
             // share a bin: a thread folds runs of one bin in registers and adds
             // each run with one LDS atomic.
             uint32_t cur = 0, run = 0;
-            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const unsigned long long key = key_at(i);
+            each([&](unsigned long long key) {
                 if (key != 0ull && (shift == top || (key >> (shift + 8)) == prefix)) {
                     const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
                     if (d == cur) {
@@ -816,7 +836,7 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
                         run = 1;
                     }
                 }
-            }
+            });
             if (run) atomicAdd(&t.hist[cur], run);
             __syncthreads();
             // The bin that holds the remaining-th largest key: the first wavefront
@@ -855,12 +875,12 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
         thr = t.shift_done >= 0 ? t.prefix << t.shift_done : t.prefix;
         if (thr == 0) thr = 1;
     }
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const unsigned long long key = key_at(i);
+    each([&](unsigned long long key) {
         if (key != 0ull && key >= thr) t.sel[atomicAdd(&t.fill, 1u)] = key;
-    }
+    });
     __syncthreads();
     const uint32_t m = t.fill;
+    if (!kSorted) return m;  // the caller takes the set, in no order
     uint32_t size = 1;
     while (size < m) size <<= 1;
     for (uint32_t i = m + threadIdx.x; i < size; i += blockDim.x) t.sel[i] = 0;

@@ -602,7 +602,8 @@ __global__ __launch_bounds__(1024) void serve_topk_chunk_kernel(const unsigned l
     const uint32_t sc = srccount[d.slot];
     const uint64_t lo = c * kServeTopkChunk;
     const uint64_t cnt = std::min<uint64_t>(kServeTopkChunk, n + 1 - lo);
-    const uint32_t m = block_topk(
+    // a chunk's candidates need no order: the merge selects and sorts
+    const uint32_t m = block_topk<false, kServeTopkChunk / 1024>(
         cnt, k,
         [&](uint64_t i) {
             const uint64_t j = lo + i;
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(1024) void serve_topk_merge_kernel(const unsigned l
     // the fold ran before this kernel: its overflow count travels with the results
     if (q == 0 && threadIdx.x == 0) *fold_copy = *fold_over;
     const unsigned long long* c = cand + done[q].cand;
-    const uint32_t m = block_topk(topk_chunks(per_region[q]) * k, k, [&](uint64_t i) { return c[i]; }, t);
+    const uint32_t m = block_topk<true, 16>(topk_chunks(per_region[q]) * k, k, [&](uint64_t i) { return c[i]; }, t);
     for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
         const unsigned long long key = t.sel[r];
         out[static_cast<uint64_t>(q) * k + r] =
