@@ -756,7 +756,8 @@ __device__ __forceinline__ void workgroup_add2(unsigned long long* d0, unsigned 
 // pass counts the items and finds the largest key, the byte passes start at
 // its highest non-zero byte and stop as soon as the chosen bucket holds
 // exactly the keys still needed.  The keys at or above the threshold are
-// gathered into sel[] and (kSorted) bitonic-sorted descending.  Returns
+// gathered into sel[] and (kSorted) sorted descending (by rank for at most
+// 256 keys, else a bitonic network).  Returns
 // min(k, non-empty items); every thread of the block calls it and gets the
 // same value.
 constexpr uint32_t kTopkMax = 1024;
@@ -881,6 +882,21 @@ __device__ uint32_t block_topk(uint64_t n, uint32_t k, KeyAt key_at, TopkScratch
     __syncthreads();
     const uint32_t m = t.fill;
     if (!kSorted) return m;  // the caller takes the set, in no order
+    if (m <= 256 && m <= blockDim.x) {
+        // few keys: each goes straight to its rank (the keys above it; keys
+        // are unique), read against every key while all threads read the
+        // same one -- one barrier instead of a bitonic network's
+        unsigned long long mine = 0;
+        uint32_t rank = 0;
+        if (threadIdx.x < m) {
+            mine = t.sel[threadIdx.x];
+            for (uint32_t j = 0; j < m; ++j) rank += t.sel[j] > mine;
+        }
+        __syncthreads();
+        if (threadIdx.x < m) t.sel[rank] = mine;
+        __syncthreads();
+        return m;
+    }
     uint32_t size = 1;
     while (size < m) size <<= 1;
     for (uint32_t i = m + threadIdx.x; i < size; i += blockDim.x) t.sel[i] = 0;
