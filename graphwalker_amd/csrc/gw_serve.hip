@@ -445,15 +445,18 @@ constexpr uint32_t kFoldEmpty = 0xFFFFFFFFu;  // never a vertex (vertex ids are 
 
 __global__ __launch_bounds__(512) void serve_fold_kernel(unsigned long long* __restrict__ arena,
                                                         uint32_t* __restrict__ occ, uint32_t* __restrict__ occn,
-                                                        const Done* __restrict__ done, uint32_t cbits,
+                                                        const Done* __restrict__ done,
+                                                        const unsigned long long* __restrict__ kcur, uint32_t cbits,
                                                         unsigned long long* __restrict__ overflow) {
     __shared__ uint32_t tkey[kFoldSlots];
     __shared__ uint32_t tcnt[kFoldSlots];
     __shared__ uint32_t s_n, s_claims, s_at;
     const Done d = done[blockIdx.y];
+    // the keys written: the query's key cursor, within the record's bound
+    const uint64_t nk = std::min<uint64_t>(kcur[d.slot], d.nkeys);
     const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kFoldKeys;
-    if (lo >= d.nkeys) return;  // workgroup-uniform
-    const uint64_t hi = std::min<uint64_t>(lo + kFoldKeys, d.nkeys);
+    if (lo >= nk) return;  // workgroup-uniform
+    const uint64_t hi = std::min<uint64_t>(lo + kFoldKeys, nk);
     for (uint32_t i = threadIdx.x; i < kFoldSlots; i += blockDim.x) {
         tkey[i] = kFoldEmpty;
         tcnt[i] = 0;
@@ -1032,6 +1035,9 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
     // 1. admit queued queries in arrival order while walk slots, a query slot
     //    and arena space for their count region are free
     const uint32_t L = S.prog.L;
+    // with a top-k and hops >= L, every walk ends in the serve step that
+    // starts it: the step completes what it admits, with one synchronisation
+    const bool one_step = S.topk && S.hops >= L;
     std::vector<Admit> adm;
     while (!S.queue.empty()) {
         const gw_query& q = S.queue.front();
@@ -1090,39 +1096,48 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                           ctx->stream>>>(S.d_cnt, S.d_remaining, S.d_kcur, hi, S.dh_cnt, S.dh_rem, S.dh_kcur,
                                          S.d_page_hits, S.dh_hits, np);
     GW_HIP(ctx, hipGetLastError());
-    GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0.f;
-    GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    if (cnt[3]) {  // the walks have moved on: the server cannot be stepped again consistently
-        S.failed = true;
-        return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
-    }
-    if (S.ooc) {
-        for (uint64_t pg = 0; pg < S.npages; ++pg) {
-            st.sampled_reads += S.hits[pg];
-            if (S.page_slot[pg] >= 0) st.sampled_resident += S.hits[pg];
+    // After the walks: the step's counters (overflow, steps, visits, walks
+    // left) and, over a host graph, the page demand.
+    auto after_walks = [&]() -> int {
+        float ms = 0.f;
+        GW_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        if (cnt[3]) {  // the walks have moved on: the server cannot be stepped again consistently
+            S.failed = true;
+            return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
         }
-        for (uint64_t pg = 0; pg < S.npages; ++pg) st.pages_resident += S.page_slot[pg] >= 0;
-    }
-    st.steps = cnt[1];
-    st.visits = cnt[2];
-    st.step_ms = ms;
-    S.live = S.live ? cnt[0] : 0;
-    S.cur ^= 1;
+        if (S.ooc) {
+            for (uint64_t pg = 0; pg < S.npages; ++pg) {
+                st.sampled_reads += S.hits[pg];
+                if (S.page_slot[pg] >= 0) st.sampled_resident += S.hits[pg];
+            }
+            for (uint64_t pg = 0; pg < S.npages; ++pg) st.pages_resident += S.page_slot[pg] >= 0;
+        }
+        st.steps = cnt[1];
+        st.visits = cnt[2];
+        st.step_ms = ms;
+        S.live = S.live ? cnt[0] : 0;
+        S.cur ^= 1;
+        return GW_OK;
+    };
     // 3. completed queries: their keys are folded into their hash regions,
     //    whose entries (named by the occupied lists) are gathered back to
-    //    back, with their slots cleared for reuse.  A query's pair count is
-    //    only known on the device: its output range is sized by its key count.
-    std::vector<ServeState::Active> still, fin;
-    for (const auto& a : S.active) (rem[a.slot] != 0 ? still : fin).push_back(a);
-    if (!fin.empty()) {
+    //    back, with their slots cleared for reuse, then (with a top-k) the
+    //    top-k of each is selected.  A query's pair count is only known on
+    //    the device: its output range is sized by a bound on its key count,
+    //    the key count itself (read back) or walks x L; the fold reads the
+    //    true count on the device.
+    std::vector<Done> dn;
+    uint64_t total = 0;
+    auto enqueue_completion = [&](const std::vector<ServeState::Active>& fin, bool bound_only) -> int {
         const uint32_t nd = static_cast<uint32_t>(fin.size());
-        std::vector<Done> dn(nd);
-        uint64_t total = 0, maxkeys = 0, ncand = 0, maxc = 1;
+        dn.assign(nd, Done{});
+        total = 0;
+        uint64_t maxkeys = 0, ncand = 0, maxc = 1;
         for (uint32_t i = 0; i < nd; ++i) {
             const auto& a = fin[i];
-            // the cursor never passes the region (an overflow fails the step above)
-            const uint64_t nk = std::min<uint64_t>(nkeys[a.slot], static_cast<uint64_t>(a.walks) * L);
+            // the cursor never passes the region (an overflow fails the step)
+            const uint64_t cap = static_cast<uint64_t>(a.walks) * L;
+            const uint64_t nk = bound_only ? cap : std::min<uint64_t>(nkeys[a.slot], cap);
             // top-k: the query's chunks' candidates follow the previous query's
             const uint64_t ch = (nk + 1 + kServeTopkChunk - 1) / kServeTopkChunk;
             dn[i] = Done{a.base, a.size, total, a.slot, a.source, 2 * (a.base + a.size), nk, ncand};
@@ -1139,30 +1154,29 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
             if (rc) return rc;
             S.pairs_cap = std::max<uint64_t>(total, 1 << 20);
         }
+        if (S.topk && ncand > S.cand_cap) {
+            gw_free(ctx, S.d_cand, S.cand_cap * 8);
+            S.d_cand = nullptr;
+            S.cand_cap = 0;
+            const uint64_t cap = std::max<uint64_t>(ncand, static_cast<uint64_t>(S.max_queries) * S.topk);
+            int rc = gw_alloc(ctx, (void**)&S.d_cand, cap * 8, "serve top-k candidates");
+            if (rc) return rc;
+            S.cand_cap = cap;
+        }
         std::copy(dn.begin(), dn.end(), S.h_done);
         GW_HIP(ctx, hipMemcpyAsync(S.d_done, S.h_done, nd * sizeof(Done), hipMemcpyHostToDevice, ctx->stream));
         if (maxkeys) {
             const uint32_t fx = static_cast<uint32_t>((maxkeys + kFoldKeys - 1) / kFoldKeys);
-            serve_fold_kernel<<<dim3(fx, nd), dim3(512), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_occn, S.d_done, S.cbits,
-                                                                         S.d_cnt + 4);
+            serve_fold_kernel<<<dim3(fx, nd), dim3(512), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_occn, S.d_done,
+                                                                         S.d_kcur, S.cbits, S.d_cnt + 4);
             GW_HIP(ctx, hipGetLastError());
         }
         const uint32_t gx = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(64, (maxkeys + 255) / 256)));
         serve_gather_kernel<<<dim3(gx, nd), dim3(256), 0, ctx->stream>>>(S.arena, S.d_occ, S.d_occn, S.d_done,
                                                                          S.d_region_n, S.d_pairs);
         GW_HIP(ctx, hipGetLastError());
-        unsigned long long fold_overflow = 0;
         if (S.topk) {
             // top-k per completed query on the device: k pairs per query cross PCIe
-            if (ncand > S.cand_cap) {
-                gw_free(ctx, S.d_cand, S.cand_cap * 8);
-                S.d_cand = nullptr;
-                S.cand_cap = 0;
-                const uint64_t cap = std::max<uint64_t>(ncand, static_cast<uint64_t>(S.max_queries) * S.topk);
-                int rc = gw_alloc(ctx, (void**)&S.d_cand, cap * 8, "serve top-k candidates");
-                if (rc) return rc;
-                S.cand_cap = cap;
-            }
             serve_topk_chunk_kernel<<<dim3(static_cast<uint32_t>(maxc), nd), dim3(1024), 0, ctx->stream>>>(
                 S.d_pairs, S.d_done, S.d_region_n, S.d_srccount, S.cbits, S.topk, S.d_cand);
             GW_HIP(ctx, hipGetLastError());
@@ -1171,52 +1185,89 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
                                                                             S.cbits, S.topk, S.dh_top, S.dh_tn,
                                                                             S.d_cnt + 4, S.dh_cnt + 4);
             GW_HIP(ctx, hipGetLastError());
-            const unsigned long long* top = S.h_top;
-            const uint32_t* tn = S.h_tn;
-            GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            fold_overflow = S.h_cnt[4];
-            if (fold_overflow) {
+        }
+        return GW_OK;
+    };
+    // The completed queries' top-k, after the synchronisation that follows
+    // enqueue_completion: results kept, arena ranges and slots freed.
+    auto take_topk = [&](const std::vector<ServeState::Active>& fin) -> int {
+        if (S.h_cnt[4]) {  // the fold's overflow count, copied by the merge kernel
+            S.failed = true;
+            return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+        }
+        const unsigned long long* top = S.h_top;
+        const uint32_t* tn = S.h_tn;
+        for (uint32_t i = 0; i < fin.size(); ++i) {
+            const auto& a = fin[i];
+            const auto* first = top + static_cast<uint64_t>(i) * S.topk;
+            S.done[a.qid].assign(first, first + tn[i]);
+            S.done_order.push_back(a.qid);
+            arena_free(S, a.base, a.total);
+            S.free_slots.push_back(a.slot);
+            std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
+        }
+        st.completed = static_cast<uint32_t>(fin.size());
+        return GW_OK;
+    };
+    int rc = GW_OK;
+    if (one_step) {
+        // Every walk ends within the serve step that starts it (hops >= L,
+        // and a query is admitted whole), so this step completes exactly the
+        // queries it admitted: their completion is enqueued behind the walks
+        // and the step synchronises once.
+        const std::vector<ServeState::Active> fin(S.active);
+        if (!fin.empty() && (rc = enqueue_completion(fin, true))) return rc;
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = after_walks())) return rc;
+        for (const auto& a : fin)
+            if (rem[a.slot] != 0) {
                 S.failed = true;
-                return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+                return gw_fail(ctx, GW_ESTATE, "gw_serve_step: a query's walks outlived their serve step");
             }
-            for (uint32_t i = 0; i < nd; ++i) {
-                const auto& a = fin[i];
-                const auto* first = top + static_cast<uint64_t>(i) * S.topk;
-                S.done[a.qid].assign(first, first + tn[i]);
-                S.done_order.push_back(a.qid);
-                arena_free(S, a.base, a.total);
-                S.free_slots.push_back(a.slot);
-                std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
-            }
-        } else {
-            std::vector<unsigned long long> per(nd);
-            std::vector<unsigned long long> all(total);
-            std::vector<uint32_t> srcc(S.max_queries);
-            GW_HIP(ctx, hipMemcpyAsync(per.data(), S.d_region_n, nd * 8ull, hipMemcpyDeviceToHost, ctx->stream));
-            if (total)
-                GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
-            GW_HIP(ctx, hipMemcpyAsync(srcc.data(), S.d_srccount, S.max_queries * 4ull, hipMemcpyDeviceToHost,
-                                       ctx->stream));
-            GW_HIP(ctx, hipMemcpyAsync(&fold_overflow, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
-            GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            if (fold_overflow) {
-                S.failed = true;
-                return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
-            }
-            for (uint32_t i = 0; i < nd; ++i) {
-                const auto& a = fin[i];
-                S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
-                if (srcc[a.slot])
-                    S.done[a.qid].push_back((static_cast<unsigned long long>(a.source) << S.cbits) | srcc[a.slot]);
-                S.done_order.push_back(a.qid);
-                arena_free(S, a.base, a.total);
-                S.free_slots.push_back(a.slot);
-                std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
+        if (!fin.empty() && (rc = take_topk(fin))) return rc;
+        S.active.clear();
+    } else {
+        GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = after_walks())) return rc;
+        std::vector<ServeState::Active> still, fin;
+        for (const auto& a : S.active) (rem[a.slot] != 0 ? still : fin).push_back(a);
+        if (!fin.empty()) {
+            if ((rc = enqueue_completion(fin, false))) return rc;
+            const uint32_t nd = static_cast<uint32_t>(fin.size());
+            if (S.topk) {
+                GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                if ((rc = take_topk(fin))) return rc;
+            } else {
+                unsigned long long fold_overflow = 0;
+                std::vector<unsigned long long> per(nd);
+                std::vector<unsigned long long> all(total);
+                std::vector<uint32_t> srcc(S.max_queries);
+                GW_HIP(ctx, hipMemcpyAsync(per.data(), S.d_region_n, nd * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+                if (total)
+                    GW_HIP(ctx, hipMemcpyAsync(all.data(), S.d_pairs, total * 8, hipMemcpyDeviceToHost, ctx->stream));
+                GW_HIP(ctx, hipMemcpyAsync(srcc.data(), S.d_srccount, S.max_queries * 4ull, hipMemcpyDeviceToHost,
+                                           ctx->stream));
+                GW_HIP(ctx, hipMemcpyAsync(&fold_overflow, S.d_cnt + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
+                GW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                if (fold_overflow) {
+                    S.failed = true;
+                    return gw_fail(ctx, GW_ENOMEM, "gw_serve_step: a query's count region overflowed; close the server");
+                }
+                for (uint32_t i = 0; i < nd; ++i) {
+                    const auto& a = fin[i];
+                    S.done[a.qid].assign(all.begin() + dn[i].out, all.begin() + dn[i].out + per[i]);
+                    if (srcc[a.slot])
+                        S.done[a.qid].push_back((static_cast<unsigned long long>(a.source) << S.cbits) | srcc[a.slot]);
+                    S.done_order.push_back(a.qid);
+                    arena_free(S, a.base, a.total);
+                    S.free_slots.push_back(a.slot);
+                    std::push_heap(S.free_slots.begin(), S.free_slots.end(), std::greater<uint32_t>());
+                }
+                st.completed = nd;
             }
         }
-        st.completed = nd;
+        S.active.swap(still);
     }
-    S.active.swap(still);
     st.live_walks = S.live;
     st.active_queries = S.active.size();
     st.queued = S.queue.size();
