@@ -436,15 +436,15 @@ def serve_sim(eng: gw.Engine, n: int, qps: float, seconds: float, seed: int = 5,
                 if in_window and t_meas is None:
                     t_meas = now
                 st = eng.serve_step()
+                t_done = time.perf_counter() - t0  # the step's completed results are on the host
                 if in_window:
                     for k_ in keys:
                         acc[k_] += st[k_]
                     nsteps += 1
                     resident = st["pages_resident"]
-                t_done = time.perf_counter() - t0
                 if topk:
                     # every completed query's top-k in one call
-                    while (q := eng.serve_results(4096)[0]).shape[0]:
+                    while (q := eng.serve_results(4096, copy=False)[0]).shape[0]:
                         qi = q.astype(np.int64)
                         lat[qi] = t_done - arrivals[qi]
                         done += qi.shape[0]

@@ -615,27 +615,30 @@ class Engine:
             v, c = v[o], c[o]
         return v, c
 
-    def serve_results(self, max_q: int = 4096):
+    def serve_results(self, max_q: int = 4096, copy: bool = True):
         """Every completed query not yet read, at most max_q, in one call:
         (qids, npairs, vertices, counts) with the queries' pairs back to back
         in completion order.  Needs serve_topk(k > 0) (a query then holds at
-        most k pairs); without it, read queries with serve_poll/serve_result."""
+        most k pairs); without it, read queries with serve_poll/serve_result.
+        copy=False returns views of the engine's buffers, valid until the next
+        call (no per-call copies in a serving loop)."""
         k = getattr(self, "_serve_k", 0)
         if k <= 0:
             raise GwError("serve_results needs serve_topk(k > 0)")
         buf = getattr(self, "_serve_buf", None)
-        if buf is None or buf[0].shape[0] < max_q or buf[2].shape[0] < max_q * k:
-            buf = (np.empty(max_q, np.uint64), np.empty(max_q, np.uint64),
-                   np.empty(max_q * k, np.uint32), np.empty(max_q * k, np.uint32))
+        if buf is None or buf[0] < max_q or buf[1][2].shape[0] < max_q * k:
+            arrs = (np.empty(max_q, np.uint64), np.empty(max_q, np.uint64),
+                    np.empty(max_q * k, np.uint32), np.empty(max_q * k, np.uint32))
+            # the ctypes arguments once per buffer, not per call
+            buf = (max_q, arrs, tuple(_ptr(a) for a in arrs), C.c_uint32())
             self._serve_buf = buf
-        q, npairs, v, c = buf
-        n = C.c_uint32()
+        _, (q, npairs, v, c), ptrs, n = buf
         # capacities from the arrays themselves: never more than they hold
-        self._check(self._L.gw_serve_results(self._ctx, max_q, _ptr(q), _ptr(npairs), _ptr(v), _ptr(c),
-                                             v.shape[0], C.byref(n)), "gw_serve_results")
+        self._check(self._L.gw_serve_results(self._ctx, max_q, *ptrs, v.shape[0], C.byref(n)), "gw_serve_results")
         m = n.value
-        tot = int(npairs[:m].sum())
-        return q[:m].copy(), npairs[:m].copy(), v[:tot].copy(), c[:tot].copy()
+        tot = int(npairs[:m].sum()) if m else 0
+        out = q[:m], npairs[:m], v[:tot], c[:tot]
+        return tuple(a.copy() for a in out) if copy else out
 
     def serve_topk(self, k: int) -> None:
         """Completed queries keep their k most-visited vertices (0 = all pairs)."""
