@@ -210,3 +210,40 @@ def test_submit_numpy_queries(engine, g12):
             engine.serve_submit(np.zeros(2, np.uint64))
     finally:
         engine.serve_close()
+
+
+def test_batched_results_capacity(engine, g12):
+    # gw_serve_results stops before a query whose pairs would pass `cap`, and
+    # refuses a call whose first query does not fit; views (copy=False) hold
+    # the same pairs as copies
+    import ctypes as C
+    off, col = g12
+    engine.graph_set(off, col)  # an earlier test may have set another graph
+    engine.serve_open(gw.program(gw.RESTART, 10, 0.15, 0, seed=SEED), 100_000, 64 << 20, 64, 10)
+    try:
+        engine.serve_topk(8)
+        # two sources with out-edges: each query then has more than 8 distinct vertices
+        s41, s42 = (int(x) for x in np.nonzero(np.diff(off) > 2)[0][:2])
+        srcs = {41: s41, 42: s42}
+        engine.serve_submit([(41, s41, 300), (42, s42, 300)])
+        engine.serve_step()
+        L, ctx = engine._L, engine._ctx
+        q = np.zeros(4, np.uint64)
+        npairs = np.zeros(4, np.uint64)
+        v = np.zeros(16, np.uint32)
+        c = np.zeros(16, np.uint32)
+        n = C.c_uint32()
+        p = gw._ptr
+        assert L.gw_serve_results(ctx, 4, p(q), p(npairs), p(v), p(c), 4, C.byref(n)) == -1  # GW_EINVAL
+        assert L.gw_serve_results(ctx, 4, p(q), p(npairs), p(v), p(c), 12, C.byref(n)) == 0
+        assert n.value == 1 and npairs[0] == 8  # the second query's 8 pairs would pass 12
+        first = q[0]
+        views = engine.serve_results(4, copy=False)
+        assert views[0].shape[0] == 1 and views[0][0] != first and views[1][0] == 8
+        for qid, src, vv, cc in ((first, srcs[int(first)], v[:8], c[:8]),
+                                 (views[0][0], srcs[int(views[0][0])], views[2], views[3])):
+            ev, ec = expected(off, col, gw.RESTART, 10, int(qid), src, 300)
+            order = np.lexsort((ev, -ec.astype(np.int64)))[:8]
+            assert np.array_equal(vv, ev[order]) and np.array_equal(cc, ec[order])
+    finally:
+        engine.serve_close()
