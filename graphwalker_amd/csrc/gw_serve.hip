@@ -173,51 +173,6 @@ __device__ __forceinline__ uint32_t rank_of(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Adds `inc` visits of vertex v to a query's hash region (arena slots [rb,
-// rb + rm]) for the lanes with `want`; every active lane calls it, and the
-// lanes with `want` belong to query slot `sl`.  One compare-and-swap either
-// claims an empty slot or returns the entry there (then an atomic add if it
-// is this vertex's).  A claimed slot is appended to the query's occupied list
-// (one atomic per wavefront), so completion reads the entries without
-// scanning the region.
-__device__ __forceinline__ void arena_add(unsigned long long* __restrict__ arena, uint32_t* __restrict__ occ,
-                                          uint32_t* __restrict__ occn, uint32_t cbits,
-                                          unsigned long long* __restrict__ overflow, bool want, uint32_t sl,
-                                          uint64_t rb, uint32_t rm, uint32_t v, uint32_t inc) {
-    constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-    uint32_t claimed = kNoSlot;
-    if (want) {
-        const unsigned long long entry = (static_cast<unsigned long long>(v) << cbits) | inc;
-        uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(v) * 0x9E3779B97F4A7C15ull) >> 32) & rm;
-        bool placed = false;
-        for (uint32_t probe = 0; probe <= rm; ++probe) {
-            unsigned long long* cell = arena + rb + h;
-            // compare-and-swap first: one request for a new vertex, two for a known one
-            const unsigned long long cur = atomicCAS(cell, 0ull, entry);
-            if (cur == 0ull) {
-                claimed = h;
-                placed = true;
-                break;
-            }
-            if ((cur >> cbits) == v) {
-                atomicAdd(cell, static_cast<unsigned long long>(inc));
-                placed = true;
-                break;
-            }
-            h = (h + 1) & rm;
-        }
-        if (!placed) atomicAdd(overflow, static_cast<unsigned long long>(inc));
-    }
-    const uint64_t cm = __ballot(claimed != kNoSlot);
-    if (cm) {
-        const uint32_t leader = __ffsll(static_cast<unsigned long long>(cm)) - 1;
-        uint32_t b0 = 0;
-        if (__lane_id() == leader) b0 = atomicAdd(&occn[sl], static_cast<uint32_t>(__popcll(cm)));
-        b0 = lane_value(b0, leader);
-        if (claimed != kNoSlot) occ[rb + b0 + rank_of(cm)] = claimed;
-    }
-}
-
 // Per-visit recorder of a served walk: the step's visit is held, then staged
 // by region_add_uniform after the step with the whole wavefront present
 // (the batched path's direct regions, one field per query slot): a query's
@@ -1049,7 +1004,7 @@ int gw_serve_step(gw_ctx* ctx, gw_serve_stats* out) {
         const uint32_t slot = S.free_slots.back();
         S.free_slots.pop_back();
         const SlotInfo si{base, static_cast<uint32_t>(size - 1), q.source, static_cast<uint32_t>(q.qid),
-                          static_cast<uint32_t>(q.qid >> 32)};
+                          static_cast<uint32_t>(q.qid >> 32), 0, 0, 0};  // srow, sdeg: written by the admit kernel
         const uint64_t koff = 2 * (base + size);
         adm.push_back(Admit{si, S.live, slot, q.walks, koff, koff + static_cast<uint64_t>(q.walks) * L});
         S.live += q.walks;

@@ -1537,7 +1537,7 @@ __global__ void noalloc_totals_kernel(const unsigned long long* __restrict__ cnt
 template <int KIND>
 hipError_t launch_noalloc(const Prog& p, const Graph& g, bool ext, bool vc, uint32_t grid, uint32_t tb, hipStream_t st,
                           const gw_walk* walks, uint64_t nw, uint32_t* counts, unsigned long long* cnt) {
-    const EventLog log{nullptr, 0};
+    const EventLog log{nullptr, 0, 0};
     const SourceTable table{};
     const KeyLog klog{};
     constexpr int VC = GW_REC_VERTEX_COUNT;
